@@ -1,0 +1,546 @@
+"""Seeded synthetic docking systems for the population-evaluation path.
+
+The reference builds its inputs from PDB/ligand files through a setup stage
+(read_lig, buildic, buildlist, assign_shift, bondedlist, calc_cleftic,
+add2_optimiz_vec).  This module produces the same *outputs* of that stage
+directly, for synthetic receptors and ligands, so that the hot path can be
+driven without file I/O:
+
+* receptor atoms uniform in a spherical shell at protein density
+  (~1 atom / 11 Å^3), 1-based types, radii U(1.5, 2.0);
+* a ligand as a random tree (degree <= 4, 1.5 Å bonds) described by
+  internal coordinates with `rec[4]` rebuild references as read by
+  LIB/read_lig.cpp:225-396, a rebuild order (`mov`, LIB/buildlist.cpp),
+  dihedral shift chains (LIB/assign_shift.cpp:80-188) and a bloops=2 bonded
+  matrix (LIB/bondedlist.cpp:8-61, LIB/update_bonded.cpp);
+* a gene map in the order add2_optimiz_vec emits it (grid index, then
+  ang/dih/dih of the global-positioning atoms, then one dihedral per flexible
+  bond, LIB/add2_optimiz_vec.cpp:89-197);
+* a cleft grid with internal coordinates relative to the FA->ori frame
+  (LIB/calc_cleftic.cpp:5-34).
+
+Everything is drawn from numpy's PCG64 with fixed seeds.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PI_REF = 3.141592  # LIB/flexaid.h:86
+
+#: per-config seeds
+CONFIG_SEED = {0: 1000, 1: 1001, 2: 1002, 3: 1003, 4: 1004}
+
+DENSITY_VOL = 11.0  # Å^3 per atom
+CAVITY_RADIUS = (20.0 ** 3 - 3.0 * 1500 * DENSITY_VOL / (4.0 * math.pi)) ** (1.0 / 3.0)
+
+
+@dataclass
+class Receptor:
+    xyz: np.ndarray      # (n,3) float32
+    radius: np.ndarray   # (n,) float32
+    type: np.ndarray     # (n,) int32, 1-based
+    center: np.ndarray   # (3,) float64, cavity centre
+
+    @property
+    def n_atoms(self) -> int:
+        return int(self.xyz.shape[0])
+
+
+@dataclass
+class Ligand:
+    rec: np.ndarray       # (n,4) int32, -1 = frame point / none
+    dis: np.ndarray       # (n,) float32
+    ang: np.ndarray
+    dih: np.ndarray
+    shift: np.ndarray
+    radius: np.ndarray
+    type: np.ndarray      # int32, 1-based
+    mov: np.ndarray       # (n,) int32 rebuild order
+    bonded: np.ndarray    # (n,n) uint8
+    gene_typ: np.ndarray  # (g,) int32
+    gene_atm: np.ndarray  # (g,) int32
+    gene_min: np.ndarray  # (g,) float64
+    gene_max: np.ndarray
+    grid_ic: np.ndarray   # (n_grid,3) float32
+    grid_xyz: np.ndarray  # (n_grid,3) float64 (for reference only)
+    ori: np.ndarray       # (3,) float32
+    globalmin: np.ndarray  # (3,) float32
+    globalmax: np.ndarray
+    maxwidth: float
+    parent: np.ndarray    # (n,) int32, tree parent (-1 for root)
+    flex_bonds: list = field(default_factory=list)
+
+    @property
+    def n_atoms(self) -> int:
+        return int(self.rec.shape[0])
+
+    @property
+    def n_genes(self) -> int:
+        return int(self.gene_typ.shape[0])
+
+    @property
+    def n_grid(self) -> int:
+        return int(self.grid_ic.shape[0])
+
+
+@dataclass
+class Params:
+    ntypes: int = 40
+    emat: np.ndarray = None          # (ntypes*ntypes,) float32
+    permeability: float = 1.0
+    dee_clash: float = 0.5
+    intramolecular: int = 1
+    solventterm: float = 0.0
+
+
+# ---------------------------------------------------------------------------
+# geometry helpers (float64; the device and oracle use their own arithmetic)
+# ---------------------------------------------------------------------------
+
+def _build_point(x1, x2, x3, dis, ang, dih):
+    """Place an atom from three references with the convention of
+    LIB/buildcc.cpp:52-129 (rotation by -ang about the plane normal, then by
+    dih about the x1->x2 axis)."""
+    x1 = np.asarray(x1, np.float64)
+    x2 = np.asarray(x2, np.float64)
+    x3 = np.asarray(x3, np.float64)
+    a = x1[1] * (x2[2] - x3[2]) + x2[1] * (x3[2] - x1[2]) + x3[1] * (x1[2] - x2[2])
+    b = x1[2] * (x2[0] - x3[0]) + x2[2] * (x3[0] - x1[0]) + x3[2] * (x1[0] - x2[0])
+    c = x1[0] * (x2[1] - x3[1]) + x2[0] * (x3[1] - x1[1]) + x3[0] * (x1[1] - x2[1])
+    n = np.array([a, b, c]) / math.sqrt(a * a + b * b + c * c)
+    u = x2 - x1
+    d = 1.0 / np.linalg.norm(u)
+    v = u * d * dis
+
+    def rot(axis, ct, st, w):
+        cx, cy, cz = axis
+        op = 1.0 - ct
+        aa, bb, cc = cx * cx, cy * cy, cz * cz
+        return np.array([
+            (cx * cz * op - cy * st) * w[2] + ((1 - aa) * ct + aa) * w[0] + (cx * cy * op + cz * st) * w[1],
+            (cy * cx * op - cz * st) * w[0] + ((1 - bb) * ct + bb) * w[1] + (cy * cz * op + cx * st) * w[2],
+            (cz * cy * op - cx * st) * w[1] + ((1 - cc) * ct + cc) * w[2] + (cz * cx * op + cy * st) * w[0],
+        ])
+
+    th = ang * PI_REF / 180.0
+    k = rot(n, math.cos(th), -math.sin(th), v)
+    ph = dih * PI_REF / 180.0
+    return rot(u * d, math.cos(ph), math.sin(ph), k) + x1
+
+
+def _angle_deg(p, q, r):
+    u = np.asarray(p, np.float64) - q
+    v = np.asarray(r, np.float64) - q
+    c = float(np.dot(u, v) / (np.linalg.norm(u) * np.linalg.norm(v)))
+    return math.degrees(math.acos(max(-1.0, min(1.0, c))))
+
+
+def _dihedral_deg(p1, p2, p3, p4):
+    b0 = np.asarray(p1, np.float64) - p2
+    b1 = np.asarray(p3, np.float64) - p2
+    b2 = np.asarray(p4, np.float64) - p3
+    b1n = b1 / np.linalg.norm(b1)
+    v = b0 - np.dot(b0, b1n) * b1n
+    w = b2 - np.dot(b2, b1n) * b1n
+    x = np.dot(v, w)
+    y = np.dot(np.cross(b1n, v), w)
+    return math.degrees(math.atan2(y, x))
+
+
+def frame_points(ori):
+    o = np.asarray(ori, np.float64)
+    return o + np.array([1.0, 0.0, 0.0]), o.copy(), o + np.array([0.0, 1.0, 0.0])
+
+
+def point_ic(p, ori):
+    """(dis, ang, dih) of a point relative to the FA->ori frame such that
+    _build_point(frame..., dis, ang, dih) reproduces it (LIB/calc_cleftic.cpp)."""
+    ca, cb, cc = frame_points(ori)
+    dis = float(np.linalg.norm(np.asarray(p) - ca))
+    ang = _angle_deg(p, ca, cb)
+    dih = _dihedral_deg(p, ca, cb, cc)
+    for cand in (dih, -dih):
+        q = _build_point(ca, cb, cc, dis, ang, cand)
+        if np.linalg.norm(q - p) < 1e-3:
+            return dis, ang, cand
+    raise RuntimeError("cleft-grid internal coordinates do not round-trip")
+
+
+def build_cartesian(lig: Ligand, dis=None, ang=None, dih=None) -> np.ndarray:
+    """float64 Cartesian coordinates from internal coordinates (mov order)."""
+    dis = lig.dis if dis is None else dis
+    ang = lig.ang if ang is None else ang
+    dih = lig.dih if dih is None else dih
+    n = lig.n_atoms
+    xyz = np.zeros((n, 3))
+    fr = frame_points(lig.ori)
+    for a in lig.mov:
+        refs = []
+        for i in range(3):
+            r = lig.rec[a, i]
+            refs.append(xyz[r] if r >= 0 else fr[i])
+        xyz[a] = _build_point(refs[0], refs[1], refs[2], float(dis[a]), float(ang[a]), float(dih[a]))
+    return xyz
+
+
+def apply_genes(lig: Ligand, icv: np.ndarray):
+    """Decode one gene vector into IC arrays (LIB/ic2cf.cpp:53-105)."""
+    dis = lig.dis.astype(np.float32).copy()
+    ang = lig.ang.astype(np.float32).copy()
+    dih = lig.dih.astype(np.float32).copy()
+    for g in range(lig.n_genes):
+        t, a, v = int(lig.gene_typ[g]), int(lig.gene_atm[g]), float(icv[g])
+        v = min(max(v, float(lig.gene_min[g])), float(lig.gene_max[g]))
+        if t == -1:
+            k = int(v)
+            dis[a], ang[a], dih[a] = lig.grid_ic[k]
+        elif t == 0:
+            dis[a] = np.float32(v)
+        elif t == 1:
+            ang[a] = np.float32(v)
+        elif t == 2:
+            dih[a] = np.float32(v)
+            j, cat = a, int(lig.rec[a, 3])
+            if cat >= 0:
+                while cat != a:
+                    dih[cat] = np.float32(dih[j] + lig.shift[cat])
+                    j, cat = cat, int(lig.rec[cat, 3])
+    return dis, ang, dih
+
+
+# ---------------------------------------------------------------------------
+# receptor
+# ---------------------------------------------------------------------------
+
+def make_receptor(rng, n_atoms, center, r_in=CAVITY_RADIUS, r_out=None, ntypes=40,
+                  solid=False) -> Receptor:
+    """Uniform-in-volume atoms in the shell r_in..r_out around `center` (or a
+    solid ball of radius r_out when `solid`)."""
+    if r_out is None:
+        r_out = (r_in ** 3 + 3.0 * n_atoms * DENSITY_VOL / (4.0 * math.pi)) ** (1.0 / 3.0)
+    lo = 0.0 if solid else r_in ** 3
+    u = rng.random(n_atoms)
+    r = np.cbrt(lo + u * (r_out ** 3 - lo))
+    v = rng.normal(size=(n_atoms, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    xyz = (np.asarray(center)[None, :] + v * r[:, None]).astype(np.float32)
+    typ = rng.integers(1, ntypes + 1, size=n_atoms).astype(np.int32)
+    rad = rng.uniform(1.5, 2.0, size=n_atoms).astype(np.float32)
+    return Receptor(xyz=xyz, radius=rad, type=typ, center=np.asarray(center, np.float64))
+
+
+# ---------------------------------------------------------------------------
+# ligand
+# ---------------------------------------------------------------------------
+
+def _random_tree(rng, n):
+    parent = [-1, 0, 1]
+    deg = [1, 2, 1]
+    for i in range(3, n):
+        # drug-like trees: mostly chains with occasional branches, attach to
+        # one of the last few atoms; degree <= 3 except for rare 4
+        while True:
+            j = int(rng.integers(max(0, i - 6), i))
+            cap = 4 if rng.random() < 0.1 else 3
+            if deg[j] < cap:
+                break
+        parent.append(j)
+        deg[j] += 1
+        deg.append(1)
+    return parent[:n]
+
+
+def make_ligand(rng, n_atoms, n_tors, ori, anchor, grid_points, ntypes=40,
+                max_tries=200) -> Ligand:
+    """Random-tree ligand whose reference pose sits at `anchor`.
+
+    `grid_points` (m,3) become cleft-grid entries 1..m; entry 0 is `anchor`.
+    """
+    assert n_atoms >= 3
+    ori = np.asarray(ori, np.float32)
+    pts = np.vstack([np.asarray(anchor, np.float64)[None, :], np.asarray(grid_points, np.float64)])
+    grid_ic = np.array([point_ic(p, ori) for p in pts], np.float32)
+    for _ in range(max_tries):
+        lig = _try_ligand(rng, n_atoms, n_tors, ori, pts, grid_ic, ntypes)
+        if lig is not None:
+            return lig
+    raise RuntimeError("could not generate a clash-free ligand")
+
+
+def _try_ligand(rng, n, n_tors, ori, pts, grid_ic, ntypes):
+    parent = _random_tree(rng, n)
+    children = [[] for _ in range(n)]
+    for i in range(1, n):
+        children[parent[i]].append(i)
+    depth = [0] * n
+    for i in range(1, n):
+        depth[i] = depth[parent[i]] + 1
+    g0, g1, g2 = 0, 1, 2
+
+    rec = -np.ones((n, 4), np.int32)
+    rec[g1, 0] = g0
+    rec[g2, 0], rec[g2, 1] = g1, g0
+    for x in range(3, n):
+        p = parent[x]
+        if p == g0:
+            rec[x, :3] = (g0, g1, g2)
+        elif p == g1:
+            rec[x, :3] = (g1, g0, g2)
+        else:
+            pp = parent[p]
+            ppp = parent[pp] if pp >= 0 else -1
+            if ppp < 0:  # pp is the root: use gpa2/gpa1 as third reference
+                ppp = g2 if p != g2 else g0
+            rec[x, :3] = (p, pp, ppp)
+
+    adj0 = [[] for _ in range(n)]
+    for i in range(1, n):
+        adj0[i].append(parent[i])
+        adj0[parent[i]].append(i)
+
+    def depth_sep_ok(x, o):
+        """True when atoms x and o are more than three bonds apart."""
+        frontier, seen = [x], {x}
+        for _ in range(3):
+            nxt = []
+            for a in frontier:
+                for b in adj0[a]:
+                    if b not in seen:
+                        seen.add(b)
+                        nxt.append(b)
+            frontier = nxt
+        return o not in seen
+
+    dis = np.full(n, 1.5, np.float32)
+    ang = np.zeros(n, np.float32)
+    dih = np.zeros(n, np.float32)
+    # sibling groups share an axis: space their dihedrals 120 degrees apart
+    ang[g2] = rng.uniform(105.0, 125.0)
+    # reference pose: anchor + a random orientation
+    dis[g0], ang[g0], dih[g0] = grid_ic[0]
+    ang[g1] = math.degrees(math.acos(rng.uniform(-1.0, 1.0)))
+    dih[g1] = rng.uniform(-180.0, 180.0)
+    dih[g2] = rng.uniform(-180.0, 180.0)
+
+    groups = {}
+    for x in range(3, n):
+        groups.setdefault((int(rec[x, 0]), int(rec[x, 1]), int(rec[x, 2])), []).append(x)
+    for sib in groups.values():
+        for x in sib:
+            ang[x] = rng.uniform(105.0, 125.0)
+
+    # rebuild order: global-positioning atoms, then breadth-first
+    rest = sorted(range(3, n), key=lambda i: (depth[i], i))
+    mov = np.array([g0, g1, g2] + rest, np.int32)
+
+    # greedy placement: each sibling group (atoms sharing rec[0..2]) gets the
+    # dihedral base, among a few tries, that keeps it furthest from atoms
+    # already built; siblings sit 120 degrees apart
+    xyz = np.zeros((n, 3))
+    built = np.zeros(n, bool)
+    fr = frame_points(ori)
+
+    def place(a):
+        refs = [xyz[r] if r >= 0 else fr[i] for i, r in enumerate(rec[a, :3])]
+        return _build_point(refs[0], refs[1], refs[2], float(dis[a]), float(ang[a]), float(dih[a]))
+
+    done_groups = set()
+    for a in mov:
+        key = (int(rec[a, 0]), int(rec[a, 1]), int(rec[a, 2]))
+        if a >= 3 and key not in done_groups:
+            done_groups.add(key)
+            sib = groups[key]
+            if key[0] == g1:      # gpa2 is a sibling here: take 120/240 from it
+                tries = [120.0]
+            elif key[0] == g0:
+                tries = [60.0]
+            else:
+                tries = list(rng.uniform(-180.0, 180.0, size=24))
+            best, best_d = None, -1.0
+            others = np.nonzero(built)[0]
+            for base in tries:
+                for k, x in enumerate(sib):
+                    dih[x] = ((base + 120.0 * k + 180.0) % 360.0) - 180.0
+                pos = np.array([place(x) for x in sib])
+                if len(others):
+                    dd = np.linalg.norm(pos[:, None, :] - xyz[others][None, :, :], axis=2)
+                    far = np.array([[depth_sep_ok(x, o) for o in others] for x in sib])
+                    md = dd[far].min() if np.any(far) else 99.0
+                else:
+                    md = 99.0
+                if md > best_d:
+                    best, best_d = base, md
+            for k, x in enumerate(sib):
+                dih[x] = ((best + 120.0 * k + 180.0) % 360.0) - 180.0
+        xyz[a] = place(a)
+        built[a] = True
+
+    # flexible bonds (p -> c) with c non-terminal and its children built on (c, p, .)
+    eligible = []
+    for c in range(n):
+        p = parent[c]
+        if p < 0 or c == g1 or not children[c]:
+            continue
+        kids = [x for x in children[c] if rec[x, 0] == c and rec[x, 1] == p]
+        if len(kids) == len(children[c]):
+            eligible.append((p, c))
+    nt = min(n_tors, len(eligible))
+    flex = [eligible[i] for i in sorted(rng.choice(len(eligible), size=nt, replace=False))] if nt else []
+    shift = np.zeros(n, np.float32)
+    gene_typ = [-1, 1, 2, 2]
+    gene_atm = [g0, g1, g1, g2]
+    pos = {int(a): k for k, a in enumerate(mov)}
+    for (p, c) in flex:
+        kids = sorted(children[c], key=lambda x: pos[x])
+        if len(kids) > 1:  # shift chain, LIB/assign_shift.cpp:160-180
+            for j in range(len(kids)):
+                k = (j + 1) % len(kids)
+                rec[kids[j], 3] = kids[k]
+                shift[kids[k]] = np.float32(dih[kids[k]] - dih[kids[j]])
+        gene_typ.append(2)
+        gene_atm.append(kids[0])
+
+    # bonded within bloops = 2 (self, 1-2, 1-3)
+    adj = [[] for _ in range(n)]
+    for i in range(1, n):
+        adj[i].append(parent[i])
+        adj[parent[i]].append(i)
+    bonded = np.zeros((n, n), np.uint8)
+    for s in range(n):
+        bonded[s, s] = 1
+        for a in adj[s]:
+            bonded[s, a] = 1
+            for b in adj[a]:
+                bonded[s, b] = 1
+
+    ng = len(gene_typ)
+    gmin = np.array([0.0] + [-180.0] * (ng - 1))
+    gmax = np.array([float(len(pts) - 1)] + [180.0] * (ng - 1))
+    lig = Ligand(rec=rec, dis=dis, ang=ang, dih=dih, shift=shift,
+                 radius=rng.uniform(1.5, 2.0, size=n).astype(np.float32),
+                 type=rng.integers(1, ntypes + 1, size=n).astype(np.int32),
+                 mov=mov, bonded=bonded,
+                 gene_typ=np.array(gene_typ, np.int32), gene_atm=np.array(gene_atm, np.int32),
+                 gene_min=gmin, gene_max=gmax, grid_ic=grid_ic, grid_xyz=pts,
+                 ori=ori, globalmin=np.zeros(3, np.float32), globalmax=np.zeros(3, np.float32),
+                 maxwidth=0.0, parent=np.array(parent, np.int32), flex_bonds=flex)
+    xyz = build_cartesian(lig)
+    # reject self-overlapping reference poses: non-bonded pairs closer than
+    # 2.2 Å (three bonds apart) or 2.5 Å (further apart)
+    d = np.linalg.norm(xyz[:, None, :] - xyz[None, :, :], axis=2)
+    sep = np.full((n, n), 99, np.int32)
+    for s in range(n):
+        sep[s, s] = 0
+        frontier, k = [s], 0
+        while frontier and k < 3:
+            k += 1
+            nxt = []
+            for a in frontier:
+                for b in adj[a]:
+                    if sep[s, b] > k:
+                        sep[s, b] = k
+                        nxt.append(b)
+            frontier = nxt
+    if np.any(d[sep == 3] < 2.2) or np.any(d[sep > 3] < 2.5):
+        return None
+    return lig
+
+
+def finalize_frame(lig: Ligand, rec: Receptor):
+    """FA->ori / globalmin / globalmax / maxwidth as calc_center computes them
+    over receptor + ligand reference pose (LIB/calc_center.cpp:8-62)."""
+    lxyz = build_cartesian(lig).astype(np.float32)
+    allxyz = np.vstack([rec.xyz, lxyz])
+    lig.globalmin = allxyz.min(axis=0).astype(np.float32)
+    lig.globalmax = allxyz.max(axis=0).astype(np.float32)
+    lig.maxwidth = float(np.max(lig.globalmax - lig.globalmin))
+
+
+def make_emat(rng, ntypes=40) -> np.ndarray:
+    m = rng.normal(size=(ntypes, ntypes))
+    m = np.triu(m) + np.triu(m, 1).T
+    return m.astype(np.float32).reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# poses
+# ---------------------------------------------------------------------------
+
+def random_genes(rng, lig: Ligand, n_poses, grid_lo=1, grid_hi=None) -> np.ndarray:
+    """Gene vectors (ic values): grid index, uniform rotation, uniform torsions."""
+    g = np.zeros((n_poses, lig.n_genes), np.float64)
+    hi = lig.n_grid - 1 if grid_hi is None else grid_hi
+    g[:, 0] = rng.integers(grid_lo, hi + 1, size=n_poses).astype(np.float64)
+    g[:, 1] = np.degrees(np.arccos(rng.uniform(-1.0, 1.0, size=n_poses)))
+    g[:, 2:] = rng.uniform(-180.0, 180.0, size=(n_poses, lig.n_genes - 2))
+    return g
+
+
+@dataclass
+class System:
+    receptor: Receptor
+    ligands: list
+    params: Params
+    genes: list          # per ligand: (n_poses, n_genes) float64
+    config: int = -1
+
+
+def _box_points(rng, center, n, edge=10.0):
+    return np.asarray(center)[None, :] + rng.uniform(-edge / 2, edge / 2, size=(n, 3))
+
+
+def make_config0(n_poses=1000, seed=None, n_rec=1500, n_lig=30, n_tors=6) -> System:
+    """Plain CF scoring: 1,500-atom shell receptor, 30-atom/6-torsion ligand,
+    `n_poses` poses with translations uniform in a 10 Å box."""
+    rng = np.random.default_rng(CONFIG_SEED[0] if seed is None else seed)
+    center = np.array([22.3, 41.7, -13.9])
+    rec = make_receptor(rng, n_rec, center, r_in=CAVITY_RADIUS, r_out=20.0)
+    ori = rec.xyz.astype(np.float64).mean(axis=0).astype(np.float32)
+    pts = _box_points(rng, center, n_poses)
+    lig = make_ligand(rng, n_lig, n_tors, ori, center, pts)
+    finalize_frame(lig, rec)
+    params = Params(ntypes=40, emat=make_emat(rng))
+    genes = np.zeros((n_poses, lig.n_genes))
+    genes[:, 0] = np.arange(1, n_poses + 1)
+    genes[:, 1] = np.degrees(np.arccos(rng.uniform(-1.0, 1.0, size=n_poses)))
+    genes[:, 2:] = rng.uniform(-180.0, 180.0, size=(n_poses, lig.n_genes - 2))
+    return System(receptor=rec, ligands=[lig], params=params, genes=[genes], config=0)
+
+
+def make_config1(pop=2000, seed=None) -> System:
+    """Plain full-GA dock shape: 4,000-atom receptor, 45-atom/10-torsion
+    ligand, one population of `pop` chromosomes (the rigid-receptor part;
+    flexible side chains are not modelled here)."""
+    rng = np.random.default_rng(CONFIG_SEED[1] if seed is None else seed)
+    center = np.array([-8.6, 17.2, 33.4])
+    rec = make_receptor(rng, 4000, center, r_in=CAVITY_RADIUS)
+    ori = rec.xyz.astype(np.float64).mean(axis=0).astype(np.float32)
+    pts = _box_points(rng, center, 1000)
+    lig = make_ligand(rng, 45, 10, ori, center, pts)
+    finalize_frame(lig, rec)
+    params = Params(ntypes=40, emat=make_emat(rng))
+    genes = random_genes(rng, lig, pop)
+    return System(receptor=rec, ligands=[lig], params=params, genes=[genes], config=1)
+
+
+def make_config2(n_ligands=256, pop=512, seed=None) -> System:
+    """Batched uneven screen: one 6,000-atom receptor, ligands with 12-80
+    heavy atoms and 0-15 torsions, `pop` chromosomes each."""
+    rng = np.random.default_rng(CONFIG_SEED[2] if seed is None else seed)
+    center = np.array([5.1, -12.4, 9.8])
+    rec = make_receptor(rng, 6000, center, r_in=CAVITY_RADIUS)
+    ori = rec.xyz.astype(np.float64).mean(axis=0).astype(np.float32)
+    params = Params(ntypes=40, emat=make_emat(rng))
+    ligs, genes = [], []
+    for _ in range(n_ligands):
+        na = int(rng.integers(12, 81))
+        nt = int(rng.integers(0, 16))
+        pts = _box_points(rng, center, 256)
+        lig = make_ligand(rng, na, nt, ori, center, pts)
+        finalize_frame(lig, rec)
+        ligs.append(lig)
+        genes.append(random_genes(rng, lig, pop))
+    return System(receptor=rec, ligands=ligs, params=params, genes=genes, config=2)

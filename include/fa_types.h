@@ -1,0 +1,131 @@
+/*
+ * fa_types.h — plain-data descriptors of one docking system for the
+ * FlexAID∆S population-evaluation path (pose build → Voronoi contact
+ * function → ensemble statistics).
+ *
+ * These structs carry what the reference's setup stage produces and its hot
+ * path consumes (FA_Global / atom / resid / gridpoint / energy_matrix in
+ * LIB/flexaid.h:122-572).  They hold borrowed pointers only; every consumer
+ * (the HIP library, the CPU oracle, the reference harness) copies what it
+ * needs.  All indices are 0-based and ligand-local unless stated otherwise.
+ *
+ * Atom ordering ("calc order") follows LIB/Vcontacts.cpp:1594-1625
+ * (index_protein walks residues 1..res_cnt): receptor atoms first, then the
+ * ligand atoms in their input order.
+ */
+#ifndef FA_TYPES_H
+#define FA_TYPES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Hard limits of the device implementation (the reference allows 100 genes,
+ * LIB/gaboom.h:33, and MAX_ATM_HET=200 hetero atoms, LIB/flexaid.h:62). */
+#define FA_MAX_LIG_ATOMS 128
+#define FA_MAX_GENES 64
+
+/* gene → internal-coordinate mapping types (optmap.typ, LIB/flexaid.h:192-197,
+ * decoded at LIB/ic2cf.cpp:53-105) */
+enum fa_gene_type {
+    FA_GENE_GRID = -1, /* index into the cleft grid: sets dis/ang/dih of atm */
+    FA_GENE_DIS = 0,
+    FA_GENE_ANG = 1,
+    FA_GENE_DIH = 2    /* dihedral; cascades along rec[3] shift chains */
+};
+
+/* Per-pose evaluation status */
+enum fa_status {
+    FA_OK = 0,
+    FA_HULL_FAIL = 1,     /* Voronoi hull did not converge → POLYHEDRON_PENALTY */
+    FA_CLASH = 2,         /* clash pre-pass ≥ CLASH_THRESHOLD */
+    FA_CAPACITY = 3,      /* a device capacity (contacts/vertices/edges) overflowed */
+    FA_GRID_RANGE = 4     /* an atom fell outside the cell grid (undefined in the reference) */
+};
+
+/* Rigid receptor part (all non-scorable atoms). */
+typedef struct fa_receptor_desc {
+    int32_t        n_atoms;
+    const float*   xyz;      /* [n_atoms*3] Å */
+    const float*   radius;   /* [n_atoms] van der Waals radius */
+    const int32_t* type;     /* [n_atoms] 1-based energy-matrix type */
+} fa_receptor_desc;
+
+/* One flexible ligand with its gene map and cleft grid. */
+typedef struct fa_ligand_desc {
+    int32_t        n_atoms;
+    int32_t        n_genes;
+    int32_t        n_grid;     /* cleft-grid entries (index 0 is a valid slot) */
+    const int32_t* rec;        /* [n_atoms*4] rebuild references, -1 = frame point / none */
+    const float*   dis;        /* [n_atoms] base internal coordinates (Å, degrees) */
+    const float*   ang;
+    const float*   dih;
+    const float*   shift;      /* [n_atoms] dihedral shift along the rec[3] chain */
+    const float*   radius;     /* [n_atoms] */
+    const int32_t* type;       /* [n_atoms] 1-based */
+    const int32_t* mov;        /* [n_atoms] rebuild order (buildlist output) */
+    const uint8_t* bonded;     /* [n_atoms*n_atoms] 1 = within bloops bonds (bonded[i][j] >= 0) */
+    const int32_t* gene_typ;   /* [n_genes] fa_gene_type */
+    const int32_t* gene_atm;   /* [n_genes] ligand-local atom */
+    const double*  gene_min;   /* [n_genes] clamp bounds (genlim.min/max) */
+    const double*  gene_max;
+    const float*   grid_ic;    /* [n_grid*3] (dis, ang, dih) of each cleft-grid point */
+    float          ori[3];     /* FA->ori, frame origin for rec = -1 */
+    float          globalmin[3];
+    float          globalmax[3];
+    float          maxwidth;
+} fa_ligand_desc;
+
+/* Scoring parameters (FA_Global fields used by vcfunction/Vcontacts). */
+typedef struct fa_params {
+    float        permeability;   /* FA->permeability */
+    float        dee_clash;      /* FA->dee_clash */
+    int32_t      intramolecular; /* FA->intramolecular */
+    float        solventterm;    /* FA->solventterm (0 = use solvent column) */
+    int32_t      ntypes;         /* FA->ntypes; type ntypes is the solvent column */
+    const float* emat;           /* [ntypes*ntypes] y value per type pair, weight = 1 */
+} fa_params;
+
+/* Per-pose result: the cfstr fields the path produces (LIB/flexaid.h:122-135)
+ * plus get_cf_evalue / get_apparent_cf_evalue (LIB/ic2cf.cpp:391-402). */
+typedef struct fa_cf_result {
+    double  com;
+    double  wal;
+    double  sas;
+    double  con;
+    double  totsas;
+    double  cf;        /* com + wal + sas + con + metal_coord */
+    double  app;       /* com + wal + sas + metal_coord */
+    int32_t rclash;
+    int32_t status;    /* fa_status */
+} fa_cf_result;
+
+/* One directed contact record (ca_struct, LIB/Vcontacts.h:127-134), in calc
+ * order indices: `from` is the scorable atom whose polyhedron produced it. */
+typedef struct fa_contact_rec {
+    int32_t from;
+    int32_t atom;
+    double  area;
+    double  dist;
+} fa_contact_rec;
+
+/* Canonical-ensemble thermodynamics (statmech::Thermodynamics,
+ * LIB/statmech.h:36-45). */
+typedef struct fa_thermo {
+    double temperature;
+    double log_Z;
+    double free_energy;
+    double mean_energy;
+    double mean_energy_sq;
+    double heat_capacity;
+    double entropy;
+    double std_energy;
+} fa_thermo;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FA_TYPES_H */

@@ -1,0 +1,297 @@
+/*
+ * ref_driver.cpp — harness that runs the reference's own CPU contact
+ * function on a system described by include/fa_types.h.
+ *
+ * TEST INFRASTRUCTURE ONLY.  It is linked against the reference sources
+ * compiled in place from LIB/ (see oracle/ref/Makefile) and is used to pin
+ * the oracle restatement; the shipped library never loads it.
+ *
+ * The in-memory system mirrors what LIB/top.cpp sets up before the GA:
+ *   - residue 1 = the receptor (type 0), residue 2 = the ligand (type 1);
+ *   - atoms are 1-based, atom number == internal index (num_atm identity);
+ *   - one optimised residue (the ligand), FA->mov[0] = rebuild order;
+ *   - VC buffers as LIB/top.cpp:207-211 and :1011-1043 allocate them;
+ *   - a first ic2cf call (the "initial structure", LIB/top.cpp:1132), then
+ *     VC->recalc = 0, then the poses — the steady state the GA sees.
+ */
+#include "flexaid.h"
+#include "gaboom.h"
+#include "Vcontacts.h"
+#include "fa_types.h"
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+struct RefSystem {
+    FA_Global* FA = nullptr;
+    VC_Global* VC = nullptr;
+    std::vector<atom> atoms;
+    std::vector<resid> residue;
+    std::vector<gridpoint> grid;
+    std::vector<int> num_atm, contacts, mov, fatm, latm;
+    std::vector<float> contributions;
+    std::vector<int*> bonded_rows;
+    std::vector<int> bonded_data;
+    std::vector<optmap> map_par;
+    std::vector<OptRes> optres;
+    std::vector<energy_matrix> emat;
+    std::vector<energy_values> evals;
+    int Rn = 0, L = 0, N = 0;
+
+    ~RefSystem()
+    {
+        if (VC) {
+            free(VC->Calc);
+            free(VC->Calclist);
+            free(VC->ca_index);
+            free(VC->seed);
+            free(VC->contlist);
+            free(VC->ca_rec);
+            free(VC->ptorder);
+            free(VC->centerpt);
+            free(VC->poly);
+            free(VC->cont);
+            free(VC->vedge);
+            free(VC);
+        }
+        delete FA;
+    }
+};
+
+void setup(RefSystem& S, const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm)
+{
+    S.Rn = rec->n_atoms;
+    S.L = lig->n_atoms;
+    S.N = S.Rn + S.L;
+    const int N = S.N, Rn = S.Rn, L = S.L;
+    S.FA = new FA_Global();
+    FA_Global* FA = S.FA;
+    S.atoms.assign(N + 1, atom{});
+    S.residue.assign(3, resid{});
+    S.fatm = {1, Rn + 1};
+    S.latm = {Rn, N};
+    // residues
+    S.residue[1].type = 0;
+    S.residue[1].fatm = &S.fatm[0];
+    S.residue[1].latm = &S.latm[0];
+    S.residue[1].rot = 0;
+    S.residue[2].type = 1;
+    S.residue[2].fatm = &S.fatm[1];
+    S.residue[2].latm = &S.latm[1];
+    S.residue[2].rot = 0;
+    S.bonded_data.assign((size_t)L * L, -1);
+    S.bonded_rows.resize(L);
+    for (int i = 0; i < L; ++i) {
+        S.bonded_rows[i] = &S.bonded_data[(size_t)i * L];
+        for (int j = 0; j < L; ++j)
+            if (lig->bonded[i * L + j]) S.bonded_rows[i][j] = (i == j) ? 0 : 1;
+    }
+    S.residue[2].bonded = S.bonded_rows.data();
+    // optimised residue
+    S.optres.assign(1, OptRes{});
+    S.optres[0].rnum = 2;
+    S.optres[0].type = 1;
+    S.optres[0].tot = L;
+    FA->optres = S.optres.data();
+    FA->num_optres = 1;
+    // atoms
+    for (int i = 0; i < Rn; ++i) {
+        atom& a = S.atoms[1 + i];
+        a.coor[0] = rec->xyz[3 * i];
+        a.coor[1] = rec->xyz[3 * i + 1];
+        a.coor[2] = rec->xyz[3 * i + 2];
+        a.number = 1 + i;
+        a.radius = rec->radius[i];
+        a.type = rec->type[i];
+        a.ofres = 1;
+        a.recs = 'r';
+        a.acs = -1.0f;
+        a.optres = nullptr;
+    }
+    for (int i = 0; i < L; ++i) {
+        atom& a = S.atoms[Rn + 1 + i];
+        a.number = Rn + 1 + i;
+        a.radius = lig->radius[i];
+        a.type = lig->type[i];
+        a.ofres = 2;
+        a.recs = 'm';
+        a.acs = -1.0f;
+        a.dis = lig->dis[i];
+        a.ang = lig->ang[i];
+        a.dih = lig->dih[i];
+        a.shift = lig->shift[i];
+        for (int k = 0; k < 4; ++k) {
+            const int r = lig->rec[4 * i + k];
+            a.rec[k] = (r >= 0) ? Rn + 1 + r : 0;
+        }
+        a.optres = &S.optres[0];
+    }
+    S.num_atm.assign(N + 2, 0);
+    for (int i = 1; i <= N; ++i) S.num_atm[i] = i;
+    FA->num_atm = S.num_atm.data();
+    S.contacts.assign(MAX_ATOM_NUMBER, 0);
+    FA->contacts = S.contacts.data();
+    FA->atm_cnt = N;
+    FA->atm_cnt_real = N;
+    FA->res_cnt = 2;
+    // gene map
+    S.map_par.assign(lig->n_genes, optmap{});
+    for (int g = 0; g < lig->n_genes; ++g) {
+        S.map_par[g].typ = lig->gene_typ[g];
+        S.map_par[g].atm = Rn + 1 + lig->gene_atm[g];
+    }
+    FA->map_par = S.map_par.data();
+    FA->npar = lig->n_genes;
+    S.mov.resize(L);
+    for (int i = 0; i < L; ++i) S.mov[i] = Rn + 1 + lig->mov[i];
+    FA->nors = 1;
+    FA->nmov[0] = L;
+    FA->mov[0] = S.mov.data();
+    FA->opt_res[0] = 2;
+    // cleft grid
+    S.grid.assign(lig->n_grid, gridpoint{});
+    for (int g = 0; g < lig->n_grid; ++g) {
+        S.grid[g].index = g;
+        S.grid[g].dis = lig->grid_ic[3 * g];
+        S.grid[g].ang = lig->grid_ic[3 * g + 1];
+        S.grid[g].dih = lig->grid_ic[3 * g + 2];
+    }
+    FA->num_grd = lig->n_grid;
+    for (int j = 0; j < 3; ++j) {
+        FA->ori[j] = lig->ori[j];
+        FA->globalmin[j] = lig->globalmin[j];
+        FA->globalmax[j] = lig->globalmax[j];
+    }
+    FA->maxwidth = lig->maxwidth;
+    // scoring parameters
+    const int nt = prm->ntypes;
+    FA->ntypes = nt;
+    S.emat.assign((size_t)nt * nt, energy_matrix{});
+    S.evals.assign((size_t)nt * nt, energy_values{});
+    for (int i = 0; i < nt * nt; ++i) {
+        S.evals[i].x = 0.0f;
+        S.evals[i].y = prm->emat[i];
+        S.evals[i].next_value = nullptr;
+        S.emat[i].type1 = i / nt + 1;
+        S.emat[i].type2 = i % nt + 1;
+        S.emat[i].weight = 1;
+        S.emat[i].energy_values = &S.evals[i];
+    }
+    FA->energy_matrix = S.emat.data();
+    S.contributions.assign((size_t)nt * nt, 0.0f);
+    FA->contributions = S.contributions.data();
+    FA->permeability = prm->permeability;
+    FA->dee_clash = prm->dee_clash;
+    FA->intramolecular = prm->intramolecular;
+    FA->solventterm = prm->solventterm;
+    FA->normalize_area = 0;
+    FA->useacs = 0;
+    FA->vindex = 0;
+    FA->bloops = 2;
+    FA->normal_modes = 0;
+    FA->useflexdee = 0;
+    FA->deelig_flex = 0;
+    FA->use_gist = 0;
+    FA->use_hbond = 0;
+    FA->use_elec = 0;
+    FA->use_metal_coord = 0;
+    FA->num_constraints = 0;
+    // Vcontacts workspace (LIB/top.cpp:207-211, :1011-1043)
+    VC_Global* VC = (VC_Global*)calloc(1, sizeof(VC_Global));
+    S.VC = VC;
+    VC->planedef = 'X';
+    VC->Calc = (atomsas*)calloc(N, sizeof(atomsas));
+    VC->Calclist = (int*)calloc(N, sizeof(int));
+    VC->ca_index = (int*)calloc(N, sizeof(int));
+    VC->seed = (int*)calloc(3 * N, sizeof(int));
+    VC->contlist = (contactlist*)calloc(10000, sizeof(contactlist));
+    VC->ca_recsize = 5 * N;
+    VC->ca_rec = (ca_struct*)calloc(VC->ca_recsize, sizeof(ca_struct));
+    VC->ptorder = (ptindex*)calloc(MAX_PT, sizeof(ptindex));
+    VC->centerpt = (vertex*)calloc(MAX_PT, sizeof(vertex));
+    VC->poly = (vertex*)calloc(MAX_POLY, sizeof(vertex));
+    VC->cont = (plane*)calloc(MAX_PT, sizeof(plane));
+    VC->vedge = (edgevector*)calloc(MAX_POLY, sizeof(edgevector));
+    for (int i = 0; i < N; ++i) {
+        VC->Calc[i].atom = nullptr;
+        VC->Calc[i].residue = nullptr;
+        VC->Calc[i].exposed = true;
+    }
+    VC->recalc = 0;
+}
+
+cfstr run_pose(RefSystem& S, const fa_ligand_desc* lig, const double* genes)
+{
+    std::vector<double> icv(lig->n_genes);
+    for (int g = 0; g < lig->n_genes; ++g) {
+        double v = genes[g];
+        if (v > lig->gene_max[g]) v = lig->gene_max[g];
+        if (v < lig->gene_min[g]) v = lig->gene_min[g];
+        icv[g] = v;
+    }
+    return ic2cf(S.FA, S.VC, S.atoms.data(), S.residue.data(), S.grid.data(), lig->n_genes, icv.data());
+}
+
+}  // namespace
+
+extern "C" {
+
+/* Evaluate poses serially with the reference ic2cf.  `lig_xyz` (optional)
+ * receives the rebuilt ligand coordinates per pose; `recs`/`rec_counts`
+ * (optional) receive the contact records of each pose (ca_rec lists of the
+ * scorable atoms, calc-order indices), at most `recs_cap_per_pose` each. */
+int fa_ref_eval(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm, int n_poses,
+                const double* genes, fa_cf_result* out, float* lig_xyz, fa_contact_rec* recs, int recs_cap_per_pose,
+                int* rec_counts)
+{
+    if (!rec || !lig || !prm || !genes || !out || n_poses < 1) return -1;
+    RefSystem S;
+    setup(S, rec, lig, prm);
+    run_pose(S, lig, genes);  // initial structure call (LIB/top.cpp:1132)
+    S.VC->recalc = 0;
+    for (int p = 0; p < n_poses; ++p) {
+        const int fail0 = S.FA->skipped, clash0 = S.FA->clashed;
+        cfstr cf = run_pose(S, lig, genes + (size_t)p * lig->n_genes);
+        fa_cf_result& r = out[p];
+        r.com = cf.com;
+        r.wal = cf.wal;
+        r.sas = cf.sas;
+        r.con = cf.con;
+        r.totsas = S.FA->optres[0].cf.totsas;
+        r.cf = get_cf_evalue(&cf);
+        r.app = get_apparent_cf_evalue(&cf);
+        r.rclash = S.FA->optres[0].cf.rclash;
+        r.status = (S.FA->skipped != fail0) ? FA_HULL_FAIL : (S.FA->clashed != clash0) ? FA_CLASH : FA_OK;
+        if (r.status != FA_OK) {
+            r.totsas = 0.0;
+            r.rclash = 0;
+        }
+        if (lig_xyz)
+            for (int i = 0; i < S.L; ++i)
+                for (int k = 0; k < 3; ++k) lig_xyz[((size_t)p * S.L + i) * 3 + k] = S.atoms[S.Rn + 1 + i].coor[k];
+        if (rec_counts) {
+            int n = 0;
+            if (r.status == FA_OK) {
+                for (int i = S.Rn; i < S.N; ++i) {
+                    for (int ci = S.VC->ca_index[i]; ci != -1; ci = S.VC->ca_rec[ci].prev) {
+                        if (recs && n < recs_cap_per_pose) {
+                            fa_contact_rec& o = recs[(size_t)p * recs_cap_per_pose + n];
+                            o.from = S.VC->ca_rec[ci].from;
+                            o.atom = S.VC->ca_rec[ci].atom;
+                            o.area = S.VC->ca_rec[ci].area;
+                            o.dist = S.VC->ca_rec[ci].dist;
+                        }
+                        ++n;
+                    }
+                }
+            }
+            rec_counts[p] = n;
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

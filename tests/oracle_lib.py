@@ -1,0 +1,106 @@
+"""Loaders for the CPU oracle and the reference harness (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use
+these; the shipped library never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from flexaidds_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libfa_oracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libfa_ref.so")
+REF_FAST_SO = os.path.join(ROOT, "oracle", "_ref", "libfa_ref_fast.so")
+
+_P = C.c_void_p
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+def load_oracle():
+    if not os.path.exists(ORACLE_SO):
+        build_oracle()
+    lib = C.CDLL(ORACLE_SO)
+    for name in ("fa_oracle_eval", "fa_oracle_eval_f32"):
+        f = getattr(lib, name)
+        f.argtypes = [_P, _P, _P, C.c_int, _P, _P, C.c_int]
+        f.restype = C.c_int
+    lib.fa_oracle_eval_pose.argtypes = [_P, _P, _P, _P, _P, _P, _P, C.c_int]
+    lib.fa_oracle_eval_pose.restype = C.c_int
+    lib.fa_oracle_build_pose.argtypes = [_P, _P, _P]
+    lib.fa_oracle_build_pose.restype = C.c_int
+    lib.fa_oracle_solve_3x3.argtypes = [_P, _P, _P, _P]
+    lib.fa_oracle_solve_3x3.restype = C.c_int
+    lib.fa_oracle_solve_2xS.argtypes = [_P, _P, C.c_float, _P, _P]
+    lib.fa_oracle_solve_2xS.restype = C.c_int
+    lib.fa_oracle_cosPQR.argtypes = [_P, _P, _P]
+    lib.fa_oracle_cosPQR.restype = C.c_double
+    lib.fa_oracle_spherical_arc.argtypes = [_P, C.c_double, _P, C.c_double, _P, C.c_double, C.c_float]
+    lib.fa_oracle_spherical_arc.restype = C.c_double
+    lib.fa_oracle_log_sum_exp.argtypes = [_P, C.c_int]
+    lib.fa_oracle_log_sum_exp.restype = C.c_double
+    lib.fa_oracle_thermo.argtypes = [_P, _P, C.c_int, C.c_double, _P]
+    lib.fa_oracle_thermo.restype = C.c_int
+    lib.fa_oracle_boltzmann.argtypes = [_P, _P, C.c_int, C.c_double, _P]
+    lib.fa_oracle_boltzmann.restype = C.c_double
+    lib.fa_oracle_log_xi.argtypes = [_P, _P, C.c_int, _P, _P]
+    lib.fa_oracle_log_xi.restype = C.c_double
+    return lib
+
+
+def load_ref(fast=False):
+    path = REF_FAST_SO if fast else REF_SO
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
+    lib.fa_ref_eval.argtypes = [_P, _P, _P, C.c_int, _P, _P, _P, _P, C.c_int, _P]
+    lib.fa_ref_eval.restype = C.c_int
+    return lib
+
+
+def _addr(s):
+    return C.cast(C.pointer(s), C.c_void_p)
+
+
+def oracle_eval(lib, desc, lig_index, genes, f32=False, n_threads=0):
+    genes = np.ascontiguousarray(genes, np.float64)
+    out = np.zeros(genes.shape[0], abi.CF_DTYPE)
+    fn = lib.fa_oracle_eval_f32 if f32 else lib.fa_oracle_eval
+    rc = fn(_addr(desc.receptor), _addr(desc.ligands[lig_index]), _addr(desc.params), genes.shape[0],
+            genes.ctypes.data, out.ctypes.data, n_threads)
+    assert rc == 0
+    return out
+
+
+def oracle_pose(lib, desc, lig_index, genes1, cap=20000):
+    genes1 = np.ascontiguousarray(genes1, np.float64)
+    out = np.zeros(1, abi.CF_DTYPE)
+    L = desc.ligands[lig_index].n_atoms
+    xyz = np.zeros((L, 3), np.float32)
+    recs = np.zeros(cap, abi.REC_DTYPE)
+    n = lib.fa_oracle_eval_pose(_addr(desc.receptor), _addr(desc.ligands[lig_index]), _addr(desc.params),
+                                genes1.ctypes.data, out.ctypes.data, xyz.ctypes.data, recs.ctypes.data, cap)
+    return out[0], xyz, recs[:min(n, cap)]
+
+
+def ref_eval(lib, desc, lig_index, genes, cap=4000):
+    genes = np.ascontiguousarray(genes, np.float64)
+    P = genes.shape[0]
+    L = desc.ligands[lig_index].n_atoms
+    out = np.zeros(P, abi.CF_DTYPE)
+    xyz = np.zeros((P, L, 3), np.float32)
+    recs = np.zeros((P, cap), abi.REC_DTYPE)
+    counts = np.zeros(P, np.int32)
+    rc = lib.fa_ref_eval(_addr(desc.receptor), _addr(desc.ligands[lig_index]), _addr(desc.params), P,
+                         genes.ctypes.data, out.ctypes.data, xyz.ctypes.data, recs.ctypes.data, cap,
+                         counts.ctypes.data)
+    assert rc == 0
+    return out, xyz, [recs[p, :min(counts[p], cap)] for p in range(P)]
