@@ -1,0 +1,512 @@
+// fa_context.hip — device context, system upload and the C ABI.
+//
+// A context owns one GPU's resident state: the receptor (SoA, calc order),
+// one receptor cell list per distinct grid, the ligand topologies, the
+// scoring parameters, and growable pose buffers.  Uploads happen once; the
+// evaluation entry points only move genes in and results out.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "fa_device.h"
+#include "flexaidds_amd.h"
+
+namespace fa {
+hipError_t launch_cf(const DevReceptor& R, const DevGrid* grids, const DevLigand* ligs, const DevParams& prm,
+                     const DevBatch& B, int n_blocks, hipStream_t stream);
+int smem_bytes();
+size_t lse_scratch_bytes();
+hipError_t launch_thermo(const double* d_energy, int stride, const int* d_count, long long n, double T,
+                         void* d_scratch, fa_thermo* d_out, hipStream_t stream);
+}  // namespace fa
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define FA_HIP(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(FA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr float CELLSIZE = 6.5f;  // LIB/Vcontacts.h:56
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct fa_ctx {
+    int device = 0;
+    int n_cu = 0;
+    int max_pop = 0;
+    // resident system
+    std::vector<void*> allocs;
+    fa::DevReceptor rec{};
+    fa::DevParams prm{};
+    fa::DevGrid* d_grids = nullptr;
+    fa::DevLigand* d_ligs = nullptr;
+    std::vector<fa::DevLigand> h_ligs;
+    std::vector<int> lig_genes;
+    int n_ligands = 0;
+    bool have_system = false;
+    // pose buffers
+    DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo;
+    std::vector<uint32_t> last_counters = std::vector<uint32_t>(4, 0);
+
+    void free_system()
+    {
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+        d_grids = nullptr;
+        d_ligs = nullptr;
+        h_ligs.clear();
+        lig_genes.clear();
+        have_system = false;
+    }
+};
+
+namespace {
+
+template <class T> int upload(fa_ctx* c, const T* h, size_t n, const T** d)
+{
+    void* p = nullptr;
+    const size_t bytes = sizeof(T) * (n ? n : 1);
+    FA_HIP(hipMalloc(&p, bytes));
+    c->allocs.push_back(p);
+    if (n) FA_HIP(hipMemcpy(p, h, sizeof(T) * n, hipMemcpyHostToDevice));
+    *d = static_cast<const T*>(p);
+    return FA_SUCCESS;
+}
+
+int ensure(DevBuf& b, size_t bytes)
+{
+    if (b.n >= bytes) return FA_SUCCESS;
+    if (b.p) FA_HIP(hipFree(b.p));
+    b.p = nullptr;
+    b.n = 0;
+    FA_HIP(hipMalloc(&b.p, bytes));
+    b.n = bytes;
+    return FA_SUCCESS;
+}
+
+struct GridKey {
+    float gmin[3], gmax[3], maxwidth;
+    bool operator<(const GridKey& o) const
+    {
+        return std::make_tuple(gmin[0], gmin[1], gmin[2], gmax[0], gmax[1], gmax[2], maxwidth) <
+               std::make_tuple(o.gmin[0], o.gmin[1], o.gmin[2], o.gmax[0], o.gmax[1], o.gmax[2], o.maxwidth);
+    }
+};
+
+int validate_ligand(const fa_ligand_desc& L, int idx)
+{
+    const std::string tag = "ligand " + std::to_string(idx) + ": ";
+    if (L.n_atoms < 3 || L.n_atoms > FA_MAX_LIG_ATOMS)
+        return fail(FA_ERR_LIMIT, tag + "n_atoms must be in [3, " + std::to_string(FA_MAX_LIG_ATOMS) + "]");
+    if (L.n_genes < 1 || L.n_genes > FA_MAX_GENES)
+        return fail(FA_ERR_LIMIT, tag + "n_genes must be in [1, " + std::to_string(FA_MAX_GENES) + "]");
+    if (L.n_grid < 1) return fail(FA_ERR_INVALID, tag + "empty cleft grid");
+    if (!L.rec || !L.dis || !L.ang || !L.dih || !L.shift || !L.radius || !L.type || !L.mov || !L.bonded ||
+        !L.gene_typ || !L.gene_atm || !L.gene_min || !L.gene_max || !L.grid_ic)
+        return fail(FA_ERR_INVALID, tag + "null array");
+    const int n = L.n_atoms;
+    std::vector<int> pos(n, -1);
+    for (int k = 0; k < n; ++k) {
+        const int a = L.mov[k];
+        if (a < 0 || a >= n || pos[a] != -1) return fail(FA_ERR_INVALID, tag + "mov is not a permutation");
+        pos[a] = k;
+    }
+    for (int a = 0; a < n; ++a) {
+        for (int i = 0; i < 3; ++i) {
+            const int r = L.rec[4 * a + i];
+            if (r < -1 || r >= n) return fail(FA_ERR_INVALID, tag + "rec out of range");
+            if (r >= 0 && pos[r] >= pos[a]) return fail(FA_ERR_INVALID, tag + "rec atom rebuilt after its dependant");
+        }
+        const int r3 = L.rec[4 * a + 3];
+        if (r3 < -1 || r3 >= n) return fail(FA_ERR_INVALID, tag + "rec[3] out of range");
+        if (L.type[a] < 1) return fail(FA_ERR_INVALID, tag + "types are 1-based");
+    }
+    for (int g = 0; g < L.n_genes; ++g) {
+        const int t = L.gene_typ[g], a = L.gene_atm[g];
+        if (t < FA_GENE_GRID || t > FA_GENE_DIH) return fail(FA_ERR_INVALID, tag + "unsupported gene type");
+        if (a < 0 || a >= n) return fail(FA_ERR_INVALID, tag + "gene atom out of range");
+        if (t == FA_GENE_DIH && L.rec[4 * a + 3] >= 0) {  // shift chain must return to the gene atom
+            int cat = L.rec[4 * a + 3], steps = 0;
+            while (cat != a && steps <= n) {
+                cat = L.rec[4 * cat + 3];
+                ++steps;
+                if (cat < 0) break;
+            }
+            if (cat != a) return fail(FA_ERR_INVALID, tag + "rec[3] shift chain does not close");
+        }
+        if (t == FA_GENE_GRID && (L.gene_min[g] < 0.0 || L.gene_max[g] > (double)(L.n_grid - 1)))
+            return fail(FA_ERR_INVALID, tag + "grid gene limits outside the cleft grid");
+    }
+    return FA_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fa_last_error(void) { return g_err.c_str(); }
+
+int fa_kernel_lds_bytes(void) { return fa::smem_bytes(); }
+
+int fa_get_device_info(int device_id, fa_device_info* out)
+{
+    if (!out) return fail(FA_ERR_INVALID, "null output");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(FA_ERR_NODEVICE, "no HIP device");
+    if (device_id < 0 || device_id >= n) return fail(FA_ERR_INVALID, "device id out of range");
+    hipDeviceProp_t p;
+    FA_HIP(hipGetDeviceProperties(&p, device_id));
+    std::memset(out, 0, sizeof(*out));
+    std::snprintf(out->name, sizeof(out->name), "%s", p.name);
+    out->compute_units = p.multiProcessorCount;
+    out->global_mem = p.totalGlobalMem;
+    out->gcn_arch = 0;
+    std::sscanf(p.gcnArchName, "gfx%d", &out->gcn_arch);
+    out->wavefront_size = p.warpSize;
+    out->max_threads_per_cu = p.maxThreadsPerMultiProcessor;
+    out->shared_mem_per_cu = (int)p.maxSharedMemoryPerMultiProcessor;
+    return FA_SUCCESS;
+}
+
+int fa_ctx_create(int device_id, int max_pop, fa_ctx** out)
+{
+    if (!out) return fail(FA_ERR_INVALID, "null output");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(FA_ERR_NODEVICE, "no HIP device");
+    if (device_id < 0 || device_id >= n) return fail(FA_ERR_INVALID, "device id out of range");
+    FA_HIP(hipSetDevice(device_id));
+    hipDeviceProp_t p;
+    FA_HIP(hipGetDeviceProperties(&p, device_id));
+    fa_ctx* c = new fa_ctx();
+    c->device = device_id;
+    c->n_cu = p.multiProcessorCount;
+    c->max_pop = max_pop > 0 ? max_pop : 4096;
+    int rc;
+    if ((rc = ensure(c->counters, 4 * sizeof(uint32_t))) != FA_SUCCESS ||
+        (rc = ensure(c->scratch, fa::lse_scratch_bytes())) != FA_SUCCESS ||
+        (rc = ensure(c->thermo, sizeof(fa_thermo))) != FA_SUCCESS ||
+        (rc = ensure(c->genes, sizeof(double) * (size_t)c->max_pop * 32)) != FA_SUCCESS ||
+        (rc = ensure(c->out, sizeof(fa_cf_result) * (size_t)c->max_pop)) != FA_SUCCESS) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return FA_SUCCESS;
+}
+
+int fa_ctx_destroy(fa_ctx* c)
+{
+    if (!c) return FA_SUCCESS;
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    c->free_system();
+    for (DevBuf* b : {&c->genes, &c->out, &c->counters, &c->recs, &c->reccnt, &c->xyz, &c->scratch, &c->thermo})
+        if (b->p) (void)hipFree(b->p);
+    delete c;
+    return FA_SUCCESS;
+}
+
+int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm, int n_ligands,
+                     const fa_ligand_desc* ligs)
+{
+    if (!c || !r || !prm || !ligs || n_ligands < 1) return fail(FA_ERR_INVALID, "null argument");
+    if (r->n_atoms < 1 || !r->xyz || !r->radius || !r->type) return fail(FA_ERR_INVALID, "empty receptor");
+    if (prm->ntypes < 1 || !prm->emat) return fail(FA_ERR_INVALID, "energy matrix missing");
+    for (int i = 0; i < r->n_atoms; ++i)
+        if (r->type[i] < 1 || r->type[i] > prm->ntypes) return fail(FA_ERR_INVALID, "receptor type out of range");
+    for (int li = 0; li < n_ligands; ++li) {
+        int rc = validate_ligand(ligs[li], li);
+        if (rc != FA_SUCCESS) return rc;
+        for (int a = 0; a < ligs[li].n_atoms; ++a)
+            if (ligs[li].type[a] > prm->ntypes) return fail(FA_ERR_INVALID, "ligand type out of range");
+    }
+    FA_HIP(hipSetDevice(c->device));
+    FA_HIP(hipDeviceSynchronize());
+    c->free_system();
+    int rc;
+    const int Rn = r->n_atoms;
+    // receptor SoA (calc order)
+    std::vector<float> x(Rn), y(Rn), z(Rn);
+    for (int i = 0; i < Rn; ++i) {
+        x[i] = r->xyz[3 * i];
+        y[i] = r->xyz[3 * i + 1];
+        z[i] = r->xyz[3 * i + 2];
+    }
+    c->rec.n = Rn;
+    if ((rc = upload(c, x.data(), Rn, &c->rec.x)) || (rc = upload(c, y.data(), Rn, &c->rec.y)) ||
+        (rc = upload(c, z.data(), Rn, &c->rec.z)) || (rc = upload(c, r->radius, Rn, &c->rec.rad)) ||
+        (rc = upload(c, r->type, Rn, &c->rec.type)))
+        return rc;
+    // parameters
+    c->prm.permeability = prm->permeability;
+    c->prm.dee_clash = prm->dee_clash;
+    c->prm.solventterm = prm->solventterm;
+    c->prm.intramolecular = prm->intramolecular;
+    c->prm.ntypes = prm->ntypes;
+    if ((rc = upload(c, prm->emat, (size_t)prm->ntypes * prm->ntypes, &c->prm.emat))) return rc;
+    // grids: one receptor cell list per distinct (rounded bounds, width)
+    // (LIB/Vcontacts.cpp:1594-1715 with no scorable atom outside the bounds)
+    std::map<GridKey, int> grid_ids;
+    std::vector<fa::DevGrid> grids;
+    c->h_ligs.resize(n_ligands);
+    c->lig_genes.resize(n_ligands);
+    for (int li = 0; li < n_ligands; ++li) {
+        const fa_ligand_desc& L = ligs[li];
+        GridKey k;
+        for (int j = 0; j < 3; ++j) {
+            k.gmin[j] = (float)((int)L.globalmin[j] + 1.0);
+            k.gmax[j] = (float)((int)L.globalmax[j] + 1.0);
+        }
+        k.maxwidth = L.maxwidth;
+        auto it = grid_ids.find(k);
+        int gid;
+        if (it != grid_ids.end()) {
+            gid = it->second;
+        } else {
+            fa::DevGrid G{};
+            for (int j = 0; j < 3; ++j) {
+                G.gmin[j] = k.gmin[j];
+                G.gmax[j] = k.gmax[j];
+            }
+            G.maxwidth = k.maxwidth;
+            G.dim = (int)(k.maxwidth / CELLSIZE) + 1;
+            const long long dim2 = (long long)G.dim * G.dim, dim3 = dim2 * G.dim;
+            if (G.dim < 1 || dim3 > (1LL << 26)) return fail(FA_ERR_LIMIT, "cell grid too large");
+            std::vector<int> box(Rn), cnt(dim3 + 1, 0);
+            for (int i = 0; i < Rn; ++i) {
+                const long long b = (long long)(int)((x[i] - G.gmin[0]) / CELLSIZE) * dim2 +
+                                    (long long)(int)((y[i] - G.gmin[1]) / CELLSIZE) * G.dim +
+                                    (int)((z[i] - G.gmin[2]) / CELLSIZE);
+                if (b < 0 || b >= dim3) return fail(FA_ERR_INVALID, "receptor atom outside the cell grid");
+                box[i] = (int)b;
+                ++cnt[b + 1];
+            }
+            for (long long b = 0; b < dim3; ++b) cnt[b + 1] += cnt[b];
+            std::vector<int> fill(cnt.begin(), cnt.end() - 1), atom(Rn);
+            std::vector<float4> xyzr(Rn);
+            for (int i = 0; i < Rn; ++i) {  // stable: calc order inside a cell
+                const int pos = fill[box[i]]++;
+                atom[pos] = i;
+                xyzr[pos] = make_float4(x[i], y[i], z[i], r->radius[i]);
+            }
+            if ((rc = upload(c, cnt.data(), (size_t)dim3 + 1, &G.cell_start)) ||
+                (rc = upload(c, atom.data(), Rn, &G.cell_atom)) || (rc = upload(c, xyzr.data(), Rn, &G.cell_xyzr)))
+                return rc;
+            gid = (int)grids.size();
+            grids.push_back(G);
+            grid_ids[k] = gid;
+        }
+        // ligand topology
+        fa::DevLigand D{};
+        D.n_atoms = L.n_atoms;
+        D.n_genes = L.n_genes;
+        D.n_grid = L.n_grid;
+        D.grid_id = gid;
+        for (int j = 0; j < 3; ++j) {
+            D.ori[j] = L.ori[j];
+            D.globalmin[j] = L.globalmin[j];
+            D.globalmax[j] = L.globalmax[j];
+        }
+        D.maxwidth = L.maxwidth;
+        const int n = L.n_atoms;
+        std::vector<unsigned long long> bits((size_t)n * 2, 0ull);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+                if (L.bonded[i * n + j]) bits[2 * i + (j >> 6)] |= 1ull << (j & 63);
+        if ((rc = upload(c, L.rec, (size_t)4 * n, &D.rec)) || (rc = upload(c, L.dis, n, &D.dis)) ||
+            (rc = upload(c, L.ang, n, &D.ang)) || (rc = upload(c, L.dih, n, &D.dih)) ||
+            (rc = upload(c, L.shift, n, &D.shift)) || (rc = upload(c, L.radius, n, &D.radius)) ||
+            (rc = upload(c, L.type, n, &D.type)) || (rc = upload(c, L.mov, n, &D.mov)) ||
+            (rc = upload(c, bits.data(), bits.size(), &D.bonded)) ||
+            (rc = upload(c, L.gene_typ, L.n_genes, &D.gene_typ)) ||
+            (rc = upload(c, L.gene_atm, L.n_genes, &D.gene_atm)) ||
+            (rc = upload(c, L.gene_min, L.n_genes, &D.gene_min)) ||
+            (rc = upload(c, L.gene_max, L.n_genes, &D.gene_max)) ||
+            (rc = upload(c, L.grid_ic, (size_t)3 * L.n_grid, &D.grid_ic)))
+            return rc;
+        c->h_ligs[li] = D;
+        c->lig_genes[li] = L.n_genes;
+    }
+    const fa::DevGrid* dg = nullptr;
+    const fa::DevLigand* dl = nullptr;
+    if ((rc = upload(c, grids.data(), grids.size(), &dg)) || (rc = upload(c, c->h_ligs.data(), n_ligands, &dl)))
+        return rc;
+    c->d_grids = const_cast<fa::DevGrid*>(dg);
+    c->d_ligs = const_cast<fa::DevLigand*>(dl);
+    c->n_ligands = n_ligands;
+    c->have_system = true;
+    return FA_SUCCESS;
+}
+
+static int launch(fa_ctx* c, int n_poses, const double* d_genes, int stride, const int* d_lig, fa_cf_result* d_out,
+                  fa_contact_rec* d_recs, int rec_cap, int* d_reccnt, float* d_xyz, int xyz_stride,
+                  hipStream_t stream)
+{
+    fa::DevBatch B{};
+    B.n_poses = n_poses;
+    B.gene_stride = stride;
+    B.genes = d_genes;
+    B.pose_lig = d_lig;
+    B.out = d_out;
+    B.recs = d_recs;
+    B.rec_cap = rec_cap;
+    B.rec_count = d_reccnt;
+    B.lig_xyz = d_xyz;
+    B.xyz_stride = xyz_stride;
+    B.counters = static_cast<unsigned int*>(c->counters.p);
+    FA_HIP(hipMemsetAsync(c->counters.p, 0, 4 * sizeof(uint32_t), stream));
+    // enough resident waves to cover the chip; each wave loops over poses
+    const int lds = fa::smem_bytes();
+    int per_cu = (160 * 1024) / (lds > 0 ? lds : 1);
+    if (per_cu < 1) per_cu = 1;
+    const int n_blocks = c->n_cu * per_cu * 4;
+    FA_HIP(fa::launch_cf(c->rec, c->d_grids, c->d_ligs, c->prm, B, n_blocks, stream));
+    return FA_SUCCESS;
+}
+
+int fa_eval_batch_device(fa_ctx* c, int n_poses, const double* genes, int gene_stride, const int* pose_lig,
+                         fa_cf_result* out, void* stream)
+{
+    if (!c) return fail(FA_ERR_INVALID, "null context");
+    if (!c->have_system) return fail(FA_ERR_NOSYSTEM, "no system uploaded");
+    if (n_poses < 0 || (n_poses > 0 && (!genes || !out))) return fail(FA_ERR_INVALID, "bad batch");
+    if (gene_stride < 1) return fail(FA_ERR_INVALID, "gene_stride must be positive");
+    FA_HIP(hipSetDevice(c->device));
+    return launch(c, n_poses, genes, gene_stride, pose_lig, out, nullptr, 0, nullptr, nullptr, 0,
+                  static_cast<hipStream_t>(stream));
+}
+
+static int eval_host(fa_ctx* c, int ligand, int n_poses, const double* genes, fa_cf_result* out, float* lig_xyz,
+                     fa_contact_rec* recs, int rec_cap, int* rec_count)
+{
+    if (!c) return fail(FA_ERR_INVALID, "null context");
+    if (!c->have_system) return fail(FA_ERR_NOSYSTEM, "no system uploaded");
+    if (ligand < 0 || ligand >= c->n_ligands) return fail(FA_ERR_INVALID, "ligand index out of range");
+    if (n_poses < 0 || (n_poses > 0 && (!genes || !out))) return fail(FA_ERR_INVALID, "bad batch");
+    if (n_poses == 0) return FA_SUCCESS;
+    FA_HIP(hipSetDevice(c->device));
+    const int G = c->lig_genes[ligand];
+    int rc;
+    if ((rc = ensure(c->genes, sizeof(double) * (size_t)n_poses * G)) ||
+        (rc = ensure(c->out, sizeof(fa_cf_result) * (size_t)n_poses)))
+        return rc;
+    std::vector<int> lig(n_poses, ligand);
+    DevBuf ligbuf;
+    if ((rc = ensure(ligbuf, sizeof(int) * (size_t)n_poses))) return rc;
+    fa_contact_rec* d_recs = nullptr;
+    int* d_cnt = nullptr;
+    float* d_xyz = nullptr;
+    const int L = c->h_ligs[ligand].n_atoms;
+    if (recs && rec_count && rec_cap > 0) {
+        if ((rc = ensure(c->recs, sizeof(fa_contact_rec) * (size_t)n_poses * rec_cap)) ||
+            (rc = ensure(c->reccnt, sizeof(int) * (size_t)n_poses)))
+            return rc;
+        d_recs = static_cast<fa_contact_rec*>(c->recs.p);
+        d_cnt = static_cast<int*>(c->reccnt.p);
+    }
+    if (lig_xyz) {
+        if ((rc = ensure(c->xyz, sizeof(float) * 3 * (size_t)n_poses * L))) return rc;
+        d_xyz = static_cast<float*>(c->xyz.p);
+    }
+    hipError_t e = hipMemcpy(c->genes.p, genes, sizeof(double) * (size_t)n_poses * G, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ligbuf.p, lig.data(), sizeof(int) * (size_t)n_poses, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(ligbuf.p);
+        return fail(FA_ERR_HIP, std::string("upload genes: ") + hipGetErrorString(e));
+    }
+    rc = launch(c, n_poses, static_cast<const double*>(c->genes.p), G, static_cast<const int*>(ligbuf.p),
+                static_cast<fa_cf_result*>(c->out.p), d_recs, rec_cap, d_cnt, d_xyz, L, nullptr);
+    if (rc == FA_SUCCESS) {
+        e = hipMemcpy(out, c->out.p, sizeof(fa_cf_result) * (size_t)n_poses, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && d_recs)
+            e = hipMemcpy(recs, d_recs, sizeof(fa_contact_rec) * (size_t)n_poses * rec_cap, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && d_cnt)
+            e = hipMemcpy(rec_count, d_cnt, sizeof(int) * (size_t)n_poses, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && d_xyz)
+            e = hipMemcpy(lig_xyz, d_xyz, sizeof(float) * 3 * (size_t)n_poses * L, hipMemcpyDeviceToHost);
+        if (e == hipSuccess)
+            e = hipMemcpy(c->last_counters.data(), c->counters.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(FA_ERR_HIP, std::string("download results: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(ligbuf.p);
+    return rc;
+}
+
+int fa_eval_batch(fa_ctx* c, int ligand, int n_poses, const double* genes, fa_cf_result* out)
+{
+    return eval_host(c, ligand, n_poses, genes, out, nullptr, nullptr, 0, nullptr);
+}
+
+int fa_eval_batch_debug(fa_ctx* c, int ligand, int n_poses, const double* genes, fa_cf_result* out, float* lig_xyz,
+                        fa_contact_rec* recs, int rec_cap, int* rec_count)
+{
+    return eval_host(c, ligand, n_poses, genes, out, lig_xyz, recs, rec_cap, rec_count);
+}
+
+int fa_thermo_device(fa_ctx* c, const double* energy, int stride, const int* count, int64_t n, double T,
+                     fa_thermo* out, void* stream)
+{
+    if (!c || !energy || !out || n < 1 || stride < 1) return fail(FA_ERR_INVALID, "bad thermo arguments");
+    if (!(T > 0.0)) return fail(FA_ERR_INVALID, "temperature must be > 0");
+    FA_HIP(hipSetDevice(c->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    FA_HIP(fa::launch_thermo(energy, stride, count, n, T, c->scratch.p, static_cast<fa_thermo*>(c->thermo.p), s));
+    FA_HIP(hipMemcpyAsync(out, c->thermo.p, sizeof(fa_thermo), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    return FA_SUCCESS;
+}
+
+int fa_thermo_host(fa_ctx* c, const double* energy, const int* count, int64_t n, double T, fa_thermo* out)
+{
+    if (!c || !energy || !out || n < 1) return fail(FA_ERR_INVALID, "bad thermo arguments");
+    FA_HIP(hipSetDevice(c->device));
+    DevBuf e, k;
+    int rc = ensure(e, sizeof(double) * (size_t)n);
+    if (rc) return rc;
+    if (count && (rc = ensure(k, sizeof(int) * (size_t)n))) {
+        (void)hipFree(e.p);
+        return rc;
+    }
+    hipError_t he = hipMemcpy(e.p, energy, sizeof(double) * (size_t)n, hipMemcpyHostToDevice);
+    if (he == hipSuccess && count) he = hipMemcpy(k.p, count, sizeof(int) * (size_t)n, hipMemcpyHostToDevice);
+    if (he != hipSuccess)
+        rc = fail(FA_ERR_HIP, std::string("thermo upload: ") + hipGetErrorString(he));
+    else
+        rc = fa_thermo_device(c, static_cast<const double*>(e.p), 1, static_cast<const int*>(k.p), n, T, out, nullptr);
+    (void)hipFree(e.p);
+    if (k.p) (void)hipFree(k.p);
+    return rc;
+}
+
+int fa_last_counters(fa_ctx* c, uint32_t* slow_grid, uint32_t* capacity)
+{
+    if (!c) return fail(FA_ERR_INVALID, "null context");
+    if (slow_grid) *slow_grid = c->last_counters[0];
+    if (capacity) *capacity = c->last_counters[1];
+    return FA_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// fa_device.h — device-side system tables shared by the kernels and the host
+// context code.  Everything here is plain data resident in HBM.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fa_types.h"
+
+namespace fa {
+
+// Receptor atoms in calc order (structure of arrays).
+struct DevReceptor {
+    int n;
+    const float* x;
+    const float* y;
+    const float* z;
+    const float* rad;
+    const int* type;
+};
+
+// Cell list of the receptor for one grid (the grid index_protein builds when
+// no scorable atom widens it).  cell_xyzr / cell_atom are the receptor atoms
+// sorted by cell, calc order inside a cell.
+struct DevGrid {
+    float gmin[3];
+    float gmax[3];
+    float maxwidth;
+    int dim;
+    const int* cell_start;   // [dim^3 + 1]
+    const int* cell_atom;    // [n_receptor] calc index
+    const float4* cell_xyzr; // [n_receptor] x, y, z, radius
+};
+
+// One ligand topology (pointers into the context's device pool).
+struct DevLigand {
+    int n_atoms;
+    int n_genes;
+    int n_grid;
+    int grid_id;
+    float ori[3];
+    float globalmin[3];
+    float globalmax[3];
+    float maxwidth;
+    const int* rec;                 // [4n]
+    const float* dis;               // [n]
+    const float* ang;
+    const float* dih;
+    const float* shift;
+    const float* radius;
+    const int* type;
+    const int* mov;                 // [n]
+    const unsigned long long* bonded; // [n * 2] bitset rows
+    const int* gene_typ;            // [g]
+    const int* gene_atm;
+    const double* gene_min;
+    const double* gene_max;
+    const float* grid_ic;           // [3 * n_grid]
+};
+
+struct DevParams {
+    float permeability;
+    float dee_clash;
+    float solventterm;
+    int intramolecular;
+    int ntypes;
+    const float* emat;              // [ntypes * ntypes]
+};
+
+// One launch over a batch of poses.
+struct DevBatch {
+    int n_poses;
+    int gene_stride;
+    const double* genes;            // [n_poses * gene_stride]
+    const int* pose_lig;            // [n_poses] or null (ligand 0)
+    fa_cf_result* out;              // [n_poses]
+    fa_contact_rec* recs;           // optional: [n_poses * rec_cap]
+    int rec_cap;
+    int* rec_count;                 // optional: [n_poses]
+    float* lig_xyz;                 // optional: [n_poses * xyz_stride * 3]
+    int xyz_stride;
+    unsigned int* counters;         // [4]: slow-grid poses, capacity overflows, work queue, spare
+};
+
+}  // namespace fa

@@ -1,0 +1,149 @@
+// statmech.hip — FP64 canonical-ensemble reduction on the GPU.
+//
+// Computes ln Z, <E>, <E^2> over energies E_i with multiplicities n_i at
+// temperature T, the quantities of statmech::StatMechEngine::compute
+// (LIB/statmech.cpp:73-173): log w_i = ln n_i - beta E_i, ln Z = LSE(log w),
+// <E> = sum E_i exp(log w_i - ln Z).  Each thread folds its elements into an
+// online log-sum-exp state (max m, s = sum exp(lw - m), s1 = sum E exp(..),
+// s2 = sum E^2 exp(..)); states merge by rescaling to the larger max.  One
+// pass over HBM, FP64 throughout.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "fa_types.h"
+
+namespace fa {
+
+constexpr double KB_KCAL = 0.001987206;  // LIB/statmech.h:26
+
+struct LseState {
+    double m, s, s1, s2;
+};
+
+__device__ __forceinline__ void lse_merge(LseState& a, const LseState& b)
+{
+    if (b.s == 0.0) return;
+    if (a.s == 0.0) {
+        a = b;
+        return;
+    }
+    if (b.m > a.m) {
+        const double f = exp(a.m - b.m);
+        a.s = a.s * f + b.s;
+        a.s1 = a.s1 * f + b.s1;
+        a.s2 = a.s2 * f + b.s2;
+        a.m = b.m;
+    } else {
+        const double f = exp(b.m - a.m);
+        a.s += b.s * f;
+        a.s1 += b.s1 * f;
+        a.s2 += b.s2 * f;
+    }
+}
+
+__device__ __forceinline__ void lse_push(LseState& a, double lw, double e)
+{
+    if (a.s == 0.0) {
+        a.m = lw;
+        a.s = 1.0;
+        a.s1 = e;
+        a.s2 = e * e;
+    } else if (lw > a.m) {
+        const double f = exp(a.m - lw);
+        a.s = a.s * f + 1.0;
+        a.s1 = a.s1 * f + e;
+        a.s2 = a.s2 * f + e * e;
+        a.m = lw;
+    } else {
+        const double w = exp(lw - a.m);
+        a.s += w;
+        a.s1 += w * e;
+        a.s2 += w * e * e;
+    }
+}
+
+__device__ void block_reduce(LseState& st, LseState* sh)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        LseState o;
+        o.m = __shfl_xor(st.m, off);
+        o.s = __shfl_xor(st.s, off);
+        o.s1 = __shfl_xor(st.s1, off);
+        o.s2 = __shfl_xor(st.s2, off);
+        lse_merge(st, o);
+    }
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) sh[w] = st;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        LseState acc = sh[0];
+        for (int k = 1; k < (int)(blockDim.x / 64); ++k) lse_merge(acc, sh[k]);
+        sh[0] = acc;
+    }
+    __syncthreads();
+    st = sh[0];
+}
+
+// pass 1: one state per block
+__global__ __launch_bounds__(256) void lse_partial_kernel(const double* __restrict__ energy, int stride,
+                                                           const int* __restrict__ count, long long n, double beta,
+                                                           LseState* __restrict__ part)
+{
+    __shared__ LseState sh[4];
+    LseState st{0.0, 0.0, 0.0, 0.0};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double e = energy[i * stride];
+        const double c = count ? (double)count[i] : 1.0;
+        lse_push(st, log(c) - beta * e, e);
+    }
+    block_reduce(st, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = st;
+}
+
+// pass 2: fold the block states, write the thermodynamics
+__global__ __launch_bounds__(256) void lse_final_kernel(const LseState* __restrict__ part, int nparts, double T,
+                                                         fa_thermo* __restrict__ out)
+{
+    __shared__ LseState sh[4];
+    LseState st{0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) lse_merge(st, part[i]);
+    block_reduce(st, sh);
+    if (threadIdx.x == 0) {
+        const double lnZ = st.m + log(st.s);
+        const double e1 = st.s1 / st.s;
+        const double e2 = st.s2 / st.s;
+        const double kT = KB_KCAL * T;
+        const double var = e2 - e1 * e1;
+        fa_thermo th;
+        th.temperature = T;
+        th.log_Z = lnZ;
+        th.free_energy = -kT * lnZ;
+        th.mean_energy = e1;
+        th.mean_energy_sq = e2;
+        th.heat_capacity = var / (kT * kT);
+        th.entropy = (e1 - th.free_energy) / T;
+        th.std_energy = sqrt(var > 0.0 ? var : 0.0);
+        *out = th;
+    }
+}
+
+constexpr int LSE_MAX_PARTS = 1024;
+
+size_t lse_scratch_bytes() { return sizeof(LseState) * LSE_MAX_PARTS; }
+
+hipError_t launch_thermo(const double* d_energy, int stride, const int* d_count, long long n, double T,
+                         void* d_scratch, fa_thermo* d_out, hipStream_t stream)
+{
+    const double beta = 1.0 / (KB_KCAL * T);
+    long long blocks = (n + 255) / 256;
+    if (blocks > LSE_MAX_PARTS) blocks = LSE_MAX_PARTS;
+    if (blocks < 1) blocks = 1;
+    LseState* part = static_cast<LseState*>(d_scratch);
+    hipLaunchKernelGGL(lse_partial_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, d_energy, stride, d_count, n,
+                       beta, part);
+    hipLaunchKernelGGL(lse_final_kernel, dim3(1), dim3(256), 0, stream, part, (int)blocks, T, d_out);
+    return hipGetLastError();
+}
+
+}  // namespace fa

@@ -1,0 +1,162 @@
+"""ctypes binding of the HIP library's C ABI (include/flexaidds_amd.h).
+
+The library is required: importing this module raises when
+`libflexaidds_amd.so` is missing, and every call raises on a non-zero status.
+There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libflexaidds_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built: run `python -m flexaidds_amd.build` (or __graft_entry__.build())")
+
+_lib = C.CDLL(LIB_PATH)
+_P = C.c_void_p
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("compute_units", C.c_int32), ("global_mem", C.c_uint64),
+                ("gcn_arch", C.c_int32), ("wavefront_size", C.c_int32), ("max_threads_per_cu", C.c_int32),
+                ("shared_mem_per_cu", C.c_int32)]
+
+
+_SIGS = {
+    "fa_get_device_info": ([C.c_int, _P], C.c_int),
+    "fa_last_error": ([], C.c_char_p),
+    "fa_ctx_create": ([C.c_int, C.c_int, _P], C.c_int),
+    "fa_ctx_destroy": ([_P], C.c_int),
+    "fa_upload_system": ([_P, _P, _P, C.c_int, _P], C.c_int),
+    "fa_eval_batch": ([_P, C.c_int, C.c_int, _P, _P], C.c_int),
+    "fa_eval_batch_device": ([_P, C.c_int, _P, C.c_int, _P, _P, _P], C.c_int),
+    "fa_eval_batch_debug": ([_P, C.c_int, C.c_int, _P, _P, _P, _P, C.c_int, _P], C.c_int),
+    "fa_thermo_device": ([_P, _P, C.c_int, _P, C.c_int64, C.c_double, _P, _P], C.c_int),
+    "fa_thermo_host": ([_P, _P, _P, C.c_int64, C.c_double, _P], C.c_int),
+    "fa_last_counters": ([_P, _P, _P], C.c_int),
+    "fa_kernel_lds_bytes": ([], C.c_int),
+}
+
+EXPORTED = tuple(_SIGS)
+
+for _name, (_args, _res) in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.argtypes = _args
+    _f.restype = _res
+
+
+class FAError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise FAError(f"status {rc}: {_lib.fa_last_error().decode()}")
+
+
+def device_info(device=0) -> dict:
+    d = DeviceInfo()
+    _check(_lib.fa_get_device_info(device, C.byref(d)))
+    return {"name": d.name.decode(), "compute_units": d.compute_units, "global_mem": d.global_mem,
+            "gcn_arch": d.gcn_arch, "wavefront_size": d.wavefront_size,
+            "max_threads_per_cu": d.max_threads_per_cu, "shared_mem_per_cu": d.shared_mem_per_cu}
+
+
+def kernel_lds_bytes() -> int:
+    return int(_lib.fa_kernel_lds_bytes())
+
+
+def _addr(s):
+    return C.cast(C.pointer(s), C.c_void_p)
+
+
+class Context:
+    """One GPU's resident docking system (receptor, ligands, parameters)."""
+
+    def __init__(self, device: int = 0, max_pop: int = 0):
+        self._ctx = C.c_void_p()
+        _check(_lib.fa_ctx_create(device, max_pop, C.byref(self._ctx)))
+        self._desc = None
+        self.n_genes = []
+
+    def close(self):
+        if self._ctx:
+            _lib.fa_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, receptor, ligands, params):
+        desc = abi.Descriptors(receptor, ligands, params)
+        arr = (abi.LigandDesc * len(desc.ligands))(*desc.ligands)
+        _check(_lib.fa_upload_system(self._ctx, _addr(desc.receptor), _addr(desc.params), len(desc.ligands),
+                                     C.cast(arr, C.c_void_p)))
+        self._desc = desc
+        self.n_genes = [l.n_genes for l in ligands]
+        self.n_atoms = [l.n_atoms for l in ligands]
+
+    def upload_system(self, system):
+        self.upload(system.receptor, system.ligands, system.params)
+
+    def eval(self, ligand: int, genes: np.ndarray) -> np.ndarray:
+        genes = np.ascontiguousarray(genes, np.float64)
+        assert genes.ndim == 2 and genes.shape[1] == self.n_genes[ligand]
+        out = np.zeros(genes.shape[0], abi.CF_DTYPE)
+        _check(_lib.fa_eval_batch(self._ctx, ligand, genes.shape[0], genes.ctypes.data, out.ctypes.data))
+        return out
+
+    def eval_debug(self, ligand: int, genes: np.ndarray, rec_cap: int = 2048):
+        genes = np.ascontiguousarray(genes, np.float64)
+        P = genes.shape[0]
+        L = self.n_atoms[ligand]
+        out = np.zeros(P, abi.CF_DTYPE)
+        xyz = np.zeros((P, L, 3), np.float32)
+        recs = np.zeros((P, rec_cap), abi.REC_DTYPE)
+        cnt = np.zeros(P, np.int32)
+        _check(_lib.fa_eval_batch_debug(self._ctx, ligand, P, genes.ctypes.data, out.ctypes.data, xyz.ctypes.data,
+                                        recs.ctypes.data, rec_cap, cnt.ctypes.data))
+        return out, xyz, [recs[p, :min(cnt[p], rec_cap)] for p in range(P)]
+
+    def eval_device(self, n_poses: int, genes_ptr: int, gene_stride: int, pose_lig_ptr: int, out_ptr: int,
+                    stream: int = 0):
+        """Enqueue an evaluation over device buffers (raw device pointers)."""
+        _check(_lib.fa_eval_batch_device(self._ctx, n_poses, C.c_void_p(genes_ptr), gene_stride,
+                                         C.c_void_p(pose_lig_ptr) if pose_lig_ptr else None,
+                                         C.c_void_p(out_ptr), C.c_void_p(stream) if stream else None))
+
+    def thermo(self, energy: np.ndarray, temperature: float = 300.0, count=None) -> dict:
+        e = np.ascontiguousarray(energy, np.float64)
+        cnt = None if count is None else np.ascontiguousarray(count, np.int32)
+        th = abi.Thermo()
+        _check(_lib.fa_thermo_host(self._ctx, e.ctypes.data, None if cnt is None else cnt.ctypes.data,
+                                   e.shape[0], temperature, C.byref(th)))
+        return {f: getattr(th, f) for f, _ in abi.Thermo._fields_}
+
+    def thermo_device(self, energy_ptr: int, stride: int, n: int, temperature: float = 300.0,
+                      count_ptr: int = 0, stream: int = 0) -> dict:
+        th = abi.Thermo()
+        _check(_lib.fa_thermo_device(self._ctx, C.c_void_p(energy_ptr), stride,
+                                     C.c_void_p(count_ptr) if count_ptr else None, n, temperature, C.byref(th),
+                                     C.c_void_p(stream) if stream else None))
+        return {f: getattr(th, f) for f, _ in abi.Thermo._fields_}
+
+    def counters(self):
+        a, b = C.c_uint32(), C.c_uint32()
+        _check(_lib.fa_last_counters(self._ctx, C.byref(a), C.byref(b)))
+        return {"slow_grid": a.value, "capacity": b.value}

@@ -1,0 +1,115 @@
+/*
+ * flexaidds_amd.h — C ABI of the MI355X population-evaluation library.
+ *
+ * Plain pointers, sizes and int status codes; no C++ or torch types.  Every
+ * entry point lists the reference interface it replaces.  The C++ drop-ins
+ * (hip_eval::, voronoi_cf::batch_eval, UnifiedHardwareDispatch ROCM routing)
+ * are thin layers over these calls (include/hip_eval.h).
+ *
+ * Ownership: contexts are created and destroyed by the caller; host arrays
+ * are borrowed for the duration of a call.  Threading: one context serves one
+ * thread at a time; distinct contexts may be used concurrently.
+ */
+#ifndef FLEXAIDDS_AMD_H
+#define FLEXAIDDS_AMD_H
+
+#include <stdint.h>
+
+#include "fa_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FA_SUCCESS 0
+#define FA_ERR_INVALID (-1)   /* bad argument or malformed system */
+#define FA_ERR_HIP (-2)       /* HIP runtime error */
+#define FA_ERR_NODEVICE (-3)  /* no usable GPU */
+#define FA_ERR_NOSYSTEM (-4)  /* no system uploaded */
+#define FA_ERR_LIMIT (-5)     /* a size exceeds a device limit */
+
+typedef struct fa_ctx fa_ctx;
+
+/* Device properties (hip_eval::DeviceInfo, LIB/hip_eval.h:48-56). */
+typedef struct fa_device_info {
+    char name[256];
+    int32_t compute_units;
+    uint64_t global_mem;
+    int32_t gcn_arch;        /* e.g. 950 */
+    int32_t wavefront_size;
+    int32_t max_threads_per_cu;
+    int32_t shared_mem_per_cu;
+} fa_device_info;
+
+/* Replaces hip_eval::get_device_info (LIB/hip_eval.h:60). */
+int fa_get_device_info(int device_id, fa_device_info* out);
+
+/* Last error message of the calling thread (never NULL). */
+const char* fa_last_error(void);
+
+/* Replaces hip_eval::init (LIB/hip_eval.h:74) and cuda_eval_init's context
+ * creation (LIB/cuda_eval.cuh:46-56): opens `device_id` and returns a context
+ * with buffers for `max_pop` poses (0 = 4096, as FLEXAIDS_HIP_DEFAULT_MAX_POP,
+ * LIB/hip_eval.hip:63-65); buffers grow on demand. */
+int fa_ctx_create(int device_id, int max_pop, fa_ctx** out);
+
+/* Replaces hip_eval::shutdown (LIB/hip_eval.h:78) / cuda_eval_shutdown
+ * (LIB/cuda_eval.cuh:75). */
+int fa_ctx_destroy(fa_ctx* ctx);
+
+/* Additive: the system upload the real contact function needs (the reference
+ * hip_eval.h has none; cuda_eval_init carries a partial one,
+ * LIB/cuda_eval.cuh:46-56).  Receptor + scoring parameters + `n_ligands`
+ * ligand topologies are copied to HBM and stay resident; receptor cell lists
+ * are built once per distinct grid.  Replaces any previous system. */
+int fa_upload_system(fa_ctx* ctx, const fa_receptor_desc* receptor, const fa_params* params, int n_ligands,
+                     const fa_ligand_desc* ligands);
+
+/* Replaces hip_eval::batch_evaluate (LIB/hip_eval.h:117-127) and
+ * cuda_eval_batch (LIB/cuda_eval.cuh:58-73): synchronous, host buffers.
+ * `genes` is [n_poses * n_genes] internal-coordinate values of ligand
+ * `ligand` (the `to_ic` doubles the GA packs, LIB/gaboom.cpp:1288-1294);
+ * `out` receives one fa_cf_result per pose. */
+int fa_eval_batch(fa_ctx* ctx, int ligand, int n_poses, const double* genes, fa_cf_result* out);
+
+/* Device-resident form for batched and served callers: all pointers are
+ * device pointers, the call only enqueues work on `stream` (a hipStream_t,
+ * NULL = default stream).  `pose_lig` (may be NULL = ligand 0) selects the
+ * ligand of each pose; rows of `genes` are `gene_stride` doubles apart.
+ * No reference counterpart (the reference path is synchronous per call). */
+int fa_eval_batch_device(fa_ctx* ctx, int n_poses, const double* genes, int gene_stride, const int* pose_lig,
+                         fa_cf_result* out, void* stream);
+
+/* Debug/parity form of fa_eval_batch: also returns the rebuilt ligand
+ * coordinates ([n_poses * n_atoms * 3], may be NULL) and each pose's contact
+ * records (ca_rec lists of LIB/Vcontacts.cpp:1149-1220, calc-order indices;
+ * at most `rec_cap` per pose in `recs` [n_poses * rec_cap], true count in
+ * `rec_count`). */
+int fa_eval_batch_debug(fa_ctx* ctx, int ligand, int n_poses, const double* genes, fa_cf_result* out,
+                        float* lig_xyz, fa_contact_rec* recs, int rec_cap, int* rec_count);
+
+/* Replaces statmech::StatMechEngine::compute (LIB/statmech.cpp:73-173) with
+ * the log-sum-exp of flexaids::log_sum_exp_dispatch
+ * (LIB/UnifiedHardwareDispatch.cpp:800-843), FP64 on the GPU.  `energy` has
+ * element i at energy[i * stride] (stride in doubles, so the `cf` column of an
+ * fa_cf_result array can be passed directly); `count` (may be NULL) holds the
+ * multiplicities.  Device pointers; `out` is a host pointer; synchronous. */
+int fa_thermo_device(fa_ctx* ctx, const double* energy, int stride, const int* count, int64_t n, double temperature,
+                     fa_thermo* out, void* stream);
+
+/* Same over host arrays (copies them in). */
+int fa_thermo_host(fa_ctx* ctx, const double* energy, const int* count, int64_t n, double temperature, fa_thermo* out);
+
+/* Counters of the last launch: poses that needed a per-pose cell grid, poses
+ * that hit a device capacity. */
+int fa_last_counters(fa_ctx* ctx, uint32_t* slow_grid, uint32_t* capacity);
+
+/* Size of the kernel's per-wave LDS workspace (bytes). */
+int fa_kernel_lds_bytes(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLEXAIDDS_AMD_H */
