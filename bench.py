@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Population-evaluation benchmark on one MI355X.
+
+A step is one pass of the hot path over one batch: one GA generation's
+population (2,000 chromosomes of the 45-atom / 10-torsion ligand against the
+4,000-atom receptor) decoded, rebuilt and scored with the Voronoi contact
+function, inputs already resident in HBM.  `value` is poses/s of the plain
+path (one ligand population per launch).  The batched form (many uneven
+ligands in one launch) is reported beside it.
+
+The roofline numbers come from the kernel's own operation counters (the
+algorithmic FLOP model, see include/fa_types.h fa_opcount) and HIP-event
+timing of each launch on the launch stream.  The CPU baseline is the oracle
+(C++ restatement, OpenMP) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Voronoi-CF poses/s on 1 MI355X, batched vs plain; p50/p99 dock latency"
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (also the unpacked FP32 rate)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(system, lig_index, genes, budget_s=12.0, threads=16):
+    """Oracle poses/s on the host cores over a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from flexaidds_amd import abi
+    lib = oracle_lib.load_oracle()
+    desc = abi.Descriptors(system.receptor, system.ligands, system.params)
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    done, t0 = 0, time.perf_counter()
+    chunk = min(len(genes), 2000)
+    k = 0
+    while time.perf_counter() - t0 < budget_s and done < 200000:
+        g = genes[(k * chunk) % len(genes):][:chunk]
+        oracle_lib.oracle_eval(lib, desc, lig_index, g, n_threads=threads)
+        done += len(g)
+        k += 1
+    dt = time.perf_counter() - t0
+    return {"value": round(done / dt, 1), "unit": "poses/s", "cores": threads, "kind": "port",
+            "sample": f"{done} poses of the same workload (receptor {system.receptor.n_atoms} atoms, "
+                      f"ligand {system.ligands[lig_index].n_atoms} atoms), oracle C++ FP64, "
+                      f"{threads} OpenMP threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--pop", type=int, default=2000)
+    ap.add_argument("--batched-ligands", type=int, default=32)
+    ap.add_argument("--batched-pop", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "poses are independent and would shard by population across GPUs, which is not run here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from flexaidds_amd import abi, hip, synth
+
+    torch.cuda.set_device(0)
+    log("building config-1 workload")
+    system = synth.make_config1(pop=args.pop)
+    lig = system.ligands[0]
+    G = lig.n_genes
+    rng = np.random.default_rng(2024)
+    n_pop = args.steps + args.warmup
+    pops = np.stack([synth.random_genes(rng, lig, args.pop) for _ in range(n_pop)])
+    ctx = hip.Context(max_pop=args.pop)
+    ctx.upload_system(system)
+    d_genes = torch.from_numpy(pops).cuda()
+    out_bytes = args.pop * abi.CF_DTYPE.itemsize
+    d_out = torch.zeros((n_pop, out_bytes), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+
+    def launch(k):
+        ctx.eval_device(args.pop, d_genes[k].data_ptr(), G, 0, d_out[k].data_ptr(), sh)
+
+    log("warmup")
+    for k in range(args.warmup):
+        launch(k)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t_begin = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_begin.record(stream)
+    for i in range(args.steps):
+        starts[i].record(stream)
+        launch(args.warmup + i)
+        ends[i].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_begin.elapsed_time(t_end)
+    launch_ms = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
+    log(f"timed {args.steps} steps: {total_ms:.2f} ms")
+
+    # algorithmic FLOPs of the timed launches (deterministic: replay untimed)
+    flop_total = 0
+    status = np.zeros(5, np.int64)
+    for i in range(args.steps):
+        launch(args.warmup + i)
+        flop_total += hip.flops(ctx.opcount())
+        res = np.frombuffer(d_out[args.warmup + i].cpu().numpy().tobytes(), abi.CF_DTYPE)
+        status += np.bincount(res["status"], minlength=5)
+    cap = ctx.counters()
+    poses = args.steps * args.pop
+    value = poses / (total_ms / 1e3)
+    achieved = flop_total / args.steps / (launch_ms.mean() / 1e3) / 1e12
+
+    # thermodynamics of the last population (FP64 GPU reduction, in place)
+    cf_off = abi.CF_DTYPE.fields["cf"][1]
+    th = ctx.thermo_device(d_out[args.warmup + args.steps - 1].data_ptr() + cf_off, abi.CF_DTYPE.itemsize // 8,
+                           args.pop, 300.0, stream=sh)
+
+    # batched form: many uneven ligands, all populations in one launch
+    batched = None
+    if args.batched_ligands > 0:
+        log(f"building batched workload: {args.batched_ligands} ligands x {args.batched_pop}")
+        bs = synth.make_config2(n_ligands=args.batched_ligands, pop=args.batched_pop)
+        bctx = hip.Context()
+        bctx.upload_system(bs)
+        GB = max(l.n_genes for l in bs.ligands)
+        P = sum(len(g) for g in bs.genes)
+        bg = np.zeros((P, GB))
+        bl = np.zeros(P, np.int32)
+        k = 0
+        for li, gl in enumerate(bs.genes):
+            bg[k:k + len(gl), :gl.shape[1]] = gl
+            bl[k:k + len(gl)] = li
+            k += len(gl)
+        perm = np.random.default_rng(5).permutation(P)
+        dbg = torch.from_numpy(bg[perm]).cuda()
+        dbl = torch.from_numpy(bl[perm]).cuda()
+        dbo = torch.zeros(P * abi.CF_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+        bctx.eval_device(P, dbg.data_ptr(), GB, dbl.data_ptr(), dbo.data_ptr(), sh)
+        torch.cuda.synchronize()
+        reps = 3
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            bctx.eval_device(P, dbg.data_ptr(), GB, dbl.data_ptr(), dbo.data_ptr(), sh)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        bms = e0.elapsed_time(e1) / reps
+        bflop = hip.flops(bctx.opcount())
+        batched = {"poses": P, "ligands": args.batched_ligands, "ms_per_launch": round(bms, 3),
+                   "poses_per_s": round(P / (bms / 1e3), 1),
+                   "achieved_tflops": round(bflop / (bms / 1e3) / 1e12, 4),
+                   "vs_plain": round(P / (bms / 1e3) / value, 3)}
+        bctx.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(system, 0, pops.reshape(-1, G))
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "poses/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain GA-generation evaluation, config-1 shape: 4,000-atom receptor shell, "
+                               "45-atom / 10-torsion ligand, population 2,000 per launch (rigid receptor)",
+                   "receptor_atoms": system.receptor.n_atoms, "ligand_atoms": lig.n_atoms, "genes": G,
+                   "population": args.pop},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 5), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 7), "traffic": None,
+                     "flop_per_launch": int(flop_total / args.steps),
+                     "launch_ms_mean": round(float(launch_ms.mean()), 4)},
+        "cpu_baseline": cpu,
+        "latency_ms": {"generation_p50": round(float(np.percentile(launch_ms, 50)), 4),
+                       "generation_p99": round(float(np.percentile(launch_ms, 99)), 4)},
+        "batched": batched,
+        "status_counts": {"ok": int(status[0]), "hull_fail": int(status[1]), "clash": int(status[2]),
+                          "capacity": int(status[3]), "grid_range": int(status[4])},
+        "capacity_overflows": cap["capacity"],
+        "free_energy_300K_last_population": round(th["free_energy"], 6),
+    }
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

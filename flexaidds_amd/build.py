@@ -14,11 +14,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libflexaidds_amd.so")
-SOURCES = ["cf_kernel.hip", "statmech.hip", "fa_context.hip"]
+SOURCES = ["cf_kernel.hip", "statmech.hip", "fa_context.hip", "hip_eval.hip"]
 ARCH = os.environ.get("FA_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}",
     "-ffp-contract=off",                      # IEEE products/sums, like the reference
     "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -69,8 +69,8 @@ struct Smem {
     unsigned char done[MAXL];
     double acc_com[MAXL], acc_wal[MAXL], acc_sas[MAXL], acc_tot[MAXL];
     unsigned char acc_rcl[MAXL];
-    // scalars
-    int NC, NV, nrec;
+    // operation counters of the current pose (fa_opcount order)
+    unsigned long long ops[FA_OPCOUNT_N];
 };
 
 // ------------------------------------------------------------ wave helpers
@@ -121,6 +121,14 @@ __device__ __forceinline__ void wave_argmin(double& v, int& i)
 }
 
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+enum OpIdx { OP_CULL = 0, OP_PLANE, OP_HULL, OP_ARC, OP_ORDER, OP_TRI, OP_SARC, OP_VC, OP_BUILD };
+
+// add a wave-uniform count to the pose's operation counter (lane 0 writes)
+__device__ __forceinline__ void count_op(unsigned long long* ops, int k, long long n)
+{
+    if (lane_id() == 0) ops[k] += (unsigned long long)n;
+}
 
 // ------------------------------------------------------------ geometry (FP64)
 // LIB/Vcontacts.cpp:1384-1413
@@ -498,7 +506,10 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
         res.status = FA_OK;
         int nrec_total = 0;
 
+        if (l < FA_OPCOUNT_N) s.ops[l] = 0ull;
+        wave_sync();
         build_pose(s, lig, B.genes + (size_t)pose * B.gene_stride);
+        count_op(s.ops, OP_BUILD, L);
         if (B.lig_xyz) {
             for (int i = l; i < L; i += WAVE) {
                 float* o = B.lig_xyz + ((size_t)pose * B.xyz_stride + i) * 3;
@@ -545,7 +556,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                 const float ax = s.lx[a], ay = s.ly[a], az = s.lz[a];
                 const float ra = lig.radius[a];
                 const float rado = ra + RW;
-                int found = 0;
+                int found = 0, tested = 0;
                 scan_boxes(s, R, G, pg, L, s.lbox[a], [&](bool v, int j, float x, float y, float z, float r) {
                     if (!v) return;
                     const bool lj = j >= Rn;
@@ -554,6 +565,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                         if (s.done[m]) return;  // an earlier atom already counted the pair
                         r = lig.radius[m];
                     }
+                    ++tested;
                     const float dx = ax - x, dy = ay - y, dz = az - z;
                     const double sq = (double)(dx * dx + dy * dy + dz * dz);
                     const double nd = (double)(rado + r + RW);
@@ -569,6 +581,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                     }
                 });
                 found = wave_sum_i(found);
+                count_op(s.ops, OP_CULL, wave_sum_i(tested));
                 if (l == 0 && found > 0) s.done[a] = 1;  // marked done once it has a contact
                 wave_sync();
             }
@@ -602,7 +615,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                 const float ra = lig.radius[a];
                 const float rado = ra + RW;
                 // -- contact list (get_contlist4, LIB/Vcontacts.cpp:1725-1839)
-                int NC = 0;
+                int NC = 0, tested = 0;
                 bool overflow = false;
                 scan_boxes(s, R, G, pg, L, s.lbox[a], [&](bool v, int j, float x, float y, float z, float r) {
                     bool in = false;
@@ -617,6 +630,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                             skip = s.done[m] && !((s.stub[2 * a + (m >> 6)] >> (m & 63)) & 1ull);
                         }
                         if (!skip) {
+                            ++tested;
                             const float dx = ax - x, dy = ay - y, dz = az - z;
                             const double sq = (double)(dx * dx + dy * dy + dz * dz);
                             const double nd = (double)(rado + r + RW);
@@ -645,12 +659,14 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                     }
                     NC += __popcll(m);
                 });
+                count_op(s.ops, OP_CULL, wave_sum_i(tested));
                 if (NC > MAXC) overflow = true;
                 if (overflow) {
                     res.status = FA_CAPACITY;
                     break;
                 }
                 if (NC == 0) continue;  // SAS = -1, atom not marked done
+                count_op(s.ops, OP_PLANE, NC);
                 // bounding planes (LIB/Vcontacts.cpp:223-241)
                 if (l < 4) {
                     const double bp[4][4] = {{0.707, 1.0, 0.0, -10.0}, {0.707, -1.0, 0.0, -10.0},
@@ -802,10 +818,12 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                         }
                     }
                     wave_argmin(bv, bi);
+                    count_op(s.ops, OP_HULL, NC - (A < NC) - (Bp < NC) - (oldC < NC));
                     double vtmin = bv;
                     int C = (bi == 0x7fffffff) ? -1 : bi;
                     double nx = sx + vtmin * V0, ny = sy + vtmin * V1, nz = sz + vtmin * V2;
                     if (nx * nx + ny * ny + nz * nz > rr) {
+                        count_op(s.ops, OP_HULL, 4 - (A >= NC) - (Bp >= NC) - (oldC >= NC));
                         for (int c = NC; c < NC + 4; ++c) {
                             if (c != A && c != Bp && c != oldC) {
                                 double vtdiv = s.A0[c] * V0 + s.A1[c] * V1 + s.A2[c] * V2;
@@ -884,11 +902,12 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                     int cnt = 0;
                     double q0[3] = {0, 0, 0}, q1[3] = {0, 0, 0};
                     bool k0 = false, k1 = false;
-                    int e0 = 0, e1 = 0;
+                    int e0 = 0, e1 = 0, solved = 0;
                     if (e < edgenum) {
                         const int st = s.es[e], en = s.ee[e];
                         const double rd = (double)rado;
                         if (rd < s.vd[st] || rd < s.vd[en]) {
+                            solved = 1;
                             double a0[4], a1[4];
                             e0 = s.ep0[e];
                             e1 = s.ep1[e];
@@ -908,6 +927,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                     }
                     const int off = wave_excl_scan(cnt);
                     const int tot = wave_sum_i(cnt);
+                    count_op(s.ops, OP_ARC, wave_sum_i(solved));
                     if (vn + tot >= MAXV) {
                         cap = true;
                     } else {
@@ -1085,9 +1105,11 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                     wave_sync();
                     // order points around each face by cosine
                     bool noline = false;
+                    int ncos = 0;
                     for (int X = l; X < NC; X += WAVE) {
                         const int sc = s.fcnt[X];
                         if (s.pflag[X] == 'X' || sc <= 3) continue;
+                        ncos += sc - 2;
                         short* pt = &s.fpt[s.foff[X]];
                         double* cs = &s.fcos[s.foff[X]];
                         const int v0 = pt[0];
@@ -1122,6 +1144,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                                 }
                     }
                     wave_sync();
+                    count_op(s.ops, OP_ORDER, wave_sum_i(ncos));
                     if (__ballot(noline) != 0ull) {
                         res.status = FA_CAPACITY;
                         break;
@@ -1185,6 +1208,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                         }
                     }
                     wave_sync();
+                    int ntri = 0, narc = 0;
                     for (int X = l; X < NC; X += WAVE) {
                         if (s.pflag[X] == 'X') continue;
                         const int NP = s.fcnt[X];
@@ -1198,6 +1222,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                             const double Pa[3] = {s.vx[pt[0]], s.vy[pt[0]], s.vz[pt[0]]};
                             const double Pb[3] = {s.vx[pt[1]], s.vy[pt[1]], s.vz[pt[1]]};
                             const double arc = spherical_arc(ctr, cd, Pa, s.vd[pt[0]], Pb, s.vd[pt[1]], rado);
+                            ++narc;
                             if (test_point(s, ctr, NC, X))
                                 ar = (2.0f * PI_REF) * ro * (ro - cd) - arc;
                             else
@@ -1216,6 +1241,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                                        (s.vd[wa] * s.vd[wb]);
                             };
                             double c0 = NP > 2 ? cN0(1) : 0.0;
+                            if (NP > 2) ntri += NP - 2;
                             for (int q = 1; q < NP - 1; ++q) {
                                 const double cq = cNN(q);
                                 const double c1 = cN0(q + 1);
@@ -1238,11 +1264,14 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                                     const double Pa[3] = {s.vx[wa], s.vy[wa], s.vz[wa]};
                                     const double Pb[3] = {s.vx[wb], s.vy[wb], s.vz[wb]};
                                     ar += spherical_arc(ctr, cd, Pa, s.vd[wa], Pb, s.vd[wb], rado);
+                                    ++narc;
                                 }
                             }
                         }
                         s.area[X] = ar;
                     }
+                    count_op(s.ops, OP_TRI, wave_sum_i(ntri));
+                    count_op(s.ops, OP_SARC, wave_sum_i(narc));
                     wave_sync();
                     if (epi > 0) {
                         // corrections onto the planes adjacent to engulfing faces, sequential
@@ -1274,6 +1303,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                                         const double Pa[3] = {s.vx[cu], s.vy[cu], s.vz[cu]};
                                         const double Pb[3] = {s.vx[nx], s.vy[nx], s.vz[nx]};
                                         s.area[comm] += spherical_arc(ctr, cd, Pa, s.vd[cu], Pb, s.vd[nx], rado);
+                                        s.ops[OP_SARC] += 1;
                                         if (NP == 2) break;
                                     }
                                 }
@@ -1292,6 +1322,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                                     const double c1[3] = {-a1[0] * a1[3], -a1[1] * a1[3], -a1[2] * a1[3]};
                                     double mx = spherical_arc(c0, fabs(a0[3]), b, ro, c, ro, rado);
                                     mx += spherical_arc(c1, fabs(a1[3]), b, ro, c, ro, rado);
+                                    s.ops[OP_SARC] += 2;
                                     s.area[eng[0]] = (2.0f * PI_REF) * ro * ro - 0.5 * mx;
                                     s.area[eng[1]] = (2.0f * PI_REF) * ro * ro - 0.5 * mx;
                                 }
@@ -1374,6 +1405,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                         seen += __popcll(m);
                     }
                     nrec_total += nvalid_total;
+                    count_op(s.ops, OP_VC, nvalid_total);
                     sas_sub = wave_sum_d(sas_sub);
                     com = wave_sum_d(com);
                     wal = wave_sum_d(wal);
@@ -1424,6 +1456,8 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
             }
         }
         if (res.status == FA_CAPACITY && l == 0) atomicAdd(&B.counters[1], 1u);
+        wave_sync();
+        if (B.ops && l < FA_OPCOUNT_N && s.ops[l]) atomicAdd(&B.ops[l], s.ops[l]);
         if (l == 0) {
             B.out[pose] = res;
             if (B.rec_count) B.rec_count[pose] = (res.status == FA_OK) ? nrec_total : 0;

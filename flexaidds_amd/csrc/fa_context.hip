@@ -24,6 +24,9 @@ int smem_bytes();
 size_t lse_scratch_bytes();
 hipError_t launch_thermo(const double* d_energy, int stride, const int* d_count, long long n, double T,
                          void* d_scratch, fa_thermo* d_out, hipStream_t stream);
+hipError_t launch_boltzmann(const double* d_energy, int stride, long long n, double T, void* d_scratch, double* d_w,
+                            hipStream_t stream);
+hipError_t launch_lse_values(const double* d_x, int stride, long long n, void* d_scratch, hipStream_t stream);
 }  // namespace fa
 
 namespace {
@@ -67,7 +70,7 @@ struct fa_ctx {
     int n_ligands = 0;
     bool have_system = false;
     // pose buffers
-    DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo;
+    DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops;
     std::vector<uint32_t> last_counters = std::vector<uint32_t>(4, 0);
 
     void free_system()
@@ -208,6 +211,7 @@ int fa_ctx_create(int device_id, int max_pop, fa_ctx** out)
     if ((rc = ensure(c->counters, 4 * sizeof(uint32_t))) != FA_SUCCESS ||
         (rc = ensure(c->scratch, fa::lse_scratch_bytes())) != FA_SUCCESS ||
         (rc = ensure(c->thermo, sizeof(fa_thermo))) != FA_SUCCESS ||
+        (rc = ensure(c->ops, sizeof(fa_opcount))) != FA_SUCCESS ||
         (rc = ensure(c->genes, sizeof(double) * (size_t)c->max_pop * 32)) != FA_SUCCESS ||
         (rc = ensure(c->out, sizeof(fa_cf_result) * (size_t)c->max_pop)) != FA_SUCCESS) {
         delete c;
@@ -223,7 +227,7 @@ int fa_ctx_destroy(fa_ctx* c)
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     c->free_system();
-    for (DevBuf* b : {&c->genes, &c->out, &c->counters, &c->recs, &c->reccnt, &c->xyz, &c->scratch, &c->thermo})
+    for (DevBuf* b : {&c->genes, &c->out, &c->counters, &c->recs, &c->reccnt, &c->xyz, &c->scratch, &c->thermo, &c->ops})
         if (b->p) (void)hipFree(b->p);
     delete c;
     return FA_SUCCESS;
@@ -377,7 +381,9 @@ static int launch(fa_ctx* c, int n_poses, const double* d_genes, int stride, con
     B.lig_xyz = d_xyz;
     B.xyz_stride = xyz_stride;
     B.counters = static_cast<unsigned int*>(c->counters.p);
+    B.ops = static_cast<unsigned long long*>(c->ops.p);
     FA_HIP(hipMemsetAsync(c->counters.p, 0, 4 * sizeof(uint32_t), stream));
+    FA_HIP(hipMemsetAsync(c->ops.p, 0, sizeof(fa_opcount), stream));
     // enough resident waves to cover the chip; each wave loops over poses
     const int lds = fa::smem_bytes();
     int per_cu = (160 * 1024) / (lds > 0 ? lds : 1);
@@ -501,11 +507,69 @@ int fa_thermo_host(fa_ctx* c, const double* energy, const int* count, int64_t n,
     return rc;
 }
 
+int fa_upload_emat(fa_ctx* c, const float* emat, int ntypes)
+{
+    if (!c || !emat) return fail(FA_ERR_INVALID, "null argument");
+    if (!c->have_system) return fail(FA_ERR_NOSYSTEM, "no system uploaded");
+    if (ntypes != c->prm.ntypes) return fail(FA_ERR_INVALID, "ntypes differs from the uploaded system");
+    FA_HIP(hipSetDevice(c->device));
+    FA_HIP(hipDeviceSynchronize());
+    FA_HIP(hipMemcpy(const_cast<float*>(c->prm.emat), emat, sizeof(float) * (size_t)ntypes * ntypes,
+                     hipMemcpyHostToDevice));
+    return FA_SUCCESS;
+}
+
+int fa_boltzmann_device(fa_ctx* c, const double* energy, int stride, int64_t n, double T, double* weights,
+                        double* log_Z, void* stream)
+{
+    if (!c || !energy || !weights || n < 1 || stride < 1) return fail(FA_ERR_INVALID, "bad boltzmann arguments");
+    if (!(T > 0.0)) return fail(FA_ERR_INVALID, "temperature must be > 0");
+    FA_HIP(hipSetDevice(c->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    FA_HIP(fa::launch_boltzmann(energy, stride, n, T, c->scratch.p, weights, s));
+    if (log_Z) {
+        double ms[2];
+        FA_HIP(hipMemcpyAsync(ms, c->scratch.p, sizeof(ms), hipMemcpyDeviceToHost, s));
+        FA_HIP(hipStreamSynchronize(s));
+        *log_Z = ms[0] + std::log(ms[1]);
+    }
+    return FA_SUCCESS;
+}
+
+int fa_log_sum_exp_device(fa_ctx* c, const double* values, int stride, int64_t n, double* out, void* stream)
+{
+    if (!c || !values || !out || stride < 1) return fail(FA_ERR_INVALID, "bad log_sum_exp arguments");
+    if (n < 1) {
+        *out = -HUGE_VAL;
+        return FA_SUCCESS;
+    }
+    FA_HIP(hipSetDevice(c->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    FA_HIP(fa::launch_lse_values(values, stride, n, c->scratch.p, s));
+    double ms[2];
+    FA_HIP(hipMemcpyAsync(ms, c->scratch.p, sizeof(ms), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    *out = std::isfinite(ms[0]) ? ms[0] + std::log(ms[1]) : ms[0];
+    return FA_SUCCESS;
+}
+
 int fa_last_counters(fa_ctx* c, uint32_t* slow_grid, uint32_t* capacity)
 {
     if (!c) return fail(FA_ERR_INVALID, "null context");
+    FA_HIP(hipSetDevice(c->device));
+    FA_HIP(hipDeviceSynchronize());
+    FA_HIP(hipMemcpy(c->last_counters.data(), c->counters.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (slow_grid) *slow_grid = c->last_counters[0];
     if (capacity) *capacity = c->last_counters[1];
+    return FA_SUCCESS;
+}
+
+int fa_last_opcount(fa_ctx* c, fa_opcount* out)
+{
+    if (!c || !out) return fail(FA_ERR_INVALID, "null argument");
+    FA_HIP(hipSetDevice(c->device));
+    FA_HIP(hipDeviceSynchronize());
+    FA_HIP(hipMemcpy(out, c->ops.p, sizeof(fa_opcount), hipMemcpyDeviceToHost));
     return FA_SUCCESS;
 }
 

@@ -80,6 +80,7 @@ struct DevBatch {
     float* lig_xyz;                 // optional: [n_poses * xyz_stride * 3]
     int xyz_stride;
     unsigned int* counters;         // [4]: slow-grid poses, capacity overflows, work queue, spare
+    unsigned long long* ops;        // optional: [FA_OPCOUNT_N] operation counters (fa_opcount order)
 };
 
 }  // namespace fa

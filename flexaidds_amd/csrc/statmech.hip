@@ -128,6 +128,28 @@ __global__ __launch_bounds__(256) void lse_final_kernel(const LseState* __restri
     }
 }
 
+// w_i = exp(-beta E_i - m) with m = max_i(-beta E_i): the unnormalised
+// weights of compute_boltzmann_batch (LIB/UnifiedHardwareDispatch.cpp:730-747)
+__global__ __launch_bounds__(256) void boltzmann_map_kernel(const double* __restrict__ energy, int stride, long long n,
+                                                             double beta, const LseState* __restrict__ total,
+                                                             double* __restrict__ w)
+{
+    const double m = total->m;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        w[i] = exp(-beta * energy[i * stride] - m);
+}
+
+// fold the block states into part[0] (single block)
+__global__ __launch_bounds__(256) void lse_fold_kernel(LseState* __restrict__ part, int nparts)
+{
+    __shared__ LseState sh[4];
+    LseState st{0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) lse_merge(st, part[i]);
+    block_reduce(st, sh);
+    __syncthreads();
+    if (threadIdx.x == 0) part[0] = st;
+}
+
 constexpr int LSE_MAX_PARTS = 1024;
 
 size_t lse_scratch_bytes() { return sizeof(LseState) * LSE_MAX_PARTS; }
@@ -143,6 +165,39 @@ hipError_t launch_thermo(const double* d_energy, int stride, const int* d_count,
     hipLaunchKernelGGL(lse_partial_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, d_energy, stride, d_count, n,
                        beta, part);
     hipLaunchKernelGGL(lse_final_kernel, dim3(1), dim3(256), 0, stream, part, (int)blocks, T, d_out);
+    return hipGetLastError();
+}
+
+// Boltzmann weights (unnormalised, max-shifted) and ln Z = m + ln sum w.
+hipError_t launch_boltzmann(const double* d_energy, int stride, long long n, double T, void* d_scratch, double* d_w,
+                            hipStream_t stream)
+{
+    const double beta = 1.0 / (KB_KCAL * T);
+    long long blocks = (n + 255) / 256;
+    if (blocks > LSE_MAX_PARTS) blocks = LSE_MAX_PARTS;
+    if (blocks < 1) blocks = 1;
+    LseState* part = static_cast<LseState*>(d_scratch);
+    hipLaunchKernelGGL(lse_partial_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, d_energy, stride,
+                       (const int*)nullptr, n, beta, part);
+    hipLaunchKernelGGL(lse_fold_kernel, dim3(1), dim3(256), 0, stream, part, (int)blocks);
+    long long mb = (n + 255) / 256;
+    if (mb > 4096) mb = 4096;
+    hipLaunchKernelGGL(boltzmann_map_kernel, dim3((unsigned)mb), dim3(256), 0, stream, d_energy, stride, n, beta,
+                       part, d_w);
+    return hipGetLastError();
+}
+
+// log-sum-exp of raw values x_i (x at stride): part[0] = (max, sum exp(x - max))
+hipError_t launch_lse_values(const double* d_x, int stride, long long n, void* d_scratch, hipStream_t stream)
+{
+    long long blocks = (n + 255) / 256;
+    if (blocks > LSE_MAX_PARTS) blocks = LSE_MAX_PARTS;
+    if (blocks < 1) blocks = 1;
+    LseState* part = static_cast<LseState*>(d_scratch);
+    // lw = log(1) - beta * e with beta = -1 gives lw = x
+    hipLaunchKernelGGL(lse_partial_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, d_x, stride,
+                       (const int*)nullptr, n, -1.0, part);
+    hipLaunchKernelGGL(lse_fold_kernel, dim3(1), dim3(256), 0, stream, part, (int)blocks);
     return hipGetLastError();
 }
 

@@ -22,6 +22,20 @@ _lib = C.CDLL(LIB_PATH)
 _P = C.c_void_p
 
 
+class OpCount(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("cull", "plane", "hull_test", "arc_solve", "face_order", "sph_triangle",
+                                          "sph_arc", "vc_contact", "build_atom")]
+
+
+#: FLOP per counted unit (the algorithmic FLOP model of the contact function)
+FLOP_WEIGHTS = {"cull": 11, "plane": 14, "hull_test": 13, "arc_solve": 82, "face_order": 18,
+                "sph_triangle": 85, "sph_arc": 60, "vc_contact": 25, "build_atom": 94}
+
+
+def flops(ops: dict) -> int:
+    return int(sum(FLOP_WEIGHTS[k] * int(v) for k, v in ops.items()))
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [("name", C.c_char * 256), ("compute_units", C.c_int32), ("global_mem", C.c_uint64),
                 ("gcn_arch", C.c_int32), ("wavefront_size", C.c_int32), ("max_threads_per_cu", C.c_int32),
@@ -40,6 +54,10 @@ _SIGS = {
     "fa_thermo_device": ([_P, _P, C.c_int, _P, C.c_int64, C.c_double, _P, _P], C.c_int),
     "fa_thermo_host": ([_P, _P, _P, C.c_int64, C.c_double, _P], C.c_int),
     "fa_last_counters": ([_P, _P, _P], C.c_int),
+    "fa_last_opcount": ([_P, _P], C.c_int),
+    "fa_upload_emat": ([_P, _P, C.c_int], C.c_int),
+    "fa_boltzmann_device": ([_P, _P, C.c_int, C.c_int64, C.c_double, _P, _P, _P], C.c_int),
+    "fa_log_sum_exp_device": ([_P, _P, C.c_int, C.c_int64, _P, _P], C.c_int),
     "fa_kernel_lds_bytes": ([], C.c_int),
 }
 
@@ -155,6 +173,11 @@ class Context:
                                      C.c_void_p(count_ptr) if count_ptr else None, n, temperature, C.byref(th),
                                      C.c_void_p(stream) if stream else None))
         return {f: getattr(th, f) for f, _ in abi.Thermo._fields_}
+
+    def opcount(self) -> dict:
+        o = OpCount()
+        _check(_lib.fa_last_opcount(self._ctx, C.byref(o)))
+        return {f: getattr(o, f) for f, _ in OpCount._fields_}
 
     def counters(self):
         a, b = C.c_uint32(), C.c_uint32()

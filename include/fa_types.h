@@ -111,6 +111,29 @@ typedef struct fa_contact_rec {
     double  dist;
 } fa_contact_rec;
 
+/* Operation counts of the contact-function algorithm, summed over poses.
+ * Each counter is one unit of the algorithmic FLOP model (weights in
+ * parentheses, FLOP per unit): cull = neighbour candidates distance-tested
+ * (11), plane = contact planes built (14), hull_test = plane intersections
+ * evaluated in the hull walk (13), arc_solve = edge/sphere solves (82),
+ * face_order = vertex cosines for face ordering (18), sph_triangle =
+ * spherical triangles (85), sph_arc = spherical arc segments (60),
+ * vc_contact = contact records scored (25), build_atom = atoms rebuilt from
+ * internal coordinates (94). */
+typedef struct fa_opcount {
+    uint64_t cull;
+    uint64_t plane;
+    uint64_t hull_test;
+    uint64_t arc_solve;
+    uint64_t face_order;
+    uint64_t sph_triangle;
+    uint64_t sph_arc;
+    uint64_t vc_contact;
+    uint64_t build_atom;
+} fa_opcount;
+
+#define FA_OPCOUNT_N 9
+
 /* Canonical-ensemble thermodynamics (statmech::Thermodynamics,
  * LIB/statmech.h:36-45). */
 typedef struct fa_thermo {

@@ -101,9 +101,29 @@ int fa_thermo_device(fa_ctx* ctx, const double* energy, int stride, const int* c
 /* Same over host arrays (copies them in). */
 int fa_thermo_host(fa_ctx* ctx, const double* energy, const int* count, int64_t n, double temperature, fa_thermo* out);
 
+/* Replaces hip_eval::upload_emat (LIB/hip_eval.h:92) for type-pair matrices
+ * with one value per pair (weight = 1): replaces the resident matrix. */
+int fa_upload_emat(fa_ctx* ctx, const float* emat, int ntypes);
+
+/* Replaces flexaids::compute_boltzmann_batch (LIB/UnifiedHardwareDispatch.cpp:
+ * 768-794, :730-747): weights[i] = exp(-beta E_i - max_j(-beta E_j)) on the GPU
+ * in FP64 and log_Z = ln sum exp(-beta E_i).  Device pointers (`log_Z` host,
+ * may be NULL); synchronous when `log_Z` is requested. */
+int fa_boltzmann_device(fa_ctx* ctx, const double* energy, int stride, int64_t n, double temperature,
+                        double* weights, double* log_Z, void* stream);
+
+/* Replaces flexaids::log_sum_exp_dispatch (LIB/UnifiedHardwareDispatch.cpp:
+ * 800-843): ln sum exp(x_i), FP64 on the GPU; n = 0 gives -inf.  Device
+ * pointer `values`, host `out`; synchronous. */
+int fa_log_sum_exp_device(fa_ctx* ctx, const double* values, int stride, int64_t n, double* out, void* stream);
+
 /* Counters of the last launch: poses that needed a per-pose cell grid, poses
- * that hit a device capacity. */
+ * that hit a device capacity.  Synchronises the device. */
 int fa_last_counters(fa_ctx* ctx, uint32_t* slow_grid, uint32_t* capacity);
+
+/* Operation counters (fa_opcount) summed over the last launch; they define
+ * the algorithmic FLOP count of that launch.  Synchronises the device. */
+int fa_last_opcount(fa_ctx* ctx, fa_opcount* out);
 
 /* Size of the kernel's per-wave LDS workspace (bytes). */
 int fa_kernel_lds_bytes(void);

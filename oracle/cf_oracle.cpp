@@ -53,6 +53,21 @@ constexpr double CLASH_THRESHOLD = 1e4;    // LIB/flexaid.h:70
 constexpr float CELLSIZE = 6.5f;           // LIB/Vcontacts.h:56
 constexpr int EDGE_LIMIT = 200;            // LIB/Vcontacts.cpp:378
 
+// Trig of the pose build: 0 = the host libm's cosf/sinf (what the reference
+// binary calls), 1 = the double-precision value rounded to float (correctly
+// rounded; what the GPU kernel computes).
+int g_trig_mode = 0;
+
+inline float trig_cos(float x) { return g_trig_mode ? (float)std::cos((double)x) : std::cos(x); }
+inline float trig_sin(float x) { return g_trig_mode ? (float)std::sin((double)x) : std::sin(x); }
+
+// per-thread operation counters (see fa_opcount)
+thread_local fa_opcount* t_ops = nullptr;
+#define OPS(field, n)                        \
+    do {                                     \
+        if (t_ops) t_ops->field += (n);      \
+    } while (0)
+
 template <class R> struct Plane {
     R Ai[4];
     R dist;
@@ -142,6 +157,7 @@ template <class R> R cosPQR(const R* P, const R* Q, const R* Rr)
 // the spherical triangle (A', B, C)
 template <class R> R spherical_arc(const Vert<R>& A, const Vert<R>& B, const Vert<R>& C, float rado)
 {
+    OPS(sph_arc, 1);
     R cAOB = (A.xi[0] * B.xi[0] + A.xi[1] * B.xi[1] + A.xi[2] * B.xi[2]) / (A.dist * B.dist);
     R cBOC = (B.xi[0] * C.xi[0] + B.xi[1] * C.xi[1] + B.xi[2] * C.xi[2]) / (B.dist * C.dist);
     R U = (1 + cAOB) * std::sqrt((1 + cBOC) / R(8));
@@ -282,6 +298,7 @@ void decode_and_build(const fa_ligand_desc* lig, const double* icv_in, float* xy
         default: break;
         }
     }
+    OPS(build_atom, n);
     for (int an = 0; an < n; ++an) {
         const int a = lig->mov[an];
         float x[4], y[4], z[4];
@@ -323,15 +340,15 @@ void decode_and_build(const fa_ligand_desc* lig, const double* icv_in, float* xy
         B = cy * cy;
         C = cz * cz;
         float angPI = (float)(ang[a] * PI_REF / 180.0f);
-        float ct = std::cos(angPI);
-        float st = -std::sin(angPI);
+        float ct = trig_cos(angPI);
+        float st = -trig_sin(angPI);
         op = 1.0f - ct;
         float xk = (cx * cz * op - cy * st) * zn + ((1.0f - A) * ct + A) * xn + (cx * cy * op + cz * st) * yn;
         float yk = (cy * cx * op - cz * st) * xn + ((1.0f - B) * ct + B) * yn + (cy * cz * op + cx * st) * zn;
         float zk = (cz * cy * op - cx * st) * yn + ((1.0f - C) * ct + C) * zn + (cz * cx * op + cy * st) * xn;
         float dihPI = (float)(dih[a] * PI_REF / 180.0f);
-        ct = std::cos(dihPI);
-        st = std::sin(dihPI);
+        ct = trig_cos(dihPI);
+        st = trig_sin(dihPI);
         op = 1.0f - ct;
         cx = (x[2] - x[1]) * d;
         cy = (y[2] - y[1]) * d;
@@ -425,6 +442,7 @@ int get_contacts(const System& S, Work<R>& W, int a, float rado, double* clash)
         for (int k = 0; k < W.boxcount[bi]; ++k) {
             const int j = W.calclist[W.boxfirst[bi] + k];
             if (W.done[j] == 'Y') continue;  // every atom is exposed here
+            OPS(cull, 1);
             const double rAB = (double)(W.rad[a] + W.rad[j]);
             const float dx = W.x[a] - W.x[j], dy = W.y[a] - W.y[j], dz = W.z[a] - W.z[j];
             const double sqrdist = (double)(dx * dx + dy * dy + dz * dz);
@@ -521,6 +539,7 @@ template <class R> int voronoi_poly(const System& S, Work<R>& W, int a, float ra
     for (int c = 0; c < NC; ++c) {
         const int j = W.contlist[c].index;
         const R atomdist = W.contlist[c].dist;
+        OPS(plane, 1);
         if (atomdist < (R)1.0e-10) continue;  // degenerate, never produced by the cull
         // float products as in LIB/Vcontacts.cpp:202 (float operands)
         const float rj = RW + W.rad[j];
@@ -569,6 +588,7 @@ template <class R> int voronoi_poly(const System& S, Work<R>& W, int a, float ra
         int C = -1;
         for (int c = 0; c < NC; ++c) {
             if (c != A && c != B && c != oldC) {
+                OPS(hull_test, 1);
                 R vtdiv = cont[c].Ai[0] * V[0] + cont[c].Ai[1] * V[1] + cont[c].Ai[2] * V[2];
                 if (vtdiv != R(0)) {
                     R vt = -(cont[c].Ai[0] * vs.xi[0] + cont[c].Ai[1] * vs.xi[1] + cont[c].Ai[2] * vs.xi[2] + cont[c].Ai[3]) / vtdiv;
@@ -583,6 +603,7 @@ template <class R> int voronoi_poly(const System& S, Work<R>& W, int a, float ra
         if (np[0] * np[0] + np[1] * np[1] + np[2] * np[2] > rr) {
             for (int c = NC; c < NC + 4; ++c) {
                 if (c != A && c != B && c != oldC) {
+                    OPS(hull_test, 1);
                     R vtdiv = cont[c].Ai[0] * V[0] + cont[c].Ai[1] * V[1] + cont[c].Ai[2] * V[2];
                     if (vtdiv != R(0)) {
                         R vt = -(cont[c].Ai[0] * vs.xi[0] + cont[c].Ai[1] * vs.xi[1] + cont[c].Ai[2] * vs.xi[2] + cont[c].Ai[3]) / vtdiv;
@@ -638,6 +659,7 @@ template <class R> int voronoi_poly(const System& S, Work<R>& W, int a, float ra
         vedge[e].arc = ((R)rado < poly[vedge[e].startpt].dist || (R)rado < poly[vedge[e].endpt].dist) ? '?' : 'X';
     for (int e = 0; e < edgenum; ++e) {
         if (vedge[e].arc == 'X') continue;
+        OPS(arc_solve, 1);
         R p0[3], p1[3];
         if (solve_2xS(cont[vedge[e].plane[0]].Ai, cont[vedge[e].plane[1]].Ai, rado, p0, p1) == -1) {
             vedge[e].arc = 'X';
@@ -727,6 +749,7 @@ template <class R> int order_faces(Work<R>& W, float rado, int NC, int NV)
             if (vi == sc) return -3;  // no second point on the common line (reference reads past the list)
             std::swap(pt[vi], pt[1]);
             std::vector<R> cs(sc);
+            OPS(face_order, sc - 2);
             for (vi = 2; vi < sc; ++vi) cs[vi] = cosPQR(poly[pt[1]].xi, poly[pt[0]].xi, poly[pt[vi]].xi);
             for (vi = 2; vi < sc - 1; ++vi)
                 for (int v2 = vi + 1; v2 < sc; ++v2)
@@ -812,6 +835,7 @@ template <class R> void calc_areas(Work<R>& W, float rado, int NC, int NV, int a
                 cNN1[vi] = (va.xi[0] * vb.xi[0] + va.xi[1] * vb.xi[1] + va.xi[2] * vb.xi[2]) / (va.dist * vb.dist);
                 if (vi != 0) cN0[vi] = (v0.xi[0] * va.xi[0] + v0.xi[1] * va.xi[1] + v0.xi[2] * va.xi[2]) / (v0.dist * va.dist);
             }
+            if (NP > 2) OPS(sph_triangle, NP - 2);
             for (int vi = 1; vi < NP - 1; ++vi) {
                 R U = std::sqrt((1 + cN0[vi]) * (1 + cNN1[vi]) * (1 + cN0[vi + 1]) / R(8));
                 R V = std::sqrt((1 - cN0[vi]) * (1 - cNN1[vi]) * (1 + cN0[vi + 1]) / R(8));
@@ -972,6 +996,7 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
         const double radA = (double)W.rad[i];
         const double radoA = radA + RW;
         double SAS = 4.0 * PI_REF * radoA * radoA;
+        OPS(vc_contact, W.records[i - Rn].size());
         for (const fa_contact_rec& r : W.records[i - Rn]) {
             const int c = r.atom;
             const double rAB = radA + (double)W.rad[c];
@@ -1041,10 +1066,11 @@ template <class R> void init_work(const System& S, Work<R>& W)
 
 template <class R>
 int eval_batch(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm, int n_poses,
-               const double* genes, fa_cf_result* out, int n_threads)
+               const double* genes, fa_cf_result* out, int n_threads, fa_opcount* ops)
 {
     if (!rec || !lig || !prm || !genes || !out || n_poses < 0) return -1;
     System S{rec, lig, prm};
+    if (ops) std::memset(ops, 0, sizeof(*ops));
 #ifdef _OPENMP
     if (n_threads <= 0) n_threads = omp_get_max_threads();
 #pragma omp parallel num_threads(n_threads)
@@ -1052,11 +1078,24 @@ int eval_batch(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_
     {
         Work<R> W;
         init_work(S, W);
+        fa_opcount local{};
+        t_ops = ops ? &local : nullptr;
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 4)
 #endif
         for (int p = 0; p < n_poses; ++p)
             out[p] = eval_pose(S, W, genes + (size_t)p * lig->n_genes, nullptr, nullptr);
+        t_ops = nullptr;
+        if (ops) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+            {
+                uint64_t* d = &ops->cull;
+                const uint64_t* s0 = &local.cull;
+                for (int k = 0; k < FA_OPCOUNT_N; ++k) d[k] += s0[k];
+            }
+        }
     }
     (void)n_threads;
     return 0;
@@ -1072,14 +1111,25 @@ extern "C" {
 int fa_oracle_eval(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm, int n_poses,
                    const double* genes, fa_cf_result* out, int n_threads)
 {
-    return eval_batch<double>(rec, lig, prm, n_poses, genes, out, n_threads);
+    return eval_batch<double>(rec, lig, prm, n_poses, genes, out, n_threads, nullptr);
+}
+
+/* As fa_oracle_eval, also summing the operation counters over the batch. */
+int fa_oracle_eval_counted(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm,
+                           int n_poses, const double* genes, fa_cf_result* out, int n_threads, fa_opcount* ops)
+{
+    return eval_batch<double>(rec, lig, prm, n_poses, genes, out, n_threads, ops);
 }
 
 int fa_oracle_eval_f32(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm, int n_poses,
                        const double* genes, fa_cf_result* out, int n_threads)
 {
-    return eval_batch<float>(rec, lig, prm, n_poses, genes, out, n_threads);
+    return eval_batch<float>(rec, lig, prm, n_poses, genes, out, n_threads, nullptr);
 }
+
+/* 0: pose-build trig from the host libm (the reference binary's bits);
+ * 1: correctly rounded float cos/sin (the GPU kernel's bits). */
+void fa_oracle_set_trig(int mode) { g_trig_mode = mode ? 1 : 0; }
 
 /* One pose with its Cartesian ligand coordinates and contact records.
  * Returns the number of records (capped at `cap`) or -1. */

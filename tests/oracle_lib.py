@@ -33,6 +33,10 @@ def load_oracle():
         f = getattr(lib, name)
         f.argtypes = [_P, _P, _P, C.c_int, _P, _P, C.c_int]
         f.restype = C.c_int
+    lib.fa_oracle_eval_counted.argtypes = [_P, _P, _P, C.c_int, _P, _P, C.c_int, _P]
+    lib.fa_oracle_eval_counted.restype = C.c_int
+    lib.fa_oracle_set_trig.argtypes = [C.c_int]
+    lib.fa_oracle_set_trig.restype = None
     lib.fa_oracle_eval_pose.argtypes = [_P, _P, _P, _P, _P, _P, _P, C.c_int]
     lib.fa_oracle_eval_pose.restype = C.c_int
     lib.fa_oracle_build_pose.argtypes = [_P, _P, _P]
@@ -78,6 +82,35 @@ def oracle_eval(lib, desc, lig_index, genes, f32=False, n_threads=0):
             genes.ctypes.data, out.ctypes.data, n_threads)
     assert rc == 0
     return out
+
+
+OPCOUNT_FIELDS = ("cull", "plane", "hull_test", "arc_solve", "face_order", "sph_triangle", "sph_arc", "vc_contact",
+                  "build_atom")
+
+
+def oracle_eval_counted(lib, desc, lig_index, genes, n_threads=0):
+    """Evaluate and return (results, operation counters summed over poses)."""
+    genes = np.ascontiguousarray(genes, np.float64)
+    out = np.zeros(genes.shape[0], abi.CF_DTYPE)
+    ops = np.zeros(len(OPCOUNT_FIELDS), np.uint64)
+    rc = lib.fa_oracle_eval_counted(_addr(desc.receptor), _addr(desc.ligands[lig_index]), _addr(desc.params),
+                                    genes.shape[0], genes.ctypes.data, out.ctypes.data, n_threads, ops.ctypes.data)
+    assert rc == 0
+    return out, dict(zip(OPCOUNT_FIELDS, (int(v) for v in ops)))
+
+
+class trig_mode:
+    """Context manager selecting the oracle's pose-build trig (0 host libm, 1 correctly rounded)."""
+
+    def __init__(self, lib, mode):
+        self.lib, self.mode = lib, mode
+
+    def __enter__(self):
+        self.lib.fa_oracle_set_trig(self.mode)
+        return self.lib
+
+    def __exit__(self, *exc):
+        self.lib.fa_oracle_set_trig(0)
 
 
 def oracle_pose(lib, desc, lig_index, genes1, cap=20000):

@@ -1,0 +1,265 @@
+"""HIP library against the CPU oracle (GPU tests).
+
+Tolerances (per pose, scored poses): |CF_gpu - CF_oracle| <= max(1e-4 |CF|,
+1e-3 kcal/mol); F at 300 K within 1e-6 relative on identical energies;
+contact-pair sets identical.  The pose build's cos/sin are correctly rounded
+on the GPU; the oracle's trig mode 1 computes the same bits, so against it
+coordinates are bit-exact and contact sets identical.  Against the host-libm
+oracle (mode 0, the reference binary's own bits) a pose may only differ where
+the oracle itself differs between the two trig modes (a documented
+discontinuity of the contact function, not a kernel error).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from flexaidds_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def tol(cf):
+    return np.maximum(1e-4 * np.abs(cf), 1e-3)
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from flexaidds_amd import hip as h
+    return h
+
+
+@pytest.fixture(scope="module")
+def sys0():
+    return synth.make_config0(n_poses=1000)
+
+
+@pytest.fixture(scope="module")
+def ctx0(hip, sys0):
+    c = hip.Context()
+    c.upload_system(sys0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def desc0(sys0):
+    return abi.Descriptors(sys0.receptor, sys0.ligands, sys0.params)
+
+
+@pytest.fixture(scope="module")
+def oracle_cr(oracle, desc0, sys0):
+    with ol.trig_mode(oracle, 1):
+        out, ops = ol.oracle_eval_counted(oracle, desc0, 0, sys0.genes[0])
+    return out, ops
+
+
+@pytest.fixture(scope="module")
+def oracle_libm(oracle, desc0, sys0):
+    return ol.oracle_eval(oracle, desc0, 0, sys0.genes[0])
+
+
+def test_device_is_gfx950(hip):
+    info = hip.device_info(0)
+    print("device", info, flush=True)
+    assert info["gcn_arch"] == 950
+    assert info["wavefront_size"] == 64
+
+
+def test_config0_against_oracle(ctx0, sys0, oracle_cr, oracle_libm):
+    g = ctx0.eval(0, sys0.genes[0])
+    ref, _ = oracle_cr
+    assert np.array_equal(g["status"], ref["status"])
+    assert ctx0.counters()["capacity"] == 0
+    ok = ref["status"] == 0
+    assert ok.sum() > 500
+    d = np.abs(g["cf"] - ref["cf"])
+    assert np.all(d[ok] <= tol(ref["cf"][ok])), np.nonzero(d[ok] > tol(ref["cf"][ok]))
+    assert d[ok].max() < 1e-9                     # same arithmetic: rounding-level only
+    for f in ("com", "wal", "sas", "totsas"):
+        assert np.abs(g[f] - ref[f])[ok].max() < 1e-8, f
+    assert np.array_equal(g["rclash"], ref["rclash"])
+    cl = ref["status"] == 2
+    assert np.all(np.abs(g["cf"] - ref["cf"])[cl] <= 1e-9 * np.abs(ref["cf"][cl]))
+    # against the host-libm oracle: only trig-ulp discontinuities may differ
+    lm = oracle_libm
+    okl = ok & (lm["status"] == 0)
+    bad = okl & (np.abs(g["cf"] - lm["cf"]) > tol(lm["cf"]))
+    trig_ties = okl & (np.abs(ref["cf"] - lm["cf"]) > tol(lm["cf"]))
+    assert np.all(bad <= trig_ties), np.nonzero(bad & ~trig_ties)
+    assert bad.sum() <= max(2, okl.sum() // 200)
+
+
+def test_pose_build_bit_exact(ctx0, sys0, oracle, desc0):
+    g = sys0.genes[0][:200]
+    _, xyz, _ = ctx0.eval_debug(0, g)
+    with ol.trig_mode(oracle, 1):
+        for p in range(len(g)):
+            ref = np.zeros((sys0.ligands[0].n_atoms, 3), np.float32)
+            gp = np.ascontiguousarray(g[p])
+            oracle.fa_oracle_build_pose(ol._addr(desc0.ligands[0]), gp.ctypes.data, ref.ctypes.data)
+            assert np.array_equal(ref, xyz[p]), p
+
+
+def test_contact_records(ctx0, sys0, oracle, desc0):
+    g = sys0.genes[0][:120]
+    out, _, recs = ctx0.eval_debug(0, g)
+    n_checked = 0
+    with ol.trig_mode(oracle, 1):
+        for p in range(len(g)):
+            o, _, orec = ol.oracle_pose(oracle, desc0, 0, g[p])
+            assert out[p]["status"] == o["status"]
+            if o["status"] != 0:
+                continue
+            a = {(int(r["from"]), int(r["atom"])): r for r in recs[p]}
+            b = {(int(r["from"]), int(r["atom"])): r for r in orec}
+            assert a.keys() == b.keys(), p
+            for k in a:
+                assert abs(a[k]["area"] - b[k]["area"]) <= 1e-9 * max(1.0, abs(b[k]["area"])), (p, k)
+                assert a[k]["dist"] == b[k]["dist"]
+            # record order per atom: reverse contact order, as ca_rec lists
+            assert [tuple(x) for x in recs[p][["from", "atom"]]] == [tuple(x) for x in orec[["from", "atom"]]]
+            n_checked += 1
+    assert n_checked > 50
+
+
+def test_opcount_matches_oracle(ctx0, sys0, oracle_cr):
+    ctx0.eval(0, sys0.genes[0])
+    assert ctx0.opcount() == oracle_cr[1]
+
+
+def test_repeatable(ctx0, sys0):
+    a = ctx0.eval(0, sys0.genes[0][:300])
+    b = ctx0.eval(0, sys0.genes[0][:300])
+    assert a.tobytes() == b.tobytes()
+
+
+def test_gene_clamping(ctx0, sys0, oracle, desc0):
+    g = sys0.genes[0][:40].copy()
+    g[:, 0] = np.where(np.arange(40) % 2, 5000.0, -3.0)   # grid index beyond both limits
+    g[:, 1:] += np.where(np.arange(40)[:, None] % 3 == 0, 400.0, 0.0)
+    out = ctx0.eval(0, g)
+    with ol.trig_mode(oracle, 1):
+        ref = ol.oracle_eval(oracle, desc0, 0, g)
+    assert np.array_equal(out["status"], ref["status"])
+    ok = ref["status"] == 0
+    assert np.all(np.abs(out["cf"] - ref["cf"])[ok] <= tol(ref["cf"][ok]))
+
+
+def test_empty_and_invalid(ctx0, sys0, hip):
+    out = ctx0.eval(0, np.zeros((0, sys0.ligands[0].n_genes)))
+    assert out.shape == (0,)
+    g = np.ascontiguousarray(sys0.genes[0][:2])
+    res = np.zeros(2, abi.CF_DTYPE)
+    assert hip._lib.fa_eval_batch(ctx0._ctx, 5, 2, g.ctypes.data, res.ctypes.data) != 0   # no such ligand
+    assert hip._lib.fa_eval_batch(ctx0._ctx, 0, -1, g.ctypes.data, res.ctypes.data) != 0  # negative size
+    bad = synth.make_config0(n_poses=4)
+    bad.ligands[0].mov = bad.ligands[0].mov[::-1].copy()  # rebuild order violates dependencies
+    c = hip.Context()
+    with pytest.raises(hip.FAError):
+        c.upload_system(bad)
+    assert hip._lib.fa_eval_batch(c._ctx, 0, 2, g.ctypes.data, res.ctypes.data) != 0      # nothing uploaded
+    c.close()
+
+
+def test_slow_grid_path(hip, oracle):
+    """Poses that place ligand atoms outside the receptor's bounding box widen
+    the cell grid per pose (LIB/Vcontacts.cpp:1626-1641)."""
+    s = synth.make_config0(n_poses=64)
+    lig = s.ligands[0]
+    center = s.receptor.center
+    far = [center + np.array([dx, 0.0, 0.0]) for dx in np.linspace(14.0, 24.0, 16)]
+    ic = np.array([synth.point_ic(p, lig.ori) for p in far], np.float32)
+    lig.grid_ic = np.vstack([lig.grid_ic, ic])
+    lig.gene_max = lig.gene_max.copy()
+    lig.gene_max[0] = lig.n_grid - 1
+    g = s.genes[0].copy()
+    g[:, 0] = lig.n_grid - 16 + (np.arange(64) % 16)
+    d = abi.Descriptors(s.receptor, s.ligands, s.params)
+    c = hip.Context()
+    c.upload_system(s)
+    out = c.eval(0, g)
+    assert c.counters()["slow_grid"] > 0
+    with ol.trig_mode(oracle, 1):
+        ref = ol.oracle_eval(oracle, d, 0, g)
+    c.close()
+    assert np.array_equal(out["status"], ref["status"])
+    ok = ref["status"] == 0
+    assert ok.sum() > 5
+    assert np.all(np.abs(out["cf"] - ref["cf"])[ok] <= tol(ref["cf"][ok]))
+
+
+def test_multi_ligand_launch(hip, oracle):
+    """Uneven ligands in one launch: each pose equals its single-ligand result."""
+    import torch
+    s = synth.make_config2(n_ligands=4, pop=48)
+    c = hip.Context()
+    c.upload_system(s)
+    G = max(l.n_genes for l in s.ligands)
+    P = sum(len(g) for g in s.genes)
+    genes = np.zeros((P, G))
+    lig_of = np.zeros(P, np.int32)
+    k = 0
+    for li, gl in enumerate(s.genes):
+        genes[k:k + len(gl), :gl.shape[1]] = gl
+        lig_of[k:k + len(gl)] = li
+        k += len(gl)
+    perm = np.random.default_rng(7).permutation(P)   # interleave ligands
+    genes, lig_of = genes[perm], lig_of[perm]
+    dg = torch.from_numpy(genes).cuda()
+    dl = torch.from_numpy(lig_of).cuda()
+    dout = torch.zeros(P * abi.CF_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    c.eval_device(P, dg.data_ptr(), G, dl.data_ptr(), dout.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    mixed = np.frombuffer(dout.cpu().numpy().tobytes(), abi.CF_DTYPE)
+    d = abi.Descriptors(s.receptor, s.ligands, s.params)
+    for li, gl in enumerate(s.genes):
+        sel = np.nonzero(lig_of == li)[0]
+        alone = c.eval(li, genes[sel, :gl.shape[1]])
+        assert alone.tobytes() == mixed[sel].tobytes(), li
+        with ol.trig_mode(oracle, 1):
+            ref = ol.oracle_eval(oracle, d, li, genes[sel, :gl.shape[1]])
+        assert np.array_equal(alone["status"], ref["status"])
+        ok = ref["status"] == 0
+        assert np.all(np.abs(alone["cf"] - ref["cf"])[ok] <= tol(ref["cf"][ok]))
+    c.close()
+
+
+def test_thermo_against_oracle(ctx0, oracle_cr, oracle):
+    ref, _ = oracle_cr
+    e = np.ascontiguousarray(ref["cf"][ref["status"] == 0])
+    th = ctx0.thermo(e, 300.0)
+    o = abi.Thermo()
+    oracle.fa_oracle_thermo(e.ctypes.data, None, len(e), 300.0, C.byref(o))
+    for f in ("log_Z", "free_energy", "mean_energy", "entropy"):
+        assert abs(th[f] - getattr(o, f)) <= 1e-6 * abs(getattr(o, f)) + 1e-12, f
+    # closed forms: equal energies give S = k ln N, multiplicity adds k ln n
+    kb = 0.001987206
+    th = ctx0.thermo(np.full(1000, -7.77), 300.0)
+    assert abs(th["entropy"] - kb * math.log(1000)) < 1e-9
+    th = ctx0.thermo(np.array([-5.0]), 300.0, count=np.array([10000]))
+    assert abs(th["free_energy"] - (-5.0 - kb * 300.0 * math.log(10000))) < 1e-9
+    th = ctx0.thermo(np.array([-10.5, -9.8, -10.2, -11.0, -9.5]), 300.0)
+    o2 = abi.Thermo()
+    ee = np.array([-10.5, -9.8, -10.2, -11.0, -9.5])
+    oracle.fa_oracle_thermo(ee.ctypes.data, None, 5, 300.0, C.byref(o2))
+    assert abs(th["free_energy"] - o2.free_energy) < 1e-9
+
+
+def test_thermo_over_results_on_device(ctx0, sys0, hip, oracle_cr, oracle):
+    """The reduction reads the cf column of the result array in place."""
+    import torch
+    g = torch.from_numpy(np.ascontiguousarray(sys0.genes[0])).cuda()
+    P = g.shape[0]
+    out = torch.zeros(P * abi.CF_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    ctx0.eval_device(P, g.data_ptr(), g.shape[1], 0, out.data_ptr(), s)
+    cf_off = abi.CF_DTYPE.fields["cf"][1]
+    th = ctx0.thermo_device(out.data_ptr() + cf_off, abi.CF_DTYPE.itemsize // 8, P, 300.0, stream=s)
+    ref, _ = oracle_cr
+    e = np.ascontiguousarray(ref["cf"])
+    o = abi.Thermo()
+    oracle.fa_oracle_thermo(e.ctypes.data, None, len(e), 300.0, C.byref(o))
+    assert abs(th["free_energy"] - o.free_energy) <= 1e-6 * abs(o.free_energy)
