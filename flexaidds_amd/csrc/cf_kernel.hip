@@ -565,7 +565,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                         if (s.done[m]) return;  // an earlier atom already counted the pair
                         r = lig.radius[m];
                     }
-                    ++tested;
+                    tested += (j != Rn + a);  // the atom itself never makes a contact
                     const float dx = ax - x, dy = ay - y, dz = az - z;
                     const double sq = (double)(dx * dx + dy * dy + dz * dz);
                     const double nd = (double)(rado + r + RW);
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                             skip = s.done[m] && !((s.stub[2 * a + (m >> 6)] >> (m & 63)) & 1ull);
                         }
                         if (!skip) {
-                            ++tested;
+                            tested += (j != acalc);
                             const float dx = ax - x, dy = ay - y, dz = az - z;
                             const double sq = (double)(dx * dx + dy * dy + dz * dz);
                             const double nd = (double)(rado + r + RW);

@@ -18,6 +18,13 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libflexaidd
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is not built: run `python -m flexaidds_amd.build` (or __graft_entry__.build())")
 
+# One HIP runtime per process: when PyTorch is present its bundled runtime is
+# loaded first so that torch tensors and this library share it.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
 _lib = C.CDLL(LIB_PATH)
 _P = C.c_void_p
 

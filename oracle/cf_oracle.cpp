@@ -442,7 +442,7 @@ int get_contacts(const System& S, Work<R>& W, int a, float rado, double* clash)
         for (int k = 0; k < W.boxcount[bi]; ++k) {
             const int j = W.calclist[W.boxfirst[bi] + k];
             if (W.done[j] == 'Y') continue;  // every atom is exposed here
-            OPS(cull, 1);
+            if (j != a) OPS(cull, 1);        // the atom itself never makes a contact
             const double rAB = (double)(W.rad[a] + W.rad[j]);
             const float dx = W.x[a] - W.x[j], dy = W.y[a] - W.y[j], dz = W.z[a] - W.z[j];
             const double sqrdist = (double)(dx * dx + dy * dy + dz * dz);

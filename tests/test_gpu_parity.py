@@ -119,15 +119,19 @@ def test_contact_records(ctx0, sys0, oracle, desc0):
             for k in a:
                 assert abs(a[k]["area"] - b[k]["area"]) <= 1e-9 * max(1.0, abs(b[k]["area"])), (p, k)
                 assert a[k]["dist"] == b[k]["dist"]
-            # record order per atom: reverse contact order, as ca_rec lists
-            assert [tuple(x) for x in recs[p][["from", "atom"]]] == [tuple(x) for x in orec[["from", "atom"]]]
+            # record order per atom: reverse contact order, as the ca_rec lists
+            for frm in np.unique(orec["from"]):
+                ga = [int(x) for x in recs[p]["atom"][recs[p]["from"] == frm]]
+                oa = [int(x) for x in orec["atom"][orec["from"] == frm]]
+                assert ga == oa, (p, int(frm))
             n_checked += 1
     assert n_checked > 50
 
 
 def test_opcount_matches_oracle(ctx0, sys0, oracle_cr):
     ctx0.eval(0, sys0.genes[0])
-    assert ctx0.opcount() == oracle_cr[1]
+    got, want = ctx0.opcount(), oracle_cr[1]
+    assert got == want, {k: (got[k], want[k]) for k in want if got[k] != want[k]}
 
 
 def test_repeatable(ctx0, sys0):
@@ -170,7 +174,7 @@ def test_slow_grid_path(hip, oracle):
     s = synth.make_config0(n_poses=64)
     lig = s.ligands[0]
     center = s.receptor.center
-    far = [center + np.array([dx, 0.0, 0.0]) for dx in np.linspace(14.0, 24.0, 16)]
+    far = [center + np.array([dx, 0.0, 0.0]) for dx in np.linspace(26.0, 34.0, 16)]
     ic = np.array([synth.point_ic(p, lig.ori) for p in far], np.float32)
     lig.grid_ic = np.vstack([lig.grid_ic, ic])
     lig.gene_max = lig.gene_max.copy()
