@@ -124,6 +124,23 @@ __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 enum OpIdx { OP_CULL = 0, OP_PLANE, OP_HULL, OP_ARC, OP_ORDER, OP_TRI, OP_SARC, OP_VC, OP_BUILD };
 
+// optional per-phase cycle accounting (debug builds of a launch only)
+enum Phase { PH_BUILD = 0, PH_PREPASS, PH_CULL, PH_FIRST, PH_HULL, PH_ARC, PH_SEED, PH_FACE, PH_AREA, PH_REC, PH_N };
+
+struct PhaseClock {
+    unsigned long long* acc;  // null when disabled
+    unsigned long long t;
+    __device__ void start() { if (acc) t = __builtin_readcyclecounter(); }
+    __device__ void stop(int ph)
+    {
+        if (acc) {
+            const unsigned long long now = __builtin_readcyclecounter();
+            if (lane_id() == 0) atomicAdd(&acc[ph], now - t);
+            t = now;
+        }
+    }
+};
+
 // add a wave-uniform count to the pose's operation counter (lane 0 writes)
 __device__ __forceinline__ void count_op(unsigned long long* ops, int k, long long n)
 {
@@ -508,6 +525,8 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
 
         if (l < FA_OPCOUNT_N) s.ops[l] = 0ull;
         wave_sync();
+        PhaseClock clk{B.prof, 0ull};
+        clk.start();
         build_pose(s, lig, B.genes + (size_t)pose * B.gene_stride);
         count_op(s.ops, OP_BUILD, L);
         if (B.lig_xyz) {
@@ -544,6 +563,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
             }
             wave_sync();
 
+            clk.stop(PH_BUILD);
             // ---- clash pre-pass (LIB/Vcontacts.cpp:36-51, :1803-1821): every
             // receptor contact and every ligand pair once (the earlier atom in
             // box order counts it), non-bonded, d < permeability * (rA + rB)
@@ -594,6 +614,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
             }
         }
 
+        clk.stop(PH_PREPASS);
         if (res.status == FA_OK) {
             // ---- contact polyhedra in box order (calc_region, LIB/Vcontacts.cpp:74-139)
             for (int i = l; i < L; i += WAVE) {
@@ -614,6 +635,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                 const float ax = s.lx[a], ay = s.ly[a], az = s.lz[a];
                 const float ra = lig.radius[a];
                 const float rado = ra + RW;
+                clk.stop(PH_REC);
                 // -- contact list (get_contlist4, LIB/Vcontacts.cpp:1725-1839)
                 int NC = 0, tested = 0;
                 bool overflow = false;
@@ -681,6 +703,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                 }
                 wave_sync();
 
+                clk.stop(PH_CULL);
                 // -- first vertex (get_firstvert, LIB/Vcontacts.cpp:499-590)
                 int pA = -1, pB = -1, pC = -1;
                 {
@@ -774,6 +797,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                     res.status = FA_CAPACITY;
                     break;
                 }
+                clk.stop(PH_FIRST);
                 // -- hull walk (LIB/Vcontacts.cpp:243-410)
                 {
                     double eA[4], eB[4], eC[4], tp[3];
@@ -896,6 +920,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                     res.app = res.wal + 1.0;
                     break;
                 }
+                clk.stop(PH_HULL);
                 // -- arc points (LIB/Vcontacts.cpp:419-457), in edge order
                 for (int base = 0; base < edgenum; base += WAVE) {
                     const int e = base + l;
@@ -1001,6 +1026,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                 for (int c = l; c < NC; c += WAVE)
                     if (s.A3[c] > 0.0) s.pflag[c] = 'E';
                 wave_sync();
+                clk.stop(PH_ARC);
                 // -- seeds for ligand atoms not seeded yet (LIB/Vcontacts.cpp:595-629)
                 for (int m = l; m < L; m += WAVE) {
                     if (s.seed[3 * m] != -1) continue;
@@ -1028,6 +1054,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                 }
                 wave_sync();
                 const int NV = vn;
+                clk.stop(PH_SEED);
                 // -- face point lists (order_faces, LIB/Vcontacts.cpp:699-831)
                 {
                     // count points per face (+1 slot for an opposite point)
@@ -1150,6 +1177,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                         break;
                     }
                 }
+                clk.stop(PH_FACE);
                 // -- areas (calc_areas + project_points, LIB/Vcontacts.cpp:838-1044, 1278-1344)
                 {
                     int epi = 0, eng[3] = {-1, -1, -1};
@@ -1336,6 +1364,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                         }
                     }
                 }
+                clk.stop(PH_AREA);
                 // -- records (save_areas) and the energy terms of atom a (vcfunction)
                 {
                     const double radA = (double)ra;
@@ -1424,6 +1453,7 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                     }
                     wave_sync();
                 }
+                clk.stop(PH_REC);
             }
             if (res.status == FA_OK) {
                 // atoms with no contact still contribute their full SAS

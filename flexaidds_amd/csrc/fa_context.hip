@@ -70,7 +70,8 @@ struct fa_ctx {
     int n_ligands = 0;
     bool have_system = false;
     // pose buffers
-    DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops;
+    DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops, prof;
+    bool profile = false;
     std::vector<uint32_t> last_counters = std::vector<uint32_t>(4, 0);
 
     void free_system()
@@ -212,6 +213,7 @@ int fa_ctx_create(int device_id, int max_pop, fa_ctx** out)
         (rc = ensure(c->scratch, fa::lse_scratch_bytes())) != FA_SUCCESS ||
         (rc = ensure(c->thermo, sizeof(fa_thermo))) != FA_SUCCESS ||
         (rc = ensure(c->ops, sizeof(fa_opcount))) != FA_SUCCESS ||
+        (rc = ensure(c->prof, 16 * sizeof(unsigned long long))) != FA_SUCCESS ||
         (rc = ensure(c->genes, sizeof(double) * (size_t)c->max_pop * 32)) != FA_SUCCESS ||
         (rc = ensure(c->out, sizeof(fa_cf_result) * (size_t)c->max_pop)) != FA_SUCCESS) {
         delete c;
@@ -227,7 +229,7 @@ int fa_ctx_destroy(fa_ctx* c)
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     c->free_system();
-    for (DevBuf* b : {&c->genes, &c->out, &c->counters, &c->recs, &c->reccnt, &c->xyz, &c->scratch, &c->thermo, &c->ops})
+    for (DevBuf* b : {&c->genes, &c->out, &c->counters, &c->recs, &c->reccnt, &c->xyz, &c->scratch, &c->thermo, &c->ops, &c->prof})
         if (b->p) (void)hipFree(b->p);
     delete c;
     return FA_SUCCESS;
@@ -384,6 +386,11 @@ static int launch(fa_ctx* c, int n_poses, const double* d_genes, int stride, con
     B.ops = static_cast<unsigned long long*>(c->ops.p);
     FA_HIP(hipMemsetAsync(c->counters.p, 0, 4 * sizeof(uint32_t), stream));
     FA_HIP(hipMemsetAsync(c->ops.p, 0, sizeof(fa_opcount), stream));
+    B.prof = nullptr;
+    if (c->profile) {
+        B.prof = static_cast<unsigned long long*>(c->prof.p);
+        FA_HIP(hipMemsetAsync(c->prof.p, 0, 16 * sizeof(unsigned long long), stream));
+    }
     // enough resident waves to cover the chip; each wave loops over poses
     const int lds = fa::smem_bytes();
     int per_cu = (160 * 1024) / (lds > 0 ? lds : 1);
@@ -561,6 +568,22 @@ int fa_last_counters(fa_ctx* c, uint32_t* slow_grid, uint32_t* capacity)
     FA_HIP(hipMemcpy(c->last_counters.data(), c->counters.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (slow_grid) *slow_grid = c->last_counters[0];
     if (capacity) *capacity = c->last_counters[1];
+    return FA_SUCCESS;
+}
+
+int fa_set_phase_profiling(fa_ctx* c, int enable)
+{
+    if (!c) return fail(FA_ERR_INVALID, "null context");
+    c->profile = enable != 0;
+    return FA_SUCCESS;
+}
+
+int fa_last_phase_cycles(fa_ctx* c, uint64_t* cycles16)
+{
+    if (!c || !cycles16) return fail(FA_ERR_INVALID, "null argument");
+    FA_HIP(hipSetDevice(c->device));
+    FA_HIP(hipDeviceSynchronize());
+    FA_HIP(hipMemcpy(cycles16, c->prof.p, 16 * sizeof(uint64_t), hipMemcpyDeviceToHost));
     return FA_SUCCESS;
 }
 

@@ -81,6 +81,7 @@ struct DevBatch {
     int xyz_stride;
     unsigned int* counters;         // [4]: slow-grid poses, capacity overflows, work queue, spare
     unsigned long long* ops;        // optional: [FA_OPCOUNT_N] operation counters (fa_opcount order)
+    unsigned long long* prof;       // optional: [16] cycles per kernel phase (summed over waves)
 };
 
 }  // namespace fa

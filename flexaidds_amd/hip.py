@@ -63,6 +63,8 @@ _SIGS = {
     "fa_last_counters": ([_P, _P, _P], C.c_int),
     "fa_last_opcount": ([_P, _P], C.c_int),
     "fa_upload_emat": ([_P, _P, C.c_int], C.c_int),
+    "fa_set_phase_profiling": ([_P, C.c_int], C.c_int),
+    "fa_last_phase_cycles": ([_P, _P], C.c_int),
     "fa_boltzmann_device": ([_P, _P, C.c_int, C.c_int64, C.c_double, _P, _P, _P], C.c_int),
     "fa_log_sum_exp_device": ([_P, _P, C.c_int, C.c_int64, _P, _P], C.c_int),
     "fa_kernel_lds_bytes": ([], C.c_int),
@@ -185,6 +187,17 @@ class Context:
         o = OpCount()
         _check(_lib.fa_last_opcount(self._ctx, C.byref(o)))
         return {f: getattr(o, f) for f, _ in OpCount._fields_}
+
+    PHASES = ("build", "prepass", "cull", "first_vertex", "hull_walk", "arc_points", "seeds", "faces", "areas",
+              "records")
+
+    def set_phase_profiling(self, on: bool):
+        _check(_lib.fa_set_phase_profiling(self._ctx, 1 if on else 0))
+
+    def phase_cycles(self) -> dict:
+        a = np.zeros(16, np.uint64)
+        _check(_lib.fa_last_phase_cycles(self._ctx, a.ctypes.data))
+        return {k: int(a[i]) for i, k in enumerate(self.PHASES)}
 
     def counters(self):
         a, b = C.c_uint32(), C.c_uint32()
