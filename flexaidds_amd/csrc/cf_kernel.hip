@@ -14,21 +14,41 @@
 //
 // Ligand atoms of a pose are processed in the reference's box order because
 // the `done`/stub contact rule and the hull seeds carry state from one atom to
-// the next (LIB/Vcontacts.cpp:1750-1836, :595-629).  Inside an atom, lanes work
-// on candidates, planes, edges, vertices and faces.
+// the next (LIB/Vcontacts.cpp:1750-1836, :595-629).  Work inside an atom is
+// spread over lanes:
+//   - neighbour cull, once per atom: the 27 cell ranges are fetched in one
+//     round (the next atom's while the current one streams), candidates go
+//     through in 64-wide chunks with the next chunk's loads in flight, and the
+//     in-range ones land in a per-wave neighbour list in HBM.  The clash
+//     pre-pass sums its walls on the way; the polyhedron pass re-reads the
+//     list instead of scanning the cells again;
+//   - hull walk: lane l keeps planes l, l+64 and edges l, l+64, ... in
+//     registers; the closest-plane search is a DPP reduction, the open-edge
+//     queue is a set of wave-uniform bit masks, so a step touches LDS only to
+//     store the new vertex;
+//   - arc points, faces, areas and records: lanes over edges / faces /
+//     contacts, compactions with ballots.
+// The LDS working set is ~19 KB, so eight waves share a CU.
 #include <hip/hip_runtime.h>
+
+#include <stdint.h>
 
 #include "fa_device.h"
 
 namespace fa {
 
 constexpr int WAVE = 64;
-constexpr int MAXC = 160;          // contacts per atom (planes)
-constexpr int MAXP = MAXC + 4;     // + 4 bounding planes
-constexpr int MAXV = 448;          // polyhedron vertices incl. arc and opposite points
-constexpr int MAXE = 256;          // hull edges (the walk aborts near 200)
-constexpr int MAXF = 1280;         // face point-list entries
+constexpr int MAXC = 128;                      // contacts (planes) of one atom
+constexpr int MAXP = MAXC + 4;                 // + 4 bounding planes
+constexpr int PSL = MAXC / WAVE;               // plane register slots per lane
+constexpr int MAXV = 160;                      // hull vertices, arc points, opposite points
+constexpr int VSL = (MAXV + WAVE - 1) / WAVE;  // vertex slots per lane
+constexpr int MAXE = 256;                      // hull edges (the walk fails at 200)
+constexpr int ESL = MAXE / WAVE;               // edge register slots per lane
+constexpr int MAXF = 384;                      // face point-list slots
 constexpr int MAXL = FA_MAX_LIG_ATOMS;
+constexpr int LSL = MAXL / WAVE;               // ligand atom slots per lane
+constexpr unsigned NONE = 0xFFu;               // packed plane id "none"
 
 constexpr double PI_REF = 3.141592;     // LIB/flexaid.h:86
 constexpr float RW = 1.4f;              // LIB/flexaid.h:88
@@ -39,38 +59,40 @@ constexpr float CELLSIZE = 6.5f;        // LIB/Vcontacts.h:56
 constexpr int EDGE_LIMIT = 200;         // LIB/Vcontacts.cpp:378
 constexpr double BIG = 9.9e+9;          // sentinel of the reference scans
 
+static_assert(MAXC % WAVE == 0 && MAXE % WAVE == 0 && MAXL % WAVE == 0, "slot layout");
+static_assert(MAXP < (int)NONE, "plane ids are packed in 8 bits");
+static_assert(MAXE >= EDGE_LIMIT + 2, "the edge limit must be reached before the edge capacity");
+
 struct Smem {
-    // planes / contacts (index c < NC are contacts, NC..NC+3 bounding planes)
+    // planes of the current atom: c < NC contacts, NC..NC+3 bounding planes
     double A0[MAXP], A1[MAXP], A2[MAXP], A3[MAXP];
-    double pdist[MAXP];
-    double area[MAXP];
-    double cdist[MAXC];
-    int pidx[MAXP];
-    char pflag[MAXP];
-    // vertices
-    double vx[MAXV], vy[MAXV], vz[MAXV], vd[MAXV];
-    short vp0[MAXV], vp1[MAXV], vp2[MAXV];
-    // edges
-    double eV0[MAXE], eV1[MAXE], eV2[MAXE];
-    short es[MAXE], ee[MAXE], ep0[MAXE], ep1[MAXE], esp[MAXE], eep[MAXE];
-    char earc[MAXE];
-    // faces (per plane point lists, CSR)
-    short foff[MAXC + 1];
-    short fcnt[MAXC];
-    short fpt[MAXF];
-    double fcos[MAXF];
-    // ligand state
-    float lx[MAXL], ly[MAXL], lz[MAXL];
-    float ldis[MAXL], lang[MAXL], ldih[MAXL];
-    int lbox[MAXL];
-    unsigned char order[MAXL];
-    int seed[MAXL * 3];
-    unsigned long long stub[MAXL * 2];
-    unsigned char done[MAXL];
-    double acc_com[MAXL], acc_wal[MAXL], acc_sas[MAXL], acc_tot[MAXL];
-    unsigned char acc_rcl[MAXL];
-    // operation counters of the current pose (fa_opcount order)
-    unsigned long long ops[FA_OPCOUNT_N];
+    // polyhedron vertices (coordinates relative to the atom)
+    double vx[MAXV], vy[MAXV], vz[MAXV];
+    // storage of phases that never overlap
+    union {
+        double fcos[MAXF];                  // face ordering cosines
+        double area[MAXC];                  // areas while engulfing corrections run
+        struct {
+            double cd[MAXC];                // contact distance, radius and index
+            float rb[MAXC];                 //   on their way into registers
+            int idx[MAXC];
+        } c;
+        struct {
+            float dis[MAXL], ang[MAXL], dih[MAXL];
+        } b;                                // internal coordinates (pose build)
+    } u;
+    unsigned long long stub[2 * MAXL];      // stub[2m..2m+1]: atoms with a record of m
+    unsigned long long ops[FA_OPCOUNT_N];   // operation counters of the pose
+    unsigned vpl[MAXV];                     // vertex planes p0 | p1 << 8 | p2 << 16
+    float lx[MAXL], ly[MAXL], lz[MAXL];     // ligand coordinates (calc order)
+    int sbox[MAXL];                         // cell of the atom of each rank
+    int nstart[MAXL + 1];                   // neighbour list of rank k: [nstart[k], nstart[k+1])
+    int seed[3 * MAXL];                     // hull seeds (calc indices)
+    short fpt[MAXF];                        // face point lists (CSR)
+    short foff[MAXC];
+    unsigned char fcnt[MAXC];
+    char pflag[MAXP];                       // 'X' off the polyhedron, 'Y' on it, 'E' engulfing
+    unsigned char order[MAXL], rank[MAXL];  // box order: calc index of rank k, rank of atom i
 };
 
 // ------------------------------------------------------------ wave helpers
@@ -106,31 +128,82 @@ __device__ __forceinline__ int wave_excl_scan(int v)
     return x - v;
 }
 
-// argmin over (value, index), ties to the lowest index: the result of a
-// sequential scan that replaces only on a strictly smaller value
-__device__ __forceinline__ void wave_argmin(double& v, int& i)
+// LDS visibility point for the whole (single-wave) work-group
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+__device__ __forceinline__ int readlane_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+__device__ __forceinline__ unsigned readlane_u(unsigned v, int lane)
 {
-    for (int off = 32; off > 0; off >>= 1) {
-        double ov = __shfl_xor(v, off);
-        int oi = __shfl_xor(i, off);
-        if (ov < v || (ov == v && oi < i)) {
-            v = ov;
-            i = oi;
-        }
-    }
+    return (unsigned)__builtin_amdgcn_readlane((int)v, lane);
 }
 
-__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+__device__ __forceinline__ float readlane_f(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v)
+{
+    // every control used below reads a lane of the same row, so no lane reads
+    // out of range; `old` (all ones) is the identity of the minimum anyway
+    const int lo = __builtin_amdgcn_update_dpp(-1, (int)(uint32_t)v, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(-1, (int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+__device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+
+// Minimum over the wave of an unsigned 64-bit key.  Must be called with all
+// lanes active.  DPP inside each 16-lane row, then the four row minima.
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
+{
+    v = umin64(v, dpp_u64<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = umin64(v, dpp_u64<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = umin64(v, dpp_u64<0x141>(v));  // row_half_mirror
+    v = umin64(v, dpp_u64<0x140>(v));  // row_mirror
+    const uint64_t r0 = readlane_u64(v, 0), r1 = readlane_u64(v, 16);
+    const uint64_t r2 = readlane_u64(v, 32), r3 = readlane_u64(v, 48);
+    return umin64(umin64(r0, r1), umin64(r2, r3));
+}
+
+// non-negative doubles order like their bit patterns
+__device__ __forceinline__ uint64_t key_of(double v) { return (uint64_t)__double_as_longlong(v); }
 
 enum OpIdx { OP_CULL = 0, OP_PLANE, OP_HULL, OP_ARC, OP_ORDER, OP_TRI, OP_SARC, OP_VC, OP_BUILD };
 
-// optional per-phase cycle accounting (debug builds of a launch only)
+// add a wave-uniform count to the pose's operation counter (lane 0 writes)
+__device__ __forceinline__ void count_op(unsigned long long* ops, int k, long long n)
+{
+    if (lane_id() == 0) ops[k] += (unsigned long long)n;
+}
+
+// optional per-phase cycle accounting (diagnostic launches only)
 enum Phase { PH_BUILD = 0, PH_PREPASS, PH_CULL, PH_FIRST, PH_HULL, PH_ARC, PH_SEED, PH_FACE, PH_AREA, PH_REC, PH_N };
 
 struct PhaseClock {
     unsigned long long* acc;  // null when disabled
     unsigned long long t;
-    __device__ void start() { if (acc) t = __builtin_readcyclecounter(); }
+    __device__ void start()
+    {
+        if (acc) t = __builtin_readcyclecounter();
+    }
     __device__ void stop(int ph)
     {
         if (acc) {
@@ -140,12 +213,6 @@ struct PhaseClock {
         }
     }
 };
-
-// add a wave-uniform count to the pose's operation counter (lane 0 writes)
-__device__ __forceinline__ void count_op(unsigned long long* ops, int k, long long n)
-{
-    if (lane_id() == 0) ops[k] += (unsigned long long)n;
-}
 
 // ------------------------------------------------------------ geometry (FP64)
 // LIB/Vcontacts.cpp:1384-1413
@@ -254,30 +321,151 @@ __device__ __forceinline__ void plane_of(const Smem& s, int c, double* a)
     a[3] = s.A3[c];
 }
 
-// LIB/Vcontacts.cpp:599-637 (all lanes compute, lane 0 stores)
-__device__ void add_vedge(Smem& s, int k, int p0, int p1, int tp, int sp)
+// the four bounding planes (LIB/Vcontacts.cpp:223-241)
+__device__ __forceinline__ void bounding_plane(int k, double* a)
 {
-    double V0 = s.A1[p0] * s.A2[p1] - s.A2[p0] * s.A1[p1];
-    double V1 = s.A2[p0] * s.A0[p1] - s.A0[p0] * s.A2[p1];
-    double V2 = s.A0[p0] * s.A1[p1] - s.A1[p0] * s.A0[p1];
-    double t0 = s.vx[sp] + V0, t1 = s.vy[sp] + V1, t2 = s.vz[sp] + V2;
-    double tv = s.A0[tp] * t0 + s.A1[tp] * t1 + s.A2[tp] * t2 + s.A3[tp];
+    a[0] = (k < 2) ? 0.707 : -0.707;
+    a[1] = (k == 0) ? 1.0 : (k == 1) ? -1.0 : 0.0;
+    a[2] = (k == 2) ? 1.0 : (k == 3) ? -1.0 : 0.0;
+    a[3] = -10.0;
+}
+
+// vertex planes, packed 8 bits each (NONE = -1)
+__device__ __forceinline__ unsigned pack_planes(int a, int b, int c)
+{
+    return ((unsigned)a & NONE) | (((unsigned)b & NONE) << 8) | (((unsigned)c & NONE) << 16);
+}
+
+__device__ __forceinline__ int plane_at(unsigned pk, int k)
+{
+    const unsigned v = (pk >> (8 * k)) & NONE;
+    return v == NONE ? -1 : (int)v;
+}
+
+__device__ __forceinline__ bool has_plane(unsigned pk, int X)
+{
+    return (int)(pk & NONE) == X || (int)((pk >> 8) & NONE) == X || (int)((pk >> 16) & NONE) == X;
+}
+
+// distance of a hull vertex from the atom (the value the reference stores)
+__device__ __forceinline__ double vnorm(const Smem& s, int v)
+{
+    const double x = s.vx[v], y = s.vy[v], z = s.vz[v];
+    return sqrt(x * x + y * y + z * z);
+}
+
+// ------------------------------------------------------------ edge registers
+// Edge e lives in lane (e & 63), slot (e >> 6) as two packed words; its
+// direction is recomputed from its planes when the walk takes it up.
+struct Edges {
+    unsigned pl[ESL];                   // plane0 | plane1 << 8 | start plane << 16 | end plane << 24
+    unsigned vt[ESL];                   // start vertex | end vertex << 16
+    unsigned long long open[ESL];       // wave-uniform: the edge still needs its end point
+};
+
+#define FA_EDGE_SLOTS(M) M(0) M(1) M(2) M(3)
+static_assert(ESL == 4, "FA_EDGE_SLOTS lists every edge slot");
+
+// new edge along planes p0 x p1 from vertex sv, turned away from plane tp
+__device__ __forceinline__ void edge_new(Edges& E, int e, int p0, int p1, int tp, int sv)
+{
+    const unsigned pl = ((unsigned)p0 & NONE) | (((unsigned)p1 & NONE) << 8) | (((unsigned)tp & NONE) << 16) |
+                        (NONE << 24);
+    const unsigned vt = (unsigned)sv | 0xFFFF0000u;
+    const bool me = lane_id() == (e & 63);
+    switch (e >> 6) {
+#define FA_NEW(K)                      \
+    case K:                            \
+        if (me) {                      \
+            E.pl[K] = pl;              \
+            E.vt[K] = vt;              \
+        }                              \
+        E.open[K] |= 1ull << (e & 63); \
+        break;
+        FA_EDGE_SLOTS(FA_NEW)
+#undef FA_NEW
+    }
+}
+
+// end point and end plane of edge e; the edge leaves the open queue
+__device__ __forceinline__ void edge_close(Edges& E, int e, int end_plane, int end_vertex)
+{
+    const bool me = lane_id() == (e & 63);
+    switch (e >> 6) {
+#define FA_CLOSE(K)                                                                   \
+    case K:                                                                           \
+        if (me) {                                                                     \
+            E.pl[K] = (E.pl[K] & 0x00FFFFFFu) | (((unsigned)end_plane & NONE) << 24); \
+            E.vt[K] = (E.vt[K] & 0xFFFFu) | ((unsigned)end_vertex << 16);             \
+        }                                                                             \
+        E.open[K] &= ~(1ull << (e & 63));                                             \
+        break;
+        FA_EDGE_SLOTS(FA_CLOSE)
+#undef FA_CLOSE
+    }
+}
+
+// packed planes and vertices of edge e (wave-uniform)
+__device__ __forceinline__ void edge_get(const Edges& E, int e, unsigned& pl, unsigned& vt)
+{
+    const int ln = e & 63;
+    pl = 0u;
+    vt = 0u;
+    switch (e >> 6) {
+#define FA_GET(K)                   \
+    case K:                         \
+        pl = readlane_u(E.pl[K], ln); \
+        vt = readlane_u(E.vt[K], ln); \
+        break;
+        FA_EDGE_SLOTS(FA_GET)
+#undef FA_GET
+    }
+}
+
+// lowest-numbered open edge, -1 when none
+__device__ __forceinline__ int first_open(const Edges& E)
+{
+#pragma unroll
+    for (int k = 0; k < ESL; ++k)
+        if (E.open[k]) return k * 64 + first_lane(E.open[k]);
+    return -1;
+}
+
+// first edge e in (lo, hi) between planes P and C (either order), -1 if none
+__device__ __forceinline__ int find_edge(const Edges& E, int lo, int hi, int P, int C)
+{
+    const int l = lane_id();
+#pragma unroll
+    for (int k = 0; k < ESL; ++k) {
+        if (k * 64 >= hi) break;
+        const int e = k * 64 + l;
+        const int q0 = (int)(E.pl[k] & NONE), q1 = (int)((E.pl[k] >> 8) & NONE);
+        const bool hit = e > lo && e < hi && ((q0 == P && q1 == C) || (q0 == C && q1 == P));
+        const unsigned long long m = __ballot(hit);
+        if (m) return k * 64 + first_lane(m);
+    }
+    return -1;
+}
+
+// LIB/Vcontacts.cpp:599-637: direction of the edge along planes p0 x p1 from
+// the point (x, y, z), turned away from plane tp (the same arithmetic at
+// every call, so the walk sees the values the reference stores)
+__device__ __forceinline__ void edge_dir(const Smem& s, int p0, int p1, int tp, double x, double y, double z,
+                                         double& V0, double& V1, double& V2)
+{
+    double a[4], b[4], t[4];
+    plane_of(s, p0, a);
+    plane_of(s, p1, b);
+    plane_of(s, tp, t);
+    V0 = a[1] * b[2] - a[2] * b[1];
+    V1 = a[2] * b[0] - a[0] * b[2];
+    V2 = a[0] * b[1] - a[1] * b[0];
+    const double t0 = x + V0, t1 = y + V1, t2 = z + V2;
+    const double tv = t[0] * t0 + t[1] * t1 + t[2] * t2 + t[3];
     if (tv > 0.0) {
         V0 = -V0;
         V1 = -V1;
         V2 = -V2;
-    }
-    if (lane_id() == 0) {
-        s.eV0[k] = V0;
-        s.eV1[k] = V1;
-        s.eV2[k] = V2;
-        s.es[k] = (short)sp;
-        s.ee[k] = -1;
-        s.ep0[k] = (short)p0;
-        s.ep1[k] = (short)p1;
-        s.esp[k] = (short)tp;
-        s.eep[k] = -1;
-        s.earc[k] = '.';
     }
 }
 
@@ -287,27 +475,27 @@ __device__ __forceinline__ void put_vertex(Smem& s, int vn, double x, double y, 
         s.vx[vn] = x;
         s.vy[vn] = y;
         s.vz[vn] = z;
-        s.vd[vn] = sqrt(x * x + y * y + z * z);
-        s.vp0[vn] = (short)a;
-        s.vp1[vn] = (short)b;
-        s.vp2[vn] = (short)c;
+        s.vpl[vn] = pack_planes(a, b, c);
     }
 }
 
 // ------------------------------------------------------------ pose build
-// LIB/ic2cf.cpp:53-105 then LIB/buildcc.cpp:6-134, lane 0 (FP32 like the
-// reference; cos/sin are taken in double and rounded, which is what the host
-// libm's correctly rounded cosf/sinf return)
+// LIB/ic2cf.cpp:53-105 then LIB/buildcc.cpp:6-134 (FP32 like the reference;
+// cos/sin are taken in double and rounded, i.e. correctly rounded floats).
+// The gene decode is serial (shift chains); the trig runs one atom per lane;
+// the rebuild is serial over atoms with the coordinates of atom i held by
+// lane i & 63, slot i >> 6.
 __device__ void build_pose(Smem& s, const DevLigand& lig, const double* icv)
 {
     const int n = lig.n_atoms;
-    for (int i = lane_id(); i < n; i += WAVE) {
-        s.ldis[i] = lig.dis[i];
-        s.lang[i] = lig.ang[i];
-        s.ldih[i] = lig.dih[i];
+    const int l = lane_id();
+    for (int i = l; i < n; i += WAVE) {
+        s.u.b.dis[i] = lig.dis[i];
+        s.u.b.ang[i] = lig.ang[i];
+        s.u.b.dih[i] = lig.dih[i];
     }
     wave_sync();
-    if (lane_id() == 0) {
+    if (l == 0) {
         for (int g = 0; g < lig.n_genes; ++g) {
             double v = icv[g];
             if (v > lig.gene_max[g]) v = lig.gene_max[g];
@@ -317,85 +505,130 @@ __device__ void build_pose(Smem& s, const DevLigand& lig, const double* icv)
             if (typ == FA_GENE_GRID) {
                 unsigned gi = (unsigned)v;
                 if ((int)gi >= lig.n_grid) gi = lig.n_grid - 1;
-                s.ldis[atm] = lig.grid_ic[3 * gi];
-                s.lang[atm] = lig.grid_ic[3 * gi + 1];
-                s.ldih[atm] = lig.grid_ic[3 * gi + 2];
+                s.u.b.dis[atm] = lig.grid_ic[3 * gi];
+                s.u.b.ang[atm] = lig.grid_ic[3 * gi + 1];
+                s.u.b.dih[atm] = lig.grid_ic[3 * gi + 2];
             } else if (typ == FA_GENE_DIS) {
-                s.ldis[atm] = (float)v;
+                s.u.b.dis[atm] = (float)v;
             } else if (typ == FA_GENE_ANG) {
-                s.lang[atm] = (float)v;
+                s.u.b.ang[atm] = (float)v;
             } else if (typ == FA_GENE_DIH) {
-                s.ldih[atm] = (float)v;
+                s.u.b.dih[atm] = (float)v;
                 int j = atm, cat = lig.rec[4 * j + 3];
                 if (cat >= 0) {
                     for (int guard = 0; cat != atm && guard < n; ++guard) {
-                        s.ldih[cat] = s.ldih[j] + lig.shift[cat];
+                        s.u.b.dih[cat] = s.u.b.dih[j] + lig.shift[cat];
                         j = cat;
                         cat = lig.rec[4 * j + 3];
                     }
                 }
             }
         }
-        for (int an = 0; an < n; ++an) {
-            const int a = lig.mov[an];
-            float x[4], y[4], z[4];
-            for (int i = 1; i <= 3; ++i) {
-                const int j = lig.rec[4 * a + i - 1];
-                if (j >= 0) {
-                    x[i] = s.lx[j];
-                    y[i] = s.ly[j];
-                    z[i] = s.lz[j];
-                } else if (i == 1) {
-                    x[i] = 1.0f + lig.ori[0];
-                    y[i] = 0.0f + lig.ori[1];
-                    z[i] = 0.0f + lig.ori[2];
-                } else if (i == 3) {
-                    x[i] = 0.0f + lig.ori[0];
-                    y[i] = 1.0f + lig.ori[1];
-                    z[i] = 0.0f + lig.ori[2];
-                } else {
-                    x[i] = 0.0f + lig.ori[0];
-                    y[i] = 0.0f + lig.ori[1];
-                    z[i] = 0.0f + lig.ori[2];
-                }
-                x[i] += 1e-10f;
-                y[i] += 1e-10f;
-                z[i] += 1e-10f;
+    }
+    wave_sync();
+    float ca[LSL], sa[LSL], cd[LSL], sd[LSL], di[LSL];
+    float X[LSL], Y[LSL], Z[LSL];
+#pragma unroll
+    for (int q = 0; q < LSL; ++q) {
+        const int a = l + WAVE * q;
+        ca[q] = sa[q] = cd[q] = sd[q] = di[q] = 0.0f;
+        X[q] = Y[q] = Z[q] = 0.0f;
+        if (a < n) {
+            const float angPI = (float)(s.u.b.ang[a] * PI_REF / 180.0f);
+            const float dihPI = (float)(s.u.b.dih[a] * PI_REF / 180.0f);
+            ca[q] = (float)cos((double)angPI);
+            sa[q] = (float)sin((double)angPI);
+            cd[q] = (float)cos((double)dihPI);
+            sd[q] = (float)sin((double)dihPI);
+            di[q] = s.u.b.dis[a];
+        }
+    }
+    static_assert(LSL == 2, "two ligand slots per lane");
+    for (int an = 0; an < n; ++an) {
+        const int a = lig.mov[an];
+        const int la = a & 63;
+        const bool hi = a >= WAVE;
+        const float cta = readlane_f(hi ? ca[1] : ca[0], la);
+        const float sta = readlane_f(hi ? sa[1] : sa[0], la);
+        const float ctd = readlane_f(hi ? cd[1] : cd[0], la);
+        const float std_ = readlane_f(hi ? sd[1] : sd[0], la);
+        const float dis = readlane_f(hi ? di[1] : di[0], la);
+        float x[4], y[4], z[4];
+        for (int i = 1; i <= 3; ++i) {
+            const int j = lig.rec[4 * a + i - 1];
+            if (j >= 0) {
+                const bool hj = j >= WAVE;
+                x[i] = readlane_f(hj ? X[1] : X[0], j & 63);
+                y[i] = readlane_f(hj ? Y[1] : Y[0], j & 63);
+                z[i] = readlane_f(hj ? Z[1] : Z[0], j & 63);
+            } else if (i == 1) {
+                x[i] = 1.0f + lig.ori[0];
+                y[i] = 0.0f + lig.ori[1];
+                z[i] = 0.0f + lig.ori[2];
+            } else if (i == 3) {
+                x[i] = 0.0f + lig.ori[0];
+                y[i] = 1.0f + lig.ori[1];
+                z[i] = 0.0f + lig.ori[2];
+            } else {
+                x[i] = 0.0f + lig.ori[0];
+                y[i] = 0.0f + lig.ori[1];
+                z[i] = 0.0f + lig.ori[2];
             }
-            float A = y[1] * (z[2] - z[3]) + y[2] * (z[3] - z[1]) + y[3] * (z[1] - z[2]);
-            float B = z[1] * (x[2] - x[3]) + z[2] * (x[3] - x[1]) + z[3] * (x[1] - x[2]);
-            float C = x[1] * (y[2] - y[3]) + x[2] * (y[3] - y[1]) + x[3] * (y[1] - y[2]);
-            float op = sqrtf(A * A + B * B + C * C);
-            float cx = A / op, cy = B / op, cz = C / op;
-            A = x[2] - x[1];
-            B = y[2] - y[1];
-            C = z[2] - z[1];
-            float d = 1.0f / sqrtf(A * A + B * B + C * C);
-            op = s.ldis[a] * d;
-            float xn = A * op, yn = B * op, zn = C * op;
-            A = cx * cx;
-            B = cy * cy;
-            C = cz * cz;
-            float angPI = (float)(s.lang[a] * PI_REF / 180.0f);
-            float ct = (float)cos((double)angPI);
-            float st = -(float)sin((double)angPI);
-            op = 1.0f - ct;
-            float xk = (cx * cz * op - cy * st) * zn + ((1.0f - A) * ct + A) * xn + (cx * cy * op + cz * st) * yn;
-            float yk = (cy * cx * op - cz * st) * xn + ((1.0f - B) * ct + B) * yn + (cy * cz * op + cx * st) * zn;
-            float zk = (cz * cy * op - cx * st) * yn + ((1.0f - C) * ct + C) * zn + (cz * cx * op + cy * st) * xn;
-            float dihPI = (float)(s.ldih[a] * PI_REF / 180.0f);
-            ct = (float)cos((double)dihPI);
-            st = (float)sin((double)dihPI);
-            op = 1.0f - ct;
-            cx = (x[2] - x[1]) * d;
-            cy = (y[2] - y[1]) * d;
-            cz = (z[2] - z[1]) * d;
-            A = cx * cx;
-            B = cy * cy;
-            C = cz * cz;
-            s.lx[a] = (cx * cz * op - cy * st) * zk + ((1.0f - A) * ct + A) * xk + (cx * cy * op + cz * st) * yk + x[1];
-            s.ly[a] = (cy * cx * op - cz * st) * xk + ((1.0f - B) * ct + B) * yk + (cy * cz * op + cx * st) * zk + y[1];
-            s.lz[a] = (cz * cy * op - cx * st) * yk + ((1.0f - C) * ct + C) * zk + (cz * cx * op + cy * st) * xk + z[1];
+            x[i] += 1e-10f;
+            y[i] += 1e-10f;
+            z[i] += 1e-10f;
+        }
+        float A = y[1] * (z[2] - z[3]) + y[2] * (z[3] - z[1]) + y[3] * (z[1] - z[2]);
+        float B = z[1] * (x[2] - x[3]) + z[2] * (x[3] - x[1]) + z[3] * (x[1] - x[2]);
+        float C = x[1] * (y[2] - y[3]) + x[2] * (y[3] - y[1]) + x[3] * (y[1] - y[2]);
+        float op = sqrtf(A * A + B * B + C * C);
+        float cx = A / op, cy = B / op, cz = C / op;
+        A = x[2] - x[1];
+        B = y[2] - y[1];
+        C = z[2] - z[1];
+        float d = 1.0f / sqrtf(A * A + B * B + C * C);
+        op = dis * d;
+        float xn = A * op, yn = B * op, zn = C * op;
+        A = cx * cx;
+        B = cy * cy;
+        C = cz * cz;
+        float ct = cta;
+        float st = -sta;
+        op = 1.0f - ct;
+        float xk = (cx * cz * op - cy * st) * zn + ((1.0f - A) * ct + A) * xn + (cx * cy * op + cz * st) * yn;
+        float yk = (cy * cx * op - cz * st) * xn + ((1.0f - B) * ct + B) * yn + (cy * cz * op + cx * st) * zn;
+        float zk = (cz * cy * op - cx * st) * yn + ((1.0f - C) * ct + C) * zn + (cz * cx * op + cy * st) * xn;
+        ct = ctd;
+        st = std_;
+        op = 1.0f - ct;
+        cx = (x[2] - x[1]) * d;
+        cy = (y[2] - y[1]) * d;
+        cz = (z[2] - z[1]) * d;
+        A = cx * cx;
+        B = cy * cy;
+        C = cz * cz;
+        const float ox = (cx * cz * op - cy * st) * zk + ((1.0f - A) * ct + A) * xk + (cx * cy * op + cz * st) * yk + x[1];
+        const float oy = (cy * cx * op - cz * st) * xk + ((1.0f - B) * ct + B) * yk + (cy * cz * op + cx * st) * zk + y[1];
+        const float oz = (cz * cy * op - cx * st) * yk + ((1.0f - C) * ct + C) * zk + (cz * cx * op + cy * st) * xk + z[1];
+        if (l == la) {
+            if (hi) {
+                X[1] = ox;
+                Y[1] = oy;
+                Z[1] = oz;
+            } else {
+                X[0] = ox;
+                Y[0] = oy;
+                Z[0] = oz;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < LSL; ++q) {
+        const int a = l + WAVE * q;
+        if (a < n) {
+            s.lx[a] = X[q];
+            s.ly[a] = Y[q];
+            s.lz[a] = Z[q];
         }
     }
     wave_sync();
@@ -409,16 +642,15 @@ struct PoseGrid {
 };
 
 // LIB/Vcontacts.cpp:1594-1658 (steady state): only scorable atoms widen it
-__device__ bool pose_grid(const Smem& s, const DevLigand& lig, const DevGrid& G, PoseGrid& pg)
+__device__ void pose_grid(const Smem& s, const DevLigand& lig, const DevGrid& G, PoseGrid& pg)
 {
     float gmin[3], gmax[3];
     for (int j = 0; j < 3; ++j) {
         gmin[j] = G.gmin[j];
         gmax[j] = G.gmax[j];
     }
-    // sequential semantics: scan ligand atoms in calc order
     int alter = 0;
-    for (int i = 0; i < lig.n_atoms; ++i) {
+    for (int i = 0; i < lig.n_atoms; ++i) {  // calc order, like the reference
         const float c[3] = {s.lx[i], s.ly[i], s.lz[i]};
         for (int j = 0; j < 3; ++j) {
             if (c[j] < gmin[j]) {
@@ -444,7 +676,6 @@ __device__ bool pose_grid(const Smem& s, const DevLigand& lig, const DevGrid& G,
     pg.dim3 = pg.dim2 * pg.dim;
     for (int j = 0; j < 3; ++j) pg.gmin[j] = gmin[j];
     pg.slow = alter != 0;
-    return true;
 }
 
 __device__ __forceinline__ int box_of(const PoseGrid& pg, float x, float y, float z)
@@ -453,59 +684,243 @@ __device__ __forceinline__ int box_of(const PoseGrid& pg, float x, float y, floa
            (int)((z - pg.gmin[2]) / CELLSIZE);
 }
 
-// ------------------------------------------------------------ candidate scan
-// Calls f(valid, calc_index, x, y, z, r) for every atom of the 27 boxes
-// around box `bz`, in the reference's order (slot, then calc order in a box:
-// receptor atoms first, then ligand atoms).  All lanes call f together.
-template <class F>
-__device__ void scan_boxes(const Smem& s, const DevReceptor& R, const DevGrid& G, const PoseGrid& pg, int L, int bz,
-                           F&& f)
+// cell of neighbourhood slot 0..26 around cell bz (LIB/Vcontacts.cpp:1738-1741)
+__device__ __forceinline__ int slot_box(const PoseGrid& pg, int bz, int slot)
 {
-    const int l = lane_id();
-    for (int slot = 0; slot < 27; ++slot) {
-        const int bi = bz + pg.dim2 * ((slot / 9) - 1) + pg.dim * (((slot / 3) % 3) - 1) + (slot % 3) - 1;
-        if (bi < 0 || bi >= pg.dim3) continue;
-        if (!pg.slow) {
-            const int b0 = G.cell_start[bi], b1 = G.cell_start[bi + 1];
-            for (int base = b0; base < b1; base += WAVE) {
-                const int k = base + l;
-                const bool v = k < b1;
-                float4 p = v ? G.cell_xyzr[k] : make_float4(0.f, 0.f, 0.f, 0.f);
-                const int j = v ? G.cell_atom[k] : -1;
-                f(v, j, p.x, p.y, p.z, p.w);
-            }
-        } else {
-            for (int base = 0; base < R.n; base += WAVE) {
-                const int j = base + l;
-                bool v = j < R.n;
-                float x = 0.f, y = 0.f, z = 0.f, r = 0.f;
-                if (v) {
-                    x = R.x[j];
-                    y = R.y[j];
-                    z = R.z[j];
-                    r = R.rad[j];
-                    v = box_of(pg, x, y, z) == bi;
+    return bz + pg.dim2 * ((slot / 9) - 1) + pg.dim * (((slot / 3) % 3) - 1) + (slot % 3) - 1;
+}
+
+// how many of the 27 slots around bz name cell bz + d
+__device__ __forceinline__ int slot_multiplicity(const PoseGrid& pg, int d)
+{
+    int mult = 0;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int r = d - pg.dim2 * dx - pg.dim * dy;
+            mult += (r >= -1 && r <= 1);
+        }
+    return mult;
+}
+
+__device__ __forceinline__ bool bit_of(unsigned long long b0, unsigned long long b1, int m)
+{
+    return ((m < 64 ? (b0 >> (m & 63)) : (b1 >> (m & 63))) & 1ull) != 0ull;
+}
+
+// ------------------------------------------------------------ neighbour cull
+// One candidate of the 27-cell scan, in the reference's order (slot, then
+// receptor atoms of the cell, then ligand atoms of the cell; calc order).
+struct Cand {
+    float x, y, z, r;
+    int j;      // calc index, -1 when the lane has no candidate
+    int m;      // ligand index (j - Rn) for ligand candidates
+    bool lj;
+};
+
+// Per-atom cull state and results (wave-uniform except `clash`).
+struct CullAcc {
+    int n_near;
+    int total, tested, found;   // lane partials until reduced
+    double clash;               // lane partial
+    bool over;
+};
+
+struct CullCtx {
+    int acalc, a;
+    float ax, ay, az, ra, rado;
+    unsigned long long brow0, brow1;   // bonded row of atom a
+    unsigned long long dpre0, dpre1;   // atoms already marked by the pre-pass
+    double permea;
+    int2* nidx;
+    float4* nxyz;
+    int cap;
+};
+
+// x^-12 (the reference calls pow(x, -12.0); this agrees to a few ulps)
+__device__ __forceinline__ double inv12(double x)
+{
+    const double x2 = x * x, x4 = x2 * x2, x12 = x4 * x4 * x4;
+    return 1.0 / x12;
+}
+
+__device__ __forceinline__ void visit(const CullCtx& cc, const Cand& c, CullAcc& acc)
+{
+    bool in = false;
+    float sq = 0.0f;
+    if (c.j >= 0 && c.j != cc.acalc) {
+        acc.total += 1;
+        const float dx = cc.ax - c.x, dy = cc.ay - c.y, dz = cc.az - c.z;
+        sq = dx * dx + dy * dy + dz * dz;
+        const double nd = (double)(cc.rado + c.r + RW);
+        in = (double)sq < nd * nd && sq != 0.0f;
+        // LIB/Vcontacts.cpp:36-51 (pre-pass): a ligand atom counted once by
+        // the earlier atom in box order, i.e. skipped once it is marked
+        const bool pskip = c.lj && bit_of(cc.dpre0, cc.dpre1, c.m);
+        if (!pskip) {
+            acc.tested += 1;
+            if (in) {
+                acc.found += 1;
+                const double dist = sqrt((double)sq);
+                const double rAB = (double)(cc.ra + c.r);
+                const double cdist = cc.permea * rAB;
+                if (dist < cdist) {
+                    const bool bonded = c.lj && bit_of(cc.brow0, cc.brow1, c.m);
+                    if (!bonded) acc.clash += (double)KWALL_F * (inv12(dist) - inv12(cdist));
                 }
-                if (__ballot(v) == 0ull) continue;
-                f(v, j, x, y, z, r);
             }
         }
-        for (int base = 0; base < L; base += WAVE) {
-            const int m = base + l;
-            const bool v = m < L && s.lbox[m] == bi;
+    }
+    const unsigned long long mk = __ballot(in);
+    if (in) {
+        const int pos = acc.n_near + lanes_below(mk);
+        if (pos < cc.cap) {
+            cc.nidx[pos] = make_int2(c.j, __float_as_int(sq));
+            cc.nxyz[pos] = make_float4(c.x, c.y, c.z, c.r);
+        }
+    }
+    acc.n_near += __popcll(mk);
+    if (acc.n_near > cc.cap) acc.over = true;
+}
+
+// Segment table of one atom's stream (lanes 0..26 hold slot 0..26).
+struct SlotTab {
+    int bi;       // cell of the slot, -1 if outside the grid
+    int rs, re;   // receptor range of the cell in the cell list
+};
+
+__device__ __forceinline__ SlotTab slot_table(const Smem& s, const DevGrid& G, const PoseGrid& pg, int k)
+{
+    SlotTab t{-1, 0, 0};
+    const int l = lane_id();
+    if (l < 27) {
+        const int b = slot_box(pg, s.sbox[k], l);
+        if (b >= 0 && b < pg.dim3) t.bi = b;
+    }
+    if (t.bi >= 0 && !pg.slow) {
+        t.rs = G.cell_start[t.bi];
+        t.re = G.cell_start[t.bi + 1];
+    }
+    return t;
+}
+
+// receptor cells from the precomputed cell list
+__device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig, int Rn, int L, const SlotTab& tab,
+                          const CullCtx& cc, CullAcc& acc)
+{
+    const int l = lane_id();
+    int lo = 0, nl = 0;  // ligand atoms of the slot's cell: ranks [lo, lo + nl)
+    if (tab.bi >= 0)
+        for (int q = 0; q < L; ++q) {
+            const int b = s.sbox[q];
+            lo += b < tab.bi;
+            nl += b == tab.bi;
+        }
+    const int nr = (tab.bi >= 0) ? tab.re - tab.rs : 0;
+    const int seg = nr + nl;
+    const int off = wave_excl_scan(seg);
+    const int T = readlane_i(off, 63);  // lanes >= 27 hold empty segments
+    auto fetch = [&](int base) {
+        Cand c{0.f, 0.f, 0.f, 0.f, -1, 0, false};
+        const int t = base + l;
+        unsigned long long ov = __ballot(l < 27 && seg > 0 && off < base + WAVE && off + seg > base);
+        int so = 0, sn = 0, sr = 0, slo = 0;
+        while (ov) {
+            const int sidx = first_lane(ov);
+            ov &= ov - 1;
+            const int o_ = readlane_i(off, sidx), g_ = readlane_i(seg, sidx);
+            const int n_ = readlane_i(nr, sidx), r_ = readlane_i(tab.rs, sidx), lo_ = readlane_i(lo, sidx);
+            if (t >= o_ && t < o_ + g_) {
+                so = o_;
+                sn = n_;
+                sr = r_;
+                slo = lo_;
+            }
+        }
+        if (t < T) {
+            const int rr = t - so;
+            if (rr < sn) {
+                const int k = sr + rr;
+                const float4 p = G.cell_xyzr[k];
+                c.x = p.x;
+                c.y = p.y;
+                c.z = p.z;
+                c.r = p.w;
+                c.j = G.cell_atom[k];
+            } else {
+                const int m = s.order[slo + rr - sn];
+                c.m = m;
+                c.x = s.lx[m];
+                c.y = s.ly[m];
+                c.z = s.lz[m];
+                c.r = lig.radius[m];
+                c.j = Rn + m;
+                c.lj = true;
+            }
+        }
+        return c;
+    };
+    if (T <= 0) return;
+    Cand cur = fetch(0);
+    for (int base = 0; base < T; base += WAVE) {
+        Cand nxt{0.f, 0.f, 0.f, 0.f, -1, 0, false};
+        if (base + WAVE < T) nxt = fetch(base + WAVE);  // in flight while this chunk is tested
+        visit(cc, cur, acc);
+        cur = nxt;
+    }
+}
+
+// grid widened by this pose (rare): the receptor cells are not precomputed,
+// so every receptor atom is binned on the fly, slot by slot
+__device__ void cull_slow(const Smem& s, const DevReceptor& R, const DevLigand& lig, const PoseGrid& pg, int L,
+                          const SlotTab& tab, const CullCtx& cc, CullAcc& acc)
+{
+    const int l = lane_id();
+    const int Rn = R.n;
+    for (int slot = 0; slot < 27; ++slot) {
+        const int sb = readlane_i(tab.bi, slot);
+        if (sb < 0) continue;
+        for (int base = 0; base < Rn + L; base += WAVE) {
+            const int t = base + l;
+            Cand c{0.f, 0.f, 0.f, 0.f, -1, 0, false};
+            bool v = false;
+            if (t < Rn) {
+                const float4 p = R.xyzr[t];
+                v = box_of(pg, p.x, p.y, p.z) == sb;
+                if (v) {
+                    c.x = p.x;
+                    c.y = p.y;
+                    c.z = p.z;
+                    c.r = p.w;
+                    c.j = t;
+                }
+            } else if (t < Rn + L) {
+                const int m = t - Rn;
+                v = s.sbox[s.rank[m]] == sb;
+                if (v) {
+                    c.m = m;
+                    c.x = s.lx[m];
+                    c.y = s.ly[m];
+                    c.z = s.lz[m];
+                    c.r = lig.radius[m];
+                    c.j = t;
+                    c.lj = true;
+                }
+            }
             if (__ballot(v) == 0ull) continue;
-            const int mm = v ? m : 0;
-            f(v, R.n + m, s.lx[mm], s.ly[mm], s.lz[mm], 0.0f /* radius looked up by caller */);
+            visit(cc, c, acc);
         }
     }
 }
 
 // ------------------------------------------------------------ kernel
-__global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevGrid* __restrict__ grids,
-                                                        const DevLigand* __restrict__ ligs, DevParams prm, DevBatch B)
+__global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const DevGrid* __restrict__ grids,
+                                                           const DevLigand* __restrict__ ligs, DevParams prm,
+                                                           DevBatch B)
 {
     __shared__ Smem s;
     const int l = lane_id();
+    int2* const nidx = B.near_idx + (size_t)blockIdx.x * B.near_cap;
+    float4* const nxyz = B.near_xyz + (size_t)blockIdx.x * B.near_cap;
     for (;;) {
         // dynamic work queue: pose costs vary a lot (clash exits, hull sizes)
         int pose = 0;
@@ -517,14 +932,13 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
         const DevGrid G = grids[lig.grid_id];
         const int L = lig.n_atoms;
         const int Rn = R.n;
-        fa_cf_result res;
-        res.com = res.wal = res.sas = res.con = res.totsas = res.cf = res.app = 0.0;
-        res.rclash = 0;
-        res.status = FA_OK;
+        int status = FA_OK;
+        double r_com = 0.0, r_wal = 0.0, r_sas = 0.0, r_tot = 0.0;
+        int r_rclash = 0;
         int nrec_total = 0;
+        double clash = 0.0;
 
         if (l < FA_OPCOUNT_N) s.ops[l] = 0ull;
-        wave_sync();
         PhaseClock clk{B.prof, 0ull};
         clk.start();
         build_pose(s, lig, B.genes + (size_t)pose * B.gene_stride);
@@ -540,265 +954,367 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
         PoseGrid pg;
         pose_grid(s, lig, G, pg);
         if (pg.slow && l == 0) atomicAdd(&B.counters[0], 1u);
-        // ligand boxes, processing order (box, calc index)
-        bool range_ok = true;
-        for (int i = l; i < L; i += WAVE) {
-            const int b = box_of(pg, s.lx[i], s.ly[i], s.lz[i]);
-            s.lbox[i] = b;
-            if (b < 0 || b >= pg.dim3) range_ok = false;
+        // cells of the ligand atoms and the processing order (cell, calc index)
+        int lb[LSL];
+        bool bad = false;
+#pragma unroll
+        for (int q = 0; q < LSL; ++q) {
+            const int i = l + WAVE * q;
+            lb[q] = 0x7fffffff;
+            if (i < L) {
+                lb[q] = box_of(pg, s.lx[i], s.ly[i], s.lz[i]);
+                if (lb[q] < 0 || lb[q] >= pg.dim3) bad = true;
+            }
         }
-        range_ok = __ballot(!range_ok) == 0ull;
-        wave_sync();
-        if (!range_ok) {
-            res.status = FA_GRID_RANGE;
+        if (__ballot(bad) != 0ull) {
+            status = FA_GRID_RANGE;
         } else {
-            for (int i = l; i < L; i += WAVE) {
-                const int bi = s.lbox[i];
-                int rank = 0;
-                for (int k = 0; k < L; ++k) {
-                    const int bk = s.lbox[k];
-                    rank += (bk < bi) || (bk == bi && k < i);
-                }
-                s.order[rank] = (unsigned char)i;
-            }
-            wave_sync();
-
-            clk.stop(PH_BUILD);
-            // ---- clash pre-pass (LIB/Vcontacts.cpp:36-51, :1803-1821): every
-            // receptor contact and every ligand pair once (the earlier atom in
-            // box order counts it), non-bonded, d < permeability * (rA + rB)
-            double clash = 0.0;
-            const double permea = (double)prm.permeability;
-            for (int i = l; i < L; i += WAVE) s.done[i] = 0;
-            wave_sync();
+            int rk[LSL] = {0, 0};
             for (int k = 0; k < L; ++k) {
-                const int a = s.order[k];
-                const float ax = s.lx[a], ay = s.ly[a], az = s.lz[a];
-                const float ra = lig.radius[a];
-                const float rado = ra + RW;
-                int found = 0, tested = 0;
-                scan_boxes(s, R, G, pg, L, s.lbox[a], [&](bool v, int j, float x, float y, float z, float r) {
-                    if (!v) return;
-                    const bool lj = j >= Rn;
-                    const int m = j - Rn;
-                    if (lj) {
-                        if (s.done[m]) return;  // an earlier atom already counted the pair
-                        r = lig.radius[m];
-                    }
-                    tested += (j != Rn + a);  // the atom itself never makes a contact
-                    const float dx = ax - x, dy = ay - y, dz = az - z;
-                    const double sq = (double)(dx * dx + dy * dy + dz * dz);
-                    const double nd = (double)(rado + r + RW);
-                    if (!(sq < nd * nd && sq != 0.0)) return;
-                    ++found;
-                    const double dist = sqrt(sq);
-                    const double rAB = (double)(ra + r);
-                    const double cd = permea * rAB;
-                    if (dist < cd) {
-                        bool bonded = false;
-                        if (lj) bonded = (lig.bonded[2 * a + (m >> 6)] >> (m & 63)) & 1ull;
-                        if (!lj || !bonded) clash += (double)KWALL_F * (pow(dist, -12.0) - pow(cd, -12.0));
-                    }
-                });
-                found = wave_sum_i(found);
-                count_op(s.ops, OP_CULL, wave_sum_i(tested));
-                if (l == 0 && found > 0) s.done[a] = 1;  // marked done once it has a contact
-                wave_sync();
+                const int bk = readlane_i(k < WAVE ? lb[0] : lb[1], k & 63);
+#pragma unroll
+                for (int q = 0; q < LSL; ++q) {
+                    const int i = l + WAVE * q;
+                    rk[q] += (bk < lb[q]) || (bk == lb[q] && k < i);
+                }
             }
-            clash = wave_sum_d(clash);
-            if (clash >= CLASH_THRESHOLD) {
-                res.wal = clash;
-                res.cf = clash + 1.0;
-                res.app = clash + 1.0;
-                res.status = FA_CLASH;
+#pragma unroll
+            for (int q = 0; q < LSL; ++q) {
+                const int i = l + WAVE * q;
+                if (i < L) {
+                    s.order[rk[q]] = (unsigned char)i;
+                    s.rank[i] = (unsigned char)rk[q];
+                    s.sbox[rk[q]] = lb[q];
+                }
             }
         }
+        wave_sync();
+        clk.stop(PH_BUILD);
 
+        // ---- neighbour lists + clash pre-pass (LIB/Vcontacts.cpp:36-51, :1725-1839):
+        // every receptor contact and every ligand pair once, non-bonded,
+        // d < permeability * (rA + rB)
+        int totq[LSL] = {0, 0};  // candidates of rank k (self excluded): lane k & 63, slot k >> 6
+        if (status == FA_OK) {
+            unsigned long long dpre0 = 0ull, dpre1 = 0ull;
+            long long pre_tested = 0;
+            CullAcc acc{0, 0, 0, 0, 0.0, false};
+            SlotTab tab = slot_table(s, G, pg, 0);
+            for (int k = 0; k < L; ++k) {
+                SlotTab next{-1, 0, 0};
+                if (k + 1 < L) next = slot_table(s, G, pg, k + 1);  // in flight during this atom
+                const int a = s.order[k];
+                CullCtx cc;
+                cc.a = a;
+                cc.acalc = Rn + a;
+                cc.ax = s.lx[a];
+                cc.ay = s.ly[a];
+                cc.az = s.lz[a];
+                cc.ra = lig.radius[a];
+                cc.rado = cc.ra + RW;
+                cc.brow0 = lig.bonded[2 * a];
+                cc.brow1 = lig.bonded[2 * a + 1];
+                cc.dpre0 = dpre0;
+                cc.dpre1 = dpre1;
+                cc.permea = (double)prm.permeability;
+                cc.nidx = nidx;
+                cc.nxyz = nxyz;
+                cc.cap = B.near_cap;
+                if (l == 0) s.nstart[k] = acc.n_near;
+                acc.total = acc.tested = acc.found = 0;
+                if (pg.slow)
+                    cull_slow(s, R, lig, pg, L, tab, cc, acc);
+                else
+                    cull_fast(s, G, lig, Rn, L, tab, cc, acc);
+                const int total = wave_sum_i(acc.total);
+                const int tested = wave_sum_i(acc.tested);
+                const int found = wave_sum_i(acc.found);
+                pre_tested += tested;
+                if (found > 0) {  // marked once it has a contact
+                    if (a < WAVE)
+                        dpre0 |= 1ull << a;
+                    else
+                        dpre1 |= 1ull << (a - WAVE);
+                }
+                if (l == (k & 63)) {
+                    if (k < WAVE)
+                        totq[0] = total;
+                    else
+                        totq[1] = total;
+                }
+                tab = next;
+            }
+            if (l == 0) s.nstart[L] = acc.n_near;
+            count_op(s.ops, OP_CULL, pre_tested);
+            clash = wave_sum_d(acc.clash);
+            if (acc.over)
+                status = FA_CAPACITY;
+            else if (clash >= CLASH_THRESHOLD)
+                status = FA_CLASH;
+        }
+        wave_sync();
         clk.stop(PH_PREPASS);
-        if (res.status == FA_OK) {
+
+        if (status == FA_OK) {
             // ---- contact polyhedra in box order (calc_region, LIB/Vcontacts.cpp:74-139)
             for (int i = l; i < L; i += WAVE) {
-                s.done[i] = 0;
                 s.seed[3 * i] = -1;
                 s.stub[2 * i] = 0ull;
                 s.stub[2 * i + 1] = 0ull;
-                s.acc_com[i] = 0.0;
-                s.acc_wal[i] = 0.0;
-                s.acc_sas[i] = 0.0;
-                s.acc_tot[i] = 0.0;
-                s.acc_rcl[i] = 0;
             }
             wave_sync();
-            for (int k = 0; k < L && res.status == FA_OK; ++k) {
+            unsigned long long dm0 = 0ull, dm1 = 0ull;  // atoms done in this pass
+            const double permea = (double)prm.permeability;
+            for (int k = 0; k < L && status == FA_OK; ++k) {
                 const int a = s.order[k];
                 const int acalc = Rn + a;
                 const float ax = s.lx[a], ay = s.ly[a], az = s.lz[a];
                 const float ra = lig.radius[a];
                 const float rado = ra + RW;
+                const double ro = (double)rado;
                 clk.stop(PH_REC);
-                // -- contact list (get_contlist4, LIB/Vcontacts.cpp:1725-1839)
-                int NC = 0, tested = 0;
-                bool overflow = false;
-                scan_boxes(s, R, G, pg, L, s.lbox[a], [&](bool v, int j, float x, float y, float z, float r) {
-                    bool in = false;
-                    double dist = 0.0;
-                    if (v) {
-                        const bool lj = j >= Rn;
-                        const int m = j - Rn;
-                        bool skip = false;
-                        if (lj) {
-                            r = lig.radius[m];
-                            // done atoms are skipped unless they recorded a contact with a
-                            skip = s.done[m] && !((s.stub[2 * a + (m >> 6)] >> (m & 63)) & 1ull);
-                        }
-                        if (!skip) {
-                            tested += (j != acalc);
-                            const float dx = ax - x, dy = ay - y, dz = az - z;
-                            const double sq = (double)(dx * dx + dy * dy + dz * dz);
-                            const double nd = (double)(rado + r + RW);
-                            if (sq < nd * nd && sq != 0.0) {
-                                in = true;
-                                dist = sqrt(sq);
-                            }
-                        }
+                // -- contact list (get_contlist4, LIB/Vcontacts.cpp:1725-1839): the
+                // neighbour list without done atoms, except those that recorded
+                // a contact with `a` (the 'C' marks)
+                const unsigned long long sk0 = dm0 & ~s.stub[2 * a];
+                const unsigned long long sk1 = dm1 & ~s.stub[2 * a + 1];
+                const int ns = s.nstart[k], ne = s.nstart[k + 1];
+                int NC = 0;
+                for (int base = ns; base < ne; base += WAVE) {
+                    const int q = base + l;
+                    bool keep = false;
+                    int j = -1;
+                    float sq = 0.0f;
+                    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (q < ne) {
+                        const int2 e = nidx[q];
+                        p = nxyz[q];
+                        j = e.x;
+                        sq = __int_as_float(e.y);
+                        keep = (j < Rn) || !bit_of(sk0, sk1, j - Rn);
                     }
-                    const unsigned long long m = __ballot(in);
-                    if (in) {
-                        const int pos = NC + lanes_below(m);
+                    const unsigned long long mk = __ballot(keep);
+                    if (keep) {
+                        const int pos = NC + lanes_below(mk);
                         if (pos < MAXC) {
-                            s.pidx[pos] = j;
-                            s.cdist[pos] = dist;
-                            // plane (voronoi_poly2, LIB/Vcontacts.cpp:181-220), 'X' extended radical
-                            const float rj = RW + r;
+                            const double dist = sqrt((double)sq);
+                            // plane (voronoi_poly2, LIB/Vcontacts.cpp:181-220), 'X' until the hull touches it
+                            const float rj = RW + p.w;
                             const double pd = (dist * dist + (double)(rado * rado) - (double)(rj * rj)) / (2 * dist);
-                            s.A0[pos] = (double)(x - ax) / dist;
-                            s.A1[pos] = (double)(y - ay) / dist;
-                            s.A2[pos] = (double)(z - az) / dist;
+                            s.A0[pos] = (double)(p.x - ax) / dist;
+                            s.A1[pos] = (double)(p.y - ay) / dist;
+                            s.A2[pos] = (double)(p.z - az) / dist;
                             s.A3[pos] = -pd;
-                            s.pdist[pos] = fabs(pd);
                             s.pflag[pos] = 'X';
+                            s.u.c.cd[pos] = dist;
+                            s.u.c.rb[pos] = p.w;
+                            s.u.c.idx[pos] = j;
                         }
                     }
-                    NC += __popcll(m);
-                });
-                count_op(s.ops, OP_CULL, wave_sum_i(tested));
-                if (NC > MAXC) overflow = true;
-                if (overflow) {
-                    res.status = FA_CAPACITY;
+                    NC += __popcll(mk);
+                }
+                // cull count of this pass: every candidate but the atom itself and
+                // the done atoms it skips (all of them in the 27 cells)
+                {
+                    int sk = 0;
+#pragma unroll
+                    for (int q = 0; q < LSL; ++q) {
+                        const int m = l + WAVE * q;
+                        if (m < L && bit_of(sk0, sk1, m)) sk += slot_multiplicity(pg, s.sbox[s.rank[m]] - s.sbox[k]);
+                    }
+                    const int tot_k = readlane_i(k < WAVE ? totq[0] : totq[1], k & 63);
+                    count_op(s.ops, OP_CULL, tot_k - wave_sum_i(sk));
+                }
+                if (NC > MAXC) {
+                    status = FA_CAPACITY;
                     break;
                 }
-                if (NC == 0) continue;  // SAS = -1, atom not marked done
+                if (NC == 0) continue;  // no contact: SAS stays whole, atom not done
                 count_op(s.ops, OP_PLANE, NC);
-                // bounding planes (LIB/Vcontacts.cpp:223-241)
                 if (l < 4) {
-                    const double bp[4][4] = {{0.707, 1.0, 0.0, -10.0}, {0.707, -1.0, 0.0, -10.0},
-                                             {-0.707, 0.0, 1.0, -10.0}, {-0.707, 0.0, -1.0, -10.0}};
-                    s.A0[NC + l] = bp[l][0];
-                    s.A1[NC + l] = bp[l][1];
-                    s.A2[NC + l] = bp[l][2];
-                    s.A3[NC + l] = bp[l][3];
-                    s.pdist[NC + l] = 0.0;
-                    s.pidx[NC + l] = -1;
+                    double bp[4];
+                    bounding_plane(l, bp);
+                    s.A0[NC + l] = bp[0];
+                    s.A1[NC + l] = bp[1];
+                    s.A2[NC + l] = bp[2];
+                    s.A3[NC + l] = bp[3];
                     s.pflag[NC + l] = 'X';
                 }
                 wave_sync();
-
+                // planes into registers: lane l holds contacts l and l + 64
+                double pa0[PSL], pa1[PSL], pa2[PSL], pa3[PSL], pcd[PSL];
+                float prb[PSL];
+                int pix[PSL];
+#pragma unroll
+                for (int q = 0; q < PSL; ++q) {
+                    const int c = l + WAVE * q;
+                    pa0[q] = pa1[q] = pa2[q] = pa3[q] = pcd[q] = 0.0;
+                    prb[q] = 0.0f;
+                    pix[q] = -1;
+                    if (c < NC) {
+                        pa0[q] = s.A0[c];
+                        pa1[q] = s.A1[c];
+                        pa2[q] = s.A2[c];
+                        pa3[q] = s.A3[c];
+                        pcd[q] = s.u.c.cd[c];
+                        prb[q] = s.u.c.rb[c];
+                        pix[q] = s.u.c.idx[c];
+                    }
+                }
+                static_assert(PSL == 2, "two plane slots per lane");
+                // calc index of contact c (wave-uniform c < NC)
+                auto atom_of = [&](int c) { return readlane_i(c < WAVE ? pix[0] : pix[1], c & 63); };
                 clk.stop(PH_CULL);
+
                 // -- first vertex (get_firstvert, LIB/Vcontacts.cpp:499-590)
                 int pA = -1, pB = -1, pC = -1;
                 {
                     const int s0 = s.seed[3 * a];
                     if (s0 != -1) {
                         const int s1 = s.seed[3 * a + 1], s2 = s.seed[3 * a + 2];
-                        int fa = -1, fb = -1, fc = -1;
-                        for (int base = 0; base < NC; base += WAVE) {
-                            const int c = base + l;
-                            const int idx = c < NC ? s.pidx[c] : -2;
-                            unsigned long long m0 = __ballot(c < NC && idx == s0);
-                            unsigned long long m1 = __ballot(c < NC && idx != s0 && idx == s1);
-                            unsigned long long m2 = __ballot(c < NC && idx != s0 && idx != s1 && idx == s2);
-                            if (m0) fa = base + 63 - __clzll((long long)m0);
-                            if (m1) fb = base + 63 - __clzll((long long)m1);
-                            if (m2) fc = base + 63 - __clzll((long long)m2);
+#pragma unroll
+                        for (int q = 0; q < PSL; ++q) {
+                            const int c = l + WAVE * q;
+                            const bool v = c < NC;
+                            const unsigned long long m0 = __ballot(v && pix[q] == s0);
+                            const unsigned long long m1 = __ballot(v && pix[q] != s0 && pix[q] == s1);
+                            const unsigned long long m2 = __ballot(v && pix[q] != s0 && pix[q] != s1 && pix[q] == s2);
+                            // the reference keeps the last match
+                            if (m0) pA = WAVE * q + 63 - __clzll((long long)m0);
+                            if (m1) pB = WAVE * q + 63 - __clzll((long long)m1);
+                            if (m2) pC = WAVE * q + 63 - __clzll((long long)m2);
                         }
-                        pA = fa;
-                        pB = fb;
-                        pC = fc;
                     }
                     if (!(pA != -1 && pB != -1 && pC != -1)) {
-                        double bv = BIG;
-                        int bi = 0x7fffffff;
-                        for (int c = l; c < NC; c += WAVE) {
-                            if (s.pdist[c] < bv) {
-                                bv = s.pdist[c];
-                                bi = c;
-                            }
+                        // a partial seed match is discarded: all three are searched again
+                        pA = pB = pC = -1;
+                        // A: the plane closest to the atom (first of equals)
+                        uint64_t kb = ~0ull;
+#pragma unroll
+                        for (int q = 0; q < PSL; ++q) {
+                            const int c = l + WAVE * q;
+                            const double d = fabs(pa3[q]);
+                            if (c < NC && d < BIG) kb = umin64(kb, key_of(d));
                         }
-                        wave_argmin(bv, bi);
-                        pA = bi;
-                        double eA[4];
-                        plane_of(s, pA, eA);
-                        bv = BIG;
-                        bi = 0x7fffffff;
-                        double ptx = 0.0, pty = 0.0, ptz = 0.0;
-                        for (int c = l; c < NC + 4; c += WAVE) {
-                            if (c == pA) continue;
-                            double eC[4], vA[4], tp[3];
-                            plane_of(s, c, eC);
-                            vA[0] = eA[1] * eC[2] - eA[2] * eC[1];
-                            vA[1] = eA[2] * eC[0] - eA[0] * eC[2];
-                            vA[2] = eA[0] * eC[1] - eA[1] * eC[0];
-                            vA[3] = 0.0;
-                            if (solve_3x3(eA, eC, vA, tp) != -1) {
-                                double d = sqrt(tp[0] * tp[0] + tp[1] * tp[1] + tp[2] * tp[2]);
-                                if (d < bv) {
-                                    bv = d;
-                                    bi = c;
-                                    ptx = tp[0];
-                                    pty = tp[1];
-                                    ptz = tp[2];
+                        const uint64_t kA = wave_min_u64(kb);
+#pragma unroll
+                        for (int q = 0; q < PSL; ++q) {
+                            const int c = l + WAVE * q;
+                            const unsigned long long m = __ballot(c < NC && kA != ~0ull && key_of(fabs(pa3[q])) == kA);
+                            if (pA < 0 && m) pA = WAVE * q + first_lane(m);
+                        }
+                        if (pA >= 0) {
+                            double eA[4];
+                            plane_of(s, pA, eA);
+                            // B: the plane whose line with A passes closest to the atom;
+                            // slot PSL holds the bounding planes (lanes 0..3)
+                            double dq[PSL + 1], tx[PSL + 1], ty[PSL + 1], tz[PSL + 1];
+                            uint64_t kbb = ~0ull;
+#pragma unroll
+                            for (int q = 0; q <= PSL; ++q) {
+                                const int c = (q < PSL) ? l + WAVE * q : NC + l;
+                                const bool ok = (q < PSL) ? (c < NC) : (l < 4);
+                                dq[q] = BIG;
+                                tx[q] = ty[q] = tz[q] = 0.0;
+                                if (ok && c != pA) {
+                                    double eC[4], vA[4], tp[3];
+                                    if (q < PSL) {
+                                        eC[0] = pa0[q];
+                                        eC[1] = pa1[q];
+                                        eC[2] = pa2[q];
+                                        eC[3] = pa3[q];
+                                    } else {
+                                        bounding_plane(l, eC);
+                                    }
+                                    vA[0] = eA[1] * eC[2] - eA[2] * eC[1];
+                                    vA[1] = eA[2] * eC[0] - eA[0] * eC[2];
+                                    vA[2] = eA[0] * eC[1] - eA[1] * eC[0];
+                                    vA[3] = 0.0;
+                                    if (solve_3x3(eA, eC, vA, tp) != -1) {
+                                        const double d = sqrt(tp[0] * tp[0] + tp[1] * tp[1] + tp[2] * tp[2]);
+                                        if (d < BIG) {
+                                            dq[q] = d;
+                                            tx[q] = tp[0];
+                                            ty[q] = tp[1];
+                                            tz[q] = tp[2];
+                                            kbb = umin64(kbb, key_of(d));
+                                        }
+                                    }
+                                }
+                            }
+                            const uint64_t kB = wave_min_u64(kbb);
+                            double ptx = 0.0, pty = 0.0, ptz = 0.0;
+#pragma unroll
+                            for (int q = 0; q <= PSL; ++q) {
+                                const unsigned long long m = __ballot(kB != ~0ull && dq[q] < BIG && key_of(dq[q]) == kB);
+                                if (pB < 0 && m) {
+                                    const int ln = first_lane(m);
+                                    pB = (q < PSL) ? WAVE * q + ln : NC + ln;
+                                    ptx = readlane_d(tx[q], ln);
+                                    pty = readlane_d(ty[q], ln);
+                                    ptz = readlane_d(tz[q], ln);
+                                }
+                            }
+                            if (pB >= 0) {
+                                double eB[4];
+                                plane_of(s, pB, eB);
+                                const double vA0 = eA[1] * eB[2] - eA[2] * eB[1];
+                                const double vA1 = eA[2] * eB[0] - eA[0] * eB[2];
+                                const double vA2 = eA[0] * eB[1] - eA[1] * eB[0];
+                                // C: the plane cutting the line A x B closest to that point
+                                double fq[PSL + 1];
+                                uint64_t kcc = ~0ull;
+#pragma unroll
+                                for (int q = 0; q <= PSL; ++q) {
+                                    const int c = (q < PSL) ? l + WAVE * q : NC + l;
+                                    const bool ok = (q < PSL) ? (c < NC) : (l < 4);
+                                    fq[q] = BIG;
+                                    if (ok && c != pA && c != pB) {
+                                        double eC[4];
+                                        if (q < PSL) {
+                                            eC[0] = pa0[q];
+                                            eC[1] = pa1[q];
+                                            eC[2] = pa2[q];
+                                            eC[3] = pa3[q];
+                                        } else {
+                                            bounding_plane(l, eC);
+                                        }
+                                        const double vtdiv = eC[0] * vA0 + eC[1] * vA1 + eC[2] * vA2;
+                                        if (vtdiv != 0.0) {
+                                            const double vt = -(eC[0] * ptx + eC[1] * pty + eC[2] * ptz + eC[3]) / vtdiv;
+                                            const double av = fabs(vt);
+                                            if (av < BIG) {
+                                                fq[q] = av;
+                                                kcc = umin64(kcc, key_of(av));
+                                            }
+                                        }
+                                    }
+                                }
+                                const uint64_t kC = wave_min_u64(kcc);
+#pragma unroll
+                                for (int q = 0; q <= PSL; ++q) {
+                                    const unsigned long long m = __ballot(kC != ~0ull && fq[q] < BIG && key_of(fq[q]) == kC);
+                                    if (pC < 0 && m) pC = (q < PSL) ? WAVE * q + first_lane(m) : NC + first_lane(m);
                                 }
                             }
                         }
-                        {
-                            double v0 = bv;
-                            int i0 = bi;
-                            wave_argmin(v0, i0);
-                            const int src = first_lane(__ballot(bi == i0 && bv == v0));
-                            ptx = __shfl(ptx, src);
-                            pty = __shfl(pty, src);
-                            ptz = __shfl(ptz, src);
-                            pB = (i0 == 0x7fffffff) ? -1 : i0;
-                            if (pB < 0) { ptx = pty = ptz = 0.0; }
-                        }
-                        double eB[4];
-                        plane_of(s, pB < 0 ? 0 : pB, eB);
-                        double vA0 = eA[1] * eB[2] - eA[2] * eB[1];
-                        double vA1 = eA[2] * eB[0] - eA[0] * eB[2];
-                        double vA2 = eA[0] * eB[1] - eA[1] * eB[0];
-                        bv = BIG;
-                        bi = 0x7fffffff;
-                        for (int c = l; c < NC + 4; c += WAVE) {
-                            if (c == pA || c == pB) continue;
-                            double vtdiv = s.A0[c] * vA0 + s.A1[c] * vA1 + s.A2[c] * vA2;
-                            if (vtdiv != 0.0) {
-                                double vt = -(s.A0[c] * ptx + s.A1[c] * pty + s.A2[c] * ptz + s.A3[c]) / vtdiv;
-                                if (fabs(vt) < bv) {
-                                    bv = fabs(vt);
-                                    bi = c;
-                                }
-                            }
-                        }
-                        wave_argmin(bv, bi);
-                        pC = (bi == 0x7fffffff) ? -1 : bi;
                     }
                 }
                 if (pA < 0 || pB < 0 || pC < 0) {
-                    res.status = FA_CAPACITY;
+                    status = FA_CAPACITY;
                     break;
                 }
                 clk.stop(PH_FIRST);
+
                 // -- hull walk (LIB/Vcontacts.cpp:243-410)
+                Edges E;
+#pragma unroll
+                for (int q = 0; q < ESL; ++q) {
+                    E.pl[q] = 0xFFFFFFFFu;
+                    E.vt[q] = 0xFFFFFFFFu;
+                    E.open[q] = 0ull;
+                }
+                int vn = 1, edgenum = 3;
                 {
                     double eA[4], eB[4], eC[4], tp[3];
                     plane_of(s, pA, eA);
@@ -811,48 +1327,71 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                         s.pflag[pB] = 'Y';
                         s.pflag[pC] = 'Y';
                     }
-                    wave_sync();
-                    add_vedge(s, 0, pA, pB, pC, 0);
-                    add_vedge(s, 1, pB, pC, pA, 0);
-                    add_vedge(s, 2, pC, pA, pB, 0);
-                    wave_sync();
+                    edge_new(E, 0, pA, pB, pC, 0);
+                    edge_new(E, 1, pB, pC, pA, 0);
+                    edge_new(E, 2, pC, pA, pB, 0);
                 }
-                int startedge = 0, edgenum = 3, vn = 1;
                 bool hull_fail = false, cap = false;
                 const double rr = (double)(rado * rado);
+                long long hull_tests = 0;
                 while (true) {
-                    while (s.ee[startedge] >= 0 && (edgenum - startedge) > 0) ++startedge;
-                    if ((edgenum - startedge) <= 0) break;
-                    const int sp = s.es[startedge];
-                    const double sx = s.vx[sp], sy = s.vy[sp], sz = s.vz[sp];
-                    const double V0 = s.eV0[startedge], V1 = s.eV1[startedge], V2 = s.eV2[startedge];
-                    const int A = s.ep0[startedge], Bp = s.ep1[startedge], oldC = s.esp[startedge];
-                    double bv = BIG;
-                    int bi = 0x7fffffff;
-                    for (int c = l; c < NC; c += WAVE) {
-                        if (c != A && c != Bp && c != oldC) {
-                            double vtdiv = s.A0[c] * V0 + s.A1[c] * V1 + s.A2[c] * V2;
-                            if (vtdiv != 0.0) {
-                                double vt = -(s.A0[c] * sx + s.A1[c] * sy + s.A2[c] * sz + s.A3[c]) / vtdiv;
-                                if (vt < bv && vt > 0.0) {
-                                    bv = vt;
-                                    bi = c;
+                    const int se = first_open(E);
+                    if (se < 0) break;
+                    unsigned epl, evt;
+                    edge_get(E, se, epl, evt);
+                    const int A = (int)(epl & NONE), Bp = (int)((epl >> 8) & NONE);
+                    const int oldC = (int)((epl >> 16) & NONE);
+                    const int sv = (int)(evt & 0xFFFFu);
+                    struct {
+                        double dx, dy, dz, px, py, pz;
+                    } ev;
+                    ev.px = s.vx[sv];
+                    ev.py = s.vy[sv];
+                    ev.pz = s.vz[sv];
+                    edge_dir(s, A, Bp, oldC, ev.px, ev.py, ev.pz, ev.dx, ev.dy, ev.dz);
+                    // closest plane cut ahead along the edge
+                    double vq[PSL];
+                    uint64_t kb = ~0ull;
+#pragma unroll
+                    for (int q = 0; q < PSL; ++q) {
+                        vq[q] = BIG;
+                        if (q * WAVE < NC) {  // wave-uniform: skip empty slots
+                            const int c = l + WAVE * q;
+                            if (c < NC && c != A && c != Bp && c != oldC) {
+                                const double vtdiv = pa0[q] * ev.dx + pa1[q] * ev.dy + pa2[q] * ev.dz;
+                                if (vtdiv != 0.0) {
+                                    const double vt =
+                                        -(pa0[q] * ev.px + pa1[q] * ev.py + pa2[q] * ev.pz + pa3[q]) / vtdiv;
+                                    if (vt < BIG && vt > 0.0) {
+                                        vq[q] = vt;
+                                        kb = umin64(kb, key_of(vt));
+                                    }
                                 }
                             }
                         }
                     }
-                    wave_argmin(bv, bi);
-                    count_op(s.ops, OP_HULL, NC - (A < NC) - (Bp < NC) - (oldC < NC));
-                    double vtmin = bv;
-                    int C = (bi == 0x7fffffff) ? -1 : bi;
-                    double nx = sx + vtmin * V0, ny = sy + vtmin * V1, nz = sz + vtmin * V2;
+                    hull_tests += NC - (A < NC) - (Bp < NC) - (oldC < NC);
+                    const uint64_t kmin = wave_min_u64(kb);
+                    int C = -1;
+                    double vtmin = BIG;
+                    if (kmin != ~0ull) {
+                        vtmin = __longlong_as_double((long long)kmin);
+#pragma unroll
+                        for (int q = 0; q < PSL; ++q) {
+                            const unsigned long long m = __ballot(vq[q] < BIG && key_of(vq[q]) == kmin);
+                            if (C < 0 && m) C = WAVE * q + first_lane(m);
+                        }
+                    }
+                    double nx = ev.px + vtmin * ev.dx, ny = ev.py + vtmin * ev.dy, nz = ev.pz + vtmin * ev.dz;
                     if (nx * nx + ny * ny + nz * nz > rr) {
-                        count_op(s.ops, OP_HULL, 4 - (A >= NC) - (Bp >= NC) - (oldC >= NC));
+                        hull_tests += 4 - (A >= NC) - (Bp >= NC) - (oldC >= NC);
                         for (int c = NC; c < NC + 4; ++c) {
                             if (c != A && c != Bp && c != oldC) {
-                                double vtdiv = s.A0[c] * V0 + s.A1[c] * V1 + s.A2[c] * V2;
+                                double b[4];
+                                bounding_plane(c - NC, b);
+                                const double vtdiv = b[0] * ev.dx + b[1] * ev.dy + b[2] * ev.dz;
                                 if (vtdiv != 0.0) {
-                                    double vt = -(s.A0[c] * sx + s.A1[c] * sy + s.A2[c] * sz + s.A3[c]) / vtdiv;
+                                    const double vt = -(b[0] * ev.px + b[1] * ev.py + b[2] * ev.pz + b[3]) / vtdiv;
                                     if (vt < vtmin && vt > 0.0) {
                                         vtmin = vt;
                                         C = c;
@@ -860,151 +1399,123 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                                 }
                             }
                         }
-                        nx = sx + vtmin * V0;
-                        ny = sy + vtmin * V1;
-                        nz = sz + vtmin * V2;
+                        nx = ev.px + vtmin * ev.dx;
+                        ny = ev.py + vtmin * ev.dy;
+                        nz = ev.pz + vtmin * ev.dz;
                     }
-                    if (C < 0 || vn + 1 >= MAXV || edgenum + 2 >= MAXE) {
+                    if (C < 0 || vn + 1 >= MAXV) {
                         cap = true;
                         break;
                     }
                     put_vertex(s, vn, nx, ny, nz, A, Bp, C);
-                    if (l == 0) {
-                        s.ee[startedge] = (short)vn;
-                        s.eep[startedge] = (short)C;
-                        s.pflag[C] = 'Y';
-                    }
-                    wave_sync();
-                    // edges (A, C) then (B, C)
+                    if (l == 0) s.pflag[C] = 'Y';
+                    edge_close(E, se, C, vn);
+                    // edges (A, C) then (B, C) among the later edges
                     for (int pass = 0; pass < 2; ++pass) {
                         const int P = pass ? Bp : A;
                         const int other = pass ? A : Bp;
-                        int found = -1;
-                        for (int base = startedge + 1; base < edgenum && found < 0; base += WAVE) {
-                            const int e = base + l;
-                            bool hit = false;
-                            if (e < edgenum) {
-                                const int q0 = s.ep0[e], q1 = s.ep1[e];
-                                hit = (q0 == P && q1 == C) || (q0 == C && q1 == P);
-                            }
-                            const unsigned long long m = __ballot(hit);
-                            if (m) found = base + first_lane(m);
-                        }
-                        if (found >= 0) {
-                            if (l == 0) {
-                                s.ee[found] = (short)vn;
-                                s.eep[found] = (short)other;
-                            }
-                            wave_sync();
+                        const int f = find_edge(E, se, edgenum, P, C);
+                        if (f >= 0) {
+                            edge_close(E, f, other, vn);
                         } else {
-                            add_vedge(s, edgenum, P, C, other, vn);
+                            edge_new(E, edgenum, P, C, other, vn);
                             ++edgenum;
-                            wave_sync();
-                            if (pass == 1 && edgenum >= EDGE_LIMIT) {
-                                hull_fail = true;
-                                break;
-                            }
+                            if (pass == 1 && edgenum >= EDGE_LIMIT) hull_fail = true;
                         }
                     }
                     if (hull_fail) break;
                     ++vn;
                 }
+                count_op(s.ops, OP_HULL, hull_tests);
                 if (cap) {
-                    res.status = FA_CAPACITY;
+                    status = FA_CAPACITY;
                     break;
                 }
                 if (hull_fail) {
-                    res.status = FA_HULL_FAIL;
-                    res.wal = (double)POLY_PENALTY;
-                    res.cf = res.wal + 1.0;
-                    res.app = res.wal + 1.0;
+                    status = FA_HULL_FAIL;
                     break;
                 }
+                wave_sync();
                 clk.stop(PH_HULL);
+
                 // -- arc points (LIB/Vcontacts.cpp:419-457), in edge order
-                for (int base = 0; base < edgenum; base += WAVE) {
-                    const int e = base + l;
-                    int cnt = 0;
-                    double q0[3] = {0, 0, 0}, q1[3] = {0, 0, 0};
-                    bool k0 = false, k1 = false;
-                    int e0 = 0, e1 = 0, solved = 0;
-                    if (e < edgenum) {
-                        const int st = s.es[e], en = s.ee[e];
-                        const double rd = (double)rado;
-                        if (rd < s.vd[st] || rd < s.vd[en]) {
-                            solved = 1;
-                            double a0[4], a1[4];
-                            e0 = s.ep0[e];
-                            e1 = s.ep1[e];
-                            plane_of(s, e0, a0);
-                            plane_of(s, e1, a1);
-                            if (solve_2xS(a0, a1, rado, q0, q1) != -1) {
-                                const int ta = s.esp[e], tb = s.eep[e];
-                                double va = s.A0[ta] * q0[0] + s.A1[ta] * q0[1] + s.A2[ta] * q0[2] + s.A3[ta];
-                                double vb = s.A0[tb] * q0[0] + s.A1[tb] * q0[1] + s.A2[tb] * q0[2] + s.A3[tb];
-                                k0 = (va < 0.0 && vb < 0.0);
-                                va = s.A0[ta] * q1[0] + s.A1[ta] * q1[1] + s.A2[ta] * q1[2] + s.A3[ta];
-                                vb = s.A0[tb] * q1[0] + s.A1[tb] * q1[1] + s.A2[tb] * q1[2] + s.A3[tb];
-                                k1 = (va < 0.0 && vb < 0.0);
+                {
+                    int nsolved = 0;
+#pragma unroll
+                    for (int q = 0; q < ESL; ++q) {
+                        if (q * WAVE >= edgenum) break;
+                        const int e = q * WAVE + l;
+                        double q0[3] = {0, 0, 0}, q1[3] = {0, 0, 0};
+                        bool k0 = false, k1 = false;
+                        const int e0 = (int)(E.pl[q] & NONE), e1 = (int)((E.pl[q] >> 8) & NONE);
+                        if (e < edgenum) {
+                            const int st = (int)(E.vt[q] & 0xFFFFu), en = (int)(E.vt[q] >> 16);
+                            if (ro < vnorm(s, st) || ro < vnorm(s, en)) {
+                                ++nsolved;
+                                double a0[4], a1[4];
+                                plane_of(s, e0, a0);
+                                plane_of(s, e1, a1);
+                                if (solve_2xS(a0, a1, rado, q0, q1) != -1) {
+                                    const int ta = (int)((E.pl[q] >> 16) & NONE), tb = (int)(E.pl[q] >> 24);
+                                    double pa[4], pb[4];
+                                    plane_of(s, ta, pa);
+                                    plane_of(s, tb, pb);
+                                    double va = pa[0] * q0[0] + pa[1] * q0[1] + pa[2] * q0[2] + pa[3];
+                                    double vb = pb[0] * q0[0] + pb[1] * q0[1] + pb[2] * q0[2] + pb[3];
+                                    k0 = (va < 0.0 && vb < 0.0);
+                                    va = pa[0] * q1[0] + pa[1] * q1[1] + pa[2] * q1[2] + pa[3];
+                                    vb = pb[0] * q1[0] + pb[1] * q1[1] + pb[2] * q1[2] + pb[3];
+                                    k1 = (va < 0.0 && vb < 0.0);
+                                }
                             }
                         }
-                        cnt = (int)k0 + (int)k1;
-                    }
-                    const int off = wave_excl_scan(cnt);
-                    const int tot = wave_sum_i(cnt);
-                    count_op(s.ops, OP_ARC, wave_sum_i(solved));
-                    if (vn + tot >= MAXV) {
-                        cap = true;
-                    } else {
+                        const int cnt = (int)k0 + (int)k1;
+                        const int off = wave_excl_scan(cnt);
+                        const int tot = wave_sum_i(cnt);
+                        if (vn + tot > MAXV) {
+                            cap = true;
+                            break;
+                        }
                         int pos = vn + off;
-                        const double rd = (double)rado;
                         if (k0) {
                             s.vx[pos] = q0[0];
                             s.vy[pos] = q0[1];
                             s.vz[pos] = q0[2];
-                            s.vd[pos] = rd;
-                            s.vp0[pos] = (short)e0;
-                            s.vp1[pos] = (short)e1;
-                            s.vp2[pos] = -1;
+                            s.vpl[pos] = pack_planes(e0, e1, -1);
                             ++pos;
                         }
                         if (k1) {
                             s.vx[pos] = q1[0];
                             s.vy[pos] = q1[1];
                             s.vz[pos] = q1[2];
-                            s.vd[pos] = rd;
-                            s.vp0[pos] = (short)e0;
-                            s.vp1[pos] = (short)e1;
-                            s.vp2[pos] = -1;
+                            s.vpl[pos] = pack_planes(e0, e1, -1);
                         }
+                        vn += tot;
                     }
-                    vn += tot;
-                    wave_sync();
-                    if (cap) break;
+                    count_op(s.ops, OP_ARC, wave_sum_i(nsolved));
                 }
                 if (cap) {
-                    res.status = FA_CAPACITY;
+                    status = FA_CAPACITY;
                     break;
                 }
-                // -- keep vertices inside the sphere (LIB/Vcontacts.cpp:460-467)
+                wave_sync();
+                // -- keep the vertices inside the sphere (LIB/Vcontacts.cpp:460-467);
+                // arc points lie on it (distance rado by definition)
                 {
                     int vj = 0;
-                    const double rd = (double)rado;
                     for (int base = 0; base < vn; base += WAVE) {
                         const int vi = base + l;
-                        const bool v = vi < vn;
-                        double x = 0, y = 0, z = 0, d = 0;
-                        short p0 = 0, p1 = 0, p2 = 0;
-                        if (v) {
+                        double x = 0, y = 0, z = 0;
+                        unsigned pk = 0;
+                        bool keep = false;
+                        if (vi < vn) {
                             x = s.vx[vi];
                             y = s.vy[vi];
                             z = s.vz[vi];
-                            d = s.vd[vi];
-                            p0 = s.vp0[vi];
-                            p1 = s.vp1[vi];
-                            p2 = s.vp2[vi];
+                            pk = s.vpl[vi];
+                            const double d = (plane_at(pk, 2) == -1) ? ro : sqrt(x * x + y * y + z * z);
+                            keep = d <= ro;
                         }
-                        const bool keep = v && d <= rd;
                         const unsigned long long m = __ballot(keep);
                         wave_sync();
                         if (keep) {
@@ -1012,92 +1523,133 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                             s.vx[pos] = x;
                             s.vy[pos] = y;
                             s.vz[pos] = z;
-                            s.vd[pos] = d;
-                            s.vp0[pos] = p0;
-                            s.vp1[pos] = p1;
-                            s.vp2[pos] = p2;
+                            s.vpl[pos] = pk;
                         }
                         vj += __popcll(m);
                         wave_sync();
                     }
                     vn = vj;
                 }
-                // -- engulfing flags (LIB/Vcontacts.cpp:470-478)
-                for (int c = l; c < NC; c += WAVE)
-                    if (s.A3[c] > 0.0) s.pflag[c] = 'E';
+                // -- engulfing planes (LIB/Vcontacts.cpp:470-478)
+#pragma unroll
+                for (int q = 0; q < PSL; ++q) {
+                    const int c = l + WAVE * q;
+                    if (c < NC && pa3[q] > 0.0) s.pflag[c] = 'E';
+                }
                 wave_sync();
+                char pfl[PSL];
+#pragma unroll
+                for (int q = 0; q < PSL; ++q) {
+                    const int c = l + WAVE * q;
+                    pfl[q] = (c < NC) ? s.pflag[c] : 'X';
+                }
                 clk.stop(PH_ARC);
-                // -- seeds for ligand atoms not seeded yet (LIB/Vcontacts.cpp:595-629)
-                for (int m = l; m < L; m += WAVE) {
-                    if (s.seed[3 * m] != -1) continue;
-                    const int mc = Rn + m;
-                    for (int vi = 0; vi < vn; ++vi) {
-                        if (s.vp2[vi] == -1) continue;
-                        const int i0 = s.pidx[s.vp0[vi]], i1 = s.pidx[s.vp1[vi]], i2 = s.pidx[s.vp2[vi]];
-                        if (i0 == mc) {
-                            s.seed[3 * m] = acalc;
-                            s.seed[3 * m + 1] = i1;
-                            s.seed[3 * m + 2] = i2;
-                            break;
-                        } else if (i1 == mc) {
-                            s.seed[3 * m] = acalc;
-                            s.seed[3 * m + 1] = i0;
-                            s.seed[3 * m + 2] = i2;
-                            break;
-                        } else if (i2 == mc) {
-                            s.seed[3 * m] = acalc;
-                            s.seed[3 * m + 1] = i0;
-                            s.seed[3 * m + 2] = i1;
-                            break;
+                const int NV = vn;
+
+                // -- seeds of ligand atoms not seeded yet (save_seeds, LIB/Vcontacts.cpp:595-629):
+                // the first hull vertex (in order) with a plane of that atom
+#pragma unroll
+                for (int q = 0; q < PSL; ++q) {
+                    const int c = l + WAVE * q;
+                    const bool cand = c < NC && pix[q] >= Rn && s.seed[3 * (pix[q] - Rn)] == -1;
+                    unsigned long long mc = __ballot(cand);
+                    while (mc) {
+                        const int cc = WAVE * q + first_lane(mc);
+                        mc &= mc - 1;
+                        int hit_v = -1, hit_pos = 0;
+                        for (int vb = 0; vb < NV && hit_v < 0; vb += WAVE) {
+                            const int vi = vb + l;
+                            int pos = -1;
+                            if (vi < NV) {
+                                const unsigned pk = s.vpl[vi];
+                                if (plane_at(pk, 2) != -1) {
+                                    if (plane_at(pk, 0) == cc)
+                                        pos = 0;
+                                    else if (plane_at(pk, 1) == cc)
+                                        pos = 1;
+                                    else if (plane_at(pk, 2) == cc)
+                                        pos = 2;
+                                }
+                            }
+                            const unsigned long long mv = __ballot(pos >= 0);
+                            if (mv) {
+                                const int ln = first_lane(mv);
+                                hit_v = vb + ln;
+                                hit_pos = readlane_i(pos, ln);
+                            }
+                        }
+                        if (hit_v >= 0) {
+                            const unsigned pk = s.vpl[hit_v];
+                            const int i0 = atom_of(plane_at(pk, 0));
+                            const int i1 = atom_of(plane_at(pk, 1));
+                            const int i2 = atom_of(plane_at(pk, 2));
+                            const int m = atom_of(cc) - Rn;
+                            if (l == 0) {
+                                s.seed[3 * m] = acalc;
+                                s.seed[3 * m + 1] = hit_pos == 0 ? i1 : i0;
+                                s.seed[3 * m + 2] = hit_pos == 2 ? i1 : i2;
+                            }
                         }
                     }
                 }
                 wave_sync();
-                const int NV = vn;
                 clk.stop(PH_SEED);
+
                 // -- face point lists (order_faces, LIB/Vcontacts.cpp:699-831)
                 {
-                    // count points per face (+1 slot for an opposite point)
-                    int off_total = 0;
-                    for (int base = 0; base < NC; base += WAVE) {
-                        const int X = base + l;
-                        int cnt = 0;
-                        if (X < NC && s.pflag[X] != 'X') {
-                            for (int vi = 0; vi < NV; ++vi)
-                                cnt += (s.vp0[vi] == X) | (s.vp1[vi] == X) | (s.vp2[vi] == X);
-                        }
-                        const int slots = cnt + (cnt > 2 ? 1 : 0);
-                        const int off = wave_excl_scan(slots);
-                        if (X < NC) {
-                            s.foff[X] = (short)(off_total + off);
-                            s.fcnt[X] = (short)cnt;
-                        }
-                        off_total += wave_sum_i(slots);
+                    unsigned vpk[VSL];
+#pragma unroll
+                    for (int k2 = 0; k2 < VSL; ++k2) {
+                        const int vi = l + WAVE * k2;
+                        vpk[k2] = (vi < NV) ? s.vpl[vi] : 0xFFFFFFFFu;
                     }
-                    if (off_total > MAXF) {
-                        res.status = FA_CAPACITY;
+                    for (int c = l; c < NC; c += WAVE) s.fcnt[c] = 0;
+                    int run = 0;
+#pragma unroll
+                    for (int q = 0; q < PSL; ++q) {
+                        const int c = l + WAVE * q;
+                        unsigned long long mX = __ballot(c < NC && pfl[q] != 'X');
+                        while (mX) {
+                            const int X = WAVE * q + first_lane(mX);
+                            mX &= mX - 1;
+                            int cnt = 0;
+#pragma unroll
+                            for (int k2 = 0; k2 < VSL; ++k2) {
+                                if (k2 * WAVE >= NV) break;
+                                const int vi = l + WAVE * k2;
+                                const bool on = vi < NV && has_plane(vpk[k2], X);
+                                const unsigned long long m = __ballot(on);
+                                if (on) {
+                                    const int pos = run + cnt + lanes_below(m);
+                                    if (pos < MAXF) s.fpt[pos] = (short)vi;
+                                }
+                                cnt += __popcll(m);
+                            }
+                            if (l == 0) {
+                                s.foff[X] = (short)run;
+                                s.fcnt[X] = (unsigned char)cnt;
+                            }
+                            run += cnt + (cnt > 2 ? 1 : 0);
+                        }
+                    }
+                    if (run > MAXF) {
+                        status = FA_CAPACITY;
                         break;
-                    }
-                    wave_sync();
-                    for (int X = l; X < NC; X += WAVE) {
-                        if (s.pflag[X] == 'X') continue;
-                        int o = s.foff[X];
-                        for (int vi = 0; vi < NV; ++vi)
-                            if (s.vp0[vi] == X || s.vp1[vi] == X || s.vp2[vi] == X) s.fpt[o++] = (short)vi;
                     }
                     wave_sync();
                     // opposite point of the first arc point, appended in plane order
                     int nv_extra = NV;
-                    for (int base = 0; base < NC; base += WAVE) {
-                        const int X = base + l;
+#pragma unroll
+                    for (int q = 0; q < PSL; ++q) {
+                        const int X = l + WAVE * q;
                         bool add = false;
                         double t[3] = {0, 0, 0};
-                        if (X < NC && s.pflag[X] != 'X' && s.fcnt[X] > 2) {
+                        if (X < NC && pfl[q] != 'X' && s.fcnt[X] > 2) {
                             const int o = s.foff[X], sc = s.fcnt[X];
-                            const double cx = -s.A0[X] * s.A3[X], cy = -s.A1[X] * s.A3[X], cz = -s.A2[X] * s.A3[X];
-                            for (int q = 0; q < sc; ++q) {
-                                const int vi = s.fpt[o + q];
-                                if (s.vp2[vi] == -1) {
+                            const double cx = -pa0[q] * pa3[q], cy = -pa1[q] * pa3[q], cz = -pa2[q] * pa3[q];
+                            for (int w = 0; w < sc; ++w) {
+                                const int vi = s.fpt[o + w];
+                                if (plane_at(s.vpl[vi], 2) == -1) {
                                     t[0] = 2.0 * cx - s.vx[vi];
                                     t[1] = 2.0 * cy - s.vy[vi];
                                     t[2] = 2.0 * cz - s.vz[vi];
@@ -1116,41 +1668,41 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                             s.vx[vi] = t[0];
                             s.vy[vi] = t[1];
                             s.vz[vi] = t[2];
-                            s.vd[vi] = (double)rado;
-                            s.vp0[vi] = (short)X;
-                            s.vp1[vi] = -1;
-                            s.vp2[vi] = -1;
+                            s.vpl[vi] = pack_planes(X, -1, -1);
                             s.fpt[s.foff[X] + s.fcnt[X]] = (short)vi;
-                            s.fcnt[X] = s.fcnt[X] + 1;
+                            s.fcnt[X] = (unsigned char)(s.fcnt[X] + 1);
                         }
                         nv_extra += __popcll(m);
                     }
                     if (cap) {
-                        res.status = FA_CAPACITY;
+                        status = FA_CAPACITY;
                         break;
                     }
                     wave_sync();
-                    // order points around each face by cosine
+                    // points of each face ordered by cosine around it
                     bool noline = false;
                     int ncos = 0;
-                    for (int X = l; X < NC; X += WAVE) {
+#pragma unroll
+                    for (int q = 0; q < PSL; ++q) {
+                        const int X = l + WAVE * q;
+                        if (X >= NC || pfl[q] == 'X') continue;
                         const int sc = s.fcnt[X];
-                        if (s.pflag[X] == 'X' || sc <= 3) continue;
-                        ncos += sc - 2;
+                        if (sc <= 3) continue;
                         short* pt = &s.fpt[s.foff[X]];
-                        double* cs = &s.fcos[s.foff[X]];
-                        const int v0 = pt[0];
-                        const int Y = (s.vp0[v0] == X) ? s.vp1[v0] : s.vp0[v0];
+                        double* cs = &s.u.fcos[s.foff[X]];
+                        const unsigned pk0 = s.vpl[pt[0]];
+                        const int Y = (plane_at(pk0, 0) == X) ? plane_at(pk0, 1) : plane_at(pk0, 0);
                         int vi;
                         for (vi = 1; vi < sc; ++vi) {
-                            const int w = pt[vi];
-                            if (s.vp0[w] == Y || s.vp1[w] == Y || s.vp2[w] == Y) break;
+                            const unsigned pk = s.vpl[pt[vi]];
+                            if (plane_at(pk, 0) == Y || plane_at(pk, 1) == Y || plane_at(pk, 2) == Y) break;
                         }
-                        if (vi == sc) {  // no second point on the common line (reference reads past its list)
+                        if (vi == sc) {  // no second point on the common line (the reference reads past its list)
                             noline = true;
                             continue;
                         }
-                        short tsw = pt[vi];
+                        ncos += sc - 2;
+                        const short tsw = pt[vi];
                         pt[vi] = pt[1];
                         pt[1] = tsw;
                         const double P1[3] = {s.vx[pt[1]], s.vy[pt[1]], s.vz[pt[1]]};
@@ -1162,10 +1714,10 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                         for (vi = 2; vi < sc - 1; ++vi)
                             for (int v2 = vi + 1; v2 < sc; ++v2)
                                 if (cs[vi] < cs[v2]) {
-                                    short ts = pt[vi];
+                                    const short ts = pt[vi];
                                     pt[vi] = pt[v2];
                                     pt[v2] = ts;
-                                    double tc = cs[vi];
+                                    const double tc = cs[vi];
                                     cs[vi] = cs[v2];
                                     cs[v2] = tc;
                                 }
@@ -1173,36 +1725,40 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                     wave_sync();
                     count_op(s.ops, OP_ORDER, wave_sum_i(ncos));
                     if (__ballot(noline) != 0ull) {
-                        res.status = FA_CAPACITY;
+                        status = FA_CAPACITY;
                         break;
                     }
                 }
                 clk.stop(PH_FACE);
+
                 // -- areas (calc_areas + project_points, LIB/Vcontacts.cpp:838-1044, 1278-1344)
+                double par[PSL] = {0.0, 0.0};
                 {
                     int epi = 0, eng[3] = {-1, -1, -1};
-                    for (int base = 0; base < NC && epi < 3; base += WAVE) {
-                        const int c = base + l;
-                        unsigned long long m = __ballot(c < NC && s.pflag[c] == 'E');
+#pragma unroll
+                    for (int q = 0; q < PSL; ++q) {
+                        const int c = l + WAVE * q;
+                        unsigned long long m = __ballot(c < NC && pfl[q] == 'E');
                         while (m && epi < 3) {
-                            eng[epi++] = base + first_lane(m);
+                            eng[epi++] = WAVE * q + first_lane(m);
                             m &= m - 1;
                         }
                     }
-                    for (int c = l; c < NC; c += WAVE) s.area[c] = 0.0;
-                    const double ro = (double)rado;
-                    if (epi > 0) {
+                    const bool projected = epi > 0;
+                    if (projected) {
                         double proj[3];
                         if (epi == 1) {
-                            const int X = eng[0];
-                            proj[0] = -s.A0[X] * s.A3[X];
-                            proj[1] = -s.A1[X] * s.A3[X];
-                            proj[2] = -s.A2[X] * s.A3[X];
+                            double a0[4];
+                            plane_of(s, eng[0], a0);
+                            proj[0] = -a0[0] * a0[3];
+                            proj[1] = -a0[1] * a0[3];
+                            proj[2] = -a0[2] * a0[3];
                         } else if (epi == 2) {
                             double a0[4], a1[4], p0[3], p1[3];
                             plane_of(s, eng[0], a0);
                             plane_of(s, eng[1], a1);
                             if (solve_2xS(a0, a1, rado, p0, p1) == -1) {
+                                // the reference uses the atom's absolute coordinates here
                                 proj[0] = (double)(ax + rado);
                                 proj[1] = (double)ay;
                                 proj[2] = (double)az;
@@ -1222,34 +1778,40 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                             proj[2] = p0[2];
                         }
                         for (int vi = l; vi < NV; vi += WAVE) {
-                            if (s.vp2[vi] == -1) continue;
-                            double P0 = s.vx[vi], P1 = s.vy[vi], P2 = s.vz[vi];
-                            double V0 = P0 - proj[0], V1 = P1 - proj[1], V2 = P2 - proj[2];
-                            double qa = V0 * V0 + V1 * V1 + V2 * V2;
-                            double qb = 2 * (P0 * V0 + P1 * V1 + P2 * V2);
-                            double qc = P0 * P0 + P1 * P1 + P2 * P2 - (double)(rado * rado);
-                            double kk = (sqrt(qb * qb - 4.0 * qa * qc) - qb) / (2 * qa);
+                            if (plane_at(s.vpl[vi], 2) == -1) continue;
+                            const double P0 = s.vx[vi], P1 = s.vy[vi], P2 = s.vz[vi];
+                            const double V0 = P0 - proj[0], V1 = P1 - proj[1], V2 = P2 - proj[2];
+                            const double qa = V0 * V0 + V1 * V1 + V2 * V2;
+                            const double qb = 2 * (P0 * V0 + P1 * V1 + P2 * V2);
+                            const double qc = P0 * P0 + P1 * P1 + P2 * P2 - (double)(rado * rado);
+                            const double kk = (sqrt(qb * qb - 4.0 * qa * qc) - qb) / (2 * qa);
                             s.vx[vi] = P0 + kk * V0;
                             s.vy[vi] = P1 + kk * V1;
                             s.vz[vi] = P2 + kk * V2;
-                            s.vd[vi] = ro;
                         }
+                        wave_sync();
                     }
-                    wave_sync();
+                    // distance of a face point: hull vertices keep |v| until projected
+                    // onto the sphere; arc and opposite points lie on it
+                    auto vdist = [&](int w) {
+                        return (projected || plane_at(s.vpl[w], 2) == -1) ? ro : vnorm(s, w);
+                    };
                     int ntri = 0, narc = 0;
-                    for (int X = l; X < NC; X += WAVE) {
-                        if (s.pflag[X] == 'X') continue;
+#pragma unroll
+                    for (int q = 0; q < PSL; ++q) {
+                        const int X = l + WAVE * q;
+                        if (X >= NC || pfl[q] == 'X') continue;
                         const int NP = s.fcnt[X];
                         const short* pt = &s.fpt[s.foff[X]];
-                        const double ctr[3] = {-s.A0[X] * s.A3[X], -s.A1[X] * s.A3[X], -s.A2[X] * s.A3[X]};
-                        const double cd = fabs(s.A3[X]);
+                        const double ctr[3] = {-pa0[q] * pa3[q], -pa1[q] * pa3[q], -pa2[q] * pa3[q]};
+                        const double cd = fabs(pa3[q]);
                         double ar = 0.0;
                         if (NP == 0) {
                             if (test_point(s, ctr, NC, X)) ar = (2.0f * PI_REF) * ro * (ro - cd);
                         } else if (NP == 2) {
                             const double Pa[3] = {s.vx[pt[0]], s.vy[pt[0]], s.vz[pt[0]]};
                             const double Pb[3] = {s.vx[pt[1]], s.vy[pt[1]], s.vz[pt[1]]};
-                            const double arc = spherical_arc(ctr, cd, Pa, s.vd[pt[0]], Pb, s.vd[pt[1]], rado);
+                            const double arc = spherical_arc(ctr, cd, Pa, vdist(pt[0]), Pb, vdist(pt[1]), rado);
                             ++narc;
                             if (test_point(s, ctr, NC, X))
                                 ar = (2.0f * PI_REF) * ro * (ro - cd) - arc;
@@ -1257,154 +1819,156 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                                 ar = arc;
                         } else {
                             const int w0 = pt[0];
-                            const double x0 = s.vx[w0], y0 = s.vy[w0], z0 = s.vz[w0], d0 = s.vd[w0];
-                            // cos to vertex 0 and to the next vertex, on the fly
-                            auto cN0 = [&](int q) {
-                                const int w = pt[q];
-                                return (x0 * s.vx[w] + y0 * s.vy[w] + z0 * s.vz[w]) / (d0 * s.vd[w]);
-                            };
-                            auto cNN = [&](int q) {
-                                const int wa = pt[q], wb = pt[(q + 1) % NP];
-                                return (s.vx[wa] * s.vx[wb] + s.vy[wa] * s.vy[wb] + s.vz[wa] * s.vz[wb]) /
-                                       (s.vd[wa] * s.vd[wb]);
-                            };
-                            double c0 = NP > 2 ? cN0(1) : 0.0;
+                            const double x0 = s.vx[w0], y0 = s.vy[w0], z0 = s.vz[w0], d0 = vdist(w0);
                             if (NP > 2) ntri += NP - 2;
-                            for (int q = 1; q < NP - 1; ++q) {
-                                const double cq = cNN(q);
-                                const double c1 = cN0(q + 1);
-                                double U = sqrt((1 + c0) * (1 + cq) * (1 + c1) / 8.0);
-                                double V = sqrt((1 - c0) * (1 - cq) * (1 + c1) / 8.0);
-                                double Wv = sqrt((1 - c0) * (1 + cq) * (1 - c1) / 8.0);
-                                double Xv = sqrt((1 + c0) * (1 - cq) * (1 - c1) / 8.0);
-                                double tS = (1 - U + V + Wv + Xv) / (1 + U - V - Wv - Xv);
-                                double tSA = (1 - U - V - Wv + Xv) / (1 + U + V + Wv - Xv);
-                                double tSB = (1 - U - V + Wv - Xv) / (1 + U + V - Wv + Xv);
-                                double tSC = (1 - U + V - Wv - Xv) / (1 + U - V + Wv + Xv);
-                                double tp = sqrt(tS * tSA * tSB * tSC);
+                            // walk the fan: cos to vertex 0 and to the next vertex
+                            int wq = (NP > 1) ? pt[1] : w0;
+                            double dq = vdist(wq);
+                            double c0 = (NP > 2) ? (x0 * s.vx[wq] + y0 * s.vy[wq] + z0 * s.vz[wq]) / (d0 * dq) : 0.0;
+                            for (int qq = 1; qq < NP - 1; ++qq) {
+                                const int wn = pt[qq + 1];
+                                const double dn = vdist(wn);
+                                const double cq = (s.vx[wq] * s.vx[wn] + s.vy[wq] * s.vy[wn] + s.vz[wq] * s.vz[wn]) / (dq * dn);
+                                const double c1 = (x0 * s.vx[wn] + y0 * s.vy[wn] + z0 * s.vz[wn]) / (d0 * dn);
+                                const double U = sqrt((1 + c0) * (1 + cq) * (1 + c1) / 8.0);
+                                const double V = sqrt((1 - c0) * (1 - cq) * (1 + c1) / 8.0);
+                                const double Wv = sqrt((1 - c0) * (1 + cq) * (1 - c1) / 8.0);
+                                const double Xv = sqrt((1 + c0) * (1 - cq) * (1 - c1) / 8.0);
+                                const double tS = (1 - U + V + Wv + Xv) / (1 + U - V - Wv - Xv);
+                                const double tSA = (1 - U - V - Wv + Xv) / (1 + U + V + Wv - Xv);
+                                const double tSB = (1 - U - V + Wv - Xv) / (1 + U + V - Wv + Xv);
+                                const double tSC = (1 - U + V - Wv - Xv) / (1 + U - V + Wv + Xv);
+                                const double tp = sqrt(tS * tSA * tSB * tSC);
                                 if (tp > 0.0) ar += 4.0 * ro * ro * atan(sqrt(tp));
                                 c0 = c1;
+                                wq = wn;
+                                dq = dn;
                             }
-                            for (int q = 0; q < NP; ++q) {
-                                const int wa = pt[q], wb = pt[(q + 1) % NP];
-                                if (s.vp2[wa] == -1 && s.vp2[wb] == -1 &&
-                                    (s.vp0[wa] + s.vp1[wa]) != (s.vp0[wb] + s.vp1[wb])) {
+                            for (int qq = 0; qq < NP; ++qq) {
+                                const int wa = pt[qq], wb = pt[(qq + 1) % NP];
+                                const unsigned ka = s.vpl[wa], kb2 = s.vpl[wb];
+                                if (plane_at(ka, 2) == -1 && plane_at(kb2, 2) == -1 &&
+                                    (plane_at(ka, 0) + plane_at(ka, 1)) != (plane_at(kb2, 0) + plane_at(kb2, 1))) {
                                     const double Pa[3] = {s.vx[wa], s.vy[wa], s.vz[wa]};
                                     const double Pb[3] = {s.vx[wb], s.vy[wb], s.vz[wb]};
-                                    ar += spherical_arc(ctr, cd, Pa, s.vd[wa], Pb, s.vd[wb], rado);
+                                    ar += spherical_arc(ctr, cd, Pa, vdist(wa), Pb, vdist(wb), rado);
                                     ++narc;
                                 }
                             }
                         }
-                        s.area[X] = ar;
+                        par[q] = ar;
                     }
                     count_op(s.ops, OP_TRI, wave_sum_i(ntri));
                     count_op(s.ops, OP_SARC, wave_sum_i(narc));
-                    wave_sync();
-                    if (epi > 0) {
-                        // corrections onto the planes adjacent to engulfing faces, sequential
+                    if (projected) {
+                        // corrections onto the planes next to engulfing faces, sequential
+                        wave_sync();
+#pragma unroll
+                        for (int q = 0; q < PSL; ++q) {
+                            const int c = l + WAVE * q;
+                            if (c < NC) s.u.area[c] = par[q];
+                        }
+                        wave_sync();
                         if (l == 0) {
                             for (int X = 0; X < NC; ++X) {
                                 if (s.pflag[X] != 'E') continue;
                                 const int NP = s.fcnt[X];
                                 const short* pt = &s.fpt[s.foff[X]];
-                                const double ctr[3] = {-s.A0[X] * s.A3[X], -s.A1[X] * s.A3[X], -s.A2[X] * s.A3[X]};
-                                const double cd = fabs(s.A3[X]);
-                                for (int q = 0; q < NP; ++q) {
-                                    const int cu = pt[q], nx = pt[(q + 1) % NP];
+                                double pX[4];
+                                plane_of(s, X, pX);
+                                const double ctr[3] = {-pX[0] * pX[3], -pX[1] * pX[3], -pX[2] * pX[3]};
+                                const double cd = fabs(pX[3]);
+                                for (int qq = 0; qq < NP; ++qq) {
+                                    const int cu = pt[qq], nx = pt[(qq + 1) % NP];
+                                    const unsigned kc = s.vpl[cu], kn = s.vpl[nx];
                                     int pa1, pa2;
-                                    if (s.vp0[cu] == X) {
-                                        pa1 = s.vp1[cu];
-                                        pa2 = s.vp2[cu];
+                                    if (plane_at(kc, 0) == X) {
+                                        pa1 = plane_at(kc, 1);
+                                        pa2 = plane_at(kc, 2);
                                     } else {
-                                        pa1 = s.vp0[cu];
-                                        pa2 = (s.vp1[cu] == X) ? s.vp2[cu] : s.vp1[cu];
+                                        pa1 = plane_at(kc, 0);
+                                        pa2 = (plane_at(kc, 1) == X) ? plane_at(kc, 2) : plane_at(kc, 1);
                                     }
+                                    const int n0 = plane_at(kn, 0), n1 = plane_at(kn, 1), n2 = plane_at(kn, 2);
                                     int comm;
-                                    if (pa1 == s.vp0[nx] || pa1 == s.vp1[nx] || pa1 == s.vp2[nx])
+                                    if (pa1 == n0 || pa1 == n1 || pa1 == n2)
                                         comm = pa1;
-                                    else if (pa2 == s.vp0[nx] || pa2 == s.vp1[nx] || pa2 == s.vp2[nx])
+                                    else if (pa2 == n0 || pa2 == n1 || pa2 == n2)
                                         comm = pa2;
                                     else
                                         continue;
                                     if (comm != -1 && s.pflag[comm] != 'E') {
                                         const double Pa[3] = {s.vx[cu], s.vy[cu], s.vz[cu]};
                                         const double Pb[3] = {s.vx[nx], s.vy[nx], s.vz[nx]};
-                                        s.area[comm] += spherical_arc(ctr, cd, Pa, s.vd[cu], Pb, s.vd[nx], rado);
+                                        s.u.area[comm] += spherical_arc(ctr, cd, Pa, ro, Pb, ro, rado);
                                         s.ops[OP_SARC] += 1;
                                         if (NP == 2) break;
                                     }
                                 }
                             }
                             if (epi == 1) {
-                                s.area[eng[0]] = (2.0f * PI_REF) * ro * (ro + s.pdist[eng[0]]);
+                                s.u.area[eng[0]] = (2.0f * PI_REF) * ro * (ro + fabs(s.A3[eng[0]]));
                             } else if (epi == 2) {
                                 double a0[4], a1[4], b[3], c[3];
                                 plane_of(s, eng[0], a0);
                                 plane_of(s, eng[1], a1);
                                 if (solve_2xS(a0, a1, rado, b, c) == -1) {
-                                    s.area[eng[0]] = (2.0f * PI_REF) * ro * ro;
-                                    s.area[eng[1]] = (2.0f * PI_REF) * ro * ro;
+                                    s.u.area[eng[0]] = (2.0f * PI_REF) * ro * ro;
+                                    s.u.area[eng[1]] = (2.0f * PI_REF) * ro * ro;
                                 } else {
                                     const double c0[3] = {-a0[0] * a0[3], -a0[1] * a0[3], -a0[2] * a0[3]};
                                     const double c1[3] = {-a1[0] * a1[3], -a1[1] * a1[3], -a1[2] * a1[3]};
                                     double mx = spherical_arc(c0, fabs(a0[3]), b, ro, c, ro, rado);
                                     mx += spherical_arc(c1, fabs(a1[3]), b, ro, c, ro, rado);
                                     s.ops[OP_SARC] += 2;
-                                    s.area[eng[0]] = (2.0f * PI_REF) * ro * ro - 0.5 * mx;
-                                    s.area[eng[1]] = (2.0f * PI_REF) * ro * ro - 0.5 * mx;
+                                    s.u.area[eng[0]] = (2.0f * PI_REF) * ro * ro - 0.5 * mx;
+                                    s.u.area[eng[1]] = (2.0f * PI_REF) * ro * ro - 0.5 * mx;
                                 }
                             }
                         }
                         wave_sync();
-                        if (epi >= 3) {
-                            for (int X = l; X < NC; X += WAVE)
-                                s.area[X] = (s.pflag[X] == 'E') ? (4.0f * PI_REF) * ro * ro / epi : 0.0;
-                            wave_sync();
+#pragma unroll
+                        for (int q = 0; q < PSL; ++q) {
+                            const int c = l + WAVE * q;
+                            if (c < NC) par[q] = s.u.area[c];
+                            if (epi >= 3 && c < NC) par[q] = (pfl[q] == 'E') ? (4.0f * PI_REF) * ro * ro / epi : 0.0;
                         }
                     }
                 }
                 clk.stop(PH_AREA);
-                // -- records (save_areas) and the energy terms of atom a (vcfunction)
+
+                // -- records (save_areas, LIB/Vcontacts.cpp:1149-1220) and the energy
+                // terms of atom a (vcfunction, LIB/vcfunction.cpp:77-527)
                 {
                     const double radA = (double)ra;
                     const double radoA = radA + RW;
-                    const double permeaD = (double)prm.permeability;
                     const double dee = (double)prm.dee_clash;
                     const int nt = prm.ntypes;
                     const int ta = lig.type[a];
                     double sas_sub = 0.0, com = 0.0, wal = 0.0;
                     int rcl = 0;
-                    int recbase = nrec_total;
+                    bool valid[PSL];
                     int nvalid_total = 0;
-                    // count valid to place records in reverse order
-                    for (int base = 0; base < NC; base += WAVE) {
-                        const int c = base + l;
-                        nvalid_total += __popcll(__ballot(c < NC && s.area[c] > 0.0 && s.pflag[c] != 'X'));
+#pragma unroll
+                    for (int q = 0; q < PSL; ++q) {
+                        const int c = l + WAVE * q;
+                        valid[q] = c < NC && par[q] > 0.0 && pfl[q] != 'X';
+                        nvalid_total += __popcll(__ballot(valid[q]));
                     }
                     int seen = 0;
-                    for (int base = 0; base < NC; base += WAVE) {
-                        const int c = base + l;
-                        const bool valid = c < NC && s.area[c] > 0.0 && s.pflag[c] != 'X';
-                        const unsigned long long m = __ballot(valid);
-                        if (valid) {
-                            const int j = s.pidx[c];
-                            const double ar = s.area[c];
-                            const double dist = s.cdist[c];
+#pragma unroll
+                    for (int q = 0; q < PSL; ++q) {
+                        const unsigned long long m = __ballot(valid[q]);
+                        if (valid[q]) {
+                            const int j = pix[q];
+                            const double ar = par[q];
+                            const double dist = pcd[q];
                             sas_sub += ar;
                             const bool lj = j >= Rn;
                             const int mj = j - Rn;
-                            float rb;
-                            int tb;
-                            if (lj) {
-                                rb = lig.radius[mj];
-                                tb = lig.type[mj];
-                            } else {
-                                rb = R.rad[j];
-                                tb = R.type[j];
-                            }
-                            if (B.recs) {
-                                const int r = recbase + (nvalid_total - 1 - (seen + lanes_below(m)));
+                            const int tb = lj ? lig.type[mj] : R.type[j];
+                            if (B.recs) {  // own list in reverse plane order
+                                const int r = nrec_total + (nvalid_total - 1 - (seen + lanes_below(m)));
                                 if (r < B.rec_cap) {
                                     fa_contact_rec& o = B.recs[(size_t)pose * B.rec_cap + r];
                                     o.from = acalc;
@@ -1415,16 +1979,17 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                             }
                             bool skip = false;
                             if (lj) {
+                                // each ligand pair scored once (by the higher calc index), bonded pairs never
                                 const bool bonded = (lig.bonded[2 * mj + (a >> 6)] >> (a & 63)) & 1ull;
                                 skip = bonded || (mj < a);
-                                if (!s.done[mj]) s.stub[2 * mj + (a >> 6)] |= (1ull << (a & 63));
+                                if (!bit_of(dm0, dm1, mj)) s.stub[2 * mj + (a >> 6)] |= (1ull << (a & 63));
                             }
                             if (!skip) {
-                                const double rAB = radA + (double)rb;
-                                const float clash_distance = (float)(permeaD * rAB);
+                                const double rAB = radA + (double)prb[q];
+                                const float clash_distance = (float)(permea * rAB);
                                 if (dist < (double)clash_distance) {
                                     const double d2 = dist * dist, d4 = d2 * d2, d6 = d4 * d2;
-                                    const double cr = permeaD * rAB, cr2 = cr * cr, cr4 = cr2 * cr2, cr6 = cr4 * cr2;
+                                    const double cr = permea * rAB, cr2 = cr * cr, cr4 = cr2 * cr2, cr6 = cr4 * cr2;
                                     wal += (double)KWALL_F * (1.0 / (d6 * d6) - 1.0 / (cr6 * cr6));
                                     if (dist <= dee * rAB) rcl = 1;
                                 }
@@ -1439,58 +2004,67 @@ __global__ __launch_bounds__(WAVE) void cf_pose_kernel(DevReceptor R, const DevG
                     com = wave_sum_d(com);
                     wal = wave_sum_d(wal);
                     rcl = wave_sum_i(rcl);
-                    if (l == 0) {
-                        double SAS = 4.0 * PI_REF * radoA * radoA - sas_sub;
-                        if (SAS < 0.0) SAS = 0.0;
-                        s.acc_com[a] = com;
-                        s.acc_wal[a] = wal;
-                        s.acc_tot[a] = SAS;
-                        s.acc_sas[a] = (prm.solventterm != 0.0f)
-                                           ? (double)prm.solventterm * SAS
-                                           : (double)prm.emat[(ta - 1) * nt + (nt - 1)] * SAS;
-                        s.acc_rcl[a] = rcl ? 1 : 0;
-                        s.done[a] = 1;
-                    }
+                    double SAS = 4.0 * PI_REF * radoA * radoA - sas_sub;
+                    if (SAS < 0.0) SAS = 0.0;
+                    r_com += com;
+                    r_wal += wal;
+                    r_tot += SAS;
+                    r_sas += (prm.solventterm != 0.0f) ? (double)prm.solventterm * SAS
+                                                        : (double)prm.emat[(ta - 1) * nt + (nt - 1)] * SAS;
+                    r_rclash |= rcl ? 1 : 0;
+                    if (a < WAVE)
+                        dm0 |= 1ull << a;
+                    else
+                        dm1 |= 1ull << (a - WAVE);
                     wave_sync();
                 }
                 clk.stop(PH_REC);
             }
-            if (res.status == FA_OK) {
-                // atoms with no contact still contribute their full SAS
+            if (status == FA_OK) {
+                // atoms with no contact at all keep their whole sphere as SAS
+                double add_tot = 0.0, add_sas = 0.0;
                 for (int i = l; i < L; i += WAVE) {
-                    if (!s.done[i]) {
+                    if (!bit_of(dm0, dm1, i)) {
                         const double radoA = (double)lig.radius[i] + RW;
                         const double SAS = 4.0 * PI_REF * radoA * radoA;
                         const int ti = lig.type[i];
-                        s.acc_com[i] = 0.0;
-                        s.acc_wal[i] = 0.0;
-                        s.acc_tot[i] = SAS;
-                        s.acc_sas[i] = (prm.solventterm != 0.0f)
-                                           ? (double)prm.solventterm * SAS
-                                           : (double)prm.emat[(ti - 1) * prm.ntypes + (prm.ntypes - 1)] * SAS;
-                        s.acc_rcl[i] = 0;
+                        add_tot += SAS;
+                        add_sas += (prm.solventterm != 0.0f)
+                                       ? (double)prm.solventterm * SAS
+                                       : (double)prm.emat[(ti - 1) * prm.ntypes + (prm.ntypes - 1)] * SAS;
                     }
                 }
-                wave_sync();
-                if (l == 0) {
-                    for (int i = 0; i < L; ++i) {
-                        res.com += s.acc_com[i];
-                        res.wal += s.acc_wal[i];
-                        res.sas += s.acc_sas[i];
-                        res.totsas += s.acc_tot[i];
-                        res.rclash |= s.acc_rcl[i];
-                    }
-                    res.cf = res.com + res.wal + res.sas + res.con;
-                    res.app = res.com + res.wal + res.sas;
-                }
+                r_tot += wave_sum_d(add_tot);
+                r_sas += wave_sum_d(add_sas);
             }
         }
-        if (res.status == FA_CAPACITY && l == 0) atomicAdd(&B.counters[1], 1u);
+        fa_cf_result res;
+        res.com = res.wal = res.sas = res.con = res.totsas = res.cf = res.app = 0.0;
+        res.rclash = 0;
+        res.status = status;
+        if (status == FA_OK) {
+            res.com = r_com;
+            res.wal = r_wal;
+            res.sas = r_sas;
+            res.totsas = r_tot;
+            res.rclash = r_rclash;
+            res.cf = res.com + res.wal + res.sas + res.con;
+            res.app = res.com + res.wal + res.sas;
+        } else if (status == FA_CLASH) {
+            res.wal = clash;
+            res.cf = clash + 1.0;  // metal_coord = 1 in the clash cfstr (LIB/ic2cf.cpp:129)
+            res.app = clash + 1.0;
+        } else if (status == FA_HULL_FAIL) {
+            res.wal = (double)POLY_PENALTY;
+            res.cf = res.wal + 1.0;
+            res.app = res.wal + 1.0;
+        }
+        if (status == FA_CAPACITY && l == 0) atomicAdd(&B.counters[1], 1u);
         wave_sync();
         if (B.ops && l < FA_OPCOUNT_N && s.ops[l]) atomicAdd(&B.ops[l], s.ops[l]);
         if (l == 0) {
             B.out[pose] = res;
-            if (B.rec_count) B.rec_count[pose] = (res.status == FA_OK) ? nrec_total : 0;
+            if (B.rec_count) B.rec_count[pose] = (status == FA_OK) ? nrec_total : 0;
         }
         wave_sync();
     }
@@ -1501,11 +2075,19 @@ hipError_t launch_cf(const DevReceptor& R, const DevGrid* grids, const DevLigand
                      const DevBatch& B, int n_blocks, hipStream_t stream)
 {
     if (B.n_poses <= 0) return hipSuccess;
-    if (n_blocks <= 0 || n_blocks > B.n_poses) n_blocks = B.n_poses;
+    if (n_blocks <= 0) n_blocks = 1;
     hipLaunchKernelGGL(cf_pose_kernel, dim3(n_blocks), dim3(WAVE), 0, stream, R, grids, ligs, prm, B);
     return hipGetLastError();
 }
 
 int smem_bytes() { return (int)sizeof(Smem); }
+
+// resident waves per CU for the pose kernel (LDS and registers), 0 on error
+int cf_waves_per_cu()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, cf_pose_kernel, WAVE, 0) != hipSuccess) return 0;
+    return n;
+}
 
 }  // namespace fa

@@ -6,6 +6,7 @@
 // evaluation entry points only move genes in and results out.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +21,7 @@
 namespace fa {
 hipError_t launch_cf(const DevReceptor& R, const DevGrid* grids, const DevLigand* ligs, const DevParams& prm,
                      const DevBatch& B, int n_blocks, hipStream_t stream);
+int cf_waves_per_cu();
 int smem_bytes();
 size_t lse_scratch_bytes();
 hipError_t launch_thermo(const double* d_energy, int stride, const int* d_count, long long n, double T,
@@ -47,6 +49,7 @@ int fail(int code, const std::string& msg)
     } while (0)
 
 constexpr float CELLSIZE = 6.5f;  // LIB/Vcontacts.h:56
+constexpr int NEAR_PER_ATOM = 160;
 
 struct DevBuf {
     void* p = nullptr;
@@ -68,9 +71,14 @@ struct fa_ctx {
     std::vector<fa::DevLigand> h_ligs;
     std::vector<int> lig_genes;
     int n_ligands = 0;
+    int max_lig_atoms = 0;
     bool have_system = false;
+    // pose-kernel grid: one resident wave per slot, each with its own
+    // neighbour-list scratch of near_cap entries
+    int n_waves = 0;
+    int near_cap = 0;
     // pose buffers
-    DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops, prof;
+    DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops, prof, near_idx, near_xyz;
     bool profile = false;
     std::vector<uint32_t> last_counters = std::vector<uint32_t>(4, 0);
 
@@ -174,6 +182,8 @@ const char* fa_last_error(void) { return g_err.c_str(); }
 
 int fa_kernel_lds_bytes(void) { return fa::smem_bytes(); }
 
+int fa_ctx_launch_waves(fa_ctx* c) { return c ? c->n_waves : 0; }
+
 int fa_get_device_info(int device_id, fa_device_info* out)
 {
     if (!out) return fail(FA_ERR_INVALID, "null output");
@@ -229,7 +239,8 @@ int fa_ctx_destroy(fa_ctx* c)
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     c->free_system();
-    for (DevBuf* b : {&c->genes, &c->out, &c->counters, &c->recs, &c->reccnt, &c->xyz, &c->scratch, &c->thermo, &c->ops, &c->prof})
+    for (DevBuf* b : {&c->genes, &c->out, &c->counters, &c->recs, &c->reccnt, &c->xyz, &c->scratch, &c->thermo,
+                      &c->ops, &c->prof, &c->near_idx, &c->near_xyz})
         if (b->p) (void)hipFree(b->p);
     delete c;
     return FA_SUCCESS;
@@ -254,17 +265,17 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
     c->free_system();
     int rc;
     const int Rn = r->n_atoms;
-    // receptor SoA (calc order)
+    // receptor atoms (calc order): x, y, z, radius packed for one 16-byte load
     std::vector<float> x(Rn), y(Rn), z(Rn);
+    std::vector<float4> rxyzr(Rn);
     for (int i = 0; i < Rn; ++i) {
         x[i] = r->xyz[3 * i];
         y[i] = r->xyz[3 * i + 1];
         z[i] = r->xyz[3 * i + 2];
+        rxyzr[i] = make_float4(x[i], y[i], z[i], r->radius[i]);
     }
     c->rec.n = Rn;
-    if ((rc = upload(c, x.data(), Rn, &c->rec.x)) || (rc = upload(c, y.data(), Rn, &c->rec.y)) ||
-        (rc = upload(c, z.data(), Rn, &c->rec.z)) || (rc = upload(c, r->radius, Rn, &c->rec.rad)) ||
-        (rc = upload(c, r->type, Rn, &c->rec.type)))
+    if ((rc = upload(c, rxyzr.data(), Rn, &c->rec.xyzr)) || (rc = upload(c, r->type, Rn, &c->rec.type)))
         return rc;
     // parameters
     c->prm.permeability = prm->permeability;
@@ -363,6 +374,18 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
     c->d_grids = const_cast<fa::DevGrid*>(dg);
     c->d_ligs = const_cast<fa::DevLigand*>(dl);
     c->n_ligands = n_ligands;
+    // neighbour-list scratch: NEAR_PER_ATOM entries per ligand atom and wave
+    // (in-range candidates of all atoms of a pose; ~77 per atom at most on
+    // dense receptors, overflow is reported as a capacity status)
+    int max_l = 0;
+    for (int li = 0; li < n_ligands; ++li) max_l = std::max(max_l, ligs[li].n_atoms);
+    c->max_lig_atoms = max_l;
+    const int per_cu = std::max(1, fa::cf_waves_per_cu());
+    c->n_waves = c->n_cu * per_cu;
+    c->near_cap = max_l * NEAR_PER_ATOM;
+    const size_t n_near = (size_t)c->n_waves * c->near_cap;
+    if ((rc = ensure(c->near_idx, sizeof(int2) * n_near)) || (rc = ensure(c->near_xyz, sizeof(float4) * n_near)))
+        return rc;
     c->have_system = true;
     return FA_SUCCESS;
 }
@@ -391,12 +414,11 @@ static int launch(fa_ctx* c, int n_poses, const double* d_genes, int stride, con
         B.prof = static_cast<unsigned long long*>(c->prof.p);
         FA_HIP(hipMemsetAsync(c->prof.p, 0, 16 * sizeof(unsigned long long), stream));
     }
-    // enough resident waves to cover the chip; each wave loops over poses
-    const int lds = fa::smem_bytes();
-    int per_cu = (160 * 1024) / (lds > 0 ? lds : 1);
-    if (per_cu < 1) per_cu = 1;
-    const int n_blocks = c->n_cu * per_cu * 4;
-    FA_HIP(fa::launch_cf(c->rec, c->d_grids, c->d_ligs, c->prm, B, n_blocks, stream));
+    B.near_idx = static_cast<int2*>(c->near_idx.p);
+    B.near_xyz = static_cast<float4*>(c->near_xyz.p);
+    B.near_cap = c->near_cap;
+    // one wave per resident slot; each wave loops over poses (work queue)
+    FA_HIP(fa::launch_cf(c->rec, c->d_grids, c->d_ligs, c->prm, B, c->n_waves, stream));
     return FA_SUCCESS;
 }
 

@@ -9,14 +9,11 @@
 
 namespace fa {
 
-// Receptor atoms in calc order (structure of arrays).
+// Receptor atoms in calc order.
 struct DevReceptor {
     int n;
-    const float* x;
-    const float* y;
-    const float* z;
-    const float* rad;
-    const int* type;
+    const float4* xyzr;             // [n] x, y, z, radius
+    const int* type;                // [n]
 };
 
 // Cell list of the receptor for one grid (the grid index_protein builds when
@@ -82,6 +79,11 @@ struct DevBatch {
     unsigned int* counters;         // [4]: slow-grid poses, capacity overflows, work queue, spare
     unsigned long long* ops;        // optional: [FA_OPCOUNT_N] operation counters (fa_opcount order)
     unsigned long long* prof;       // optional: [16] cycles per kernel phase (summed over waves)
+    // per-wave neighbour lists (HBM scratch): wave w owns entries
+    // [w * near_cap, (w + 1) * near_cap) of both arrays
+    int2* near_idx;                 // calc index, squared distance (float bits)
+    float4* near_xyz;               // x, y, z, radius of the neighbour
+    int near_cap;
 };
 
 }  // namespace fa

@@ -68,6 +68,7 @@ _SIGS = {
     "fa_boltzmann_device": ([_P, _P, C.c_int, C.c_int64, C.c_double, _P, _P, _P], C.c_int),
     "fa_log_sum_exp_device": ([_P, _P, C.c_int, C.c_int64, _P, _P], C.c_int),
     "fa_kernel_lds_bytes": ([], C.c_int),
+    "fa_ctx_launch_waves": ([_P], C.c_int),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -198,6 +199,10 @@ class Context:
         a = np.zeros(16, np.uint64)
         _check(_lib.fa_last_phase_cycles(self._ctx, a.ctypes.data))
         return {k: int(a[i]) for i, k in enumerate(self.PHASES)}
+
+    def launch_waves(self) -> int:
+        """Waves one pose launch runs (resident slots on the device)."""
+        return int(_lib.fa_ctx_launch_waves(self._ctx))
 
     def counters(self):
         a, b = C.c_uint32(), C.c_uint32()

@@ -135,6 +135,11 @@ int fa_last_phase_cycles(fa_ctx* ctx, uint64_t* cycles16);
 /* Size of the kernel's per-wave LDS workspace (bytes). */
 int fa_kernel_lds_bytes(void);
 
+/* Waves a pose launch of this context runs (one per resident slot on the
+ * device, each looping over poses); 0 before a system is uploaded.  Launches
+ * of many times this many poses keep every slot busy to the end. */
+int fa_ctx_launch_waves(fa_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

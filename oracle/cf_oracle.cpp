@@ -68,6 +68,23 @@ thread_local fa_opcount* t_ops = nullptr;
         if (t_ops) t_ops->field += (n);      \
     } while (0)
 
+// capacity statistics: largest sizes seen (sizing of the GPU kernel's
+// fixed per-wave arrays); null unless requested
+struct CapStats {
+    int near_pose;   // in-range candidates of all atoms of a pose, done or not
+    int nc;          // contacts of one atom (main pass)
+    int edges;       // hull edges of one atom
+    int vn;          // hull vertices + arc points before the sphere cut
+    int nv_opp;      // vertices after the opposite points are appended
+    int face_slots;  // sum over faces of (points + 1 if more than 2 points)
+    int face_slots_run;
+};
+thread_local CapStats* t_caps = nullptr;
+#define CAPMAX(field, v)                                      \
+    do {                                                      \
+        if (t_caps && (v) > t_caps->field) t_caps->field = (v); \
+    } while (0)
+
 template <class R> struct Plane {
     R Ai[4];
     R dist;
@@ -242,6 +259,7 @@ template <class R> struct Work {
     std::vector<std::vector<int>> stubs;   // per ligand atom: earlier atoms with a record to it
     std::vector<std::vector<fa_contact_rec>> records; // per ligand atom: own valid contacts (list order)
     std::vector<int> boxfirst, boxcount;
+    int near_count = 0;
     // per atomzero scratch
     std::vector<Cont<R>> contlist;
     std::vector<Plane<R>> cont;
@@ -441,6 +459,12 @@ int get_contacts(const System& S, Work<R>& W, int a, float rado, double* clash)
         if (bi < 0 || bi >= dim3) continue;
         for (int k = 0; k < W.boxcount[bi]; ++k) {
             const int j = W.calclist[W.boxfirst[bi] + k];
+            if (t_caps && clash && j != a) {
+                const float ex = W.x[a] - W.x[j], ey = W.y[a] - W.y[j], ez = W.z[a] - W.z[j];
+                const double sq = (double)(ex * ex + ey * ey + ez * ez);
+                const double nd = (double)(rado + W.rad[j] + RW);
+                if (sq < nd * nd && sq != 0.0) ++W.near_count;
+            }
             if (W.done[j] == 'Y') continue;  // every atom is exposed here
             if (j != a) OPS(cull, 1);        // the atom itself never makes a contact
             const double rAB = (double)(W.rad[a] + W.rad[j]);
@@ -654,6 +678,7 @@ template <class R> int voronoi_poly(const System& S, Work<R>& W, int a, float ra
         }
         ++vn;
     }
+    CAPMAX(edges, edgenum);
     // arc points on edges that cross the sphere (LIB/Vcontacts.cpp:419-457)
     for (int e = 0; e < edgenum; ++e)
         vedge[e].arc = ((R)rado < poly[vedge[e].startpt].dist || (R)rado < poly[vedge[e].endpt].dist) ? '?' : 'X';
@@ -678,6 +703,7 @@ template <class R> int voronoi_poly(const System& S, Work<R>& W, int a, float ra
             }
         }
     }
+    CAPMAX(vn, vn);
     // keep vertices inside the sphere (LIB/Vcontacts.cpp:460-467)
     int vj = 0;
     for (int vi = 0; vi < vn; ++vi)
@@ -711,6 +737,7 @@ template <class R> int order_faces(Work<R>& W, float rado, int NC, int NV)
 {
     Plane<R>* cont = W.cont.data();
     std::vector<Vert<R>>& poly = W.poly;
+    if (t_caps) t_caps->face_slots_run = 0;
     for (int X = 0; X < NC; ++X) {
         std::vector<int>& pt = W.ptorder[X];
         pt.clear();
@@ -718,6 +745,7 @@ template <class R> int order_faces(Work<R>& W, float rado, int NC, int NV)
         for (int vi = 0; vi < NV; ++vi)
             if (poly[vi].plane[0] == X || poly[vi].plane[1] == X || poly[vi].plane[2] == X) pt.push_back(vi);
         int sc = (int)pt.size();
+        if (t_caps) t_caps->face_slots_run += sc + (sc > 2 ? 1 : 0);
         if (sc > 2) {
             for (int vi = 0; vi < sc; ++vi) {
                 if (poly[pt[vi]].plane[2] == -1) {
@@ -758,6 +786,10 @@ template <class R> int order_faces(Work<R>& W, float rado, int NC, int NV)
                         std::swap(cs[vi], cs[v2]);
                     }
         }
+    }
+    if (t_caps) {
+        CAPMAX(face_slots, t_caps->face_slots_run);
+        CAPMAX(nv_opp, NV);
     }
     return NV;
 }
@@ -932,11 +964,13 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
     std::fill(W.done.begin(), W.done.end(), 'N');
     for (auto& s : W.stubs) s.clear();
     double clash = 0.0;
+    W.near_count = 0;
     for (int i = 0; i < N; ++i) {
         const int a = W.calclist[i];
         if (a < Rn) continue;
         get_contacts(S, W, a, W.rad[a] + RW, &clash);
     }
+    CAPMAX(near_pose, W.near_count);
     if (clash >= CLASH_THRESHOLD) {
         res.wal = clash;
         res.cf = clash + 1.0;  // metal_coord = 1 in the clash cfstr (LIB/ic2cf.cpp:129)
@@ -953,6 +987,7 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
         if (a < Rn) continue;
         const float rado = W.rad[a] + RW;
         const int NC = get_contacts(S, W, a, rado, nullptr);
+        CAPMAX(nc, NC);
         if (NC == 0) continue;
         if ((int)W.cont.size() < NC + 4) W.cont.resize(NC + 4);
         if ((int)W.ptorder.size() < NC) W.ptorder.resize(NC);
@@ -1066,7 +1101,7 @@ template <class R> void init_work(const System& S, Work<R>& W)
 
 template <class R>
 int eval_batch(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm, int n_poses,
-               const double* genes, fa_cf_result* out, int n_threads, fa_opcount* ops)
+               const double* genes, fa_cf_result* out, int n_threads, fa_opcount* ops, CapStats* caps = nullptr)
 {
     if (!rec || !lig || !prm || !genes || !out || n_poses < 0) return -1;
     System S{rec, lig, prm};
@@ -1079,13 +1114,26 @@ int eval_batch(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_
         Work<R> W;
         init_work(S, W);
         fa_opcount local{};
+        CapStats lcaps{};
         t_ops = ops ? &local : nullptr;
+        t_caps = caps ? &lcaps : nullptr;
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 4)
 #endif
         for (int p = 0; p < n_poses; ++p)
             out[p] = eval_pose(S, W, genes + (size_t)p * lig->n_genes, nullptr, nullptr);
         t_ops = nullptr;
+        t_caps = nullptr;
+        if (caps) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+            {
+                int* d = &caps->near_pose;
+                const int* s0 = &lcaps.near_pose;
+                for (int k = 0; k < 6; ++k) d[k] = std::max(d[k], s0[k]);
+            }
+        }
         if (ops) {
 #ifdef _OPENMP
 #pragma omp critical
@@ -1125,6 +1173,19 @@ int fa_oracle_eval_f32(const fa_receptor_desc* rec, const fa_ligand_desc* lig, c
                        const double* genes, fa_cf_result* out, int n_threads)
 {
     return eval_batch<float>(rec, lig, prm, n_poses, genes, out, n_threads, nullptr);
+}
+
+/* Largest per-atom / per-pose working sizes over a batch (6 ints: in-range
+ * candidates per pose, contacts, hull edges, vertices before the sphere cut,
+ * vertices with opposite points, face-list slots). */
+int fa_oracle_cap_stats(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm, int n_poses,
+                        const double* genes, fa_cf_result* out, int n_threads, int* stats6)
+{
+    CapStats c{};
+    const int rc = eval_batch<double>(rec, lig, prm, n_poses, genes, out, n_threads, nullptr, &c);
+    const int v[6] = {c.near_pose, c.nc, c.edges, c.vn, c.nv_opp, c.face_slots};
+    for (int k = 0; k < 6; ++k) stats6[k] = v[k];
+    return rc;
 }
 
 /* 0: pose-build trig from the host libm (the reference binary's bits);

@@ -71,7 +71,8 @@ def test_device_is_gfx950(hip):
 def test_config0_against_oracle(ctx0, sys0, oracle_cr, oracle_libm):
     g = ctx0.eval(0, sys0.genes[0])
     ref, _ = oracle_cr
-    assert np.array_equal(g["status"], ref["status"])
+    diff = np.nonzero(g["status"] != ref["status"])[0]
+    assert diff.size == 0, [(int(i), int(g["status"][i]), int(ref["status"][i])) for i in diff[:20]]
     assert ctx0.counters()["capacity"] == 0
     ok = ref["status"] == 0
     assert ok.sum() > 500
