@@ -302,6 +302,33 @@ __device__ double spherical_arc(const double* A, double Ad, const double* B, dou
     return ro * (angBAC * (ro - Ad) - 4.0 * ro * atan(sqrt(tanprod)));
 }
 
+// out-of-line copy for the rare engulfing corrections (keeps the inlined
+// transcendental code to one site of low register pressure)
+__device__ __noinline__ double spherical_arc_call(double ax, double ay, double az, double ad, double bx, double by,
+                                                  double bz, double bd, double cx, double cy, double cz, double cd,
+                                                  float rado)
+{
+    const double A[3] = {ax, ay, az}, B[3] = {bx, by, bz}, C[3] = {cx, cy, cz};
+    return spherical_arc(A, ad, B, bd, C, cd, rado);
+}
+
+// one spherical triangle of a face fan (LIB/Vcontacts.cpp:926-940): the term
+// the reference adds for the triangle with cosines c0 (vertex 0 to q),
+// cq (q to q+1) and c1 (vertex 0 to q+1); zero when it adds nothing
+__device__ __forceinline__ double sph_triangle(double c0, double cq, double c1, double ro)
+{
+    const double U = sqrt((1 + c0) * (1 + cq) * (1 + c1) / 8.0);
+    const double V = sqrt((1 - c0) * (1 - cq) * (1 + c1) / 8.0);
+    const double Wv = sqrt((1 - c0) * (1 + cq) * (1 - c1) / 8.0);
+    const double Xv = sqrt((1 + c0) * (1 - cq) * (1 - c1) / 8.0);
+    const double tS = (1 - U + V + Wv + Xv) / (1 + U - V - Wv - Xv);
+    const double tSA = (1 - U - V - Wv + Xv) / (1 + U + V + Wv - Xv);
+    const double tSB = (1 - U - V + Wv - Xv) / (1 + U + V - Wv + Xv);
+    const double tSC = (1 - U + V - Wv - Xv) / (1 + U - V + Wv + Xv);
+    const double tp = sqrt(tS * tSA * tSB * tSC);
+    return (tp > 0.0) ? 4.0 * ro * ro * atan(sqrt(tp)) : 0.0;
+}
+
 // LIB/Vcontacts.cpp:1357-1372, sequential over planes (one lane)
 __device__ __forceinline__ bool test_point(const Smem& s, const double* p, int NC, int X)
 {
@@ -345,6 +372,16 @@ __device__ __forceinline__ int plane_at(unsigned pk, int k)
 __device__ __forceinline__ bool has_plane(unsigned pk, int X)
 {
     return (int)(pk & NONE) == X || (int)((pk >> 8) & NONE) == X || (int)((pk >> 16) & NONE) == X;
+}
+
+// consecutive face points that bound an arc of the face (LIB/Vcontacts.cpp:942-954):
+// two arc points (no third plane) on different plane pairs
+__device__ __forceinline__ int arc_pair(unsigned ka, unsigned kb)
+{
+    return (plane_at(ka, 2) == -1 && plane_at(kb, 2) == -1 &&
+            (plane_at(ka, 0) + plane_at(ka, 1)) != (plane_at(kb, 0) + plane_at(kb, 1)))
+               ? 1
+               : 0;
 }
 
 // distance of a hull vertex from the atom (the value the reference stores)
@@ -743,10 +780,12 @@ __device__ __forceinline__ double inv12(double x)
     return 1.0 / x12;
 }
 
-__device__ __forceinline__ void visit(const CullCtx& cc, const Cand& c, CullAcc& acc)
+// Tests one candidate: counts, clash term, and whether it is in contact
+// range (sq returns the squared distance as the reference computes it).
+__device__ __forceinline__ bool cand_test(const CullCtx& cc, const Cand& c, CullAcc& acc, float& sq)
 {
     bool in = false;
-    float sq = 0.0f;
+    sq = 0.0f;
     if (c.j >= 0 && c.j != cc.acalc) {
         acc.total += 1;
         const float dx = cc.ax - c.x, dy = cc.ay - c.y, dz = cc.az - c.z;
@@ -770,6 +809,12 @@ __device__ __forceinline__ void visit(const CullCtx& cc, const Cand& c, CullAcc&
             }
         }
     }
+    return in;
+}
+
+// Appends the lanes with `in` set to the neighbour list, in lane order.
+__device__ __forceinline__ void cand_emit(const CullCtx& cc, const Cand& c, bool in, float sq, CullAcc& acc)
+{
     const unsigned long long mk = __ballot(in);
     if (in) {
         const int pos = acc.n_near + lanes_below(mk);
@@ -780,6 +825,13 @@ __device__ __forceinline__ void visit(const CullCtx& cc, const Cand& c, CullAcc&
     }
     acc.n_near += __popcll(mk);
     if (acc.n_near > cc.cap) acc.over = true;
+}
+
+__device__ __forceinline__ void visit(const CullCtx& cc, const Cand& c, CullAcc& acc)
+{
+    float sq;
+    const bool in = cand_test(cc, c, acc, sq);
+    cand_emit(cc, c, in, sq, acc);
 }
 
 // Segment table of one atom's stream (lanes 0..26 hold slot 0..26).
@@ -869,46 +921,139 @@ __device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig,
     }
 }
 
-// grid widened by this pose (rare): the receptor cells are not precomputed,
-// so every receptor atom is binned on the fly, slot by slot
+// Ligand atoms of cell sb (ranks [lo, lo + nl) of the box order), visited.
+__device__ __forceinline__ void visit_ligand_cell(const Smem& s, const DevLigand& lig, int Rn, int L, int sb,
+                                                  const CullCtx& cc, CullAcc& acc)
+{
+    int lo = 0, nl = 0;
+    for (int q = 0; q < L; ++q) {
+        const int b = s.sbox[q];
+        lo += b < sb;
+        nl += b == sb;
+    }
+    for (int b = 0; b < nl; b += WAVE) {
+        const int r = b + lane_id();
+        Cand c{0.f, 0.f, 0.f, 0.f, -1, 0, false};
+        if (r < nl) {
+            const int m = s.order[lo + r];
+            c.m = m;
+            c.x = s.lx[m];
+            c.y = s.ly[m];
+            c.z = s.lz[m];
+            c.r = lig.radius[m];
+            c.j = Rn + m;
+            c.lj = true;
+        }
+        visit(cc, c, acc);
+    }
+}
+
+// neighbourhood slot (0..26) naming cell bz + d, -1 if none; with dim >= 3
+// at most one (dx, dy, dz) gives the offset
+__device__ __forceinline__ int slot_of(const PoseGrid& pg, int d)
+{
+    int slot = -1;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int r = d - pg.dim2 * dx - pg.dim * dy;
+            if (r >= -1 && r <= 1) slot = (dx + 1) * 9 + (dy + 1) * 3 + (r + 1);
+        }
+    return slot;
+}
+
+constexpr int SLOW_TMP = 3 * WAVE;  // in-range receptor atoms of one ligand atom (slow path)
+
+// Grid widened by this pose: the receptor cells are not precomputed.  One
+// streamed pass bins every receptor atom on the fly and tests the ones in
+// the 27 cells (their in-range entries parked, with their slot, at the tail
+// of the wave's neighbour buffer); a second pass emits them slot by slot,
+// each slot's ligand atoms after its receptor atoms, as the reference scans.
 __device__ void cull_slow(const Smem& s, const DevReceptor& R, const DevLigand& lig, const PoseGrid& pg, int L,
-                          const SlotTab& tab, const CullCtx& cc, CullAcc& acc)
+                          const SlotTab& tab, int bz, const CullCtx& cc, CullAcc& acc)
+{
+    const int l = lane_id();
+    const int Rn = R.n;
+    int2* const tidx = cc.nidx + cc.cap;   // SLOW_TMP entries past the list capacity
+    float4* const txyz = cc.nxyz + cc.cap;
+    int ntmp = 0;
+    float4 nxt = (l < Rn) ? R.xyzr[l] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int base = 0; base < Rn; base += WAVE) {
+        const int t = base + l;
+        const float4 p = nxt;
+        if (base + WAVE < Rn) nxt = (t + WAVE < Rn) ? R.xyzr[t + WAVE] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int slot = (t < Rn) ? slot_of(pg, box_of(pg, p.x, p.y, p.z) - bz) : -1;
+        if (__ballot(slot >= 0) == 0ull) continue;
+        Cand c{p.x, p.y, p.z, p.w, slot >= 0 ? t : -1, 0, false};
+        float sq;
+        const bool in = cand_test(cc, c, acc, sq);
+        const unsigned long long mk = __ballot(in);
+        if (in) {
+            const int pos = ntmp + lanes_below(mk);
+            if (pos < SLOW_TMP) {
+                tidx[pos] = make_int2(t | (slot << 26), __float_as_int(sq));
+                txyz[pos] = p;
+            }
+        }
+        ntmp += __popcll(mk);
+    }
+    if (ntmp > SLOW_TMP) {
+        acc.over = true;
+        ntmp = SLOW_TMP;
+    }
+    wave_sync();  // parked entries visible to every lane
+    int2 ti[SLOW_TMP / WAVE];
+    float4 tx[SLOW_TMP / WAVE];
+    int tslot[SLOW_TMP / WAVE];
+#pragma unroll
+    for (int q = 0; q < SLOW_TMP / WAVE; ++q) {
+        const int e = q * WAVE + l;
+        ti[q] = make_int2(-1, 0);
+        tx[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        tslot[q] = -1;
+        if (e < ntmp) {
+            ti[q] = tidx[e];
+            tx[q] = txyz[e];
+            tslot[q] = ti[q].x >> 26;
+            ti[q].x &= (1 << 26) - 1;
+        }
+    }
+    for (int slot = 0; slot < 27; ++slot) {
+        const int sb = readlane_i(tab.bi, slot);
+        if (sb < 0) continue;
+#pragma unroll
+        for (int q = 0; q < SLOW_TMP / WAVE; ++q) {
+            const bool mine = tslot[q] == slot;
+            if (__ballot(mine) == 0ull) continue;
+            const Cand c{tx[q].x, tx[q].y, tx[q].z, tx[q].w, ti[q].x, 0, false};
+            cand_emit(cc, c, mine, __int_as_float(ti[q].y), acc);
+        }
+        visit_ligand_cell(s, lig, Rn, L, sb, cc, acc);
+    }
+}
+
+// Coarse grids (fewer than three cells a side) name a cell from several
+// slots; there every slot is scanned over all receptor atoms.
+__device__ void cull_slow_coarse(const Smem& s, const DevReceptor& R, const DevLigand& lig, const PoseGrid& pg, int L,
+                                 const SlotTab& tab, const CullCtx& cc, CullAcc& acc)
 {
     const int l = lane_id();
     const int Rn = R.n;
     for (int slot = 0; slot < 27; ++slot) {
         const int sb = readlane_i(tab.bi, slot);
         if (sb < 0) continue;
-        for (int base = 0; base < Rn + L; base += WAVE) {
+        for (int base = 0; base < Rn; base += WAVE) {
             const int t = base + l;
             Cand c{0.f, 0.f, 0.f, 0.f, -1, 0, false};
             bool v = false;
             if (t < Rn) {
                 const float4 p = R.xyzr[t];
                 v = box_of(pg, p.x, p.y, p.z) == sb;
-                if (v) {
-                    c.x = p.x;
-                    c.y = p.y;
-                    c.z = p.z;
-                    c.r = p.w;
-                    c.j = t;
-                }
-            } else if (t < Rn + L) {
-                const int m = t - Rn;
-                v = s.sbox[s.rank[m]] == sb;
-                if (v) {
-                    c.m = m;
-                    c.x = s.lx[m];
-                    c.y = s.ly[m];
-                    c.z = s.lz[m];
-                    c.r = lig.radius[m];
-                    c.j = t;
-                    c.lj = true;
-                }
+                if (v) c = Cand{p.x, p.y, p.z, p.w, t, 0, false};
             }
             if (__ballot(v) == 0ull) continue;
             visit(cc, c, acc);
         }
+        visit_ligand_cell(s, lig, Rn, L, sb, cc, acc);
     }
 }
 
@@ -1019,13 +1164,15 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 cc.permea = (double)prm.permeability;
                 cc.nidx = nidx;
                 cc.nxyz = nxyz;
-                cc.cap = B.near_cap;
+                cc.cap = B.near_cap - SLOW_TMP;  // the tail parks slow-path entries
                 if (l == 0) s.nstart[k] = acc.n_near;
                 acc.total = acc.tested = acc.found = 0;
-                if (pg.slow)
-                    cull_slow(s, R, lig, pg, L, tab, cc, acc);
-                else
+                if (!pg.slow)
                     cull_fast(s, G, lig, Rn, L, tab, cc, acc);
+                else if (pg.dim >= 3)
+                    cull_slow(s, R, lig, pg, L, tab, s.sbox[k], cc, acc);
+                else
+                    cull_slow_coarse(s, R, lig, pg, L, tab, cc, acc);
                 const int total = wave_sum_i(acc.total);
                 const int tested = wave_sum_i(acc.tested);
                 const int found = wave_sum_i(acc.found);
@@ -1796,70 +1943,117 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     auto vdist = [&](int w) {
                         return (projected || plane_at(s.vpl[w], 2) == -1) ? ro : vnorm(s, w);
                     };
-                    int ntri = 0, narc = 0;
+                    // Jobs: the spherical triangles of each face's fan (NP - 2 of them)
+                    // and its boundary arcs are evaluated one per lane into u.fcos;
+                    // each face's lane then sums its own terms in the reference's
+                    // order (triangles, then arcs), so the sums are bit-identical.
+                    int ntq[PSL], naq[PSL], toff[PSL], aoff[PSL];
+                    int NT = 0, NA = 0;
+#pragma unroll
+                    for (int q = 0; q < PSL; ++q) {
+                        const int X = l + WAVE * q;
+                        ntq[q] = naq[q] = 0;
+                        if (X < NC && pfl[q] != 'X') {
+                            const int NP = s.fcnt[X];
+                            if (NP == 2) {
+                                naq[q] = 1;
+                            } else if (NP > 2) {
+                                ntq[q] = NP - 2;
+                                const short* pt = &s.fpt[s.foff[X]];
+                                for (int qq = 0; qq < NP; ++qq)
+                                    naq[q] += arc_pair(s.vpl[pt[qq]], s.vpl[pt[(qq + 1) % NP]]);
+                            }
+                        }
+                        toff[q] = NT + wave_excl_scan(ntq[q]);
+                        aoff[q] = NA + wave_excl_scan(naq[q]);
+                        NT += wave_sum_i(ntq[q]);
+                        NA += wave_sum_i(naq[q]);
+                    }
+                    count_op(s.ops, OP_TRI, NT);
+                    count_op(s.ops, OP_SARC, NA);
+                    wave_sync();  // u.fcos is free again
+                    // face and index of job t in a list laid out by (toff, ntq)
+                    auto job_face = [&](int base, int t, const int* off, const int* cnt, int& X, int& u) {
+                        X = -1;
+                        u = 0;
+#pragma unroll
+                        for (int q = 0; q < PSL; ++q) {
+                            unsigned long long ov = __ballot(cnt[q] > 0 && off[q] < base + WAVE && off[q] + cnt[q] > base);
+                            while (ov) {
+                                const int fl = first_lane(ov);
+                                ov &= ov - 1;
+                                const int o_ = readlane_i(off[q], fl), n_ = readlane_i(cnt[q], fl);
+                                if (t >= o_ && t < o_ + n_) {
+                                    X = WAVE * q + fl;
+                                    u = t - o_;
+                                }
+                            }
+                        }
+                    };
+                    for (int base = 0; base < NT; base += WAVE) {
+                        const int t = base + l;
+                        int X, u;
+                        job_face(base, t, toff, ntq, X, u);
+                        if (t < NT) {
+                            const short* pt = &s.fpt[s.foff[X]];
+                            const int w0 = pt[0], wq = pt[u + 1], wn = pt[u + 2];
+                            const double x0 = s.vx[w0], y0 = s.vy[w0], z0 = s.vz[w0], d0 = vdist(w0);
+                            const double dq = vdist(wq), dn = vdist(wn);
+                            const double c0 = (x0 * s.vx[wq] + y0 * s.vy[wq] + z0 * s.vz[wq]) / (d0 * dq);
+                            const double cq = (s.vx[wq] * s.vx[wn] + s.vy[wq] * s.vy[wn] + s.vz[wq] * s.vz[wn]) / (dq * dn);
+                            const double c1 = (x0 * s.vx[wn] + y0 * s.vy[wn] + z0 * s.vz[wn]) / (d0 * dn);
+                            s.u.fcos[t] = sph_triangle(c0, cq, c1, ro);
+                        }
+                    }
+                    wave_sync();
+#pragma unroll
+                    for (int q = 0; q < PSL; ++q)
+                        for (int u = 0; u < ntq[q]; ++u) par[q] += s.u.fcos[toff[q] + u];
+                    wave_sync();
+                    for (int base = 0; base < NA; base += WAVE) {
+                        const int t = base + l;
+                        int X, u;
+                        job_face(base, t, aoff, naq, X, u);
+                        if (t < NA) {
+                            const short* pt = &s.fpt[s.foff[X]];
+                            const int NP = s.fcnt[X];
+                            int wa = pt[0], wb = pt[1];
+                            if (NP > 2) {  // the u-th qualifying boundary arc
+                                int k2 = -1;
+                                for (int qq = 0; qq < NP && k2 < u; ++qq) {
+                                    wa = pt[qq];
+                                    wb = pt[(qq + 1) % NP];
+                                    k2 += arc_pair(s.vpl[wa], s.vpl[wb]);
+                                }
+                            }
+                            double pX[4];
+                            plane_of(s, X, pX);
+                            const double ctr[3] = {-pX[0] * pX[3], -pX[1] * pX[3], -pX[2] * pX[3]};
+                            const double Pa[3] = {s.vx[wa], s.vy[wa], s.vz[wa]};
+                            const double Pb[3] = {s.vx[wb], s.vy[wb], s.vz[wb]};
+                            s.u.fcos[t] = spherical_arc(ctr, fabs(pX[3]), Pa, vdist(wa), Pb, vdist(wb), rado);
+                        }
+                    }
+                    wave_sync();
 #pragma unroll
                     for (int q = 0; q < PSL; ++q) {
                         const int X = l + WAVE * q;
                         if (X >= NC || pfl[q] == 'X') continue;
                         const int NP = s.fcnt[X];
-                        const short* pt = &s.fpt[s.foff[X]];
                         const double ctr[3] = {-pa0[q] * pa3[q], -pa1[q] * pa3[q], -pa2[q] * pa3[q]};
                         const double cd = fabs(pa3[q]);
-                        double ar = 0.0;
                         if (NP == 0) {
-                            if (test_point(s, ctr, NC, X)) ar = (2.0f * PI_REF) * ro * (ro - cd);
+                            if (test_point(s, ctr, NC, X)) par[q] = (2.0f * PI_REF) * ro * (ro - cd);
                         } else if (NP == 2) {
-                            const double Pa[3] = {s.vx[pt[0]], s.vy[pt[0]], s.vz[pt[0]]};
-                            const double Pb[3] = {s.vx[pt[1]], s.vy[pt[1]], s.vz[pt[1]]};
-                            const double arc = spherical_arc(ctr, cd, Pa, vdist(pt[0]), Pb, vdist(pt[1]), rado);
-                            ++narc;
+                            const double arc = s.u.fcos[aoff[q]];
                             if (test_point(s, ctr, NC, X))
-                                ar = (2.0f * PI_REF) * ro * (ro - cd) - arc;
+                                par[q] = (2.0f * PI_REF) * ro * (ro - cd) - arc;
                             else
-                                ar = arc;
+                                par[q] = arc;
                         } else {
-                            const int w0 = pt[0];
-                            const double x0 = s.vx[w0], y0 = s.vy[w0], z0 = s.vz[w0], d0 = vdist(w0);
-                            if (NP > 2) ntri += NP - 2;
-                            // walk the fan: cos to vertex 0 and to the next vertex
-                            int wq = (NP > 1) ? pt[1] : w0;
-                            double dq = vdist(wq);
-                            double c0 = (NP > 2) ? (x0 * s.vx[wq] + y0 * s.vy[wq] + z0 * s.vz[wq]) / (d0 * dq) : 0.0;
-                            for (int qq = 1; qq < NP - 1; ++qq) {
-                                const int wn = pt[qq + 1];
-                                const double dn = vdist(wn);
-                                const double cq = (s.vx[wq] * s.vx[wn] + s.vy[wq] * s.vy[wn] + s.vz[wq] * s.vz[wn]) / (dq * dn);
-                                const double c1 = (x0 * s.vx[wn] + y0 * s.vy[wn] + z0 * s.vz[wn]) / (d0 * dn);
-                                const double U = sqrt((1 + c0) * (1 + cq) * (1 + c1) / 8.0);
-                                const double V = sqrt((1 - c0) * (1 - cq) * (1 + c1) / 8.0);
-                                const double Wv = sqrt((1 - c0) * (1 + cq) * (1 - c1) / 8.0);
-                                const double Xv = sqrt((1 + c0) * (1 - cq) * (1 - c1) / 8.0);
-                                const double tS = (1 - U + V + Wv + Xv) / (1 + U - V - Wv - Xv);
-                                const double tSA = (1 - U - V - Wv + Xv) / (1 + U + V + Wv - Xv);
-                                const double tSB = (1 - U - V + Wv - Xv) / (1 + U + V - Wv + Xv);
-                                const double tSC = (1 - U + V - Wv - Xv) / (1 + U - V + Wv + Xv);
-                                const double tp = sqrt(tS * tSA * tSB * tSC);
-                                if (tp > 0.0) ar += 4.0 * ro * ro * atan(sqrt(tp));
-                                c0 = c1;
-                                wq = wn;
-                                dq = dn;
-                            }
-                            for (int qq = 0; qq < NP; ++qq) {
-                                const int wa = pt[qq], wb = pt[(qq + 1) % NP];
-                                const unsigned ka = s.vpl[wa], kb2 = s.vpl[wb];
-                                if (plane_at(ka, 2) == -1 && plane_at(kb2, 2) == -1 &&
-                                    (plane_at(ka, 0) + plane_at(ka, 1)) != (plane_at(kb2, 0) + plane_at(kb2, 1))) {
-                                    const double Pa[3] = {s.vx[wa], s.vy[wa], s.vz[wa]};
-                                    const double Pb[3] = {s.vx[wb], s.vy[wb], s.vz[wb]};
-                                    ar += spherical_arc(ctr, cd, Pa, vdist(wa), Pb, vdist(wb), rado);
-                                    ++narc;
-                                }
-                            }
+                            for (int u = 0; u < naq[q]; ++u) par[q] += s.u.fcos[aoff[q] + u];
                         }
-                        par[q] = ar;
                     }
-                    count_op(s.ops, OP_TRI, wave_sum_i(ntri));
-                    count_op(s.ops, OP_SARC, wave_sum_i(narc));
                     if (projected) {
                         // corrections onto the planes next to engulfing faces, sequential
                         wave_sync();
@@ -1898,9 +2092,9 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                                     else
                                         continue;
                                     if (comm != -1 && s.pflag[comm] != 'E') {
-                                        const double Pa[3] = {s.vx[cu], s.vy[cu], s.vz[cu]};
-                                        const double Pb[3] = {s.vx[nx], s.vy[nx], s.vz[nx]};
-                                        s.u.area[comm] += spherical_arc(ctr, cd, Pa, ro, Pb, ro, rado);
+                                        s.u.area[comm] += spherical_arc_call(ctr[0], ctr[1], ctr[2], cd, s.vx[cu], s.vy[cu],
+                                                                             s.vz[cu], ro, s.vx[nx], s.vy[nx], s.vz[nx],
+                                                                             ro, rado);
                                         s.ops[OP_SARC] += 1;
                                         if (NP == 2) break;
                                     }
@@ -1918,8 +2112,10 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                                 } else {
                                     const double c0[3] = {-a0[0] * a0[3], -a0[1] * a0[3], -a0[2] * a0[3]};
                                     const double c1[3] = {-a1[0] * a1[3], -a1[1] * a1[3], -a1[2] * a1[3]};
-                                    double mx = spherical_arc(c0, fabs(a0[3]), b, ro, c, ro, rado);
-                                    mx += spherical_arc(c1, fabs(a1[3]), b, ro, c, ro, rado);
+                                    double mx = spherical_arc_call(c0[0], c0[1], c0[2], fabs(a0[3]), b[0], b[1], b[2], ro,
+                                                                   c[0], c[1], c[2], ro, rado);
+                                    mx += spherical_arc_call(c1[0], c1[1], c1[2], fabs(a1[3]), b[0], b[1], b[2], ro, c[0],
+                                                             c[1], c[2], ro, rado);
                                     s.ops[OP_SARC] += 2;
                                     s.u.area[eng[0]] = (2.0f * PI_REF) * ro * ro - 0.5 * mx;
                                     s.u.area[eng[1]] = (2.0f * PI_REF) * ro * ro - 0.5 * mx;
