@@ -45,7 +45,7 @@ def cpu_baseline(system, lig_index, genes, budget_s=12.0, threads=16):
     done, t0 = 0, time.perf_counter()
     chunk = min(len(genes), 2000)
     k = 0
-    while time.perf_counter() - t0 < budget_s and done < 200000:
+    while time.perf_counter() - t0 < budget_s:
         g = genes[(k * chunk) % len(genes):][:chunk]
         oracle_lib.oracle_eval(lib, desc, lig_index, g, n_threads=threads)
         done += len(g)
@@ -57,6 +57,50 @@ def cpu_baseline(system, lig_index, genes, budget_s=12.0, threads=16):
                       f"{threads} OpenMP threads, {dt:.1f} s"}
 
 
+def population_sweep():
+    """Launch time against poses per launch (plain config-1 workload): one
+    pose alone shows the single-pose latency, large launches the throughput
+    once every resident wave has several poses."""
+    import torch
+    from flexaidds_amd import abi, hip, synth
+    system = synth.make_config1(pop=16384)
+    lig = system.ligands[0]
+    genes = synth.random_genes(np.random.default_rng(7), lig, 16384)
+    ctx = hip.Context(max_pop=16384)
+    ctx.upload_system(system)
+    d_genes = torch.from_numpy(genes).cuda()
+    d_out = torch.zeros(16384 * abi.CF_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    st = torch.cuda.current_stream()
+    rows = []
+    for n in (1, 16, 256, 1024, 2048, 4096, 8192, 16384):
+        ctx.eval_device(n, d_genes.data_ptr(), lig.n_genes, 0, d_out.data_ptr(), st.cuda_stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(3):
+            ctx.eval_device(n, d_genes.data_ptr(), lig.n_genes, 0, d_out.data_ptr(), st.cuda_stream)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 3
+        rows.append({"poses": n, "ms": round(ms, 4), "poses_per_s": round(n / ms * 1e3, 1)})
+        log(json.dumps(rows[-1]))
+    print(json.dumps({"sweep": rows, "launch_waves": ctx.launch_waves()}), flush=True)
+    ctx.close()
+
+
+def measured_traffic(pop):
+    """HBM bytes per launch of the pose kernel for this workload, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated on a copy
+    kernel), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "cf_pose_kernel_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return rec.get("bytes_per_launch") if rec.get("workload") == f"plain config-1 generation, {pop} poses" else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +110,10 @@ def main():
     ap.add_argument("--batched-ligands", type=int, default=32)
     ap.add_argument("--batched-pop", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep", action="store_true",
+                    help="only time one launch per population size (1 .. 16,384 poses of the plain workload) and exit")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (calibrates rocprofv3 byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -77,6 +125,18 @@ def main():
     from flexaidds_amd import abi, hip, synth
 
     torch.cuda.set_device(0)
+    if args.copy_probe > 0:
+        n = args.copy_probe * (1 << 20)
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "copies": 5}), flush=True)
+        return
+    if args.sweep:
+        population_sweep()
+        return
     log("building config-1 workload")
     system = synth.make_config1(pop=args.pop)
     lig = system.ligands[0]
@@ -189,7 +249,7 @@ def main():
                    "receptor_atoms": system.receptor.n_atoms, "ligand_atoms": lig.n_atoms, "genes": G,
                    "population": args.pop},
         "roofline": {"bound": "valu", "achieved": round(achieved, 5), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 7), "traffic": None,
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 7), "traffic": measured_traffic(args.pop),
                      "flop_per_launch": int(flop_total / args.steps),
                      "launch_ms_mean": round(float(launch_ms.mean()), 4)},
         "cpu_baseline": cpu,

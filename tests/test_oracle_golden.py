@@ -1,0 +1,61 @@
+"""Oracle against the golden fixtures written by the reference built from
+source (tests/golden/make_golden.py).  The oracle's pose-build trig mode 0
+calls the host libm's cosf/sinf like the reference binary, so everything is
+compared bit for bit: statuses, cfstr terms, ligand coordinates and the
+contact records (area and distance of every directed pair)."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from flexaidds_amd import abi, synth
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "ref_*.npz")))
+
+
+def _system(config):
+    if config == 0:
+        return synth.make_config0(n_poses=1000)
+    return synth.make_config2(n_ligands=4, pop=64)
+
+
+@pytest.fixture(scope="module")
+def systems():
+    return {}
+
+
+def test_fixtures_present():
+    assert len(FIXTURES) >= 5
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
+def test_oracle_matches_reference_fixture(path, oracle, systems):
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    from make_golden import system_digest
+    g = np.load(path, allow_pickle=False)
+    config, li = int(g["config"]), int(g["lig_index"])
+    if config not in systems:
+        systems[config] = _system(config)
+    s = systems[config]
+    assert system_digest(s, li) == str(g["digest"]), "synthetic generator drifted from the fixture"
+    desc = abi.Descriptors(s.receptor, s.ligands, s.params)
+    genes = g["genes"]
+    with ol.trig_mode(oracle, 0):
+        out = ol.oracle_eval(oracle, desc, li, genes)
+        ref = g["out"]
+        assert np.array_equal(out["status"], ref["status"])
+        for f in ("com", "wal", "sas", "con", "totsas", "cf", "app", "rclash"):
+            assert np.array_equal(out[f], ref[f]), f
+        n_rec = int(g["rec_pose"].max()) + 1 if len(g["rec_pose"]) else 0
+        for p in range(len(genes)):
+            o, xyz, orec = ol.oracle_pose(oracle, desc, li, genes[p])
+            assert np.array_equal(xyz, g["xyz"][p]), p
+            if p < n_rec:
+                mine = sorted((int(r["from"]), int(r["atom"]), float(r["area"]), float(r["dist"])) for r in orec)
+                rows = g["rec"][g["rec_pose"] == p]
+                theirs = sorted((int(r["from"]), int(r["atom"]), float(r["area"]), float(r["dist"])) for r in rows)
+                assert mine == theirs, p
