@@ -105,28 +105,59 @@ __device__ __forceinline__ int lanes_below(unsigned long long m)
 
 __device__ __forceinline__ int first_lane(unsigned long long m) { return __ffsll((long long)m) - 1; }
 
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    // lanes whose source is outside their row read 0
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Wave reductions and scans: DPP inside each 16-lane row (quad_perm,
+// row_half_mirror, row_mirror butterflies; row_shr for scans), then the four
+// row results through readlane.  All lanes must be active.
 __device__ __forceinline__ int wave_sum_i(int v)
 {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
+    v += dpp_i<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_i<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_i<0x141>(v);  // row_half_mirror
+    v += dpp_i<0x140>(v);  // row_mirror
+    return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) + __builtin_amdgcn_readlane(v, 32) +
+           __builtin_amdgcn_readlane(v, 48);
 }
+
+__device__ __forceinline__ double readlane_d(double v, int lane);
 
 __device__ __forceinline__ double wave_sum_d(double v)
 {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
-__device__ __forceinline__ int wave_excl_scan(int v)
+__device__ __forceinline__ int wave_incl_scan(int v)
 {
-    int x = v;
-    const int l = lane_id();
-    for (int off = 1; off < WAVE; off <<= 1) {
-        int y = __shfl_up(x, off);
-        if (l >= off) x += y;
-    }
-    return x - v;
+    v += dpp_i<0x111>(v);  // row_shr:1
+    v += dpp_i<0x112>(v);  // row_shr:2
+    v += dpp_i<0x114>(v);  // row_shr:4
+    v += dpp_i<0x118>(v);  // row_shr:8
+    const int r0 = __builtin_amdgcn_readlane(v, 15), r1 = __builtin_amdgcn_readlane(v, 31);
+    const int r2 = __builtin_amdgcn_readlane(v, 47);
+    const int l = __lane_id();
+    return v + (l >= 16 ? r0 : 0) + (l >= 32 ? r1 : 0) + (l >= 48 ? r2 : 0);
 }
+
+__device__ __forceinline__ int wave_excl_scan(int v) { return wave_incl_scan(v) - v; }
 
 // LDS visibility point for the whole (single-wave) work-group
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
@@ -504,6 +535,104 @@ __device__ __forceinline__ void edge_dir(const Smem& s, int p0, int p1, int tp, 
         V1 = -V1;
         V2 = -V2;
     }
+}
+
+// One step of the hull walk (LIB/Vcontacts.cpp:271-335) for edge e: the
+// search over the contact planes (lane-wise quotients, DPP minimum), then the
+// bounding planes when the point leaves the sphere, and the new vertex.
+struct HullProbe {
+    int A, B, oldC, C;
+    double dx, dy, dz, px, py, pz;
+    double vtmin;
+    int tests;
+};
+
+struct HullStep {
+    int A, B, C;
+    double nx, ny, nz;
+    int tests;
+};
+
+__device__ __forceinline__ HullProbe hull_search(const Smem& s, const Edges& E, int e, int NC, const double* pa0,
+                                                 const double* pa1, const double* pa2, const double* pa3)
+{
+    HullProbe h;
+    unsigned epl, evt;
+    edge_get(E, e, epl, evt);
+    h.A = (int)(epl & NONE);
+    h.B = (int)((epl >> 8) & NONE);
+    h.oldC = (int)((epl >> 16) & NONE);
+    const int sv = (int)(evt & 0xFFFFu);
+    h.px = s.vx[sv];
+    h.py = s.vy[sv];
+    h.pz = s.vz[sv];
+    edge_dir(s, h.A, h.B, h.oldC, h.px, h.py, h.pz, h.dx, h.dy, h.dz);
+    // closest plane cut ahead along the edge: lane-wise quotients, DPP minimum
+    const int l = lane_id();
+    uint64_t kq[PSL];
+    uint64_t kb = ~0ull;
+#pragma unroll
+    for (int q = 0; q < PSL; ++q) {
+        kq[q] = ~0ull;
+        if (q * WAVE < NC) {  // wave-uniform: skip empty slots
+            const int c = l + WAVE * q;
+            if (c < NC && c != h.A && c != h.B && c != h.oldC) {
+                const double vtdiv = pa0[q] * h.dx + pa1[q] * h.dy + pa2[q] * h.dz;
+                if (vtdiv != 0.0) {
+                    const double vt = -(pa0[q] * h.px + pa1[q] * h.py + pa2[q] * h.pz + pa3[q]) / vtdiv;
+                    if (vt < BIG && vt > 0.0) kq[q] = key_of(vt);
+                }
+            }
+            kb = umin64(kb, kq[q]);
+        }
+    }
+    h.tests = NC - (h.A < NC) - (h.B < NC) - (h.oldC < NC);
+    const uint64_t kmin = wave_min_u64(kb);
+    h.C = -1;
+    h.vtmin = BIG;
+    if (kmin != ~0ull) {
+        h.vtmin = __longlong_as_double((long long)kmin);
+#pragma unroll
+        for (int q = 0; q < PSL; ++q) {
+            const unsigned long long m = __ballot(kq[q] == kmin);
+            if (h.C < 0 && m) h.C = WAVE * q + first_lane(m);
+        }
+    }
+    return h;
+}
+
+__device__ __forceinline__ HullStep hull_finish(const HullProbe& h, int NC, double rr)
+{
+    HullStep w;
+    w.A = h.A;
+    w.B = h.B;
+    w.C = h.C;
+    w.tests = h.tests;
+    double vtmin = h.vtmin;
+    w.nx = h.px + vtmin * h.dx;
+    w.ny = h.py + vtmin * h.dy;
+    w.nz = h.pz + vtmin * h.dz;
+    if (w.nx * w.nx + w.ny * w.ny + w.nz * w.nz > rr) {
+        w.tests += 4 - (h.A >= NC) - (h.B >= NC) - (h.oldC >= NC);
+        for (int c = NC; c < NC + 4; ++c) {
+            if (c != h.A && c != h.B && c != h.oldC) {
+                double b[4];
+                bounding_plane(c - NC, b);
+                const double vtdiv = b[0] * h.dx + b[1] * h.dy + b[2] * h.dz;
+                if (vtdiv != 0.0) {
+                    const double vt = -(b[0] * h.px + b[1] * h.py + b[2] * h.pz + b[3]) / vtdiv;
+                    if (vt < vtmin && vt > 0.0) {
+                        vtmin = vt;
+                        w.C = c;
+                    }
+                }
+            }
+        }
+        w.nx = h.px + vtmin * h.dx;
+        w.ny = h.py + vtmin * h.dy;
+        w.nz = h.pz + vtmin * h.dz;
+    }
+    return w;
 }
 
 __device__ __forceinline__ void put_vertex(Smem& s, int vn, double x, double y, double z, int a, int b, int c)
@@ -912,12 +1041,25 @@ __device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig,
         return c;
     };
     if (T <= 0) return;
-    Cand cur = fetch(0);
-    for (int base = 0; base < T; base += WAVE) {
-        Cand nxt{0.f, 0.f, 0.f, 0.f, -1, 0, false};
-        if (base + WAVE < T) nxt = fetch(base + WAVE);  // in flight while this chunk is tested
-        visit(cc, cur, acc);
-        cur = nxt;
+    // four chunks in flight: the loads of the next ones are issued before a
+    // chunk is tested, so an atom waits about once for L2 instead of per chunk
+    const Cand none{0.f, 0.f, 0.f, 0.f, -1, 0, false};
+    Cand c0 = fetch(0), c1 = none, c2 = none, c3 = none;
+    if (WAVE < T) c1 = fetch(WAVE);
+    if (2 * WAVE < T) c2 = fetch(2 * WAVE);
+    if (3 * WAVE < T) c3 = fetch(3 * WAVE);
+    for (int base = 0; base < T; base += 4 * WAVE) {
+        visit(cc, c0, acc);
+        if (base + 4 * WAVE < T) c0 = fetch(base + 4 * WAVE);
+        if (base + WAVE >= T) break;
+        visit(cc, c1, acc);
+        if (base + 5 * WAVE < T) c1 = fetch(base + 5 * WAVE);
+        if (base + 2 * WAVE >= T) break;
+        visit(cc, c2, acc);
+        if (base + 6 * WAVE < T) c2 = fetch(base + 6 * WAVE);
+        if (base + 3 * WAVE >= T) break;
+        visit(cc, c3, acc);
+        if (base + 7 * WAVE < T) c3 = fetch(base + 7 * WAVE);
     }
 }
 
@@ -1070,7 +1212,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
         // dynamic work queue: pose costs vary a lot (clash exits, hull sizes)
         int pose = 0;
         if (l == 0) pose = (int)atomicAdd(&B.counters[2], 1u);
-        pose = __shfl(pose, 0);
+        pose = __builtin_amdgcn_readfirstlane(pose);
         if (pose >= B.n_poses) break;
         const int li = B.pose_lig ? B.pose_lig[pose] : 0;
         const DevLigand lig = ligs[li];
@@ -1484,88 +1626,24 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 while (true) {
                     const int se = first_open(E);
                     if (se < 0) break;
-                    unsigned epl, evt;
-                    edge_get(E, se, epl, evt);
-                    const int A = (int)(epl & NONE), Bp = (int)((epl >> 8) & NONE);
-                    const int oldC = (int)((epl >> 16) & NONE);
-                    const int sv = (int)(evt & 0xFFFFu);
-                    struct {
-                        double dx, dy, dz, px, py, pz;
-                    } ev;
-                    ev.px = s.vx[sv];
-                    ev.py = s.vy[sv];
-                    ev.pz = s.vz[sv];
-                    edge_dir(s, A, Bp, oldC, ev.px, ev.py, ev.pz, ev.dx, ev.dy, ev.dz);
-                    // closest plane cut ahead along the edge
-                    double vq[PSL];
-                    uint64_t kb = ~0ull;
-#pragma unroll
-                    for (int q = 0; q < PSL; ++q) {
-                        vq[q] = BIG;
-                        if (q * WAVE < NC) {  // wave-uniform: skip empty slots
-                            const int c = l + WAVE * q;
-                            if (c < NC && c != A && c != Bp && c != oldC) {
-                                const double vtdiv = pa0[q] * ev.dx + pa1[q] * ev.dy + pa2[q] * ev.dz;
-                                if (vtdiv != 0.0) {
-                                    const double vt =
-                                        -(pa0[q] * ev.px + pa1[q] * ev.py + pa2[q] * ev.pz + pa3[q]) / vtdiv;
-                                    if (vt < BIG && vt > 0.0) {
-                                        vq[q] = vt;
-                                        kb = umin64(kb, key_of(vt));
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    hull_tests += NC - (A < NC) - (Bp < NC) - (oldC < NC);
-                    const uint64_t kmin = wave_min_u64(kb);
-                    int C = -1;
-                    double vtmin = BIG;
-                    if (kmin != ~0ull) {
-                        vtmin = __longlong_as_double((long long)kmin);
-#pragma unroll
-                        for (int q = 0; q < PSL; ++q) {
-                            const unsigned long long m = __ballot(vq[q] < BIG && key_of(vq[q]) == kmin);
-                            if (C < 0 && m) C = WAVE * q + first_lane(m);
-                        }
-                    }
-                    double nx = ev.px + vtmin * ev.dx, ny = ev.py + vtmin * ev.dy, nz = ev.pz + vtmin * ev.dz;
-                    if (nx * nx + ny * ny + nz * nz > rr) {
-                        hull_tests += 4 - (A >= NC) - (Bp >= NC) - (oldC >= NC);
-                        for (int c = NC; c < NC + 4; ++c) {
-                            if (c != A && c != Bp && c != oldC) {
-                                double b[4];
-                                bounding_plane(c - NC, b);
-                                const double vtdiv = b[0] * ev.dx + b[1] * ev.dy + b[2] * ev.dz;
-                                if (vtdiv != 0.0) {
-                                    const double vt = -(b[0] * ev.px + b[1] * ev.py + b[2] * ev.pz + b[3]) / vtdiv;
-                                    if (vt < vtmin && vt > 0.0) {
-                                        vtmin = vt;
-                                        C = c;
-                                    }
-                                }
-                            }
-                        }
-                        nx = ev.px + vtmin * ev.dx;
-                        ny = ev.py + vtmin * ev.dy;
-                        nz = ev.pz + vtmin * ev.dz;
-                    }
-                    if (C < 0 || vn + 1 >= MAXV) {
+                    const HullStep w = hull_finish(hull_search(s, E, se, NC, pa0, pa1, pa2, pa3), NC, rr);
+                    hull_tests += w.tests;
+                    if (w.C < 0 || vn + 1 >= MAXV) {
                         cap = true;
                         break;
                     }
-                    put_vertex(s, vn, nx, ny, nz, A, Bp, C);
-                    if (l == 0) s.pflag[C] = 'Y';
-                    edge_close(E, se, C, vn);
+                    put_vertex(s, vn, w.nx, w.ny, w.nz, w.A, w.B, w.C);
+                    if (l == 0) s.pflag[w.C] = 'Y';
+                    edge_close(E, se, w.C, vn);
                     // edges (A, C) then (B, C) among the later edges
                     for (int pass = 0; pass < 2; ++pass) {
-                        const int P = pass ? Bp : A;
-                        const int other = pass ? A : Bp;
-                        const int f = find_edge(E, se, edgenum, P, C);
+                        const int P = pass ? w.B : w.A;
+                        const int other = pass ? w.A : w.B;
+                        const int f = find_edge(E, se, edgenum, P, w.C);
                         if (f >= 0) {
                             edge_close(E, f, other, vn);
                         } else {
-                            edge_new(E, edgenum, P, C, other, vn);
+                            edge_new(E, edgenum, P, w.C, other, vn);
                             ++edgenum;
                             if (pass == 1 && edgenum >= EDGE_LIMIT) hull_fail = true;
                         }
