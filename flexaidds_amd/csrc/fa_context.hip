@@ -15,84 +15,30 @@
 #include <tuple>
 #include <vector>
 
-#include "fa_device.h"
-#include "flexaidds_amd.h"
-
-namespace fa {
-hipError_t launch_cf(const DevReceptor& R, const DevGrid* grids, const DevLigand* ligs, const DevParams& prm,
-                     const DevBatch& B, int n_blocks, hipStream_t stream);
-int cf_waves_per_cu();
-int smem_bytes();
-size_t lse_scratch_bytes();
-hipError_t launch_thermo(const double* d_energy, int stride, const int* d_count, long long n, double T,
-                         void* d_scratch, fa_thermo* d_out, hipStream_t stream);
-hipError_t launch_boltzmann(const double* d_energy, int stride, long long n, double T, void* d_scratch, double* d_w,
-                            hipStream_t stream);
-hipError_t launch_lse_values(const double* d_x, int stride, long long n, void* d_scratch, hipStream_t stream);
-}  // namespace fa
+#include "fa_ctx.h"
 
 namespace {
-
 thread_local std::string g_err;
+constexpr float CELLSIZE = 6.5f;  // LIB/Vcontacts.h:56
+constexpr int NEAR_PER_ATOM = 160;
+}  // namespace
 
-int fail(int code, const std::string& msg)
+int fa_fail(int code, const std::string& msg)
 {
     g_err = msg;
     return code;
 }
 
-#define FA_HIP(expr)                                                                      \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail(FA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-constexpr float CELLSIZE = 6.5f;  // LIB/Vcontacts.h:56
-constexpr int NEAR_PER_ATOM = 160;
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t n = 0;
-};
-
-}  // namespace
-
-struct fa_ctx {
-    int device = 0;
-    int n_cu = 0;
-    int max_pop = 0;
-    // resident system
-    std::vector<void*> allocs;
-    fa::DevReceptor rec{};
-    fa::DevParams prm{};
-    fa::DevGrid* d_grids = nullptr;
-    fa::DevLigand* d_ligs = nullptr;
-    std::vector<fa::DevLigand> h_ligs;
-    std::vector<int> lig_genes;
-    int n_ligands = 0;
-    int max_lig_atoms = 0;
-    bool have_system = false;
-    // pose-kernel grid: one resident wave per slot, each with its own
-    // neighbour-list scratch of near_cap entries
-    int n_waves = 0;
-    int near_cap = 0;
-    // pose buffers
-    DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops, prof, near_idx, near_xyz;
-    bool profile = false;
-    std::vector<uint32_t> last_counters = std::vector<uint32_t>(4, 0);
-
-    void free_system()
-    {
-        for (void* p : allocs) (void)hipFree(p);
-        allocs.clear();
-        d_grids = nullptr;
-        d_ligs = nullptr;
-        h_ligs.clear();
-        lig_genes.clear();
-        have_system = false;
-    }
-};
+int fa_ensure(DevBuf& b, size_t bytes)
+{
+    if (b.n >= bytes) return FA_SUCCESS;
+    if (b.p) FA_HIP(hipFree(b.p));
+    b.p = nullptr;
+    b.n = 0;
+    FA_HIP(hipMalloc(&b.p, bytes));
+    b.n = bytes;
+    return FA_SUCCESS;
+}
 
 namespace {
 
@@ -104,17 +50,6 @@ template <class T> int upload(fa_ctx* c, const T* h, size_t n, const T** d)
     c->allocs.push_back(p);
     if (n) FA_HIP(hipMemcpy(p, h, sizeof(T) * n, hipMemcpyHostToDevice));
     *d = static_cast<const T*>(p);
-    return FA_SUCCESS;
-}
-
-int ensure(DevBuf& b, size_t bytes)
-{
-    if (b.n >= bytes) return FA_SUCCESS;
-    if (b.p) FA_HIP(hipFree(b.p));
-    b.p = nullptr;
-    b.n = 0;
-    FA_HIP(hipMalloc(&b.p, bytes));
-    b.n = bytes;
     return FA_SUCCESS;
 }
 
@@ -131,34 +66,34 @@ int validate_ligand(const fa_ligand_desc& L, int idx)
 {
     const std::string tag = "ligand " + std::to_string(idx) + ": ";
     if (L.n_atoms < 3 || L.n_atoms > FA_MAX_LIG_ATOMS)
-        return fail(FA_ERR_LIMIT, tag + "n_atoms must be in [3, " + std::to_string(FA_MAX_LIG_ATOMS) + "]");
+        return fa_fail(FA_ERR_LIMIT, tag + "n_atoms must be in [3, " + std::to_string(FA_MAX_LIG_ATOMS) + "]");
     if (L.n_genes < 1 || L.n_genes > FA_MAX_GENES)
-        return fail(FA_ERR_LIMIT, tag + "n_genes must be in [1, " + std::to_string(FA_MAX_GENES) + "]");
-    if (L.n_grid < 1) return fail(FA_ERR_INVALID, tag + "empty cleft grid");
+        return fa_fail(FA_ERR_LIMIT, tag + "n_genes must be in [1, " + std::to_string(FA_MAX_GENES) + "]");
+    if (L.n_grid < 1) return fa_fail(FA_ERR_INVALID, tag + "empty cleft grid");
     if (!L.rec || !L.dis || !L.ang || !L.dih || !L.shift || !L.radius || !L.type || !L.mov || !L.bonded ||
         !L.gene_typ || !L.gene_atm || !L.gene_min || !L.gene_max || !L.grid_ic)
-        return fail(FA_ERR_INVALID, tag + "null array");
+        return fa_fail(FA_ERR_INVALID, tag + "null array");
     const int n = L.n_atoms;
     std::vector<int> pos(n, -1);
     for (int k = 0; k < n; ++k) {
         const int a = L.mov[k];
-        if (a < 0 || a >= n || pos[a] != -1) return fail(FA_ERR_INVALID, tag + "mov is not a permutation");
+        if (a < 0 || a >= n || pos[a] != -1) return fa_fail(FA_ERR_INVALID, tag + "mov is not a permutation");
         pos[a] = k;
     }
     for (int a = 0; a < n; ++a) {
         for (int i = 0; i < 3; ++i) {
             const int r = L.rec[4 * a + i];
-            if (r < -1 || r >= n) return fail(FA_ERR_INVALID, tag + "rec out of range");
-            if (r >= 0 && pos[r] >= pos[a]) return fail(FA_ERR_INVALID, tag + "rec atom rebuilt after its dependant");
+            if (r < -1 || r >= n) return fa_fail(FA_ERR_INVALID, tag + "rec out of range");
+            if (r >= 0 && pos[r] >= pos[a]) return fa_fail(FA_ERR_INVALID, tag + "rec atom rebuilt after its dependant");
         }
         const int r3 = L.rec[4 * a + 3];
-        if (r3 < -1 || r3 >= n) return fail(FA_ERR_INVALID, tag + "rec[3] out of range");
-        if (L.type[a] < 1) return fail(FA_ERR_INVALID, tag + "types are 1-based");
+        if (r3 < -1 || r3 >= n) return fa_fail(FA_ERR_INVALID, tag + "rec[3] out of range");
+        if (L.type[a] < 1) return fa_fail(FA_ERR_INVALID, tag + "types are 1-based");
     }
     for (int g = 0; g < L.n_genes; ++g) {
         const int t = L.gene_typ[g], a = L.gene_atm[g];
-        if (t < FA_GENE_GRID || t > FA_GENE_DIH) return fail(FA_ERR_INVALID, tag + "unsupported gene type");
-        if (a < 0 || a >= n) return fail(FA_ERR_INVALID, tag + "gene atom out of range");
+        if (t < FA_GENE_GRID || t > FA_GENE_DIH) return fa_fail(FA_ERR_INVALID, tag + "unsupported gene type");
+        if (a < 0 || a >= n) return fa_fail(FA_ERR_INVALID, tag + "gene atom out of range");
         if (t == FA_GENE_DIH && L.rec[4 * a + 3] >= 0) {  // shift chain must return to the gene atom
             int cat = L.rec[4 * a + 3], steps = 0;
             while (cat != a && steps <= n) {
@@ -166,15 +101,47 @@ int validate_ligand(const fa_ligand_desc& L, int idx)
                 ++steps;
                 if (cat < 0) break;
             }
-            if (cat != a) return fail(FA_ERR_INVALID, tag + "rec[3] shift chain does not close");
+            if (cat != a) return fa_fail(FA_ERR_INVALID, tag + "rec[3] shift chain does not close");
         }
         if (t == FA_GENE_GRID && (L.gene_min[g] < 0.0 || L.gene_max[g] > (double)(L.n_grid - 1)))
-            return fail(FA_ERR_INVALID, tag + "grid gene limits outside the cleft grid");
+            return fa_fail(FA_ERR_INVALID, tag + "grid gene limits outside the cleft grid");
     }
     return FA_SUCCESS;
 }
 
 }  // namespace
+
+int fa_launch_poses(fa_ctx* c, int n_poses, const double* d_genes, int stride, const int* d_lig, fa_cf_result* d_out,
+                  fa_contact_rec* d_recs, int rec_cap, int* d_reccnt, float* d_xyz, int xyz_stride,
+                  hipStream_t stream)
+{
+    fa::DevBatch B{};
+    B.n_poses = n_poses;
+    B.gene_stride = stride;
+    B.genes = d_genes;
+    B.pose_lig = d_lig;
+    B.out = d_out;
+    B.recs = d_recs;
+    B.rec_cap = rec_cap;
+    B.rec_count = d_reccnt;
+    B.lig_xyz = d_xyz;
+    B.xyz_stride = xyz_stride;
+    B.counters = static_cast<unsigned int*>(c->counters.p);
+    B.ops = static_cast<unsigned long long*>(c->ops.p);
+    FA_HIP(fa::launch_zero_u32(static_cast<unsigned int*>(c->counters.p), 4, stream));
+    FA_HIP(fa::launch_zero_u32(static_cast<unsigned int*>(c->ops.p), (int)(sizeof(fa_opcount) / 4), stream));
+    B.prof = nullptr;
+    if (c->profile) {
+        B.prof = static_cast<unsigned long long*>(c->prof.p);
+        FA_HIP(fa::launch_zero_u32(static_cast<unsigned int*>(c->prof.p), 32, stream));
+    }
+    B.near_idx = static_cast<int2*>(c->near_idx.p);
+    B.near_xyz = static_cast<float4*>(c->near_xyz.p);
+    B.near_cap = c->near_cap;
+    // one wave per resident slot; each wave loops over poses (work queue)
+    FA_HIP(fa::launch_cf(c->rec, c->d_grids, c->d_ligs, c->prm, B, c->n_waves, stream));
+    return FA_SUCCESS;
+}
 
 extern "C" {
 
@@ -186,10 +153,10 @@ int fa_ctx_launch_waves(fa_ctx* c) { return c ? c->n_waves : 0; }
 
 int fa_get_device_info(int device_id, fa_device_info* out)
 {
-    if (!out) return fail(FA_ERR_INVALID, "null output");
+    if (!out) return fa_fail(FA_ERR_INVALID, "null output");
     int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(FA_ERR_NODEVICE, "no HIP device");
-    if (device_id < 0 || device_id >= n) return fail(FA_ERR_INVALID, "device id out of range");
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fa_fail(FA_ERR_NODEVICE, "no HIP device");
+    if (device_id < 0 || device_id >= n) return fa_fail(FA_ERR_INVALID, "device id out of range");
     hipDeviceProp_t p;
     FA_HIP(hipGetDeviceProperties(&p, device_id));
     std::memset(out, 0, sizeof(*out));
@@ -206,11 +173,11 @@ int fa_get_device_info(int device_id, fa_device_info* out)
 
 int fa_ctx_create(int device_id, int max_pop, fa_ctx** out)
 {
-    if (!out) return fail(FA_ERR_INVALID, "null output");
+    if (!out) return fa_fail(FA_ERR_INVALID, "null output");
     *out = nullptr;
     int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(FA_ERR_NODEVICE, "no HIP device");
-    if (device_id < 0 || device_id >= n) return fail(FA_ERR_INVALID, "device id out of range");
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fa_fail(FA_ERR_NODEVICE, "no HIP device");
+    if (device_id < 0 || device_id >= n) return fa_fail(FA_ERR_INVALID, "device id out of range");
     FA_HIP(hipSetDevice(device_id));
     hipDeviceProp_t p;
     FA_HIP(hipGetDeviceProperties(&p, device_id));
@@ -219,13 +186,13 @@ int fa_ctx_create(int device_id, int max_pop, fa_ctx** out)
     c->n_cu = p.multiProcessorCount;
     c->max_pop = max_pop > 0 ? max_pop : 4096;
     int rc;
-    if ((rc = ensure(c->counters, 4 * sizeof(uint32_t))) != FA_SUCCESS ||
-        (rc = ensure(c->scratch, fa::lse_scratch_bytes())) != FA_SUCCESS ||
-        (rc = ensure(c->thermo, sizeof(fa_thermo))) != FA_SUCCESS ||
-        (rc = ensure(c->ops, sizeof(fa_opcount))) != FA_SUCCESS ||
-        (rc = ensure(c->prof, 16 * sizeof(unsigned long long))) != FA_SUCCESS ||
-        (rc = ensure(c->genes, sizeof(double) * (size_t)c->max_pop * 32)) != FA_SUCCESS ||
-        (rc = ensure(c->out, sizeof(fa_cf_result) * (size_t)c->max_pop)) != FA_SUCCESS) {
+    if ((rc = fa_ensure(c->counters, 4 * sizeof(uint32_t))) != FA_SUCCESS ||
+        (rc = fa_ensure(c->scratch, fa::lse_scratch_bytes())) != FA_SUCCESS ||
+        (rc = fa_ensure(c->thermo, sizeof(fa_thermo))) != FA_SUCCESS ||
+        (rc = fa_ensure(c->ops, sizeof(fa_opcount))) != FA_SUCCESS ||
+        (rc = fa_ensure(c->prof, 16 * sizeof(unsigned long long))) != FA_SUCCESS ||
+        (rc = fa_ensure(c->genes, sizeof(double) * (size_t)c->max_pop * 32)) != FA_SUCCESS ||
+        (rc = fa_ensure(c->out, sizeof(fa_cf_result) * (size_t)c->max_pop)) != FA_SUCCESS) {
         delete c;
         return rc;
     }
@@ -249,16 +216,16 @@ int fa_ctx_destroy(fa_ctx* c)
 int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm, int n_ligands,
                      const fa_ligand_desc* ligs)
 {
-    if (!c || !r || !prm || !ligs || n_ligands < 1) return fail(FA_ERR_INVALID, "null argument");
-    if (r->n_atoms < 1 || !r->xyz || !r->radius || !r->type) return fail(FA_ERR_INVALID, "empty receptor");
-    if (prm->ntypes < 1 || !prm->emat) return fail(FA_ERR_INVALID, "energy matrix missing");
+    if (!c || !r || !prm || !ligs || n_ligands < 1) return fa_fail(FA_ERR_INVALID, "null argument");
+    if (r->n_atoms < 1 || !r->xyz || !r->radius || !r->type) return fa_fail(FA_ERR_INVALID, "empty receptor");
+    if (prm->ntypes < 1 || !prm->emat) return fa_fail(FA_ERR_INVALID, "energy matrix missing");
     for (int i = 0; i < r->n_atoms; ++i)
-        if (r->type[i] < 1 || r->type[i] > prm->ntypes) return fail(FA_ERR_INVALID, "receptor type out of range");
+        if (r->type[i] < 1 || r->type[i] > prm->ntypes) return fa_fail(FA_ERR_INVALID, "receptor type out of range");
     for (int li = 0; li < n_ligands; ++li) {
         int rc = validate_ligand(ligs[li], li);
         if (rc != FA_SUCCESS) return rc;
         for (int a = 0; a < ligs[li].n_atoms; ++a)
-            if (ligs[li].type[a] > prm->ntypes) return fail(FA_ERR_INVALID, "ligand type out of range");
+            if (ligs[li].type[a] > prm->ntypes) return fa_fail(FA_ERR_INVALID, "ligand type out of range");
     }
     FA_HIP(hipSetDevice(c->device));
     FA_HIP(hipDeviceSynchronize());
@@ -311,13 +278,13 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
             G.maxwidth = k.maxwidth;
             G.dim = (int)(k.maxwidth / CELLSIZE) + 1;
             const long long dim2 = (long long)G.dim * G.dim, dim3 = dim2 * G.dim;
-            if (G.dim < 1 || dim3 > (1LL << 26)) return fail(FA_ERR_LIMIT, "cell grid too large");
+            if (G.dim < 1 || dim3 > (1LL << 26)) return fa_fail(FA_ERR_LIMIT, "cell grid too large");
             std::vector<int> box(Rn), cnt(dim3 + 1, 0);
             for (int i = 0; i < Rn; ++i) {
                 const long long b = (long long)(int)((x[i] - G.gmin[0]) / CELLSIZE) * dim2 +
                                     (long long)(int)((y[i] - G.gmin[1]) / CELLSIZE) * G.dim +
                                     (int)((z[i] - G.gmin[2]) / CELLSIZE);
-                if (b < 0 || b >= dim3) return fail(FA_ERR_INVALID, "receptor atom outside the cell grid");
+                if (b < 0 || b >= dim3) return fa_fail(FA_ERR_INVALID, "receptor atom outside the cell grid");
                 box[i] = (int)b;
                 ++cnt[b + 1];
             }
@@ -384,95 +351,64 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
     c->n_waves = c->n_cu * per_cu;
     c->near_cap = max_l * NEAR_PER_ATOM;
     const size_t n_near = (size_t)c->n_waves * c->near_cap;
-    if ((rc = ensure(c->near_idx, sizeof(int2) * n_near)) || (rc = ensure(c->near_xyz, sizeof(float4) * n_near)))
+    if ((rc = fa_ensure(c->near_idx, sizeof(int2) * n_near)) || (rc = fa_ensure(c->near_xyz, sizeof(float4) * n_near)))
         return rc;
     c->have_system = true;
     return FA_SUCCESS;
 }
 
-static int launch(fa_ctx* c, int n_poses, const double* d_genes, int stride, const int* d_lig, fa_cf_result* d_out,
-                  fa_contact_rec* d_recs, int rec_cap, int* d_reccnt, float* d_xyz, int xyz_stride,
-                  hipStream_t stream)
-{
-    fa::DevBatch B{};
-    B.n_poses = n_poses;
-    B.gene_stride = stride;
-    B.genes = d_genes;
-    B.pose_lig = d_lig;
-    B.out = d_out;
-    B.recs = d_recs;
-    B.rec_cap = rec_cap;
-    B.rec_count = d_reccnt;
-    B.lig_xyz = d_xyz;
-    B.xyz_stride = xyz_stride;
-    B.counters = static_cast<unsigned int*>(c->counters.p);
-    B.ops = static_cast<unsigned long long*>(c->ops.p);
-    FA_HIP(hipMemsetAsync(c->counters.p, 0, 4 * sizeof(uint32_t), stream));
-    FA_HIP(hipMemsetAsync(c->ops.p, 0, sizeof(fa_opcount), stream));
-    B.prof = nullptr;
-    if (c->profile) {
-        B.prof = static_cast<unsigned long long*>(c->prof.p);
-        FA_HIP(hipMemsetAsync(c->prof.p, 0, 16 * sizeof(unsigned long long), stream));
-    }
-    B.near_idx = static_cast<int2*>(c->near_idx.p);
-    B.near_xyz = static_cast<float4*>(c->near_xyz.p);
-    B.near_cap = c->near_cap;
-    // one wave per resident slot; each wave loops over poses (work queue)
-    FA_HIP(fa::launch_cf(c->rec, c->d_grids, c->d_ligs, c->prm, B, c->n_waves, stream));
-    return FA_SUCCESS;
-}
 
 int fa_eval_batch_device(fa_ctx* c, int n_poses, const double* genes, int gene_stride, const int* pose_lig,
                          fa_cf_result* out, void* stream)
 {
-    if (!c) return fail(FA_ERR_INVALID, "null context");
-    if (!c->have_system) return fail(FA_ERR_NOSYSTEM, "no system uploaded");
-    if (n_poses < 0 || (n_poses > 0 && (!genes || !out))) return fail(FA_ERR_INVALID, "bad batch");
-    if (gene_stride < 1) return fail(FA_ERR_INVALID, "gene_stride must be positive");
+    if (!c) return fa_fail(FA_ERR_INVALID, "null context");
+    if (!c->have_system) return fa_fail(FA_ERR_NOSYSTEM, "no system uploaded");
+    if (n_poses < 0 || (n_poses > 0 && (!genes || !out))) return fa_fail(FA_ERR_INVALID, "bad batch");
+    if (gene_stride < 1) return fa_fail(FA_ERR_INVALID, "gene_stride must be positive");
     FA_HIP(hipSetDevice(c->device));
-    return launch(c, n_poses, genes, gene_stride, pose_lig, out, nullptr, 0, nullptr, nullptr, 0,
+    return fa_launch_poses(c, n_poses, genes, gene_stride, pose_lig, out, nullptr, 0, nullptr, nullptr, 0,
                   static_cast<hipStream_t>(stream));
 }
 
 static int eval_host(fa_ctx* c, int ligand, int n_poses, const double* genes, fa_cf_result* out, float* lig_xyz,
                      fa_contact_rec* recs, int rec_cap, int* rec_count)
 {
-    if (!c) return fail(FA_ERR_INVALID, "null context");
-    if (!c->have_system) return fail(FA_ERR_NOSYSTEM, "no system uploaded");
-    if (ligand < 0 || ligand >= c->n_ligands) return fail(FA_ERR_INVALID, "ligand index out of range");
-    if (n_poses < 0 || (n_poses > 0 && (!genes || !out))) return fail(FA_ERR_INVALID, "bad batch");
+    if (!c) return fa_fail(FA_ERR_INVALID, "null context");
+    if (!c->have_system) return fa_fail(FA_ERR_NOSYSTEM, "no system uploaded");
+    if (ligand < 0 || ligand >= c->n_ligands) return fa_fail(FA_ERR_INVALID, "ligand index out of range");
+    if (n_poses < 0 || (n_poses > 0 && (!genes || !out))) return fa_fail(FA_ERR_INVALID, "bad batch");
     if (n_poses == 0) return FA_SUCCESS;
     FA_HIP(hipSetDevice(c->device));
     const int G = c->lig_genes[ligand];
     int rc;
-    if ((rc = ensure(c->genes, sizeof(double) * (size_t)n_poses * G)) ||
-        (rc = ensure(c->out, sizeof(fa_cf_result) * (size_t)n_poses)))
+    if ((rc = fa_ensure(c->genes, sizeof(double) * (size_t)n_poses * G)) ||
+        (rc = fa_ensure(c->out, sizeof(fa_cf_result) * (size_t)n_poses)))
         return rc;
     std::vector<int> lig(n_poses, ligand);
     DevBuf ligbuf;
-    if ((rc = ensure(ligbuf, sizeof(int) * (size_t)n_poses))) return rc;
+    if ((rc = fa_ensure(ligbuf, sizeof(int) * (size_t)n_poses))) return rc;
     fa_contact_rec* d_recs = nullptr;
     int* d_cnt = nullptr;
     float* d_xyz = nullptr;
     const int L = c->h_ligs[ligand].n_atoms;
     if (recs && rec_count && rec_cap > 0) {
-        if ((rc = ensure(c->recs, sizeof(fa_contact_rec) * (size_t)n_poses * rec_cap)) ||
-            (rc = ensure(c->reccnt, sizeof(int) * (size_t)n_poses)))
+        if ((rc = fa_ensure(c->recs, sizeof(fa_contact_rec) * (size_t)n_poses * rec_cap)) ||
+            (rc = fa_ensure(c->reccnt, sizeof(int) * (size_t)n_poses)))
             return rc;
         d_recs = static_cast<fa_contact_rec*>(c->recs.p);
         d_cnt = static_cast<int*>(c->reccnt.p);
     }
     if (lig_xyz) {
-        if ((rc = ensure(c->xyz, sizeof(float) * 3 * (size_t)n_poses * L))) return rc;
+        if ((rc = fa_ensure(c->xyz, sizeof(float) * 3 * (size_t)n_poses * L))) return rc;
         d_xyz = static_cast<float*>(c->xyz.p);
     }
     hipError_t e = hipMemcpy(c->genes.p, genes, sizeof(double) * (size_t)n_poses * G, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ligbuf.p, lig.data(), sizeof(int) * (size_t)n_poses, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(ligbuf.p);
-        return fail(FA_ERR_HIP, std::string("upload genes: ") + hipGetErrorString(e));
+        return fa_fail(FA_ERR_HIP, std::string("upload genes: ") + hipGetErrorString(e));
     }
-    rc = launch(c, n_poses, static_cast<const double*>(c->genes.p), G, static_cast<const int*>(ligbuf.p),
+    rc = fa_launch_poses(c, n_poses, static_cast<const double*>(c->genes.p), G, static_cast<const int*>(ligbuf.p),
                 static_cast<fa_cf_result*>(c->out.p), d_recs, rec_cap, d_cnt, d_xyz, L, nullptr);
     if (rc == FA_SUCCESS) {
         e = hipMemcpy(out, c->out.p, sizeof(fa_cf_result) * (size_t)n_poses, hipMemcpyDeviceToHost);
@@ -484,7 +420,7 @@ static int eval_host(fa_ctx* c, int ligand, int n_poses, const double* genes, fa
             e = hipMemcpy(lig_xyz, d_xyz, sizeof(float) * 3 * (size_t)n_poses * L, hipMemcpyDeviceToHost);
         if (e == hipSuccess)
             e = hipMemcpy(c->last_counters.data(), c->counters.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) rc = fail(FA_ERR_HIP, std::string("download results: ") + hipGetErrorString(e));
+        if (e != hipSuccess) rc = fa_fail(FA_ERR_HIP, std::string("download results: ") + hipGetErrorString(e));
     }
     (void)hipFree(ligbuf.p);
     return rc;
@@ -504,8 +440,8 @@ int fa_eval_batch_debug(fa_ctx* c, int ligand, int n_poses, const double* genes,
 int fa_thermo_device(fa_ctx* c, const double* energy, int stride, const int* count, int64_t n, double T,
                      fa_thermo* out, void* stream)
 {
-    if (!c || !energy || !out || n < 1 || stride < 1) return fail(FA_ERR_INVALID, "bad thermo arguments");
-    if (!(T > 0.0)) return fail(FA_ERR_INVALID, "temperature must be > 0");
+    if (!c || !energy || !out || n < 1 || stride < 1) return fa_fail(FA_ERR_INVALID, "bad thermo arguments");
+    if (!(T > 0.0)) return fa_fail(FA_ERR_INVALID, "temperature must be > 0");
     FA_HIP(hipSetDevice(c->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     FA_HIP(fa::launch_thermo(energy, stride, count, n, T, c->scratch.p, static_cast<fa_thermo*>(c->thermo.p), s));
@@ -516,19 +452,19 @@ int fa_thermo_device(fa_ctx* c, const double* energy, int stride, const int* cou
 
 int fa_thermo_host(fa_ctx* c, const double* energy, const int* count, int64_t n, double T, fa_thermo* out)
 {
-    if (!c || !energy || !out || n < 1) return fail(FA_ERR_INVALID, "bad thermo arguments");
+    if (!c || !energy || !out || n < 1) return fa_fail(FA_ERR_INVALID, "bad thermo arguments");
     FA_HIP(hipSetDevice(c->device));
     DevBuf e, k;
-    int rc = ensure(e, sizeof(double) * (size_t)n);
+    int rc = fa_ensure(e, sizeof(double) * (size_t)n);
     if (rc) return rc;
-    if (count && (rc = ensure(k, sizeof(int) * (size_t)n))) {
+    if (count && (rc = fa_ensure(k, sizeof(int) * (size_t)n))) {
         (void)hipFree(e.p);
         return rc;
     }
     hipError_t he = hipMemcpy(e.p, energy, sizeof(double) * (size_t)n, hipMemcpyHostToDevice);
     if (he == hipSuccess && count) he = hipMemcpy(k.p, count, sizeof(int) * (size_t)n, hipMemcpyHostToDevice);
     if (he != hipSuccess)
-        rc = fail(FA_ERR_HIP, std::string("thermo upload: ") + hipGetErrorString(he));
+        rc = fa_fail(FA_ERR_HIP, std::string("thermo upload: ") + hipGetErrorString(he));
     else
         rc = fa_thermo_device(c, static_cast<const double*>(e.p), 1, static_cast<const int*>(k.p), n, T, out, nullptr);
     (void)hipFree(e.p);
@@ -538,9 +474,9 @@ int fa_thermo_host(fa_ctx* c, const double* energy, const int* count, int64_t n,
 
 int fa_upload_emat(fa_ctx* c, const float* emat, int ntypes)
 {
-    if (!c || !emat) return fail(FA_ERR_INVALID, "null argument");
-    if (!c->have_system) return fail(FA_ERR_NOSYSTEM, "no system uploaded");
-    if (ntypes != c->prm.ntypes) return fail(FA_ERR_INVALID, "ntypes differs from the uploaded system");
+    if (!c || !emat) return fa_fail(FA_ERR_INVALID, "null argument");
+    if (!c->have_system) return fa_fail(FA_ERR_NOSYSTEM, "no system uploaded");
+    if (ntypes != c->prm.ntypes) return fa_fail(FA_ERR_INVALID, "ntypes differs from the uploaded system");
     FA_HIP(hipSetDevice(c->device));
     FA_HIP(hipDeviceSynchronize());
     FA_HIP(hipMemcpy(const_cast<float*>(c->prm.emat), emat, sizeof(float) * (size_t)ntypes * ntypes,
@@ -551,8 +487,8 @@ int fa_upload_emat(fa_ctx* c, const float* emat, int ntypes)
 int fa_boltzmann_device(fa_ctx* c, const double* energy, int stride, int64_t n, double T, double* weights,
                         double* log_Z, void* stream)
 {
-    if (!c || !energy || !weights || n < 1 || stride < 1) return fail(FA_ERR_INVALID, "bad boltzmann arguments");
-    if (!(T > 0.0)) return fail(FA_ERR_INVALID, "temperature must be > 0");
+    if (!c || !energy || !weights || n < 1 || stride < 1) return fa_fail(FA_ERR_INVALID, "bad boltzmann arguments");
+    if (!(T > 0.0)) return fa_fail(FA_ERR_INVALID, "temperature must be > 0");
     FA_HIP(hipSetDevice(c->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     FA_HIP(fa::launch_boltzmann(energy, stride, n, T, c->scratch.p, weights, s));
@@ -567,7 +503,7 @@ int fa_boltzmann_device(fa_ctx* c, const double* energy, int stride, int64_t n, 
 
 int fa_log_sum_exp_device(fa_ctx* c, const double* values, int stride, int64_t n, double* out, void* stream)
 {
-    if (!c || !values || !out || stride < 1) return fail(FA_ERR_INVALID, "bad log_sum_exp arguments");
+    if (!c || !values || !out || stride < 1) return fa_fail(FA_ERR_INVALID, "bad log_sum_exp arguments");
     if (n < 1) {
         *out = -HUGE_VAL;
         return FA_SUCCESS;
@@ -584,7 +520,7 @@ int fa_log_sum_exp_device(fa_ctx* c, const double* values, int stride, int64_t n
 
 int fa_last_counters(fa_ctx* c, uint32_t* slow_grid, uint32_t* capacity)
 {
-    if (!c) return fail(FA_ERR_INVALID, "null context");
+    if (!c) return fa_fail(FA_ERR_INVALID, "null context");
     FA_HIP(hipSetDevice(c->device));
     FA_HIP(hipDeviceSynchronize());
     FA_HIP(hipMemcpy(c->last_counters.data(), c->counters.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -595,14 +531,14 @@ int fa_last_counters(fa_ctx* c, uint32_t* slow_grid, uint32_t* capacity)
 
 int fa_set_phase_profiling(fa_ctx* c, int enable)
 {
-    if (!c) return fail(FA_ERR_INVALID, "null context");
+    if (!c) return fa_fail(FA_ERR_INVALID, "null context");
     c->profile = enable != 0;
     return FA_SUCCESS;
 }
 
 int fa_last_phase_cycles(fa_ctx* c, uint64_t* cycles16)
 {
-    if (!c || !cycles16) return fail(FA_ERR_INVALID, "null argument");
+    if (!c || !cycles16) return fa_fail(FA_ERR_INVALID, "null argument");
     FA_HIP(hipSetDevice(c->device));
     FA_HIP(hipDeviceSynchronize());
     FA_HIP(hipMemcpy(cycles16, c->prof.p, 16 * sizeof(uint64_t), hipMemcpyDeviceToHost));
@@ -611,7 +547,7 @@ int fa_last_phase_cycles(fa_ctx* c, uint64_t* cycles16)
 
 int fa_last_opcount(fa_ctx* c, fa_opcount* out)
 {
-    if (!c || !out) return fail(FA_ERR_INVALID, "null argument");
+    if (!c || !out) return fa_fail(FA_ERR_INVALID, "null argument");
     FA_HIP(hipSetDevice(c->device));
     FA_HIP(hipDeviceSynchronize());
     FA_HIP(hipMemcpy(out, c->ops.p, sizeof(fa_opcount), hipMemcpyDeviceToHost));

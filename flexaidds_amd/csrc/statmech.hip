@@ -11,57 +11,9 @@
 
 #include <math.h>
 
-#include "fa_types.h"
+#include "lse.h"
 
 namespace fa {
-
-constexpr double KB_KCAL = 0.001987206;  // LIB/statmech.h:26
-
-struct LseState {
-    double m, s, s1, s2;
-};
-
-__device__ __forceinline__ void lse_merge(LseState& a, const LseState& b)
-{
-    if (b.s == 0.0) return;
-    if (a.s == 0.0) {
-        a = b;
-        return;
-    }
-    if (b.m > a.m) {
-        const double f = exp(a.m - b.m);
-        a.s = a.s * f + b.s;
-        a.s1 = a.s1 * f + b.s1;
-        a.s2 = a.s2 * f + b.s2;
-        a.m = b.m;
-    } else {
-        const double f = exp(b.m - a.m);
-        a.s += b.s * f;
-        a.s1 += b.s1 * f;
-        a.s2 += b.s2 * f;
-    }
-}
-
-__device__ __forceinline__ void lse_push(LseState& a, double lw, double e)
-{
-    if (a.s == 0.0) {
-        a.m = lw;
-        a.s = 1.0;
-        a.s1 = e;
-        a.s2 = e * e;
-    } else if (lw > a.m) {
-        const double f = exp(a.m - lw);
-        a.s = a.s * f + 1.0;
-        a.s1 = a.s1 * f + e;
-        a.s2 = a.s2 * f + e * e;
-        a.m = lw;
-    } else {
-        const double w = exp(lw - a.m);
-        a.s += w;
-        a.s1 += w * e;
-        a.s2 += w * e * e;
-    }
-}
 
 __device__ void block_reduce(LseState& st, LseState* sh)
 {
@@ -109,23 +61,7 @@ __global__ __launch_bounds__(256) void lse_final_kernel(const LseState* __restri
     LseState st{0.0, 0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) lse_merge(st, part[i]);
     block_reduce(st, sh);
-    if (threadIdx.x == 0) {
-        const double lnZ = st.m + log(st.s);
-        const double e1 = st.s1 / st.s;
-        const double e2 = st.s2 / st.s;
-        const double kT = KB_KCAL * T;
-        const double var = e2 - e1 * e1;
-        fa_thermo th;
-        th.temperature = T;
-        th.log_Z = lnZ;
-        th.free_energy = -kT * lnZ;
-        th.mean_energy = e1;
-        th.mean_energy_sq = e2;
-        th.heat_capacity = var / (kT * kT);
-        th.entropy = (e1 - th.free_energy) / T;
-        th.std_energy = sqrt(var > 0.0 ? var : 0.0);
-        *out = th;
-    }
+    if (threadIdx.x == 0) *out = thermo_of(st, T);
 }
 
 // w_i = exp(-beta E_i - m) with m = max_i(-beta E_i): the unnormalised
@@ -148,6 +84,21 @@ __global__ __launch_bounds__(256) void lse_fold_kernel(LseState* __restrict__ pa
     block_reduce(st, sh);
     __syncthreads();
     if (threadIdx.x == 0) part[0] = st;
+}
+
+// clears n 32-bit words (a kernel instead of a memset node, so that launches
+// captured into a HIP graph replay correctly)
+__global__ void zero_u32_kernel(unsigned int* __restrict__ p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+hipError_t launch_zero_u32(unsigned int* p, int n_words, hipStream_t stream)
+{
+    if (n_words <= 0) return hipSuccess;
+    const int blocks = (n_words + 255) / 256 > 1024 ? 1024 : (n_words + 255) / 256;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(blocks), dim3(256), 0, stream, p, n_words);
+    return hipGetLastError();
 }
 
 constexpr int LSE_MAX_PARTS = 1024;

@@ -69,6 +69,12 @@ _SIGS = {
     "fa_log_sum_exp_device": ([_P, _P, C.c_int, C.c_int64, _P, _P], C.c_int),
     "fa_kernel_lds_bytes": ([], C.c_int),
     "fa_ctx_launch_waves": ([_P], C.c_int),
+    "fa_ga_create": ([_P, _P, _P, _P], C.c_int),
+    "fa_ga_run": ([_P, C.c_int, C.c_int, _P], C.c_int),
+    "fa_ga_population": ([_P, _P, _P, _P], C.c_int),
+    "fa_ga_thermo": ([_P, _P, _P], C.c_int),
+    "fa_ga_generation": ([_P, _P], C.c_int),
+    "fa_ga_destroy": ([_P], C.c_int),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -208,3 +214,67 @@ class Context:
         a, b = C.c_uint32(), C.c_uint32()
         _check(_lib.fa_last_counters(self._ctx, C.byref(a), C.byref(b)))
         return {"slow_grid": a.value, "capacity": b.value}
+
+
+class GeneLim(C.Structure):
+    _fields_ = [("min", C.c_double), ("max", C.c_double), ("del_", C.c_double), ("map", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class GAParams(C.Structure):
+    _fields_ = [("ligand", C.c_int32), ("population", C.c_int32), ("offspring", C.c_int32),
+                ("intragenes", C.c_int32), ("cross_rate", C.c_double), ("mut_rate", C.c_double),
+                ("temperature", C.c_double), ("seed", C.c_uint64)]
+
+
+class GA:
+    """Genetic algorithm running on the device (fa_ga_* of the C ABI).
+
+    `limits` is a sequence of (min, max, del, map) per gene of the ligand."""
+
+    def __init__(self, ctx: Context, limits, ligand=0, population=2000, offspring=None, cross_rate=0.8,
+                 mut_rate=0.03, temperature=300.0, seed=1, intragenes=False):
+        self.ctx = ctx
+        n = len(limits)
+        self._lim = (GeneLim * n)(*[GeneLim(float(a), float(b), float(d), int(m), 0) for a, b, d, m in limits])
+        self.params = GAParams(ligand, population, population if offspring is None else offspring,
+                               1 if intragenes else 0, cross_rate, mut_rate, temperature, seed)
+        self.genes = n
+        self._ga = C.c_void_p()
+        _check(_lib.fa_ga_create(ctx._ctx, C.byref(self.params), self._lim, C.byref(self._ga)))
+
+    def run(self, generations: int, graph: bool = True) -> float:
+        ms = C.c_float()
+        _check(_lib.fa_ga_run(self._ga, generations, 1 if graph else 0, C.byref(ms)))
+        return float(ms.value)
+
+    def population(self):
+        """(genes int32 [rows, G], ic [rows, G], results) with rows = population + offspring."""
+        rows = self.params.population + self.params.offspring
+        g = np.zeros((rows, self.genes), np.int32)
+        ic = np.zeros((rows, self.genes), np.float64)
+        res = np.zeros(rows, abi.CF_DTYPE)
+        _check(_lib.fa_ga_population(self._ga, g.ctypes.data, ic.ctypes.data, res.ctypes.data))
+        return g, ic, res
+
+    def thermo(self):
+        t = abi.Thermo()
+        n = C.c_uint64()
+        _check(_lib.fa_ga_thermo(self._ga, C.byref(t), C.byref(n)))
+        return {f: getattr(t, f) for f, _ in abi.Thermo._fields_}, int(n.value)
+
+    def generation(self) -> int:
+        g = C.c_uint64()
+        _check(_lib.fa_ga_generation(self._ga, C.byref(g)))
+        return int(g.value)
+
+    def close(self):
+        if self._ga:
+            _lib.fa_ga_destroy(self._ga)
+            self._ga = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

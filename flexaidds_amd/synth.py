@@ -488,6 +488,20 @@ class System:
     config: int = -1
 
 
+def ga_limits(lig: Ligand, angle_step=2.0):
+    """Gene limits (min, max, del, map) for the GA: the cleft-grid index gene
+    in steps of one point (an index gene: map = 1), angles and dihedrals in
+    `angle_step` degrees over the ligand's gene range."""
+    out = []
+    for g in range(lig.n_genes):
+        lo, hi = float(lig.gene_min[g]), float(lig.gene_max[g])
+        if int(lig.gene_typ[g]) == -1:  # grid-index gene
+            out.append((lo, hi, 1.0, 1))
+        else:
+            out.append((lo, hi, angle_step, 0))
+    return out
+
+
 def _box_points(rng, center, n, edge=10.0):
     return np.asarray(center)[None, :] + rng.uniform(-edge / 2, edge / 2, size=(n, 3))
 

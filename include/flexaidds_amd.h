@@ -132,6 +132,54 @@ int fa_last_opcount(fa_ctx* ctx, fa_opcount* out);
 int fa_set_phase_profiling(fa_ctx* ctx, int enable);
 int fa_last_phase_cycles(fa_ctx* ctx, uint64_t* cycles16);
 
+/* ---------------------------------------------------------------------
+ * Device GA (replaces the generation loop of GA(), LIB/gaboom.cpp:346-500,
+ * with BOOM reproduction, LIB/gaboom.cpp:886-1036, and LINEAR fitness,
+ * LIB/gaboom.cpp:1625-1633).  One generation = roulette selection of parent
+ * pairs (roullete_wheel, :1158-1182), bitwise crossover at cross_rate
+ * (crossover, :2517-2618), per-bit mutation at mut_rate (mutate, :2620-2641),
+ * genes to internal coordinates (genetoic, :2887-2906), evaluation of the
+ * offspring, survivors = best `population` of parents + offspring by energy
+ * (QuickSort by evalue; ties keep index order), rank fitness, and the
+ * ensemble reduction over every evaluated pose at `temperature`.
+ * Random draws come from include/fa_rng.h; duplicate rejection is not done.
+ * ------------------------------------------------------------------- */
+typedef struct fa_gene_lim {   /* LIB/gaboom.h:46-54 (genlim) */
+    double min, max, del;
+    int32_t map;               /* 1 for index genes: one more bin (set_bins, LIB/gaboom.cpp:2291-2306) */
+    int32_t pad;
+} fa_gene_lim;
+
+typedef struct fa_ga_params {
+    int32_t ligand;            /* ligand of the context to dock */
+    int32_t population;        /* survivors per generation (num_chrom) */
+    int32_t offspring;         /* children per generation (BOOM: pbfrac * num_chrom) */
+    int32_t intragenes;        /* crossover points inside genes */
+    double cross_rate;         /* e.g. 0.8 */
+    double mut_rate;           /* per bit, e.g. 0.03 */
+    double temperature;        /* K, ensemble reduction */
+    uint64_t seed;
+} fa_ga_params;
+
+typedef struct fa_ga fa_ga;
+
+/* Allocates the GA on the context's device, draws and evaluates the initial
+ * population (generate_random_individual, LIB/gaboom.cpp:1871-1902). */
+int fa_ga_create(fa_ctx* ctx, const fa_ga_params* params, const fa_gene_lim* limits, fa_ga** out);
+/* Runs `generations` generations on the GA's stream and waits.  With
+ * use_graph != 0 one generation is captured once as a HIP graph and
+ * replayed.  *ms (optional) receives the device time of the run. */
+int fa_ga_run(fa_ga* ga, int generations, int use_graph, float* ms);
+/* Rows [0, population): survivors by increasing energy; rows
+ * [population, population + offspring): the last generation's children as
+ * bred.  Any output may be null.  Synchronous. */
+int fa_ga_population(fa_ga* ga, int32_t* genes, double* ic, fa_cf_result* results);
+/* Ensemble thermodynamics over every pose evaluated so far (initial
+ * population and all children), FP64 on the device; *n_poses counts them. */
+int fa_ga_thermo(fa_ga* ga, fa_thermo* out, uint64_t* n_poses);
+int fa_ga_generation(fa_ga* ga, uint64_t* generation);
+int fa_ga_destroy(fa_ga* ga);
+
 /* Size of the kernel's per-wave LDS workspace (bytes). */
 int fa_kernel_lds_bytes(void);
 

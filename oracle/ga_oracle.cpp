@@ -1,0 +1,204 @@
+/*
+ * ga_oracle.cpp — CPU restatement of the device GA's operators.
+ *
+ * TEST INFRASTRUCTURE ONLY (tests/, bench.py's CPU baseline); the shipped
+ * library never links it.  The operators follow the reference line by line,
+ * with its RandomDouble()/dice() draws taken from include/fa_rng.h in the
+ * layout documented there, so a generation can be replayed exactly:
+ *
+ *   genetoic          LIB/gaboom.cpp:2887-2906
+ *   set_bins          LIB/gaboom.cpp:2291-2306
+ *   roullete_wheel    LIB/gaboom.cpp:1158-1182 (sequential scan, as written)
+ *   crossover         LIB/gaboom.cpp:2517-2618
+ *   mutate            LIB/gaboom.cpp:2620-2641
+ *   reproduce (BOOM)  LIB/gaboom.cpp:886-973, without duplicate rejection
+ *   LINEAR fitness    LIB/gaboom.cpp:1625-1633
+ */
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+#include <numeric>
+#include <vector>
+
+#include "fa_rng.h"
+#include "flexaidds_amd.h"
+
+namespace {
+
+double genetoic(const fa_gene_lim& g, double bin, int32_t gene)
+{
+    int i = 0;
+    double tot = bin;
+    while (tot < gene / ((double)FA_MAX_RANDOM_VALUE + 1.0)) {
+        tot += bin;
+        i++;
+    }
+    return g.min + g.del * (double)i;
+}
+
+double nbins(const fa_gene_lim& g)
+{
+    double nbin = (g.max - g.min) / g.del;
+    if (nbin - (int)nbin > 0.0) nbin += 1.0;
+    if (g.map) nbin += 1.0;
+    return nbin;
+}
+
+int roulette(const double* fit, int n, double u)
+{
+    double tot = 0.0;
+    for (int i = 0; i < n; i++) tot += fit[i];
+    if (tot <= 0.0) return static_cast<int>(u * n) % n;
+    const double r = u * tot;
+    int i = 0;
+    tot = 0.0;
+    while (tot <= r && i < n) {
+        tot += fit[i];
+        i++;
+    }
+    i--;
+    if (i < 0) i = 0;
+    return i;
+}
+
+void crossover(int32_t* john, int32_t* mary, int num_genes, int intragenes, uint64_t seed, uint64_t gen,
+               uint64_t stream)
+{
+    auto rd = [&](int k) { return fa_rng_u01(seed, gen, stream, k); };
+    int gen_a = (int)(rd(FA_DRAW_GEN_A) * (double)num_genes);
+    int gen_b = (int)(rd(FA_DRAW_GEN_B) * (double)num_genes);
+    if (gen_a >= num_genes) gen_a = num_genes - 1;
+    if (gen_b >= num_genes) gen_b = num_genes - 1;
+    int pnt_a, pnt_b, aux_gen, aux_pnt;
+    if (intragenes) {
+        pnt_a = (int)(rd(FA_DRAW_PNT_A) * 32.0);
+        pnt_b = (int)(rd(FA_DRAW_PNT_B) * 32.0);
+        if (gen_a > gen_b) {
+            aux_gen = gen_a;
+            aux_pnt = pnt_a;
+            gen_a = gen_b;
+            pnt_a = pnt_b;
+            gen_b = aux_gen;
+            pnt_b = aux_pnt;
+        }
+        if (gen_a == gen_b && pnt_a < pnt_b) {
+            aux_pnt = pnt_b;
+            pnt_b = pnt_a;
+            pnt_a = aux_pnt;
+        }
+    } else {
+        if (gen_a > gen_b) {
+            aux_gen = gen_a;
+            gen_a = gen_b;
+            gen_b = aux_gen;
+        }
+        if (gen_a != gen_b) {
+            pnt_a = (rd(FA_DRAW_PNT_A) < 0.5) ? 32 : 0;
+            pnt_b = (rd(FA_DRAW_PNT_B) < 0.5) ? 32 : 0;
+            if ((gen_b - gen_a) == 1 && pnt_a == 0 && pnt_b == 32) pnt_b = pnt_a;
+        } else {
+            pnt_a = 32;
+            pnt_b = 0;
+        }
+    }
+    for (int j = gen_a; j <= gen_b; j++) {
+        unsigned int optr = 1u;
+        aux_pnt = (j == gen_a) ? pnt_a : 32;
+        for (int i = 0; i < aux_pnt; i++) optr |= (optr << 1);
+        const unsigned int uj = (unsigned int)john[j], um = (unsigned int)mary[j];
+        john[j] = (int32_t)((uj & ~optr) | (um & optr));
+        mary[j] = (int32_t)((uj & optr) | (um & ~optr));
+    }
+    if (pnt_b > 0) {
+        unsigned int optr = 1u;
+        for (int i = 0; i < pnt_b - 1; i++) optr |= (optr << 1);
+        const unsigned int uj = (unsigned int)john[gen_b], um = (unsigned int)mary[gen_b];
+        john[gen_b] = (int32_t)((uj & ~optr) | (um & optr));
+        mary[gen_b] = (int32_t)((uj & optr) | (um & ~optr));
+    }
+}
+
+void mutate(int32_t* john, int num_genes, double mut_rate, uint64_t seed, uint64_t gen, uint64_t stream, int child)
+{
+    for (int j = 0; j < num_genes; j++) {
+        unsigned int optr = 0u, test = 1u;
+        for (int i = 0; i < 32; i++) {
+            if (fa_rng_u01(seed, gen, stream, FA_DRAW_MUT + ((uint64_t)child * num_genes + j) * 32 + i) < mut_rate)
+                optr |= test;
+            test <<= 1;
+        }
+        john[j] ^= (int32_t)optr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+double fa_oracle_genetoic(const fa_gene_lim* g, int32_t gene) { return genetoic(*g, 1.0 / nbins(*g), gene); }
+
+/* initial population: genes [pop * G] and their internal coordinates */
+void fa_oracle_ga_init(uint64_t seed, int pop, int G, const fa_gene_lim* lim, int32_t* genes, double* ic)
+{
+    for (int c = 0; c < pop; ++c)
+        for (int j = 0; j < G; ++j) {
+            const int32_t g = fa_rng_gene(seed, 0, (uint64_t)c, (uint64_t)j);
+            genes[(std::size_t)c * G + j] = g;
+            ic[(std::size_t)c * G + j] = genetoic(lim[j], 1.0 / nbins(lim[j]), g);
+        }
+}
+
+/* one generation's children from parents ranked best first (LINEAR fitness) */
+void fa_oracle_ga_breed(uint64_t seed, uint64_t gen, int pop, int n_new, int G, int intragenes, double cross_rate,
+                        double mut_rate, const fa_gene_lim* lim, const int32_t* parents, int32_t* child_genes,
+                        double* child_ic)
+{
+    std::vector<double> fit(pop);
+    for (int i = 0; i < pop; ++i) fit[i] = (double)(pop - i);
+    std::vector<int32_t> john(G), mary(G);
+    for (int t = 0; 2 * t < n_new; ++t) {
+        const int p1 = roulette(fit.data(), pop, fa_rng_u01(seed, gen, t, FA_DRAW_P1));
+        const int p2 = roulette(fit.data(), pop, fa_rng_u01(seed, gen, t, FA_DRAW_P2));
+        for (int j = 0; j < G; ++j) {
+            john[j] = parents[(std::size_t)p1 * G + j];
+            mary[j] = parents[(std::size_t)p2 * G + j];
+        }
+        if (fa_rng_u01(seed, gen, t, FA_DRAW_CROSS) < cross_rate) crossover(john.data(), mary.data(), G, intragenes, seed, gen, t);
+        mutate(john.data(), G, mut_rate, seed, gen, t, 0);
+        mutate(mary.data(), G, mut_rate, seed, gen, t, 1);
+        for (int k = 0; k < 2 && 2 * t + k < n_new; ++k) {
+            const int32_t* src = k ? mary.data() : john.data();
+            for (int j = 0; j < G; ++j) {
+                child_genes[(std::size_t)(2 * t + k) * G + j] = src[j];
+                child_ic[(std::size_t)(2 * t + k) * G + j] = genetoic(lim[j], 1.0 / nbins(lim[j]), src[j]);
+            }
+        }
+    }
+}
+
+/* the best `pop` of `n` rows by energy, ties by index (stable order) */
+void fa_oracle_ga_survivors(int n, int pop, const double* energy, int32_t* order)
+{
+    std::vector<int32_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return energy[a] < energy[b]; });
+    for (int i = 0; i < pop && i < n; ++i) order[i] = idx[i];
+}
+
+/* a single draw, for distribution tests */
+double fa_oracle_rng_u01(uint64_t seed, uint64_t gen, uint64_t stream, uint64_t k)
+{
+    return fa_rng_u01(seed, gen, stream, k);
+}
+
+/* one roulette pick over `fit` (distribution tests) */
+int fa_oracle_roulette(const double* fit, int n, double u) { return roulette(fit, n, u); }
+
+/* crossover of two chromosomes with the draws of (seed, gen, stream) */
+void fa_oracle_crossover(int32_t* john, int32_t* mary, int G, int intragenes, uint64_t seed, uint64_t gen,
+                         uint64_t stream)
+{
+    crossover(john, mary, G, intragenes, seed, gen, stream);
+}
+
+}  // extern "C"

@@ -137,3 +137,47 @@ def ref_eval(lib, desc, lig_index, genes, cap=4000):
                          counts.ctypes.data)
     assert rc == 0
     return out, xyz, [recs[p, :min(counts[p], cap)] for p in range(P)]
+
+
+class GeneLim(C.Structure):
+    _fields_ = [("min", C.c_double), ("max", C.c_double), ("del_", C.c_double), ("map", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+def gene_lims(limits):
+    return (GeneLim * len(limits))(*[GeneLim(float(a), float(b), float(d), int(m), 0) for a, b, d, m in limits])
+
+
+def ga_bind(lib):
+    lib.fa_oracle_ga_init.argtypes = [C.c_uint64, C.c_int, C.c_int, _P, _P, _P]
+    lib.fa_oracle_ga_breed.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                       C.c_double, _P, _P, _P, _P]
+    lib.fa_oracle_ga_survivors.argtypes = [C.c_int, C.c_int, _P, _P]
+    return lib
+
+
+def ga_init(lib, seed, pop, limits):
+    G = len(limits)
+    genes = np.zeros((pop, G), np.int32)
+    ic = np.zeros((pop, G))
+    lim = gene_lims(limits)
+    lib.fa_oracle_ga_init(seed, pop, G, lim, genes.ctypes.data, ic.ctypes.data)
+    return genes, ic
+
+
+def ga_breed(lib, seed, gen, parents, n_new, limits, cross=0.8, mut=0.03, intragenes=0):
+    parents = np.ascontiguousarray(parents, np.int32)
+    pop, G = parents.shape
+    child = np.zeros((n_new, G), np.int32)
+    ic = np.zeros((n_new, G))
+    lim = gene_lims(limits)
+    lib.fa_oracle_ga_breed(seed, gen, pop, n_new, G, intragenes, cross, mut, lim, parents.ctypes.data,
+                           child.ctypes.data, ic.ctypes.data)
+    return child, ic
+
+
+def ga_survivors(lib, energy, pop):
+    energy = np.ascontiguousarray(energy, np.float64)
+    order = np.zeros(pop, np.int32)
+    lib.fa_oracle_ga_survivors(len(energy), pop, energy.ctypes.data, order.ctypes.data)
+    return order

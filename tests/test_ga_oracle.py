@@ -1,0 +1,116 @@
+"""The GA oracle's operators against what their definitions fix: genetoic's
+bin arithmetic, crossover as a bit swap, the per-bit mutation rate, roulette
+selection in proportion to fitness, and the uniformity of the counter-based
+draws (include/fa_rng.h)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from flexaidds_amd import synth
+
+MAXR = 2147483647
+
+
+class Lim(C.Structure):
+    _fields_ = [("min", C.c_double), ("max", C.c_double), ("del_", C.c_double), ("map", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+@pytest.fixture(scope="module")
+def ga(oracle):
+    lib = oracle
+    lib.fa_oracle_genetoic.argtypes = [C.c_void_p, C.c_int32]
+    lib.fa_oracle_genetoic.restype = C.c_double
+    lib.fa_oracle_ga_init.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.fa_oracle_ga_breed.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                       C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.fa_oracle_rng_u01.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]
+    lib.fa_oracle_rng_u01.restype = C.c_double
+    lib.fa_oracle_roulette.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    lib.fa_oracle_roulette.restype = C.c_int
+    lib.fa_oracle_crossover.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
+                                        C.c_uint64]
+    return lib
+
+
+def test_genetoic_closed_form(ga):
+    # 64 bins of 1/64 add up exactly: i = ceil(64 u) - 1 for u > 0, else 0
+    lim = Lim(-10.0, 54.0, 1.0, 0, 0)
+    rng = np.random.default_rng(3)
+    genes = np.concatenate([[0, 1, 2, MAXR, MAXR - 1, -5, 1 << 25, (1 << 25) + 1],
+                            rng.integers(-2**31, 2**31 - 1, 2000)]).astype(np.int64)
+    for g in genes:
+        u = g / 2.0**31
+        i = max(int(np.ceil(64 * u)) - 1, 0) if u > 0 else 0
+        assert ga.fa_oracle_genetoic(C.byref(lim), int(g)) == -10.0 + i, g
+
+
+def test_genetoic_index_gene_covers_every_point(ga):
+    lim = Lim(0.0, 99.0, 1.0, 1, 0)   # 100 grid points, map gene: 100 bins
+    vals = {ga.fa_oracle_genetoic(C.byref(lim), int(g)) for g in np.linspace(1, MAXR, 20000).astype(np.int64)}
+    assert vals == {float(v) for v in range(100)}
+
+
+def test_crossover_is_a_bit_swap(ga):
+    rng = np.random.default_rng(11)
+    n_cross = 0
+    for t in range(300):
+        G = int(rng.integers(1, 20))
+        j0 = rng.integers(-2**31, 2**31 - 1, G).astype(np.int32)
+        m0 = rng.integers(-2**31, 2**31 - 1, G).astype(np.int32)
+        j1, m1 = j0.copy(), m0.copy()
+        ga.fa_oracle_crossover(j1.ctypes.data, m1.ctypes.data, G, int(t % 2), 7, 3, t)
+        assert np.array_equal(j1 ^ m1, j0 ^ m0)          # every bit position keeps its pair
+        assert np.array_equal(j1 & m1, j0 & m0)
+        n_cross += int(not np.array_equal(j1, j0))
+    assert n_cross > 200
+
+
+def test_mutation_flips_bits_at_the_rate(ga):
+    G, pop, n_new, rate = 12, 4, 4000, 0.03
+    lim = (Lim * G)(*[Lim(-180.0, 180.0, 2.0, 0, 0) for _ in range(G)])
+    parents = np.full((pop, G), 0x1234567, np.int32)     # identical parents: crossover is a no-op
+    child = np.zeros((n_new, G), np.int32)
+    ic = np.zeros((n_new, G))
+    ga.fa_oracle_ga_breed(5, 1, pop, n_new, G, 0, 0.8, rate, lim, parents.ctypes.data, child.ctypes.data,
+                          ic.ctypes.data)
+    flips = np.unpackbits((child ^ parents[0]).view(np.uint8)).sum()
+    n = n_new * G * 32
+    assert abs(flips - rate * n) < 5 * np.sqrt(n * rate * (1 - rate))
+
+
+def test_roulette_in_proportion_to_fitness(ga):
+    pop = 50
+    fit = np.array([pop - i for i in range(pop)], np.float64)     # LINEAR fitness
+    N = 127500                                                     # = 100 * sum(fit)
+    counts = np.zeros(pop, int)
+    for k in range(N):
+        counts[ga.fa_oracle_roulette(fit.ctypes.data, pop, (k + 0.5) / N)] += 1
+    assert np.all(np.abs(counts - N * fit / fit.sum()) <= 1)
+
+
+def test_draws_are_uniform(ga):
+    u = np.array([ga.fa_oracle_rng_u01(1, 2, s, k) for s in range(200) for k in range(200)])
+    assert np.all((u >= 0) & (u < 1))
+    assert abs(u.mean() - 0.5) < 0.005 and abs(u.var() - 1 / 12) < 0.002
+    # neighbouring counters are uncorrelated
+    assert abs(np.corrcoef(u[:-1], u[1:])[0, 1]) < 0.02
+
+
+def test_init_population_uses_the_limits(ga):
+    s = synth.make_config0(n_poses=4)
+    lig = s.ligands[0]
+    lims = synth.ga_limits(lig)
+    G = len(lims)
+    lim = (Lim * G)(*[Lim(a, b, d, m, 0) for a, b, d, m in lims])
+    genes = np.zeros((500, G), np.int32)
+    ic = np.zeros((500, G))
+    ga.fa_oracle_ga_init(9, 500, G, lim, genes.ctypes.data, ic.ctypes.data)
+    assert np.all(genes >= 1)
+    lo = np.array([l[0] for l in lims])
+    hi = np.array([l[1] for l in lims])
+    step = np.array([l[2] for l in lims])
+    assert np.all(ic >= lo) and np.all(ic <= hi + step)
+    assert np.allclose((ic - lo) / step, np.round((ic - lo) / step))
