@@ -518,6 +518,14 @@ int fa_log_sum_exp_device(fa_ctx* c, const double* values, int stride, int64_t n
     return FA_SUCCESS;
 }
 
+int fa_grand_device(fa_ctx* c, const double* log_Z, int stride, const double* log_c, int n, double* out, void* stream)
+{
+    if (!c || !log_Z || !log_c || !out || n < 0 || stride < 1) return fa_fail(FA_ERR_INVALID, "bad grand arguments");
+    FA_HIP(hipSetDevice(c->device));
+    FA_HIP(fa::launch_grand(log_Z, stride, log_c, n, out, static_cast<hipStream_t>(stream)));
+    return FA_SUCCESS;
+}
+
 int fa_last_counters(fa_ctx* c, uint32_t* slow_grid, uint32_t* capacity)
 {
     if (!c) return fa_fail(FA_ERR_INVALID, "null context");

@@ -25,6 +25,8 @@ hipError_t launch_boltzmann(const double* d_energy, int stride, long long n, dou
                             hipStream_t stream);
 hipError_t launch_lse_values(const double* d_x, int stride, long long n, void* d_scratch, hipStream_t stream);
 hipError_t launch_zero_u32(unsigned int* p, int n_words, hipStream_t stream);
+hipError_t launch_grand(const double* d_log_Z, int stride, const double* d_log_c, int n, double* d_out,
+                        hipStream_t stream);
 }  // namespace fa
 
 // error state of the calling thread (fa_last_error)

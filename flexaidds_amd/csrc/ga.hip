@@ -14,8 +14,13 @@
 //   copy back   survivors into the population rows
 //   fitness     rank fitness and its prefix sums for the next roulette
 //
-// Nothing is allocated or memset inside, so the sequence is captured once as
-// a HIP graph and replayed.  Random draws are counter-based (fa_rng.h): a
+// A GA holds J independent jobs (ligands) that advance together: rows
+// [0, J*pop) are the parents of job 0, 1, ... and rows [J*pop, J*(pop+new))
+// their children, so one pose launch scores every job's children (the
+// batched form; J = 1 is the plain GA).  Each job keeps its own FP64
+// ensemble, which gives its ln Z and, over jobs, the grand partition
+// function.  Nothing is allocated or memset inside, so the sequence is
+// captured once as a HIP graph and replayed.  Random draws are counter-based (fa_rng.h): a
 // generation is a pure function of the population, the seed and the
 // generation number.
 #include <hip/hip_runtime.h>
@@ -32,24 +37,34 @@
 namespace fa {
 
 struct GaDev {
-    int pop, n_new, n_all, G;
+    int jobs, pop, n_new, n_all, Gmax;
     int intragenes;
     double cross, mut, beta;
     uint64_t seed;
-    const double* lmin;    // [G]
-    const double* ldel;    // [G]
-    const double* lbin;    // [G]
-    int32_t* gi;           // [n_all * G] genes
-    double* gic;           // [n_all * G] internal coordinates
-    fa_cf_result* res;     // [n_all]
-    int32_t* gi2;          // [pop * G] survivors being gathered
+    const int* G;          // [jobs] genes of each job
+    const double* lmin;    // [jobs * Gmax]
+    const double* ldel;
+    const double* lbin;
+    int32_t* gi;           // [jobs * n_all * Gmax] genes: parents, then children
+    double* gic;           // internal coordinates, same rows
+    fa_cf_result* res;     // [jobs * n_all]
+    int32_t* gi2;          // [jobs * pop * Gmax] survivors being gathered
     double* gic2;
     fa_cf_result* res2;
-    double* cum;           // [pop] inclusive prefix sums of the fitness
-    LseState* ens;         // ensemble of every evaluated pose
+    double* cum;           // [pop] inclusive prefix sums of the (LINEAR) fitness
+    LseState* ens;         // [jobs] ensemble of every evaluated pose
     unsigned long long* n_eval;
     unsigned long long* gen;
 };
+
+// row of parent r / child i of job k
+__device__ __forceinline__ size_t parent_row(const GaDev& g, int k, int r) { return (size_t)k * g.pop + r; }
+__device__ __forceinline__ size_t child_row(const GaDev& g, int k, int i)
+{
+    return (size_t)g.jobs * g.pop + (size_t)k * g.n_new + i;
+}
+// draw stream of item t of job k (job 0 uses t itself)
+__device__ __forceinline__ uint64_t job_stream(int k, int t) { return ((uint64_t)k << 32) | (uint32_t)t; }
 
 // LIB/gaboom.cpp:2887-2906 with RandomDouble(gene) = gene / (2^31)
 __host__ __device__ inline double genetoic(double mn, double del, double bin, int32_t gene)
@@ -162,52 +177,68 @@ __global__ void ga_tick_kernel(unsigned long long* gen)
 // initial population, generation 0 (LIB/gaboom.cpp:1871-1902)
 __global__ void ga_init_kernel(GaDev g)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.pop * g.G) return;
-    const int c = t / g.G, j = t % g.G;
-    const int32_t gene = fa_rng_gene(g.seed, 0ull, (uint64_t)c, (uint64_t)j);
-    g.gi[t] = gene;
-    g.gic[t] = genetoic(g.lmin[j], g.ldel[j], g.lbin[j], gene);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.jobs * g.pop * g.Gmax) return;
+    const int k = (int)(t / ((long long)g.pop * g.Gmax));
+    const int c = (int)((t / g.Gmax) % g.pop), j = (int)(t % g.Gmax);
+    const size_t at = parent_row(g, k, c) * g.Gmax + j;
+    if (j >= g.G[k]) {
+        g.gi[at] = 0;
+        g.gic[at] = 0.0;
+        return;
+    }
+    const int32_t gene = fa_rng_gene(g.seed, 0ull, job_stream(k, c), (uint64_t)j);
+    g.gi[at] = gene;
+    g.gic[at] = genetoic(g.lmin[k * g.Gmax + j], g.ldel[k * g.Gmax + j], g.lbin[k * g.Gmax + j], gene);
 }
 
 // one thread per offspring pair (LIB/gaboom.cpp:920-973)
 __global__ void ga_breed_kernel(GaDev g)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int pairs = (g.n_new + 1) / 2;
-    if (t >= pairs) return;
+    const long long tt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tt >= (long long)g.jobs * pairs) return;
+    const int k = (int)(tt / pairs), t = (int)(tt % pairs);
     const uint64_t gen = *g.gen;
-    const int G = g.G;
-    const int p1 = roulette(g.cum, g.pop, fa_rng_u01(g.seed, gen, t, FA_DRAW_P1));
-    const int p2 = roulette(g.cum, g.pop, fa_rng_u01(g.seed, gen, t, FA_DRAW_P2));
+    const uint64_t st = job_stream(k, t);
+    const int G = g.G[k], Gm = g.Gmax;
+    const double* lmin = g.lmin + (size_t)k * Gm;
+    const double* ldel = g.ldel + (size_t)k * Gm;
+    const double* lbin = g.lbin + (size_t)k * Gm;
+    const int p1 = roulette(g.cum, g.pop, fa_rng_u01(g.seed, gen, st, FA_DRAW_P1));
+    const int p2 = roulette(g.cum, g.pop, fa_rng_u01(g.seed, gen, st, FA_DRAW_P2));
     int32_t john[FA_MAX_GENES], mary[FA_MAX_GENES];
     for (int j = 0; j < G; ++j) {
-        john[j] = g.gi[(size_t)p1 * G + j];
-        mary[j] = g.gi[(size_t)p2 * G + j];
+        john[j] = g.gi[parent_row(g, k, p1) * Gm + j];
+        mary[j] = g.gi[parent_row(g, k, p2) * Gm + j];
     }
-    if (fa_rng_u01(g.seed, gen, t, FA_DRAW_CROSS) < g.cross) crossover(john, mary, G, g.intragenes, g.seed, gen, t);
-    mutate(john, G, g.mut, g.seed, gen, t, 0);
-    mutate(mary, G, g.mut, g.seed, gen, t, 1);
-    const int c1 = g.pop + 2 * t;
-    for (int j = 0; j < G; ++j) {
-        g.gi[(size_t)c1 * G + j] = john[j];
-        g.gic[(size_t)c1 * G + j] = genetoic(g.lmin[j], g.ldel[j], g.lbin[j], john[j]);
+    if (fa_rng_u01(g.seed, gen, st, FA_DRAW_CROSS) < g.cross) crossover(john, mary, G, g.intragenes, g.seed, gen, st);
+    mutate(john, G, g.mut, g.seed, gen, st, 0);
+    mutate(mary, G, g.mut, g.seed, gen, st, 1);
+    // genes past the job's own count stay 0 (rows are Gmax wide)
+    const size_t c1 = child_row(g, k, 2 * t);
+    for (int j = 0; j < Gm; ++j) {
+        g.gi[c1 * Gm + j] = (j < G) ? john[j] : 0;
+        g.gic[c1 * Gm + j] = (j < G) ? genetoic(lmin[j], ldel[j], lbin[j], john[j]) : 0.0;
     }
     if (2 * t + 1 < g.n_new) {
-        for (int j = 0; j < G; ++j) {
-            g.gi[(size_t)(c1 + 1) * G + j] = mary[j];
-            g.gic[(size_t)(c1 + 1) * G + j] = genetoic(g.lmin[j], g.ldel[j], g.lbin[j], mary[j]);
+        for (int j = 0; j < Gm; ++j) {
+            g.gi[(c1 + 1) * Gm + j] = (j < G) ? mary[j] : 0;
+            g.gic[(c1 + 1) * Gm + j] = (j < G) ? genetoic(lmin[j], ldel[j], lbin[j], mary[j]) : 0.0;
         }
     }
 }
 
-// energies of rows [off, off + n) into the ensemble state (one block)
-__global__ __launch_bounds__(256) void ga_ensemble_kernel(GaDev g, int off, int n)
+// energies of each job's parents (children = 0) or children (children = 1)
+// into that job's ensemble state; one block per job
+__global__ __launch_bounds__(256) void ga_ensemble_kernel(GaDev g, int children)
 {
     __shared__ LseState sh[256];
+    const int k = blockIdx.x;
+    const int n = children ? g.n_new : g.pop;
     LseState st{0.0, 0.0, 0.0, 0.0};
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const double e = g.res[off + k].cf;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double e = g.res[children ? child_row(g, k, i) : parent_row(g, k, i)].cf;
         lse_push(st, -g.beta * e, e);
     }
     sh[threadIdx.x] = st;
@@ -217,10 +248,10 @@ __global__ __launch_bounds__(256) void ga_ensemble_kernel(GaDev g, int off, int 
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        LseState a = *g.ens;
+        LseState a = g.ens[k];
         lse_merge(a, sh[0]);
-        *g.ens = a;
-        *g.n_eval += (unsigned long long)n;
+        g.ens[k] = a;
+        atomicAdd(g.n_eval, (unsigned long long)n);
     }
 }
 
@@ -231,42 +262,52 @@ __device__ __forceinline__ uint64_t order_key(double e)
     return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
 }
 
-// rank of each of rows [0, n) by energy, ties by index; the best `pop`
-// rows are gathered in order (QuickSort by evalue, LIB/gaboom.cpp:2674-2726)
+// rank of each of a job's first n candidates (its parents, then its
+// children) by energy, ties by candidate index; the best `pop` gathered in
+// order (QuickSort by evalue, LIB/gaboom.cpp:2674-2726).  Job = blockIdx.y.
+__device__ __forceinline__ size_t cand_row(const GaDev& g, int k, int r)
+{
+    return r < g.pop ? parent_row(g, k, r) : child_row(g, k, r - g.pop);
+}
+
 __global__ __launch_bounds__(256) void ga_rank_kernel(GaDev g, int n)
 {
     __shared__ uint64_t keys[1024];
+    const int k = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t ki = (i < n) ? order_key(g.res[i].cf) : 0ull;
+    const uint64_t ki = (i < n) ? order_key(g.res[cand_row(g, k, i)].cf) : 0ull;
     int rank = 0;
     for (int base = 0; base < n; base += 1024) {
         __syncthreads();
-        for (int k = threadIdx.x; k < 1024 && base + k < n; k += blockDim.x) keys[k] = order_key(g.res[base + k].cf);
+        for (int q = threadIdx.x; q < 1024 && base + q < n; q += blockDim.x)
+            keys[q] = order_key(g.res[cand_row(g, k, base + q)].cf);
         __syncthreads();
         const int lim = min(1024, n - base);
         if (i < n)
-            for (int k = 0; k < lim; ++k) {
-                const uint64_t kj = keys[k];
-                rank += (kj < ki) || (kj == ki && base + k < i);
+            for (int q = 0; q < lim; ++q) {
+                const uint64_t kj = keys[q];
+                rank += (kj < ki) || (kj == ki && base + q < i);
             }
     }
     if (i < n && rank < g.pop) {
-        for (int j = 0; j < g.G; ++j) {
-            g.gi2[(size_t)rank * g.G + j] = g.gi[(size_t)i * g.G + j];
-            g.gic2[(size_t)rank * g.G + j] = g.gic[(size_t)i * g.G + j];
+        const size_t src = cand_row(g, k, i), dst = parent_row(g, k, rank);
+        for (int j = 0; j < g.Gmax; ++j) {
+            g.gi2[dst * g.Gmax + j] = g.gi[src * g.Gmax + j];
+            g.gic2[dst * g.Gmax + j] = g.gic[src * g.Gmax + j];
         }
-        g.res2[rank] = g.res[i];
+        g.res2[dst] = g.res[src];
     }
 }
 
 __global__ void ga_copy_back_kernel(GaDev g)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < g.pop * g.G) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long rows = (long long)g.jobs * g.pop;
+    if (t < rows * g.Gmax) {
         g.gi[t] = g.gi2[t];
         g.gic[t] = g.gic2[t];
     }
-    if (t < g.pop) g.res[t] = g.res2[t];
+    if (t < rows) g.res[t] = g.res2[t];
 }
 
 // LINEAR fitness pop - rank and its inclusive prefix sums (one block; the
@@ -292,9 +333,11 @@ __global__ __launch_bounds__(256) void ga_fitness_kernel(GaDev g)
     }
 }
 
+// each job's ensemble thermodynamics, out[k] for job k
 __global__ void ga_thermo_kernel(GaDev g, double T, fa_thermo* out)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *out = thermo_of(*g.ens, T);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < g.jobs) out[k] = thermo_of(g.ens[k], T);
 }
 
 }  // namespace fa
@@ -302,9 +345,9 @@ __global__ void ga_thermo_kernel(GaDev g, double T, fa_thermo* out)
 struct fa_ga {
     fa_ctx* ctx = nullptr;
     fa_ga_params p{};
-    int G = 0;
+    int jobs = 0, Gmax = 0;
     fa::GaDev d{};
-    DevBuf gi, gic, res, gi2, gic2, res2, cum, lim, ens, cnt, lig, thermo;
+    DevBuf gi, gic, res, gi2, gic2, res2, cum, lim, ngenes, ens, cnt, lig, thermo, grand;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -314,99 +357,122 @@ namespace {
 
 int ga_blocks(long long n) { return (int)std::max(1LL, (n + 255) / 256); }
 
-// survivors of rows [0, n) into [0, pop), then their fitness
+// each job's survivors among its first n candidates into its parent rows,
+// then the (shared) rank fitness
 int ga_select(fa_ga* ga, int n)
 {
     fa::GaDev& d = ga->d;
-    hipLaunchKernelGGL(fa::ga_rank_kernel, dim3(ga_blocks(n)), dim3(256), 0, ga->stream, d, n);
-    hipLaunchKernelGGL(fa::ga_copy_back_kernel, dim3(ga_blocks((long long)d.pop * d.G)), dim3(256), 0, ga->stream, d);
+    hipLaunchKernelGGL(fa::ga_rank_kernel, dim3(ga_blocks(n), d.jobs), dim3(256), 0, ga->stream, d, n);
+    hipLaunchKernelGGL(fa::ga_copy_back_kernel, dim3(ga_blocks((long long)d.jobs * d.pop * d.Gmax)), dim3(256), 0,
+                       ga->stream, d);
     hipLaunchKernelGGL(fa::ga_fitness_kernel, dim3(1), dim3(256), 0, ga->stream, d);
     FA_HIP(hipGetLastError());
     return FA_SUCCESS;
 }
 
-// one generation, all on ga->stream (capturable)
+// one generation of every job, all on ga->stream (capturable)
 int ga_generation(fa_ga* ga)
 {
     fa::GaDev& d = ga->d;
+    const size_t rows_parents = (size_t)d.jobs * d.pop;
     hipLaunchKernelGGL(fa::ga_tick_kernel, dim3(1), dim3(64), 0, ga->stream, d.gen);
-    hipLaunchKernelGGL(fa::ga_breed_kernel, dim3(ga_blocks((d.n_new + 1) / 2)), dim3(256), 0, ga->stream, d);
+    hipLaunchKernelGGL(fa::ga_breed_kernel, dim3(ga_blocks((long long)d.jobs * ((d.n_new + 1) / 2))), dim3(256), 0,
+                       ga->stream, d);
     FA_HIP(hipGetLastError());
-    int rc = fa_launch_poses(ga->ctx, d.n_new, d.gic + (size_t)d.pop * d.G, d.G, static_cast<const int*>(ga->lig.p),
-                             d.res + d.pop, nullptr, 0, nullptr, nullptr, 0, ga->stream);
+    const int* lig = static_cast<const int*>(ga->lig.p) + rows_parents;
+    int rc = fa_launch_poses(ga->ctx, d.jobs * d.n_new, d.gic + rows_parents * d.Gmax, d.Gmax, lig,
+                             d.res + rows_parents, nullptr, 0, nullptr, nullptr, 0, ga->stream);
     if (rc != FA_SUCCESS) return rc;
-    hipLaunchKernelGGL(fa::ga_ensemble_kernel, dim3(1), dim3(256), 0, ga->stream, d, d.pop, d.n_new);
+    hipLaunchKernelGGL(fa::ga_ensemble_kernel, dim3(d.jobs), dim3(256), 0, ga->stream, d, 1);
     return ga_select(ga, d.n_all);
 }
 
-}  // namespace
-
-extern "C" {
-
-int fa_ga_create(fa_ctx* c, const fa_ga_params* p, const fa_gene_lim* lim, fa_ga** out)
+int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, const fa_gene_lim* lim, fa_ga** out)
 {
-    if (!c || !p || !lim || !out) return fa_fail(FA_ERR_INVALID, "null argument");
+    if (!c || !p || !lim || !out || !ligands) return fa_fail(FA_ERR_INVALID, "null argument");
     *out = nullptr;
     if (!c->have_system) return fa_fail(FA_ERR_NOSYSTEM, "no system uploaded");
-    if (p->ligand < 0 || p->ligand >= c->n_ligands) return fa_fail(FA_ERR_INVALID, "ligand index out of range");
+    if (n_jobs < 1) return fa_fail(FA_ERR_INVALID, "at least one job");
     if (p->population < 2 || p->offspring < 1) return fa_fail(FA_ERR_INVALID, "population >= 2 and offspring >= 1");
     if (p->temperature <= 0.0) return fa_fail(FA_ERR_INVALID, "temperature must be positive");
-    const int G = c->lig_genes[p->ligand];
-    std::vector<double> host(3 * (size_t)G);
-    for (int j = 0; j < G; ++j) {
-        // set_bins, LIB/gaboom.cpp:2291-2306
-        if (!(lim[j].del > 0.0) || !(lim[j].max >= lim[j].min))
-            return fa_fail(FA_ERR_INVALID, "gene limits need del > 0 and max >= min");
-        double nbin = (lim[j].max - lim[j].min) / lim[j].del;
-        if (nbin - (int)nbin > 0.0) nbin += 1.0;
-        if (lim[j].map) nbin += 1.0;
-        if (!(nbin >= 1.0) || nbin > 1.0e6) return fa_fail(FA_ERR_LIMIT, "between 1 and 1e6 bins per gene");
-        host[j] = lim[j].min;
-        host[G + j] = lim[j].del;
-        host[2 * G + j] = 1.0 / nbin;
+    const long long pop = p->population, n_new = p->offspring;
+    if ((long long)n_jobs * (pop + n_new) > (1LL << 30) || pop + n_new > (1LL << 30))
+        return fa_fail(FA_ERR_LIMIT, "jobs * (population + offspring) above 2^30");
+    int Gmax = 0;
+    std::vector<int> G(n_jobs);
+    for (int k = 0; k < n_jobs; ++k) {
+        if (ligands[k] < 0 || ligands[k] >= c->n_ligands) return fa_fail(FA_ERR_INVALID, "ligand index out of range");
+        G[k] = c->lig_genes[ligands[k]];
+        Gmax = std::max(Gmax, G[k]);
     }
+    if (Gmax < 1 || Gmax > FA_MAX_GENES) return fa_fail(FA_ERR_LIMIT, "between 1 and FA_MAX_GENES genes per ligand");
+    // set_bins, LIB/gaboom.cpp:2291-2306, per job; padded to Gmax
+    std::vector<double> host(3 * (size_t)n_jobs * Gmax, 0.0);
+    const size_t plane = (size_t)n_jobs * Gmax;
+    for (int k = 0, off = 0; k < n_jobs; off += G[k], ++k)
+        for (int j = 0; j < G[k]; ++j) {
+            const fa_gene_lim& L = lim[off + j];
+            if (!(L.del > 0.0) || !(L.max >= L.min)) return fa_fail(FA_ERR_INVALID, "gene limits need del > 0 and max >= min");
+            double nbin = (L.max - L.min) / L.del;
+            if (nbin - (int)nbin > 0.0) nbin += 1.0;
+            if (L.map) nbin += 1.0;
+            if (!(nbin >= 1.0) || nbin > 1.0e6) return fa_fail(FA_ERR_LIMIT, "between 1 and 1e6 bins per gene");
+            host[(size_t)k * Gmax + j] = L.min;
+            host[plane + (size_t)k * Gmax + j] = L.del;
+            host[2 * plane + (size_t)k * Gmax + j] = 1.0 / nbin;
+        }
     FA_HIP(hipSetDevice(c->device));
     fa_ga* ga = new fa_ga();
     ga->ctx = c;
     ga->p = *p;
-    ga->G = G;
-    const int pop = p->population, n_new = p->offspring, n_all = pop + n_new;
-    int rc;
+    ga->jobs = n_jobs;
+    ga->Gmax = Gmax;
+    const size_t J = (size_t)n_jobs, n_all = (size_t)(pop + n_new);
     auto fail_free = [&](int code) {
         fa_ga_destroy(ga);
         return code;
     };
-    if ((rc = fa_ensure(ga->gi, sizeof(int32_t) * (size_t)n_all * G)) ||
-        (rc = fa_ensure(ga->gic, sizeof(double) * (size_t)n_all * G)) ||
-        (rc = fa_ensure(ga->res, sizeof(fa_cf_result) * (size_t)n_all)) ||
-        (rc = fa_ensure(ga->gi2, sizeof(int32_t) * (size_t)pop * G)) ||
-        (rc = fa_ensure(ga->gic2, sizeof(double) * (size_t)pop * G)) ||
-        (rc = fa_ensure(ga->res2, sizeof(fa_cf_result) * (size_t)pop)) ||
-        (rc = fa_ensure(ga->cum, sizeof(double) * (size_t)pop)) || (rc = fa_ensure(ga->lim, sizeof(double) * host.size())) ||
-        (rc = fa_ensure(ga->ens, sizeof(fa::LseState))) || (rc = fa_ensure(ga->cnt, 2 * sizeof(unsigned long long))) ||
-        (rc = fa_ensure(ga->lig, sizeof(int) * (size_t)n_new)) || (rc = fa_ensure(ga->thermo, sizeof(fa_thermo))))
+    int rc;
+    if ((rc = fa_ensure(ga->gi, sizeof(int32_t) * J * n_all * Gmax)) ||
+        (rc = fa_ensure(ga->gic, sizeof(double) * J * n_all * Gmax)) ||
+        (rc = fa_ensure(ga->res, sizeof(fa_cf_result) * J * n_all)) ||
+        (rc = fa_ensure(ga->gi2, sizeof(int32_t) * J * pop * Gmax)) ||
+        (rc = fa_ensure(ga->gic2, sizeof(double) * J * pop * Gmax)) ||
+        (rc = fa_ensure(ga->res2, sizeof(fa_cf_result) * J * pop)) || (rc = fa_ensure(ga->cum, sizeof(double) * pop)) ||
+        (rc = fa_ensure(ga->lim, sizeof(double) * host.size())) || (rc = fa_ensure(ga->ngenes, sizeof(int) * J)) ||
+        (rc = fa_ensure(ga->ens, sizeof(fa::LseState) * J)) || (rc = fa_ensure(ga->cnt, 2 * sizeof(unsigned long long))) ||
+        (rc = fa_ensure(ga->lig, sizeof(int) * J * n_all)) || (rc = fa_ensure(ga->thermo, sizeof(fa_thermo) * J)) ||
+        (rc = fa_ensure(ga->grand, sizeof(double) * (2 * J + 2))))
         return fail_free(rc);
-    std::vector<int> lig(n_new, p->ligand);
+    // ligand of every row: parents of job 0, 1, ..., then children of job 0, 1, ...
+    std::vector<int> lig(J * n_all);
+    for (size_t k = 0; k < J; ++k) {
+        std::fill(lig.begin() + k * pop, lig.begin() + (k + 1) * pop, ligands[k]);
+        std::fill(lig.begin() + J * pop + k * n_new, lig.begin() + J * pop + (k + 1) * n_new, ligands[k]);
+    }
     hipError_t e = hipMemcpy(ga->lim.p, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ga->lig.p, lig.data(), sizeof(int) * lig.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(ga->ens.p, 0, sizeof(fa::LseState));
+    if (e == hipSuccess) e = hipMemcpy(ga->ngenes.p, G.data(), sizeof(int) * J, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(ga->ens.p, 0, sizeof(fa::LseState) * J);
     if (e == hipSuccess) e = hipMemset(ga->cnt.p, 0, 2 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ga->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail_free(fa_fail(FA_ERR_HIP, std::string("GA setup: ") + hipGetErrorString(e)));
     fa::GaDev& d = ga->d;
-    d.pop = pop;
-    d.n_new = n_new;
-    d.n_all = n_all;
-    d.G = G;
+    d.jobs = n_jobs;
+    d.pop = (int)pop;
+    d.n_new = (int)n_new;
+    d.n_all = (int)n_all;
+    d.Gmax = Gmax;
     d.intragenes = p->intragenes ? 1 : 0;
     d.cross = p->cross_rate;
     d.mut = p->mut_rate;
     d.beta = 1.0 / (fa::KB_KCAL * p->temperature);
     d.seed = p->seed;
+    d.G = static_cast<const int*>(ga->ngenes.p);
     const double* lp = static_cast<const double*>(ga->lim.p);
     d.lmin = lp;
-    d.ldel = lp + G;
-    d.lbin = lp + 2 * G;
+    d.ldel = lp + plane;
+    d.lbin = lp + 2 * plane;
     d.gi = static_cast<int32_t*>(ga->gi.p);
     d.gic = static_cast<double*>(ga->gic.p);
     d.res = static_cast<fa_cf_result*>(ga->res.p);
@@ -417,29 +483,44 @@ int fa_ga_create(fa_ctx* c, const fa_ga_params* p, const fa_gene_lim* lim, fa_ga
     d.ens = static_cast<fa::LseState*>(ga->ens.p);
     d.n_eval = static_cast<unsigned long long*>(ga->cnt.p);
     d.gen = d.n_eval + 1;
-    // initial population: draw, evaluate, fold, rank
-    hipLaunchKernelGGL(fa::ga_init_kernel, dim3(ga_blocks((long long)pop * G)), dim3(256), 0, ga->stream, d);
+    // initial population of every job: draw, evaluate, fold, rank
+    hipLaunchKernelGGL(fa::ga_init_kernel, dim3(ga_blocks((long long)J * pop * Gmax)), dim3(256), 0, ga->stream, d);
     if ((e = hipGetLastError()) != hipSuccess)
         return fail_free(fa_fail(FA_ERR_HIP, std::string("GA init: ") + hipGetErrorString(e)));
-    std::vector<int> lig_pop(pop, p->ligand);
-    DevBuf lig0;
-    if ((rc = fa_ensure(lig0, sizeof(int) * (size_t)pop))) return fail_free(rc);
-    e = hipMemcpy(lig0.p, lig_pop.data(), sizeof(int) * lig_pop.size(), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        (void)hipFree(lig0.p);
-        return fail_free(fa_fail(FA_ERR_HIP, std::string("GA init: ") + hipGetErrorString(e)));
-    }
-    rc = fa_launch_poses(c, pop, d.gic, G, static_cast<const int*>(lig0.p), d.res, nullptr, 0, nullptr, nullptr, 0,
-                         ga->stream);
+    rc = fa_launch_poses(c, (int)(J * pop), d.gic, Gmax, static_cast<const int*>(ga->lig.p), d.res, nullptr, 0, nullptr,
+                         nullptr, 0, ga->stream);
     if (rc == FA_SUCCESS) {
-        hipLaunchKernelGGL(fa::ga_ensemble_kernel, dim3(1), dim3(256), 0, ga->stream, d, 0, pop);
-        rc = ga_select(ga, pop);
+        hipLaunchKernelGGL(fa::ga_ensemble_kernel, dim3(n_jobs), dim3(256), 0, ga->stream, d, 0);
+        rc = ga_select(ga, (int)pop);
     }
     e = hipStreamSynchronize(ga->stream);
-    (void)hipFree(lig0.p);
     if (rc != FA_SUCCESS) return fail_free(rc);
     if (e != hipSuccess) return fail_free(fa_fail(FA_ERR_HIP, std::string("GA init: ") + hipGetErrorString(e)));
     *out = ga;
+    return FA_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fa_ga_create(fa_ctx* c, const fa_ga_params* p, const fa_gene_lim* lim, fa_ga** out)
+{
+    if (!p) return fa_fail(FA_ERR_INVALID, "null argument");
+    return ga_build(c, p, 1, &p->ligand, lim, out);
+}
+
+int fa_ga_create_batch(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, const fa_gene_lim* lim,
+                       fa_ga** out)
+{
+    return ga_build(c, p, n_jobs, ligands, lim, out);
+}
+
+int fa_ga_shape(fa_ga* ga, int* n_jobs, int* max_genes)
+{
+    if (!ga) return fa_fail(FA_ERR_INVALID, "null GA");
+    if (n_jobs) *n_jobs = ga->jobs;
+    if (max_genes) *max_genes = ga->Gmax;
     return FA_SUCCESS;
 }
 
@@ -488,9 +569,9 @@ int fa_ga_population(fa_ga* ga, int32_t* genes, double* ic, fa_cf_result* result
     if (!ga) return fa_fail(FA_ERR_INVALID, "null GA");
     FA_HIP(hipSetDevice(ga->ctx->device));
     FA_HIP(hipStreamSynchronize(ga->stream));
-    const size_t rows = (size_t)ga->d.n_all;
-    if (genes) FA_HIP(hipMemcpy(genes, ga->gi.p, sizeof(int32_t) * rows * ga->G, hipMemcpyDeviceToHost));
-    if (ic) FA_HIP(hipMemcpy(ic, ga->gic.p, sizeof(double) * rows * ga->G, hipMemcpyDeviceToHost));
+    const size_t rows = (size_t)ga->jobs * ga->d.n_all;
+    if (genes) FA_HIP(hipMemcpy(genes, ga->gi.p, sizeof(int32_t) * rows * ga->Gmax, hipMemcpyDeviceToHost));
+    if (ic) FA_HIP(hipMemcpy(ic, ga->gic.p, sizeof(double) * rows * ga->Gmax, hipMemcpyDeviceToHost));
     if (results) FA_HIP(hipMemcpy(results, ga->res.p, sizeof(fa_cf_result) * rows, hipMemcpyDeviceToHost));
     return FA_SUCCESS;
 }
@@ -499,12 +580,37 @@ int fa_ga_thermo(fa_ga* ga, fa_thermo* out, uint64_t* n_poses)
 {
     if (!ga || !out) return fa_fail(FA_ERR_INVALID, "null argument");
     FA_HIP(hipSetDevice(ga->ctx->device));
-    hipLaunchKernelGGL(fa::ga_thermo_kernel, dim3(1), dim3(64), 0, ga->stream, ga->d, ga->p.temperature,
-                       static_cast<fa_thermo*>(ga->thermo.p));
+    hipLaunchKernelGGL(fa::ga_thermo_kernel, dim3(ga_blocks(ga->jobs)), dim3(256), 0, ga->stream, ga->d,
+                       ga->p.temperature, static_cast<fa_thermo*>(ga->thermo.p));
     FA_HIP(hipGetLastError());
     FA_HIP(hipStreamSynchronize(ga->stream));
-    FA_HIP(hipMemcpy(out, ga->thermo.p, sizeof(fa_thermo), hipMemcpyDeviceToHost));
+    FA_HIP(hipMemcpy(out, ga->thermo.p, sizeof(fa_thermo) * ga->jobs, hipMemcpyDeviceToHost));
     if (n_poses) FA_HIP(hipMemcpy(n_poses, ga->d.n_eval, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return FA_SUCCESS;
+}
+
+int fa_ga_grand(fa_ga* ga, const double* conc_M, double* log_xi, double* p_bound, double* p_empty)
+{
+    if (!ga || !conc_M) return fa_fail(FA_ERR_INVALID, "null argument");
+    const int J = ga->jobs;
+    std::vector<double> lc(J);
+    for (int k = 0; k < J; ++k) {
+        if (!(conc_M[k] > 0.0)) return fa_fail(FA_ERR_INVALID, "concentrations must be positive");
+        lc[k] = std::log(conc_M[k]);
+    }
+    FA_HIP(hipSetDevice(ga->ctx->device));
+    double* g = static_cast<double*>(ga->grand.p);  // [log c (J) | log Xi, p_empty, p_bound (J)]
+    FA_HIP(hipMemcpyAsync(g, lc.data(), sizeof(double) * J, hipMemcpyHostToDevice, ga->stream));
+    fa_thermo* th = static_cast<fa_thermo*>(ga->thermo.p);
+    hipLaunchKernelGGL(fa::ga_thermo_kernel, dim3(ga_blocks(J)), dim3(256), 0, ga->stream, ga->d, ga->p.temperature, th);
+    FA_HIP(hipGetLastError());
+    FA_HIP(fa::launch_grand(&th->log_Z, (int)(sizeof(fa_thermo) / sizeof(double)), g, J, g + J, ga->stream));
+    std::vector<double> r(J + 2);
+    FA_HIP(hipMemcpyAsync(r.data(), g + J, sizeof(double) * (J + 2), hipMemcpyDeviceToHost, ga->stream));
+    FA_HIP(hipStreamSynchronize(ga->stream));
+    if (log_xi) *log_xi = r[0];
+    if (p_empty) *p_empty = r[1];
+    if (p_bound) std::copy(r.begin() + 2, r.end(), p_bound);
     return FA_SUCCESS;
 }
 
@@ -524,8 +630,8 @@ int fa_ga_destroy(fa_ga* ga)
     if (ga->stream) (void)hipStreamSynchronize(ga->stream);
     if (ga->exec) (void)hipGraphExecDestroy(ga->exec);
     if (ga->graph) (void)hipGraphDestroy(ga->graph);
-    for (DevBuf* b : {&ga->gi, &ga->gic, &ga->res, &ga->gi2, &ga->gic2, &ga->res2, &ga->cum, &ga->lim, &ga->ens,
-                      &ga->cnt, &ga->lig, &ga->thermo})
+    for (DevBuf* b : {&ga->gi, &ga->gic, &ga->res, &ga->gi2, &ga->gic2, &ga->res2, &ga->cum, &ga->lim, &ga->ngenes,
+                      &ga->ens, &ga->cnt, &ga->lig, &ga->thermo, &ga->grand})
         if (b->p) (void)hipFree(b->p);
     if (ga->stream) (void)hipStreamDestroy(ga->stream);
     delete ga;

@@ -152,4 +152,45 @@ hipError_t launch_lse_values(const double* d_x, int stride, long long n, void* d
     return hipGetLastError();
 }
 
+// Grand partition function over n ligands (GrandPartitionFunction::
+// compute_log_Xi_fresh and the occupancies, LIB/GrandPartitionFunction.cpp:112-160):
+// log zZ_i = ln c_i + ln Z_i, ln Xi = LSE(0, log zZ_i), p_i = exp(log zZ_i -
+// ln Xi), p_empty = exp(-ln Xi).  One block; out = [ln Xi, p_empty, p_0..].
+__global__ __launch_bounds__(256) void grand_kernel(const double* __restrict__ log_Z, int stride,
+                                                    const double* __restrict__ log_c, int n, double* __restrict__ out)
+{
+    __shared__ double sh[256];
+    double m = 0.0;  // the empty state's term is exp(0)
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, log_c[i] + log_Z[(size_t)i * stride]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    m = sh[0];
+    __syncthreads();
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += exp(log_c[i] + log_Z[(size_t)i * stride] - m);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double lx = m + log(exp(-m) + sh[0]);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) out[2 + i] = exp(log_c[i] + log_Z[(size_t)i * stride] - lx);
+    if (threadIdx.x == 0) {
+        out[0] = lx;
+        out[1] = exp(-lx);
+    }
+}
+
+hipError_t launch_grand(const double* d_log_Z, int stride, const double* d_log_c, int n, double* d_out,
+                        hipStream_t stream)
+{
+    hipLaunchKernelGGL(grand_kernel, dim3(1), dim3(256), 0, stream, d_log_Z, stride, d_log_c, n, d_out);
+    return hipGetLastError();
+}
+
 }  // namespace fa

@@ -70,6 +70,10 @@ _SIGS = {
     "fa_kernel_lds_bytes": ([], C.c_int),
     "fa_ctx_launch_waves": ([_P], C.c_int),
     "fa_ga_create": ([_P, _P, _P, _P], C.c_int),
+    "fa_ga_create_batch": ([_P, _P, C.c_int, _P, _P, _P], C.c_int),
+    "fa_ga_shape": ([_P, _P, _P], C.c_int),
+    "fa_ga_grand": ([_P, _P, _P, _P, _P], C.c_int),
+    "fa_grand_device": ([_P, _P, C.c_int, _P, C.c_int, _P, _P], C.c_int),
     "fa_ga_run": ([_P, C.c_int, C.c_int, _P], C.c_int),
     "fa_ga_population": ([_P, _P, _P, _P], C.c_int),
     "fa_ga_thermo": ([_P, _P, _P], C.c_int),
@@ -190,6 +194,12 @@ class Context:
                                      C.c_void_p(stream) if stream else None))
         return {f: getattr(th, f) for f, _ in abi.Thermo._fields_}
 
+    def grand_device(self, log_z_ptr: int, stride: int, log_c_ptr: int, n: int, out_ptr: int, stream: int = 0):
+        """Grand partition function on the device: out[0] = ln Xi, out[1] = p_empty,
+        out[2 + i] = p_bound of ligand i (device pointers, asynchronous)."""
+        _check(_lib.fa_grand_device(self._ctx, C.c_void_p(log_z_ptr), stride, C.c_void_p(log_c_ptr), n,
+                                    C.c_void_p(out_ptr), C.c_void_p(stream) if stream else None))
+
     def opcount(self) -> dict:
         o = OpCount()
         _check(_lib.fa_last_opcount(self._ctx, C.byref(o)))
@@ -230,18 +240,32 @@ class GAParams(C.Structure):
 class GA:
     """Genetic algorithm running on the device (fa_ga_* of the C ABI).
 
-    `limits` is a sequence of (min, max, del, map) per gene of the ligand."""
+    `limits` is a sequence of (min, max, del, map) per gene of the ligand.
+    With `ligands` (a sequence of ligand indices) the GA is batched: one job
+    per ligand, `limits` then a sequence of per-job limit sequences."""
 
     def __init__(self, ctx: Context, limits, ligand=0, population=2000, offspring=None, cross_rate=0.8,
-                 mut_rate=0.03, temperature=300.0, seed=1, intragenes=False):
+                 mut_rate=0.03, temperature=300.0, seed=1, intragenes=False, ligands=None):
         self.ctx = ctx
-        n = len(limits)
-        self._lim = (GeneLim * n)(*[GeneLim(float(a), float(b), float(d), int(m), 0) for a, b, d, m in limits])
+        per_job = [limits] if ligands is None else list(limits)
+        flat = [x for job in per_job for x in job]
+        self._lim = (GeneLim * len(flat))(*[GeneLim(float(a), float(b), float(d), int(m), 0)
+                                             for a, b, d, m in flat])
         self.params = GAParams(ligand, population, population if offspring is None else offspring,
                                1 if intragenes else 0, cross_rate, mut_rate, temperature, seed)
-        self.genes = n
+        self.job_genes = [len(j) for j in per_job]
         self._ga = C.c_void_p()
-        _check(_lib.fa_ga_create(ctx._ctx, C.byref(self.params), self._lim, C.byref(self._ga)))
+        if ligands is None:
+            _check(_lib.fa_ga_create(ctx._ctx, C.byref(self.params), self._lim, C.byref(self._ga)))
+        else:
+            if len(ligands) != len(per_job):
+                raise ValueError("one limit sequence per ligand")
+            self._ligs = (C.c_int * len(ligands))(*ligands)
+            _check(_lib.fa_ga_create_batch(ctx._ctx, C.byref(self.params), len(ligands), self._ligs, self._lim,
+                                           C.byref(self._ga)))
+        j, g = C.c_int(), C.c_int()
+        _check(_lib.fa_ga_shape(self._ga, C.byref(j), C.byref(g)))
+        self.jobs, self.genes = int(j.value), int(g.value)
 
     def run(self, generations: int, graph: bool = True) -> float:
         ms = C.c_float()
@@ -249,19 +273,43 @@ class GA:
         return float(ms.value)
 
     def population(self):
-        """(genes int32 [rows, G], ic [rows, G], results) with rows = population + offspring."""
-        rows = self.params.population + self.params.offspring
+        """(genes int32 [rows, Gmax], ic [rows, Gmax], results [rows]); rows are the
+        survivors of each job (population each), then the children of each job."""
+        rows = self.jobs * (self.params.population + self.params.offspring)
         g = np.zeros((rows, self.genes), np.int32)
         ic = np.zeros((rows, self.genes), np.float64)
         res = np.zeros(rows, abi.CF_DTYPE)
         _check(_lib.fa_ga_population(self._ga, g.ctypes.data, ic.ctypes.data, res.ctypes.data))
         return g, ic, res
 
+    def job_rows(self, k: int):
+        """(survivor rows, child rows) of job k in population()'s arrays."""
+        pop, new = self.params.population, self.params.offspring
+        return (slice(k * pop, (k + 1) * pop),
+                slice(self.jobs * pop + k * new, self.jobs * pop + (k + 1) * new))
+
     def thermo(self):
-        t = abi.Thermo()
+        """Job 0's ensemble (dict) and the poses evaluated over all jobs."""
+        t, n = self.thermo_jobs()
+        return t[0], n
+
+    def thermo_jobs(self):
+        """Each job's ensemble thermodynamics (list of dicts), poses evaluated."""
+        t = (abi.Thermo * self.jobs)()
         n = C.c_uint64()
-        _check(_lib.fa_ga_thermo(self._ga, C.byref(t), C.byref(n)))
-        return {f: getattr(t, f) for f, _ in abi.Thermo._fields_}, int(n.value)
+        _check(_lib.fa_ga_thermo(self._ga, t, C.byref(n)))
+        return [{f: getattr(x, f) for f, _ in abi.Thermo._fields_} for x in t], int(n.value)
+
+    def grand(self, conc_M):
+        """Grand partition function over the jobs at concentrations conc_M (mol/L):
+        {"log_xi", "p_bound" [jobs], "p_empty"}."""
+        c = np.ascontiguousarray(conc_M, np.float64)
+        if c.shape != (self.jobs,):
+            raise ValueError("one concentration per job")
+        lx, pe = C.c_double(), C.c_double()
+        pb = np.zeros(self.jobs)
+        _check(_lib.fa_ga_grand(self._ga, c.ctypes.data, C.byref(lx), pb.ctypes.data, C.byref(pe)))
+        return {"log_xi": float(lx.value), "p_bound": pb, "p_empty": float(pe.value)}
 
     def generation(self) -> int:
         g = C.c_uint64()

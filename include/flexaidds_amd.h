@@ -117,6 +117,15 @@ int fa_boltzmann_device(fa_ctx* ctx, const double* energy, int stride, int64_t n
  * pointer `values`, host `out`; synchronous. */
 int fa_log_sum_exp_device(fa_ctx* ctx, const double* values, int stride, int64_t n, double* out, void* stream);
 
+/* Device grand partition function: log_Z[i * stride] and log_c[i] (ln of
+ * concentration) for n ligands, device pointers; out (device, n + 2
+ * doubles) receives [ln Xi, p_empty, p_0 .. p_{n-1}] (GrandPartitionFunction
+ * compute_log_Xi_fresh / binding_probability / empty_probability,
+ * LIB/GrandPartitionFunction.cpp:112-160).
+ * Asynchronous on `stream`. */
+int fa_grand_device(fa_ctx* ctx, const double* log_Z, int stride, const double* log_c, int n, double* out,
+                    void* stream);
+
 /* Counters of the last launch: poses that needed a per-pose cell grid, poses
  * that hit a device capacity.  Synchronises the device. */
 int fa_last_counters(fa_ctx* ctx, uint32_t* slow_grid, uint32_t* capacity);
@@ -166,17 +175,36 @@ typedef struct fa_ga fa_ga;
 /* Allocates the GA on the context's device, draws and evaluates the initial
  * population (generate_random_individual, LIB/gaboom.cpp:1871-1902). */
 int fa_ga_create(fa_ctx* ctx, const fa_ga_params* params, const fa_gene_lim* limits, fa_ga** out);
+/* Batched form: n_jobs ligands (ligands[k], params->ligand ignored) docked
+ * together with the same GA parameters; each generation breeds every job and
+ * scores all children in one pose launch.  limits holds each job's gene
+ * limits back to back (job k's G_k genes follow job k-1's).  Job k draws
+ * from streams (k << 32) | item, so job 0 reproduces the single-job GA.
+ * Replaces looping GA() over ligands (LIB/gaboom.cpp:346-500). */
+int fa_ga_create_batch(fa_ctx* ctx, const fa_ga_params* params, int n_jobs, const int* ligands,
+                       const fa_gene_lim* limits, fa_ga** out);
+/* Number of jobs and the row stride (largest gene count) of the population. */
+int fa_ga_shape(fa_ga* ga, int* n_jobs, int* max_genes);
 /* Runs `generations` generations on the GA's stream and waits.  With
  * use_graph != 0 one generation is captured once as a HIP graph and
  * replayed.  *ms (optional) receives the device time of the run. */
 int fa_ga_run(fa_ga* ga, int generations, int use_graph, float* ms);
-/* Rows [0, population): survivors by increasing energy; rows
- * [population, population + offspring): the last generation's children as
- * bred.  Any output may be null.  Synchronous. */
+/* Rows of max_genes values: [0, J*population) the survivors of job 0, 1, ...
+ * (population rows each, by increasing energy); then [J*population,
+ * J*(population + offspring)) the last generation's children of job 0, 1,
+ * ... as bred.  Genes past a job's own count are 0.  Any output may be
+ * null.  Synchronous. */
 int fa_ga_population(fa_ga* ga, int32_t* genes, double* ic, fa_cf_result* results);
-/* Ensemble thermodynamics over every pose evaluated so far (initial
- * population and all children), FP64 on the device; *n_poses counts them. */
+/* Ensemble thermodynamics of each job (out[n_jobs]) over every pose it has
+ * evaluated (initial population and all children), FP64 on the device;
+ * *n_poses counts the poses of all jobs. */
 int fa_ga_thermo(fa_ga* ga, fa_thermo* out, uint64_t* n_poses);
+/* Grand partition function over the jobs' ensembles at concentrations
+ * conc_M[n_jobs] (mol/L): ln Xi = LSE(0, ln c_k + ln Z_k), p_bound[k] =
+ * exp(ln c_k + ln Z_k - ln Xi), p_empty = exp(-ln Xi)
+ * (GrandPartitionFunction::log_Xi, binding_probability, empty_probability,
+ * LIB/GrandPartitionFunction.h:85-91).  Outputs may be null. */
+int fa_ga_grand(fa_ga* ga, const double* conc_M, double* log_xi, double* p_bound, double* p_empty);
 int fa_ga_generation(fa_ga* ga, uint64_t* generation);
 int fa_ga_destroy(fa_ga* ga);
 

@@ -137,19 +137,22 @@ extern "C" {
 
 double fa_oracle_genetoic(const fa_gene_lim* g, int32_t gene) { return genetoic(*g, 1.0 / nbins(*g), gene); }
 
-/* initial population: genes [pop * G] and their internal coordinates */
-void fa_oracle_ga_init(uint64_t seed, int pop, int G, const fa_gene_lim* lim, int32_t* genes, double* ic)
+/* initial population of job `job` (draw streams (job << 32) | c): genes
+ * [pop * G] and their internal coordinates */
+void fa_oracle_ga_init(uint64_t seed, uint64_t job, int pop, int G, const fa_gene_lim* lim, int32_t* genes,
+                       double* ic)
 {
     for (int c = 0; c < pop; ++c)
         for (int j = 0; j < G; ++j) {
-            const int32_t g = fa_rng_gene(seed, 0, (uint64_t)c, (uint64_t)j);
+            const int32_t g = fa_rng_gene(seed, 0, (job << 32) | (uint64_t)c, (uint64_t)j);
             genes[(std::size_t)c * G + j] = g;
             ic[(std::size_t)c * G + j] = genetoic(lim[j], 1.0 / nbins(lim[j]), g);
         }
 }
 
-/* one generation's children from parents ranked best first (LINEAR fitness) */
-void fa_oracle_ga_breed(uint64_t seed, uint64_t gen, int pop, int n_new, int G, int intragenes, double cross_rate,
+/* one generation's children of job `job` from its parents ranked best first
+ * (LINEAR fitness); pair t draws from stream (job << 32) | t */
+void fa_oracle_ga_breed(uint64_t seed, uint64_t job, uint64_t gen, int pop, int n_new, int G, int intragenes, double cross_rate,
                         double mut_rate, const fa_gene_lim* lim, const int32_t* parents, int32_t* child_genes,
                         double* child_ic)
 {
@@ -157,15 +160,17 @@ void fa_oracle_ga_breed(uint64_t seed, uint64_t gen, int pop, int n_new, int G, 
     for (int i = 0; i < pop; ++i) fit[i] = (double)(pop - i);
     std::vector<int32_t> john(G), mary(G);
     for (int t = 0; 2 * t < n_new; ++t) {
-        const int p1 = roulette(fit.data(), pop, fa_rng_u01(seed, gen, t, FA_DRAW_P1));
-        const int p2 = roulette(fit.data(), pop, fa_rng_u01(seed, gen, t, FA_DRAW_P2));
+        const uint64_t st = (job << 32) | (uint64_t)t;
+        const int p1 = roulette(fit.data(), pop, fa_rng_u01(seed, gen, st, FA_DRAW_P1));
+        const int p2 = roulette(fit.data(), pop, fa_rng_u01(seed, gen, st, FA_DRAW_P2));
         for (int j = 0; j < G; ++j) {
             john[j] = parents[(std::size_t)p1 * G + j];
             mary[j] = parents[(std::size_t)p2 * G + j];
         }
-        if (fa_rng_u01(seed, gen, t, FA_DRAW_CROSS) < cross_rate) crossover(john.data(), mary.data(), G, intragenes, seed, gen, t);
-        mutate(john.data(), G, mut_rate, seed, gen, t, 0);
-        mutate(mary.data(), G, mut_rate, seed, gen, t, 1);
+        if (fa_rng_u01(seed, gen, st, FA_DRAW_CROSS) < cross_rate)
+            crossover(john.data(), mary.data(), G, intragenes, seed, gen, st);
+        mutate(john.data(), G, mut_rate, seed, gen, st, 0);
+        mutate(mary.data(), G, mut_rate, seed, gen, st, 1);
         for (int k = 0; k < 2 && 2 * t + k < n_new; ++k) {
             const int32_t* src = k ? mary.data() : john.data();
             for (int j = 0; j < G; ++j) {

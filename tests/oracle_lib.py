@@ -149,29 +149,29 @@ def gene_lims(limits):
 
 
 def ga_bind(lib):
-    lib.fa_oracle_ga_init.argtypes = [C.c_uint64, C.c_int, C.c_int, _P, _P, _P]
-    lib.fa_oracle_ga_breed.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+    lib.fa_oracle_ga_init.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, _P, _P, _P]
+    lib.fa_oracle_ga_breed.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                        C.c_double, _P, _P, _P, _P]
     lib.fa_oracle_ga_survivors.argtypes = [C.c_int, C.c_int, _P, _P]
     return lib
 
 
-def ga_init(lib, seed, pop, limits):
+def ga_init(lib, seed, pop, limits, job=0):
     G = len(limits)
     genes = np.zeros((pop, G), np.int32)
     ic = np.zeros((pop, G))
     lim = gene_lims(limits)
-    lib.fa_oracle_ga_init(seed, pop, G, lim, genes.ctypes.data, ic.ctypes.data)
+    lib.fa_oracle_ga_init(seed, job, pop, G, lim, genes.ctypes.data, ic.ctypes.data)
     return genes, ic
 
 
-def ga_breed(lib, seed, gen, parents, n_new, limits, cross=0.8, mut=0.03, intragenes=0):
+def ga_breed(lib, seed, gen, parents, n_new, limits, cross=0.8, mut=0.03, intragenes=0, job=0):
     parents = np.ascontiguousarray(parents, np.int32)
     pop, G = parents.shape
     child = np.zeros((n_new, G), np.int32)
     ic = np.zeros((n_new, G))
     lim = gene_lims(limits)
-    lib.fa_oracle_ga_breed(seed, gen, pop, n_new, G, intragenes, cross, mut, lim, parents.ctypes.data,
+    lib.fa_oracle_ga_breed(seed, job, gen, pop, n_new, G, intragenes, cross, mut, lim, parents.ctypes.data,
                            child.ctypes.data, ic.ctypes.data)
     return child, ic
 

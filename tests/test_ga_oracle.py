@@ -23,8 +23,9 @@ def ga(oracle):
     lib = oracle
     lib.fa_oracle_genetoic.argtypes = [C.c_void_p, C.c_int32]
     lib.fa_oracle_genetoic.restype = C.c_double
-    lib.fa_oracle_ga_init.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
-    lib.fa_oracle_ga_breed.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+    lib.fa_oracle_ga_init.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]
+    lib.fa_oracle_ga_breed.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                        C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.fa_oracle_rng_u01.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]
     lib.fa_oracle_rng_u01.restype = C.c_double
@@ -74,7 +75,7 @@ def test_mutation_flips_bits_at_the_rate(ga):
     parents = np.full((pop, G), 0x1234567, np.int32)     # identical parents: crossover is a no-op
     child = np.zeros((n_new, G), np.int32)
     ic = np.zeros((n_new, G))
-    ga.fa_oracle_ga_breed(5, 1, pop, n_new, G, 0, 0.8, rate, lim, parents.ctypes.data, child.ctypes.data,
+    ga.fa_oracle_ga_breed(5, 0, 1, pop, n_new, G, 0, 0.8, rate, lim, parents.ctypes.data, child.ctypes.data,
                           ic.ctypes.data)
     flips = np.unpackbits((child ^ parents[0]).view(np.uint8)).sum()
     n = n_new * G * 32
@@ -107,10 +108,27 @@ def test_init_population_uses_the_limits(ga):
     lim = (Lim * G)(*[Lim(a, b, d, m, 0) for a, b, d, m in lims])
     genes = np.zeros((500, G), np.int32)
     ic = np.zeros((500, G))
-    ga.fa_oracle_ga_init(9, 500, G, lim, genes.ctypes.data, ic.ctypes.data)
+    ga.fa_oracle_ga_init(9, 0, 500, G, lim, genes.ctypes.data, ic.ctypes.data)
     assert np.all(genes >= 1)
     lo = np.array([l[0] for l in lims])
     hi = np.array([l[1] for l in lims])
     step = np.array([l[2] for l in lims])
     assert np.all(ic >= lo) and np.all(ic <= hi + step)
     assert np.allclose((ic - lo) / step, np.round((ic - lo) / step))
+
+
+def test_jobs_draw_from_disjoint_streams(ga):
+    # job k's item t draws from stream (k << 32) | t: job 0 is the plain GA,
+    # other jobs are independent of it
+    G, pop = 6, 300
+    lim = (Lim * G)(*[Lim(-180.0, 180.0, 2.0, 0, 0) for _ in range(G)])
+    rows = []
+    for job in (0, 1, 2):
+        genes = np.zeros((pop, G), np.int32)
+        ic = np.zeros((pop, G))
+        ga.fa_oracle_ga_init(4, job, pop, G, lim, genes.ctypes.data, ic.ctypes.data)
+        rows.append(genes)
+    assert not np.array_equal(rows[0], rows[1]) and not np.array_equal(rows[1], rows[2])
+    # a gene's high bits agree between jobs no more often than chance
+    same = np.mean((rows[0] >> 24) == (rows[1] >> 24))
+    assert same < 0.02
