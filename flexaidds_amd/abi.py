@@ -38,7 +38,8 @@ class LigandDesc(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("permeability", C.c_float), ("dee_clash", C.c_float), ("intramolecular", C.c_int32),
-                ("solventterm", C.c_float), ("ntypes", C.c_int32), ("emat", _f32p)]
+                ("solventterm", C.c_float), ("ntypes", C.c_int32), ("emat", _f32p),
+                ("normalize_area", C.c_int32), ("dens_off", _i32p), ("dens_xy", _f32p)]
 
 
 class CFResult(C.Structure):
@@ -116,4 +117,9 @@ class Descriptors:
         d.permeability, d.dee_clash = p.permeability, p.dee_clash
         d.intramolecular, d.solventterm, d.ntypes = p.intramolecular, p.solventterm, p.ntypes
         d.emat = _ptr(self._arr(p.emat, np.float32), _f32p)
+        d.normalize_area = int(getattr(p, "normalize_area", 0))
+        off = getattr(p, "dens_off", None)
+        if off is not None:
+            d.dens_off = _ptr(self._arr(off, np.int32), _i32p)
+            d.dens_xy = _ptr(self._arr(np.asarray(p.dens_xy, np.float32).reshape(-1), np.float32), _f32p)
         return d

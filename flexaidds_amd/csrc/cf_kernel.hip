@@ -346,6 +346,31 @@ __device__ __noinline__ double spherical_arc_call(double ax, double ay, double a
 // one spherical triangle of a face fan (LIB/Vcontacts.cpp:926-940): the term
 // the reference adds for the triangle with cosines c0 (vertex 0 to q),
 // cq (q to q+1) and c1 (vertex 0 to q+1); zero when it adds nothing
+// Scored value of one type pair k for `area` of an atom whose whole sphere
+// is `surf`: weight-1 pairs give y * area (y * area / surf with
+// normalize_area, LIB/vcfunction.cpp:343-353, :479-491); density pairs give
+// get_yval(area / surf) (LIB/vcfunction.cpp:576-621): 0 left of the first
+// node, the last y right of the last node, linear in between, the node
+// differences taken in float as the reference's float fields are.
+__device__ __forceinline__ double pair_term(const DevParams& prm, int k, double area, double surf)
+{
+    if (prm.dens_off) {
+        const int b = prm.dens_off[k], e = prm.dens_off[k + 1];
+        if (e > b) {
+            const double rel = area / surf;
+            int i = b;
+            while (i + 1 < e && rel > (double)prm.dens_xy[i + 1].x) ++i;
+            const float2 p0 = prm.dens_xy[i];
+            if ((double)p0.x > rel) return 0.0;
+            if (i + 1 == e) return (double)p0.y;
+            const float2 p1 = prm.dens_xy[i + 1];
+            return (double)p0.y + (rel - (double)p0.x) / (double)(p1.x - p0.x) * (double)(p1.y - p0.y);
+        }
+    }
+    const double y = (double)prm.emat[k];
+    return prm.normalize_area ? y * area / surf : y * area;
+}
+
 __device__ __forceinline__ double sph_triangle(double c0, double cq, double c1, double ro)
 {
     const double U = sqrt((1 + c0) * (1 + cq) * (1 + c1) / 8.0);
@@ -2216,6 +2241,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 {
                     const double radA = (double)ra;
                     const double radoA = radA + RW;
+                    const double surfA = 4.0 * PI_REF * radoA * radoA;
                     const double dee = (double)prm.dee_clash;
                     const int nt = prm.ntypes;
                     const int ta = lig.type[a];
@@ -2267,7 +2293,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                                     wal += (double)KWALL_F * (1.0 / (d6 * d6) - 1.0 / (cr6 * cr6));
                                     if (dist <= dee * rAB) rcl = 1;
                                 }
-                                if (prm.intramolecular || !lj) com += (double)prm.emat[(ta - 1) * nt + (tb - 1)] * ar;
+                                if (prm.intramolecular || !lj) com += pair_term(prm, (ta - 1) * nt + (tb - 1), ar, surfA);
                             }
                         }
                         seen += __popcll(m);
@@ -2278,13 +2304,13 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     com = wave_sum_d(com);
                     wal = wave_sum_d(wal);
                     rcl = wave_sum_i(rcl);
-                    double SAS = 4.0 * PI_REF * radoA * radoA - sas_sub;
+                    double SAS = surfA - sas_sub;
                     if (SAS < 0.0) SAS = 0.0;
                     r_com += com;
                     r_wal += wal;
                     r_tot += SAS;
                     r_sas += (prm.solventterm != 0.0f) ? (double)prm.solventterm * SAS
-                                                        : (double)prm.emat[(ta - 1) * nt + (nt - 1)] * SAS;
+                                                        : pair_term(prm, (ta - 1) * nt + (nt - 1), SAS, surfA);
                     r_rclash |= rcl ? 1 : 0;
                     if (a < WAVE)
                         dm0 |= 1ull << a;
@@ -2305,7 +2331,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                         add_tot += SAS;
                         add_sas += (prm.solventterm != 0.0f)
                                        ? (double)prm.solventterm * SAS
-                                       : (double)prm.emat[(ti - 1) * prm.ntypes + (prm.ntypes - 1)] * SAS;
+                                       : pair_term(prm, (ti - 1) * prm.ntypes + (prm.ntypes - 1), SAS, SAS);
                     }
                 }
                 r_tot += wave_sum_d(add_tot);

@@ -219,6 +219,13 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
     if (!c || !r || !prm || !ligs || n_ligands < 1) return fa_fail(FA_ERR_INVALID, "null argument");
     if (r->n_atoms < 1 || !r->xyz || !r->radius || !r->type) return fa_fail(FA_ERR_INVALID, "empty receptor");
     if (prm->ntypes < 1 || !prm->emat) return fa_fail(FA_ERR_INVALID, "energy matrix missing");
+    const size_t npairs = (size_t)prm->ntypes * prm->ntypes;
+    if (prm->dens_off) {
+        if (prm->dens_off[0] != 0) return fa_fail(FA_ERR_INVALID, "density offsets must start at 0");
+        for (size_t k = 0; k < npairs; ++k)
+            if (prm->dens_off[k + 1] < prm->dens_off[k]) return fa_fail(FA_ERR_INVALID, "density offsets must not decrease");
+        if (prm->dens_off[npairs] > 0 && !prm->dens_xy) return fa_fail(FA_ERR_INVALID, "density nodes missing");
+    }
     for (int i = 0; i < r->n_atoms; ++i)
         if (r->type[i] < 1 || r->type[i] > prm->ntypes) return fa_fail(FA_ERR_INVALID, "receptor type out of range");
     for (int li = 0; li < n_ligands; ++li) {
@@ -250,7 +257,17 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
     c->prm.solventterm = prm->solventterm;
     c->prm.intramolecular = prm->intramolecular;
     c->prm.ntypes = prm->ntypes;
-    if ((rc = upload(c, prm->emat, (size_t)prm->ntypes * prm->ntypes, &c->prm.emat))) return rc;
+    if ((rc = upload(c, prm->emat, npairs, &c->prm.emat))) return rc;
+    c->prm.normalize_area = prm->normalize_area ? 1 : 0;
+    c->prm.dens_off = nullptr;
+    c->prm.dens_xy = nullptr;
+    if (prm->dens_off && prm->dens_off[npairs] > 0) {
+        const size_t nn = (size_t)prm->dens_off[npairs];
+        std::vector<float2> xy(nn);
+        for (size_t i = 0; i < nn; ++i) xy[i] = make_float2(prm->dens_xy[2 * i], prm->dens_xy[2 * i + 1]);
+        if ((rc = upload(c, prm->dens_off, npairs + 1, &c->prm.dens_off)) || (rc = upload(c, xy.data(), nn, &c->prm.dens_xy)))
+            return rc;
+    }
     // grids: one receptor cell list per distinct (rounded bounds, width)
     // (LIB/Vcontacts.cpp:1594-1715 with no scorable atom outside the bounds)
     std::map<GridKey, int> grid_ids;

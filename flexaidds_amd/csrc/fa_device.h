@@ -62,6 +62,9 @@ struct DevParams {
     int intramolecular;
     int ntypes;
     const float* emat;              // [ntypes * ntypes]
+    int normalize_area;
+    const int* dens_off;            // [ntypes * ntypes + 1] or null (all pairs weight 1)
+    const float2* dens_xy;          // density nodes (x, y)
 };
 
 // One launch over a batch of poses.

@@ -94,6 +94,9 @@ class Params:
     dee_clash: float = 0.5
     intramolecular: int = 1
     solventterm: float = 0.0
+    normalize_area: int = 0
+    dens_off: np.ndarray = None      # (ntypes*ntypes + 1,) int32, density-function pairs (optional)
+    dens_xy: np.ndarray = None       # (n_nodes, 2) float32
 
 
 # ---------------------------------------------------------------------------
@@ -463,6 +466,31 @@ def make_emat(rng, ntypes=40) -> np.ndarray:
     m = rng.normal(size=(ntypes, ntypes))
     m = np.triu(m) + np.triu(m, 1).T
     return m.astype(np.float32).reshape(-1)
+
+
+def make_density_emat(rng, ntypes=40, frac=0.5, max_nodes=6):
+    """Density-function pairs for a fraction of the type pairs (symmetric):
+    2..max_nodes nodes with x ascending in [0, 1.2) (relative contact area)
+    and y ~ N(0, 1).  Returns (dens_off, dens_xy) in the fa_params layout;
+    pairs left out stay weight-1 pairs of the value matrix."""
+    lists = {}
+    for i in range(ntypes):
+        for j in range(i, ntypes):
+            if rng.uniform() < frac:
+                n = int(rng.integers(2, max_nodes + 1))
+                x = np.sort(rng.uniform(0.0, 1.2, n))
+                y = rng.normal(size=n)
+                lists[(i, j)] = np.stack([x, y], axis=1).astype(np.float32)
+    off = np.zeros(ntypes * ntypes + 1, np.int32)
+    xy = []
+    for k in range(ntypes * ntypes):
+        i, j = divmod(k, ntypes)
+        nodes = lists.get((min(i, j), max(i, j)))
+        if nodes is not None:
+            xy.append(nodes)
+        off[k + 1] = off[k] + (0 if nodes is None else len(nodes))
+    dens_xy = np.concatenate(xy) if xy else np.zeros((0, 2), np.float32)
+    return off, dens_xy
 
 
 # ---------------------------------------------------------------------------

@@ -86,6 +86,14 @@ typedef struct fa_params {
     float        solventterm;    /* FA->solventterm (0 = use solvent column) */
     int32_t      ntypes;         /* FA->ntypes; type ntypes is the solvent column */
     const float* emat;           /* [ntypes*ntypes] y value per type pair, weight = 1 */
+    int32_t      normalize_area; /* FA->normalize_area: weight-1 terms use area / (4 pi rado^2) */
+    /* Optional density-function pairs (energy_matrix.weight = 0, read_emat,
+     * LIB/read_emat.cpp:103-138; get_yval, LIB/vcfunction.cpp:576-621):
+     * pair k = (t1-1)*ntypes + (t2-1) has nodes [dens_off[k], dens_off[k+1])
+     * of dens_xy, (x, y) in file order; an empty range makes pair k a
+     * weight-1 pair with value emat[k].  Null dens_off: every pair weight 1. */
+    const int32_t* dens_off;     /* [ntypes*ntypes + 1], dens_off[0] = 0 */
+    const float*   dens_xy;      /* [2 * dens_off[ntypes*ntypes]] */
 } fa_params;
 
 /* Per-pose result: the cfstr fields the path produces (LIB/flexaid.h:122-135)

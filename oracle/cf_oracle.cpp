@@ -940,6 +940,29 @@ template <class R> void calc_areas(Work<R>& W, float rado, int NC, int NV, int a
 }
 
 // ----------------------------------------------------------- one pose
+// One type pair's scored value for `area` of an atom whose whole sphere is
+// `surf`: weight 1 -> y * area (or y * area / surf with normalize_area,
+// LIB/vcfunction.cpp:343-353, :479-491); density -> get_yval(area / surf),
+// LIB/vcfunction.cpp:576-621, walking the nodes as the linked list is walked.
+double pair_term(const fa_params* prm, int k, double area, double surf)
+{
+    if (prm->dens_off && prm->dens_off[k + 1] > prm->dens_off[k]) {
+        const int e = prm->dens_off[k + 1];
+        const float* xy = prm->dens_xy;
+        const double rel = area / surf;
+        int i = prm->dens_off[k];
+        while (i + 1 < e && rel > (double)xy[2 * (i + 1)]) ++i;
+        const float x0 = xy[2 * i], y0 = xy[2 * i + 1];
+        if ((double)x0 > rel) return 0.0;            // no left bound data
+        if (i + 1 == e) return (double)y0;           // no right bound data
+        const float x1 = xy[2 * (i + 1)], y1 = xy[2 * (i + 1) + 1];
+        const float dx = x1 - x0, dy = y1 - y0;      // float fields, float differences
+        return (double)y0 + (rel - (double)x0) / (double)dx * (double)dy;
+    }
+    const double y = (double)prm->emat[k];
+    return prm->normalize_area ? y * area / surf : y * area;
+}
+
 template <class R>
 fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xyz_out,
                        std::vector<fa_contact_rec>* recs_out)
@@ -1031,12 +1054,13 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
         const double radA = (double)W.rad[i];
         const double radoA = radA + RW;
         double SAS = 4.0 * PI_REF * radoA * radoA;
+        const double surfA = SAS;
         OPS(vc_contact, W.records[i - Rn].size());
         for (const fa_contact_rec& r : W.records[i - Rn]) {
             const int c = r.atom;
             const double rAB = radA + (double)W.rad[c];
             const double area = r.area;
-            const double yval = (double)prm->emat[(W.type[i] - 1) * nt + (W.type[c] - 1)];
+            const int pk = (W.type[i] - 1) * nt + (W.type[c] - 1);
             SAS -= area;
             const bool intra = (c >= Rn);
             if (intra && lig->bonded[(c - Rn) * L + (i - Rn)]) continue;
@@ -1048,14 +1072,14 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
                 res.wal += (double)KWALL_F * (1.0 / (d6 * d6) - 1.0 / (cr6 * cr6));
                 if (r.dist <= dee * rAB) res.rclash = 1;
             }
-            if (prm->intramolecular || !intra) res.com += yval * area;
+            if (prm->intramolecular || !intra) res.com += pair_term(prm, pk, area, surfA);
         }
         if (SAS < 0.0) SAS = 0.0;
         res.totsas += SAS;
         if (prm->solventterm != 0.0f)
             res.sas += (double)prm->solventterm * SAS;
         else
-            res.sas += (double)prm->emat[(W.type[i] - 1) * nt + (nt - 1)] * SAS;
+            res.sas += pair_term(prm, (W.type[i] - 1) * nt + (nt - 1), SAS, surfA);
     }
     res.cf = res.com + res.wal + res.sas + res.con;
     res.app = res.com + res.wal + res.sas;
@@ -1227,6 +1251,12 @@ int fa_oracle_solve_2xS(const double* a0, const double* a1, float rado, double* 
     return solve_2xS<double>(a0, a1, rado, p0, p1);
 }
 double fa_oracle_cosPQR(const double* P, const double* Q, const double* Rr) { return cosPQR<double>(P, Q, Rr); }
+
+/* the scored value of type pair k (get_yval + contribution) */
+double fa_oracle_pair_term(const fa_params* prm, int k, double area, double surf)
+{
+    return pair_term(prm, k, area, surf);
+}
 double fa_oracle_spherical_arc(const double* A, double Ad, const double* B, double Bd, const double* C, double Cd,
                                float rado)
 {

@@ -169,16 +169,31 @@ void setup(RefSystem& S, const fa_receptor_desc* rec, const fa_ligand_desc* lig,
     // scoring parameters
     const int nt = prm->ntypes;
     FA->ntypes = nt;
+    // weight-1 pairs: one value (read_emat, LIB/read_emat.cpp:86-101);
+    // density pairs: their (x, y) nodes as a linked list (:103-138)
+    const int n_dens = prm->dens_off ? prm->dens_off[nt * nt] : 0;
     S.emat.assign((size_t)nt * nt, energy_matrix{});
-    S.evals.assign((size_t)nt * nt, energy_values{});
+    S.evals.assign((size_t)nt * nt + n_dens, energy_values{});
     for (int i = 0; i < nt * nt; ++i) {
-        S.evals[i].x = 0.0f;
-        S.evals[i].y = prm->emat[i];
-        S.evals[i].next_value = nullptr;
         S.emat[i].type1 = i / nt + 1;
         S.emat[i].type2 = i % nt + 1;
-        S.emat[i].weight = 1;
-        S.emat[i].energy_values = &S.evals[i];
+        const int b = prm->dens_off ? prm->dens_off[i] : 0, e = prm->dens_off ? prm->dens_off[i + 1] : 0;
+        if (e > b) {
+            for (int q = b; q < e; ++q) {
+                energy_values& v = S.evals[(size_t)nt * nt + q];
+                v.x = prm->dens_xy[2 * q];
+                v.y = prm->dens_xy[2 * q + 1];
+                v.next_value = (q + 1 < e) ? &S.evals[(size_t)nt * nt + q + 1] : nullptr;
+            }
+            S.emat[i].weight = 0;
+            S.emat[i].energy_values = &S.evals[(size_t)nt * nt + b];
+        } else {
+            S.evals[i].x = -1.0f;
+            S.evals[i].y = prm->emat[i];
+            S.evals[i].next_value = nullptr;
+            S.emat[i].weight = 1;
+            S.emat[i].energy_values = &S.evals[i];
+        }
     }
     FA->energy_matrix = S.emat.data();
     S.contributions.assign((size_t)nt * nt, 0.0f);
@@ -187,7 +202,7 @@ void setup(RefSystem& S, const fa_receptor_desc* rec, const fa_ligand_desc* lig,
     FA->dee_clash = prm->dee_clash;
     FA->intramolecular = prm->intramolecular;
     FA->solventterm = prm->solventterm;
-    FA->normalize_area = 0;
+    FA->normalize_area = prm->normalize_area ? 1 : 0;
     FA->useacs = 0;
     FA->vindex = 0;
     FA->bloops = 2;

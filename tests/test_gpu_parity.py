@@ -93,6 +93,30 @@ def test_config0_against_oracle(ctx0, sys0, oracle_cr, oracle_libm):
     assert bad.sum() <= max(2, okl.sum() // 200)
 
 
+@pytest.mark.parametrize("normalize", [0, 1])
+def test_density_matrix_against_oracle(hip, sys0, oracle, normalize):
+    # density-function pairs (weight 0) for half the type pairs, with and
+    # without normalize_area: same tolerance as the weight-1 matrix
+    import dataclasses
+    off, xy = synth.make_density_emat(np.random.default_rng(17), sys0.params.ntypes, frac=0.5)
+    prm = dataclasses.replace(sys0.params, dens_off=off, dens_xy=xy, normalize_area=normalize)
+    s = dataclasses.replace(sys0, params=prm)
+    genes = sys0.genes[0][:400]
+    with hip.Context() as c:
+        c.upload_system(s)
+        g = c.eval(0, genes)
+    desc = abi.Descriptors(s.receptor, s.ligands, prm)
+    with ol.trig_mode(oracle, 1):
+        ref = ol.oracle_eval(oracle, desc, 0, genes)
+    assert np.array_equal(g["status"], ref["status"])
+    ok = ref["status"] == 0
+    assert ok.sum() > 200
+    d = np.abs(g["cf"] - ref["cf"])
+    assert np.all(d[ok] <= tol(ref["cf"][ok]))
+    for f in ("com", "sas"):
+        assert np.abs(g[f] - ref[f])[ok].max() < 1e-8, f
+
+
 def test_pose_build_bit_exact(ctx0, sys0, oracle, desc0):
     g = sys0.genes[0][:200]
     _, xyz, _ = ctx0.eval_debug(0, g)

@@ -63,3 +63,24 @@ def test_fast_math_reference_sensitivity(oracle, sys0):
     off = np.abs(out["cf"] - f_out["cf"])[ok] > _tol(out["cf"][ok])
     print(f"fast-math reference outside the CF tolerance on {off.sum()} of {ok.sum()} scored poses")
     assert off.sum() <= 0.2 * ok.sum()
+
+
+@pytest.mark.parametrize("normalize", [0, 1])
+def test_strict_reference_bit_exact_density_matrix(oracle, sys0, normalize):
+    # half of the type pairs as density functions (energy_matrix.weight = 0),
+    # the rest weight 1, with and without normalize_area
+    import dataclasses
+    off, xy = synth.make_density_emat(np.random.default_rng(17), sys0.params.ntypes, frac=0.5)
+    prm = dataclasses.replace(sys0.params, dens_off=off, dens_xy=xy, normalize_area=normalize)
+    desc = abi.Descriptors(sys0.receptor, sys0.ligands, prm)
+    genes = sys0.genes[0][:300]
+    r_out, _, _ = ol.ref_eval(ref, desc, 0, genes)
+    with ol.trig_mode(oracle, 0):
+        out = ol.oracle_eval(oracle, desc, 0, genes)
+    assert np.array_equal(out["status"], r_out["status"])
+    for f in ("com", "sas", "cf"):
+        assert np.array_equal(out[f], r_out[f]), f
+    # the density terms changed the scores (the test is not vacuous)
+    plain = ol.oracle_eval(oracle, abi.Descriptors(sys0.receptor, sys0.ligands, sys0.params), 0, genes)
+    ok = out["status"] == 0
+    assert np.mean(out["com"][ok] != plain["com"][ok]) > 0.9
