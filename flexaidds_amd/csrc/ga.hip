@@ -12,7 +12,10 @@
 //   rank        parents + offspring ranked by energy (ties by index), the
 //               best `population` gathered in order
 //   copy back   survivors into the population rows
-//   fitness     rank fitness and its prefix sums for the next roulette
+//   fitness     LINEAR, PSHARE or SMFREE fitness of the survivors
+//               (LIB/gaboom.cpp:1627-1747) and its running sums for the
+//               next roulette; PSHARE then orders the survivors by fitness
+//               as GA() does (LIB/gaboom.cpp:498-500)
 //
 // A GA holds J independent jobs (ligands) that advance together: rows
 // [0, J*pop) are the parents of job 0, 1, ... and rows [J*pop, J*(pop+new))
@@ -51,7 +54,13 @@ struct GaDev {
     int32_t* gi2;          // [jobs * pop * Gmax] survivors being gathered
     double* gic2;
     fa_cf_result* res2;
-    double* cum;           // [pop] inclusive prefix sums of the (LINEAR) fitness
+    int model;             // FA_FIT_LINEAR / FA_FIT_PSHARE / FA_FIT_SMFREE
+    double alpha, ew;      // sharing exponent, SMFREE entropy weight
+    const double* sig;     // [jobs] sharing radius of each job
+    double* fit;           // [jobs * pop] fitness of the parent rows
+    double* fit2;          // [jobs * pop] fitness being gathered
+    double* share;         // [jobs * pop] niche counts
+    double* cum;           // [jobs * pop] running sums of the fitness (roulette)
     LseState* ens;         // [jobs] ensemble of every evaluated pose
     unsigned long long* n_eval;
     unsigned long long* gen;
@@ -205,8 +214,9 @@ __global__ void ga_breed_kernel(GaDev g)
     const double* lmin = g.lmin + (size_t)k * Gm;
     const double* ldel = g.ldel + (size_t)k * Gm;
     const double* lbin = g.lbin + (size_t)k * Gm;
-    const int p1 = roulette(g.cum, g.pop, fa_rng_u01(g.seed, gen, st, FA_DRAW_P1));
-    const int p2 = roulette(g.cum, g.pop, fa_rng_u01(g.seed, gen, st, FA_DRAW_P2));
+    const double* cum = g.cum + (size_t)k * g.pop;
+    const int p1 = roulette(cum, g.pop, fa_rng_u01(g.seed, gen, st, FA_DRAW_P1));
+    const int p2 = roulette(cum, g.pop, fa_rng_u01(g.seed, gen, st, FA_DRAW_P2));
     int32_t john[FA_MAX_GENES], mary[FA_MAX_GENES];
     for (int j = 0; j < G; ++j) {
         john[j] = g.gi[parent_row(g, k, p1) * Gm + j];
@@ -310,11 +320,12 @@ __global__ void ga_copy_back_kernel(GaDev g)
     if (t < rows) g.res[t] = g.res2[t];
 }
 
-// LINEAR fitness pop - rank and its inclusive prefix sums (one block; the
-// values are integers, so the sums are exact in any order)
-__global__ __launch_bounds__(256) void ga_fitness_kernel(GaDev g)
+// LINEAR fitness pop - rank and its inclusive prefix sums, one block per
+// job (the values are integers, so the sums are exact in any order)
+__global__ __launch_bounds__(256) void ga_linear_kernel(GaDev g)
 {
     __shared__ double part[256];
+    const int k = blockIdx.x;
     double carry = 0.0;
     for (int base = 0; base < g.pop; base += 256) {
         const int r = base + threadIdx.x;
@@ -327,9 +338,173 @@ __global__ __launch_bounds__(256) void ga_fitness_kernel(GaDev g)
             part[threadIdx.x] += v;
             __syncthreads();
         }
-        if (r < g.pop) g.cum[r] = carry + part[threadIdx.x];
+        if (r < g.pop) {
+            g.fit[(size_t)k * g.pop + r] = f;
+            g.cum[(size_t)k * g.pop + r] = carry + part[threadIdx.x];
+        }
         carry += part[255];
         __syncthreads();
+    }
+}
+
+// Niche counts (LIB/gaboom.cpp:1649-1660): for survivor i of job k,
+// sum over survivors j of 1 - (rmsp / sig)^alpha where rmsp <= sig, with
+// rmsp = sqrt(sum of squared internal-coordinate differences / genes)
+// (calc_rmsp, LIB/gaboom.cpp:2874-2885).  Survivors j stream through LDS
+// in tiles; each thread keeps its own row's sums in the reference's order.
+constexpr int SHARE_TILE = 32;
+
+__global__ __launch_bounds__(256) void ga_share_kernel(GaDev g)
+{
+    __shared__ double tile[SHARE_TILE * FA_MAX_GENES];
+    const int k = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int G = g.G[k], Gm = g.Gmax;
+    const double sig = g.sig[k];
+    const double* own = g.gic + parent_row(g, k, min(i, g.pop - 1)) * Gm;
+    double pshare = 0.0;
+    for (int base = 0; base < g.pop; base += SHARE_TILE) {
+        const int rows = min(SHARE_TILE, g.pop - base);
+        __syncthreads();
+        for (int q = threadIdx.x; q < rows * G; q += blockDim.x) {
+            const int r = q / G, j = q % G;
+            tile[r * FA_MAX_GENES + j] = g.gic[parent_row(g, k, base + r) * Gm + j];
+        }
+        __syncthreads();
+        if (i < g.pop)
+            for (int r = 0; r < rows; ++r) {
+                double ss = 0.0;
+                for (int j = 0; j < G; ++j) {
+                    const double d = own[j] - tile[r * FA_MAX_GENES + j];
+                    ss += d * d;
+                }
+                const double rmsp = sqrt(ss / (double)G);
+                if (rmsp <= sig) pshare += 1.0 - pow(rmsp / sig, g.alpha);
+            }
+    }
+    if (i < g.pop) g.share[(size_t)k * g.pop + i] = pshare;
+}
+
+__device__ double block_sum(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ double block_max(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// PSHARE / SMFREE fitness of job k's survivors (rows by increasing energy),
+// one block per job (LIB/gaboom.cpp:1637-1747).  SMFREE's Boltzmann weights
+// follow StatMechEngine::boltzmann_weights over the survivors' energies
+// (LIB/statmech.cpp:178-201 with compute_boltzmann_batch's exp(-beta (E -
+// Emin)), LIB/UnifiedHardwareDispatch.cpp:608-624), normalised by their
+// maximum.
+__global__ __launch_bounds__(256) void ga_shared_fitness_kernel(GaDev g)
+{
+    __shared__ double sh[256];
+    const int k = blockIdx.x;
+    const int n = g.pop;
+    double Z = 1.0, maxw = 1.0, emin = 0.0;
+    const bool smfree = g.model == FA_FIT_SMFREE;
+    if (smfree) {
+        double m = INFINITY;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmin(m, g.res[parent_row(g, k, i)].cf);
+        emin = -block_max(-m, sh);
+        double z = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) z += exp(-g.beta * (g.res[parent_row(g, k, i)].cf - emin));
+        Z = block_sum(z, sh);
+        double mw = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            mw = fmax(mw, exp(-g.beta * (g.res[parent_row(g, k, i)].cf - emin)) / Z);
+        maxw = block_max(mw, sh);
+        if (maxw <= 0.0) maxw = 1.0;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double ps = g.share[(size_t)k * n + i];
+        double f;
+        if (smfree) {
+            const double w = (Z > 0.0) ? exp(-g.beta * (g.res[parent_row(g, k, i)].cf - emin)) / Z : 0.0;
+            const double rank = (double)(n - i) / (double)n;
+            const double blended = (1.0 - g.ew) * rank + g.ew * (w / maxw);
+            f = blended * (double)n / ps;
+        } else {
+            f = (double)(n - i) / ps;
+        }
+        g.fit[(size_t)k * n + i] = f;
+    }
+}
+
+// position of each of job k's survivors when ordered by decreasing fitness
+// (QuickSort(chrom, 0, num_chrom - 1, false) in GA(), LIB/gaboom.cpp:498-500;
+// ties keep row order), gathered into the second buffers
+__global__ __launch_bounds__(256) void ga_fitness_order_kernel(GaDev g)
+{
+    __shared__ uint64_t keys[1024];
+    const int k = blockIdx.y;
+    const int n = g.pop;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t ki = (i < n) ? order_key(-g.fit[(size_t)k * n + i]) : 0ull;
+    int rank = 0;
+    for (int base = 0; base < n; base += 1024) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < 1024 && base + q < n; q += blockDim.x)
+            keys[q] = order_key(-g.fit[(size_t)k * n + base + q]);
+        __syncthreads();
+        const int lim = min(1024, n - base);
+        if (i < n)
+            for (int q = 0; q < lim; ++q) {
+                const uint64_t kj = keys[q];
+                rank += (kj < ki) || (kj == ki && base + q < i);
+            }
+    }
+    if (i < n) {
+        const size_t src = parent_row(g, k, i), dst = parent_row(g, k, rank);
+        for (int j = 0; j < g.Gmax; ++j) {
+            g.gi2[dst * g.Gmax + j] = g.gi[src * g.Gmax + j];
+            g.gic2[dst * g.Gmax + j] = g.gic[src * g.Gmax + j];
+        }
+        g.res2[dst] = g.res[src];
+        g.fit2[(size_t)k * n + rank] = g.fit[(size_t)k * n + i];
+    }
+}
+
+__global__ void ga_fit_back_kernel(GaDev g)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (long long)g.jobs * g.pop) g.fit[t] = g.fit2[t];
+}
+
+// running sums of a job's fitness in row order, one thread per job: the
+// sequential order of roullete_wheel's loop (LIB/gaboom.cpp:1158-1182), so
+// a draw picks the row the reference's scan would
+__global__ void ga_cum_kernel(GaDev g)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.jobs) return;
+    const double* f = g.fit + (size_t)k * g.pop;
+    double* c = g.cum + (size_t)k * g.pop;
+    double tot = 0.0;
+    for (int i = 0; i < g.pop; ++i) {
+        tot += f[i];
+        c[i] = tot;
     }
 }
 
@@ -347,7 +522,7 @@ struct fa_ga {
     fa_ga_params p{};
     int jobs = 0, Gmax = 0;
     fa::GaDev d{};
-    DevBuf gi, gic, res, gi2, gic2, res2, cum, lim, ngenes, ens, cnt, lig, thermo, grand;
+    DevBuf gi, gic, res, gi2, gic2, res2, cum, lim, ngenes, ens, cnt, lig, thermo, grand, fit, fit2, share, sig;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -365,7 +540,20 @@ int ga_select(fa_ga* ga, int n)
     hipLaunchKernelGGL(fa::ga_rank_kernel, dim3(ga_blocks(n), d.jobs), dim3(256), 0, ga->stream, d, n);
     hipLaunchKernelGGL(fa::ga_copy_back_kernel, dim3(ga_blocks((long long)d.jobs * d.pop * d.Gmax)), dim3(256), 0,
                        ga->stream, d);
-    hipLaunchKernelGGL(fa::ga_fitness_kernel, dim3(1), dim3(256), 0, ga->stream, d);
+    if (d.model == FA_FIT_LINEAR) {
+        hipLaunchKernelGGL(fa::ga_linear_kernel, dim3(d.jobs), dim3(256), 0, ga->stream, d);
+    } else {
+        hipLaunchKernelGGL(fa::ga_share_kernel, dim3(ga_blocks(d.pop), d.jobs), dim3(256), 0, ga->stream, d);
+        hipLaunchKernelGGL(fa::ga_shared_fitness_kernel, dim3(d.jobs), dim3(256), 0, ga->stream, d);
+        if (d.model == FA_FIT_PSHARE) {
+            hipLaunchKernelGGL(fa::ga_fitness_order_kernel, dim3(ga_blocks(d.pop), d.jobs), dim3(256), 0, ga->stream, d);
+            hipLaunchKernelGGL(fa::ga_copy_back_kernel, dim3(ga_blocks((long long)d.jobs * d.pop * d.Gmax)), dim3(256),
+                               0, ga->stream, d);
+            hipLaunchKernelGGL(fa::ga_fit_back_kernel, dim3(ga_blocks((long long)d.jobs * d.pop)), dim3(256), 0,
+                               ga->stream, d);
+        }
+        hipLaunchKernelGGL(fa::ga_cum_kernel, dim3(ga_blocks(d.jobs)), dim3(256), 0, ga->stream, d);
+    }
     FA_HIP(hipGetLastError());
     return FA_SUCCESS;
 }
@@ -395,6 +583,14 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     if (n_jobs < 1) return fa_fail(FA_ERR_INVALID, "at least one job");
     if (p->population < 2 || p->offspring < 1) return fa_fail(FA_ERR_INVALID, "population >= 2 and offspring >= 1");
     if (p->temperature <= 0.0) return fa_fail(FA_ERR_INVALID, "temperature must be positive");
+    if (p->fitness < FA_FIT_LINEAR || p->fitness > FA_FIT_SMFREE) return fa_fail(FA_ERR_INVALID, "unknown fitness model");
+    if (p->fitness != FA_FIT_LINEAR) {
+        if (!(p->share_alpha > 0.0)) return fa_fail(FA_ERR_INVALID, "sharing alpha must be positive");
+        if (!(p->sig_share > 0.0) && !(p->share_peaks > 0.0 && p->share_scale > 0.0))
+            return fa_fail(FA_ERR_INVALID, "sharing needs sig_share > 0 or peaks and scale > 0");
+        if (!(p->entropy_weight >= 0.0 && p->entropy_weight <= 1.0))
+            return fa_fail(FA_ERR_INVALID, "entropy weight must lie in [0, 1]");
+    }
     const long long pop = p->population, n_new = p->offspring;
     if ((long long)n_jobs * (pop + n_new) > (1LL << 30) || pop + n_new > (1LL << 30))
         return fa_fail(FA_ERR_LIMIT, "jobs * (population + offspring) above 2^30");
@@ -406,10 +602,21 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
         Gmax = std::max(Gmax, G[k]);
     }
     if (Gmax < 1 || Gmax > FA_MAX_GENES) return fa_fail(FA_ERR_LIMIT, "between 1 and FA_MAX_GENES genes per ligand");
-    // set_bins, LIB/gaboom.cpp:2291-2306, per job; padded to Gmax
-    std::vector<double> host(3 * (size_t)n_jobs * Gmax, 0.0);
+    // set_bins, LIB/gaboom.cpp:2291-2306, per job; padded to Gmax.  Niche
+    // radius per job: sqrt(mean (max - min)^2) / (2 peaks^(1/genes)) / scale
+    // (LIB/gaboom.cpp:300-309) unless given.
+    std::vector<double> host(3 * (size_t)n_jobs * Gmax, 0.0), sig(n_jobs, 1.0);
     const size_t plane = (size_t)n_jobs * Gmax;
-    for (int k = 0, off = 0; k < n_jobs; off += G[k], ++k)
+    for (int k = 0, off = 0; k < n_jobs; off += G[k], ++k) {
+        if (p->sig_share > 0.0) {
+            sig[k] = p->sig_share;
+        } else if (p->fitness != FA_FIT_LINEAR) {
+            double ss = 0.0;
+            for (int j = 0; j < G[k]; ++j) ss += (lim[off + j].max - lim[off + j].min) * (lim[off + j].max - lim[off + j].min);
+            sig[k] = std::sqrt(ss / (double)G[k]) / (2.0 * std::pow(p->share_peaks, 1.0 / (double)G[k]));
+            sig[k] /= p->share_scale;
+            if (!(sig[k] > 0.0)) return fa_fail(FA_ERR_INVALID, "gene limits give a zero niche radius");
+        }
         for (int j = 0; j < G[k]; ++j) {
             const fa_gene_lim& L = lim[off + j];
             if (!(L.del > 0.0) || !(L.max >= L.min)) return fa_fail(FA_ERR_INVALID, "gene limits need del > 0 and max >= min");
@@ -421,6 +628,7 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
             host[plane + (size_t)k * Gmax + j] = L.del;
             host[2 * plane + (size_t)k * Gmax + j] = 1.0 / nbin;
         }
+    }
     FA_HIP(hipSetDevice(c->device));
     fa_ga* ga = new fa_ga();
     ga->ctx = c;
@@ -438,7 +646,9 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
         (rc = fa_ensure(ga->res, sizeof(fa_cf_result) * J * n_all)) ||
         (rc = fa_ensure(ga->gi2, sizeof(int32_t) * J * pop * Gmax)) ||
         (rc = fa_ensure(ga->gic2, sizeof(double) * J * pop * Gmax)) ||
-        (rc = fa_ensure(ga->res2, sizeof(fa_cf_result) * J * pop)) || (rc = fa_ensure(ga->cum, sizeof(double) * pop)) ||
+        (rc = fa_ensure(ga->res2, sizeof(fa_cf_result) * J * pop)) || (rc = fa_ensure(ga->cum, sizeof(double) * J * pop)) ||
+        (rc = fa_ensure(ga->fit, sizeof(double) * J * pop)) || (rc = fa_ensure(ga->fit2, sizeof(double) * J * pop)) ||
+        (rc = fa_ensure(ga->share, sizeof(double) * J * pop)) || (rc = fa_ensure(ga->sig, sizeof(double) * J)) ||
         (rc = fa_ensure(ga->lim, sizeof(double) * host.size())) || (rc = fa_ensure(ga->ngenes, sizeof(int) * J)) ||
         (rc = fa_ensure(ga->ens, sizeof(fa::LseState) * J)) || (rc = fa_ensure(ga->cnt, 2 * sizeof(unsigned long long))) ||
         (rc = fa_ensure(ga->lig, sizeof(int) * J * n_all)) || (rc = fa_ensure(ga->thermo, sizeof(fa_thermo) * J)) ||
@@ -453,6 +663,7 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     hipError_t e = hipMemcpy(ga->lim.p, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ga->lig.p, lig.data(), sizeof(int) * lig.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ga->ngenes.p, G.data(), sizeof(int) * J, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ga->sig.p, sig.data(), sizeof(double) * J, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(ga->ens.p, 0, sizeof(fa::LseState) * J);
     if (e == hipSuccess) e = hipMemset(ga->cnt.p, 0, 2 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ga->stream, hipStreamNonBlocking);
@@ -480,6 +691,13 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     d.gic2 = static_cast<double*>(ga->gic2.p);
     d.res2 = static_cast<fa_cf_result*>(ga->res2.p);
     d.cum = static_cast<double*>(ga->cum.p);
+    d.model = p->fitness;
+    d.alpha = p->share_alpha;
+    d.ew = p->entropy_weight;
+    d.sig = static_cast<const double*>(ga->sig.p);
+    d.fit = static_cast<double*>(ga->fit.p);
+    d.fit2 = static_cast<double*>(ga->fit2.p);
+    d.share = static_cast<double*>(ga->share.p);
     d.ens = static_cast<fa::LseState*>(ga->ens.p);
     d.n_eval = static_cast<unsigned long long*>(ga->cnt.p);
     d.gen = d.n_eval + 1;
@@ -623,6 +841,17 @@ int fa_ga_generation(fa_ga* ga, uint64_t* generation)
     return FA_SUCCESS;
 }
 
+int fa_ga_fitness(fa_ga* ga, double* fitness, double* sig_share)
+{
+    if (!ga) return fa_fail(FA_ERR_INVALID, "null GA");
+    FA_HIP(hipSetDevice(ga->ctx->device));
+    FA_HIP(hipStreamSynchronize(ga->stream));
+    const size_t rows = (size_t)ga->jobs * ga->d.pop;
+    if (fitness) FA_HIP(hipMemcpy(fitness, ga->fit.p, sizeof(double) * rows, hipMemcpyDeviceToHost));
+    if (sig_share) FA_HIP(hipMemcpy(sig_share, ga->sig.p, sizeof(double) * ga->jobs, hipMemcpyDeviceToHost));
+    return FA_SUCCESS;
+}
+
 int fa_ga_destroy(fa_ga* ga)
 {
     if (!ga) return FA_SUCCESS;
@@ -631,7 +860,8 @@ int fa_ga_destroy(fa_ga* ga)
     if (ga->exec) (void)hipGraphExecDestroy(ga->exec);
     if (ga->graph) (void)hipGraphDestroy(ga->graph);
     for (DevBuf* b : {&ga->gi, &ga->gic, &ga->res, &ga->gi2, &ga->gic2, &ga->res2, &ga->cum, &ga->lim, &ga->ngenes,
-                      &ga->ens, &ga->cnt, &ga->lig, &ga->thermo, &ga->grand})
+                      &ga->ens, &ga->cnt, &ga->lig, &ga->thermo, &ga->grand, &ga->fit, &ga->fit2, &ga->share,
+                      &ga->sig})
         if (b->p) (void)hipFree(b->p);
     if (ga->stream) (void)hipStreamDestroy(ga->stream);
     delete ga;

@@ -72,6 +72,7 @@ _SIGS = {
     "fa_ga_create": ([_P, _P, _P, _P], C.c_int),
     "fa_ga_create_batch": ([_P, _P, C.c_int, _P, _P, _P], C.c_int),
     "fa_ga_shape": ([_P, _P, _P], C.c_int),
+    "fa_ga_fitness": ([_P, _P, _P], C.c_int),
     "fa_ga_grand": ([_P, _P, _P, _P, _P], C.c_int),
     "fa_grand_device": ([_P, _P, C.c_int, _P, C.c_int, _P, _P], C.c_int),
     "fa_ga_run": ([_P, C.c_int, C.c_int, _P], C.c_int),
@@ -234,7 +235,12 @@ class GeneLim(C.Structure):
 class GAParams(C.Structure):
     _fields_ = [("ligand", C.c_int32), ("population", C.c_int32), ("offspring", C.c_int32),
                 ("intragenes", C.c_int32), ("cross_rate", C.c_double), ("mut_rate", C.c_double),
-                ("temperature", C.c_double), ("seed", C.c_uint64)]
+                ("temperature", C.c_double), ("seed", C.c_uint64), ("fitness", C.c_int32), ("pad", C.c_int32),
+                ("share_alpha", C.c_double), ("share_peaks", C.c_double), ("share_scale", C.c_double),
+                ("sig_share", C.c_double), ("entropy_weight", C.c_double)]
+
+
+FITNESS = {"LINEAR": 0, "PSHARE": 1, "SMFREE": 2}
 
 
 class GA:
@@ -245,14 +251,17 @@ class GA:
     per ligand, `limits` then a sequence of per-job limit sequences."""
 
     def __init__(self, ctx: Context, limits, ligand=0, population=2000, offspring=None, cross_rate=0.8,
-                 mut_rate=0.03, temperature=300.0, seed=1, intragenes=False, ligands=None):
+                 mut_rate=0.03, temperature=300.0, seed=1, intragenes=False, ligands=None, fitness="LINEAR",
+                 share_alpha=1.0, share_peaks=5.0, share_scale=10.0, sig_share=0.0, entropy_weight=0.5):
         self.ctx = ctx
         per_job = [limits] if ligands is None else list(limits)
         flat = [x for job in per_job for x in job]
         self._lim = (GeneLim * len(flat))(*[GeneLim(float(a), float(b), float(d), int(m), 0)
                                              for a, b, d, m in flat])
+        model = FITNESS[fitness] if isinstance(fitness, str) else int(fitness)
         self.params = GAParams(ligand, population, population if offspring is None else offspring,
-                               1 if intragenes else 0, cross_rate, mut_rate, temperature, seed)
+                               1 if intragenes else 0, cross_rate, mut_rate, temperature, seed, model, 0,
+                               share_alpha, share_peaks, share_scale, sig_share, entropy_weight)
         self.job_genes = [len(j) for j in per_job]
         self._ga = C.c_void_p()
         if ligands is None:
@@ -281,6 +290,13 @@ class GA:
         res = np.zeros(rows, abi.CF_DTYPE)
         _check(_lib.fa_ga_population(self._ga, g.ctypes.data, ic.ctypes.data, res.ctypes.data))
         return g, ic, res
+
+    def fitness(self):
+        """(fitness of the parent rows [jobs * population], niche radius per job)."""
+        f = np.zeros(self.jobs * self.params.population)
+        sig = np.zeros(self.jobs)
+        _check(_lib.fa_ga_fitness(self._ga, f.ctypes.data, sig.ctypes.data))
+        return f, sig
 
     def job_rows(self, k: int):
         """(survivor rows, child rows) of job k in population()'s arrays."""

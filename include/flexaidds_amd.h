@@ -159,6 +159,13 @@ typedef struct fa_gene_lim {   /* LIB/gaboom.h:46-54 (genlim) */
     int32_t pad;
 } fa_gene_lim;
 
+/* Fitness models (GB->fitness_model, LIB/gaboom.cpp:1627-1747). */
+enum fa_fitness_model {
+    FA_FIT_LINEAR = 0,         /* num_chrom - rank */
+    FA_FIT_PSHARE = 1,         /* (num_chrom - rank) / niche count; survivors then ordered by fitness */
+    FA_FIT_SMFREE = 2          /* rank blended with Boltzmann weight at `temperature`, / niche count */
+};
+
 typedef struct fa_ga_params {
     int32_t ligand;            /* ligand of the context to dock */
     int32_t population;        /* survivors per generation (num_chrom) */
@@ -166,8 +173,16 @@ typedef struct fa_ga_params {
     int32_t intragenes;        /* crossover points inside genes */
     double cross_rate;         /* e.g. 0.8 */
     double mut_rate;           /* per bit, e.g. 0.03 */
-    double temperature;        /* K, ensemble reduction */
+    double temperature;        /* K, ensemble reduction and SMFREE weights */
     uint64_t seed;
+    int32_t fitness;           /* fa_fitness_model */
+    int32_t pad;
+    double share_alpha;        /* sharing_alpha (reference default 1) */
+    double share_peaks;        /* sharing_peaks (5) */
+    double share_scale;        /* sharing_scale (10) */
+    double sig_share;          /* > 0: niche radius for every job; else derived per job from its
+                                  limits as LIB/gaboom.cpp:300-309 does */
+    double entropy_weight;     /* SMFREE blend w (0.5) */
 } fa_ga_params;
 
 typedef struct fa_ga fa_ga;
@@ -206,6 +221,10 @@ int fa_ga_thermo(fa_ga* ga, fa_thermo* out, uint64_t* n_poses);
  * LIB/GrandPartitionFunction.h:85-91).  Outputs may be null. */
 int fa_ga_grand(fa_ga* ga, const double* conc_M, double* log_xi, double* p_bound, double* p_empty);
 int fa_ga_generation(fa_ga* ga, uint64_t* generation);
+/* Fitness of the parent rows ([n_jobs * population], rows as in
+ * fa_ga_population) as the next generation's roulette uses it, and each
+ * job's niche radius (sig_share, [n_jobs]); either may be null. */
+int fa_ga_fitness(fa_ga* ga, double* fitness, double* sig_share);
 int fa_ga_destroy(fa_ga* ga);
 
 /* Size of the kernel's per-wave LDS workspace (bytes). */

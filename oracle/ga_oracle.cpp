@@ -12,9 +12,15 @@
  *   crossover         LIB/gaboom.cpp:2517-2618
  *   mutate            LIB/gaboom.cpp:2620-2641
  *   reproduce (BOOM)  LIB/gaboom.cpp:886-973, without duplicate rejection
- *   LINEAR fitness    LIB/gaboom.cpp:1625-1633
+ *   LINEAR fitness    LIB/gaboom.cpp:1627-1635
+ *   PSHARE fitness    LIB/gaboom.cpp:1637-1664 (calc_rmsp, :2874-2885)
+ *   SMFREE fitness    LIB/gaboom.cpp:1666-1747 (boltzmann_weights,
+ *                     LIB/statmech.cpp:178-201; boltzmann_scalar,
+ *                     LIB/UnifiedHardwareDispatch.cpp:608-624)
+ *   sig_share         LIB/gaboom.cpp:300-309
  */
 #include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <numeric>
@@ -150,14 +156,13 @@ void fa_oracle_ga_init(uint64_t seed, uint64_t job, int pop, int G, const fa_gen
         }
 }
 
-/* one generation's children of job `job` from its parents ranked best first
- * (LINEAR fitness); pair t draws from stream (job << 32) | t */
-void fa_oracle_ga_breed(uint64_t seed, uint64_t job, uint64_t gen, int pop, int n_new, int G, int intragenes, double cross_rate,
-                        double mut_rate, const fa_gene_lim* lim, const int32_t* parents, int32_t* child_genes,
-                        double* child_ic)
+/* one generation's children of job `job` from its parents with roulette
+ * fitness fit[pop]; pair t draws from stream (job << 32) | t */
+void fa_oracle_ga_breed_fit(uint64_t seed, uint64_t job, uint64_t gen, int pop, int n_new, int G, int intragenes,
+                            double cross_rate, double mut_rate, const fa_gene_lim* lim, const double* fit_in,
+                            const int32_t* parents, int32_t* child_genes, double* child_ic)
 {
-    std::vector<double> fit(pop);
-    for (int i = 0; i < pop; ++i) fit[i] = (double)(pop - i);
+    std::vector<double> fit(fit_in, fit_in + pop);
     std::vector<int32_t> john(G), mary(G);
     for (int t = 0; 2 * t < n_new; ++t) {
         const uint64_t st = (job << 32) | (uint64_t)t;
@@ -178,6 +183,74 @@ void fa_oracle_ga_breed(uint64_t seed, uint64_t job, uint64_t gen, int pop, int 
                 child_ic[(std::size_t)(2 * t + k) * G + j] = genetoic(lim[j], 1.0 / nbins(lim[j]), src[j]);
             }
         }
+    }
+}
+
+/* as fa_oracle_ga_breed_fit with LINEAR fitness (parents ranked best first) */
+void fa_oracle_ga_breed(uint64_t seed, uint64_t job, uint64_t gen, int pop, int n_new, int G, int intragenes,
+                        double cross_rate, double mut_rate, const fa_gene_lim* lim, const int32_t* parents,
+                        int32_t* child_genes, double* child_ic)
+{
+    std::vector<double> fit(pop);
+    for (int i = 0; i < pop; ++i) fit[i] = (double)(pop - i);
+    fa_oracle_ga_breed_fit(seed, job, gen, pop, n_new, G, intragenes, cross_rate, mut_rate, lim, fit.data(), parents,
+                           child_genes, child_ic);
+}
+
+/* niche radius from the gene limits */
+double fa_oracle_sig_share(const fa_gene_lim* lim, int G, double peaks, double scale)
+{
+    double sig = 0.0;
+    for (int i = 0; i < G; i++) sig += (lim[i].max - lim[i].min) * (lim[i].max - lim[i].min);
+    sig = std::sqrt(sig / (double)G) / (2.0 * std::pow(peaks, (1.0 / (double)G)));
+    return sig / scale;
+}
+
+/* fitness of `pop` survivors ordered by increasing energy: model 0 LINEAR,
+ * 1 PSHARE, 2 SMFREE (temperature T in K) */
+void fa_oracle_ga_fitness(int model, int pop, int G, const double* ic, const double* energy, double T, double sig,
+                          double alpha, double ew, double* fit)
+{
+    auto rmsp = [&](int a, int b) {
+        double ss = 0.0;
+        for (int j = 0; j < G; ++j) {
+            const double d = ic[(std::size_t)a * G + j] - ic[(std::size_t)b * G + j];
+            ss += d * d;
+        }
+        return std::sqrt(ss / (double)G);
+    };
+    auto niche = [&](int i) {
+        double pshare = 0.0;
+        for (int j = 0; j < pop; j++) {
+            const double r = rmsp(i, j);
+            if (r <= sig) pshare += (1.0 - std::pow((r / sig), alpha));
+        }
+        return pshare;
+    };
+    if (model == 0) {
+        for (int i = 0; i < pop; i++) fit[i] = (double)(pop - i);
+        return;
+    }
+    if (model == 1) {
+        for (int i = 0; i < pop; i++) fit[i] = (double)(pop - i) / niche(i);
+        return;
+    }
+    const double kB = 0.001987206;  /* kcal/(mol K), LIB/statmech.h:26 */
+    const double beta = 1.0 / (kB * T);
+    const double emin = *std::min_element(energy, energy + pop);
+    std::vector<double> w(pop);
+    for (int i = 0; i < pop; ++i) w[i] = std::exp(-beta * (energy[i] - emin));
+    double Z = 0.0;
+    for (int i = 0; i < pop; ++i) Z += w[i];
+    for (int i = 0; i < pop; ++i) w[i] = (Z > 0.0) ? w[i] / Z : 0.0;
+    double max_bw = 0.0;
+    for (int i = 0; i < pop; ++i)
+        if (w[i] > max_bw) max_bw = w[i];
+    if (max_bw <= 0.0) max_bw = 1.0;
+    for (int i = 0; i < pop; ++i) {
+        const double rank = (double)(pop - i) / (double)pop;
+        const double blended = (1.0 - ew) * rank + ew * (w[i] / max_bw);
+        fit[i] = blended * (double)pop / niche(i);
     }
 }
 

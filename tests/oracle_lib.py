@@ -153,7 +153,38 @@ def ga_bind(lib):
     lib.fa_oracle_ga_breed.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                        C.c_double, _P, _P, _P, _P]
     lib.fa_oracle_ga_survivors.argtypes = [C.c_int, C.c_int, _P, _P]
+    lib.fa_oracle_ga_breed_fit.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           C.c_double, C.c_double, _P, _P, _P, _P, _P]
+    lib.fa_oracle_ga_fitness.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, _P]
+    lib.fa_oracle_sig_share.argtypes = [_P, C.c_int, C.c_double, C.c_double]
+    lib.fa_oracle_sig_share.restype = C.c_double
     return lib
+
+
+def ga_breed_fit(lib, seed, gen, parents, fit, n_new, limits, cross=0.8, mut=0.03, intragenes=0, job=0):
+    parents = np.ascontiguousarray(parents, np.int32)
+    fit = np.ascontiguousarray(fit, np.float64)
+    pop, G = parents.shape
+    child = np.zeros((n_new, G), np.int32)
+    ic = np.zeros((n_new, G))
+    lim = gene_lims(limits)
+    lib.fa_oracle_ga_breed_fit(seed, job, gen, pop, n_new, G, intragenes, cross, mut, lim, fit.ctypes.data,
+                               parents.ctypes.data, child.ctypes.data, ic.ctypes.data)
+    return child, ic
+
+
+def ga_fitness(lib, model, ic, energy, T=300.0, sig=1.0, alpha=1.0, ew=0.5):
+    ic = np.ascontiguousarray(ic, np.float64)
+    energy = np.ascontiguousarray(energy, np.float64)
+    pop, G = ic.shape
+    fit = np.zeros(pop)
+    lib.fa_oracle_ga_fitness(model, pop, G, ic.ctypes.data, energy.ctypes.data, T, sig, alpha, ew, fit.ctypes.data)
+    return fit
+
+
+def sig_share(lib, limits, peaks=5.0, scale=10.0):
+    return lib.fa_oracle_sig_share(gene_lims(limits), len(limits), peaks, scale)
 
 
 def ga_init(lib, seed, pop, limits, job=0):

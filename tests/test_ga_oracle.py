@@ -132,3 +132,57 @@ def test_jobs_draw_from_disjoint_streams(ga):
     # a gene's high bits agree between jobs no more often than chance
     same = np.mean((rows[0] >> 24) == (rows[1] >> 24))
     assert same < 0.02
+
+
+@pytest.fixture(scope="module")
+def gao(oracle):
+    return ol.ga_bind(oracle)
+
+
+def test_sig_share_closed_form(gao):
+    # every gene spans R: sqrt(mean R^2) = R
+    for G, R, peaks, scale in [(4, 360.0, 5.0, 10.0), (10, 2.0, 3.0, 1.0), (1, 7.0, 5.0, 10.0)]:
+        lims = [(-R / 2, R / 2, 1.0, 0)] * G
+        assert abs(ol.sig_share(gao, lims, peaks, scale) - R / (2.0 * peaks ** (1.0 / G)) / scale) < 1e-12
+
+
+def test_pshare_niche_counts(gao):
+    pop, G = 40, 5
+    rng = np.random.default_rng(2)
+    # all rows identical: every niche count is pop
+    same = np.tile(rng.normal(size=G), (pop, 1))
+    fit = ol.ga_fitness(gao, 1, same, np.arange(pop, dtype=float), sig=1.0)
+    assert np.allclose(fit, (pop - np.arange(pop)) / pop, rtol=0, atol=1e-15)
+    # rows far apart: niche counts 1, LINEAR fitness
+    far = np.arange(pop)[:, None] * 100.0 + np.zeros((1, G))
+    fit = ol.ga_fitness(gao, 1, far, np.arange(pop, dtype=float), sig=1.0)
+    assert np.array_equal(fit, (pop - np.arange(pop)).astype(float))
+    # two rows at rmsp r < sig share 1 - (r/sig)^alpha with each other
+    two = np.zeros((2, G))
+    two[1] = 0.3                                    # rmsp = 0.3
+    for alpha in (1.0, 2.0, 0.5):
+        fit = ol.ga_fitness(gao, 1, two, np.array([0.0, 1.0]), sig=1.2, alpha=alpha)
+        share = 1.0 + (1.0 - (0.3 / 1.2) ** alpha)
+        assert np.allclose(fit, np.array([2.0, 1.0]) / share, rtol=1e-14)
+
+
+def test_smfree_limits(gao):
+    pop, G = 30, 3
+    far = np.arange(pop)[:, None] * 100.0 + np.zeros((1, G))
+    E = np.sort(np.random.default_rng(8).normal(scale=0.5, size=pop))
+    # entropy weight 0: rank only (niche counts 1)
+    fit = ol.ga_fitness(gao, 2, far, E, T=300.0, sig=1.0, ew=0.0)
+    assert np.allclose(fit, (pop - np.arange(pop)).astype(float), rtol=1e-15)
+    # entropy weight 1: pop * Boltzmann factor relative to the best pose
+    fit = ol.ga_fitness(gao, 2, far, E, T=300.0, sig=1.0, ew=1.0)
+    beta = 1.0 / (0.001987206 * 300.0)
+    assert np.allclose(fit, pop * np.exp(-beta * (E - E.min())), rtol=1e-12)
+
+
+def test_breed_fit_linear_equals_breed(gao):
+    G, pop = 6, 50
+    lims = [(-180.0, 180.0, 2.0, 0)] * G
+    parents, _ = ol.ga_init(gao, 3, pop, lims)
+    a, ai = ol.ga_breed(gao, 3, 2, parents, 40, lims)
+    b, bi = ol.ga_breed_fit(gao, 3, 2, parents, (pop - np.arange(pop)).astype(float), 40, lims)
+    assert np.array_equal(a, b) and np.array_equal(ai, bi)

@@ -135,3 +135,41 @@ def ol_thermo(oracle, energies, T):
     out = abi.Thermo()
     oracle.fa_oracle_thermo(e.ctypes.data, None, len(e), T, C.byref(out))
     return {f: getattr(out, f) for f, _ in abi.Thermo._fields_}
+
+
+@pytest.mark.parametrize("model", ["PSHARE", "SMFREE"])
+def test_fitness_models(hip, ctx, limits, orc, model):
+    # niche radius from the limits, fitness of the survivors against the
+    # oracle's (to rounding: the device's exp/pow/sqrt differ by ulps), PSHARE
+    # survivors ordered by fitness, and children bred from the device fitness
+    # exactly as the oracle breeds them
+    code = {"PSHARE": 1, "SMFREE": 2}[model]
+    T = 300.0
+    ga = hip.GA(ctx, limits, population=POP, offspring=NEW, seed=SEED + 3, fitness=model, temperature=T,
+                entropy_weight=0.5)
+    f0, sig = ga.fitness()
+    assert abs(sig[0] - ol.sig_share(orc, limits)) <= 1e-12 * sig[0]
+    for gen in range(3):
+        g, ic, r = ga.population()
+        f, _ = ga.fitness()
+        E = r["cf"][:POP]
+        order = np.argsort(E, kind="stable")                     # survivors by increasing energy
+        ref = ol.ga_fitness(orc, code, ic[order], E[order], T=T, sig=sig[0], alpha=1.0, ew=0.5)
+        # rows of equal energy (duplicate children) may take their ranks in
+        # either order: compare those as a multiset, the others row by row
+        Es = E[order]
+        tied = np.zeros(POP, bool)
+        tied[1:] |= Es[1:] == Es[:-1]
+        tied[:-1] |= Es[1:] == Es[:-1]
+        assert (~tied).sum() > POP // 2
+        assert np.allclose(f[order][~tied], ref[~tied], rtol=1e-12, atol=0), gen
+        assert np.allclose(np.sort(f), np.sort(ref), rtol=1e-12, atol=0), gen
+        if model == "PSHARE":
+            assert np.all(np.diff(f) <= 0), gen                  # ordered by decreasing fitness
+        else:
+            assert np.all(np.diff(E) >= 0), gen                  # SMFREE keeps the energy order
+        ga.run(1, graph=(gen % 2 == 1))
+        g1, _, _ = ga.population()
+        child, cic = ol.ga_breed_fit(orc, SEED + 3, gen + 1, g[:POP], f, NEW, limits)
+        assert np.array_equal(g1[POP:], child), gen
+    ga.close()
