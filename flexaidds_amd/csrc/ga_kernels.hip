@@ -1,0 +1,560 @@
+// ga_kernels.hip — one genetic-algorithm generation on the device.
+//
+// The reference's generation (GA(), LIB/gaboom.cpp:346-500, with BOOM
+// reproduction, :886-1036, and the fitness models, :1627-1747) becomes a
+// fixed sequence of launches on one stream:
+//
+//   tick        generation counter(s) + 1 (the counter seeds the draws)
+//   breed       one thread per offspring pair: roulette parents, crossover,
+//               mutation, genes -> internal coordinates
+//   poses       the contact-function kernel over the offspring (caller)
+//   ensemble    fold the offspring energies into each job's FP64 ensemble
+//   rank        parents + offspring ranked by energy (ties by index), the
+//               best `population` gathered in order
+//   copy back   survivors into the population rows
+//   fitness     LINEAR, PSHARE or SMFREE fitness of the survivors and its
+//               running sums for the next roulette; PSHARE then orders the
+//               survivors by fitness as GA() does (LIB/gaboom.cpp:498-500)
+//
+// Random draws are counter-based (fa_rng.h): a generation is a pure function
+// of the population, the seed and the generation number, so a job's result
+// does not depend on which other jobs share its launches.
+#include "ga_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "fa_rng.h"
+
+namespace fa {
+
+__device__ __forceinline__ int slot_of(const GaDev& g, int a) { return g.act ? g.act[a] : a; }
+__device__ __forceinline__ size_t parent_row(const GaDev& g, int k, int r) { return (size_t)k * g.pop + r; }
+__device__ __forceinline__ size_t child_row(const GaDev& g, int a, int i)
+{
+    return (size_t)g.jobs * g.pop + (size_t)a * g.n_new + i;
+}
+__device__ __forceinline__ size_t stage_row(const GaDev& g, int a, int c) { return g.stage + (size_t)a * g.pop + c; }
+__device__ __forceinline__ uint64_t stream_of(const GaDev& g, int k, int t)
+{
+    return g.job_streams ? (((uint64_t)k << 32) | (uint32_t)t) : (uint64_t)(uint32_t)t;
+}
+__device__ __forceinline__ uint64_t seed_of(const GaDev& g, int k) { return g.seeds ? g.seeds[k] : g.seed; }
+__device__ __forceinline__ uint64_t gen_of(const GaDev& g, int k) { return g.gens ? g.gens[k] : *g.gen; }
+
+// LIB/gaboom.cpp:2887-2906 with RandomDouble(gene) = gene / (2^31)
+__host__ __device__ inline double genetoic(double mn, double del, double bin, int32_t gene)
+{
+    const double u = gene / ((double)FA_MAX_RANDOM_VALUE + 1.0);
+    int i = 0;
+    double tot = bin;
+    while (tot < u) {
+        tot += bin;
+        ++i;
+    }
+    return mn + del * (double)i;
+}
+
+// LIB/gaboom.cpp:1158-1182: first index whose running fitness exceeds r
+__device__ __forceinline__ int roulette(const double* cum, int n, double u)
+{
+    const double tot = cum[n - 1];
+    if (tot <= 0.0) return (int)(u * n) % n;
+    const double r = u * tot;
+    int lo = 0, hi = n - 1;  // smallest i with cum[i] > r, n - 1 if none
+    if (!(cum[hi] > r)) return n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cum[mid] > r)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ unsigned low_mask(int n)
+{
+    // the reference's mask loop: optr = 1; repeat n times optr |= optr << 1
+    unsigned optr = 1u;
+    for (int i = 0; i < n; ++i) optr |= (optr << 1);
+    return optr;
+}
+
+// LIB/gaboom.cpp:2517-2618
+__device__ void crossover(int32_t* john, int32_t* mary, int G, int intragenes, uint64_t seed, uint64_t gen,
+                          uint64_t stream)
+{
+    int gen_a = (int)(fa_rng_u01(seed, gen, stream, FA_DRAW_GEN_A) * (double)G);
+    int gen_b = (int)(fa_rng_u01(seed, gen, stream, FA_DRAW_GEN_B) * (double)G);
+    if (gen_a >= G) gen_a = G - 1;
+    if (gen_b >= G) gen_b = G - 1;
+    int pnt_a, pnt_b;
+    if (intragenes) {
+        pnt_a = (int)(fa_rng_u01(seed, gen, stream, FA_DRAW_PNT_A) * 32.0);
+        pnt_b = (int)(fa_rng_u01(seed, gen, stream, FA_DRAW_PNT_B) * 32.0);
+        if (gen_a > gen_b) {
+            const int ag = gen_a, ap = pnt_a;
+            gen_a = gen_b;
+            pnt_a = pnt_b;
+            gen_b = ag;
+            pnt_b = ap;
+        }
+        if (gen_a == gen_b && pnt_a < pnt_b) {
+            const int ap = pnt_b;
+            pnt_b = pnt_a;
+            pnt_a = ap;
+        }
+    } else {
+        if (gen_a > gen_b) {
+            const int ag = gen_a;
+            gen_a = gen_b;
+            gen_b = ag;
+        }
+        if (gen_a != gen_b) {
+            pnt_a = (fa_rng_u01(seed, gen, stream, FA_DRAW_PNT_A) < 0.5) ? 32 : 0;
+            pnt_b = (fa_rng_u01(seed, gen, stream, FA_DRAW_PNT_B) < 0.5) ? 32 : 0;
+            if ((gen_b - gen_a) == 1 && pnt_a == 0 && pnt_b == 32) pnt_b = pnt_a;
+        } else {
+            pnt_a = 32;
+            pnt_b = 0;
+        }
+    }
+    for (int j = gen_a; j <= gen_b; ++j) {
+        const unsigned optr = low_mask((j == gen_a) ? pnt_a : 32);
+        const unsigned uj = (unsigned)john[j], um = (unsigned)mary[j];
+        john[j] = (int32_t)((uj & ~optr) | (um & optr));
+        mary[j] = (int32_t)((uj & optr) | (um & ~optr));
+    }
+    if (pnt_b > 0) {
+        const unsigned optr = low_mask(pnt_b - 1);
+        const unsigned uj = (unsigned)john[gen_b], um = (unsigned)mary[gen_b];
+        john[gen_b] = (int32_t)((uj & ~optr) | (um & optr));
+        mary[gen_b] = (int32_t)((uj & optr) | (um & ~optr));
+    }
+}
+
+// LIB/gaboom.cpp:2620-2641
+__device__ void mutate(int32_t* genes, int G, double rate, uint64_t seed, uint64_t gen, uint64_t stream, int child)
+{
+    for (int j = 0; j < G; ++j) {
+        unsigned optr = 0u;
+        for (int i = 0; i < 32; ++i)
+            if (fa_rng_u01(seed, gen, stream, FA_DRAW_MUT + ((uint64_t)child * G + j) * 32 + i) < rate) optr |= 1u << i;
+        genes[j] ^= (int32_t)optr;
+    }
+}
+
+__global__ void ga_tick_kernel(GaDev g)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g.gens) {
+        if (a < g.n_act) g.gens[slot_of(g, a)] += 1ull;
+    } else if (a == 0) {
+        *g.gen += 1ull;
+    }
+}
+
+// initial population of each job into its staged rows, generation 0
+// (LIB/gaboom.cpp:1871-1902); genes past a job's own count are 0
+__global__ void ga_init_kernel(GaDev g)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.n_act * g.pop * g.Gmax) return;
+    const int a = (int)(t / ((long long)g.pop * g.Gmax));
+    const int c = (int)((t / g.Gmax) % g.pop), j = (int)(t % g.Gmax);
+    const int k = slot_of(g, a);
+    const size_t at = stage_row(g, a, c) * g.Gmax + j;
+    if (j >= g.G[k]) {
+        g.gi[at] = 0;
+        g.gic[at] = 0.0;
+        return;
+    }
+    const size_t l = (size_t)k * g.Gmax + j;
+    const int32_t gene = fa_rng_gene(seed_of(g, k), 0ull, stream_of(g, k, c), (uint64_t)j);
+    g.gi[at] = gene;
+    g.gic[at] = genetoic(g.lmin[l], g.ldel[l], g.lbin[l], gene);
+}
+
+// one thread per offspring pair (LIB/gaboom.cpp:920-973)
+__global__ void ga_breed_kernel(GaDev g)
+{
+    const int pairs = (g.n_new + 1) / 2;
+    const long long tt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tt >= (long long)g.n_act * pairs) return;
+    const int a = (int)(tt / pairs), t = (int)(tt % pairs);
+    const int k = slot_of(g, a);
+    const uint64_t gen = gen_of(g, k), seed = seed_of(g, k);
+    const uint64_t st = stream_of(g, k, t);
+    const int G = g.G[k], Gm = g.Gmax;
+    const double* lmin = g.lmin + (size_t)k * Gm;
+    const double* ldel = g.ldel + (size_t)k * Gm;
+    const double* lbin = g.lbin + (size_t)k * Gm;
+    const double* cum = g.cum + (size_t)k * g.pop;
+    const int p1 = roulette(cum, g.pop, fa_rng_u01(seed, gen, st, FA_DRAW_P1));
+    const int p2 = roulette(cum, g.pop, fa_rng_u01(seed, gen, st, FA_DRAW_P2));
+    int32_t john[FA_MAX_GENES], mary[FA_MAX_GENES];
+    for (int j = 0; j < G; ++j) {
+        john[j] = g.gi[parent_row(g, k, p1) * Gm + j];
+        mary[j] = g.gi[parent_row(g, k, p2) * Gm + j];
+    }
+    if (fa_rng_u01(seed, gen, st, FA_DRAW_CROSS) < g.cross) crossover(john, mary, G, g.intragenes, seed, gen, st);
+    mutate(john, G, g.mut, seed, gen, st, 0);
+    mutate(mary, G, g.mut, seed, gen, st, 1);
+    // genes past the job's own count stay 0 (rows are Gmax wide)
+    const size_t c1 = child_row(g, a, 2 * t);
+    for (int j = 0; j < Gm; ++j) {
+        g.gi[c1 * Gm + j] = (j < G) ? john[j] : 0;
+        g.gic[c1 * Gm + j] = (j < G) ? genetoic(lmin[j], ldel[j], lbin[j], john[j]) : 0.0;
+    }
+    if (2 * t + 1 < g.n_new) {
+        for (int j = 0; j < Gm; ++j) {
+            g.gi[(c1 + 1) * Gm + j] = (j < G) ? mary[j] : 0;
+            g.gic[(c1 + 1) * Gm + j] = (j < G) ? genetoic(lmin[j], ldel[j], lbin[j], mary[j]) : 0.0;
+        }
+    }
+}
+
+// energies of each job's children (children = 1) or staged initial
+// population (children = 0) into its ensemble state; one block per job
+__global__ __launch_bounds__(256) void ga_ensemble_kernel(GaDev g, int children)
+{
+    __shared__ LseState sh[256];
+    const int a = blockIdx.x;
+    const int k = slot_of(g, a);
+    const int n = children ? g.n_new : g.pop;
+    LseState st{0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double e = g.res[children ? child_row(g, a, i) : stage_row(g, a, i)].cf;
+        lse_push(st, -g.beta * e, e);
+    }
+    sh[threadIdx.x] = st;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) lse_merge(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        LseState acc = g.ens[k];
+        lse_merge(acc, sh[0]);
+        g.ens[k] = acc;
+        atomicAdd(g.n_eval, (unsigned long long)n);
+    }
+}
+
+// total order on energies (NaN last)
+__device__ __forceinline__ uint64_t order_key(double e)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(e);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// candidate r of the job at position a: its parents then its children, or
+// its staged initial population
+__device__ __forceinline__ size_t cand_row(const GaDev& g, int a, int k, int r, int initial)
+{
+    if (initial) return stage_row(g, a, r);
+    return r < g.pop ? parent_row(g, k, r) : child_row(g, a, r - g.pop);
+}
+
+// rank of each of a job's n candidates by energy, ties by candidate index;
+// the best `pop` gathered in order (QuickSort by evalue,
+// LIB/gaboom.cpp:2674-2726).  Job position = blockIdx.y.
+__global__ __launch_bounds__(256) void ga_rank_kernel(GaDev g, int initial)
+{
+    __shared__ uint64_t keys[1024];
+    const int a = blockIdx.y;
+    const int k = slot_of(g, a);
+    const int n = initial ? g.pop : g.n_all;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t ki = (i < n) ? order_key(g.res[cand_row(g, a, k, i, initial)].cf) : 0ull;
+    int rank = 0;
+    for (int base = 0; base < n; base += 1024) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < 1024 && base + q < n; q += blockDim.x)
+            keys[q] = order_key(g.res[cand_row(g, a, k, base + q, initial)].cf);
+        __syncthreads();
+        const int lim = min(1024, n - base);
+        if (i < n)
+            for (int q = 0; q < lim; ++q) {
+                const uint64_t kj = keys[q];
+                rank += (kj < ki) || (kj == ki && base + q < i);
+            }
+    }
+    if (i < n && rank < g.pop) {
+        const size_t src = cand_row(g, a, k, i, initial), dst = parent_row(g, k, rank);
+        for (int j = 0; j < g.Gmax; ++j) {
+            g.gi2[dst * g.Gmax + j] = g.gi[src * g.Gmax + j];
+            g.gic2[dst * g.Gmax + j] = g.gic[src * g.Gmax + j];
+        }
+        g.res2[dst] = g.res[src];
+    }
+}
+
+// gathered survivors (and, with fit_too, their fitness) into the parent rows
+__global__ void ga_copy_back_kernel(GaDev g, int fit_too)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = (long long)g.pop * g.Gmax;
+    if (t < (long long)g.n_act * per) {
+        const int a = (int)(t / per);
+        const size_t at = (size_t)slot_of(g, a) * per + (size_t)(t % per);
+        g.gi[at] = g.gi2[at];
+        g.gic[at] = g.gic2[at];
+    }
+    if (t < (long long)g.n_act * g.pop) {
+        const int a = (int)(t / g.pop);
+        const size_t at = (size_t)slot_of(g, a) * g.pop + (size_t)(t % g.pop);
+        g.res[at] = g.res2[at];
+        if (fit_too) g.fit[at] = g.fit2[at];
+    }
+}
+
+// LINEAR fitness pop - rank and its inclusive prefix sums, one block per
+// job (the values are integers, so the sums are exact in any order)
+__global__ __launch_bounds__(256) void ga_linear_kernel(GaDev g)
+{
+    __shared__ double part[256];
+    const int k = slot_of(g, blockIdx.x);
+    double carry = 0.0;
+    for (int base = 0; base < g.pop; base += 256) {
+        const int r = base + threadIdx.x;
+        const double f = (r < g.pop) ? (double)(g.pop - r) : 0.0;
+        part[threadIdx.x] = f;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const double v = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0.0;
+            __syncthreads();
+            part[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (r < g.pop) {
+            g.fit[(size_t)k * g.pop + r] = f;
+            g.cum[(size_t)k * g.pop + r] = carry + part[threadIdx.x];
+        }
+        carry += part[255];
+        __syncthreads();
+    }
+}
+
+// Niche counts (LIB/gaboom.cpp:1649-1660): for survivor i of a job, the sum
+// over its survivors j of 1 - (rmsp / sig)^alpha where rmsp <= sig, with
+// rmsp = sqrt(sum of squared internal-coordinate differences / genes)
+// (calc_rmsp, LIB/gaboom.cpp:2874-2885).  Survivors j stream through LDS in
+// tiles; each thread keeps its own row's sums in the reference's order.
+constexpr int SHARE_TILE = 32;
+
+__global__ __launch_bounds__(256) void ga_share_kernel(GaDev g)
+{
+    __shared__ double tile[SHARE_TILE * FA_MAX_GENES];
+    const int k = slot_of(g, blockIdx.y);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int G = g.G[k], Gm = g.Gmax;
+    const double sig = g.sig[k];
+    const double* own = g.gic + parent_row(g, k, min(i, g.pop - 1)) * Gm;
+    double pshare = 0.0;
+    for (int base = 0; base < g.pop; base += SHARE_TILE) {
+        const int rows = min(SHARE_TILE, g.pop - base);
+        __syncthreads();
+        for (int q = threadIdx.x; q < rows * G; q += blockDim.x) {
+            const int r = q / G, j = q % G;
+            tile[r * FA_MAX_GENES + j] = g.gic[parent_row(g, k, base + r) * Gm + j];
+        }
+        __syncthreads();
+        if (i < g.pop)
+            for (int r = 0; r < rows; ++r) {
+                double ss = 0.0;
+                for (int j = 0; j < G; ++j) {
+                    const double d = own[j] - tile[r * FA_MAX_GENES + j];
+                    ss += d * d;
+                }
+                const double rmsp = sqrt(ss / (double)G);
+                if (rmsp <= sig) pshare += 1.0 - pow(rmsp / sig, g.alpha);
+            }
+    }
+    if (i < g.pop) g.share[(size_t)k * g.pop + i] = pshare;
+}
+
+__device__ double block_sum(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ double block_max(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// PSHARE / SMFREE fitness of a job's survivors (rows by increasing energy),
+// one block per job (LIB/gaboom.cpp:1637-1747).  SMFREE's Boltzmann weights
+// follow StatMechEngine::boltzmann_weights over the survivors' energies
+// (LIB/statmech.cpp:178-201 with compute_boltzmann_batch's exp(-beta (E -
+// Emin)), LIB/UnifiedHardwareDispatch.cpp:608-624), normalised by their
+// maximum.
+__global__ __launch_bounds__(256) void ga_shared_fitness_kernel(GaDev g)
+{
+    __shared__ double sh[256];
+    const int k = slot_of(g, blockIdx.x);
+    const int n = g.pop;
+    double Z = 1.0, maxw = 1.0, emin = 0.0;
+    const bool smfree = g.model == FA_FIT_SMFREE;
+    if (smfree) {
+        double m = INFINITY;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmin(m, g.res[parent_row(g, k, i)].cf);
+        emin = -block_max(-m, sh);
+        double z = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) z += exp(-g.beta * (g.res[parent_row(g, k, i)].cf - emin));
+        Z = block_sum(z, sh);
+        double mw = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            mw = fmax(mw, exp(-g.beta * (g.res[parent_row(g, k, i)].cf - emin)) / Z);
+        maxw = block_max(mw, sh);
+        if (maxw <= 0.0) maxw = 1.0;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double ps = g.share[(size_t)k * n + i];
+        double f;
+        if (smfree) {
+            const double w = (Z > 0.0) ? exp(-g.beta * (g.res[parent_row(g, k, i)].cf - emin)) / Z : 0.0;
+            const double rank = (double)(n - i) / (double)n;
+            const double blended = (1.0 - g.ew) * rank + g.ew * (w / maxw);
+            f = blended * (double)n / ps;
+        } else {
+            f = (double)(n - i) / ps;
+        }
+        g.fit[(size_t)k * n + i] = f;
+    }
+}
+
+// position of each of a job's survivors when ordered by decreasing fitness
+// (QuickSort(chrom, 0, num_chrom - 1, false) in GA(), LIB/gaboom.cpp:498-500;
+// ties keep row order), gathered into the second buffers
+__global__ __launch_bounds__(256) void ga_fitness_order_kernel(GaDev g)
+{
+    __shared__ uint64_t keys[1024];
+    const int k = slot_of(g, blockIdx.y);
+    const int n = g.pop;
+    const double* f = g.fit + (size_t)k * n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t ki = (i < n) ? order_key(-f[i]) : 0ull;
+    int rank = 0;
+    for (int base = 0; base < n; base += 1024) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < 1024 && base + q < n; q += blockDim.x) keys[q] = order_key(-f[base + q]);
+        __syncthreads();
+        const int lim = min(1024, n - base);
+        if (i < n)
+            for (int q = 0; q < lim; ++q) {
+                const uint64_t kj = keys[q];
+                rank += (kj < ki) || (kj == ki && base + q < i);
+            }
+    }
+    if (i < n) {
+        const size_t src = parent_row(g, k, i), dst = parent_row(g, k, rank);
+        for (int j = 0; j < g.Gmax; ++j) {
+            g.gi2[dst * g.Gmax + j] = g.gi[src * g.Gmax + j];
+            g.gic2[dst * g.Gmax + j] = g.gic[src * g.Gmax + j];
+        }
+        g.res2[dst] = g.res[src];
+        g.fit2[(size_t)k * n + rank] = f[i];
+    }
+}
+
+// running sums of a job's fitness in row order, one thread per job: the
+// sequential order of roullete_wheel's loop (LIB/gaboom.cpp:1158-1182), so
+// a draw picks the row the reference's scan would
+__global__ void ga_cum_kernel(GaDev g)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= g.n_act) return;
+    const int k = slot_of(g, a);
+    const double* f = g.fit + (size_t)k * g.pop;
+    double* c = g.cum + (size_t)k * g.pop;
+    double tot = 0.0;
+    for (int i = 0; i < g.pop; ++i) {
+        tot += f[i];
+        c[i] = tot;
+    }
+}
+
+// ensemble thermodynamics of slots [0, n)
+__global__ void ga_thermo_kernel(GaDev g, int n, double T, fa_thermo* out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = thermo_of(g.ens[k], T);
+}
+
+namespace {
+unsigned blocks_of(long long n) { return (unsigned)std::max(1LL, (n + 255) / 256); }
+}  // namespace
+
+void ga_launch_tick(const GaDev& g, hipStream_t s)
+{
+    if (g.n_act < 1) return;
+    hipLaunchKernelGGL(ga_tick_kernel, dim3(blocks_of(g.n_act)), dim3(256), 0, s, g);
+}
+
+void ga_launch_init(const GaDev& g, hipStream_t s)
+{
+    if (g.n_act < 1) return;
+    hipLaunchKernelGGL(ga_init_kernel, dim3(blocks_of((long long)g.n_act * g.pop * g.Gmax)), dim3(256), 0, s, g);
+}
+
+void ga_launch_breed(const GaDev& g, hipStream_t s)
+{
+    if (g.n_act < 1) return;
+    hipLaunchKernelGGL(ga_breed_kernel, dim3(blocks_of((long long)g.n_act * ((g.n_new + 1) / 2))), dim3(256), 0, s,
+                       g);
+}
+
+void ga_launch_ensemble(const GaDev& g, int children, hipStream_t s)
+{
+    if (g.n_act < 1) return;
+    hipLaunchKernelGGL(ga_ensemble_kernel, dim3(g.n_act), dim3(256), 0, s, g, children);
+}
+
+void ga_launch_select(const GaDev& g, int initial, hipStream_t s)
+{
+    if (g.n_act < 1) return;
+    const int n = initial ? g.pop : g.n_all;
+    const unsigned rows = blocks_of((long long)g.n_act * g.pop * g.Gmax);
+    hipLaunchKernelGGL(ga_rank_kernel, dim3(blocks_of(n), g.n_act), dim3(256), 0, s, g, initial);
+    hipLaunchKernelGGL(ga_copy_back_kernel, dim3(rows), dim3(256), 0, s, g, 0);
+    if (g.model == FA_FIT_LINEAR) {
+        hipLaunchKernelGGL(ga_linear_kernel, dim3(g.n_act), dim3(256), 0, s, g);
+        return;
+    }
+    hipLaunchKernelGGL(ga_share_kernel, dim3(blocks_of(g.pop), g.n_act), dim3(256), 0, s, g);
+    hipLaunchKernelGGL(ga_shared_fitness_kernel, dim3(g.n_act), dim3(256), 0, s, g);
+    if (g.model == FA_FIT_PSHARE) {
+        hipLaunchKernelGGL(ga_fitness_order_kernel, dim3(blocks_of(g.pop), g.n_act), dim3(256), 0, s, g);
+        hipLaunchKernelGGL(ga_copy_back_kernel, dim3(rows), dim3(256), 0, s, g, 1);
+    }
+    hipLaunchKernelGGL(ga_cum_kernel, dim3(blocks_of(g.n_act)), dim3(256), 0, s, g);
+}
+
+void ga_launch_thermo(const GaDev& g, int n_slots, double T, fa_thermo* out, hipStream_t s)
+{
+    if (n_slots < 1) return;
+    hipLaunchKernelGGL(ga_thermo_kernel, dim3(blocks_of(n_slots)), dim3(256), 0, s, g, n_slots, T, out);
+}
+
+}  // namespace fa

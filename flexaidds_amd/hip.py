@@ -76,6 +76,13 @@ _SIGS = {
     "fa_ga_grand": ([_P, _P, _P, _P, _P], C.c_int),
     "fa_grand_device": ([_P, _P, C.c_int, _P, C.c_int, _P, _P], C.c_int),
     "fa_ga_run": ([_P, C.c_int, C.c_int, _P], C.c_int),
+    "fa_pool_create": ([_P, _P, _P], C.c_int),
+    "fa_pool_shape": ([_P, _P, _P, _P], C.c_int),
+    "fa_pool_submit": ([_P, _P, _P], C.c_int),
+    "fa_pool_step": ([_P, _P, _P], C.c_int),
+    "fa_pool_collect": ([_P, _P, C.c_int, _P], C.c_int),
+    "fa_pool_population": ([_P, C.c_int64, _P, _P, _P], C.c_int),
+    "fa_pool_destroy": ([_P], C.c_int),
     "fa_ga_population": ([_P, _P, _P, _P], C.c_int),
     "fa_ga_thermo": ([_P, _P, _P], C.c_int),
     "fa_ga_generation": ([_P, _P], C.c_int),
@@ -336,6 +343,98 @@ class GA:
         if self._ga:
             _lib.fa_ga_destroy(self._ga)
             self._ga = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+MAX_GENES = 64   # FA_MAX_GENES
+
+
+class PoolParams(C.Structure):
+    _fields_ = [("population", C.c_int32), ("offspring", C.c_int32), ("intragenes", C.c_int32),
+                ("fitness", C.c_int32), ("cross_rate", C.c_double), ("mut_rate", C.c_double),
+                ("temperature", C.c_double), ("share_alpha", C.c_double), ("share_peaks", C.c_double),
+                ("share_scale", C.c_double), ("sig_share", C.c_double), ("entropy_weight", C.c_double),
+                ("max_slots", C.c_int32), ("max_genes", C.c_int32), ("hbm_budget", C.c_uint64)]
+
+
+class Job(C.Structure):
+    _fields_ = [("ligand", C.c_int32), ("generations", C.c_int32), ("seed", C.c_uint64),
+                ("limits", C.POINTER(GeneLim))]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("id", C.c_int64), ("ligand", C.c_int32), ("generations", C.c_int32), ("n_genes", C.c_int32),
+                ("pad", C.c_int32), ("best", abi.CFResult), ("best_genes", C.c_int32 * MAX_GENES),
+                ("best_ic", C.c_double * MAX_GENES), ("thermo", abi.Thermo), ("n_poses", C.c_uint64)]
+
+
+class Pool:
+    """Continuous-batching docking pool on one context (fa_pool_* of the C ABI).
+
+    Jobs are submitted as they arrive; every step() admits queued jobs into
+    free slots and advances all resident jobs by one generation in one pose
+    launch; collect() returns the jobs that step finished."""
+
+    def __init__(self, ctx: Context, population=1024, offspring=None, max_slots=64, hbm_budget=8 << 30,
+                 cross_rate=0.8, mut_rate=0.03, temperature=300.0, intragenes=False, fitness="LINEAR",
+                 share_alpha=1.0, share_peaks=5.0, share_scale=10.0, sig_share=0.0, entropy_weight=0.5,
+                 max_genes=0):
+        self.ctx = ctx
+        model = FITNESS[fitness] if isinstance(fitness, str) else int(fitness)
+        self.params = PoolParams(population, population if offspring is None else offspring,
+                                 1 if intragenes else 0, model, cross_rate, mut_rate, temperature, share_alpha,
+                                 share_peaks, share_scale, sig_share, entropy_weight, max_slots, max_genes,
+                                 int(hbm_budget))
+        self._pool = C.c_void_p()
+        _check(_lib.fa_pool_create(ctx._ctx, C.byref(self.params), C.byref(self._pool)))
+        s, b, g = C.c_int(), C.c_uint64(), C.c_int()
+        _check(_lib.fa_pool_shape(self._pool, C.byref(s), C.byref(b), C.byref(g)))
+        self.slots, self.slot_bytes, self.genes = int(s.value), int(b.value), int(g.value)
+        self._out = (JobResult * self.slots)()
+
+    def submit(self, ligand: int, limits, generations: int, seed: int) -> int:
+        lim = (GeneLim * len(limits))(*[GeneLim(float(a), float(b), float(d), int(m), 0) for a, b, d, m in limits])
+        job = Job(ligand, generations, seed, lim)
+        jid = C.c_int64()
+        _check(_lib.fa_pool_submit(self._pool, C.byref(job), C.byref(jid)))
+        return int(jid.value)
+
+    def step(self):
+        """(jobs still resident after this step, jobs queued)."""
+        r, q = C.c_int(), C.c_int()
+        _check(_lib.fa_pool_step(self._pool, C.byref(r), C.byref(q)))
+        return int(r.value), int(q.value)
+
+    def collect(self):
+        n = C.c_int()
+        _check(_lib.fa_pool_collect(self._pool, self._out, self.slots, C.byref(n)))
+        out = []
+        for r in self._out[:n.value]:
+            out.append({"id": int(r.id), "ligand": int(r.ligand), "generations": int(r.generations),
+                        "best": {f: getattr(r.best, f) for f, _ in abi.CFResult._fields_},
+                        "best_genes": np.array(r.best_genes[:r.n_genes], np.int32),
+                        "best_ic": np.array(r.best_ic[:r.n_genes]),
+                        "thermo": {f: getattr(r.thermo, f) for f, _ in abi.Thermo._fields_},
+                        "n_poses": int(r.n_poses)})
+        return out
+
+    def population(self, job_id: int):
+        pop = self.params.population
+        g = np.zeros((pop, self.genes), np.int32)
+        ic = np.zeros((pop, self.genes))
+        res = np.zeros(pop, abi.CF_DTYPE)
+        _check(_lib.fa_pool_population(self._pool, job_id, g.ctypes.data, ic.ctypes.data, res.ctypes.data))
+        return g, ic, res
+
+    def close(self):
+        if self._pool:
+            _lib.fa_pool_destroy(self._pool)
+            self._pool = C.c_void_p()
 
     def __del__(self):
         try:

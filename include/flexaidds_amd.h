@@ -227,6 +227,69 @@ int fa_ga_generation(fa_ga* ga, uint64_t* generation);
 int fa_ga_fitness(fa_ga* ga, double* fitness, double* sig_share);
 int fa_ga_destroy(fa_ga* ga);
 
+/* ---------------------------------------------------------------------
+ * Docking pool: continuous batching of docking jobs on one context.
+ * Replaces running GA() once per request (LIB/gaboom.cpp:346-500) with a
+ * resident pool that advances every in-flight job by one generation per
+ * step and scores all of their children, plus the initial populations of
+ * jobs admitted at that step, in one pose launch.  Jobs are admitted in
+ * arrival order while slots are free; the slot count is fixed at creation
+ * from the HBM budget.  A job's result equals fa_ga's for the same seed and
+ * parameters, whatever else shares the pool.  One pool per context and
+ * thread; pools of different contexts may be stepped back to back so their
+ * work overlaps on the device.
+ * ------------------------------------------------------------------- */
+typedef struct fa_pool_params {
+    int32_t population;        /* survivors per generation, every job */
+    int32_t offspring;         /* children per generation */
+    int32_t intragenes;
+    int32_t fitness;           /* fa_fitness_model */
+    double cross_rate;
+    double mut_rate;
+    double temperature;        /* K */
+    double share_alpha, share_peaks, share_scale, sig_share, entropy_weight;  /* as fa_ga_params */
+    int32_t max_slots;         /* upper bound on resident jobs */
+    int32_t max_genes;         /* row width; 0 = the largest gene count of the context's ligands */
+    uint64_t hbm_budget;       /* bytes of job state; slots = min(max_slots, budget / slot bytes) */
+} fa_pool_params;
+
+typedef struct fa_job {
+    int32_t ligand;            /* ligand of the context */
+    int32_t generations;       /* generations after the initial population */
+    uint64_t seed;
+    const fa_gene_lim* limits; /* [genes of the ligand]; copied at submission */
+} fa_job;
+
+typedef struct fa_job_result {
+    int64_t id;
+    int32_t ligand;
+    int32_t generations;
+    int32_t n_genes;
+    int32_t pad;
+    fa_cf_result best;                   /* lowest-energy survivor */
+    int32_t best_genes[FA_MAX_GENES];
+    double best_ic[FA_MAX_GENES];
+    fa_thermo thermo;                    /* ensemble of every pose the job evaluated */
+    uint64_t n_poses;
+} fa_job_result;
+
+typedef struct fa_pool fa_pool;
+
+int fa_pool_create(fa_ctx* ctx, const fa_pool_params* params, fa_pool** out);
+/* Resident job slots, device bytes per slot, row width. */
+int fa_pool_shape(fa_pool* pool, int* slots, uint64_t* slot_bytes, int* max_genes);
+/* Queues a job (host side; no device work). */
+int fa_pool_submit(fa_pool* pool, const fa_job* job, int64_t* id);
+/* Waits for the previous step, admits queued jobs into free slots and
+ * enqueues one generation of every resident job; returns without waiting.
+ * *n_running: jobs still resident after this step; *n_queued: waiting. */
+int fa_pool_step(fa_pool* pool, int* n_running, int* n_queued);
+/* Waits for the last step and returns the jobs it finished (up to max). */
+int fa_pool_collect(fa_pool* pool, fa_job_result* out, int max, int* n);
+/* Survivors of a resident job (population rows of max_genes values). */
+int fa_pool_population(fa_pool* pool, int64_t job_id, int32_t* genes, double* ic, fa_cf_result* results);
+int fa_pool_destroy(fa_pool* pool);
+
 /* Size of the kernel's per-wave LDS workspace (bytes). */
 int fa_kernel_lds_bytes(void);
 

@@ -47,3 +47,30 @@ def test_struct_layouts_match_header():
     assert C.sizeof(abi.CFResult) == 64
     assert C.sizeof(abi.ContactRec) == 24
     assert C.sizeof(abi.Thermo) == 64
+
+
+def test_struct_sizes_against_the_c_compiler(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler gives
+    the header's struct."""
+    import subprocess
+    from flexaidds_amd import abi, hip
+    mirrors = {"fa_params": abi.Params, "fa_ligand_desc": abi.LigandDesc, "fa_receptor_desc": abi.ReceptorDesc,
+               "fa_cf_result": abi.CFResult, "fa_thermo": abi.Thermo, "fa_gene_lim": hip.GeneLim,
+               "fa_ga_params": hip.GAParams, "fa_pool_params": hip.PoolParams, "fa_job": hip.Job,
+               "fa_job_result": hip.JobResult}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "flexaidds_amd.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            cf = {"del_": "del", "from_": "from"}.get(f, f)
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {cf}));')
+    lines.append("return 0; }")
+    src = tmp_path / "sizes.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split("\n") if l)
+    for cname, py in mirrors.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)

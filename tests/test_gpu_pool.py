@@ -1,0 +1,147 @@
+"""Continuous-batching docking pool (GPU tests).
+
+A job's draws depend only on its own seed and generation, so its result must
+be bit-identical alone in a pool, in a mix of jobs that arrive and finish at
+other steps, and as the single-job GA (fa_ga) with the same seed.  Admission
+is bounded by the pool's slots and every submitted job finishes once."""
+import numpy as np
+import pytest
+
+from flexaidds_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+POP, NEW = 128, 128
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from flexaidds_amd import hip as h
+    return h
+
+
+@pytest.fixture(scope="module")
+def system():
+    return synth.make_config2(n_ligands=5, pop=8)
+
+
+@pytest.fixture(scope="module")
+def ctx(hip, system):
+    c = hip.Context()
+    c.upload_system(system)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def limits(system):
+    return [synth.ga_limits(l) for l in system.ligands]
+
+
+def _drain(pool, max_steps=1000):
+    out = []
+    for _ in range(max_steps):
+        running, queued = pool.step()
+        out += pool.collect()
+        if running == 0 and queued == 0:
+            return out
+    raise AssertionError("pool did not drain")
+
+
+def _same(a, b):
+    assert a["best"] == b["best"]
+    assert np.array_equal(a["best_genes"], b["best_genes"])
+    assert np.array_equal(a["best_ic"], b["best_ic"])
+    assert a["thermo"] == b["thermo"]
+    assert a["n_poses"] == b["n_poses"]
+
+
+@pytest.mark.parametrize("fitness", ["LINEAR", "PSHARE"])
+def test_job_equals_single_ga(hip, ctx, limits, fitness):
+    gens, seed, lig = 6, 1234, 3
+    ga = hip.GA(ctx, limits[lig], ligand=lig, population=POP, offspring=NEW, seed=seed, fitness=fitness)
+    ga.run(gens, graph=True)
+    g, ic, res = ga.population()
+    th, n = ga.thermo()
+    ga.close()
+    pool = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=4, fitness=fitness)
+    jid = pool.submit(lig, limits[lig], gens, seed)
+    out = _drain(pool)
+    pool.close()
+    assert len(out) == 1 and out[0]["id"] == jid and out[0]["generations"] == gens
+    r = out[0]
+    best = int(np.argmin(res["cf"][:POP]))
+    assert r["best"]["cf"] == res["cf"][best]
+    G = len(limits[lig])
+    assert np.array_equal(r["best_genes"], g[best, :G])
+    assert np.array_equal(r["best_ic"], ic[best, :G])
+    assert r["thermo"] == th
+    assert r["n_poses"] == n == POP + gens * NEW
+
+
+def test_result_same_alone_and_in_mix(hip, ctx, limits):
+    target = dict(ligand=1, generations=8, seed=99)
+    alone = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=8)
+    alone.submit(target["ligand"], limits[1], target["generations"], target["seed"])
+    ref = _drain(alone)
+    alone.close()
+    assert len(ref) == 1
+    # a mix: jobs of every ligand arriving at different steps, with other
+    # lengths, the target arriving third, more jobs than slots
+    mix = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=3)
+    arrivals = {0: [(0, 5, 1), (4, 3, 2)], 2: [(1, 8, 99)], 3: [(2, 2, 3), (3, 9, 4)], 5: [(0, 1, 5)]}
+    ids, out = {}, []
+    for step in range(400):
+        for lig, gens, seed in arrivals.get(step, []):
+            ids[mix.submit(lig, limits[lig], gens, seed)] = (lig, gens, seed)
+        running, queued = mix.step()
+        assert running <= mix.slots
+        out += mix.collect()
+        if step > 5 and running == 0 and queued == 0:
+            break
+    mix.close()
+    assert sorted(r["id"] for r in out) == sorted(ids)
+    got = [r for r in out if ids[r["id"]] == (1, 8, 99)]
+    assert len(got) == 1
+    _same(got[0], ref[0])
+    for r in out:
+        lig, gens, seed = ids[r["id"]]
+        assert r["ligand"] == lig and r["generations"] == gens
+        assert r["n_poses"] == POP + gens * NEW
+
+
+def test_admission_bounded_by_slots(hip, ctx, limits):
+    pool = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=2)
+    assert pool.slots == 2
+    ids = [pool.submit(k % 5, limits[k % 5], 3, 10 + k) for k in range(5)]
+    seen = []
+    for _ in range(100):
+        running, queued = pool.step()
+        assert running <= 2 and running + queued <= 5
+        seen += pool.collect()
+        if running == 0 and queued == 0:
+            break
+    pool.close()
+    assert sorted(r["id"] for r in seen) == ids
+
+
+def test_hbm_budget_sets_slots(hip, ctx):
+    probe = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=1)
+    per = probe.slot_bytes
+    probe.close()
+    p = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=100, hbm_budget=3 * per + per // 2)
+    assert p.slots == 3
+    p.close()
+    with pytest.raises(hip.FAError):
+        hip.Pool(ctx, population=POP, offspring=NEW, max_slots=4, hbm_budget=per // 2)
+
+
+def test_zero_generation_job(hip, ctx, limits):
+    pool = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=2)
+    jid = pool.submit(0, limits[0], 0, 5)
+    pool.step()
+    out = pool.collect()
+    assert [r["id"] for r in out] == [jid]
+    assert out[0]["n_poses"] == POP and out[0]["generations"] == 0
+    assert pool.step() == (0, 0)
+    pool.close()
