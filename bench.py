@@ -5,8 +5,18 @@ A step is one pass of the hot path over one batch: one GA generation's
 population (2,000 chromosomes of the 45-atom / 10-torsion ligand against the
 4,000-atom receptor) decoded, rebuilt and scored with the Voronoi contact
 function, inputs already resident in HBM.  `value` is poses/s of the plain
-path (one ligand population per launch).  The batched form (many uneven
-ligands in one launch) is reported beside it.
+path (one ligand population per launch).  Beside it:
+
+  ga_dock   the full plain dock of that shape on the device GA (2,000 x 500
+            generations, one generation captured as a HIP graph)
+  batched   the batched screen: many uneven ligands (12-80 atoms, 0-15
+            torsions, 512 chromosomes each) evolved together, every job's
+            children in one pose launch per generation
+  served    the served stream: 8 resident receptors (2k-10k atoms), Poisson
+            requests (1,024 x 100 generations each) admitted into per-receptor
+            docking pools as they arrive; p50/p99 latency, aggregate poses/s,
+            and the plain path on the same mix (one request at a time, one
+            synchronous launch per generation)
 
 The roofline numbers come from the kernel's own operation counters (the
 algorithmic FLOP model, see include/fa_types.h fa_opcount) and HIP-event
@@ -101,14 +111,99 @@ def measured_traffic(pop):
     return rec.get("bytes_per_launch") if rec.get("workload") == f"plain config-1 generation, {pop} poses" else None
 
 
+def ga_dock_leg(ctx, system, args):
+    """The plain full dock: one ligand, population = offspring = --pop, the
+    given generations as replays of one captured generation."""
+    from flexaidds_amd import hip, synth
+    lig = system.ligands[0]
+    log(f"GA dock: {args.pop} x {args.ga_generations} generations")
+    t0 = time.perf_counter()
+    ga = hip.GA(ctx, synth.ga_limits(lig), population=args.pop, offspring=args.pop, seed=2024, temperature=300.0)
+    ms = ga.run(args.ga_generations, graph=True)
+    th, n = ga.thermo()
+    _, _, res = ga.population()
+    wall = time.perf_counter() - t0
+    ga.close()
+    poses = args.pop * args.ga_generations
+    return {"population": args.pop, "generations": args.ga_generations, "poses": int(n),
+            "generations_ms": round(ms, 2), "ms_per_generation": round(ms / args.ga_generations, 4),
+            "poses_per_s": round(poses / (ms / 1e3), 1), "dock_wall_s": round(wall, 3),
+            "best_cf": round(float(res["cf"][0]), 6), "free_energy_300K": round(th["free_energy"], 6)}
+
+
+def batched_leg(args):
+    """The batched screen: every ligand a job of one batched GA."""
+    import torch  # noqa: F401  (same HIP runtime as the library)
+    from flexaidds_amd import hip, synth
+    log(f"batched screen: {args.batched_ligands} ligands x {args.batched_pop}")
+    bs = synth.make_config2(n_ligands=args.batched_ligands, pop=8)
+    bctx = hip.Context()
+    bctx.upload_system(bs)
+    J = len(bs.ligands)
+    ga = hip.GA(bctx, [synth.ga_limits(l) for l in bs.ligands], ligands=list(range(J)), population=args.batched_pop,
+                offspring=args.batched_pop, seed=77, temperature=300.0)
+    ga.run(1, graph=True)                                     # capture + warm
+    ms = ga.run(args.batched_generations, graph=True)
+    ths, n = ga.thermo_jobs()
+    conc = np.exp(np.random.default_rng(3).uniform(np.log(1e-9), np.log(1e-3), J))
+    gpf = ga.grand(conc)
+    ga.close()
+    bctx.close()
+    poses = J * args.batched_pop * args.batched_generations
+    return {"ligands": J, "population": args.batched_pop, "generations": args.batched_generations,
+            "poses_per_generation": J * args.batched_pop, "ms_per_generation": round(ms / args.batched_generations, 3),
+            "poses_per_s": round(poses / (ms / 1e3), 1), "ln_xi_300K": round(gpf["log_xi"], 6),
+            "p_empty": gpf["p_empty"]}
+
+
+def served_leg(args):
+    """The served stream, and the plain path on the same kind of requests."""
+    from flexaidds_amd import hip, serve, synth
+    log(f"served stream: {args.served_requests} requests at {args.served_rate}/s")
+    systems = synth.make_config3(n_receptors=8, ligands_per_receptor=args.served_library)
+    reqs = synth.poisson_requests(np.random.default_rng(11), systems, args.served_rate, args.served_requests,
+                                  generations=args.served_generations)
+    srv = serve.Server(systems, population=args.served_pop, max_slots=64, hbm_budget_per_pool=2 << 30)
+    # warm every pool's kernels with one short job, untimed
+    srv.run([{"t": 0.0, "receptor": r, "ligand": 0, "generations": 1, "seed": 1} for r in range(len(systems))],
+            realtime=False)
+    got = srv.run(reqs, realtime=True)
+    summ = got.summary()
+    # plain path: one request at a time, one synchronous launch per generation
+    plain_gens, plain_ms, plain_poses = 0, 0.0, 0
+    for r in reqs[:2]:
+        c = srv.ctx[r["receptor"]]
+        ga = hip.GA(c, srv.limits[r["receptor"]][r["ligand"]], ligand=r["ligand"], population=args.served_pop,
+                    offspring=args.served_pop, seed=r["seed"])
+        for _ in range(10):
+            plain_ms += ga.run(1, graph=False)
+            plain_gens += 1
+            plain_poses += args.served_pop
+        ga.close()
+    srv.close()
+    plain = plain_poses / (plain_ms / 1e3)
+    summ.update({"receptors": len(systems), "rate_hz": args.served_rate, "population": args.served_pop,
+                 "generations": args.served_generations,
+                 "offered_poses_per_s": round(args.served_rate * args.served_pop * (args.served_generations + 1), 1),
+                 "plain_poses_per_s": round(plain, 1), "vs_plain": round(summ["poses_per_s"] / plain, 3)})
+    return summ
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--pop", type=int, default=2000)
-    ap.add_argument("--batched-ligands", type=int, default=32)
+    ap.add_argument("--ga-generations", type=int, default=500, help="plain GA dock length (0: skip)")
+    ap.add_argument("--batched-ligands", type=int, default=64, help="batched screen ligands (0: skip)")
     ap.add_argument("--batched-pop", type=int, default=512)
+    ap.add_argument("--batched-generations", type=int, default=10)
+    ap.add_argument("--served-requests", type=int, default=40, help="served stream requests (0: skip)")
+    ap.add_argument("--served-rate", type=float, default=50.0)
+    ap.add_argument("--served-library", type=int, default=6, help="resident ligands per receptor")
+    ap.add_argument("--served-generations", type=int, default=100)
+    ap.add_argument("--served-pop", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true",
                     help="only time one launch per population size (1 .. 16,384 poses of the plain workload) and exit")
@@ -193,42 +288,9 @@ def main():
     th = ctx.thermo_device(d_out[args.warmup + args.steps - 1].data_ptr() + cf_off, abi.CF_DTYPE.itemsize // 8,
                            args.pop, 300.0, stream=sh)
 
-    # batched form: many uneven ligands, all populations in one launch
-    batched = None
-    if args.batched_ligands > 0:
-        log(f"building batched workload: {args.batched_ligands} ligands x {args.batched_pop}")
-        bs = synth.make_config2(n_ligands=args.batched_ligands, pop=args.batched_pop)
-        bctx = hip.Context()
-        bctx.upload_system(bs)
-        GB = max(l.n_genes for l in bs.ligands)
-        P = sum(len(g) for g in bs.genes)
-        bg = np.zeros((P, GB))
-        bl = np.zeros(P, np.int32)
-        k = 0
-        for li, gl in enumerate(bs.genes):
-            bg[k:k + len(gl), :gl.shape[1]] = gl
-            bl[k:k + len(gl)] = li
-            k += len(gl)
-        perm = np.random.default_rng(5).permutation(P)
-        dbg = torch.from_numpy(bg[perm]).cuda()
-        dbl = torch.from_numpy(bl[perm]).cuda()
-        dbo = torch.zeros(P * abi.CF_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
-        bctx.eval_device(P, dbg.data_ptr(), GB, dbl.data_ptr(), dbo.data_ptr(), sh)
-        torch.cuda.synchronize()
-        reps = 3
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            bctx.eval_device(P, dbg.data_ptr(), GB, dbl.data_ptr(), dbo.data_ptr(), sh)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        bms = e0.elapsed_time(e1) / reps
-        bflop = hip.flops(bctx.opcount())
-        batched = {"poses": P, "ligands": args.batched_ligands, "ms_per_launch": round(bms, 3),
-                   "poses_per_s": round(P / (bms / 1e3), 1),
-                   "achieved_tflops": round(bflop / (bms / 1e3) / 1e12, 4),
-                   "vs_plain": round(P / (bms / 1e3) / value, 3)}
-        bctx.close()
+    ga_dock = ga_dock_leg(ctx, system, args) if args.ga_generations > 0 else None
+    batched = batched_leg(args) if args.batched_ligands > 0 else None
+    served = served_leg(args) if args.served_requests > 0 else None
 
     cpu = None if args.no_cpu else cpu_baseline(system, 0, pops.reshape(-1, G))
     line = {
@@ -255,7 +317,9 @@ def main():
         "cpu_baseline": cpu,
         "latency_ms": {"generation_p50": round(float(np.percentile(launch_ms, 50)), 4),
                        "generation_p99": round(float(np.percentile(launch_ms, 99)), 4)},
+        "ga_dock": ga_dock,
         "batched": batched,
+        "served": served,
         "status_counts": {"ok": int(status[0]), "hull_fail": int(status[1]), "clash": int(status[2]),
                           "capacity": int(status[3]), "grid_range": int(status[4])},
         "capacity_overflows": cap["capacity"],

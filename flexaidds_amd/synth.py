@@ -304,8 +304,8 @@ def _try_ligand(rng, n, n_tors, ori, pts, grid_ic, ntypes):
         adj0[i].append(parent[i])
         adj0[parent[i]].append(i)
 
-    def depth_sep_ok(x, o):
-        """True when atoms x and o are more than three bonds apart."""
+    def within3(x):
+        """Atoms at most three bonds from x (x included)."""
         frontier, seen = [x], {x}
         for _ in range(3):
             nxt = []
@@ -315,7 +315,13 @@ def _try_ligand(rng, n, n_tors, ori, pts, grid_ic, ntypes):
                         seen.add(b)
                         nxt.append(b)
             frontier = nxt
-        return o not in seen
+        return seen
+
+    near3 = [within3(x) for x in range(n)]
+
+    def depth_sep_ok(x, o):
+        """True when atoms x and o are more than three bonds apart."""
+        return o not in near3[x]
 
     dis = np.full(n, 1.5, np.float32)
     ang = np.zeros(n, np.float32)
@@ -586,3 +592,48 @@ def make_config2(n_ligands=256, pop=512, seed=None) -> System:
         ligs.append(lig)
         genes.append(random_genes(rng, lig, pop))
     return System(receptor=rec, ligands=ligs, params=params, genes=genes, config=2)
+
+
+def config3_ligand_size(rng):
+    """Heavy atoms log-normal(mu = ln 35, sigma = 0.4) clipped to 10..100;
+    torsions about atoms / 5."""
+    na = int(np.clip(round(float(rng.lognormal(math.log(35.0), 0.4))), 10, 100))
+    return na, int(round(na / 5.0))
+
+
+def make_config3(n_receptors=8, ligands_per_receptor=16, seed=None, n_poses=8) -> list:
+    """Served stream: `n_receptors` resident receptors of 2,000..10,000 atoms
+    (cleft shells as in config 0), each with a resident library of
+    `ligands_per_receptor` ligands sized as config3_ligand_size draws.  A
+    request names a receptor and one of its ligands."""
+    rng = np.random.default_rng(CONFIG_SEED[3] if seed is None else seed)
+    sizes = np.linspace(2000, 10000, n_receptors).round().astype(int)
+    systems = []
+    for n_rec in sizes:
+        center = rng.uniform(-30.0, 30.0, size=3)
+        rec = make_receptor(rng, int(n_rec), center, r_in=CAVITY_RADIUS)
+        ori = rec.xyz.astype(np.float64).mean(axis=0).astype(np.float32)
+        params = Params(ntypes=40, emat=make_emat(rng))
+        ligs, genes = [], []
+        for _ in range(ligands_per_receptor):
+            na, nt = config3_ligand_size(rng)
+            pts = _box_points(rng, center, 256)
+            lig = make_ligand(rng, na, nt, ori, center, pts)
+            finalize_frame(lig, rec)
+            ligs.append(lig)
+            genes.append(random_genes(rng, lig, n_poses))
+        systems.append(System(receptor=rec, ligands=ligs, params=params, genes=genes, config=3))
+    return systems
+
+
+def poisson_requests(rng, systems, rate_hz, n_requests, generations=100):
+    """Request stream: Poisson arrivals at `rate_hz`; receptor uniform, ligand
+    uniform in its library.  Returns dicts sorted by arrival time (s)."""
+    t = np.cumsum(rng.exponential(1.0 / rate_hz, size=n_requests))
+    out = []
+    for i in range(n_requests):
+        r = int(rng.integers(0, len(systems)))
+        l = int(rng.integers(0, len(systems[r].ligands)))
+        out.append({"t": float(t[i]), "receptor": r, "ligand": l, "generations": generations,
+                    "seed": int(rng.integers(1, 2**62))})
+    return out

@@ -145,3 +145,25 @@ def test_zero_generation_job(hip, ctx, limits):
     assert out[0]["n_poses"] == POP and out[0]["generations"] == 0
     assert pool.step() == (0, 0)
     pool.close()
+
+
+def test_server_results_match_pools_alone(hip):
+    # two resident receptors stepped back to back (their pools overlap on the
+    # device), requests arriving together: each result equals the same job
+    # alone in a fresh pool of its receptor
+    from flexaidds_amd import serve
+    systems = synth.make_config3(n_receptors=2, ligands_per_receptor=2, n_poses=2)
+    reqs = [{"t": 0.0, "receptor": k % 2, "ligand": (k // 2) % 2, "generations": 3 + k % 3, "seed": 500 + k}
+            for k in range(6)]
+    srv = serve.Server(systems, population=64, max_slots=2, hbm_budget_per_pool=1 << 28)
+    got = srv.run(reqs, realtime=False)
+    assert sorted(got.results) == list(range(len(reqs)))
+    for i, r in enumerate(reqs):
+        alone = hip.Pool(srv.ctx[r["receptor"]], population=64, max_slots=1)
+        alone.submit(r["ligand"], srv.limits[r["receptor"]][r["ligand"]], r["generations"], r["seed"])
+        ref = _drain(alone)
+        alone.close()
+        _same(got.results[i], ref[0])
+    s = got.summary()
+    assert s["requests"] == len(reqs) and s["poses"] == sum(64 + r["generations"] * 64 for r in reqs)
+    srv.close()
