@@ -22,7 +22,7 @@ struct fa_ga {
     fa_ga_params p{};
     int jobs = 0, Gmax = 0;
     fa::GaDev d{};
-    DevBuf gi, gic, res, gi2, gic2, res2, cum, lim, ngenes, ens, cnt, lig, thermo, grand, fit, fit2, share, sig;
+    DevBuf rank, gi, gic, res, gi2, gic2, res2, cum, lim, ngenes, ens, cnt, lig, thermo, grand, fit, fit2, share, sig;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -116,7 +116,7 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
         return code;
     };
     int rc;
-    if ((rc = fa_ensure(ga->gi, sizeof(int32_t) * J * n_all * Gmax)) ||
+    if ((rc = fa_ensure(ga->rank, sizeof(int) * J * n_all)) || (rc = fa_ensure(ga->gi, sizeof(int32_t) * J * n_all * Gmax)) ||
         (rc = fa_ensure(ga->gic, sizeof(double) * J * n_all * Gmax)) ||
         (rc = fa_ensure(ga->res, sizeof(fa_cf_result) * J * n_all)) ||
         (rc = fa_ensure(ga->gi2, sizeof(int32_t) * J * pop * Gmax)) ||
@@ -166,6 +166,7 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     d.lmin = lp;
     d.ldel = lp + plane;
     d.lbin = lp + 2 * plane;
+    d.rank = static_cast<int*>(ga->rank.p);
     d.gi = static_cast<int32_t*>(ga->gi.p);
     d.gic = static_cast<double*>(ga->gic.p);
     d.res = static_cast<fa_cf_result*>(ga->res.p);
@@ -342,7 +343,7 @@ int fa_ga_destroy(fa_ga* ga)
     if (ga->stream) (void)hipStreamSynchronize(ga->stream);
     if (ga->exec) (void)hipGraphExecDestroy(ga->exec);
     if (ga->graph) (void)hipGraphDestroy(ga->graph);
-    for (DevBuf* b : {&ga->gi, &ga->gic, &ga->res, &ga->gi2, &ga->gic2, &ga->res2, &ga->cum, &ga->lim, &ga->ngenes,
+    for (DevBuf* b : {&ga->rank, &ga->gi, &ga->gic, &ga->res, &ga->gi2, &ga->gic2, &ga->res2, &ga->cum, &ga->lim, &ga->ngenes,
                       &ga->ens, &ga->cnt, &ga->lig, &ga->thermo, &ga->grand, &ga->fit, &ga->fit2, &ga->share,
                       &ga->sig})
         if (b->p) (void)hipFree(b->p);

@@ -38,6 +38,7 @@ struct GaDev {
     int32_t* gi;                   // genes of every row
     double* gic;                   // internal coordinates, same rows
     fa_cf_result* res;             // energies of every row
+    int* rank;                     // [jobs * n_all] candidate ranks (by launch position)
     int32_t* gi2;                  // [jobs * pop * Gmax] survivors being gathered
     double* gic2;
     fa_cf_result* res2;

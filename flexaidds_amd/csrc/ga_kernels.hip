@@ -5,8 +5,9 @@
 // fixed sequence of launches on one stream:
 //
 //   tick        generation counter(s) + 1 (the counter seeds the draws)
-//   breed       one thread per offspring pair: roulette parents, crossover,
-//               mutation, genes -> internal coordinates
+//   breed       one thread per offspring pair: roulette parents, crossover;
+//               then one thread per child gene: mutation, gene -> internal
+//               coordinate
 //   poses       the contact-function kernel over the offspring (caller)
 //   ensemble    fold the offspring energies into each job's FP64 ensemble
 //   rank        parents + offspring ranked by energy (ties by index), the
@@ -137,16 +138,6 @@ __device__ void crossover(int32_t* john, int32_t* mary, int G, int intragenes, u
 }
 
 // LIB/gaboom.cpp:2620-2641
-__device__ void mutate(int32_t* genes, int G, double rate, uint64_t seed, uint64_t gen, uint64_t stream, int child)
-{
-    for (int j = 0; j < G; ++j) {
-        unsigned optr = 0u;
-        for (int i = 0; i < 32; ++i)
-            if (fa_rng_u01(seed, gen, stream, FA_DRAW_MUT + ((uint64_t)child * G + j) * 32 + i) < rate) optr |= 1u << i;
-        genes[j] ^= (int32_t)optr;
-    }
-}
-
 __global__ void ga_tick_kernel(GaDev g)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,8 +169,13 @@ __global__ void ga_init_kernel(GaDev g)
     g.gic[at] = genetoic(g.lmin[l], g.ldel[l], g.lbin[l], gene);
 }
 
-// one thread per offspring pair (LIB/gaboom.cpp:920-973)
-__global__ void ga_breed_kernel(GaDev g)
+// Reproduction (LIB/gaboom.cpp:920-973) in two launches: one thread per
+// offspring pair picks the parents by roulette and crosses them over into the
+// children rows; then one thread per child gene applies that gene's 32
+// mutation draws and converts it to an internal coordinate.  The draws are
+// the ones a single thread per pair would make (fa_rng.h layout), so the
+// split changes nothing but the parallelism.
+__global__ void ga_pair_kernel(GaDev g)
 {
     const int pairs = (g.n_new + 1) / 2;
     const long long tt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -189,9 +185,6 @@ __global__ void ga_breed_kernel(GaDev g)
     const uint64_t gen = gen_of(g, k), seed = seed_of(g, k);
     const uint64_t st = stream_of(g, k, t);
     const int G = g.G[k], Gm = g.Gmax;
-    const double* lmin = g.lmin + (size_t)k * Gm;
-    const double* ldel = g.ldel + (size_t)k * Gm;
-    const double* lbin = g.lbin + (size_t)k * Gm;
     const double* cum = g.cum + (size_t)k * g.pop;
     const int p1 = roulette(cum, g.pop, fa_rng_u01(seed, gen, st, FA_DRAW_P1));
     const int p2 = roulette(cum, g.pop, fa_rng_u01(seed, gen, st, FA_DRAW_P2));
@@ -201,20 +194,38 @@ __global__ void ga_breed_kernel(GaDev g)
         mary[j] = g.gi[parent_row(g, k, p2) * Gm + j];
     }
     if (fa_rng_u01(seed, gen, st, FA_DRAW_CROSS) < g.cross) crossover(john, mary, G, g.intragenes, seed, gen, st);
-    mutate(john, G, g.mut, seed, gen, st, 0);
-    mutate(mary, G, g.mut, seed, gen, st, 1);
-    // genes past the job's own count stay 0 (rows are Gmax wide)
     const size_t c1 = child_row(g, a, 2 * t);
-    for (int j = 0; j < Gm; ++j) {
-        g.gi[c1 * Gm + j] = (j < G) ? john[j] : 0;
-        g.gic[c1 * Gm + j] = (j < G) ? genetoic(lmin[j], ldel[j], lbin[j], john[j]) : 0.0;
+    for (int j = 0; j < G; ++j) g.gi[c1 * Gm + j] = john[j];
+    if (2 * t + 1 < g.n_new)
+        for (int j = 0; j < G; ++j) g.gi[(c1 + 1) * Gm + j] = mary[j];
+}
+
+// per child gene: mutation (LIB/gaboom.cpp:2620-2641) and genetoic; genes
+// past the job's own count are 0 (rows are Gmax wide)
+__global__ void ga_mutate_kernel(GaDev g)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.n_act * g.n_new * g.Gmax) return;
+    const int a = (int)(t / ((long long)g.n_new * g.Gmax));
+    const int i = (int)((t / g.Gmax) % g.n_new), j = (int)(t % g.Gmax);
+    const int k = slot_of(g, a);
+    const int G = g.G[k];
+    const size_t at = child_row(g, a, i) * g.Gmax + j;
+    if (j >= G) {
+        g.gi[at] = 0;
+        g.gic[at] = 0.0;
+        return;
     }
-    if (2 * t + 1 < g.n_new) {
-        for (int j = 0; j < Gm; ++j) {
-            g.gi[(c1 + 1) * Gm + j] = (j < G) ? mary[j] : 0;
-            g.gic[(c1 + 1) * Gm + j] = (j < G) ? genetoic(lmin[j], ldel[j], lbin[j], mary[j]) : 0.0;
-        }
-    }
+    const uint64_t gen = gen_of(g, k), seed = seed_of(g, k);
+    const uint64_t st = stream_of(g, k, i >> 1);
+    const uint64_t base = FA_DRAW_MUT + ((uint64_t)(i & 1) * G + j) * 32;
+    unsigned optr = 0u;
+    for (int b = 0; b < 32; ++b)
+        if (fa_rng_u01(seed, gen, st, base + b) < g.mut) optr |= 1u << b;
+    const int32_t gene = g.gi[at] ^ (int32_t)optr;
+    const size_t l = (size_t)k * g.Gmax + j;
+    g.gi[at] = gene;
+    g.gic[at] = genetoic(g.lmin[l], g.ldel[l], g.lbin[l], gene);
 }
 
 // energies of each job's children (children = 1) or staged initial
@@ -259,38 +270,55 @@ __device__ __forceinline__ size_t cand_row(const GaDev& g, int a, int k, int r, 
     return r < g.pop ? parent_row(g, k, r) : child_row(g, a, r - g.pop);
 }
 
-// rank of each of a job's n candidates by energy, ties by candidate index;
-// the best `pop` gathered in order (QuickSort by evalue,
-// LIB/gaboom.cpp:2674-2726).  Job position = blockIdx.y.
-__global__ __launch_bounds__(256) void ga_rank_kernel(GaDev g, int initial)
+// Rank of each of a job's n candidates by energy, ties by candidate index
+// (a total order, so ranks are a permutation); the best `pop` are gathered
+// in order (QuickSort by evalue, LIB/gaboom.cpp:2674-2726).  The counting is
+// split over candidate chunks so that one job's ranking spreads over many
+// blocks: zero, count (integer atomics: order-free), gather.
+constexpr int RANK_CHUNK = 1024;
+
+__global__ void ga_rank_zero_kernel(GaDev g, int n)
 {
-    __shared__ uint64_t keys[1024];
-    const int a = blockIdx.y;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (long long)g.n_act * n) g.rank[(t / n) * (size_t)g.n_all + (t % n)] = 0;
+}
+
+__global__ __launch_bounds__(256) void ga_rank_count_kernel(GaDev g, int initial)
+{
+    __shared__ uint64_t keys[RANK_CHUNK];
+    const int a = blockIdx.z;
     const int k = slot_of(g, a);
     const int n = initial ? g.pop : g.n_all;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t ki = (i < n) ? order_key(g.res[cand_row(g, a, k, i, initial)].cf) : 0ull;
-    int rank = 0;
-    for (int base = 0; base < n; base += 1024) {
-        __syncthreads();
-        for (int q = threadIdx.x; q < 1024 && base + q < n; q += blockDim.x)
-            keys[q] = order_key(g.res[cand_row(g, a, k, base + q, initial)].cf);
-        __syncthreads();
-        const int lim = min(1024, n - base);
-        if (i < n)
-            for (int q = 0; q < lim; ++q) {
-                const uint64_t kj = keys[q];
-                rank += (kj < ki) || (kj == ki && base + q < i);
-            }
+    const int base = blockIdx.y * RANK_CHUNK;
+    const int lim = min(RANK_CHUNK, n - base);
+    for (int q = threadIdx.x; q < lim; q += blockDim.x) keys[q] = order_key(g.res[cand_row(g, a, k, base + q, initial)].cf);
+    __syncthreads();
+    if (i >= n) return;
+    const uint64_t ki = order_key(g.res[cand_row(g, a, k, i, initial)].cf);
+    int cnt = 0;
+    for (int q = 0; q < lim; ++q) {
+        const uint64_t kj = keys[q];
+        cnt += (kj < ki) || (kj == ki && base + q < i);
     }
-    if (i < n && rank < g.pop) {
-        const size_t src = cand_row(g, a, k, i, initial), dst = parent_row(g, k, rank);
-        for (int j = 0; j < g.Gmax; ++j) {
-            g.gi2[dst * g.Gmax + j] = g.gi[src * g.Gmax + j];
-            g.gic2[dst * g.Gmax + j] = g.gic[src * g.Gmax + j];
-        }
-        g.res2[dst] = g.res[src];
+    if (cnt) atomicAdd(&g.rank[(size_t)a * g.n_all + i], cnt);
+}
+
+__global__ void ga_rank_gather_kernel(GaDev g, int initial)
+{
+    const int n = initial ? g.pop : g.n_all;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.n_act * n) return;
+    const int a = (int)(t / n), i = (int)(t % n);
+    const int k = slot_of(g, a);
+    const int rank = g.rank[(size_t)a * g.n_all + i];
+    if (rank >= g.pop) return;
+    const size_t src = cand_row(g, a, k, i, initial), dst = parent_row(g, k, rank);
+    for (int j = 0; j < g.Gmax; ++j) {
+        g.gi2[dst * g.Gmax + j] = g.gi[src * g.Gmax + j];
+        g.gic2[dst * g.Gmax + j] = g.gic[src * g.Gmax + j];
     }
+    g.res2[dst] = g.res[src];
 }
 
 // gathered survivors (and, with fit_too, their fitness) into the parent rows
@@ -521,8 +549,8 @@ void ga_launch_init(const GaDev& g, hipStream_t s)
 void ga_launch_breed(const GaDev& g, hipStream_t s)
 {
     if (g.n_act < 1) return;
-    hipLaunchKernelGGL(ga_breed_kernel, dim3(blocks_of((long long)g.n_act * ((g.n_new + 1) / 2))), dim3(256), 0, s,
-                       g);
+    hipLaunchKernelGGL(ga_pair_kernel, dim3(blocks_of((long long)g.n_act * ((g.n_new + 1) / 2))), dim3(256), 0, s, g);
+    hipLaunchKernelGGL(ga_mutate_kernel, dim3(blocks_of((long long)g.n_act * g.n_new * g.Gmax)), dim3(256), 0, s, g);
 }
 
 void ga_launch_ensemble(const GaDev& g, int children, hipStream_t s)
@@ -536,7 +564,11 @@ void ga_launch_select(const GaDev& g, int initial, hipStream_t s)
     if (g.n_act < 1) return;
     const int n = initial ? g.pop : g.n_all;
     const unsigned rows = blocks_of((long long)g.n_act * g.pop * g.Gmax);
-    hipLaunchKernelGGL(ga_rank_kernel, dim3(blocks_of(n), g.n_act), dim3(256), 0, s, g, initial);
+    const long long cand = (long long)g.n_act * n;
+    hipLaunchKernelGGL(ga_rank_zero_kernel, dim3(blocks_of(cand)), dim3(256), 0, s, g, n);
+    hipLaunchKernelGGL(ga_rank_count_kernel, dim3(blocks_of(n), (n + RANK_CHUNK - 1) / RANK_CHUNK, g.n_act), dim3(256),
+                       0, s, g, initial);
+    hipLaunchKernelGGL(ga_rank_gather_kernel, dim3(blocks_of(cand)), dim3(256), 0, s, g, initial);
     hipLaunchKernelGGL(ga_copy_back_kernel, dim3(rows), dim3(256), 0, s, g, 0);
     if (g.model == FA_FIT_LINEAR) {
         hipLaunchKernelGGL(ga_linear_kernel, dim3(g.n_act), dim3(256), 0, s, g);
