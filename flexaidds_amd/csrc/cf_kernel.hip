@@ -201,17 +201,48 @@ __device__ __forceinline__ uint64_t dpp_u64(uint64_t v)
 
 __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
-// Minimum over the wave of an unsigned 64-bit key.  Must be called with all
-// lanes active.  DPP inside each 16-lane row, then the four row minima.
+// one DPP step of a 32-bit minimum: lanes of rows outside ROWS, or without a
+// source, combine with themselves
+template <int CTRL, int ROWS>
+__device__ __forceinline__ unsigned dpp_min_u32(unsigned v)
+{
+    const unsigned o = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROWS, 0xF, false);
+    return v < o ? v : o;
+}
+
+// one in-row DPP step of a 32-bit minimum: every lane has a source in its
+// row for the controls used, so bound_ctrl (read 0 without a source) never
+// applies and the move can fuse into the minimum
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_row_min_u32(unsigned v)
+{
+    const unsigned o = (unsigned)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+    return v < o ? v : o;
+}
+
+// Minimum over the wave of an unsigned 32-bit value, all lanes active:
+// butterflies inside each row, then row_bcast:15 (rows 1, 3) and
+// row_bcast:31 (rows 2, 3) carry the lower rows up; lane 63 ends with the
+// wave minimum.
+__device__ __forceinline__ unsigned wave_min_u32(unsigned v)
+{
+    v = dpp_row_min_u32<0xB1>(v);    // quad_perm [1,0,3,2]
+    v = dpp_row_min_u32<0x4E>(v);    // quad_perm [2,3,0,1]
+    v = dpp_row_min_u32<0x141>(v);   // row_half_mirror
+    v = dpp_row_min_u32<0x140>(v);   // row_mirror
+    v = dpp_min_u32<0x142, 0xA>(v);  // row_bcast:15
+    v = dpp_min_u32<0x143, 0xC>(v);  // row_bcast:31
+    return readlane_u(v, 63);
+}
+
+// Minimum over the wave of an unsigned 64-bit key, all lanes active: the
+// minimum high word, then the minimum low word among the lanes that hold it.
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
 {
-    v = umin64(v, dpp_u64<0xB1>(v));   // quad_perm [1,0,3,2]
-    v = umin64(v, dpp_u64<0x4E>(v));   // quad_perm [2,3,0,1]
-    v = umin64(v, dpp_u64<0x141>(v));  // row_half_mirror
-    v = umin64(v, dpp_u64<0x140>(v));  // row_mirror
-    const uint64_t r0 = readlane_u64(v, 0), r1 = readlane_u64(v, 16);
-    const uint64_t r2 = readlane_u64(v, 32), r3 = readlane_u64(v, 48);
-    return umin64(umin64(r0, r1), umin64(r2, r3));
+    const unsigned hi = (unsigned)(v >> 32);
+    const unsigned mh = wave_min_u32(hi);
+    const unsigned ml = wave_min_u32(hi == mh ? (unsigned)v : 0xFFFFFFFFu);
+    return ((uint64_t)mh << 32) | ml;
 }
 
 // non-negative doubles order like their bit patterns
@@ -449,11 +480,11 @@ __device__ __forceinline__ double vnorm(const Smem& s, int v)
 
 // ------------------------------------------------------------ edge registers
 // Edge e lives in lane (e & 63), slot (e >> 6) as two packed words; its
-// direction is recomputed from its planes when the walk takes it up.
+// direction is recomputed from its planes when the walk takes it up.  An
+// edge is open (still needs its end point) while its end plane is NONE.
 struct Edges {
     unsigned pl[ESL];                   // plane0 | plane1 << 8 | start plane << 16 | end plane << 24
     unsigned vt[ESL];                   // start vertex | end vertex << 16
-    unsigned long long open[ESL];       // wave-uniform: the edge still needs its end point
 };
 
 #define FA_EDGE_SLOTS(M) M(0) M(1) M(2) M(3)
@@ -473,7 +504,6 @@ __device__ __forceinline__ void edge_new(Edges& E, int e, int p0, int p1, int tp
             E.pl[K] = pl;              \
             E.vt[K] = vt;              \
         }                              \
-        E.open[K] |= 1ull << (e & 63); \
         break;
         FA_EDGE_SLOTS(FA_NEW)
 #undef FA_NEW
@@ -491,7 +521,6 @@ __device__ __forceinline__ void edge_close(Edges& E, int e, int end_plane, int e
             E.pl[K] = (E.pl[K] & 0x00FFFFFFu) | (((unsigned)end_plane & NONE) << 24); \
             E.vt[K] = (E.vt[K] & 0xFFFFu) | ((unsigned)end_vertex << 16);             \
         }                                                                             \
-        E.open[K] &= ~(1ull << (e & 63));                                             \
         break;
         FA_EDGE_SLOTS(FA_CLOSE)
 #undef FA_CLOSE
@@ -515,12 +544,21 @@ __device__ __forceinline__ void edge_get(const Edges& E, int e, unsigned& pl, un
     }
 }
 
-// lowest-numbered open edge, -1 when none
-__device__ __forceinline__ int first_open(const Edges& E)
+// Lowest-numbered open edge among edges [from, edgenum), -1 when none.  The
+// walk always takes the lowest open edge, new edges get the next number and
+// edges only ever close, so the lowest open edge never moves down: the
+// caller passes the previous one as `from`.
+__device__ __forceinline__ int first_open(const Edges& E, int from, int edgenum)
 {
+    const int l = lane_id();
 #pragma unroll
-    for (int k = 0; k < ESL; ++k)
-        if (E.open[k]) return k * 64 + first_lane(E.open[k]);
+    for (int k = 0; k < ESL; ++k) {
+        if (k * 64 >= edgenum) break;
+        if ((k + 1) * 64 <= from) continue;
+        const int e = k * 64 + l;
+        const unsigned long long m = __ballot(e >= from && e < edgenum && (E.pl[k] >> 24) == NONE);
+        if (m) return k * 64 + first_lane(m);
+    }
     return -1;
 }
 
@@ -528,12 +566,13 @@ __device__ __forceinline__ int first_open(const Edges& E)
 __device__ __forceinline__ int find_edge(const Edges& E, int lo, int hi, int P, int C)
 {
     const int l = lane_id();
+    const unsigned pc = (unsigned)P | ((unsigned)C << 8), cp = (unsigned)C | ((unsigned)P << 8);
 #pragma unroll
     for (int k = 0; k < ESL; ++k) {
         if (k * 64 >= hi) break;
         const int e = k * 64 + l;
-        const int q0 = (int)(E.pl[k] & NONE), q1 = (int)((E.pl[k] >> 8) & NONE);
-        const bool hit = e > lo && e < hi && ((q0 == P && q1 == C) || (q0 == C && q1 == P));
+        const unsigned both = E.pl[k] & 0xFFFFu;  // plane0 | plane1 << 8
+        const bool hit = e > lo && e < hi && (both == pc || both == cp);
         const unsigned long long m = __ballot(hit);
         if (m) return k * 64 + first_lane(m);
     }
@@ -1626,7 +1665,6 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 for (int q = 0; q < ESL; ++q) {
                     E.pl[q] = 0xFFFFFFFFu;
                     E.vt[q] = 0xFFFFFFFFu;
-                    E.open[q] = 0ull;
                 }
                 int vn = 1, edgenum = 3;
                 {
@@ -1648,8 +1686,9 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 bool hull_fail = false, cap = false;
                 const double rr = (double)(rado * rado);
                 long long hull_tests = 0;
+                int se = 0;
                 while (true) {
-                    const int se = first_open(E);
+                    se = first_open(E, se, edgenum);
                     if (se < 0) break;
                     const HullStep w = hull_finish(hull_search(s, E, se, NC, pa0, pa1, pa2, pa3), NC, rr);
                     hull_tests += w.tests;
