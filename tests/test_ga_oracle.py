@@ -186,3 +186,25 @@ def test_breed_fit_linear_equals_breed(gao):
     a, ai = ol.ga_breed(gao, 3, 2, parents, 40, lims)
     b, bi = ol.ga_breed_fit(gao, 3, 2, parents, (pop - np.arange(pop)).astype(float), 40, lims)
     assert np.array_equal(a, b) and np.array_equal(ai, bi)
+
+
+def test_crossover_swaps_one_contiguous_segment(ga):
+    # with john all zeros and mary all ones, john's set bits after crossover
+    # are exactly the swapped positions; reading genes in order and each
+    # gene's bits from 31 down to 0, they must form a single run (two-point
+    # crossover): a mask built in the wrong direction or an off-by-one at a
+    # cut leaves a gap or a stray bit
+    rng = np.random.default_rng(21)
+    runs = 0
+    for t in range(400):
+        G = int(rng.integers(1, 12))
+        john = np.zeros(G, np.int32)
+        mary = np.full(G, -1, np.int32)
+        ga.fa_oracle_crossover(john.ctypes.data, mary.ctypes.data, G, int(t % 2), 13, 2, t)
+        bits = np.unpackbits(john.astype('>u4').view(np.uint8))        # gene order, bit 31 first
+        assert np.array_equal(bits ^ 1, np.unpackbits(mary.astype('>u4').view(np.uint8)))
+        on = np.nonzero(bits)[0]
+        if on.size:
+            assert on[-1] - on[0] + 1 == on.size, (t, G, bits)
+            runs += 1
+    assert runs > 300
