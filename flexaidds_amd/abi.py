@@ -24,6 +24,11 @@ class ReceptorDesc(C.Structure):
     _fields_ = [("n_atoms", C.c_int32), ("xyz", _f32p), ("radius", _f32p), ("type", _i32p)]
 
 
+class Constraint(C.Structure):
+    _fields_ = [("atom", C.c_int32), ("partner", C.c_int32), ("partner_is_ligand", C.c_int32), ("type", C.c_int32),
+                ("bond_len", C.c_float), ("max_dist", C.c_float)]
+
+
 class LigandDesc(C.Structure):
     _fields_ = [
         ("n_atoms", C.c_int32), ("n_genes", C.c_int32), ("n_grid", C.c_int32),
@@ -32,7 +37,7 @@ class LigandDesc(C.Structure):
         ("gene_typ", _i32p), ("gene_atm", _i32p), ("gene_min", _f64p), ("gene_max", _f64p),
         ("grid_ic", _f32p),
         ("ori", C.c_float * 3), ("globalmin", C.c_float * 3), ("globalmax", C.c_float * 3),
-        ("maxwidth", C.c_float),
+        ("maxwidth", C.c_float), ("n_cons", C.c_int32), ("cons", C.POINTER(Constraint)),
     ]
 
 
@@ -110,6 +115,13 @@ class Descriptors:
             d.globalmin[k] = float(l.globalmin[k])
             d.globalmax[k] = float(l.globalmax[k])
         d.maxwidth = float(l.maxwidth)
+        cons = getattr(l, "cons", None) or []
+        d.n_cons = len(cons)
+        if cons:
+            arr = (Constraint * len(cons))(*[Constraint(int(a), int(p), int(pl), int(t), float(b), float(m))
+                                             for a, p, pl, t, b, m in cons])
+            self._keep.append(arr)
+            d.cons = C.cast(arr, C.POINTER(Constraint))
         return d
 
     def _prm(self, p):

@@ -377,6 +377,15 @@ __device__ __noinline__ double spherical_arc_call(double ax, double ay, double a
 // one spherical triangle of a face fan (LIB/Vcontacts.cpp:926-940): the term
 // the reference adds for the triangle with cosines c0 (vertex 0 to q),
 // cq (q to q+1) and c1 (vertex 0 to q+1); zero when it adds nothing
+constexpr double KDIST = 1.0e3;  // LIB/flexaid.h:83
+
+// GetValueFromGaussian (LIB/geometry.cpp:3-5): 1 at x = opt, 0 at x = zero
+// and 2 opt - zero, an even power outside
+__device__ __forceinline__ double gaussian_bell(double x, double opt, double zero)
+{
+    return pow(-(x - zero) * (x - (2 * opt - zero)) / pow(zero - opt, 2.0), 50.0);
+}
+
 // Scored value of one type pair k for `area` of an atom whose whole sphere
 // is `surf`: weight-1 pairs give y * area (y * area / surf with
 // normalize_area, LIB/vcfunction.cpp:343-353, :479-491); density pairs give
@@ -1284,7 +1293,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
         const int L = lig.n_atoms;
         const int Rn = R.n;
         int status = FA_OK;
-        double r_com = 0.0, r_wal = 0.0, r_sas = 0.0, r_tot = 0.0;
+        double r_com = 0.0, r_wal = 0.0, r_sas = 0.0, r_tot = 0.0, r_con = 0.0;
         int r_rclash = 0;
         int nrec_total = 0;
         double clash = 0.0;
@@ -2284,7 +2293,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     const double dee = (double)prm.dee_clash;
                     const int nt = prm.ntypes;
                     const int ta = lig.type[a];
-                    double sas_sub = 0.0, com = 0.0, wal = 0.0;
+                    double sas_sub = 0.0, com = 0.0, wal = 0.0, con = 0.0;
                     int rcl = 0;
                     bool valid[PSL];
                     int nvalid_total = 0;
@@ -2325,14 +2334,33 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                             }
                             if (!skip) {
                                 const double rAB = radA + (double)prb[q];
-                                const float clash_distance = (float)(permea * rAB);
+                                // covalent constraint of the pair (LIB/vcfunction.cpp:270-310): the
+                                // first constraint of `a` naming this partner decides
+                                bool covalent = false;
+                                double dist_opt = 0.0;
+                                if (lig.cons_off) {
+                                    const int c1 = lig.cons_off[a + 1];
+                                    for (int k = lig.cons_off[a]; k < c1; ++k) {
+                                        if (lig.cons_part[k] != j) continue;
+                                        if (lig.cons_type[k] == 1) {
+                                            covalent = true;
+                                            dist_opt = (double)lig.cons_bond[k];
+                                            con -= KDIST * gaussian_bell(dist, dist_opt, (double)lig.cons_max[k]);
+                                        }
+                                        break;
+                                    }
+                                }
+                                float clash_distance = (float)(permea * rAB);
+                                if (covalent && permea * dist_opt < permea * rAB) clash_distance = (float)(permea * dist_opt);
                                 if (dist < (double)clash_distance) {
+                                    // the wall keeps permeability * rAB as its reference distance
                                     const double d2 = dist * dist, d4 = d2 * d2, d6 = d4 * d2;
                                     const double cr = permea * rAB, cr2 = cr * cr, cr4 = cr2 * cr2, cr6 = cr4 * cr2;
                                     wal += (double)KWALL_F * (1.0 / (d6 * d6) - 1.0 / (cr6 * cr6));
                                     if (dist <= dee * rAB) rcl = 1;
                                 }
-                                if (prm.intramolecular || !lj) com += pair_term(prm, (ta - 1) * nt + (tb - 1), ar, surfA);
+                                if (!covalent && (prm.intramolecular || !lj))
+                                    com += pair_term(prm, (ta - 1) * nt + (tb - 1), ar, surfA);
                             }
                         }
                         seen += __popcll(m);
@@ -2342,6 +2370,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     sas_sub = wave_sum_d(sas_sub);
                     com = wave_sum_d(com);
                     wal = wave_sum_d(wal);
+                    if (lig.cons_off) r_con += wave_sum_d(con);
                     rcl = wave_sum_i(rcl);
                     double SAS = surfA - sas_sub;
                     if (SAS < 0.0) SAS = 0.0;
@@ -2387,6 +2416,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
             res.sas = r_sas;
             res.totsas = r_tot;
             res.rclash = r_rclash;
+            res.con = lig.con0 + r_con;  // KDIST per covalent constraint, less the satisfied ones
             res.cf = res.com + res.wal + res.sas + res.con;
             res.app = res.com + res.wal + res.sas;
         } else if (status == FA_CLASH) {

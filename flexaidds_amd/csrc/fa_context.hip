@@ -106,6 +106,14 @@ int validate_ligand(const fa_ligand_desc& L, int idx)
         if (t == FA_GENE_GRID && (L.gene_min[g] < 0.0 || L.gene_max[g] > (double)(L.n_grid - 1)))
             return fa_fail(FA_ERR_INVALID, tag + "grid gene limits outside the cleft grid");
     }
+    if (L.n_cons < 0 || (L.n_cons > 0 && !L.cons)) return fa_fail(FA_ERR_INVALID, tag + "bad constraint list");
+    for (int q = 0; q < L.n_cons; ++q) {
+        const fa_constraint& k = L.cons[q];
+        if (k.atom < 0 || k.atom >= n) return fa_fail(FA_ERR_INVALID, tag + "constraint atom out of range");
+        if (k.partner < 0 || (k.partner_is_ligand && (k.partner >= n || k.partner == k.atom)))
+            return fa_fail(FA_ERR_INVALID, tag + "constraint partner out of range");
+        if (k.type != 0 && k.type != 1) return fa_fail(FA_ERR_INVALID, tag + "constraint type must be 0 or 1");
+    }
     return FA_SUCCESS;
 }
 
@@ -348,6 +356,38 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
             (rc = upload(c, L.gene_max, L.n_genes, &D.gene_max)) ||
             (rc = upload(c, L.grid_ic, (size_t)3 * L.n_grid, &D.grid_ic)))
             return rc;
+        // constraints per scorable atom: a ligand pair is listed on both of
+        // its atoms (each counts its own starting penalty)
+        D.cons_off = nullptr;
+        D.con0 = 0.0;
+        if (L.n_cons > 0) {
+            std::vector<std::vector<int>> at(n);
+            for (int q = 0; q < L.n_cons; ++q) {
+                if (L.cons[q].partner >= Rn && !L.cons[q].partner_is_ligand)
+                    return fa_fail(FA_ERR_INVALID, "ligand " + std::to_string(li) + ": constraint partner out of range");
+                at[L.cons[q].atom].push_back(q);
+                if (L.cons[q].partner_is_ligand) at[L.cons[q].partner].push_back(q);
+            }
+            std::vector<int> off(n + 1, 0), part, typ;
+            std::vector<float> bond, mx;
+            for (int a = 0; a < n; ++a) {
+                for (int q : at[a]) {
+                    const fa_constraint& k = L.cons[q];
+                    const int other = (k.partner_is_ligand && k.partner == a) ? k.atom : k.partner;
+                    part.push_back(k.partner_is_ligand ? Rn + other : other);
+                    typ.push_back(k.type);
+                    bond.push_back(k.bond_len);
+                    mx.push_back(k.max_dist);
+                    if (k.type == 1) D.con0 += 1.0e3;  // KDIST, LIB/flexaid.h:83
+                }
+                off[a + 1] = (int)part.size();
+            }
+            const size_t m = part.size();
+            if ((rc = upload(c, off.data(), off.size(), &D.cons_off)) || (rc = upload(c, part.data(), m, &D.cons_part)) ||
+                (rc = upload(c, typ.data(), m, &D.cons_type)) || (rc = upload(c, bond.data(), m, &D.cons_bond)) ||
+                (rc = upload(c, mx.data(), m, &D.cons_max)))
+                return rc;
+        }
         c->h_ligs[li] = D;
         c->lig_genes[li] = L.n_genes;
     }

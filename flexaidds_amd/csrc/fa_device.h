@@ -53,6 +53,14 @@ struct DevLigand {
     const double* gene_min;
     const double* gene_max;
     const float* grid_ic;           // [3 * n_grid]
+    // distance constraints of each atom: [cons_off[a], cons_off[a + 1]) of
+    // the arrays below, partner as a calc index (receptor i, ligand Rn + k)
+    const int* cons_off;            // [n + 1], null without constraints
+    const int* cons_part;
+    const int* cons_type;
+    const float* cons_bond;
+    const float* cons_max;
+    double con0;                    // KDIST per covalent constraint of each atom
 };
 
 struct DevParams {

@@ -72,6 +72,8 @@ class Ligand:
     maxwidth: float
     parent: np.ndarray    # (n,) int32, tree parent (-1 for root)
     flex_bonds: list = field(default_factory=list)
+    # distance constraints: (atom, partner, partner_is_ligand, type, bond_len, max_dist)
+    cons: list = field(default_factory=list)
 
     @property
     def n_atoms(self) -> int:
@@ -637,3 +639,27 @@ def poisson_requests(rng, systems, rate_hz, n_requests, generations=100):
         out.append({"t": float(t[i]), "receptor": r, "ligand": l, "generations": generations,
                     "seed": int(rng.integers(1, 2**62))})
     return out
+
+
+def add_constraints(system: System, lig_index=0, n_receptor=2, n_ligand=1, bond_len=3.2, max_dist=4.6, seed=0):
+    """Covalent (type 1) distance constraints on one ligand: `n_receptor`
+    pairs between random ligand atoms and the receptor atoms nearest the
+    cleft centre, and the `n_ligand` closest non-bonded ligand pairs of the
+    reference pose (often in contact); contact distances fall inside the
+    bell [2 bond_len - max_dist, max_dist] or, outside it, on the even-power
+    tails the reference's Gaussian has."""
+    rng = np.random.default_rng(seed)
+    lig = system.ligands[lig_index]
+    rec = system.receptor
+    d = np.linalg.norm(rec.xyz.astype(np.float64) - rec.center[None, :], axis=1)
+    near = np.argsort(d)[:max(8, n_receptor)]
+    atoms = rng.choice(lig.n_atoms, size=n_receptor, replace=False)
+    cons = [(int(atoms[q]), int(near[q]), 0, 1, float(bond_len), float(max_dist)) for q in range(n_receptor)]
+    xyz = build_cartesian(lig).astype(np.float64)
+    dd = np.linalg.norm(xyz[:, None, :] - xyz[None, :, :], axis=2)
+    pairs = sorted((dd[a, b], a, b) for a in range(lig.n_atoms) for b in range(a + 1, lig.n_atoms)
+                   if not lig.bonded[a, b])
+    for _, a, b in pairs[:n_ligand]:
+        cons.append((int(a), int(b), 1, 1, float(bond_len), float(max_dist)))
+    lig.cons = cons
+    return cons

@@ -53,6 +53,23 @@ typedef struct fa_receptor_desc {
     const int32_t* type;     /* [n_atoms] 1-based energy-matrix type */
 } fa_receptor_desc;
 
+/* Distance constraint (constraint_str, LIB/flexaid.h:168-190) between ligand
+ * atom `atom` and a receptor atom (partner_is_ligand = 0: receptor index)
+ * or another ligand atom.  Covalent (type 1) constraints score in
+ * vcfunction (LIB/vcfunction.cpp:146-166, :270-310): +KDIST for each
+ * constrained scorable atom, -KDIST * GetValueFromGaussian(d, bond_len,
+ * max_dist) when the pair is in contact, the clash distance shortened to
+ * permeability * bond_len, and no contact energy for the pair.  Type 0
+ * constraints do not score. */
+typedef struct fa_constraint {
+    int32_t atom;
+    int32_t partner;
+    int32_t partner_is_ligand;
+    int32_t type;
+    float   bond_len;
+    float   max_dist;
+} fa_constraint;
+
 /* One flexible ligand with its gene map and cleft grid. */
 typedef struct fa_ligand_desc {
     int32_t        n_atoms;
@@ -76,6 +93,8 @@ typedef struct fa_ligand_desc {
     float          globalmin[3];
     float          globalmax[3];
     float          maxwidth;
+    int32_t        n_cons;     /* distance constraints (optional) */
+    const fa_constraint* cons; /* [n_cons], in the order the scorer looks them up */
 } fa_ligand_desc;
 
 /* Scoring parameters (FA_Global fields used by vcfunction/Vcontacts). */

@@ -940,6 +940,14 @@ template <class R> void calc_areas(Work<R>& W, float rado, int NC, int NV, int a
 }
 
 // ----------------------------------------------------------- one pose
+constexpr double KDIST = 1.0e3;  // LIB/flexaid.h:83
+
+// GetValueFromGaussian, LIB/geometry.cpp:3-5
+double gaussian_bell(double x, double opt, double zero)
+{
+    return std::pow(-(x - zero) * (x - (2 * opt - zero)) / std::pow(zero - opt, 2.0), 50.0);
+}
+
 // One type pair's scored value for `area` of an atom whose whole sphere is
 // `surf`: weight 1 -> y * area (or y * area / surf with normalize_area,
 // LIB/vcfunction.cpp:343-353, :479-491); density -> get_yval(area / surf),
@@ -1055,6 +1063,22 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
         const double radoA = radA + RW;
         double SAS = 4.0 * PI_REF * radoA * radoA;
         const double surfA = SAS;
+        // constraints of this atom in lookup order (atoms[].cons): a ligand
+        // pair is listed on both of its atoms
+        std::vector<const fa_constraint*> cons;
+        std::vector<int> cons_partner;  // calc index
+        for (int q = 0; q < lig->n_cons; ++q) {
+            const fa_constraint& k = lig->cons[q];
+            if (k.atom == i - Rn) {
+                cons.push_back(&k);
+                cons_partner.push_back(k.partner_is_ligand ? Rn + k.partner : k.partner);
+            } else if (k.partner_is_ligand && k.partner == i - Rn) {
+                cons.push_back(&k);
+                cons_partner.push_back(Rn + k.atom);
+            }
+        }
+        for (const fa_constraint* k : cons)  // LIB/vcfunction.cpp:146-166
+            if (k->type == 1) res.con += KDIST;
         OPS(vc_contact, W.records[i - Rn].size());
         for (const fa_contact_rec& r : W.records[i - Rn]) {
             const int c = r.atom;
@@ -1065,14 +1089,27 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
             const bool intra = (c >= Rn);
             if (intra && lig->bonded[(c - Rn) * L + (i - Rn)]) continue;
             if (intra && c < i) continue;  // FA->contacts: already visited
-            const float clash_distance = (float)(permea * rAB);
+            // covalent constraint (LIB/vcfunction.cpp:270-310)
+            bool covalent = false;
+            double dist_opt = 0.0;
+            for (size_t q = 0; q < cons.size(); ++q) {
+                if (cons_partner[q] != c) continue;
+                if (cons[q]->type == 1) {
+                    covalent = true;
+                    dist_opt = cons[q]->bond_len;
+                    res.con -= KDIST * gaussian_bell(r.dist, dist_opt, cons[q]->max_dist);
+                }
+                break;
+            }
+            float clash_distance = permea * rAB;
+            if (covalent && permea * dist_opt < permea * rAB) clash_distance = permea * dist_opt;
             if (r.dist < (double)clash_distance) {
                 const double d = r.dist, d2 = d * d, d4 = d2 * d2, d6 = d4 * d2;
                 const double cr = permea * rAB, cr2 = cr * cr, cr4 = cr2 * cr2, cr6 = cr4 * cr2;
                 res.wal += (double)KWALL_F * (1.0 / (d6 * d6) - 1.0 / (cr6 * cr6));
                 if (r.dist <= dee * rAB) res.rclash = 1;
             }
-            if (prm->intramolecular || !intra) res.com += pair_term(prm, pk, area, surfA);
+            if (!covalent && (prm->intramolecular || !intra)) res.com += pair_term(prm, pk, area, surfA);
         }
         if (SAS < 0.0) SAS = 0.0;
         res.totsas += SAS;

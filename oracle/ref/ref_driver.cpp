@@ -39,6 +39,8 @@ struct RefSystem {
     std::vector<OptRes> optres;
     std::vector<energy_matrix> emat;
     std::vector<energy_values> evals;
+    std::vector<constraint> cons;
+    std::vector<std::vector<constraint*>> cons_of;  // per atom number
     int Rn = 0, L = 0, N = 0;
 
     ~RefSystem()
@@ -129,6 +131,30 @@ void setup(RefSystem& S, const fa_receptor_desc* rec, const fa_ligand_desc* lig,
         }
         a.optres = &S.optres[0];
     }
+    // distance constraints (read_constraints' result): every atom of a
+    // constraint points at it, in constraint order
+    S.cons.assign(lig->n_cons, constraint{});
+    S.cons_of.assign(N + 1, {});
+    for (int q = 0; q < lig->n_cons; ++q) {
+        const fa_constraint& k = lig->cons[q];
+        constraint& c = S.cons[q];
+        c.id = q;
+        c.type = k.type;
+        c.bond_len = k.bond_len;
+        c.max_dist = k.max_dist;
+        const int a1 = Rn + 1 + k.atom;
+        const int a2 = k.partner_is_ligand ? Rn + 1 + k.partner : 1 + k.partner;
+        c.anum1 = a1;
+        c.anum2 = a2;
+        S.cons_of[a1].push_back(&c);
+        S.cons_of[a2].push_back(&c);
+    }
+    for (int i = 1; i <= N; ++i) {
+        S.atoms[i].ncons = (int)S.cons_of[i].size();
+        S.atoms[i].cons = S.cons_of[i].empty() ? nullptr : S.cons_of[i].data();
+    }
+    FA->constraints = S.cons.empty() ? nullptr : S.cons.data();
+    FA->num_constraints = (int)S.cons.size();
     S.num_atm.assign(N + 2, 0);
     for (int i = 1; i <= N; ++i) S.num_atm[i] = i;
     FA->num_atm = S.num_atm.data();

@@ -54,7 +54,7 @@ def test_struct_sizes_against_the_c_compiler(tmp_path):
     the header's struct."""
     import subprocess
     from flexaidds_amd import abi, hip
-    mirrors = {"fa_params": abi.Params, "fa_ligand_desc": abi.LigandDesc, "fa_receptor_desc": abi.ReceptorDesc,
+    mirrors = {"fa_params": abi.Params, "fa_ligand_desc": abi.LigandDesc, "fa_constraint": abi.Constraint, "fa_receptor_desc": abi.ReceptorDesc,
                "fa_cf_result": abi.CFResult, "fa_thermo": abi.Thermo, "fa_gene_lim": hip.GeneLim,
                "fa_ga_params": hip.GAParams, "fa_pool_params": hip.PoolParams, "fa_job": hip.Job,
                "fa_job_result": hip.JobResult}

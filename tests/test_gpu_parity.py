@@ -117,6 +117,28 @@ def test_density_matrix_against_oracle(hip, sys0, oracle, normalize):
         assert np.abs(g[f] - ref[f])[ok].max() < 1e-8, f
 
 
+def test_constraints_against_oracle(hip, oracle):
+    # covalent constraints: con term, shortened clash distance, skipped
+    # contact energy; the reference's Gaussian explodes outside its window,
+    # so the tolerance is relative for those poses
+    s = synth.make_config0(n_poses=600)
+    synth.add_constraints(s, n_receptor=3, n_ligand=2)
+    genes = s.genes[0]
+    with hip.Context() as c:
+        c.upload_system(s)
+        g = c.eval(0, genes)
+    desc = abi.Descriptors(s.receptor, s.ligands, s.params)
+    with ol.trig_mode(oracle, 1):
+        ref = ol.oracle_eval(oracle, desc, 0, genes)
+    assert np.array_equal(g["status"], ref["status"])
+    ok = ref["status"] == 0
+    assert np.sum(ref["con"][ok] < 7000.0 - 1e-6) >= 40
+    for f in ("cf", "con", "com", "wal"):
+        d = np.abs(g[f] - ref[f])[ok]
+        assert np.all(d <= np.maximum(1e-4 * np.abs(ref[f][ok]), 1e-3)), f
+    assert np.array_equal(g["rclash"], ref["rclash"])
+
+
 def test_pose_build_bit_exact(ctx0, sys0, oracle, desc0):
     g = sys0.genes[0][:200]
     _, xyz, _ = ctx0.eval_debug(0, g)

@@ -84,3 +84,27 @@ def test_strict_reference_bit_exact_density_matrix(oracle, sys0, normalize):
     plain = ol.oracle_eval(oracle, abi.Descriptors(sys0.receptor, sys0.ligands, sys0.params), 0, genes)
     ok = out["status"] == 0
     assert np.mean(out["com"][ok] != plain["com"][ok]) > 0.9
+
+
+def test_strict_reference_bit_exact_constraints(oracle):
+    # covalent constraints (ligand-receptor and ligand-ligand): the con term,
+    # the shortened clash distance and the skipped contact energy
+    s = synth.make_config0(n_poses=400)
+    cons = synth.add_constraints(s, n_receptor=3, n_ligand=2)
+    start = 1000.0 * (3 + 2 * 2)                  # KDIST per constrained ligand atom
+    desc = abi.Descriptors(s.receptor, s.ligands, s.params)
+    genes = s.genes[0][:300]
+    r_out, _, _ = ref_eval_cons(desc, genes)
+    with ol.trig_mode(oracle, 0):
+        out = ol.oracle_eval(oracle, desc, 0, genes)
+    assert np.array_equal(out["status"], r_out["status"])
+    for f in ("com", "wal", "sas", "con", "cf", "rclash"):
+        assert np.array_equal(out[f], r_out[f]), f
+    ok = out["status"] == 0
+    assert np.all(out["con"][ok] <= start)
+    # many poses bring a constrained pair into contact (con below the start)
+    assert np.sum(out["con"][ok] < start - 1e-6) >= 20
+
+
+def ref_eval_cons(desc, genes):
+    return ol.ref_eval(ref, desc, 0, genes)
