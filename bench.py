@@ -157,36 +157,53 @@ def batched_leg(args):
 
 
 def served_leg(args):
-    """The served stream, and the plain path on the same kind of requests."""
+    """The served stream, and the plain path on the same kind of requests.
+
+    Scenarios on the same resident receptors: the specified stream (Poisson
+    at --served-rate, every resident job advanced each step), the same stream
+    with at most --served-cap running jobs per receptor, and a stream at
+    --served-low-rate, below what the device scores."""
     from flexaidds_amd import hip, serve, synth
     log(f"served stream: {args.served_requests} requests at {args.served_rate}/s")
     systems = synth.make_config3(n_receptors=8, ligands_per_receptor=args.served_library)
-    reqs = synth.poisson_requests(np.random.default_rng(11), systems, args.served_rate, args.served_requests,
-                                  generations=args.served_generations)
     srv = serve.Server(systems, population=args.served_pop, max_slots=64, hbm_budget_per_pool=2 << 30)
     # warm every pool's kernels with one short job, untimed
     srv.run([{"t": 0.0, "receptor": r, "ligand": 0, "generations": 1, "seed": 1} for r in range(len(systems))],
             realtime=False)
-    got = srv.run(reqs, realtime=True)
-    summ = got.summary()
+
+    def scenario(rate, n, cap, seed):
+        for p in srv.pools:
+            p.set_max_running(cap)
+        reqs = synth.poisson_requests(np.random.default_rng(seed), systems, rate, n,
+                                      generations=args.served_generations)
+        got = srv.run(reqs, realtime=True).summary()
+        got.update({"rate_hz": rate, "max_running_per_receptor": cap,
+                    "offered_poses_per_s": round(rate * args.served_pop * (args.served_generations + 1), 1)})
+        return got, reqs
+
+    main_s, reqs = scenario(args.served_rate, args.served_requests, 0, 11)
+    scen = [main_s]
+    if args.served_cap > 0:
+        scen.append(scenario(args.served_rate, args.served_requests, args.served_cap, 11)[0])
+    if args.served_low_rate > 0:
+        scen.append(scenario(args.served_low_rate, max(8, args.served_requests // 2), 0, 12)[0])
     # plain path: one request at a time, one synchronous launch per generation
-    plain_gens, plain_ms, plain_poses = 0, 0.0, 0
+    plain_ms, plain_poses = 0.0, 0
     for r in reqs[:2]:
         c = srv.ctx[r["receptor"]]
         ga = hip.GA(c, srv.limits[r["receptor"]][r["ligand"]], ligand=r["ligand"], population=args.served_pop,
                     offspring=args.served_pop, seed=r["seed"])
         for _ in range(10):
             plain_ms += ga.run(1, graph=False)
-            plain_gens += 1
             plain_poses += args.served_pop
         ga.close()
     srv.close()
     plain = plain_poses / (plain_ms / 1e3)
-    summ.update({"receptors": len(systems), "rate_hz": args.served_rate, "population": args.served_pop,
-                 "generations": args.served_generations,
-                 "offered_poses_per_s": round(args.served_rate * args.served_pop * (args.served_generations + 1), 1),
-                 "plain_poses_per_s": round(plain, 1), "vs_plain": round(summ["poses_per_s"] / plain, 3)})
-    return summ
+    out = dict(main_s)
+    out.update({"receptors": len(systems), "population": args.served_pop, "generations": args.served_generations,
+                "plain_poses_per_s": round(plain, 1), "vs_plain": round(main_s["poses_per_s"] / plain, 3),
+                "scenarios": scen})
+    return out
 
 
 def main():
@@ -204,6 +221,8 @@ def main():
     ap.add_argument("--served-library", type=int, default=6, help="resident ligands per receptor")
     ap.add_argument("--served-generations", type=int, default=100)
     ap.add_argument("--served-pop", type=int, default=1024)
+    ap.add_argument("--served-cap", type=int, default=2, help="running jobs per receptor in the capped scenario")
+    ap.add_argument("--served-low-rate", type=float, default=6.0, help="arrival rate below capacity (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true",
                     help="only time one launch per population size (1 .. 16,384 poses of the plain workload) and exit")

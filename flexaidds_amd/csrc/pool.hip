@@ -207,6 +207,7 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
     if (p->fitness != FA_FIT_LINEAR && !(p->sig_share > 0.0) && !(p->share_peaks > 0.0 && p->share_scale > 0.0))
         return fa_fail(FA_ERR_INVALID, "sharing needs sig_share > 0 or peaks and scale > 0");
     if (p->max_slots < 1) return fa_fail(FA_ERR_INVALID, "max_slots >= 1");
+    if (p->max_running < 0) return fa_fail(FA_ERR_INVALID, "max_running >= 0");
     int Gmax = p->max_genes;
     if (Gmax <= 0)
         for (int g : c->lig_genes) Gmax = std::max(Gmax, g);
@@ -347,9 +348,14 @@ int fa_pool_step(fa_pool* pl, int* n_running, int* n_queued)
     int n_run = 0, n_adm = 0, n_done = 0;
     for (int k = 0; k < S; ++k)
         if (pl->slot[k].busy && pl->slot[k].ready) run[n_run++] = k;
-    // admission: queued jobs into free slots, in arrival order
-    for (int k = 0; k < S && !pl->queue.empty(); ++k) {
+    // admission: queued jobs into free slots, in arrival order, while the
+    // resident count stays within max_running
+    int resident = 0;
+    for (int k = 0; k < S; ++k) resident += pl->slot[k].busy;
+    const int cap = (p.max_running > 0) ? std::min(p.max_running, S) : S;
+    for (int k = 0; k < S && !pl->queue.empty() && resident < cap; ++k) {
         if (pl->slot[k].busy) continue;
+        ++resident;
         PoolJob j = std::move(pl->queue.front());
         pl->queue.pop_front();
         PoolSlot& sl = pl->slot[k];
@@ -494,6 +500,13 @@ int fa_pool_population(fa_pool* pl, int64_t job_id, int32_t* genes, double* ic, 
     if (ic) FA_HIP(hipMemcpy(ic, pl->d.gic + (size_t)k * pop * Gm, sizeof(double) * pop * Gm, hipMemcpyDeviceToHost));
     if (results)
         FA_HIP(hipMemcpy(results, pl->d.res + (size_t)k * pop, sizeof(fa_cf_result) * pop, hipMemcpyDeviceToHost));
+    return FA_SUCCESS;
+}
+
+int fa_pool_set_max_running(fa_pool* pl, int max_running)
+{
+    if (!pl || max_running < 0) return fa_fail(FA_ERR_INVALID, "bad pool or max_running");
+    pl->p.max_running = max_running;
     return FA_SUCCESS;
 }
 

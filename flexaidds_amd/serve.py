@@ -41,7 +41,7 @@ class Server:
     """Resident receptors with one docking pool each."""
 
     def __init__(self, systems, population=1024, offspring=None, max_slots=64, hbm_budget_per_pool=4 << 30,
-                 temperature=300.0, fitness="LINEAR", device=0):
+                 temperature=300.0, fitness="LINEAR", device=0, max_running=0):
         self.systems = systems
         self.ctx, self.pools, self.limits = [], [], []
         for s in systems:
@@ -49,7 +49,8 @@ class Server:
             c.upload_system(s)
             self.ctx.append(c)
             self.pools.append(hip.Pool(c, population=population, offspring=offspring, max_slots=max_slots,
-                                       hbm_budget=hbm_budget_per_pool, temperature=temperature, fitness=fitness))
+                                       hbm_budget=hbm_budget_per_pool, temperature=temperature, fitness=fitness,
+                                       max_running=max_running))
             self.limits.append([synth.ga_limits(l) for l in s.ligands])
 
     def close(self):

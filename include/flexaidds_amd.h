@@ -251,6 +251,9 @@ typedef struct fa_pool_params {
     int32_t max_slots;         /* upper bound on resident jobs */
     int32_t max_genes;         /* row width; 0 = the largest gene count of the context's ligands */
     uint64_t hbm_budget;       /* bytes of job state; slots = min(max_slots, budget / slot bytes) */
+    int32_t max_running;       /* at most this many resident jobs (0: every slot); a smaller
+                                  batch per step finishes each job sooner */
+    int32_t pad;
 } fa_pool_params;
 
 typedef struct fa_job {
@@ -288,6 +291,8 @@ int fa_pool_step(fa_pool* pool, int* n_running, int* n_queued);
 int fa_pool_collect(fa_pool* pool, fa_job_result* out, int max, int* n);
 /* Survivors of a resident job (population rows of max_genes values). */
 int fa_pool_population(fa_pool* pool, int64_t job_id, int32_t* genes, double* ic, fa_cf_result* results);
+/* Changes the cap on resident jobs for later admissions (0: every slot). */
+int fa_pool_set_max_running(fa_pool* pool, int max_running);
 int fa_pool_destroy(fa_pool* pool);
 
 /* Size of the kernel's per-wave LDS workspace (bytes). */

@@ -125,6 +125,29 @@ def test_admission_bounded_by_slots(hip, ctx, limits):
     assert sorted(r["id"] for r in seen) == ids
 
 
+def test_running_cap(hip, ctx, limits):
+    # at most max_running jobs resident; the rest wait in arrival order and
+    # every result is still the one the job gets alone
+    pool = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=4, max_running=2)
+    ids = [pool.submit(k % 5, limits[k % 5], 2 + k % 3, 40 + k) for k in range(5)]
+    seen = []
+    for _ in range(100):
+        running, queued = pool.step()
+        assert running <= 2
+        seen += pool.collect()
+        if running == 0 and queued == 0:
+            break
+    pool.close()
+    assert sorted(r["id"] for r in seen) == ids
+    for r in seen:
+        k = ids.index(r["id"])
+        alone = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=1)
+        alone.submit(k % 5, limits[k % 5], 2 + k % 3, 40 + k)
+        ref = _drain(alone)
+        alone.close()
+        _same(r, ref[0])
+
+
 def test_hbm_budget_sets_slots(hip, ctx):
     probe = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=1)
     per = probe.slot_bytes
