@@ -105,6 +105,10 @@ __device__ __forceinline__ int lanes_below(unsigned long long m)
 
 __device__ __forceinline__ int first_lane(unsigned long long m) { return __ffsll((long long)m) - 1; }
 
+// a wave-uniform value moved to a scalar register, so that branches on it
+// compile to scalar branches instead of exec-mask save/restore sequences
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v)
 {
@@ -559,6 +563,8 @@ __device__ __forceinline__ void edge_get(const Edges& E, int e, unsigned& pl, un
 // caller passes the previous one as `from`.
 __device__ __forceinline__ int first_open(const Edges& E, int from, int edgenum)
 {
+    from = uni(from);
+    edgenum = uni(edgenum);
     const int l = lane_id();
 #pragma unroll
     for (int k = 0; k < ESL; ++k) {
@@ -574,6 +580,10 @@ __device__ __forceinline__ int first_open(const Edges& E, int from, int edgenum)
 // first edge e in (lo, hi) between planes P and C (either order), -1 if none
 __device__ __forceinline__ int find_edge(const Edges& E, int lo, int hi, int P, int C)
 {
+    lo = uni(lo);
+    hi = uni(hi);
+    P = uni(P);
+    C = uni(C);
     const int l = lane_id();
     const unsigned pc = (unsigned)P | ((unsigned)C << 8), cp = (unsigned)C | ((unsigned)P << 8);
 #pragma unroll
@@ -631,11 +641,13 @@ __device__ __forceinline__ HullProbe hull_search(const Smem& s, const Edges& E, 
 {
     HullProbe h;
     unsigned epl, evt;
+    e = uni(e);
+    NC = uni(NC);
     edge_get(E, e, epl, evt);
-    h.A = (int)(epl & NONE);
-    h.B = (int)((epl >> 8) & NONE);
-    h.oldC = (int)((epl >> 16) & NONE);
-    const int sv = (int)(evt & 0xFFFFu);
+    h.A = uni((int)(epl & NONE));
+    h.B = uni((int)((epl >> 8) & NONE));
+    h.oldC = uni((int)((epl >> 16) & NONE));
+    const int sv = uni((int)(evt & 0xFFFFu));
     h.px = s.vx[sv];
     h.py = s.vy[sv];
     h.pz = s.vz[sv];
@@ -648,14 +660,14 @@ __device__ __forceinline__ HullProbe hull_search(const Smem& s, const Edges& E, 
     for (int q = 0; q < PSL; ++q) {
         kq[q] = ~0ull;
         if (q * WAVE < NC) {  // wave-uniform: skip empty slots
+            // every lane computes its quotient (no divergent branches); lanes
+            // without a candidate plane, a zero divisor (the quotient is then
+            // inf or NaN) or a cut behind the point keep the empty key
             const int c = l + WAVE * q;
-            if (c < NC && c != h.A && c != h.B && c != h.oldC) {
-                const double vtdiv = pa0[q] * h.dx + pa1[q] * h.dy + pa2[q] * h.dz;
-                if (vtdiv != 0.0) {
-                    const double vt = -(pa0[q] * h.px + pa1[q] * h.py + pa2[q] * h.pz + pa3[q]) / vtdiv;
-                    if (vt < BIG && vt > 0.0) kq[q] = key_of(vt);
-                }
-            }
+            const double vtdiv = pa0[q] * h.dx + pa1[q] * h.dy + pa2[q] * h.dz;
+            const double vt = -(pa0[q] * h.px + pa1[q] * h.py + pa2[q] * h.pz + pa3[q]) / vtdiv;
+            const bool ok = c < NC && c != h.A && c != h.B && c != h.oldC && vtdiv != 0.0 && vt < BIG && vt > 0.0;
+            kq[q] = ok ? key_of(vt) : ~0ull;
             kb = umin64(kb, kq[q]);
         }
     }
@@ -676,6 +688,7 @@ __device__ __forceinline__ HullProbe hull_search(const Smem& s, const Edges& E, 
 
 __device__ __forceinline__ HullStep hull_finish(const HullProbe& h, int NC, double rr)
 {
+    NC = uni(NC);
     HullStep w;
     w.A = h.A;
     w.B = h.B;
