@@ -999,31 +999,26 @@ __device__ __forceinline__ double inv12(double x)
 // range (sq returns the squared distance as the reference computes it).
 __device__ __forceinline__ bool cand_test(const CullCtx& cc, const Cand& c, CullAcc& acc, float& sq)
 {
-    bool in = false;
-    sq = 0.0f;
-    if (c.j >= 0 && c.j != cc.acalc) {
-        acc.total += 1;
-        const float dx = cc.ax - c.x, dy = cc.ay - c.y, dz = cc.az - c.z;
-        sq = dx * dx + dy * dy + dz * dz;
-        const double nd = (double)(cc.rado + c.r + RW);
-        in = (double)sq < nd * nd && sq != 0.0f;
-        // LIB/Vcontacts.cpp:36-51 (pre-pass): a ligand atom counted once by
-        // the earlier atom in box order, i.e. skipped once it is marked
-        const bool pskip = c.lj && bit_of(cc.dpre0, cc.dpre1, c.m);
-        if (!pskip) {
-            acc.tested += 1;
-            if (in) {
-                acc.found += 1;
-                const double dist = sqrt((double)sq);
-                const double rAB = (double)(cc.ra + c.r);
-                const double cdist = cc.permea * rAB;
-                if (dist < cdist) {
-                    const bool bonded = c.lj && bit_of(cc.brow0, cc.brow1, c.m);
-                    if (!bonded) acc.clash += (double)KWALL_F * (inv12(dist) - inv12(cdist));
-                }
-            }
-        }
-    }
+    // the tests are evaluated on every lane and combined (no nested
+    // divergent branches); only the rare clash term branches
+    const bool valid = c.j >= 0 && c.j != cc.acalc;
+    const float dx = cc.ax - c.x, dy = cc.ay - c.y, dz = cc.az - c.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const double nd = (double)(cc.rado + c.r + RW);
+    const bool in = valid && (double)d2 < nd * nd && d2 != 0.0f;
+    // LIB/Vcontacts.cpp:36-51 (pre-pass): a ligand atom counted once by
+    // the earlier atom in box order, i.e. skipped once it is marked
+    const bool pskip = c.lj && bit_of(cc.dpre0, cc.dpre1, c.m);
+    const bool tested = valid && !pskip;
+    const bool found = tested && in;
+    acc.total += valid ? 1 : 0;
+    acc.tested += tested ? 1 : 0;
+    acc.found += found ? 1 : 0;
+    sq = valid ? d2 : 0.0f;
+    const double dist = sqrt((double)d2);
+    const double cdist = cc.permea * (double)(cc.ra + c.r);
+    const bool bonded = c.lj && bit_of(cc.brow0, cc.brow1, c.m);
+    if (found && dist < cdist && !bonded) acc.clash += (double)KWALL_F * (inv12(dist) - inv12(cdist));
     return in;
 }
 
@@ -1500,6 +1495,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     const int tot_k = readlane_i(k < WAVE ? totq[0] : totq[1], k & 63);
                     count_op(s.ops, OP_CULL, tot_k - wave_sum_i(sk));
                 }
+                NC = uni(NC);
                 if (NC > MAXC) {
                     status = FA_CAPACITY;
                     break;
@@ -1746,6 +1742,8 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     status = FA_HULL_FAIL;
                     break;
                 }
+                edgenum = uni(edgenum);
+                vn = uni(vn);
                 wave_sync();
                 clk.stop(PH_HULL);
 
@@ -1855,7 +1853,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     pfl[q] = (c < NC) ? s.pflag[c] : 'X';
                 }
                 clk.stop(PH_ARC);
-                const int NV = vn;
+                const int NV = uni(vn);
 
                 // -- seeds of ligand atoms not seeded yet (save_seeds, LIB/Vcontacts.cpp:595-629):
                 // the first hull vertex (in order) with a plane of that atom
