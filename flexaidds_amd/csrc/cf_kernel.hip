@@ -31,6 +31,7 @@
 // The LDS working set is ~19 KB, so eight waves share a CU.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <stdint.h>
 
 #include "fa_device.h"
@@ -262,6 +263,7 @@ __device__ __forceinline__ void count_op(unsigned long long* ops, int k, long lo
 
 // optional per-phase cycle accounting (diagnostic launches only)
 enum Phase { PH_BUILD = 0, PH_PREPASS, PH_CULL, PH_FIRST, PH_HULL, PH_ARC, PH_SEED, PH_FACE, PH_AREA, PH_REC, PH_N };
+constexpr int PROF_POSE0 = 16;  // per-pose cycles follow the 16 phase slots
 
 struct PhaseClock {
     unsigned long long* acc;  // null when disabled
@@ -1044,6 +1046,29 @@ __device__ __forceinline__ void visit(const CullCtx& cc, const Cand& c, CullAcc&
     cand_emit(cc, c, in, sq, acc);
 }
 
+// Cells of the ligand atoms by rank, which are nondecreasing: ranks l and
+// l + 64 on lane l (past the last rank: INT_MAX).
+struct RankCells {
+    int b0, b1;
+};
+
+__device__ __forceinline__ RankCells rank_cells(const Smem& s, int L)
+{
+    const int l = lane_id();
+    return RankCells{l < L ? s.sbox[l] : INT_MAX, l + WAVE < L ? s.sbox[l + WAVE] : INT_MAX};
+}
+
+// ligand atoms of the wave-uniform cell sb: ranks [lo, lo + nl)
+__device__ __forceinline__ void cell_span(const RankCells& rc, int L, int sb, int& lo, int& nl)
+{
+    lo = __popcll(__ballot(rc.b0 < sb));
+    nl = __popcll(__ballot(rc.b0 == sb));
+    if (L > WAVE) {
+        lo += __popcll(__ballot(rc.b1 < sb));
+        nl += __popcll(__ballot(rc.b1 == sb));
+    }
+}
+
 // Segment table of one atom's stream (lanes 0..26 hold slot 0..26).
 struct SlotTab {
     int bi;       // cell of the slot, -1 if outside the grid
@@ -1071,12 +1096,18 @@ __device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig,
 {
     const int l = lane_id();
     int lo = 0, nl = 0;  // ligand atoms of the slot's cell: ranks [lo, lo + nl)
-    if (tab.bi >= 0)
-        for (int q = 0; q < L; ++q) {
-            const int b = s.sbox[q];
-            lo += b < tab.bi;
-            nl += b == tab.bi;
+    {
+        const RankCells rc = rank_cells(s, L);
+        for (unsigned long long m = __ballot(tab.bi >= 0); m; m &= m - 1) {
+            const int q = first_lane(m);
+            int a, n;
+            cell_span(rc, L, readlane_i(tab.bi, q), a, n);
+            if (l == q) {
+                lo = a;
+                nl = n;
+            }
         }
+    }
     const int nr = (tab.bi >= 0) ? tab.re - tab.rs : 0;
     const int seg = nr + nl;
     const int off = wave_excl_scan(seg);
@@ -1145,15 +1176,11 @@ __device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig,
 }
 
 // Ligand atoms of cell sb (ranks [lo, lo + nl) of the box order), visited.
-__device__ __forceinline__ void visit_ligand_cell(const Smem& s, const DevLigand& lig, int Rn, int L, int sb,
-                                                  const CullCtx& cc, CullAcc& acc)
+__device__ __forceinline__ void visit_ligand_cell(const Smem& s, const DevLigand& lig, int Rn, int L,
+                                                  const RankCells& rc, int sb, const CullCtx& cc, CullAcc& acc)
 {
-    int lo = 0, nl = 0;
-    for (int q = 0; q < L; ++q) {
-        const int b = s.sbox[q];
-        lo += b < sb;
-        nl += b == sb;
-    }
+    int lo, nl;
+    cell_span(rc, L, sb, lo, nl);
     for (int b = 0; b < nl; b += WAVE) {
         const int r = b + lane_id();
         Cand c{0.f, 0.f, 0.f, 0.f, -1, 0, false};
@@ -1186,35 +1213,69 @@ __device__ __forceinline__ int slot_of(const PoseGrid& pg, int d)
 
 constexpr int SLOW_TMP = 3 * WAVE;  // in-range receptor atoms of one ligand atom (slow path)
 
-// Grid widened by this pose: the receptor cells are not precomputed.  One
-// streamed pass bins every receptor atom on the fly and tests the ones in
-// the 27 cells (their in-range entries parked, with their slot, at the tail
-// of the wave's neighbour buffer); a second pass emits them slot by slot,
-// each slot's ligand atoms after its receptor atoms, as the reference scans.
+// Per-wave slow-path scratch: the cell of every receptor atom in the pose's
+// widened grid, then the receptor atoms of one ligand atom's 27 cells.
+__device__ __forceinline__ size_t slow_rows(int Rn) { return (size_t)((Rn + 3) & ~3); }
+
+// Grid widened by this pose: the receptor cells are not precomputed.  The
+// cells of the receptor atoms (computed once per pose) are streamed four per
+// lane and the atoms in the 27 cells gathered in index order with their slot;
+// those are tested (the in-range entries parked, with their slot, at the
+// tail of the wave's neighbour buffer) and a last pass emits them slot by
+// slot, each slot's ligand atoms after its receptor atoms, as the reference
+// scans.
 __device__ void cull_slow(const Smem& s, const DevReceptor& R, const DevLigand& lig, const PoseGrid& pg, int L,
-                          const SlotTab& tab, int bz, const CullCtx& cc, CullAcc& acc)
+                          const SlotTab& tab, int bz, const CullCtx& cc, CullAcc& acc, const int* rbox, int* rcand)
 {
     const int l = lane_id();
     const int Rn = R.n;
+    const int Rq = (int)(slow_rows(Rn) >> 2);
+    const int4* const rbox4 = reinterpret_cast<const int4*>(rbox);
+    int ncand = 0;
+    int4 nxt = make_int4(-1, -1, -1, -1);
+    if (l < Rq) nxt = rbox4[l];
+    for (int base = 0; base < Rq; base += WAVE) {
+        const int4 bx = nxt;
+        if (base + WAVE < Rq) nxt = (base + WAVE + l < Rq) ? rbox4[base + WAVE + l] : make_int4(-1, -1, -1, -1);
+        const int t0 = 4 * (base + l);
+        // slots of the lane's four atoms (cells of rows past Rn are never read)
+        const int s0 = (t0 < Rn) ? slot_of(pg, bx.x - bz) : -1;
+        const int s1 = (t0 + 1 < Rn) ? slot_of(pg, bx.y - bz) : -1;
+        const int s2 = (t0 + 2 < Rn) ? slot_of(pg, bx.z - bz) : -1;
+        const int s3 = (t0 + 3 < Rn) ? slot_of(pg, bx.w - bz) : -1;
+        const int cnt = (s0 >= 0) + (s1 >= 0) + (s2 >= 0) + (s3 >= 0);
+        if (__ballot(cnt > 0) == 0ull) continue;
+        int pos = ncand + wave_excl_scan(cnt);
+        if (s0 >= 0) rcand[pos++] = t0 | (s0 << 26);
+        if (s1 >= 0) rcand[pos++] = (t0 + 1) | (s1 << 26);
+        if (s2 >= 0) rcand[pos++] = (t0 + 2) | (s2 << 26);
+        if (s3 >= 0) rcand[pos++] = (t0 + 3) | (s3 << 26);
+        ncand += wave_sum_i(cnt);
+    }
+    wave_sync();  // gathered atoms visible to every lane
     int2* const tidx = cc.nidx + cc.cap;   // SLOW_TMP entries past the list capacity
     float4* const txyz = cc.nxyz + cc.cap;
     int ntmp = 0;
-    float4 nxt = (l < Rn) ? R.xyzr[l] : make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int base = 0; base < Rn; base += WAVE) {
-        const int t = base + l;
-        const float4 p = nxt;
-        if (base + WAVE < Rn) nxt = (t + WAVE < Rn) ? R.xyzr[t + WAVE] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const int slot = (t < Rn) ? slot_of(pg, box_of(pg, p.x, p.y, p.z) - bz) : -1;
-        if (__ballot(slot >= 0) == 0ull) continue;
-        Cand c{p.x, p.y, p.z, p.w, slot >= 0 ? t : -1, 0, false};
+    // the gathered atoms in chunks, the next chunk's loads in flight
+    int v = (l < ncand) ? rcand[l] : -1;
+    float4 p = (v >= 0) ? R.xyzr[v & ((1 << 26) - 1)] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int base = 0; base < ncand; base += WAVE) {
+        const int cv = v;
+        const float4 cp = p;
+        if (base + WAVE < ncand) {
+            v = (base + WAVE + l < ncand) ? rcand[base + WAVE + l] : -1;
+            p = (v >= 0) ? R.xyzr[v & ((1 << 26) - 1)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const int t = (cv >= 0) ? (cv & ((1 << 26) - 1)) : -1;
+        Cand c{cp.x, cp.y, cp.z, cp.w, t, 0, false};
         float sq;
         const bool in = cand_test(cc, c, acc, sq);
         const unsigned long long mk = __ballot(in);
         if (in) {
-            const int pos = ntmp + lanes_below(mk);
-            if (pos < SLOW_TMP) {
-                tidx[pos] = make_int2(t | (slot << 26), __float_as_int(sq));
-                txyz[pos] = p;
+            const int q = ntmp + lanes_below(mk);
+            if (q < SLOW_TMP) {
+                tidx[q] = make_int2(cv, __float_as_int(sq));
+                txyz[q] = cp;
             }
         }
         ntmp += __popcll(mk);
@@ -1240,6 +1301,7 @@ __device__ void cull_slow(const Smem& s, const DevReceptor& R, const DevLigand& 
             ti[q].x &= (1 << 26) - 1;
         }
     }
+    const RankCells rc = rank_cells(s, L);
     for (int slot = 0; slot < 27; ++slot) {
         const int sb = readlane_i(tab.bi, slot);
         if (sb < 0) continue;
@@ -1250,7 +1312,7 @@ __device__ void cull_slow(const Smem& s, const DevReceptor& R, const DevLigand& 
             const Cand c{tx[q].x, tx[q].y, tx[q].z, tx[q].w, ti[q].x, 0, false};
             cand_emit(cc, c, mine, __int_as_float(ti[q].y), acc);
         }
-        visit_ligand_cell(s, lig, Rn, L, sb, cc, acc);
+        visit_ligand_cell(s, lig, Rn, L, rc, sb, cc, acc);
     }
 }
 
@@ -1261,6 +1323,7 @@ __device__ void cull_slow_coarse(const Smem& s, const DevReceptor& R, const DevL
 {
     const int l = lane_id();
     const int Rn = R.n;
+    const RankCells rc = rank_cells(s, L);
     for (int slot = 0; slot < 27; ++slot) {
         const int sb = readlane_i(tab.bi, slot);
         if (sb < 0) continue;
@@ -1276,7 +1339,7 @@ __device__ void cull_slow_coarse(const Smem& s, const DevReceptor& R, const DevL
             if (__ballot(v) == 0ull) continue;
             visit(cc, c, acc);
         }
-        visit_ligand_cell(s, lig, Rn, L, sb, cc, acc);
+        visit_ligand_cell(s, lig, Rn, L, rc, sb, cc, acc);
     }
 }
 
@@ -1309,6 +1372,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
         if (l < FA_OPCOUNT_N) s.ops[l] = 0ull;
         PhaseClock clk{B.prof, 0ull};
         clk.start();
+        const unsigned long long pose_t0 = B.prof ? __builtin_readcyclecounter() : 0ull;
         build_pose(s, lig, B.genes + (size_t)pose * B.gene_stride);
         count_op(s.ops, OP_BUILD, L);
         if (B.lig_xyz) {
@@ -1367,6 +1431,16 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
             unsigned long long dpre0 = 0ull, dpre1 = 0ull;
             long long pre_tested = 0;
             CullAcc acc{0, 0, 0, 0, 0.0, false};
+            // widened grid: every receptor atom's cell once for the pose
+            int* const wb = B.rbox + 2 * slow_rows(R.n) * blockIdx.x;
+            int* const rcand = wb + slow_rows(R.n);
+            if (pg.slow && pg.dim >= 3) {
+                for (int t = l; t < R.n; t += WAVE) {
+                    const float4 p = R.xyzr[t];
+                    wb[t] = box_of(pg, p.x, p.y, p.z);
+                }
+                wave_sync();
+            }
             SlotTab tab = slot_table(s, G, pg, 0);
             for (int k = 0; k < L; ++k) {
                 SlotTab next{-1, 0, 0};
@@ -1393,7 +1467,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 if (!pg.slow)
                     cull_fast(s, G, lig, Rn, L, tab, cc, acc);
                 else if (pg.dim >= 3)
-                    cull_slow(s, R, lig, pg, L, tab, s.sbox[k], cc, acc);
+                    cull_slow(s, R, lig, pg, L, tab, s.sbox[k], cc, acc, wb, rcand);
                 else
                     cull_slow_coarse(s, R, lig, pg, L, tab, cc, acc);
                 const int total = wave_sum_i(acc.total);
@@ -2445,6 +2519,9 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
         if (l == 0) {
             B.out[pose] = res;
             if (B.rec_count) B.rec_count[pose] = (status == FA_OK) ? nrec_total : 0;
+            // diagnostic launches: the pose's cycles, top bit set on a widened grid
+            if (B.prof)
+                B.prof[PROF_POSE0 + pose] = (__builtin_readcyclecounter() - pose_t0) | (pg.slow ? (1ull << 63) : 0ull);
         }
         wave_sync();
     }

@@ -140,11 +140,16 @@ int fa_launch_poses(fa_ctx* c, int n_poses, const double* d_genes, int stride, c
     FA_HIP(fa::launch_zero_u32(static_cast<unsigned int*>(c->ops.p), (int)(sizeof(fa_opcount) / 4), stream));
     B.prof = nullptr;
     if (c->profile) {
+        // 16 phase slots, then one cycle count per pose
+        int rc = fa_ensure(c->prof, sizeof(unsigned long long) * (16 + (size_t)std::max(n_poses, 1)));
+        if (rc) return rc;
+        c->prof_poses = n_poses;
         B.prof = static_cast<unsigned long long*>(c->prof.p);
         FA_HIP(fa::launch_zero_u32(static_cast<unsigned int*>(c->prof.p), 32, stream));
     }
     B.near_idx = static_cast<int2*>(c->near_idx.p);
     B.near_xyz = static_cast<float4*>(c->near_xyz.p);
+    B.rbox = static_cast<int*>(c->rbox.p);
     B.near_cap = c->near_cap;
     // one wave per resident slot; each wave loops over poses (work queue)
     FA_HIP(fa::launch_cf(c->rec, c->d_grids, c->d_ligs, c->prm, B, c->n_waves, stream));
@@ -215,7 +220,7 @@ int fa_ctx_destroy(fa_ctx* c)
     (void)hipDeviceSynchronize();
     c->free_system();
     for (DevBuf* b : {&c->genes, &c->out, &c->counters, &c->recs, &c->reccnt, &c->xyz, &c->scratch, &c->thermo,
-                      &c->ops, &c->prof, &c->near_idx, &c->near_xyz})
+                      &c->ops, &c->prof, &c->near_idx, &c->near_xyz, &c->rbox})
         if (b->p) (void)hipFree(b->p);
     delete c;
     return FA_SUCCESS;
@@ -408,7 +413,8 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
     c->n_waves = c->n_cu * per_cu;
     c->near_cap = max_l * NEAR_PER_ATOM;
     const size_t n_near = (size_t)c->n_waves * c->near_cap;
-    if ((rc = fa_ensure(c->near_idx, sizeof(int2) * n_near)) || (rc = fa_ensure(c->near_xyz, sizeof(float4) * n_near)))
+    if ((rc = fa_ensure(c->near_idx, sizeof(int2) * n_near)) || (rc = fa_ensure(c->near_xyz, sizeof(float4) * n_near)) ||
+        (rc = fa_ensure(c->rbox, sizeof(int) * (size_t)c->n_waves * 2 * (size_t)((Rn + 3) & ~3))))
         return rc;
     c->have_system = true;
     return FA_SUCCESS;
@@ -607,6 +613,17 @@ int fa_last_phase_cycles(fa_ctx* c, uint64_t* cycles16)
     FA_HIP(hipSetDevice(c->device));
     FA_HIP(hipDeviceSynchronize());
     FA_HIP(hipMemcpy(cycles16, c->prof.p, 16 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return FA_SUCCESS;
+}
+
+int fa_last_pose_cycles(fa_ctx* c, int n, uint64_t* cycles)
+{
+    if (!c || !cycles || n < 0) return fa_fail(FA_ERR_INVALID, "null argument");
+    if (!c->profile || n > c->prof_poses) return fa_fail(FA_ERR_INVALID, "no profiled launch of that many poses");
+    FA_HIP(hipSetDevice(c->device));
+    FA_HIP(hipDeviceSynchronize());
+    FA_HIP(hipMemcpy(cycles, static_cast<unsigned long long*>(c->prof.p) + 16, sizeof(uint64_t) * n,
+                     hipMemcpyDeviceToHost));
     return FA_SUCCESS;
 }
 

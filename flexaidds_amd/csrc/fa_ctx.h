@@ -67,7 +67,8 @@ struct fa_ctx {
     int n_waves = 0;
     int near_cap = 0;
     // pose buffers
-    DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops, prof, near_idx, near_xyz;
+    DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops, prof, near_idx, near_xyz, rbox;
+    int prof_poses = 0;  // poses of the last profiled launch
     bool profile = false;
     std::vector<uint32_t> last_counters = std::vector<uint32_t>(4, 0);
 

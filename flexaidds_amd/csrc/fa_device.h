@@ -95,6 +95,11 @@ struct DevBatch {
     int2* near_idx;                 // calc index, squared distance (float bits)
     float4* near_xyz;               // x, y, z, radius of the neighbour
     int near_cap;
+    // per-wave slow-path scratch: with R4 = receptor atoms rounded up to a
+    // multiple of 4, wave w owns [2 * R4 * w, 2 * R4 * (w + 1)): the cell of
+    // every receptor atom in a pose's widened grid, then the atoms gathered
+    // for one ligand atom
+    int* rbox;
 };
 
 }  // namespace fa

@@ -65,6 +65,7 @@ _SIGS = {
     "fa_upload_emat": ([_P, _P, C.c_int], C.c_int),
     "fa_set_phase_profiling": ([_P, C.c_int], C.c_int),
     "fa_last_phase_cycles": ([_P, _P], C.c_int),
+    "fa_last_pose_cycles": ([_P, C.c_int, _P], C.c_int),
     "fa_boltzmann_device": ([_P, _P, C.c_int, C.c_int64, C.c_double, _P, _P, _P], C.c_int),
     "fa_log_sum_exp_device": ([_P, _P, C.c_int, C.c_int64, _P, _P], C.c_int),
     "fa_kernel_lds_bytes": ([], C.c_int),
@@ -224,6 +225,12 @@ class Context:
         a = np.zeros(16, np.uint64)
         _check(_lib.fa_last_phase_cycles(self._ctx, a.ctypes.data))
         return {k: int(a[i]) for i, k in enumerate(self.PHASES)}
+
+    def pose_cycles(self, n: int):
+        """(cycles per pose, widened-grid flag per pose) of the last profiled launch."""
+        raw = np.zeros(n, np.uint64)
+        _check(_lib.fa_last_pose_cycles(self._ctx, n, raw.ctypes.data))
+        return (raw & np.uint64((1 << 63) - 1)).astype(np.int64), (raw >> np.uint64(63)).astype(bool)
 
     def launch_waves(self) -> int:
         """Waves one pose launch runs (resident slots on the device)."""

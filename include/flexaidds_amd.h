@@ -140,6 +140,10 @@ int fa_last_opcount(fa_ctx* ctx, fa_opcount* out);
  * counters read by fa_last_phase_cycles (synchronises the device). */
 int fa_set_phase_profiling(fa_ctx* ctx, int enable);
 int fa_last_phase_cycles(fa_ctx* ctx, uint64_t* cycles16);
+/* Per-pose clock cycles of the last profiled launch (n <= its poses); the
+ * top bit marks poses that needed a widened cell grid.  Diagnostics only:
+ * profiling grows a buffer, so it stays off under graph capture. */
+int fa_last_pose_cycles(fa_ctx* ctx, int n, uint64_t* cycles);
 
 /* ---------------------------------------------------------------------
  * Device GA (replaces the generation loop of GA(), LIB/gaboom.cpp:346-500,

@@ -232,7 +232,15 @@ def test_slow_grid_path(hip, oracle):
     c = hip.Context()
     c.upload_system(s)
     out = c.eval(0, g)
-    assert c.counters()["slow_grid"] > 0
+    n_slow = c.counters()["slow_grid"]
+    assert n_slow > 0
+    # per-pose cycle counts flag exactly the widened-grid poses
+    c.set_phase_profiling(True)
+    again = c.eval(0, g)
+    cyc, slow = c.pose_cycles(len(g))
+    c.set_phase_profiling(False)
+    assert np.array_equal(again["cf"], out["cf"])
+    assert slow.sum() == n_slow and np.all(cyc > 0)
     with ol.trig_mode(oracle, 1):
         ref = ol.oracle_eval(oracle, d, 0, g)
     c.close()
