@@ -502,8 +502,10 @@ struct Edges {
     unsigned vt[ESL];                   // start vertex | end vertex << 16
 };
 
-#define FA_EDGE_SLOTS(M) M(0) M(1) M(2) M(3)
-static_assert(ESL == 4, "FA_EDGE_SLOTS lists every edge slot");
+// The slot of an edge is wave-uniform, but every update is written as a
+// select over all slots with constant indices (no branch per slot): a
+// store or read under a switch on the slot gets merged into one access at
+// a computed index, which moves the whole array to scratch memory.
 
 // new edge along planes p0 x p1 from vertex sv, turned away from plane tp
 __device__ __forceinline__ void edge_new(Edges& E, int e, int p0, int p1, int tp, int sv)
@@ -511,52 +513,45 @@ __device__ __forceinline__ void edge_new(Edges& E, int e, int p0, int p1, int tp
     const unsigned pl = ((unsigned)p0 & NONE) | (((unsigned)p1 & NONE) << 8) | (((unsigned)tp & NONE) << 16) |
                         (NONE << 24);
     const unsigned vt = (unsigned)sv | 0xFFFF0000u;
+    e = uni(e);
     const bool me = lane_id() == (e & 63);
-    switch (e >> 6) {
-#define FA_NEW(K)                      \
-    case K:                            \
-        if (me) {                      \
-            E.pl[K] = pl;              \
-            E.vt[K] = vt;              \
-        }                              \
-        break;
-        FA_EDGE_SLOTS(FA_NEW)
-#undef FA_NEW
+    const int k = e >> 6;
+#pragma unroll
+    for (int K = 0; K < ESL; ++K) {
+        const bool hit = me && k == K;
+        E.pl[K] = hit ? pl : E.pl[K];
+        E.vt[K] = hit ? vt : E.vt[K];
     }
 }
 
 // end point and end plane of edge e; the edge leaves the open queue
 __device__ __forceinline__ void edge_close(Edges& E, int e, int end_plane, int end_vertex)
 {
+    e = uni(e);
     const bool me = lane_id() == (e & 63);
-    switch (e >> 6) {
-#define FA_CLOSE(K)                                                                   \
-    case K:                                                                           \
-        if (me) {                                                                     \
-            E.pl[K] = (E.pl[K] & 0x00FFFFFFu) | (((unsigned)end_plane & NONE) << 24); \
-            E.vt[K] = (E.vt[K] & 0xFFFFu) | ((unsigned)end_vertex << 16);             \
-        }                                                                             \
-        break;
-        FA_EDGE_SLOTS(FA_CLOSE)
-#undef FA_CLOSE
+    const int k = e >> 6;
+    const unsigned ep = ((unsigned)end_plane & NONE) << 24, ev = (unsigned)end_vertex << 16;
+#pragma unroll
+    for (int K = 0; K < ESL; ++K) {
+        const bool hit = me && k == K;
+        E.pl[K] = hit ? ((E.pl[K] & 0x00FFFFFFu) | ep) : E.pl[K];
+        E.vt[K] = hit ? ((E.vt[K] & 0xFFFFu) | ev) : E.vt[K];
     }
 }
 
 // packed planes and vertices of edge e (wave-uniform)
 __device__ __forceinline__ void edge_get(const Edges& E, int e, unsigned& pl, unsigned& vt)
 {
-    const int ln = e & 63;
-    pl = 0u;
-    vt = 0u;
-    switch (e >> 6) {
-#define FA_GET(K)                   \
-    case K:                         \
-        pl = readlane_u(E.pl[K], ln); \
-        vt = readlane_u(E.vt[K], ln); \
-        break;
-        FA_EDGE_SLOTS(FA_GET)
-#undef FA_GET
+    e = uni(e);
+    const int k = e >> 6;
+    unsigned p = E.pl[0], v = E.vt[0];
+#pragma unroll
+    for (int K = 1; K < ESL; ++K) {
+        p = (k == K) ? E.pl[K] : p;
+        v = (k == K) ? E.vt[K] : v;
     }
+    pl = readlane_u(p, e & 63);
+    vt = readlane_u(v, e & 63);
 }
 
 // Lowest-numbered open edge among edges [from, edgenum), -1 when none.  The
@@ -702,18 +697,24 @@ __device__ __forceinline__ HullStep hull_finish(const HullProbe& h, int NC, doub
     w.nz = h.pz + vtmin * h.dz;
     if (w.nx * w.nx + w.ny * w.ny + w.nz * w.nz > rr) {
         w.tests += 4 - (h.A >= NC) - (h.B >= NC) - (h.oldC >= NC);
-        for (int c = NC; c < NC + 4; ++c) {
-            if (c != h.A && c != h.B && c != h.oldC) {
-                double b[4];
-                bounding_plane(c - NC, b);
-                const double vtdiv = b[0] * h.dx + b[1] * h.dy + b[2] * h.dz;
-                if (vtdiv != 0.0) {
-                    const double vt = -(b[0] * h.px + b[1] * h.py + b[2] * h.pz + b[3]) / vtdiv;
-                    if (vt < vtmin && vt > 0.0) {
-                        vtmin = vt;
-                        w.C = c;
-                    }
-                }
+        // the four bounding planes NC + k on lanes k = 0..3 (one quotient
+        // each), then taken in plane order: the first strictly smaller cut
+        // ahead wins, as in a scan over them
+        const int l = lane_id();
+        const int c = NC + l;
+        double b[4];
+        bounding_plane(l & 3, b);
+        const double vtdiv = b[0] * h.dx + b[1] * h.dy + b[2] * h.dz;
+        const double vt = -(b[0] * h.px + b[1] * h.py + b[2] * h.pz + b[3]) / vtdiv;
+        const bool ok = l < 4 && c != h.A && c != h.B && c != h.oldC && vtdiv != 0.0 && vt > 0.0;
+        const unsigned long long okm = __ballot(ok);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!((okm >> k) & 1ull)) continue;
+            const double v = readlane_d(vt, k);
+            if (v < vtmin) {
+                vtmin = v;
+                w.C = NC + k;
             }
         }
         w.nx = h.px + vtmin * h.dx;
