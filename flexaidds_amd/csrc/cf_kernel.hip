@@ -64,6 +64,8 @@ static_assert(MAXC % WAVE == 0 && MAXE % WAVE == 0 && MAXL % WAVE == 0, "slot la
 static_assert(MAXP < (int)NONE, "plane ids are packed in 8 bits");
 static_assert(MAXE >= EDGE_LIMIT + 2, "the edge limit must be reached before the edge capacity");
 
+constexpr int N_PHASES = 10;  // phases of the optional cycle accounting
+
 struct Smem {
     // planes of the current atom: c < NC contacts, NC..NC+3 bounding planes
     double A0[MAXP], A1[MAXP], A2[MAXP], A3[MAXP];
@@ -84,6 +86,7 @@ struct Smem {
     } u;
     unsigned long long stub[2 * MAXL];      // stub[2m..2m+1]: atoms with a record of m
     unsigned long long ops[FA_OPCOUNT_N];   // operation counters of the pose
+    unsigned long long phase[N_PHASES];     // diagnostic launches: cycles per phase, this wave
     unsigned vpl[MAXV];                     // vertex planes p0 | p1 << 8 | p2 << 16
     float lx[MAXL], ly[MAXL], lz[MAXL];     // ligand coordinates (calc order)
     int sbox[MAXL];                         // cell of the atom of each rank
@@ -263,10 +266,14 @@ __device__ __forceinline__ void count_op(unsigned long long* ops, int k, long lo
 
 // optional per-phase cycle accounting (diagnostic launches only)
 enum Phase { PH_BUILD = 0, PH_PREPASS, PH_CULL, PH_FIRST, PH_HULL, PH_ARC, PH_SEED, PH_FACE, PH_AREA, PH_REC, PH_N };
+static_assert(PH_N == N_PHASES, "phase slots");
 constexpr int PROF_POSE0 = 16;  // per-pose cycles follow the 16 phase slots
 
+// Phase cycles are summed per wave in LDS (no global traffic between
+// phases, which would hold up the wave's next global load behind a
+// contended atomic) and added to the launch totals once, at the wave's exit.
 struct PhaseClock {
-    unsigned long long* acc;  // null when disabled
+    unsigned long long* acc;  // the wave's LDS slots, null when disabled
     unsigned long long t;
     __device__ void start()
     {
@@ -1353,6 +1360,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
     const int l = lane_id();
     int2* const nidx = B.near_idx + (size_t)blockIdx.x * B.near_cap;
     float4* const nxyz = B.near_xyz + (size_t)blockIdx.x * B.near_cap;
+    if (B.prof && l < N_PHASES) s.phase[l] = 0ull;
     for (;;) {
         // dynamic work queue: pose costs vary a lot (clash exits, hull sizes)
         int pose = 0;
@@ -1371,7 +1379,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
         double clash = 0.0;
 
         if (l < FA_OPCOUNT_N) s.ops[l] = 0ull;
-        PhaseClock clk{B.prof, 0ull};
+        PhaseClock clk{B.prof ? s.phase : nullptr, 0ull};
         clk.start();
         const unsigned long long pose_t0 = B.prof ? __builtin_readcyclecounter() : 0ull;
         build_pose(s, lig, B.genes + (size_t)pose * B.gene_stride);
@@ -2526,6 +2534,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
         }
         wave_sync();
     }
+    if (B.prof && l < N_PHASES) atomicAdd(&B.prof[l], s.phase[l]);
 }
 
 // host-side launcher (same translation unit as the kernel)
