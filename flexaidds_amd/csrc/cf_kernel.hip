@@ -702,7 +702,10 @@ __device__ __forceinline__ HullStep hull_finish(const HullProbe& h, int NC, doub
     w.nx = h.px + vtmin * h.dx;
     w.ny = h.py + vtmin * h.dy;
     w.nz = h.pz + vtmin * h.dz;
-    if (w.nx * w.nx + w.ny * w.ny + w.nz * w.nz > rr) {
+    // the point and cut values are the same on every lane (loaded from
+    // uniform LDS addresses), but the compiler cannot tell: the decisions
+    // go through scalar registers so that the walk stays a scalar loop
+    if (uni(w.nx * w.nx + w.ny * w.ny + w.nz * w.nz > rr ? 1 : 0)) {
         w.tests += 4 - (h.A >= NC) - (h.B >= NC) - (h.oldC >= NC);
         // the four bounding planes NC + k on lanes k = 0..3 (one quotient
         // each), then taken in plane order: the first strictly smaller cut
@@ -719,7 +722,7 @@ __device__ __forceinline__ HullStep hull_finish(const HullProbe& h, int NC, doub
         for (int k = 0; k < 4; ++k) {
             if (!((okm >> k) & 1ull)) continue;
             const double v = readlane_d(vt, k);
-            if (v < vtmin) {
+            if (uni(v < vtmin ? 1 : 0)) {
                 vtmin = v;
                 w.C = NC + k;
             }
@@ -728,6 +731,7 @@ __device__ __forceinline__ HullStep hull_finish(const HullProbe& h, int NC, doub
         w.ny = h.py + vtmin * h.dy;
         w.nz = h.pz + vtmin * h.dz;
     }
+    w.C = uni(w.C);
     return w;
 }
 
@@ -1025,10 +1029,16 @@ __device__ __forceinline__ bool cand_test(const CullCtx& cc, const Cand& c, Cull
     acc.tested += tested ? 1 : 0;
     acc.found += found ? 1 : 0;
     sq = valid ? d2 : 0.0f;
-    const double dist = sqrt((double)d2);
     const double cdist = cc.permea * (double)(cc.ra + c.r);
     const bool bonded = c.lj && bit_of(cc.brow0, cc.brow1, c.m);
-    if (found && dist < cdist && !bonded) acc.clash += (double)KWALL_F * (inv12(dist) - inv12(cdist));
+    // the distance (a square root) only where the wall can apply: the
+    // rounded root is below cdist only if d2 < cdist^2, which the widened
+    // bound keeps; most chunks have no such lane
+    const bool near = found && !bonded && (double)d2 < cdist * cdist * (1.0 + 1e-9);
+    if (__ballot(near) != 0ull) {
+        const double dist = sqrt((double)d2);
+        if (near && dist < cdist) acc.clash += (double)KWALL_F * (inv12(dist) - inv12(cdist));
+    }
     return in;
 }
 
