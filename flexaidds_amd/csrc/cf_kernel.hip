@@ -956,6 +956,19 @@ __device__ __forceinline__ int slot_box(const PoseGrid& pg, int bz, int slot)
     return bz + pg.dim2 * ((slot / 9) - 1) + pg.dim * (((slot / 3) % 3) - 1) + (slot % 3) - 1;
 }
 
+// neighbourhood slot (0..26) naming cell bz + d, -1 if none; with dim >= 3
+// at most one (dx, dy, dz) gives the offset
+__device__ __forceinline__ int slot_of(const PoseGrid& pg, int d)
+{
+    int slot = -1;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int r = d - pg.dim2 * dx - pg.dim * dy;
+            if (r >= -1 && r <= 1) slot = (dx + 1) * 9 + (dy + 1) * 3 + (r + 1);
+        }
+    return slot;
+}
+
 // how many of the 27 slots around bz name cell bz + d
 __device__ __forceinline__ int slot_multiplicity(const PoseGrid& pg, int d)
 {
@@ -1109,12 +1122,35 @@ __device__ __forceinline__ SlotTab slot_table(const Smem& s, const DevGrid& G, c
 }
 
 // receptor cells from the precomputed cell list
-__device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig, int Rn, int L, const SlotTab& tab,
-                          const CullCtx& cc, CullAcc& acc)
+__device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig, const PoseGrid& pg, int Rn, int L,
+                          const SlotTab& tab, int bz, int* span, const CullCtx& cc, CullAcc& acc)
 {
     const int l = lane_id();
     int lo = 0, nl = 0;  // ligand atoms of the slot's cell: ranks [lo, lo + nl)
-    {
+    if (pg.dim >= 3) {
+        // one pass over the ranks: with three or more cells a side a cell
+        // offset names at most one slot, and the ranks of a cell are
+        // consecutive (count and first rank per slot, LDS atomics)
+        if (l < 27) {
+            span[l] = 0;
+            span[32 + l] = INT_MAX;
+        }
+        wave_sync();
+        for (int r = l; r < L; r += WAVE) {
+            const int sl = slot_of(pg, s.sbox[r] - bz);
+            if (sl >= 0) {
+                atomicAdd(&span[sl], 1);
+                atomicMin(&span[32 + sl], r);
+            }
+        }
+        wave_sync();
+        if (l < 27 && tab.bi >= 0) {
+            nl = span[l];
+            lo = nl > 0 ? span[32 + l] : 0;
+        }
+        wave_sync();  // the slots are read before the next atom clears them
+    } else {
+        // coarse grids: a cell can be named by several slots
         const RankCells rc = rank_cells(s, L);
         for (unsigned long long m = __ballot(tab.bi >= 0); m; m &= m - 1) {
             const int q = first_lane(m);
@@ -1130,27 +1166,25 @@ __device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig,
     const int seg = nr + nl;
     const int off = wave_excl_scan(seg);
     const int T = readlane_i(off, 63);  // lanes >= 27 hold empty segments
+    // per segment (lane < 27): stream position t maps to receptor cell-list
+    // entry t + rbase while t < rend, else to ligand rank t + lbase
+    const int rbase = tab.rs - off, rend = off + nr, lbase = lo - off - nr;
     auto fetch = [&](int base) {
         Cand c{0.f, 0.f, 0.f, 0.f, -1, 0, false};
         const int t = base + l;
+        // the segment of t: the last non-empty one starting at or before t
+        // among those overlapping the chunk, then its fields by lane shuffle
         unsigned long long ov = __ballot(l < 27 && seg > 0 && off < base + WAVE && off + seg > base);
-        int so = 0, sn = 0, sr = 0, slo = 0;
+        int sidx = 0;
         while (ov) {
-            const int sidx = first_lane(ov);
+            const int q = first_lane(ov);
             ov &= ov - 1;
-            const int o_ = readlane_i(off, sidx), g_ = readlane_i(seg, sidx);
-            const int n_ = readlane_i(nr, sidx), r_ = readlane_i(tab.rs, sidx), lo_ = readlane_i(lo, sidx);
-            if (t >= o_ && t < o_ + g_) {
-                so = o_;
-                sn = n_;
-                sr = r_;
-                slo = lo_;
-            }
+            if (t >= readlane_i(off, q)) sidx = q;
         }
+        const int rb = __shfl(rbase, sidx), re = __shfl(rend, sidx), lb = __shfl(lbase, sidx);
         if (t < T) {
-            const int rr = t - so;
-            if (rr < sn) {
-                const int k = sr + rr;
+            if (t < re) {
+                const int k = t + rb;
                 const float4 p = G.cell_xyzr[k];
                 c.x = p.x;
                 c.y = p.y;
@@ -1158,7 +1192,7 @@ __device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig,
                 c.r = p.w;
                 c.j = G.cell_atom[k];
             } else {
-                const int m = s.order[slo + rr - sn];
+                const int m = s.order[t + lb];
                 c.m = m;
                 c.x = s.lx[m];
                 c.y = s.ly[m];
@@ -1214,19 +1248,6 @@ __device__ __forceinline__ void visit_ligand_cell(const Smem& s, const DevLigand
         }
         visit(cc, c, acc);
     }
-}
-
-// neighbourhood slot (0..26) naming cell bz + d, -1 if none; with dim >= 3
-// at most one (dx, dy, dz) gives the offset
-__device__ __forceinline__ int slot_of(const PoseGrid& pg, int d)
-{
-    int slot = -1;
-    for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int r = d - pg.dim2 * dx - pg.dim * dy;
-            if (r >= -1 && r <= 1) slot = (dx + 1) * 9 + (dy + 1) * 3 + (r + 1);
-        }
-    return slot;
 }
 
 constexpr int SLOW_TMP = 3 * WAVE;  // in-range receptor atoms of one ligand atom (slow path)
@@ -1484,7 +1505,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 if (l == 0) s.nstart[k] = acc.n_near;
                 acc.total = acc.tested = acc.found = 0;
                 if (!pg.slow)
-                    cull_fast(s, G, lig, Rn, L, tab, cc, acc);
+                    cull_fast(s, G, lig, pg, Rn, L, tab, s.sbox[k], s.u.c.idx, cc, acc);
                 else if (pg.dim >= 3)
                     cull_slow(s, R, lig, pg, L, tab, s.sbox[k], cc, acc, wb, rcand);
                 else
