@@ -102,6 +102,10 @@ struct Smem {
 // ------------------------------------------------------------ wave helpers
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// lane mask of a predicate (bool-typed: an int-typed predicate makes the
+// compiler turn the condition into 0/1 and compare it again first)
+__device__ __forceinline__ unsigned long long ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 __device__ __forceinline__ int lanes_below(unsigned long long m)
 {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -438,15 +442,34 @@ __device__ __forceinline__ double sph_triangle(double c0, double cq, double c1, 
     return (tp > 0.0) ? 4.0 * ro * ro * atan(sqrt(tp)) : 0.0;
 }
 
-// LIB/Vcontacts.cpp:1357-1372, sequential over planes (one lane)
-__device__ __forceinline__ bool test_point(const Smem& s, const double* p, int NC, int X)
+// test_point (LIB/Vcontacts.cpp:1357-1372) for the points t of the lanes
+// with `need` (a point of the face of plane X = lane + 64 q): a point is kept
+// unless a polyhedron plane other than X has it on its outer side.  The
+// points are taken one at a time and every plane is tested on its own lane
+// (planes c < NC in registers: lane c & 63, slot c >> 6), with the
+// reference's sums.
+__device__ __forceinline__ bool points_inside(bool need, const double* t, int q, int NC, const double* pa0,
+                                              const double* pa1, const double* pa2, const double* pa3,
+                                              const char* pfl)
 {
-    for (int cp = 0; cp < NC; ++cp) {
-        if (cp != X && s.pflag[cp] != 'X') {
-            if (s.A0[cp] * p[0] + s.A1[cp] * p[1] + s.A2[cp] * p[2] + s.A3[cp] > 0.0) return false;
+    const int l = lane_id();
+    NC = uni(NC);
+    unsigned long long kept = 0ull;
+    for (unsigned long long nm = ballot(need); nm; nm &= nm - 1) {
+        const int xl = first_lane(nm);
+        const int Xc = WAVE * q + xl;
+        const double t0 = readlane_d(t[0], xl), t1 = readlane_d(t[1], xl), t2 = readlane_d(t[2], xl);
+        bool outside = false;
+#pragma unroll
+        for (int q2 = 0; q2 < PSL; ++q2) {
+            if (q2 * WAVE >= NC) break;  // wave-uniform
+            const int c = l + WAVE * q2;
+            const bool above = pa0[q2] * t0 + pa1[q2] * t1 + pa2[q2] * t2 + pa3[q2] > 0.0;
+            outside = outside || (c < NC && c != Xc && pfl[q2] != 'X' && above);
         }
+        if (ballot(outside) == 0ull) kept |= 1ull << xl;
     }
-    return true;
+    return need && ((kept >> l) & 1ull);
 }
 
 __device__ __forceinline__ void plane_of(const Smem& s, int c, double* a)
@@ -575,7 +598,7 @@ __device__ __forceinline__ int first_open(const Edges& E, int from, int edgenum)
         if (k * 64 >= edgenum) break;
         if ((k + 1) * 64 <= from) continue;
         const int e = k * 64 + l;
-        const unsigned long long m = __ballot(e >= from && e < edgenum && (E.pl[k] >> 24) == NONE);
+        const unsigned long long m = ballot(e >= from && e < edgenum && (E.pl[k] >> 24) == NONE);
         if (m) return k * 64 + first_lane(m);
     }
     return -1;
@@ -596,7 +619,7 @@ __device__ __forceinline__ int find_edge(const Edges& E, int lo, int hi, int P, 
         const int e = k * 64 + l;
         const unsigned both = E.pl[k] & 0xFFFFu;  // plane0 | plane1 << 8
         const bool hit = e > lo && e < hi && (both == pc || both == cp);
-        const unsigned long long m = __ballot(hit);
+        const unsigned long long m = ballot(hit);
         if (m) return k * 64 + first_lane(m);
     }
     return -1;
@@ -683,7 +706,7 @@ __device__ __forceinline__ HullProbe hull_search(const Smem& s, const Edges& E, 
         h.vtmin = __longlong_as_double((long long)kmin);
 #pragma unroll
         for (int q = 0; q < PSL; ++q) {
-            const unsigned long long m = __ballot(kq[q] == kmin);
+            const unsigned long long m = ballot(kq[q] == kmin);
             if (h.C < 0 && m) h.C = WAVE * q + first_lane(m);
         }
     }
@@ -717,7 +740,7 @@ __device__ __forceinline__ HullStep hull_finish(const HullProbe& h, int NC, doub
         const double vtdiv = b[0] * h.dx + b[1] * h.dy + b[2] * h.dz;
         const double vt = -(b[0] * h.px + b[1] * h.py + b[2] * h.pz + b[3]) / vtdiv;
         const bool ok = l < 4 && c != h.A && c != h.B && c != h.oldC && vtdiv != 0.0 && vt > 0.0;
-        const unsigned long long okm = __ballot(ok);
+        const unsigned long long okm = ballot(ok);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (!((okm >> k) & 1ull)) continue;
@@ -1048,7 +1071,7 @@ __device__ __forceinline__ bool cand_test(const CullCtx& cc, const Cand& c, Cull
     // rounded root is below cdist only if d2 < cdist^2, which the widened
     // bound keeps; most chunks have no such lane
     const bool near = found && !bonded && (double)d2 < cdist * cdist * (1.0 + 1e-9);
-    if (__ballot(near) != 0ull) {
+    if (ballot(near) != 0ull) {
         const double dist = sqrt((double)d2);
         if (near && dist < cdist) acc.clash += (double)KWALL_F * (inv12(dist) - inv12(cdist));
     }
@@ -1058,7 +1081,7 @@ __device__ __forceinline__ bool cand_test(const CullCtx& cc, const Cand& c, Cull
 // Appends the lanes with `in` set to the neighbour list, in lane order.
 __device__ __forceinline__ void cand_emit(const CullCtx& cc, const Cand& c, bool in, float sq, CullAcc& acc)
 {
-    const unsigned long long mk = __ballot(in);
+    const unsigned long long mk = ballot(in);
     if (in) {
         const int pos = acc.n_near + lanes_below(mk);
         if (pos < cc.cap) {
@@ -1092,11 +1115,11 @@ __device__ __forceinline__ RankCells rank_cells(const Smem& s, int L)
 // ligand atoms of the wave-uniform cell sb: ranks [lo, lo + nl)
 __device__ __forceinline__ void cell_span(const RankCells& rc, int L, int sb, int& lo, int& nl)
 {
-    lo = __popcll(__ballot(rc.b0 < sb));
-    nl = __popcll(__ballot(rc.b0 == sb));
+    lo = __popcll(ballot(rc.b0 < sb));
+    nl = __popcll(ballot(rc.b0 == sb));
     if (L > WAVE) {
-        lo += __popcll(__ballot(rc.b1 < sb));
-        nl += __popcll(__ballot(rc.b1 == sb));
+        lo += __popcll(ballot(rc.b1 < sb));
+        nl += __popcll(ballot(rc.b1 == sb));
     }
 }
 
@@ -1152,7 +1175,7 @@ __device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig,
     } else {
         // coarse grids: a cell can be named by several slots
         const RankCells rc = rank_cells(s, L);
-        for (unsigned long long m = __ballot(tab.bi >= 0); m; m &= m - 1) {
+        for (unsigned long long m = ballot(tab.bi >= 0); m; m &= m - 1) {
             const int q = first_lane(m);
             int a, n;
             cell_span(rc, L, readlane_i(tab.bi, q), a, n);
@@ -1174,7 +1197,7 @@ __device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig,
         const int t = base + l;
         // the segment of t: the last non-empty one starting at or before t
         // among those overlapping the chunk, then its fields by lane shuffle
-        unsigned long long ov = __ballot(l < 27 && seg > 0 && off < base + WAVE && off + seg > base);
+        unsigned long long ov = ballot(l < 27 && seg > 0 && off < base + WAVE && off + seg > base);
         int sidx = 0;
         while (ov) {
             const int q = first_lane(ov);
@@ -1283,7 +1306,7 @@ __device__ void cull_slow(const Smem& s, const DevReceptor& R, const DevLigand& 
         const int s2 = (t0 + 2 < Rn) ? slot_of(pg, bx.z - bz) : -1;
         const int s3 = (t0 + 3 < Rn) ? slot_of(pg, bx.w - bz) : -1;
         const int cnt = (s0 >= 0) + (s1 >= 0) + (s2 >= 0) + (s3 >= 0);
-        if (__ballot(cnt > 0) == 0ull) continue;
+        if (ballot(cnt > 0) == 0ull) continue;
         int pos = ncand + wave_excl_scan(cnt);
         if (s0 >= 0) rcand[pos++] = t0 | (s0 << 26);
         if (s1 >= 0) rcand[pos++] = (t0 + 1) | (s1 << 26);
@@ -1309,7 +1332,7 @@ __device__ void cull_slow(const Smem& s, const DevReceptor& R, const DevLigand& 
         Cand c{cp.x, cp.y, cp.z, cp.w, t, 0, false};
         float sq;
         const bool in = cand_test(cc, c, acc, sq);
-        const unsigned long long mk = __ballot(in);
+        const unsigned long long mk = ballot(in);
         if (in) {
             const int q = ntmp + lanes_below(mk);
             if (q < SLOW_TMP) {
@@ -1347,7 +1370,7 @@ __device__ void cull_slow(const Smem& s, const DevReceptor& R, const DevLigand& 
 #pragma unroll
         for (int q = 0; q < SLOW_TMP / WAVE; ++q) {
             const bool mine = tslot[q] == slot;
-            if (__ballot(mine) == 0ull) continue;
+            if (ballot(mine) == 0ull) continue;
             const Cand c{tx[q].x, tx[q].y, tx[q].z, tx[q].w, ti[q].x, 0, false};
             cand_emit(cc, c, mine, __int_as_float(ti[q].y), acc);
         }
@@ -1375,7 +1398,7 @@ __device__ void cull_slow_coarse(const Smem& s, const DevReceptor& R, const DevL
                 v = box_of(pg, p.x, p.y, p.z) == sb;
                 if (v) c = Cand{p.x, p.y, p.z, p.w, t, 0, false};
             }
-            if (__ballot(v) == 0ull) continue;
+            if (ballot(v) == 0ull) continue;
             visit(cc, c, acc);
         }
         visit_ligand_cell(s, lig, Rn, L, rc, sb, cc, acc);
@@ -1438,7 +1461,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 if (lb[q] < 0 || lb[q] >= pg.dim3) bad = true;
             }
         }
-        if (__ballot(bad) != 0ull) {
+        if (ballot(bad) != 0ull) {
             status = FA_GRID_RANGE;
         } else {
             int rk[LSL] = {0, 0};
@@ -1577,7 +1600,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                         sq = __int_as_float(e.y);
                         keep = (j < Rn) || !bit_of(sk0, sk1, j - Rn);
                     }
-                    const unsigned long long mk = __ballot(keep);
+                    const unsigned long long mk = ballot(keep);
                     if (keep) {
                         const int pos = NC + lanes_below(mk);
                         if (pos < MAXC) {
@@ -1661,9 +1684,9 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                         for (int q = 0; q < PSL; ++q) {
                             const int c = l + WAVE * q;
                             const bool v = c < NC;
-                            const unsigned long long m0 = __ballot(v && pix[q] == s0);
-                            const unsigned long long m1 = __ballot(v && pix[q] != s0 && pix[q] == s1);
-                            const unsigned long long m2 = __ballot(v && pix[q] != s0 && pix[q] != s1 && pix[q] == s2);
+                            const unsigned long long m0 = ballot(v && pix[q] == s0);
+                            const unsigned long long m1 = ballot(v && pix[q] != s0 && pix[q] == s1);
+                            const unsigned long long m2 = ballot(v && pix[q] != s0 && pix[q] != s1 && pix[q] == s2);
                             // the reference keeps the last match
                             if (m0) pA = WAVE * q + 63 - __clzll((long long)m0);
                             if (m1) pB = WAVE * q + 63 - __clzll((long long)m1);
@@ -1685,7 +1708,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
 #pragma unroll
                         for (int q = 0; q < PSL; ++q) {
                             const int c = l + WAVE * q;
-                            const unsigned long long m = __ballot(c < NC && kA != ~0ull && key_of(fabs(pa3[q])) == kA);
+                            const unsigned long long m = ballot(c < NC && kA != ~0ull && key_of(fabs(pa3[q])) == kA);
                             if (pA < 0 && m) pA = WAVE * q + first_lane(m);
                         }
                         if (pA >= 0) {
@@ -1731,7 +1754,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                             double ptx = 0.0, pty = 0.0, ptz = 0.0;
 #pragma unroll
                             for (int q = 0; q <= PSL; ++q) {
-                                const unsigned long long m = __ballot(kB != ~0ull && dq[q] < BIG && key_of(dq[q]) == kB);
+                                const unsigned long long m = ballot(kB != ~0ull && dq[q] < BIG && key_of(dq[q]) == kB);
                                 if (pB < 0 && m) {
                                     const int ln = first_lane(m);
                                     pB = (q < PSL) ? WAVE * q + ln : NC + ln;
@@ -1778,7 +1801,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                                 const uint64_t kC = wave_min_u64(kcc);
 #pragma unroll
                                 for (int q = 0; q <= PSL; ++q) {
-                                    const unsigned long long m = __ballot(kC != ~0ull && fq[q] < BIG && key_of(fq[q]) == kC);
+                                    const unsigned long long m = ballot(kC != ~0ull && fq[q] < BIG && key_of(fq[q]) == kC);
                                     if (pC < 0 && m) pC = (q < PSL) ? WAVE * q + first_lane(m) : NC + first_lane(m);
                                 }
                             }
@@ -1939,7 +1962,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                             const double d = (plane_at(pk, 2) == -1) ? ro : sqrt(x * x + y * y + z * z);
                             keep = d <= ro;
                         }
-                        const unsigned long long m = __ballot(keep);
+                        const unsigned long long m = ballot(keep);
                         wave_sync();
                         if (keep) {
                             const int pos = vj + lanes_below(m);
@@ -1975,7 +1998,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 for (int q = 0; q < PSL; ++q) {
                     const int c = l + WAVE * q;
                     const bool cand = c < NC && pix[q] >= Rn && s.seed[3 * (pix[q] - Rn)] == -1;
-                    unsigned long long mc = __ballot(cand);
+                    unsigned long long mc = ballot(cand);
                     while (mc) {
                         const int cc = WAVE * q + first_lane(mc);
                         mc &= mc - 1;
@@ -1994,7 +2017,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                                         pos = 2;
                                 }
                             }
-                            const unsigned long long mv = __ballot(pos >= 0);
+                            const unsigned long long mv = ballot(pos >= 0);
                             if (mv) {
                                 const int ln = first_lane(mv);
                                 hit_v = vb + ln;
@@ -2031,7 +2054,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
 #pragma unroll
                     for (int q = 0; q < PSL; ++q) {
                         const int c = l + WAVE * q;
-                        unsigned long long mX = __ballot(c < NC && pfl[q] != 'X');
+                        unsigned long long mX = ballot(c < NC && pfl[q] != 'X');
                         while (mX) {
                             const int X = WAVE * q + first_lane(mX);
                             mX &= mX - 1;
@@ -2041,7 +2064,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                                 if (k2 * WAVE >= NV) break;
                                 const int vi = l + WAVE * k2;
                                 const bool on = vi < NV && has_plane(vpk[k2], X);
-                                const unsigned long long m = __ballot(on);
+                                const unsigned long long m = ballot(on);
                                 if (on) {
                                     const int pos = run + cnt + lanes_below(m);
                                     if (pos < MAXF) s.fpt[pos] = (short)vi;
@@ -2065,7 +2088,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
 #pragma unroll
                     for (int q = 0; q < PSL; ++q) {
                         const int X = l + WAVE * q;
-                        bool add = false;
+                        bool need = false;
                         double t[3] = {0, 0, 0};
                         if (X < NC && pfl[q] != 'X' && s.fcnt[X] > 2) {
                             const int o = s.foff[X], sc = s.fcnt[X];
@@ -2076,12 +2099,13 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                                     t[0] = 2.0 * cx - s.vx[vi];
                                     t[1] = 2.0 * cy - s.vy[vi];
                                     t[2] = 2.0 * cz - s.vz[vi];
-                                    add = test_point(s, t, NC, X);
+                                    need = true;
                                     break;
                                 }
                             }
                         }
-                        const unsigned long long m = __ballot(add);
+                        const bool add = points_inside(need, t, q, NC, pa0, pa1, pa2, pa3, pfl);
+                        const unsigned long long m = ballot(add);
                         if (nv_extra + __popcll(m) > MAXV) {
                             cap = true;
                             break;
@@ -2147,7 +2171,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     }
                     wave_sync();
                     count_op(s.ops, OP_ORDER, wave_sum_i(ncos));
-                    if (__ballot(noline) != 0ull) {
+                    if (ballot(noline) != 0ull) {
                         status = FA_CAPACITY;
                         break;
                     }
@@ -2161,7 +2185,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
 #pragma unroll
                     for (int q = 0; q < PSL; ++q) {
                         const int c = l + WAVE * q;
-                        unsigned long long m = __ballot(c < NC && pfl[q] == 'E');
+                        unsigned long long m = ballot(c < NC && pfl[q] == 'E');
                         while (m && epi < 3) {
                             eng[epi++] = WAVE * q + first_lane(m);
                             m &= m - 1;
@@ -2254,7 +2278,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                         u = 0;
 #pragma unroll
                         for (int q = 0; q < PSL; ++q) {
-                            unsigned long long ov = __ballot(cnt[q] > 0 && off[q] < base + WAVE && off[q] + cnt[q] > base);
+                            unsigned long long ov = ballot(cnt[q] > 0 && off[q] < base + WAVE && off[q] + cnt[q] > base);
                             while (ov) {
                                 const int fl = first_lane(ov);
                                 ov &= ov - 1;
@@ -2314,15 +2338,19 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
 #pragma unroll
                     for (int q = 0; q < PSL; ++q) {
                         const int X = l + WAVE * q;
-                        if (X >= NC || pfl[q] == 'X') continue;
-                        const int NP = s.fcnt[X];
+                        const bool face = X < NC && pfl[q] != 'X';
+                        const int NP = face ? s.fcnt[X] : -1;
                         const double ctr[3] = {-pa0[q] * pa3[q], -pa1[q] * pa3[q], -pa2[q] * pa3[q]};
                         const double cd = fabs(pa3[q]);
+                        // faces with no point or one arc keep their cap when the
+                        // plane's centre is inside the polyhedron
+                        const bool kept = points_inside(NP == 0 || NP == 2, ctr, q, NC, pa0, pa1, pa2, pa3, pfl);
+                        if (!face) continue;
                         if (NP == 0) {
-                            if (test_point(s, ctr, NC, X)) par[q] = (2.0f * PI_REF) * ro * (ro - cd);
+                            if (kept) par[q] = (2.0f * PI_REF) * ro * (ro - cd);
                         } else if (NP == 2) {
                             const double arc = s.u.fcos[aoff[q]];
-                            if (test_point(s, ctr, NC, X))
+                            if (kept)
                                 par[q] = (2.0f * PI_REF) * ro * (ro - cd) - arc;
                             else
                                 par[q] = arc;
@@ -2426,12 +2454,12 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     for (int q = 0; q < PSL; ++q) {
                         const int c = l + WAVE * q;
                         valid[q] = c < NC && par[q] > 0.0 && pfl[q] != 'X';
-                        nvalid_total += __popcll(__ballot(valid[q]));
+                        nvalid_total += __popcll(ballot(valid[q]));
                     }
                     int seen = 0;
 #pragma unroll
                     for (int q = 0; q < PSL; ++q) {
-                        const unsigned long long m = __ballot(valid[q]);
+                        const unsigned long long m = ballot(valid[q]);
                         if (valid[q]) {
                             const int j = pix[q];
                             const double ar = par[q];
