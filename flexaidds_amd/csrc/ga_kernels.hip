@@ -5,8 +5,8 @@
 // fixed sequence of launches on one stream:
 //
 //   tick        generation counter(s) + 1 (the counter seeds the draws)
-//   breed       one thread per offspring pair: roulette parents, crossover;
-//               then one thread per child gene: mutation, gene -> internal
+//   breed       one wave per offspring pair: roulette parents, crossover;
+//               then one quad per child gene: mutation, gene -> internal
 //               coordinate
 //   poses       the contact-function kernel over the offspring (caller)
 //   ensemble    fold the offspring energies into each job's FP64 ensemble
@@ -58,24 +58,6 @@ __host__ __device__ inline double genetoic(double mn, double del, double bin, in
     return mn + del * (double)i;
 }
 
-// LIB/gaboom.cpp:1158-1182: first index whose running fitness exceeds r
-__device__ __forceinline__ int roulette(const double* cum, int n, double u)
-{
-    const double tot = cum[n - 1];
-    if (tot <= 0.0) return (int)(u * n) % n;
-    const double r = u * tot;
-    int lo = 0, hi = n - 1;  // smallest i with cum[i] > r, n - 1 if none
-    if (!(cum[hi] > r)) return n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (cum[mid] > r)
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
-    return lo;
-}
-
 __device__ __forceinline__ unsigned low_mask(int n)
 {
     // the reference's mask loop: optr = 1; repeat n times optr |= optr << 1
@@ -84,15 +66,41 @@ __device__ __forceinline__ unsigned low_mask(int n)
     return optr;
 }
 
-// LIB/gaboom.cpp:2517-2618
-__device__ void crossover(int32_t* john, int32_t* mary, int G, int intragenes, uint64_t seed, uint64_t gen,
-                          uint64_t stream)
+// Roulette pick (roullete_wheel, LIB/gaboom.cpp:1158-1182, scans the running
+// fitness sums)
+// by a whole wave (lane l of 64): the smallest i with cum[i] > u * total
+// over non-decreasing running sums (the last row when none, u * rows when
+// the total is not positive), found by narrowing the range 64 blocks at a
+// time: two rounds of loads for up to 4,096 rows
+__device__ __forceinline__ int roulette_wave(const double* cum, int n, double u, int l)
 {
-    int gen_a = (int)(fa_rng_u01(seed, gen, stream, FA_DRAW_GEN_A) * (double)G);
-    int gen_b = (int)(fa_rng_u01(seed, gen, stream, FA_DRAW_GEN_B) * (double)G);
+    const double tot = cum[n - 1];
+    if (tot <= 0.0) return (int)(u * n) % n;
+    const double r = u * tot;
+    if (!(cum[n - 1] > r)) return n - 1;
+    int lo = 0, hi = n - 1;  // answer in [lo, hi], cum[hi] > r
+    while (lo < hi) {
+        const int S = (hi - lo + 64) / 64;  // block length
+        const int idx = min(lo + (l + 1) * S - 1, hi);
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(cum[idx] > r);
+        const int b = __ffsll((long long)m) - 1;  // first block whose last row passes
+        const int nhi = min(lo + (b + 1) * S - 1, hi);
+        lo = lo + b * S;
+        hi = (S == 1) ? lo : nhi;
+    }
+    return lo;
+}
+
+// crossover points of one pair (LIB/gaboom.cpp:2517-2618): genes
+// [gen_a, gen_b] swap, gen_a from bit pnt_a, then gen_b's low pnt_b - 1 bits
+// swap back when pnt_b > 0
+__device__ __forceinline__ void cross_points(int G, int intragenes, uint64_t seed, uint64_t gen, uint64_t stream,
+                                             int& gen_a, int& gen_b, int& pnt_a, int& pnt_b)
+{
+    gen_a = (int)(fa_rng_u01(seed, gen, stream, FA_DRAW_GEN_A) * (double)G);
+    gen_b = (int)(fa_rng_u01(seed, gen, stream, FA_DRAW_GEN_B) * (double)G);
     if (gen_a >= G) gen_a = G - 1;
     if (gen_b >= G) gen_b = G - 1;
-    int pnt_a, pnt_b;
     if (intragenes) {
         pnt_a = (int)(fa_rng_u01(seed, gen, stream, FA_DRAW_PNT_A) * 32.0);
         pnt_b = (int)(fa_rng_u01(seed, gen, stream, FA_DRAW_PNT_B) * 32.0);
@@ -122,18 +130,6 @@ __device__ void crossover(int32_t* john, int32_t* mary, int G, int intragenes, u
             pnt_a = 32;
             pnt_b = 0;
         }
-    }
-    for (int j = gen_a; j <= gen_b; ++j) {
-        const unsigned optr = low_mask((j == gen_a) ? pnt_a : 32);
-        const unsigned uj = (unsigned)john[j], um = (unsigned)mary[j];
-        john[j] = (int32_t)((uj & ~optr) | (um & optr));
-        mary[j] = (int32_t)((uj & optr) | (um & ~optr));
-    }
-    if (pnt_b > 0) {
-        const unsigned optr = low_mask(pnt_b - 1);
-        const unsigned uj = (unsigned)john[gen_b], um = (unsigned)mary[gen_b];
-        john[gen_b] = (int32_t)((uj & ~optr) | (um & optr));
-        mary[gen_b] = (int32_t)((uj & optr) | (um & ~optr));
     }
 }
 
@@ -169,63 +165,91 @@ __global__ void ga_init_kernel(GaDev g)
     g.gic[at] = genetoic(g.lmin[l], g.ldel[l], g.lbin[l], gene);
 }
 
-// Reproduction (LIB/gaboom.cpp:920-973) in two launches: one thread per
+// Reproduction (LIB/gaboom.cpp:920-973) in two launches: one wave per
 // offspring pair picks the parents by roulette and crosses them over into the
-// children rows; then one thread per child gene applies that gene's 32
-// mutation draws and converts it to an internal coordinate.  The draws are
-// the ones a single thread per pair would make (fa_rng.h layout), so the
-// split changes nothing but the parallelism.
-__global__ void ga_pair_kernel(GaDev g)
+// children rows (lane j: gene j); then one quad per child gene applies that
+// gene's 32 mutation draws and converts it to an internal coordinate.  The
+// draws are the ones a single thread per pair would make (fa_rng.h layout),
+// so the split changes nothing but the parallelism.
+__global__ __launch_bounds__(256) void ga_pair_kernel(GaDev g)
 {
+    // one wave per offspring pair, lane j holding gene j of both children
     const int pairs = (g.n_new + 1) / 2;
-    const long long tt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tt >= (long long)g.n_act * pairs) return;
+    const long long tt = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int l = threadIdx.x & 63;
+    if (tt >= (long long)g.n_act * pairs) return;  // whole waves
     const int a = (int)(tt / pairs), t = (int)(tt % pairs);
     const int k = slot_of(g, a);
     const uint64_t gen = gen_of(g, k), seed = seed_of(g, k);
     const uint64_t st = stream_of(g, k, t);
     const int G = g.G[k], Gm = g.Gmax;
     const double* cum = g.cum + (size_t)k * g.pop;
-    const int p1 = roulette(cum, g.pop, fa_rng_u01(seed, gen, st, FA_DRAW_P1));
-    const int p2 = roulette(cum, g.pop, fa_rng_u01(seed, gen, st, FA_DRAW_P2));
-    int32_t john[FA_MAX_GENES], mary[FA_MAX_GENES];
-    for (int j = 0; j < G; ++j) {
-        john[j] = g.gi[parent_row(g, k, p1) * Gm + j];
-        mary[j] = g.gi[parent_row(g, k, p2) * Gm + j];
+    const int p1 = roulette_wave(cum, g.pop, fa_rng_u01(seed, gen, st, FA_DRAW_P1), l);
+    const int p2 = roulette_wave(cum, g.pop, fa_rng_u01(seed, gen, st, FA_DRAW_P2), l);
+    unsigned jo = 0u, ma = 0u;
+    if (l < G) {
+        jo = (unsigned)g.gi[parent_row(g, k, p1) * Gm + l];
+        ma = (unsigned)g.gi[parent_row(g, k, p2) * Gm + l];
     }
-    if (fa_rng_u01(seed, gen, st, FA_DRAW_CROSS) < g.cross) crossover(john, mary, G, g.intragenes, seed, gen, st);
+    if (fa_rng_u01(seed, gen, st, FA_DRAW_CROSS) < g.cross) {
+        int ga, gb, pa, pb;
+        cross_points(G, g.intragenes, seed, gen, st, ga, gb, pa, pb);
+        // the reference's crossover on gene l: the span swap, then the end
+        // gene's second swap
+        if (l >= ga && l <= gb) {
+            const unsigned optr = low_mask((l == ga) ? pa : 32);
+            const unsigned nj = (jo & ~optr) | (ma & optr), nm = (jo & optr) | (ma & ~optr);
+            jo = nj;
+            ma = nm;
+        }
+        if (pb > 0 && l == gb) {
+            const unsigned optr = low_mask(pb - 1);
+            const unsigned nj = (jo & ~optr) | (ma & optr), nm = (jo & optr) | (ma & ~optr);
+            jo = nj;
+            ma = nm;
+        }
+    }
     const size_t c1 = child_row(g, a, 2 * t);
-    for (int j = 0; j < G; ++j) g.gi[c1 * Gm + j] = john[j];
-    if (2 * t + 1 < g.n_new)
-        for (int j = 0; j < G; ++j) g.gi[(c1 + 1) * Gm + j] = mary[j];
+    if (l < G) {
+        g.gi[c1 * Gm + l] = (int32_t)jo;
+        if (2 * t + 1 < g.n_new) g.gi[(c1 + 1) * Gm + l] = (int32_t)ma;
+    }
 }
 
 // per child gene: mutation (LIB/gaboom.cpp:2620-2641) and genetoic; genes
-// past the job's own count are 0 (rows are Gmax wide)
-__global__ void ga_mutate_kernel(GaDev g)
+// past the job's own count are 0 (rows are Gmax wide).  Four lanes per gene
+// (one quad), eight of the gene's 32 bit draws each; the quad ORs its flip
+// masks and its first lane writes the gene.  Grid y is the job.
+__global__ __launch_bounds__(256) void ga_mutate_kernel(GaDev g)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)g.n_act * g.n_new * g.Gmax) return;
-    const int a = (int)(t / ((long long)g.n_new * g.Gmax));
-    const int i = (int)((t / g.Gmax) % g.n_new), j = (int)(t % g.Gmax);
+    const int a = blockIdx.y;
+    const int tq = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gi = tq >> 2, part = tq & 3;  // child gene of the job, quarter of its bits
+    const bool live = gi < g.n_new * g.Gmax;
+    const int i = gi / g.Gmax, j = gi - i * g.Gmax;
     const int k = slot_of(g, a);
     const int G = g.G[k];
+    unsigned optr = 0u;
+    if (live && j < G) {
+        const uint64_t gen = gen_of(g, k), seed = seed_of(g, k);
+        const uint64_t st = stream_of(g, k, i >> 1);
+        const uint64_t base = FA_DRAW_MUT + ((uint64_t)(i & 1) * G + j) * 32 + 8 * part;
+        for (int b = 0; b < 8; ++b)
+            if (fa_rng_u01(seed, gen, st, base + b) < g.mut) optr |= 1u << (8 * part + b);
+    }
+    optr |= (unsigned)__shfl_xor((int)optr, 1);
+    optr |= (unsigned)__shfl_xor((int)optr, 2);
+    if (!live || part != 0) return;
     const size_t at = child_row(g, a, i) * g.Gmax + j;
     if (j >= G) {
         g.gi[at] = 0;
         g.gic[at] = 0.0;
         return;
     }
-    const uint64_t gen = gen_of(g, k), seed = seed_of(g, k);
-    const uint64_t st = stream_of(g, k, i >> 1);
-    const uint64_t base = FA_DRAW_MUT + ((uint64_t)(i & 1) * G + j) * 32;
-    unsigned optr = 0u;
-    for (int b = 0; b < 32; ++b)
-        if (fa_rng_u01(seed, gen, st, base + b) < g.mut) optr |= 1u << b;
     const int32_t gene = g.gi[at] ^ (int32_t)optr;
-    const size_t l = (size_t)k * g.Gmax + j;
+    const size_t lk = (size_t)k * g.Gmax + j;
     g.gi[at] = gene;
-    g.gic[at] = genetoic(g.lmin[l], g.ldel[l], g.lbin[l], gene);
+    g.gic[at] = genetoic(g.lmin[lk], g.ldel[lk], g.lbin[lk], gene);
 }
 
 // energies of each job's children (children = 1) or staged initial
@@ -275,7 +299,7 @@ __device__ __forceinline__ size_t cand_row(const GaDev& g, int a, int k, int r, 
 // in order (QuickSort by evalue, LIB/gaboom.cpp:2674-2726).  The counting is
 // split over candidate chunks so that one job's ranking spreads over many
 // blocks: zero, count (integer atomics: order-free), gather.
-constexpr int RANK_CHUNK = 1024;
+constexpr int RANK_CHUNK = 256;  // candidates compared per block (more blocks in flight)
 
 __global__ void ga_rank_zero_kernel(GaDev g, int n)
 {
@@ -297,6 +321,7 @@ __global__ __launch_bounds__(256) void ga_rank_count_kernel(GaDev g, int initial
     if (i >= n) return;
     const uint64_t ki = order_key(g.res[cand_row(g, a, k, i, initial)].cf);
     int cnt = 0;
+#pragma unroll 8
     for (int q = 0; q < lim; ++q) {
         const uint64_t kj = keys[q];
         cnt += (kj < ki) || (kj == ki && base + q < i);
@@ -549,8 +574,11 @@ void ga_launch_init(const GaDev& g, hipStream_t s)
 void ga_launch_breed(const GaDev& g, hipStream_t s)
 {
     if (g.n_act < 1) return;
-    hipLaunchKernelGGL(ga_pair_kernel, dim3(blocks_of((long long)g.n_act * ((g.n_new + 1) / 2))), dim3(256), 0, s, g);
-    hipLaunchKernelGGL(ga_mutate_kernel, dim3(blocks_of((long long)g.n_act * g.n_new * g.Gmax)), dim3(256), 0, s, g);
+    // four pairs per 256-thread block (a wave each); 64 child genes per
+    // block (a quad each), one row of blocks per job
+    const long long pairs = (long long)g.n_act * ((g.n_new + 1) / 2);
+    hipLaunchKernelGGL(ga_pair_kernel, dim3((unsigned)std::max(1LL, (pairs + 3) / 4)), dim3(256), 0, s, g);
+    hipLaunchKernelGGL(ga_mutate_kernel, dim3(blocks_of(4ll * g.n_new * g.Gmax), g.n_act), dim3(256), 0, s, g);
 }
 
 void ga_launch_ensemble(const GaDev& g, int children, hipStream_t s)
