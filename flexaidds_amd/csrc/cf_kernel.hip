@@ -783,14 +783,25 @@ __device__ void build_pose(Smem& s, const DevLigand& lig, const double* icv)
         s.u.b.ang[i] = lig.ang[i];
         s.u.b.dih[i] = lig.dih[i];
     }
+    // the genes' values, clamps, atoms and types on lanes (one load round),
+    // then the decode in gene order on lane 0 from those registers
+    double gv = 0.0;
+    int gatm = 0, gtyp = 0;
+    if (l < lig.n_genes) {
+        gv = icv[l];
+        const double gmax = lig.gene_max[l], gmin = lig.gene_min[l];
+        if (gv > gmax) gv = gmax;
+        if (gv < gmin) gv = gmin;
+        gatm = lig.gene_atm[l];
+        gtyp = lig.gene_typ[l];
+    }
     wave_sync();
-    if (l == 0) {
-        for (int g = 0; g < lig.n_genes; ++g) {
-            double v = icv[g];
-            if (v > lig.gene_max[g]) v = lig.gene_max[g];
-            if (v < lig.gene_min[g]) v = lig.gene_min[g];
-            const int atm = lig.gene_atm[g];
-            const int typ = lig.gene_typ[g];
+    const int ng = lig.n_genes;
+    for (int g = 0; g < ng; ++g) {
+        const double v = readlane_d(gv, g);
+        const int atm = readlane_i(gatm, g);
+        const int typ = readlane_i(gtyp, g);
+        if (l == 0) {
             if (typ == FA_GENE_GRID) {
                 unsigned gi = (unsigned)v;
                 if ((int)gi >= lig.n_grid) gi = lig.n_grid - 1;
@@ -833,8 +844,23 @@ __device__ void build_pose(Smem& s, const DevLigand& lig, const double* icv)
         }
     }
     static_assert(LSL == 2, "two ligand slots per lane");
+    // the rebuild order and the reference atoms of every atom on lanes (atom
+    // i: lane i & 63, slot i >> 6), read back by lane broadcast in the loop
+    int mv[LSL], r0[LSL], r1[LSL], r2[LSL];
+#pragma unroll
+    for (int q = 0; q < LSL; ++q) {
+        const int i = l + WAVE * q;
+        mv[q] = r0[q] = r1[q] = r2[q] = -1;
+        if (i < n) {
+            mv[q] = lig.mov[i];
+            r0[q] = lig.rec[4 * i];
+            r1[q] = lig.rec[4 * i + 1];
+            r2[q] = lig.rec[4 * i + 2];
+        }
+    }
+    const float ox0 = lig.ori[0], oy0 = lig.ori[1], oz0 = lig.ori[2];
     for (int an = 0; an < n; ++an) {
-        const int a = lig.mov[an];
+        const int a = readlane_i(an < WAVE ? mv[0] : mv[1], an & 63);
         const int la = a & 63;
         const bool hi = a >= WAVE;
         const float cta = readlane_f(hi ? ca[1] : ca[0], la);
@@ -842,26 +868,29 @@ __device__ void build_pose(Smem& s, const DevLigand& lig, const double* icv)
         const float ctd = readlane_f(hi ? cd[1] : cd[0], la);
         const float std_ = readlane_f(hi ? sd[1] : sd[0], la);
         const float dis = readlane_f(hi ? di[1] : di[0], la);
+        const int rj[3] = {readlane_i(hi ? r0[1] : r0[0], la), readlane_i(hi ? r1[1] : r1[0], la),
+                           readlane_i(hi ? r2[1] : r2[0], la)};
         float x[4], y[4], z[4];
+#pragma unroll
         for (int i = 1; i <= 3; ++i) {
-            const int j = lig.rec[4 * a + i - 1];
+            const int j = rj[i - 1];
             if (j >= 0) {
                 const bool hj = j >= WAVE;
                 x[i] = readlane_f(hj ? X[1] : X[0], j & 63);
                 y[i] = readlane_f(hj ? Y[1] : Y[0], j & 63);
                 z[i] = readlane_f(hj ? Z[1] : Z[0], j & 63);
             } else if (i == 1) {
-                x[i] = 1.0f + lig.ori[0];
-                y[i] = 0.0f + lig.ori[1];
-                z[i] = 0.0f + lig.ori[2];
+                x[i] = 1.0f + ox0;
+                y[i] = 0.0f + oy0;
+                z[i] = 0.0f + oz0;
             } else if (i == 3) {
-                x[i] = 0.0f + lig.ori[0];
-                y[i] = 1.0f + lig.ori[1];
-                z[i] = 0.0f + lig.ori[2];
+                x[i] = 0.0f + ox0;
+                y[i] = 1.0f + oy0;
+                z[i] = 0.0f + oz0;
             } else {
-                x[i] = 0.0f + lig.ori[0];
-                y[i] = 0.0f + lig.ori[1];
-                z[i] = 0.0f + lig.ori[2];
+                x[i] = 0.0f + ox0;
+                y[i] = 0.0f + oy0;
+                z[i] = 0.0f + oz0;
             }
             x[i] += 1e-10f;
             y[i] += 1e-10f;
