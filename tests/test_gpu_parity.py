@@ -322,3 +322,56 @@ def test_thermo_over_results_on_device(ctx0, sys0, hip, oracle_cr, oracle):
     o = abi.Thermo()
     oracle.fa_oracle_thermo(e.ctypes.data, None, len(e), 300.0, C.byref(o))
     assert abs(th["free_energy"] - o.free_energy) <= 1e-6 * abs(o.free_energy)
+
+
+def _bit_level_parity(out, ref, min_ok):
+    """Statuses equal, scored poses within the CF tolerance and in practice
+    at rounding level (the same arithmetic), clash walls equal."""
+    diff = np.nonzero(out["status"] != ref["status"])[0]
+    assert diff.size == 0, [(int(i), int(out["status"][i]), int(ref["status"][i])) for i in diff[:20]]
+    ok = ref["status"] == 0
+    assert ok.sum() >= min_ok
+    d = np.abs(out["cf"] - ref["cf"])
+    assert np.all(d[ok] <= tol(ref["cf"][ok]))
+    assert np.all(d[ok] < 1e-9)
+    cl = ref["status"] == 2
+    assert np.all(np.abs(out["cf"] - ref["cf"])[cl] <= 1e-9 * np.abs(ref["cf"][cl]))
+
+
+def test_config1_against_oracle(hip, oracle):
+    """The benchmark's own workload (4,000-atom receptor, 45-atom /
+    10-torsion ligand) on 400 of its poses, with the operation counters."""
+    s = synth.make_config1(pop=400)
+    d = abi.Descriptors(s.receptor, s.ligands, s.params)
+    c = hip.Context()
+    c.upload_system(s)
+    out = c.eval(0, s.genes[0])
+    ops = c.opcount()
+    caps = c.counters()["capacity"]
+    c.close()
+    with ol.trig_mode(oracle, 1):
+        ref, rops = ol.oracle_eval_counted(oracle, d, 0, s.genes[0])
+    assert caps == 0
+    _bit_level_parity(out, ref, min_ok=100)
+    assert ops == rops
+
+
+def test_config3_receptor_sizes_against_oracle(hip, oracle):
+    """Served-stream receptors of 2,000, 6,000 and 10,000 atoms, each with
+    ligands of the stream's size distribution, 48 poses per ligand."""
+    systems = synth.make_config3(n_receptors=3, ligands_per_receptor=3, n_poses=48)
+    for s in systems:
+        d = abi.Descriptors(s.receptor, s.ligands, s.params)
+        c = hip.Context()
+        c.upload_system(s)
+        n_ok = 0
+        for li, gl in enumerate(s.genes):
+            out = c.eval(li, gl)
+            assert c.counters()["capacity"] == 0
+            with ol.trig_mode(oracle, 1):
+                ref = ol.oracle_eval(oracle, d, li, gl)
+            _bit_level_parity(out, ref, min_ok=0)
+            n_ok += int((ref["status"] == 0).sum())
+        c.close()
+        print("receptor", s.receptor.n_atoms, "scored poses", n_ok, flush=True)
+        assert n_ok > 0
