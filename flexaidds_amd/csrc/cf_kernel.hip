@@ -546,8 +546,13 @@ __device__ __forceinline__ void edge_new(Edges& E, int e, int p0, int p1, int tp
     e = uni(e);
     const bool me = lane_id() == (e & 63);
     const int k = e >> 6;
+    if (k == 0) {  // the common case: the first 64 edges
+        E.pl[0] = me ? pl : E.pl[0];
+        E.vt[0] = me ? vt : E.vt[0];
+        return;
+    }
 #pragma unroll
-    for (int K = 0; K < ESL; ++K) {
+    for (int K = 1; K < ESL; ++K) {
         const bool hit = me && k == K;
         E.pl[K] = hit ? pl : E.pl[K];
         E.vt[K] = hit ? vt : E.vt[K];
@@ -561,8 +566,13 @@ __device__ __forceinline__ void edge_close(Edges& E, int e, int end_plane, int e
     const bool me = lane_id() == (e & 63);
     const int k = e >> 6;
     const unsigned ep = ((unsigned)end_plane & NONE) << 24, ev = (unsigned)end_vertex << 16;
+    if (k == 0) {  // the common case: the first 64 edges
+        E.pl[0] = me ? ((E.pl[0] & 0x00FFFFFFu) | ep) : E.pl[0];
+        E.vt[0] = me ? ((E.vt[0] & 0xFFFFu) | ev) : E.vt[0];
+        return;
+    }
 #pragma unroll
-    for (int K = 0; K < ESL; ++K) {
+    for (int K = 1; K < ESL; ++K) {
         const bool hit = me && k == K;
         E.pl[K] = hit ? ((E.pl[K] & 0x00FFFFFFu) | ep) : E.pl[K];
         E.vt[K] = hit ? ((E.vt[K] & 0xFFFFu) | ev) : E.vt[K];
