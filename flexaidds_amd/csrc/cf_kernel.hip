@@ -2287,11 +2287,13 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     // each face's lane then sums its own terms in the reference's
                     // order (triangles, then arcs), so the sums are bit-identical.
                     int ntq[PSL], naq[PSL], toff[PSL], aoff[PSL];
+                    unsigned amk[PSL];  // boundary arcs of a face: bit qq for the pair (qq, qq + 1), faces of <= 32 points
                     int NT = 0, NA = 0;
 #pragma unroll
                     for (int q = 0; q < PSL; ++q) {
                         const int X = l + WAVE * q;
                         ntq[q] = naq[q] = 0;
+                        amk[q] = 0u;
                         if (X < NC && pfl[q] != 'X') {
                             const int NP = s.fcnt[X];
                             if (NP == 2) {
@@ -2299,8 +2301,20 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                             } else if (NP > 2) {
                                 ntq[q] = NP - 2;
                                 const short* pt = &s.fpt[s.foff[X]];
-                                for (int qq = 0; qq < NP; ++qq)
-                                    naq[q] += arc_pair(s.vpl[pt[qq]], s.vpl[pt[(qq + 1) % NP]]);
+                                // consecutive point pairs, the next point's index read
+                                // one pair ahead so each pair waits for one LDS read
+                                const unsigned kfirst = s.vpl[pt[0]];
+                                unsigned kcur = kfirst;
+                                int wn = pt[1];
+                                for (int qq = 0; qq < NP; ++qq) {
+                                    const int wnn = (qq + 2 < NP) ? pt[qq + 2] : 0;
+                                    const unsigned knext = (qq + 1 < NP) ? s.vpl[wn] : kfirst;
+                                    const int ap = arc_pair(kcur, knext);
+                                    naq[q] += ap;
+                                    if (qq < 32) amk[q] |= (unsigned)ap << qq;
+                                    kcur = knext;
+                                    wn = wnn;
+                                }
                             }
                         }
                         toff[q] = NT + wave_excl_scan(ntq[q]);
@@ -2353,17 +2367,27 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                         const int t = base + l;
                         int X, u;
                         job_face(base, t, aoff, naq, X, u);
+                        // the arc mask of the job's face, from the face's lane
+                        const unsigned am0 = (unsigned)__shfl((int)amk[0], X & 63);
+                        const unsigned am1 = (unsigned)__shfl((int)amk[1], X & 63);
+                        static_assert(PSL == 2, "two face slots per lane");
                         if (t < NA) {
                             const short* pt = &s.fpt[s.foff[X]];
                             const int NP = s.fcnt[X];
                             int wa = pt[0], wb = pt[1];
-                            if (NP > 2) {  // the u-th qualifying boundary arc
+                            if (NP > 32) {  // the u-th qualifying boundary arc, by scan
                                 int k2 = -1;
                                 for (int qq = 0; qq < NP && k2 < u; ++qq) {
                                     wa = pt[qq];
                                     wb = pt[(qq + 1) % NP];
                                     k2 += arc_pair(s.vpl[wa], s.vpl[wb]);
                                 }
+                            } else if (NP > 2) {  // the u-th set bit of the face's arc mask
+                                unsigned m = (X < WAVE) ? am0 : am1;
+                                for (int i = 0; i < u; ++i) m &= m - 1u;
+                                const int qq = __ffs((int)m) - 1;
+                                wa = pt[qq];
+                                wb = pt[(qq + 1) % NP];
                             }
                             double pX[4];
                             plane_of(s, X, pX);
