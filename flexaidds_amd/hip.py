@@ -16,7 +16,7 @@ from . import abi
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libflexaidds_amd.so")
 
 if not os.path.exists(LIB_PATH):
-    raise ImportError(f"{LIB_PATH} is not built: run `python -m flexaidds_amd.build` (or __graft_entry__.build())")
+    raise ImportError(f"{LIB_PATH} is not built: run `python -m flexaidds_amd.build`")
 
 # One HIP runtime per process: when PyTorch is present its bundled runtime is
 # loaded first so that torch tensors and this library share it.

@@ -1,6 +1,6 @@
 """Loaders for the CPU oracle and the reference harness (test infrastructure).
 
-Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use
+Only tests/, the entry module's smoke() and bench.py's CPU-baseline leg use
 these; the shipped library never does.
 """
 from __future__ import annotations
