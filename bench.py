@@ -156,6 +156,35 @@ def batched_leg(args):
             "p_empty": gpf["p_empty"]}
 
 
+def grand_leg(args):
+    """Config-4 shape: a blind dock of competing ligands on a 20,000-atom
+    globule as one batched GA, then the grand partition function over the
+    ligands' ensembles at their concentrations.  A few generations are timed;
+    the full 300-generation run is projected from them."""
+    import torch  # noqa: F401  (same HIP runtime as the library)
+    from flexaidds_amd import hip, synth
+    log(f"grand-canonical blind dock: {args.grand_ligands} ligands x {args.grand_pop}")
+    gs, conc = synth.make_config4(n_ligands=args.grand_ligands, pop=8)
+    gctx = hip.Context()
+    gctx.upload_system(gs)
+    J = len(gs.ligands)
+    ga = hip.GA(gctx, [synth.ga_limits(l) for l in gs.ligands], ligands=list(range(J)), population=args.grand_pop,
+                offspring=args.grand_pop, seed=404, temperature=300.0)
+    ga.run(1, graph=True)                                     # capture + warm
+    ms = ga.run(args.grand_generations, graph=True)
+    gpf = ga.grand(conc)
+    ga.close()
+    gctx.close()
+    poses = J * args.grand_pop * args.grand_generations
+    per_gen = ms / args.grand_generations
+    return {"ligands": J, "receptor_atoms": gs.receptor.n_atoms, "population": args.grand_pop,
+            "generations": args.grand_generations, "poses_per_generation": J * args.grand_pop,
+            "ms_per_generation": round(per_gen, 3), "poses_per_s": round(poses / (ms / 1e3), 1),
+            "projected_300_generations_s": round(300 * per_gen / 1e3, 1),
+            "ln_xi_300K": round(gpf["log_xi"], 6), "p_empty": gpf["p_empty"],
+            "max_occupancy": round(float(np.max(gpf["p_bound"])), 6)}
+
+
 def served_leg(args):
     """The served stream, and the plain path on the same kind of requests.
 
@@ -223,6 +252,9 @@ def main():
     ap.add_argument("--served-pop", type=int, default=1024)
     ap.add_argument("--served-cap", type=int, default=2, help="running jobs per receptor in the capped scenario")
     ap.add_argument("--served-low-rate", type=float, default=6.0, help="arrival rate below capacity (0: skip)")
+    ap.add_argument("--grand-ligands", type=int, default=64, help="config-4 blind dock ligands (0: skip)")
+    ap.add_argument("--grand-pop", type=int, default=32768)
+    ap.add_argument("--grand-generations", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true",
                     help="only time one launch per population size (1 .. 16,384 poses of the plain workload) and exit")
@@ -310,6 +342,7 @@ def main():
     ga_dock = ga_dock_leg(ctx, system, args) if args.ga_generations > 0 else None
     batched = batched_leg(args) if args.batched_ligands > 0 else None
     served = served_leg(args) if args.served_requests > 0 else None
+    grand = grand_leg(args) if args.grand_ligands > 0 else None
 
     cpu = None if args.no_cpu else cpu_baseline(system, 0, pops.reshape(-1, G))
     line = {
@@ -339,6 +372,7 @@ def main():
         "ga_dock": ga_dock,
         "batched": batched,
         "served": served,
+        "grand": grand,
         "status_counts": {"ok": int(status[0]), "hull_fail": int(status[1]), "clash": int(status[2]),
                           "capacity": int(status[3]), "grid_range": int(status[4])},
         "capacity_overflows": cap["capacity"],

@@ -628,6 +628,37 @@ def make_config3(n_receptors=8, ligands_per_receptor=16, seed=None, n_poses=8) -
     return systems
 
 
+def make_config4(n_ligands=64, pop=8, n_rec=20000, n_grid=2000, seed=None):
+    """Large blind dock: a solid globule of `n_rec` atoms at protein density
+    (radius ~37 Å for 20,000), `n_ligands` competing ligands of 20-60 heavy
+    atoms (torsions about atoms / 5) whose translation grids cover the whole
+    surface (points uniform in a shell from 4 Å inside to 4 Å outside the
+    globule's radius), and concentrations log-uniform over 1 nM - 1 mM.
+    Returns (system, concentrations in M)."""
+    rng = np.random.default_rng(CONFIG_SEED[4] if seed is None else seed)
+    center = np.array([0.0, 0.0, 0.0])
+    R = (3.0 * n_rec * DENSITY_VOL / (4.0 * math.pi)) ** (1.0 / 3.0)
+    rec = make_receptor(rng, n_rec, center, r_in=0.0, r_out=R, solid=True)
+    ori = rec.xyz.astype(np.float64).mean(axis=0).astype(np.float32)
+    params = Params(ntypes=40, emat=make_emat(rng))
+    ligs, genes = [], []
+    for _ in range(n_ligands):
+        na = int(rng.integers(20, 61))
+        nt = int(round(na / 5.0))
+        u = rng.random(n_grid)
+        r = np.cbrt((R - 4.0) ** 3 + u * ((R + 4.0) ** 3 - (R - 4.0) ** 3))
+        v = rng.normal(size=(n_grid, 3))
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        pts = center[None, :] + v * r[:, None]
+        anchor = center + v[0] * (R + 4.0)
+        lig = make_ligand(rng, na, nt, ori, anchor, pts)
+        finalize_frame(lig, rec)
+        ligs.append(lig)
+        genes.append(random_genes(rng, lig, pop))
+    conc = np.exp(rng.uniform(math.log(1e-9), math.log(1e-3), n_ligands))
+    return System(receptor=rec, ligands=ligs, params=params, genes=genes, config=4), conc
+
+
 def poisson_requests(rng, systems, rate_hz, n_requests, generations=100):
     """Request stream: Poisson arrivals at `rate_hz`; receptor uniform, ligand
     uniform in its library.  Returns dicts sorted by arrival time (s)."""

@@ -375,3 +375,25 @@ def test_config3_receptor_sizes_against_oracle(hip, oracle):
         c.close()
         print("receptor", s.receptor.n_atoms, "scored poses", n_ok, flush=True)
         assert n_ok > 0
+
+
+def test_config4_globule_against_oracle(hip, oracle):
+    """The blind-dock receptor: a 20,000-atom globule (its cell lists far
+    beyond one CU's LDS), ligands of 20-60 atoms placed over the whole
+    surface, 48 poses for each of three ligands."""
+    s, conc = synth.make_config4(n_ligands=3, pop=48)
+    assert s.receptor.n_atoms == 20000 and conc.shape == (3,)
+    d = abi.Descriptors(s.receptor, s.ligands, s.params)
+    c = hip.Context()
+    c.upload_system(s)
+    n_ok = 0
+    for li, gl in enumerate(s.genes):
+        out = c.eval(li, gl)
+        assert c.counters()["capacity"] == 0
+        with ol.trig_mode(oracle, 1):
+            ref = ol.oracle_eval(oracle, d, li, gl)
+        _bit_level_parity(out, ref, min_ok=0)
+        n_ok += int((ref["status"] == 0).sum())
+    c.close()
+    print("scored poses", n_ok, flush=True)
+    assert n_ok > 0
