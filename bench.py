@@ -17,6 +17,9 @@ path (one ligand population per launch).  Beside it:
             docking pools as they arrive; p50/p99 latency, aggregate poses/s,
             and the plain path on the same mix (one request at a time, one
             synchronous launch per generation)
+  grand     the blind dock of 64 competing ligands on a 20,000-atom globule
+            (32,768 chromosomes each, one batched GA) and the grand partition
+            function over them; a few generations timed, 300 projected
 
 The roofline numbers come from the kernel's own operation counters (the
 algorithmic FLOP model, see include/fa_types.h fa_opcount) and HIP-event
