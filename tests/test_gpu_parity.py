@@ -397,3 +397,52 @@ def test_config4_globule_against_oracle(hip, oracle):
     c.close()
     print("scored poses", n_ok, flush=True)
     assert n_ok > 0
+
+
+def test_coarse_grid_receptor_against_oracle(hip, oracle):
+    """A receptor of 60 atoms in a 6 Å ball: fewer than three cells a side, so
+    a cell can be named by several of an atom's 27 slots (the coarse paths
+    of both the precomputed and the widened grid)."""
+    rng = np.random.default_rng(2718)
+    center = np.array([3.0, -2.0, 5.0])
+    rec = synth.make_receptor(rng, 60, center, r_in=0.0, r_out=6.0, solid=True)
+    ori = rec.xyz.astype(np.float64).mean(axis=0).astype(np.float32)
+    pts = center[None, :] + rng.uniform(-9.0, 9.0, size=(160, 3))
+    lig = synth.make_ligand(rng, 20, 3, ori, center + np.array([7.0, 0.0, 0.0]), pts)
+    synth.finalize_frame(lig, rec)
+    params = synth.Params(ntypes=40, emat=synth.make_emat(rng))
+    genes = synth.random_genes(rng, lig, 160)
+    s = synth.System(receptor=rec, ligands=[lig], params=params, genes=[genes], config=0)
+    d = abi.Descriptors(s.receptor, s.ligands, s.params)
+    with hip.Context() as c:
+        c.upload_system(s)
+        out = c.eval(0, genes)
+        caps = c.counters()["capacity"]
+    with ol.trig_mode(oracle, 1):
+        ref = ol.oracle_eval(oracle, d, 0, genes)
+    assert caps == 0
+    _bit_level_parity(out, ref, min_ok=5)
+
+
+def test_largest_ligand_against_oracle(hip, oracle):
+    """A ligand at the atom limit (128 atoms, 20 torsions): ranks and atoms
+    in both lane slots everywhere in the pose."""
+    rng = np.random.default_rng(1618)
+    center = np.array([-4.0, 6.0, 1.0])
+    rec = synth.make_receptor(rng, 3000, center, r_in=24.0)   # a cavity the ligand fits in
+    ori = rec.xyz.astype(np.float64).mean(axis=0).astype(np.float32)
+    pts = center[None, :] + rng.uniform(-4.0, 4.0, size=(300, 3))
+    lig = synth.make_ligand(rng, 128, 20, ori, center, pts)
+    synth.finalize_frame(lig, rec)
+    params = synth.Params(ntypes=40, emat=synth.make_emat(rng))
+    s = synth.System(receptor=rec, ligands=[lig], params=params, genes=[synth.random_genes(rng, lig, 300)])
+    assert s.ligands[0].n_atoms == 128
+    d = abi.Descriptors(s.receptor, s.ligands, s.params)
+    with hip.Context() as c:
+        c.upload_system(s)
+        out = c.eval(0, s.genes[0])
+        caps = c.counters()["capacity"]
+    with ol.trig_mode(oracle, 1):
+        ref = ol.oracle_eval(oracle, d, 0, s.genes[0])
+    assert caps == 0
+    _bit_level_parity(out, ref, min_ok=5)
