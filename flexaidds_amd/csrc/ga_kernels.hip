@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "bin_index.h"
 #include "fa_rng.h"
 
 namespace fa {
@@ -45,17 +46,12 @@ __device__ __forceinline__ uint64_t stream_of(const GaDev& g, int k, int t)
 __device__ __forceinline__ uint64_t seed_of(const GaDev& g, int k) { return g.seeds ? g.seeds[k] : g.seed; }
 __device__ __forceinline__ uint64_t gen_of(const GaDev& g, int k) { return g.gens ? g.gens[k] : *g.gen; }
 
-// LIB/gaboom.cpp:2887-2906 with RandomDouble(gene) = gene / (2^31)
+// LIB/gaboom.cpp:2887-2906 with RandomDouble(gene) = gene / (2^31); the bin
+// count of its accumulation loop without the loop (bin_index.h)
 __host__ __device__ inline double genetoic(double mn, double del, double bin, int32_t gene)
 {
     const double u = gene / ((double)FA_MAX_RANDOM_VALUE + 1.0);
-    int i = 0;
-    double tot = bin;
-    while (tot < u) {
-        tot += bin;
-        ++i;
-    }
-    return mn + del * (double)i;
+    return mn + del * (double)fa_bin_index(u, bin);
 }
 
 __device__ __forceinline__ unsigned low_mask(int n)
