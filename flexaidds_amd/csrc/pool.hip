@@ -510,6 +510,21 @@ int fa_pool_set_max_running(fa_pool* pl, int max_running)
     return FA_SUCCESS;
 }
 
+int fa_pool_busy(fa_pool* pl, int* busy)
+{
+    if (!pl || !busy) return fa_fail(FA_ERR_INVALID, "null argument");
+    *busy = 0;
+    if (!pl->in_flight) return FA_SUCCESS;
+    FA_HIP(hipSetDevice(pl->ctx->device));
+    const hipError_t e = hipStreamQuery(pl->stream);
+    if (e == hipErrorNotReady) {
+        *busy = 1;
+        return FA_SUCCESS;
+    }
+    FA_HIP(e);
+    return FA_SUCCESS;
+}
+
 int fa_pool_destroy(fa_pool* pl)
 {
     if (!pl) return FA_SUCCESS;

@@ -84,6 +84,7 @@ _SIGS = {
     "fa_pool_collect": ([_P, _P, C.c_int, _P], C.c_int),
     "fa_pool_population": ([_P, C.c_int64, _P, _P, _P], C.c_int),
     "fa_pool_set_max_running": ([_P, C.c_int], C.c_int),
+    "fa_pool_busy": ([_P, _P], C.c_int),
     "fa_pool_destroy": ([_P], C.c_int),
     "fa_ga_population": ([_P, _P, _P, _P], C.c_int),
     "fa_ga_thermo": ([_P, _P, _P], C.c_int),
@@ -434,6 +435,12 @@ class Pool:
 
     def set_max_running(self, n: int):
         _check(_lib.fa_pool_set_max_running(self._pool, int(n)))
+
+    def busy(self) -> bool:
+        """True while the last step still runs on the device (never waits)."""
+        b = C.c_int()
+        _check(_lib.fa_pool_busy(self._pool, C.byref(b)))
+        return bool(b.value)
 
     def population(self, job_id: int):
         pop = self.params.population

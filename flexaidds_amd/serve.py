@@ -3,10 +3,11 @@
 Each receptor is uploaded once into its own context (resident in HBM with
 its ligand library) and gets a docking pool (fa_pool_*) whose slot count
 follows from an HBM budget.  The loop admits requests as their arrival time
-passes, steps every pool that has work (each step enqueues one generation of
-all of that pool's resident jobs and returns), then collects the finished
-jobs of every pool, so the pools' launches overlap on the device and jobs
-that arrive while others run join at the next generation.
+passes and steps every pool that has work as soon as that pool's previous
+step has ended (each step enqueues one generation of all of the pool's
+resident jobs on the pool's stream and returns); finished jobs are collected
+then.  The pools' launches overlap on the device, and jobs that arrive while
+others run join their pool at its next generation.
 
 There is no CPU fallback: the pools run through the HIP library.
 """
@@ -62,14 +63,20 @@ class Server:
     def run(self, requests, realtime=True, on_step=None) -> Served:
         """Serve `requests` (dicts with t, receptor, ligand, generations, seed)
         in arrival order.  With realtime, a request is admitted once the wall
-        clock passes its arrival time; otherwise all arrive at once."""
+        clock passes its arrival time; otherwise all arrive at once.
+
+        Each pool is stepped as soon as its own previous step has ended (a
+        non-blocking query), so one pool's slow launch does not hold the
+        others back and the pools' launches keep overlapping on the device."""
         out = Served()
         order = sorted(range(len(requests)), key=lambda i: requests[i]["t"])
         nxt = 0
         owner = {}                                   # (pool, job id) -> request index
-        busy = [0] * len(self.pools)                 # resident + queued jobs per pool
+        P = len(self.pools)
+        busy = [0] * P                               # resident + queued jobs per pool
+        inflight = [False] * P                       # a step enqueued and not yet collected
         t0 = time.perf_counter()
-        while nxt < len(order) or any(busy):
+        while nxt < len(order) or any(busy) or any(inflight):
             now = time.perf_counter() - t0
             while nxt < len(order) and (not realtime or requests[order[nxt]]["t"] <= now):
                 i = order[nxt]
@@ -79,22 +86,31 @@ class Server:
                 owner[(p, jid)] = i
                 busy[p] += 1
                 nxt += 1
-            stepped = [p for p in range(len(self.pools)) if busy[p]]
-            if not stepped:
+            progressed = False
+            for p in range(P):
+                if inflight[p]:
+                    if self.pools[p].busy():
+                        continue
+                    for res in self.pools[p].collect():
+                        i = owner.pop((p, res["id"]))
+                        done = time.perf_counter() - t0
+                        out.results[i] = res
+                        out.latency_s[i] = done - (requests[i]["t"] if realtime else 0.0)
+                        out.poses += res["n_poses"]
+                        busy[p] -= 1
+                    inflight[p] = False
+                    progressed = True
+                if busy[p] > 0:
+                    self.pools[p].step()
+                    inflight[p] = True
+                    out.steps += 1
+                    progressed = True
+            if progressed:
+                if on_step:
+                    on_step(out)
+            elif not any(inflight) and nxt < len(order):
                 time.sleep(max(0.0, requests[order[nxt]]["t"] - (time.perf_counter() - t0)))
-                continue
-            for p in stepped:
-                self.pools[p].step()
-            out.steps += 1
-            for p in stepped:
-                for res in self.pools[p].collect():
-                    i = owner.pop((p, res["id"]))
-                    done = time.perf_counter() - t0
-                    out.results[i] = res
-                    out.latency_s[i] = done - (requests[i]["t"] if realtime else 0.0)
-                    out.poses += res["n_poses"]
-                    busy[p] -= 1
-            if on_step:
-                on_step(out)
+            else:
+                time.sleep(0.0002)
         out.wall_s = time.perf_counter() - t0
         return out

@@ -297,6 +297,10 @@ int fa_pool_collect(fa_pool* pool, fa_job_result* out, int max, int* n);
 int fa_pool_population(fa_pool* pool, int64_t job_id, int32_t* genes, double* ic, fa_cf_result* results);
 /* Changes the cap on resident jobs for later admissions (0: every slot). */
 int fa_pool_set_max_running(fa_pool* pool, int max_running);
+/* *busy = 1 while the last step still runs on the device, else 0; never
+ * waits (lets a host loop step each pool as soon as its own step ends).
+ * No counterpart in the reference (its GA loop is synchronous). */
+int fa_pool_busy(fa_pool* pool, int* busy);
 int fa_pool_destroy(fa_pool* pool);
 
 /* Size of the kernel's per-wave LDS workspace (bytes). */

@@ -190,3 +190,30 @@ def test_server_results_match_pools_alone(hip):
     s = got.summary()
     assert s["requests"] == len(reqs) and s["poses"] == sum(64 + r["generations"] * 64 for r in reqs)
     srv.close()
+
+
+def test_busy_query(hip, ctx, limits):
+    # the non-blocking query reports a step in flight until it ends, never
+    # waits, and stepping after it went idle gives the usual result
+    import time
+    pool = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=2)
+    assert not pool.busy()
+    jid = pool.submit(1, limits[1], 2, 77)
+    out = []
+    t0 = time.time()
+    for _ in range(20):
+        running, queued = pool.step()
+        while pool.busy():
+            assert time.time() - t0 < 60
+            time.sleep(0.0005)
+        out += pool.collect()
+        if running == 0 and queued == 0:
+            break
+    assert not pool.busy()
+    pool.close()
+    alone = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=1)
+    alone.submit(1, limits[1], 2, 77)
+    ref = _drain(alone)
+    alone.close()
+    assert [r["id"] for r in out] == [jid]
+    _same(out[0], ref[0])
