@@ -73,23 +73,26 @@ def test_initial_population(hip, ctx, system, limits, orc, oracle):
     ga.close()
 
 
-def test_generation_matches_oracle(hip, ctx, system, limits, orc, oracle):
-    ga = hip.GA(ctx, limits, population=POP, offspring=NEW, seed=SEED)
+@pytest.mark.parametrize("pop,new", [(POP, NEW), (5000, 5000)])
+def test_generation_matches_oracle(hip, ctx, system, limits, orc, oracle, pop, new):
+    # (5000, 5000): the roulette narrows 5,000 rows in three rounds and the
+    # ranking compares 10,000 candidates in 40 chunks
+    ga = hip.GA(ctx, limits, population=pop, offspring=new, seed=SEED)
     g0, ic0, r0 = ga.population()
     ga.run(1, graph=False)
     g1, ic1, r1 = ga.population()
     assert ga.generation() == 1
-    child, cic = ol.ga_breed(orc, SEED, 1, g0[:POP], NEW, limits)
-    assert np.array_equal(g1[POP:], child)
-    assert np.array_equal(ic1[POP:], cic)
+    child, cic = ol.ga_breed(orc, SEED, 1, g0[:pop], new, limits)
+    assert np.array_equal(g1[pop:], child)
+    assert np.array_equal(ic1[pop:], cic)
     ref = _cf_oracle(oracle, system, cic)
-    assert np.array_equal(r1["status"][POP:], ref["status"])
-    assert np.all(np.abs(r1["cf"][POP:] - ref["cf"]) <= tol(ref["cf"]))
+    assert np.array_equal(r1["status"][pop:], ref["status"])
+    assert np.all(np.abs(r1["cf"][pop:] - ref["cf"]) <= tol(ref["cf"]))
     # survivors: stable ranking of parents + children by the GPU energies
-    energy = np.concatenate([r0["cf"][:POP], r1["cf"][POP:]])
-    pool = np.concatenate([g0[:POP], child])
-    order = ol.ga_survivors(orc, energy, POP)
-    assert np.array_equal(g1[:POP], pool[order])
+    energy = np.concatenate([r0["cf"][:pop], r1["cf"][pop:]])
+    pool = np.concatenate([g0[:pop], child])
+    order = ol.ga_survivors(orc, energy, pop)
+    assert np.array_equal(g1[:pop], pool[order])
     assert r1["cf"][0] <= r0["cf"][0]
     ga.close()
 
