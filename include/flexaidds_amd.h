@@ -134,10 +134,11 @@ int fa_last_counters(fa_ctx* ctx, uint32_t* slow_grid, uint32_t* capacity);
  * the algorithmic FLOP count of that launch.  Synchronises the device. */
 int fa_last_opcount(fa_ctx* ctx, fa_opcount* out);
 
-/* Diagnostics: when enabled, launches sum the clock cycles each wave spends
- * in the kernel's phases (pose build, pre-pass, neighbour cull, first vertex,
- * hull walk, arc points, seeds, face ordering, areas, records) into 16
- * counters read by fa_last_phase_cycles (synchronises the device). */
+/* Diagnostics (no counterpart in the reference; for profiling the kernel):
+ * when enabled, launches sum the clock cycles each wave spends in the
+ * kernel's phases (pose build, pre-pass, neighbour cull, first vertex, hull
+ * walk, arc points, seeds, face ordering, areas, records) into 16 counters
+ * read by fa_last_phase_cycles (synchronises the device). */
 int fa_set_phase_profiling(fa_ctx* ctx, int enable);
 int fa_last_phase_cycles(fa_ctx* ctx, uint64_t* cycles16);
 /* Per-pose clock cycles of the last profiled launch (n <= its poses); the
