@@ -279,11 +279,11 @@ constexpr int PROF_POSE0 = 16;  // per-pose cycles follow the 16 phase slots
 struct PhaseClock {
     unsigned long long* acc;  // the wave's LDS slots, null when disabled
     unsigned long long t;
-    __device__ void start()
+    __device__ __forceinline__ void start()
     {
         if (acc) t = __builtin_readcyclecounter();
     }
-    __device__ void stop(int ph)
+    __device__ __forceinline__ void stop(int ph)
     {
         if (acc) {
             const unsigned long long now = __builtin_readcyclecounter();
