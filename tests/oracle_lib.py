@@ -17,6 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libfa_oracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libfa_ref.so")
 REF_FAST_SO = os.path.join(ROOT, "oracle", "_ref", "libfa_ref_fast.so")
+REF_READER_SO = os.path.join(ROOT, "oracle", "_ref", "libfa_ref_reader.so")
 
 _P = C.c_void_p
 
@@ -68,6 +69,41 @@ def load_ref(fast=False):
     lib.fa_ref_eval.argtypes = [_P, _P, _P, C.c_int, _P, _P, _P, _P, C.c_int, _P]
     lib.fa_ref_eval.restype = C.c_int
     return lib
+
+
+def load_ref_reader():
+    if not os.path.exists(REF_READER_SO):
+        return None
+    lib = C.CDLL(REF_READER_SO)
+    lib.fa_ref_read_receptor.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
+                                         _P, _P, _P, _P, _P, _P, _P, _P]
+    lib.fa_ref_read_receptor.restype = C.c_int
+    return lib
+
+
+def ref_read_receptor(lib, pdb, amino_def, nucl_def, tmp_path, ntypes=40, exclude_het=0, remove_water=1,
+                      keep_ions=0, keep_structural_waters=0, structural_water_bfactor_max=20.0,
+                      max_atoms=200000):
+    """The reference's receptor-file chain on a PDB file (oracle/ref/ref_reader.cpp)."""
+    n = max_atoms
+    xyz = np.zeros((n, 3), np.float32)
+    rad = np.zeros(n, np.float32)
+    typ, num, ofres, nbond = (np.zeros(n, np.int32) for _ in range(4))
+    isp = np.zeros(1, np.int32)
+    cen = np.zeros(11, np.float32)
+    r = lib.fa_ref_read_receptor(pdb.encode(), amino_def.encode(), nucl_def.encode(), tmp_path.encode(),
+                                 ntypes, exclude_het, remove_water, keep_ions, keep_structural_waters,
+                                 structural_water_bfactor_max, n, _ptr(xyz), _ptr(rad), _ptr(typ),
+                                 _ptr(num), _ptr(ofres), _ptr(nbond), _ptr(isp), _ptr(cen))
+    if r < 0:
+        raise ValueError("more atoms than max_atoms")
+    return dict(xyz=xyz[:r], radius=rad[:r], type=typ[:r], number=num[:r], ofres=ofres[:r] - 1,
+                nbond=nbond[:r], is_protein=bool(isp[0]), center=cen)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
 
 
 def _addr(s):

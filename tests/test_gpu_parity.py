@@ -446,3 +446,24 @@ def test_largest_ligand_against_oracle(hip, oracle):
         ref = ol.oracle_eval(oracle, d, 0, s.genes[0])
     assert caps == 0
     _bit_level_parity(out, ref, min_ok=5)
+
+
+def test_rna_receptor_file_against_oracle(hip, oracle):
+    """The reference's RNA target as its own reader chain reads it (real
+    geometry, bonded receptor atoms 1.2-1.6 A apart), a generated ligand in
+    three of its grooves, 800 poses each."""
+    from file_systems import rna_receptor_system
+    n_ok = 0
+    for groove in range(3):
+        s = rna_receptor_system(41 + groove, groove=groove, n_poses=800, spread=3.0)
+        d = abi.Descriptors(s.receptor, s.ligands, s.params)
+        with hip.Context() as c:
+            c.upload_system(s)
+            out = c.eval(0, s.genes[0])
+            caps = c.counters()["capacity"]
+        with ol.trig_mode(oracle, 1):
+            ref = ol.oracle_eval(oracle, d, 0, s.genes[0])
+        assert caps == 0
+        _bit_level_parity(out, ref, min_ok=5)
+        n_ok += int((ref["status"] == 0).sum())
+    assert n_ok >= 60

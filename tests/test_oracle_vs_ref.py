@@ -108,3 +108,19 @@ def test_strict_reference_bit_exact_constraints(oracle):
 
 def ref_eval_cons(desc, genes):
     return ol.ref_eval(ref, desc, 0, genes)
+
+
+def test_strict_reference_bit_exact_rna_receptor_file(oracle):
+    """The reference's RNA target as its reader chain reads it, with a
+    generated ligand in two of its grooves."""
+    from file_systems import rna_receptor_system
+    for groove in (0, 1):
+        s = rna_receptor_system(21 + groove, groove=groove, n_poses=400)
+        desc = abi.Descriptors(s.receptor, s.ligands, s.params)
+        r_out, _, _ = ol.ref_eval(ref, desc, 0, s.genes[0])
+        with ol.trig_mode(oracle, 0):
+            out = ol.oracle_eval(oracle, desc, 0, s.genes[0])
+        assert (r_out["status"] == 0).sum() >= 10
+        assert np.array_equal(out["status"], r_out["status"])
+        for f in ("com", "wal", "sas", "totsas", "cf", "rclash"):
+            assert np.array_equal(out[f], r_out[f]), f
