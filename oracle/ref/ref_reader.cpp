@@ -76,3 +76,65 @@ extern "C" int fa_ref_read_receptor(const char* pdb, const char* amino_def, cons
     center[10] = FA->maxwidth;
     return n;
 }
+
+/*
+ * The reference's automatic binding-site grid (LIB/top.cpp:904-916):
+ * detect_cleft over the receptor atoms (1-based array, atm_cnt_real = n),
+ * generate_grid over its spheres, calc_cleftic in the ori frame.  Built
+ * without OpenMP, so the probes come in the serial branch's order.
+ * Returns the number of grid points (cleftgrid[1 .. num_grd-1] copied out),
+ * -1 when no cleft is found, -2 when the output arrays are too small.
+ */
+#include "CleftDetector.h"
+
+#include <vector>
+
+gridpoint* generate_grid(FA_Global* FA, sphere* spheres, atom* atoms, resid* residue);
+void calc_cleftic(FA_Global* FA, gridpoint* cleftgrid);
+
+extern "C" int fa_ref_cleft_grid(const float* xyz, const float* radius, int n, const float* ori, float spacer,
+                                 int max_points, float* points, float* ic, int max_spheres, float* spheres,
+                                 int* n_spheres)
+{
+    std::vector<atom> atoms(n + 1);
+    std::memset(atoms.data(), 0, sizeof(atom) * (n + 1));
+    for (int i = 1; i <= n; ++i) {
+        for (int j = 0; j < 3; ++j) atoms[i].coor[j] = xyz[3 * (i - 1) + j];
+        atoms[i].radius = radius[i - 1];
+    }
+    std::vector<resid> residue(2);
+    std::memset(residue.data(), 0, sizeof(resid) * 2);
+    int gpa[3] = {1, 1, 1};
+    residue[1].gpa = gpa;
+    FA_Global* FA = new FA_Global();
+    FA->res_cnt = 1;
+    FA->MIN_CLEFTGRID_POINTS = 100;
+    FA->spacer_length = spacer;
+    for (int j = 0; j < 3; ++j) FA->ori[j] = ori[j];
+
+    sphere* sph = detect_cleft(atoms.data(), residue.data(), n, 1);
+    if (!sph) return -1;
+    int ns = 0;
+    for (sphere* s = sph; s; s = s->prev, ++ns) {
+        if (ns < max_spheres) {
+            spheres[4 * ns + 0] = s->center[0];
+            spheres[4 * ns + 1] = s->center[1];
+            spheres[4 * ns + 2] = s->center[2];
+            spheres[4 * ns + 3] = s->radius;
+        }
+    }
+    *n_spheres = ns;
+    gridpoint* g = generate_grid(FA, sph, atoms.data(), residue.data());
+    calc_cleftic(FA, g);
+    free_sphere_list(sph);
+    const int ng = FA->num_grd - 1;
+    if (ng > max_points || ns > max_spheres) return -2;
+    for (int i = 1; i <= ng; ++i) {
+        for (int j = 0; j < 3; ++j) points[3 * (i - 1) + j] = g[i].coor[j];
+        ic[3 * (i - 1) + 0] = g[i].dis;
+        ic[3 * (i - 1) + 1] = g[i].ang;
+        ic[3 * (i - 1) + 2] = g[i].dih;
+    }
+    free(g);
+    return ng;
+}

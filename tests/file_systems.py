@@ -30,3 +30,22 @@ def rna_receptor_system(seed, groove=0, n_atoms=30, n_tors=6, n_poses=600, sprea
     params = synth.Params(ntypes=40, emat=synth.make_emat(rng))
     return synth.System(receptor=rec, ligands=[lig], params=params,
                         genes=[synth.random_genes(rng, lig, n_poses)])
+
+
+def rna_cleft_system(seed, n_atoms=30, n_tors=6, n_poses=1000):
+    """The same RNA target with the reference's automatic binding-site grid
+    (gap-sphere cleft, grid at 0.375 A, internal coordinates as calc_cleftic
+    computes them) as the ligand's translation genes."""
+    from flexaidds_amd import cleft
+    g = np.load(GOLDEN)
+    rng = np.random.default_rng(seed)
+    ori = g["center"][:3].astype(np.float32)
+    cg = cleft.cleft_grid(g["xyz"], g["radius"], ori)
+    anchor = cg.points.astype(np.float64).mean(axis=0)
+    rec = synth.Receptor(xyz=g["xyz"].copy(), radius=g["radius"].copy(), type=g["type"].copy(), center=anchor)
+    lig = synth.make_ligand(rng, n_atoms, n_tors, ori, anchor, cg.points)
+    lig.grid_ic[1:] = cg.ic                         # entries 1..m exactly as the reference's
+    synth.finalize_frame(lig, rec)
+    params = synth.Params(ntypes=40, emat=synth.make_emat(rng))
+    return synth.System(receptor=rec, ligands=[lig], params=params,
+                        genes=[synth.random_genes(rng, lig, n_poses)])

@@ -467,3 +467,20 @@ def test_rna_receptor_file_against_oracle(hip, oracle):
         _bit_level_parity(out, ref, min_ok=5)
         n_ok += int((ref["status"] == 0).sum())
     assert n_ok >= 60
+
+
+def test_rna_cleft_grid_against_oracle(hip, oracle):
+    """The RNA target with the reference's automatic binding-site grid
+    (9,043 cleft points, internal coordinates as calc_cleftic computes them)
+    as the translation genes: 1,000 poses over the whole cleft."""
+    from file_systems import rna_cleft_system
+    s = rna_cleft_system(51)
+    d = abi.Descriptors(s.receptor, s.ligands, s.params)
+    with hip.Context() as c:
+        c.upload_system(s)
+        out = c.eval(0, s.genes[0])
+        caps = c.counters()["capacity"]
+    with ol.trig_mode(oracle, 1):
+        ref = ol.oracle_eval(oracle, d, 0, s.genes[0])
+    assert caps == 0
+    _bit_level_parity(out, ref, min_ok=100)

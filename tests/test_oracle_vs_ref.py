@@ -124,3 +124,17 @@ def test_strict_reference_bit_exact_rna_receptor_file(oracle):
         assert np.array_equal(out["status"], r_out["status"])
         for f in ("com", "wal", "sas", "totsas", "cf", "rclash"):
             assert np.array_equal(out[f], r_out[f]), f
+
+
+def test_strict_reference_bit_exact_rna_cleft_grid(oracle):
+    """The RNA target with the reference's automatic binding-site grid."""
+    from file_systems import rna_cleft_system
+    s = rna_cleft_system(52, n_poses=300)
+    desc = abi.Descriptors(s.receptor, s.ligands, s.params)
+    r_out, _, _ = ol.ref_eval(ref, desc, 0, s.genes[0])
+    with ol.trig_mode(oracle, 0):
+        out = ol.oracle_eval(oracle, desc, 0, s.genes[0])
+    assert (r_out["status"] == 0).sum() >= 20
+    assert np.array_equal(out["status"], r_out["status"])
+    for f in ("com", "wal", "sas", "totsas", "cf", "rclash"):
+        assert np.array_equal(out[f], r_out[f]), f
