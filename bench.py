@@ -20,6 +20,9 @@ path (one ligand population per launch).  Beside it:
   grand     the blind dock of 64 competing ligands on a 20,000-atom globule
             (32,768 chromosomes each, one batched GA) and the grand partition
             function over them; a few generations timed, 300 projected
+  real_inputs  the plain launch and a device-GA dock on a receptor read from
+            a PDB file (the reference's RNA target) with the automatic
+            binding-site grid built from it
 
 The roofline numbers come from the kernel's own operation counters (the
 algorithmic FLOP model, see include/fa_types.h fa_opcount) and HIP-event
@@ -192,6 +195,62 @@ def grand_leg(args):
             "max_occupancy": round(float(np.max(gpf["p_bound"])), 6)}
 
 
+def real_leg(args):
+    """The plain launch on real inputs: the reference's RNA target as its own
+    reader chain reads it (tests/golden/receptor_guanine.npz, 1,097 atoms),
+    its automatic binding-site grid (gap-sphere cleft, 9,043 points at
+    0.375 A) as the translation genes, a 45-atom / 10-torsion ligand,
+    population --pop per launch; then a device-GA dock of --real-generations."""
+    import torch
+    from flexaidds_amd import abi, cleft, hip, synth
+    g = np.load(os.path.join(ROOT, "tests", "golden", "receptor_guanine.npz"))
+    ori = g["center"][:3].astype(np.float32)
+    cg = cleft.cleft_grid(g["xyz"], g["radius"], ori)
+    anchor = cg.points.astype(np.float64).mean(axis=0)
+    rng = np.random.default_rng(3003)
+    rec = synth.Receptor(xyz=g["xyz"].copy(), radius=g["radius"].copy(), type=g["type"].copy(), center=anchor)
+    lig = synth.make_ligand(rng, 45, 10, ori, anchor, cg.points)
+    lig.grid_ic[1:] = cg.ic
+    synth.finalize_frame(lig, rec)
+    system = synth.System(receptor=rec, ligands=[lig], params=synth.Params(ntypes=40, emat=synth.make_emat(rng)),
+                          genes=[])
+    log(f"real inputs: {rec.n_atoms}-atom RNA target, {len(cg.points)} cleft points")
+    G = lig.n_genes
+    n = args.real_steps + 2
+    pops = np.stack([synth.random_genes(rng, lig, args.pop) for _ in range(n)])
+    ctx = hip.Context(max_pop=args.pop)
+    ctx.upload_system(system)
+    d_genes = torch.from_numpy(pops).cuda()
+    d_out = torch.zeros((n, args.pop * abi.CF_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    for k in range(2):
+        ctx.eval_device(args.pop, d_genes[k].data_ptr(), G, 0, d_out[k].data_ptr(), sh)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for k in range(2, n):
+        ctx.eval_device(args.pop, d_genes[k].data_ptr(), G, 0, d_out[k].data_ptr(), sh)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1)
+    res = np.frombuffer(d_out[2:].cpu().numpy().tobytes(), abi.CF_DTYPE)
+    ga = hip.GA(ctx, synth.ga_limits(lig), population=args.pop, offspring=args.pop, seed=77, temperature=300.0)
+    ga_ms = ga.run(args.real_generations, graph=True)
+    _, _, best = ga.population()
+    ga.close()
+    caps = ctx.counters()["capacity"]
+    ctx.close()
+    return {"receptor_atoms": rec.n_atoms, "cleft_points": int(len(cg.points)), "cleft_spheres": int(len(cg.spheres)),
+            "ligand_atoms": lig.n_atoms, "population": args.pop, "launches": args.real_steps,
+            "ms_per_launch": round(ms / args.real_steps, 4),
+            "poses_per_s": round(args.real_steps * args.pop / (ms / 1e3), 1),
+            "scored_fraction": round(float((res["status"] == 0).mean()), 4), "capacity_overflows": caps,
+            "ga_generations": args.real_generations,
+            "ga_ms_per_generation": round(ga_ms / max(args.real_generations, 1), 4),
+            "ga_best_cf": round(float(best["cf"][0]), 6)}
+
+
 def served_leg(args):
     """The served stream, and the plain path on the same kind of requests.
 
@@ -262,6 +321,8 @@ def main():
     ap.add_argument("--grand-ligands", type=int, default=64, help="config-4 blind dock ligands (0: skip)")
     ap.add_argument("--grand-pop", type=int, default=32768)
     ap.add_argument("--grand-generations", type=int, default=2)
+    ap.add_argument("--real-steps", type=int, default=20, help="launches on the RNA target's cleft (0: skip)")
+    ap.add_argument("--real-generations", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true",
                     help="only time one launch per population size (1 .. 16,384 poses of the plain workload) and exit")
@@ -350,6 +411,7 @@ def main():
     batched = batched_leg(args) if args.batched_ligands > 0 else None
     served = served_leg(args) if args.served_requests > 0 else None
     grand = grand_leg(args) if args.grand_ligands > 0 else None
+    real = real_leg(args) if args.real_steps > 0 else None
 
     cpu = None if args.no_cpu else cpu_baseline(system, 0, pops.reshape(-1, G))
     line = {
@@ -380,6 +442,7 @@ def main():
         "batched": batched,
         "served": served,
         "grand": grand,
+        "real_inputs": real,
         "status_counts": {"ok": int(status[0]), "hull_fail": int(status[1]), "clash": int(status[2]),
                           "capacity": int(status[3]), "grid_range": int(status[4])},
         "capacity_overflows": cap["capacity"],
