@@ -1460,7 +1460,18 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
         if (l == 0) pose = (int)atomicAdd(&B.counters[2], 1u);
         pose = __builtin_amdgcn_readfirstlane(pose);
         if (pose >= B.n_poses) break;
-        const int li = B.pose_lig ? B.pose_lig[pose] : 0;
+        const int li = uni(B.pose_lig ? B.pose_lig[pose] : 0);
+        if (li < 0 || li >= B.n_ligands) {  // a ligand the context does not hold: not scored
+            if (l == 0) {
+                fa_cf_result bad{};
+                bad.cf = bad.app = __longlong_as_double(0x7ff0000000000000ll);
+                bad.status = FA_BAD_LIGAND;
+                B.out[pose] = bad;
+                if (B.rec_count) B.rec_count[pose] = 0;
+                atomicAdd(&B.totals[3], 1ull);
+            }
+            continue;
+        }
         const DevLigand lig = ligs[li];
         const DevGrid G = grids[lig.grid_id];
         const int L = lig.n_atoms;
@@ -1487,7 +1498,10 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
         }
         PoseGrid pg;
         pose_grid(s, lig, G, pg);
-        if (pg.slow && l == 0) atomicAdd(&B.counters[0], 1u);
+        if (pg.slow && l == 0) {
+            atomicAdd(&B.counters[0], 1u);
+            atomicAdd(&B.totals[0], 1ull);
+        }
         // cells of the ligand atoms and the processing order (cell, calc index)
         int lb[LSL];
         bool bad = false;
@@ -1593,10 +1607,12 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
             if (l == 0) s.nstart[L] = acc.n_near;
             count_op(s.ops, OP_CULL, pre_tested);
             clash = wave_sum_d(acc.clash);
-            if (acc.over)
-                status = FA_CAPACITY;
-            else if (clash >= CLASH_THRESHOLD)
+            // the clash sum is complete even when the neighbour list overflowed
+            // (every candidate is tested), so a clash is decided first
+            if (clash >= CLASH_THRESHOLD)
                 status = FA_CLASH;
+            else if (acc.over)
+                status = FA_CAPACITY;
         }
         wave_sync();
         clk.stop(PH_PREPASS);
@@ -2643,8 +2659,16 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
             res.wal = (double)POLY_PENALTY;
             res.cf = res.wal + 1.0;
             res.app = res.wal + 1.0;
+        } else {
+            // not scored (a device capacity or an atom outside the cell grid):
+            // +inf ranks after every scored pose and weighs 0 in ensembles
+            res.cf = res.app = __longlong_as_double(0x7ff0000000000000ll);
         }
-        if (status == FA_CAPACITY && l == 0) atomicAdd(&B.counters[1], 1u);
+        if (status == FA_CAPACITY && l == 0) {
+            atomicAdd(&B.counters[1], 1u);
+            atomicAdd(&B.totals[1], 1ull);
+        }
+        if (status == FA_GRID_RANGE && l == 0) atomicAdd(&B.totals[2], 1ull);
         wave_sync();
         if (B.ops && l < FA_OPCOUNT_N && s.ops[l]) atomicAdd(&B.ops[l], s.ops[l]);
         if (l == 0) {

@@ -134,7 +134,9 @@ int fa_launch_poses(fa_ctx* c, int n_poses, const double* d_genes, int stride, c
     B.rec_count = d_reccnt;
     B.lig_xyz = d_xyz;
     B.xyz_stride = xyz_stride;
+    B.n_ligands = c->n_ligands;
     B.counters = static_cast<unsigned int*>(c->counters.p);
+    B.totals = static_cast<unsigned long long*>(c->totals.p);
     B.ops = static_cast<unsigned long long*>(c->ops.p);
     FA_HIP(fa::launch_zero_u32(static_cast<unsigned int*>(c->counters.p), 4, stream));
     FA_HIP(fa::launch_zero_u32(static_cast<unsigned int*>(c->ops.p), (int)(sizeof(fa_opcount) / 4), stream));
@@ -205,9 +207,14 @@ int fa_ctx_create(int device_id, int max_pop, fa_ctx** out)
         (rc = fa_ensure(c->ops, sizeof(fa_opcount))) != FA_SUCCESS ||
         (rc = fa_ensure(c->prof, 16 * sizeof(unsigned long long))) != FA_SUCCESS ||
         (rc = fa_ensure(c->genes, sizeof(double) * (size_t)c->max_pop * 32)) != FA_SUCCESS ||
-        (rc = fa_ensure(c->out, sizeof(fa_cf_result) * (size_t)c->max_pop)) != FA_SUCCESS) {
+        (rc = fa_ensure(c->out, sizeof(fa_cf_result) * (size_t)c->max_pop)) != FA_SUCCESS ||
+        (rc = fa_ensure(c->totals, 4 * sizeof(unsigned long long))) != FA_SUCCESS) {
         delete c;
         return rc;
+    }
+    if (hipMemset(c->totals.p, 0, 4 * sizeof(unsigned long long)) != hipSuccess) {
+        delete c;
+        return fa_fail(FA_ERR_HIP, "clear totals");
     }
     *out = c;
     return FA_SUCCESS;
@@ -220,7 +227,7 @@ int fa_ctx_destroy(fa_ctx* c)
     (void)hipDeviceSynchronize();
     c->free_system();
     for (DevBuf* b : {&c->genes, &c->out, &c->counters, &c->recs, &c->reccnt, &c->xyz, &c->scratch, &c->thermo,
-                      &c->ops, &c->prof, &c->near_idx, &c->near_xyz, &c->rbox})
+                      &c->ops, &c->prof, &c->near_idx, &c->near_xyz, &c->rbox, &c->totals})
         if (b->p) (void)hipFree(b->p);
     delete c;
     return FA_SUCCESS;
@@ -230,6 +237,8 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
                      const fa_ligand_desc* ligs)
 {
     if (!c || !r || !prm || !ligs || n_ligands < 1) return fa_fail(FA_ERR_INVALID, "null argument");
+    if (c->live_users > 0)
+        return fa_fail(FA_ERR_INVALID, "a GA or docking pool still uses the resident system; destroy it first");
     if (r->n_atoms < 1 || !r->xyz || !r->radius || !r->type) return fa_fail(FA_ERR_INVALID, "empty receptor");
     if (prm->ntypes < 1 || !prm->emat) return fa_fail(FA_ERR_INVALID, "energy matrix missing");
     const size_t npairs = (size_t)prm->ntypes * prm->ntypes;
@@ -250,6 +259,7 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
     FA_HIP(hipSetDevice(c->device));
     FA_HIP(hipDeviceSynchronize());
     c->free_system();
+    c->max_genes = 0;
     int rc;
     const int Rn = r->n_atoms;
     // receptor atoms (calc order): x, y, z, radius packed for one 16-byte load
@@ -395,6 +405,7 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
         }
         c->h_ligs[li] = D;
         c->lig_genes[li] = L.n_genes;
+        c->max_genes = std::max(c->max_genes, L.n_genes);
     }
     const fa::DevGrid* dg = nullptr;
     const fa::DevLigand* dl = nullptr;
@@ -427,7 +438,8 @@ int fa_eval_batch_device(fa_ctx* c, int n_poses, const double* genes, int gene_s
     if (!c) return fa_fail(FA_ERR_INVALID, "null context");
     if (!c->have_system) return fa_fail(FA_ERR_NOSYSTEM, "no system uploaded");
     if (n_poses < 0 || (n_poses > 0 && (!genes || !out))) return fa_fail(FA_ERR_INVALID, "bad batch");
-    if (gene_stride < 1) return fa_fail(FA_ERR_INVALID, "gene_stride must be positive");
+    if (gene_stride < c->max_genes)
+        return fa_fail(FA_ERR_INVALID, "gene_stride below the largest gene count of the uploaded ligands");
     FA_HIP(hipSetDevice(c->device));
     return fa_launch_poses(c, n_poses, genes, gene_stride, pose_lig, out, nullptr, 0, nullptr, nullptr, 0,
                   static_cast<hipStream_t>(stream));
@@ -566,7 +578,7 @@ int fa_boltzmann_device(fa_ctx* c, const double* energy, int stride, int64_t n, 
 
 int fa_log_sum_exp_device(fa_ctx* c, const double* values, int stride, int64_t n, double* out, void* stream)
 {
-    if (!c || !values || !out || stride < 1) return fa_fail(FA_ERR_INVALID, "bad log_sum_exp arguments");
+    if (!c || !out || stride < 1 || (n > 0 && !values)) return fa_fail(FA_ERR_INVALID, "bad log_sum_exp arguments");
     if (n < 1) {
         *out = -HUGE_VAL;
         return FA_SUCCESS;
@@ -589,6 +601,24 @@ int fa_grand_device(fa_ctx* c, const double* log_Z, int stride, const double* lo
     return FA_SUCCESS;
 }
 
+int fa_ctx_totals(fa_ctx* c, uint64_t* totals4)
+{
+    if (!c || !totals4) return fa_fail(FA_ERR_INVALID, "null argument");
+    FA_HIP(hipSetDevice(c->device));
+    FA_HIP(hipDeviceSynchronize());
+    FA_HIP(hipMemcpy(totals4, c->totals.p, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return FA_SUCCESS;
+}
+
+int fa_ctx_reset_totals(fa_ctx* c)
+{
+    if (!c) return fa_fail(FA_ERR_INVALID, "null context");
+    FA_HIP(hipSetDevice(c->device));
+    FA_HIP(hipDeviceSynchronize());
+    FA_HIP(hipMemset(c->totals.p, 0, 4 * sizeof(uint64_t)));
+    return FA_SUCCESS;
+}
+
 int fa_last_counters(fa_ctx* c, uint32_t* slow_grid, uint32_t* capacity)
 {
     if (!c) return fa_fail(FA_ERR_INVALID, "null context");
@@ -603,6 +633,8 @@ int fa_last_counters(fa_ctx* c, uint32_t* slow_grid, uint32_t* capacity)
 int fa_set_phase_profiling(fa_ctx* c, int enable)
 {
     if (!c) return fa_fail(FA_ERR_INVALID, "null context");
+    if (c->live_users > 0 && (enable != 0) != c->profile)
+        return fa_fail(FA_ERR_INVALID, "a GA or docking pool uses this context; destroy it before changing profiling");
     c->profile = enable != 0;
     return FA_SUCCESS;
 }

@@ -68,9 +68,16 @@ struct fa_ctx {
     int near_cap = 0;
     // pose buffers
     DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops, prof, near_idx, near_xyz, rbox;
+    DevBuf totals;  // 4 x u64 unscored-pose counts since the last reset (not cleared per launch)
+    int max_genes = 0;  // largest gene count of the uploaded ligands
     int prof_poses = 0;  // poses of the last profiled launch
     bool profile = false;
     std::vector<uint32_t> last_counters = std::vector<uint32_t>(4, 0);
+    // GAs and pools built on this context: their buffers and captured graphs
+    // hold device pointers into the resident system, so while any is alive
+    // the system may not be replaced (fa_upload_system) nor the profiling
+    // buffers regrown (fa_set_phase_profiling)
+    int live_users = 0;
 
     void free_system()
     {

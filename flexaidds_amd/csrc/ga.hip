@@ -107,6 +107,7 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     FA_HIP(hipSetDevice(c->device));
     fa_ga* ga = new fa_ga();
     ga->ctx = c;
+    ++c->live_users;  // released by fa_ga_destroy
     ga->p = *p;
     ga->jobs = n_jobs;
     ga->Gmax = Gmax;
@@ -348,6 +349,7 @@ int fa_ga_destroy(fa_ga* ga)
                       &ga->sig})
         if (b->p) (void)hipFree(b->p);
     if (ga->stream) (void)hipStreamDestroy(ga->stream);
+    --ga->ctx->live_users;
     delete ga;
     return FA_SUCCESS;
 }

@@ -469,12 +469,15 @@ __global__ __launch_bounds__(256) void ga_shared_fitness_kernel(GaDev g)
         double m = INFINITY;
         for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmin(m, g.res[parent_row(g, k, i)].cf);
         emin = -block_max(-m, sh);
+        // unscored poses (energy +inf) have weight 0, also when every
+        // survivor is one (no finite minimum)
+        auto bw = [&](double e) { return (e < INFINITY) ? exp(-g.beta * (e - emin)) : 0.0; };
         double z = 0.0;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) z += exp(-g.beta * (g.res[parent_row(g, k, i)].cf - emin));
+        for (int i = threadIdx.x; i < n; i += blockDim.x) z += bw(g.res[parent_row(g, k, i)].cf);
         Z = block_sum(z, sh);
         double mw = 0.0;
-        for (int i = threadIdx.x; i < n; i += blockDim.x)
-            mw = fmax(mw, exp(-g.beta * (g.res[parent_row(g, k, i)].cf - emin)) / Z);
+        if (Z > 0.0)
+            for (int i = threadIdx.x; i < n; i += blockDim.x) mw = fmax(mw, bw(g.res[parent_row(g, k, i)].cf) / Z);
         maxw = block_max(mw, sh);
         if (maxw <= 0.0) maxw = 1.0;
     }
@@ -482,7 +485,8 @@ __global__ __launch_bounds__(256) void ga_shared_fitness_kernel(GaDev g)
         const double ps = g.share[(size_t)k * n + i];
         double f;
         if (smfree) {
-            const double w = (Z > 0.0) ? exp(-g.beta * (g.res[parent_row(g, k, i)].cf - emin)) / Z : 0.0;
+            const double e = g.res[parent_row(g, k, i)].cf;
+            const double w = (Z > 0.0 && e < INFINITY) ? exp(-g.beta * (e - emin)) / Z : 0.0;
             const double rank = (double)(n - i) / (double)n;
             const double blended = (1.0 - g.ew) * rank + g.ew * (w / maxw);
             f = blended * (double)n / ps;

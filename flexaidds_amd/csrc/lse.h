@@ -42,8 +42,12 @@ __device__ __forceinline__ void lse_merge(LseState& a, const LseState& b)
     }
 }
 
+// A sample of log-weight -inf (an energy of +inf: a pose the evaluator could
+// not score) has weight exactly 0: it adds nothing to Z, <E> or <E^2> and is
+// skipped rather than folded as 0 * inf.
 __device__ __forceinline__ void lse_push(LseState& a, double lw, double e)
 {
+    if (lw == -INFINITY) return;
     if (a.s == 0.0) {
         a.m = lw;
         a.s = 1.0;

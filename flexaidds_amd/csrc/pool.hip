@@ -225,6 +225,7 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
     FA_HIP(hipSetDevice(c->device));
     fa_pool* pl = new fa_pool();
     pl->ctx = c;
+    ++c->live_users;  // released by fa_pool_destroy
     pl->p = *p;
     pl->slots = (int)slots;
     pl->Gmax = Gmax;
@@ -538,6 +539,7 @@ int fa_pool_destroy(fa_pool* pl)
     if (pl->h_lists) (void)hipHostFree(pl->h_lists);
     if (pl->h_out) (void)hipHostFree(pl->h_out);
     if (pl->stream) (void)hipStreamDestroy(pl->stream);
+    --pl->ctx->live_users;
     delete pl;
     return FA_SUCCESS;
 }

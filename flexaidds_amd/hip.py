@@ -62,6 +62,8 @@ _SIGS = {
     "fa_thermo_host": ([_P, _P, _P, C.c_int64, C.c_double, _P], C.c_int),
     "fa_last_counters": ([_P, _P, _P], C.c_int),
     "fa_last_opcount": ([_P, _P], C.c_int),
+    "fa_ctx_totals": ([_P, _P], C.c_int),
+    "fa_ctx_reset_totals": ([_P], C.c_int),
     "fa_upload_emat": ([_P, _P, C.c_int], C.c_int),
     "fa_set_phase_profiling": ([_P, C.c_int], C.c_int),
     "fa_last_phase_cycles": ([_P, _P], C.c_int),
@@ -205,6 +207,23 @@ class Context:
                                      C.c_void_p(stream) if stream else None))
         return {f: getattr(th, f) for f, _ in abi.Thermo._fields_}
 
+    def boltzmann_device(self, energy_ptr: int, stride: int, n: int, temperature: float, weights_ptr: int,
+                         want_log_z: bool = True, stream: int = 0):
+        """compute_boltzmann_batch on the device: weights[i] = exp(-beta (E_i - E_min))
+        into `weights_ptr` (device); returns ln Z when asked (then synchronous)."""
+        lz = C.c_double()
+        _check(_lib.fa_boltzmann_device(self._ctx, C.c_void_p(energy_ptr), stride, n, temperature,
+                                        C.c_void_p(weights_ptr), C.byref(lz) if want_log_z else None,
+                                        C.c_void_p(stream) if stream else None))
+        return lz.value if want_log_z else None
+
+    def log_sum_exp_device(self, values_ptr: int, stride: int, n: int, stream: int = 0) -> float:
+        """log_sum_exp_dispatch over device values (synchronous)."""
+        out = C.c_double()
+        _check(_lib.fa_log_sum_exp_device(self._ctx, C.c_void_p(values_ptr) if values_ptr else None, stride, n,
+                                          C.byref(out), C.c_void_p(stream) if stream else None))
+        return out.value
+
     def grand_device(self, log_z_ptr: int, stride: int, log_c_ptr: int, n: int, out_ptr: int, stream: int = 0):
         """Grand partition function on the device: out[0] = ln Xi, out[1] = p_empty,
         out[2 + i] = p_bound of ligand i (device pointers, asynchronous)."""
@@ -232,6 +251,15 @@ class Context:
         raw = np.zeros(n, np.uint64)
         _check(_lib.fa_last_pose_cycles(self._ctx, n, raw.ctypes.data))
         return (raw & np.uint64((1 << 63) - 1)).astype(np.int64), (raw >> np.uint64(63)).astype(bool)
+
+    def totals(self) -> dict:
+        """Unscored and widened-grid poses over every launch since the last reset."""
+        a = np.zeros(4, np.uint64)
+        _check(_lib.fa_ctx_totals(self._ctx, a.ctypes.data))
+        return {"slow_grid": int(a[0]), "capacity": int(a[1]), "grid_range": int(a[2]), "bad_ligand": int(a[3])}
+
+    def reset_totals(self):
+        _check(_lib.fa_ctx_reset_totals(self._ctx))
 
     def launch_waves(self) -> int:
         """Waves one pose launch runs (resident slots on the device)."""

@@ -36,13 +36,18 @@ enum fa_gene_type {
     FA_GENE_DIH = 2    /* dihedral; cascades along rec[3] shift chains */
 };
 
-/* Per-pose evaluation status */
+/* Per-pose evaluation status.  FA_OK, FA_HULL_FAIL and FA_CLASH carry the
+ * reference's energies (the clash and hull-failure cfstr of LIB/ic2cf.cpp:129
+ * with LIB/vcfunction.cpp:56-75).  The others mark poses the device did not
+ * score: their cf and app are +inf, so they rank after every scored pose in
+ * GA selection and carry Boltzmann weight 0 in every ensemble reduction. */
 enum fa_status {
     FA_OK = 0,
     FA_HULL_FAIL = 1,     /* Voronoi hull did not converge → POLYHEDRON_PENALTY */
-    FA_CLASH = 2,         /* clash pre-pass ≥ CLASH_THRESHOLD */
+    FA_CLASH = 2,         /* clash pre-pass ≥ CLASH_THRESHOLD (decided before any capacity check) */
     FA_CAPACITY = 3,      /* a device capacity (contacts/vertices/edges) overflowed */
-    FA_GRID_RANGE = 4     /* an atom fell outside the cell grid (undefined in the reference) */
+    FA_GRID_RANGE = 4,    /* an atom fell outside the cell grid (undefined in the reference) */
+    FA_BAD_LIGAND = 5     /* the pose names a ligand index the context does not hold */
 };
 
 /* Rigid receptor part (all non-scorable atoms). */

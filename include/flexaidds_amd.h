@@ -62,7 +62,12 @@ int fa_ctx_destroy(fa_ctx* ctx);
  * hip_eval.h has none; cuda_eval_init carries a partial one,
  * LIB/cuda_eval.cuh:46-56).  Receptor + scoring parameters + `n_ligands`
  * ligand topologies are copied to HBM and stay resident; receptor cell lists
- * are built once per distinct grid.  Replaces any previous system. */
+ * are built once per distinct grid.  Replaces any previous system.
+ * Lifetime rule: a GA (fa_ga_*) or docking pool (fa_pool_*) built on the
+ * context keeps device pointers into the resident system (also inside its
+ * captured HIP graph), so while one is alive this call fails with
+ * FA_ERR_INVALID and changes nothing; destroy them first.  The context itself
+ * must outlive them too. */
 int fa_upload_system(fa_ctx* ctx, const fa_receptor_desc* receptor, const fa_params* params, int n_ligands,
                      const fa_ligand_desc* ligands);
 
@@ -77,6 +82,9 @@ int fa_eval_batch(fa_ctx* ctx, int ligand, int n_poses, const double* genes, fa_
  * device pointers, the call only enqueues work on `stream` (a hipStream_t,
  * NULL = default stream).  `pose_lig` (may be NULL = ligand 0) selects the
  * ligand of each pose; rows of `genes` are `gene_stride` doubles apart.
+ * Preconditions: gene_stride >= the largest gene count of the uploaded
+ * ligands (checked: FA_ERR_INVALID otherwise); a pose_lig entry outside
+ * [0, n_ligands) is not read through but returned as status FA_BAD_LIGAND.
  * No reference counterpart (the reference path is synchronous per call). */
 int fa_eval_batch_device(fa_ctx* ctx, int n_poses, const double* genes, int gene_stride, const int* pose_lig,
                          fa_cf_result* out, void* stream);
@@ -130,6 +138,15 @@ int fa_grand_device(fa_ctx* ctx, const double* log_Z, int stride, const double* 
  * that hit a device capacity.  Synchronises the device. */
 int fa_last_counters(fa_ctx* ctx, uint32_t* slow_grid, uint32_t* capacity);
 
+/* Poses the device did not score (status FA_CAPACITY, FA_GRID_RANGE,
+ * FA_BAD_LIGAND) and poses that needed a widened cell grid, summed over every
+ * launch of the context since its creation or the last fa_ctx_reset_totals
+ * (single evaluations, GA generations, pool steps alike):
+ * totals4 = [slow_grid, capacity, grid_range, bad_ligand].  Synchronises the
+ * device.  No counterpart in the reference (it has no device limits). */
+int fa_ctx_totals(fa_ctx* ctx, uint64_t* totals4);
+int fa_ctx_reset_totals(fa_ctx* ctx);
+
 /* Operation counters (fa_opcount) summed over the last launch; they define
  * the algorithmic FLOP count of that launch.  Synchronises the device. */
 int fa_last_opcount(fa_ctx* ctx, fa_opcount* out);
@@ -139,6 +156,8 @@ int fa_last_opcount(fa_ctx* ctx, fa_opcount* out);
  * kernel's phases (pose build, pre-pass, neighbour cull, first vertex, hull
  * walk, arc points, seeds, face ordering, areas, records) into 16 counters
  * read by fa_last_phase_cycles (synchronises the device). */
+/* Changing the setting while a GA or pool of the context is alive fails with
+ * FA_ERR_INVALID (their captured graphs hold the profiling buffer). */
 int fa_set_phase_profiling(fa_ctx* ctx, int enable);
 int fa_last_phase_cycles(fa_ctx* ctx, uint64_t* cycles16);
 /* Per-pose clock cycles of the last profiled launch (n <= its poses); the

@@ -8,6 +8,9 @@
  *                           moments, F, S, Cv, σ_E)
  *   fa_oracle_boltzmann     LIB/UnifiedHardwareDispatch.cpp:730-747 and
  *                           LIB/statmech.cpp:178-201 (weights with multiplicity)
+ *   fa_oracle_boltzmann_batch  LIB/UnifiedHardwareDispatch.cpp:730-746, 768-794
+ *                           (compute_boltzmann_batch: unnormalised weights
+ *                           exp(−β(E−Emin)), log_Z = ln Σw − β Emin)
  *   fa_oracle_log_xi        LIB/GrandPartitionFunction.cpp:24-40, 112-160
  *
  * Parity status: pinned by closed forms (two-state and equal-energy
@@ -83,6 +86,31 @@ double fa_oracle_boltzmann(const double* energy, const int* count, int n, double
         for (int i = 0; i < n; ++i) w[i] = (zm > 0.0) ? (count ? count[i] : 1) * raw[i] / zm : 0.0;
     }
     return std::log(sum) - beta * emin;
+}
+
+/* compute_boltzmann_batch (LIB/UnifiedHardwareDispatch.cpp:768-794, the
+ * Eigen path :730-746) on n >= 1 energies at inverse temperature beta:
+ * w[i] = exp(−β (E_i − E_min)) (not normalised), *log_Z = ln Σ w − β E_min,
+ * *e_min = E_min.  n = 0 gives log_Z = 0, e_min = 0 as the reference's
+ * empty-span result does. */
+int fa_oracle_boltzmann_batch(const double* energy, int n, double beta, double* w, double* log_Z, double* e_min)
+{
+    if (n <= 0) {
+        if (log_Z) *log_Z = 0.0;
+        if (e_min) *e_min = 0.0;
+        return 0;
+    }
+    double emin = energy[0];
+    for (int i = 1; i < n; ++i) emin = std::min(emin, energy[i]);
+    double sum = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double wi = std::exp(-beta * (energy[i] - emin));
+        if (w) w[i] = wi;
+        sum += wi;
+    }
+    if (log_Z) *log_Z = std::log(sum) - beta * emin;
+    if (e_min) *e_min = emin;
+    return 0;
 }
 
 /* ln Ξ = LSE(0, ln c_i + ln Z_i); p_i = exp(ln c_i + ln Z_i − ln Ξ);

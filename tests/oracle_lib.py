@@ -56,6 +56,8 @@ def load_oracle():
     lib.fa_oracle_thermo.restype = C.c_int
     lib.fa_oracle_boltzmann.argtypes = [_P, _P, C.c_int, C.c_double, _P]
     lib.fa_oracle_boltzmann.restype = C.c_double
+    lib.fa_oracle_boltzmann_batch.argtypes = [_P, C.c_int, C.c_double, _P, _P, _P]
+    lib.fa_oracle_boltzmann_batch.restype = C.c_int
     lib.fa_oracle_log_xi.argtypes = [_P, _P, C.c_int, _P, _P]
     lib.fa_oracle_log_xi.restype = C.c_double
     return lib

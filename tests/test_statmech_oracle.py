@@ -124,3 +124,62 @@ def test_grand_partition(oracle):
     c1 = np.array([1.0])
     lx1 = oracle.fa_oracle_log_xi(one.ctypes.data, c1.ctypes.data, 1, pb.ctypes.data, C.byref(pe))
     assert abs(lx1 - math.log(1 + math.exp(10))) < 1e-10
+
+
+BETA300 = 1.0 / (KB * T)  # kB of LIB/statmech.h:26
+
+
+def boltz_batch(o, e, beta):
+    e = np.ascontiguousarray(e, np.float64)
+    w = np.zeros_like(e)
+    lz, em = C.c_double(), C.c_double()
+    assert o.fa_oracle_boltzmann_batch(e.ctypes.data, len(e), beta, w.ctypes.data, C.byref(lz), C.byref(em)) == 0
+    return w, lz.value, em.value
+
+
+def test_boltzmann_batch_two_state_closed_form(oracle):
+    # compute_boltzmann_batch returns exp(-beta (E - Emin)) unnormalised:
+    # the lowest state weighs exactly 1, the other exp(-beta dE)
+    w, lz, em = boltz_batch(oracle, [-9.0, -10.0], BETA300)
+    assert em == -10.0 and w[1] == 1.0
+    assert abs(w[0] - math.exp(-BETA300 * 1.0)) < 1e-15
+    assert abs(w[0] / w[1] - math.exp(-BETA300 * (-9.0 - -10.0))) < 1e-15
+    # ln Z = ln(exp(10 b) + exp(9 b)) = 10 b + log1p(exp(-b))
+    assert abs(lz - (10.0 * BETA300 + math.log1p(math.exp(-BETA300)))) < 1e-12
+    # the temperature enters only through beta = 1 / (kB T): kB = 0.001987206
+    assert abs(BETA300 - 1.677396975) < 1e-8
+
+
+def test_boltzmann_batch_single_and_extremes(oracle):
+    w, lz, em = boltz_batch(oracle, [-42.5], BETA300)
+    assert w[0] == 1.0 and abs(lz - 42.5 * BETA300) < 1e-12 and em == -42.5
+    # +-500 kcal/mol: the high state underflows to 0, ln Z stays the ground state's
+    w, lz, em = boltz_batch(oracle, [-500.0, 500.0, -499.0], BETA300)
+    assert w[0] == 1.0 and w[1] == 0.0 and abs(w[2] - math.exp(-BETA300)) < 1e-15
+    assert abs(lz - (500.0 * BETA300 + math.log1p(math.exp(-BETA300)))) < 1e-9
+    # empty span: the reference's {weights {}, log_Z 0, E_min 0}
+    w, lz, em = boltz_batch(oracle, np.zeros(0), BETA300)
+    assert lz == 0.0 and em == 0.0
+
+
+def test_boltzmann_weights_multiplicity_closed_form(oracle):
+    # StatMechEngine::boltzmann_weights: w_i = n_i exp(-b E_i) / sum_j n_j exp(-b E_j)
+    e = np.array([0.0, 1.0, 2.0])
+    n = np.array([1, 2, 3], np.int32)
+    w = np.zeros(3)
+    lz = oracle.fa_oracle_boltzmann(e.ctypes.data, n.ctypes.data, 3, T, w.ctypes.data)
+    b = BETA300
+    raw = np.array([1.0, 2.0 * math.exp(-b), 3.0 * math.exp(-2.0 * b)])
+    assert np.allclose(w, raw / raw.sum(), rtol=0, atol=1e-15)
+    # the returned ln Z is the multiplicity-free batch's (helmholtz, LIB/statmech.cpp:213-224)
+    assert abs(lz - math.log(1.0 + math.exp(-b) + math.exp(-2.0 * b))) < 1e-14
+    # without multiplicities every state weighs 1 x its factor
+    w1 = np.zeros(3)
+    oracle.fa_oracle_boltzmann(e.ctypes.data, None, 3, T, w1.ctypes.data)
+    raw1 = np.array([1.0, math.exp(-b), math.exp(-2.0 * b)])
+    assert np.allclose(w1, raw1 / raw1.sum(), rtol=0, atol=1e-15)
+    # the temperature is read as Kelvin with kB = 0.001987206: at 600 K the
+    # ratio of neighbouring states is exp(-b / 2)
+    w6 = np.zeros(3)
+    oracle.fa_oracle_boltzmann(e.ctypes.data, None, 3, 600.0, w6.ctypes.data)
+    assert abs(w6[1] / w6[0] - math.exp(-b / 2.0)) < 1e-14
