@@ -12,7 +12,12 @@ import numpy as np
 FA_MAX_LIG_ATOMS = 128
 FA_MAX_GENES = 64
 
-FA_OK, FA_HULL_FAIL, FA_CLASH, FA_CAPACITY, FA_GRID_RANGE = 0, 1, 2, 3, 4
+FA_MAX_FLEX_RESIDUES = 32
+FA_MAX_FLEX_ATOMS = 128
+FA_MAX_SCORABLE = 128
+
+FA_OK, FA_HULL_FAIL, FA_CLASH, FA_CAPACITY, FA_GRID_RANGE, FA_BAD_LIGAND = 0, 1, 2, 3, 4, 5
+FA_GENE_GRID, FA_GENE_DIS, FA_GENE_ANG, FA_GENE_DIH, FA_GENE_ROT = -1, 0, 1, 2, 4
 
 _f32p = C.POINTER(C.c_float)
 _i32p = C.POINTER(C.c_int32)
@@ -20,8 +25,14 @@ _f64p = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)
 
 
+class FlexResidue(C.Structure):
+    _fields_ = [("first", C.c_int32), ("n_atoms", C.c_int32), ("n_rot", C.c_int32), ("scorable", _u8p),
+                ("rot_xyz", _f32p), ("bonded", _u8p)]
+
+
 class ReceptorDesc(C.Structure):
-    _fields_ = [("n_atoms", C.c_int32), ("xyz", _f32p), ("radius", _f32p), ("type", _i32p)]
+    _fields_ = [("n_atoms", C.c_int32), ("xyz", _f32p), ("radius", _f32p), ("type", _i32p),
+                ("n_flex", C.c_int32), ("flex", C.POINTER(FlexResidue))]
 
 
 class Constraint(C.Structure):
@@ -94,6 +105,17 @@ class Descriptors:
         d.xyz = _ptr(self._arr(r.xyz.reshape(-1), np.float32), _f32p)
         d.radius = _ptr(self._arr(r.radius, np.float32), _f32p)
         d.type = _ptr(self._arr(r.type, np.int32), _i32p)
+        flex = getattr(r, "flex", None) or []
+        d.n_flex = len(flex)
+        if flex:
+            arr = (FlexResidue * len(flex))()
+            for k, f in enumerate(flex):
+                arr[k].first, arr[k].n_atoms, arr[k].n_rot = int(f.first), int(f.n_atoms), int(f.n_rot)
+                arr[k].scorable = _ptr(self._arr(f.scorable, np.uint8), _u8p)
+                arr[k].rot_xyz = _ptr(self._arr(f.rot_xyz.reshape(-1), np.float32), _f32p)
+                arr[k].bonded = _ptr(self._arr(f.bonded.reshape(-1), np.uint8), _u8p)
+            self._keep.append(arr)
+            d.flex = C.cast(arr, C.POINTER(FlexResidue))
         return d
 
     def _lig(self, l):

@@ -47,7 +47,7 @@ constexpr int VSL = (MAXV + WAVE - 1) / WAVE;  // vertex slots per lane
 constexpr int MAXE = 256;                      // hull edges (the walk fails at 200)
 constexpr int ESL = MAXE / WAVE;               // edge register slots per lane
 constexpr int MAXF = 384;                      // face point-list slots
-constexpr int MAXL = FA_MAX_LIG_ATOMS;
+constexpr int MAXL = FA_MAX_SCORABLE;       // scorable atoms of a pose (side chains + ligand)
 constexpr int LSL = MAXL / WAVE;               // ligand atom slots per lane
 constexpr unsigned NONE = 0xFFu;               // packed plane id "none"
 
@@ -61,6 +61,7 @@ constexpr int EDGE_LIMIT = 200;         // LIB/Vcontacts.cpp:378
 constexpr double BIG = 9.9e+9;          // sentinel of the reference scans
 
 static_assert(MAXC % WAVE == 0 && MAXE % WAVE == 0 && MAXL % WAVE == 0, "slot layout");
+static_assert(FA_MAX_LIG_ATOMS <= MAXL, "a ligand fits the scorable-atom slots");
 static_assert(MAXP < (int)NONE, "plane ids are packed in 8 bits");
 static_assert(MAXE >= EDGE_LIMIT + 2, "the edge limit must be reached before the edge capacity");
 
@@ -88,7 +89,10 @@ struct Smem {
     unsigned long long ops[FA_OPCOUNT_N];   // operation counters of the pose
     unsigned long long phase[N_PHASES];     // diagnostic launches: cycles per phase, this wave
     unsigned vpl[MAXV];                     // vertex planes p0 | p1 << 8 | p2 << 16
-    float lx[MAXL], ly[MAXL], lz[MAXL];     // ligand coordinates (calc order)
+    float lx[MAXL], ly[MAXL], lz[MAXL];     // scorable atoms: side-chain atoms, then the ligand's
+    float lr[MAXL];                         //   radius
+    int lcalc[MAXL];                        //   calc index
+    int rot[FA_MAX_FLEX_RESIDUES];          // conformation of each flexible residue
     int sbox[MAXL];                         // cell of the atom of each rank
     int nstart[MAXL + 1];                   // neighbour list of rank k: [nstart[k], nstart[k+1])
     int seed[3 * MAXL];                     // hull seeds (calc indices)
@@ -96,6 +100,7 @@ struct Smem {
     short foff[MAXC];
     unsigned char fcnt[MAXC];
     char pflag[MAXP];                       // 'X' off the polyhedron, 'Y' on it, 'E' engulfing
+    signed char pmv[MAXC];                  // scorable index of each contact, -1 for rigid atoms
     unsigned char order[MAXL], rank[MAXL];  // box order: calc index of rank k, rank of atom i
 };
 
@@ -784,10 +789,12 @@ __device__ __forceinline__ void put_vertex(Smem& s, int vn, double x, double y, 
 // The gene decode is serial (shift chains); the trig runs one atom per lane;
 // the rebuild is serial over atoms with the coordinates of atom i held by
 // lane i & 63, slot i >> 6.
-__device__ void build_pose(Smem& s, const DevLigand& lig, const double* icv)
+__device__ void build_pose(Smem& s, const DevReceptor& R, const DevLigand& lig, const double* icv)
 {
     const int n = lig.n_atoms;
+    const int ns = R.n_sc;  // ligand atom i is scorable atom ns + i
     const int l = lane_id();
+    if (l < FA_MAX_FLEX_RESIDUES) s.rot[l] = 0;
     for (int i = l; i < n; i += WAVE) {
         s.u.b.dis[i] = lig.dis[i];
         s.u.b.ang[i] = lig.ang[i];
@@ -822,6 +829,11 @@ __device__ void build_pose(Smem& s, const DevLigand& lig, const double* icv)
                 s.u.b.dis[atm] = (float)v;
             } else if (typ == FA_GENE_ANG) {
                 s.u.b.ang[atm] = (float)v;
+            } else if (typ == FA_GENE_ROT) {  // rot_idx = (uint)(icv + 0.5f), LIB/ic2cf.cpp:90-97
+                unsigned r = (unsigned)(v + 0.5f);
+                const int nr = R.flex_nrot[atm];
+                if ((int)r >= nr) r = (unsigned)(nr - 1);
+                s.rot[atm] = (int)r;
             } else if (typ == FA_GENE_DIH) {
                 s.u.b.dih[atm] = (float)v;
                 int j = atm, cat = lig.rec[4 * j + 3];
@@ -954,10 +966,22 @@ __device__ void build_pose(Smem& s, const DevLigand& lig, const double* icv)
     for (int q = 0; q < LSL; ++q) {
         const int a = l + WAVE * q;
         if (a < n) {
-            s.lx[a] = X[q];
-            s.ly[a] = Y[q];
-            s.lz[a] = Z[q];
+            s.lx[ns + a] = X[q];
+            s.ly[ns + a] = Y[q];
+            s.lz[ns + a] = Z[q];
+            s.lr[ns + a] = lig.radius[a];
+            s.lcalc[ns + a] = R.n + a;
         }
+    }
+    // side chains in the conformation their genes select (the atoms
+    // fatm[rot]..latm[rot] that index_protein takes, LIB/Vcontacts.cpp:1598-1606)
+    for (int m = l; m < ns; m += WAVE) {
+        const float4 p = R.sc_rot[R.sc_rot_off[m] + s.rot[R.sc_res[m]]];
+        s.lx[m] = p.x;
+        s.ly[m] = p.y;
+        s.lz[m] = p.z;
+        s.lr[m] = p.w;
+        s.lcalc[m] = R.sc_calc[m];
     }
     wave_sync();
 }
@@ -970,7 +994,7 @@ struct PoseGrid {
 };
 
 // LIB/Vcontacts.cpp:1594-1658 (steady state): only scorable atoms widen it
-__device__ void pose_grid(const Smem& s, const DevLigand& lig, const DevGrid& G, PoseGrid& pg)
+__device__ void pose_grid(const Smem& s, int M, const DevGrid& G, PoseGrid& pg)
 {
     float gmin[3], gmax[3];
     for (int j = 0; j < 3; ++j) {
@@ -978,7 +1002,7 @@ __device__ void pose_grid(const Smem& s, const DevLigand& lig, const DevGrid& G,
         gmax[j] = G.gmax[j];
     }
     int alter = 0;
-    for (int i = 0; i < lig.n_atoms; ++i) {  // calc order, like the reference
+    for (int i = 0; i < M; ++i) {  // calc order, like the reference
         const float c[3] = {s.lx[i], s.ly[i], s.lz[i]};
         for (int j = 0; j < 3; ++j) {
             if (c[j] < gmin[j]) {
@@ -1054,8 +1078,8 @@ __device__ __forceinline__ bool bit_of(unsigned long long b0, unsigned long long
 struct Cand {
     float x, y, z, r;
     int j;      // calc index, -1 when the lane has no candidate
-    int m;      // ligand index (j - Rn) for ligand candidates
-    bool lj;
+    int m;      // scorable-atom index for scorable candidates
+    bool lj;    // a scorable candidate (side-chain or ligand atom)
 };
 
 // Per-atom cull state and results (wave-uniform except `clash`).
@@ -1069,7 +1093,8 @@ struct CullAcc {
 struct CullCtx {
     int acalc, a;
     float ax, ay, az, ra, rado;
-    unsigned long long brow0, brow1;   // bonded row of atom a
+    int rlo, rhi;                      // calc range of a's residue (the ligand or a side chain)
+    unsigned long long brow0, brow1;   // bonded row of atom a over calc index - rlo
     unsigned long long dpre0, dpre1;   // atoms already marked by the pre-pass
     double permea;
     int2* nidx;
@@ -1105,7 +1130,8 @@ __device__ __forceinline__ bool cand_test(const CullCtx& cc, const Cand& c, Cull
     acc.found += found ? 1 : 0;
     sq = valid ? d2 : 0.0f;
     const double cdist = cc.permea * (double)(cc.ra + c.r);
-    const bool bonded = c.lj && bit_of(cc.brow0, cc.brow1, c.m);
+    // bonded: same residue and within bloops (LIB/Vcontacts.cpp:1813-1816)
+    const bool bonded = c.j >= cc.rlo && c.j < cc.rhi && bit_of(cc.brow0, cc.brow1, c.j - cc.rlo);
     // the distance (a square root) only where the wall can apply: the
     // rounded root is below cdist only if d2 < cdist^2, which the widened
     // bound keeps; most chunks have no such lane
@@ -1124,7 +1150,8 @@ __device__ __forceinline__ void cand_emit(const CullCtx& cc, const Cand& c, bool
     if (in) {
         const int pos = acc.n_near + lanes_below(mk);
         if (pos < cc.cap) {
-            cc.nidx[pos] = make_int2(c.j, __float_as_int(sq));
+            // a scorable neighbour is kept as -1 - (its scorable index)
+            cc.nidx[pos] = make_int2(c.lj ? -1 - c.m : c.j, __float_as_int(sq));
             cc.nxyz[pos] = make_float4(c.x, c.y, c.z, c.r);
         }
     }
@@ -1259,8 +1286,8 @@ __device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig,
                 c.x = s.lx[m];
                 c.y = s.ly[m];
                 c.z = s.lz[m];
-                c.r = lig.radius[m];
-                c.j = Rn + m;
+                c.r = s.lr[m];
+                c.j = s.lcalc[m];
                 c.lj = true;
             }
         }
@@ -1304,8 +1331,8 @@ __device__ __forceinline__ void visit_ligand_cell(const Smem& s, const DevLigand
             c.x = s.lx[m];
             c.y = s.ly[m];
             c.z = s.lz[m];
-            c.r = lig.radius[m];
-            c.j = Rn + m;
+            c.r = s.lr[m];
+            c.j = s.lcalc[m];
             c.lj = true;
         }
         visit(cc, c, acc);
@@ -1432,7 +1459,7 @@ __device__ void cull_slow_coarse(const Smem& s, const DevReceptor& R, const DevL
             const int t = base + l;
             Cand c{0.f, 0.f, 0.f, 0.f, -1, 0, false};
             bool v = false;
-            if (t < Rn) {
+            if (t < Rn && !(R.sc_flag && R.sc_flag[t])) {  // side-chain atoms come with the pose
                 const float4 p = R.xyzr[t];
                 v = box_of(pg, p.x, p.y, p.z) == sb;
                 if (v) c = Cand{p.x, p.y, p.z, p.w, t, 0, false};
@@ -1474,8 +1501,24 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
         }
         const DevLigand lig = ligs[li];
         const DevGrid G = grids[lig.grid_id];
-        const int L = lig.n_atoms;
+        const int NSC = R.n_sc;          // scorable side-chain atoms come first
+        const int L = NSC + lig.n_atoms;  // scorable atoms of the pose
         const int Rn = R.n;
+        // residue of scorable atom a: calc range and bonded row over calc - lo
+        auto residue_of = [&](int a, int& lo, int& hi, unsigned long long& b0, unsigned long long& b1) {
+            if (a < NSC) {
+                const int2 rg = R.sc_range[a];
+                lo = rg.x;
+                hi = rg.y;
+                b0 = R.sc_bonded[2 * a];
+                b1 = R.sc_bonded[2 * a + 1];
+            } else {
+                lo = Rn;
+                hi = Rn + lig.n_atoms;
+                b0 = lig.bonded[2 * (a - NSC)];
+                b1 = lig.bonded[2 * (a - NSC) + 1];
+            }
+        };
         int status = FA_OK;
         double r_com = 0.0, r_wal = 0.0, r_sas = 0.0, r_tot = 0.0, r_con = 0.0;
         int r_rclash = 0;
@@ -1486,18 +1529,18 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
         PhaseClock clk{B.prof ? s.phase : nullptr, 0ull};
         clk.start();
         const unsigned long long pose_t0 = B.prof ? __builtin_readcyclecounter() : 0ull;
-        build_pose(s, lig, B.genes + (size_t)pose * B.gene_stride);
-        count_op(s.ops, OP_BUILD, L);
+        build_pose(s, R, lig, B.genes + (size_t)pose * B.gene_stride);
+        count_op(s.ops, OP_BUILD, lig.n_atoms);
         if (B.lig_xyz) {
-            for (int i = l; i < L; i += WAVE) {
+            for (int i = l; i < lig.n_atoms; i += WAVE) {
                 float* o = B.lig_xyz + ((size_t)pose * B.xyz_stride + i) * 3;
-                o[0] = s.lx[i];
-                o[1] = s.ly[i];
-                o[2] = s.lz[i];
+                o[0] = s.lx[NSC + i];
+                o[1] = s.ly[NSC + i];
+                o[2] = s.lz[NSC + i];
             }
         }
         PoseGrid pg;
-        pose_grid(s, lig, G, pg);
+        pose_grid(s, L, G, pg);
         if (pg.slow && l == 0) {
             atomicAdd(&B.counters[0], 1u);
             atomicAdd(&B.totals[0], 1ull);
@@ -1553,7 +1596,8 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
             if (pg.slow && pg.dim >= 3) {
                 for (int t = l; t < R.n; t += WAVE) {
                     const float4 p = R.xyzr[t];
-                    wb[t] = box_of(pg, p.x, p.y, p.z);
+                    // side-chain atoms come with the pose: a cell no slot names
+                    wb[t] = (R.sc_flag && R.sc_flag[t]) ? INT_MIN / 4 : box_of(pg, p.x, p.y, p.z);
                 }
                 wave_sync();
             }
@@ -1564,14 +1608,13 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 const int a = s.order[k];
                 CullCtx cc;
                 cc.a = a;
-                cc.acalc = Rn + a;
+                cc.acalc = s.lcalc[a];
                 cc.ax = s.lx[a];
                 cc.ay = s.ly[a];
                 cc.az = s.lz[a];
-                cc.ra = lig.radius[a];
+                cc.ra = s.lr[a];
                 cc.rado = cc.ra + RW;
-                cc.brow0 = lig.bonded[2 * a];
-                cc.brow1 = lig.bonded[2 * a + 1];
+                residue_of(a, cc.rlo, cc.rhi, cc.brow0, cc.brow1);
                 cc.dpre0 = dpre0;
                 cc.dpre1 = dpre1;
                 cc.permea = (double)prm.permeability;
@@ -1629,9 +1672,9 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
             const double permea = (double)prm.permeability;
             for (int k = 0; k < L && status == FA_OK; ++k) {
                 const int a = s.order[k];
-                const int acalc = Rn + a;
+                const int acalc = s.lcalc[a];
                 const float ax = s.lx[a], ay = s.ly[a], az = s.lz[a];
-                const float ra = lig.radius[a];
+                const float ra = s.lr[a];
                 const float rado = ra + RW;
                 const double ro = (double)rado;
                 clk.stop(PH_REC);
@@ -1645,15 +1688,16 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 for (int base = ns; base < ne; base += WAVE) {
                     const int q = base + l;
                     bool keep = false;
-                    int j = -1;
+                    int j = -1, mv = -1;
                     float sq = 0.0f;
                     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
                     if (q < ne) {
                         const int2 e = nidx[q];
                         p = nxyz[q];
-                        j = e.x;
+                        mv = e.x < 0 ? -1 - e.x : -1;  // scorable neighbours are stored as -1 - index
+                        j = mv >= 0 ? s.lcalc[mv] : e.x;
                         sq = __int_as_float(e.y);
-                        keep = (j < Rn) || !bit_of(sk0, sk1, j - Rn);
+                        keep = mv < 0 || !bit_of(sk0, sk1, mv);
                     }
                     const unsigned long long mk = ballot(keep);
                     if (keep) {
@@ -1671,6 +1715,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                             s.u.c.cd[pos] = dist;
                             s.u.c.rb[pos] = p.w;
                             s.u.c.idx[pos] = j;
+                            s.pmv[pos] = (signed char)mv;
                         }
                     }
                     NC += __popcll(mk);
@@ -2047,12 +2092,13 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 clk.stop(PH_ARC);
                 const int NV = uni(vn);
 
-                // -- seeds of ligand atoms not seeded yet (save_seeds, LIB/Vcontacts.cpp:595-629):
+                // -- seeds of scorable atoms not seeded yet (save_seeds, LIB/Vcontacts.cpp:595-629):
                 // the first hull vertex (in order) with a plane of that atom
 #pragma unroll
                 for (int q = 0; q < PSL; ++q) {
                     const int c = l + WAVE * q;
-                    const bool cand = c < NC && pix[q] >= Rn && s.seed[3 * (pix[q] - Rn)] == -1;
+                    const int cm = c < NC ? (int)s.pmv[c] : -1;
+                    const bool cand = cm >= 0 && s.seed[3 * cm] == -1;
                     unsigned long long mc = ballot(cand);
                     while (mc) {
                         const int cc = WAVE * q + first_lane(mc);
@@ -2084,7 +2130,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                             const int i0 = atom_of(plane_at(pk, 0));
                             const int i1 = atom_of(plane_at(pk, 1));
                             const int i2 = atom_of(plane_at(pk, 2));
-                            const int m = atom_of(cc) - Rn;
+                            const int m = s.pmv[cc];
                             if (l == 0) {
                                 s.seed[3 * m] = acalc;
                                 s.seed[3 * m + 1] = hit_pos == 0 ? i1 : i0;
@@ -2524,7 +2570,10 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     const double surfA = 4.0 * PI_REF * radoA * radoA;
                     const double dee = (double)prm.dee_clash;
                     const int nt = prm.ntypes;
-                    const int ta = lig.type[a];
+                    const int ta = acalc < Rn ? R.type[acalc] : lig.type[acalc - Rn];
+                    int rlo, rhi;  // a's residue (the ligand or its side chain) and bonded row
+                    unsigned long long rb0, rb1;
+                    residue_of(a, rlo, rhi, rb0, rb1);
                     double sas_sub = 0.0, com = 0.0, wal = 0.0, con = 0.0;
                     int rcl = 0;
                     bool valid[PSL];
@@ -2544,9 +2593,9 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                             const double ar = par[q];
                             const double dist = pcd[q];
                             sas_sub += ar;
-                            const bool lj = j >= Rn;
-                            const int mj = j - Rn;
-                            const int tb = lj ? lig.type[mj] : R.type[j];
+                            const int mj = s.pmv[l + WAVE * q];  // scorable index, -1 for a rigid atom
+                            const bool same = j >= rlo && j < rhi;
+                            const int tb = j < Rn ? R.type[j] : lig.type[j - Rn];
                             if (B.recs) {  // own list in reverse plane order
                                 const int r = nrec_total + (nvalid_total - 1 - (seen + lanes_below(m)));
                                 if (r < B.rec_cap) {
@@ -2557,22 +2606,21 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                                     o.dist = dist;
                                 }
                             }
-                            bool skip = false;
-                            if (lj) {
-                                // each ligand pair scored once (by the higher calc index), bonded pairs never
-                                const bool bonded = (lig.bonded[2 * mj + (a >> 6)] >> (a & 63)) & 1ull;
-                                skip = bonded || (mj < a);
-                                if (!bit_of(dm0, dm1, mj)) s.stub[2 * mj + (a >> 6)] |= (1ull << (a & 63));
-                            }
+                            // bonded atoms of the residue never score; a pair of scorable
+                            // atoms scores once, at the lower calc index (FA->contacts,
+                            // LIB/vcfunction.cpp:230-250); scorable indices follow calc order
+                            const bool bonded = same && bit_of(rb0, rb1, j - rlo);
+                            const bool skip = bonded || (mj >= 0 && mj < a);
+                            if (mj >= 0 && !bit_of(dm0, dm1, mj)) s.stub[2 * mj + (a >> 6)] |= (1ull << (a & 63));
                             if (!skip) {
                                 const double rAB = radA + (double)prb[q];
                                 // covalent constraint of the pair (LIB/vcfunction.cpp:270-310): the
                                 // first constraint of `a` naming this partner decides
                                 bool covalent = false;
                                 double dist_opt = 0.0;
-                                if (lig.cons_off) {
-                                    const int c1 = lig.cons_off[a + 1];
-                                    for (int k = lig.cons_off[a]; k < c1; ++k) {
+                                if (lig.cons_off && a >= NSC) {  // constraints are the ligand's
+                                    const int c1 = lig.cons_off[a - NSC + 1];
+                                    for (int k = lig.cons_off[a - NSC]; k < c1; ++k) {
                                         if (lig.cons_part[k] != j) continue;
                                         if (lig.cons_type[k] == 1) {
                                             covalent = true;
@@ -2591,7 +2639,10 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                                     wal += (double)KWALL_F * (1.0 / (d6 * d6) - 1.0 / (cr6 * cr6));
                                     if (dist <= dee * rAB) rcl = 1;
                                 }
-                                if (!covalent && (prm.intramolecular || !lj))
+                                // intramolecular: one residue, or both protein residues
+                                // (a side-chain atom and any receptor atom), LIB/vcfunction.cpp:219-229
+                                const bool intramol = same || (acalc < Rn && j < Rn);
+                                if (!covalent && (prm.intramolecular || !intramol))
                                     com += pair_term(prm, (ta - 1) * nt + (tb - 1), ar, surfA);
                             }
                         }
@@ -2625,9 +2676,10 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 double add_tot = 0.0, add_sas = 0.0;
                 for (int i = l; i < L; i += WAVE) {
                     if (!bit_of(dm0, dm1, i)) {
-                        const double radoA = (double)lig.radius[i] + RW;
+                        const double radoA = (double)s.lr[i] + RW;
                         const double SAS = 4.0 * PI_REF * radoA * radoA;
-                        const int ti = lig.type[i];
+                        const int ci = s.lcalc[i];
+                        const int ti = ci < Rn ? R.type[ci] : lig.type[ci - Rn];
                         add_tot += SAS;
                         add_sas += (prm.solventterm != 0.0f)
                                        ? (double)prm.solventterm * SAS

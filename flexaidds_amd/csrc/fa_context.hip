@@ -21,6 +21,7 @@ namespace {
 thread_local std::string g_err;
 constexpr float CELLSIZE = 6.5f;  // LIB/Vcontacts.h:56
 constexpr int NEAR_PER_ATOM = 160;
+constexpr int NEAR_TAIL = 192;  // the pose kernel parks slow-path candidates past the lists (SLOW_TMP)
 }  // namespace
 
 int fa_fail(int code, const std::string& msg)
@@ -62,11 +63,38 @@ struct GridKey {
     }
 };
 
-int validate_ligand(const fa_ligand_desc& L, int idx)
+int validate_flex(const fa_receptor_desc& r)
+{
+    if (r.n_flex < 0 || r.n_flex > FA_MAX_FLEX_RESIDUES || (r.n_flex > 0 && !r.flex))
+        return fa_fail(FA_ERR_LIMIT, "between 0 and FA_MAX_FLEX_RESIDUES flexible residues");
+    int end = 0, n_sc = 0;
+    for (int k = 0; k < r.n_flex; ++k) {
+        const fa_flex_residue& f = r.flex[k];
+        const std::string tag = "flexible residue " + std::to_string(k) + ": ";
+        if (f.n_atoms < 1 || f.n_atoms > FA_MAX_FLEX_ATOMS || f.n_rot < 1)
+            return fa_fail(FA_ERR_LIMIT, tag + "1..FA_MAX_FLEX_ATOMS atoms and at least one conformation");
+        if (f.first < end || f.first + f.n_atoms > r.n_atoms)
+            return fa_fail(FA_ERR_INVALID, tag + "atoms must lie in the receptor, after the previous residue's");
+        if (!f.scorable || !f.rot_xyz || !f.bonded) return fa_fail(FA_ERR_INVALID, tag + "null array");
+        for (int i = 0; i < f.n_atoms; ++i) {
+            for (int q = 0; q < 3; ++q)
+                if (f.rot_xyz[3 * i + q] != r.xyz[3 * (f.first + i) + q])
+                    return fa_fail(FA_ERR_INVALID, tag + "conformation 0 must equal the receptor coordinates");
+            n_sc += f.scorable[i] ? 1 : 0;
+        }
+        end = f.first + f.n_atoms;
+    }
+    if (n_sc >= FA_MAX_SCORABLE) return fa_fail(FA_ERR_LIMIT, "too many scorable side-chain atoms");
+    return FA_SUCCESS;
+}
+
+int validate_ligand(const fa_ligand_desc& L, int idx, const fa_receptor_desc& R, int n_sc)
 {
     const std::string tag = "ligand " + std::to_string(idx) + ": ";
     if (L.n_atoms < 3 || L.n_atoms > FA_MAX_LIG_ATOMS)
         return fa_fail(FA_ERR_LIMIT, tag + "n_atoms must be in [3, " + std::to_string(FA_MAX_LIG_ATOMS) + "]");
+    if (L.n_atoms + n_sc > FA_MAX_SCORABLE)
+        return fa_fail(FA_ERR_LIMIT, tag + "ligand and side-chain atoms above FA_MAX_SCORABLE");
     if (L.n_genes < 1 || L.n_genes > FA_MAX_GENES)
         return fa_fail(FA_ERR_LIMIT, tag + "n_genes must be in [1, " + std::to_string(FA_MAX_GENES) + "]");
     if (L.n_grid < 1) return fa_fail(FA_ERR_INVALID, tag + "empty cleft grid");
@@ -92,7 +120,14 @@ int validate_ligand(const fa_ligand_desc& L, int idx)
     }
     for (int g = 0; g < L.n_genes; ++g) {
         const int t = L.gene_typ[g], a = L.gene_atm[g];
-        if (t < FA_GENE_GRID || t > FA_GENE_DIH) return fa_fail(FA_ERR_INVALID, tag + "unsupported gene type");
+        if (t == FA_GENE_ROT) {  // rotamer of flexible residue a
+            if (a < 0 || a >= R.n_flex) return fa_fail(FA_ERR_INVALID, tag + "rotamer gene names no flexible residue");
+            if (L.gene_min[g] < 0.0 || L.gene_max[g] > (double)(R.flex[a].n_rot - 1))
+                return fa_fail(FA_ERR_INVALID, tag + "rotamer gene limits outside the residue's conformations");
+            continue;
+        }
+        if (t < FA_GENE_GRID || t > FA_GENE_DIH)
+            return fa_fail(FA_ERR_INVALID, tag + "unsupported gene type (normal-mode genes are not implemented)");
         if (a < 0 || a >= n) return fa_fail(FA_ERR_INVALID, tag + "gene atom out of range");
         if (t == FA_GENE_DIH && L.rec[4 * a + 3] >= 0) {  // shift chain must return to the gene atom
             int cat = L.rec[4 * a + 3], steps = 0;
@@ -250,8 +285,12 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
     }
     for (int i = 0; i < r->n_atoms; ++i)
         if (r->type[i] < 1 || r->type[i] > prm->ntypes) return fa_fail(FA_ERR_INVALID, "receptor type out of range");
+    if (int rc = validate_flex(*r)) return rc;
+    int n_sc = 0;
+    for (int k = 0; k < r->n_flex; ++k)
+        for (int i = 0; i < r->flex[k].n_atoms; ++i) n_sc += r->flex[k].scorable[i] ? 1 : 0;
     for (int li = 0; li < n_ligands; ++li) {
-        int rc = validate_ligand(ligs[li], li);
+        int rc = validate_ligand(ligs[li], li, *r, n_sc);
         if (rc != FA_SUCCESS) return rc;
         for (int a = 0; a < ligs[li].n_atoms; ++a)
             if (ligs[li].type[a] > prm->ntypes) return fa_fail(FA_ERR_INVALID, "ligand type out of range");
@@ -274,6 +313,58 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
     c->rec.n = Rn;
     if ((rc = upload(c, rxyzr.data(), Rn, &c->rec.xyzr)) || (rc = upload(c, r->type, Rn, &c->rec.type)))
         return rc;
+    // flexible side chains: scorable atoms in calc order with their
+    // conformations, residue ranges and bonded bits; they stay out of the
+    // receptor cell lists (placed per pose)
+    std::vector<unsigned char> scflag(Rn, 0);
+    {
+        std::vector<int> sc_calc, sc_res, sc_off, nrot;
+        std::vector<float4> sc_rot;
+        std::vector<int2> sc_range;
+        std::vector<unsigned long long> sc_bits;
+        for (int k = 0; k < r->n_flex; ++k) {
+            const fa_flex_residue& f = r->flex[k];
+            nrot.push_back(f.n_rot);
+            for (int i = 0; i < f.n_atoms; ++i) {
+                if (!f.scorable[i]) continue;
+                const int ci = f.first + i;
+                scflag[ci] = 1;
+                sc_calc.push_back(ci);
+                sc_res.push_back(k);
+                sc_off.push_back((int)sc_rot.size());
+                for (int q = 0; q < f.n_rot; ++q) {
+                    const float* p = f.rot_xyz + ((size_t)q * f.n_atoms + i) * 3;
+                    sc_rot.push_back(make_float4(p[0], p[1], p[2], r->radius[ci]));
+                }
+                sc_range.push_back(make_int2(f.first, f.first + f.n_atoms));
+                unsigned long long b[2] = {0ull, 0ull};
+                for (int j = 0; j < f.n_atoms; ++j)
+                    if (f.bonded[i * f.n_atoms + j]) b[j >> 6] |= 1ull << (j & 63);
+                sc_bits.push_back(b[0]);
+                sc_bits.push_back(b[1]);
+            }
+        }
+        c->rec.n_sc = (int)sc_calc.size();
+        c->rec.n_flex = r->n_flex;
+        c->n_sc = c->rec.n_sc;
+        if (c->rec.n_sc > 0) {
+            if ((rc = upload(c, sc_calc.data(), sc_calc.size(), &c->rec.sc_calc)) ||
+                (rc = upload(c, sc_res.data(), sc_res.size(), &c->rec.sc_res)) ||
+                (rc = upload(c, sc_off.data(), sc_off.size(), &c->rec.sc_rot_off)) ||
+                (rc = upload(c, sc_rot.data(), sc_rot.size(), &c->rec.sc_rot)) ||
+                (rc = upload(c, nrot.data(), nrot.size(), &c->rec.flex_nrot)) ||
+                (rc = upload(c, sc_range.data(), sc_range.size(), &c->rec.sc_range)) ||
+                (rc = upload(c, sc_bits.data(), sc_bits.size(), &c->rec.sc_bonded)) ||
+                (rc = upload(c, scflag.data(), scflag.size(), &c->rec.sc_flag)))
+                return rc;
+        } else {
+            c->rec.sc_calc = c->rec.sc_res = c->rec.sc_rot_off = c->rec.flex_nrot = nullptr;
+            c->rec.sc_rot = nullptr;
+            c->rec.sc_range = nullptr;
+            c->rec.sc_bonded = nullptr;
+            c->rec.sc_flag = nullptr;
+        }
+    }
     // parameters
     c->prm.permeability = prm->permeability;
     c->prm.dee_clash = prm->dee_clash;
@@ -320,24 +411,29 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
             const long long dim2 = (long long)G.dim * G.dim, dim3 = dim2 * G.dim;
             if (G.dim < 1 || dim3 > (1LL << 26)) return fa_fail(FA_ERR_LIMIT, "cell grid too large");
             std::vector<int> box(Rn), cnt(dim3 + 1, 0);
+            int n_rigid = 0;
             for (int i = 0; i < Rn; ++i) {
+                if (scflag[i]) continue;  // side-chain atoms are placed per pose
                 const long long b = (long long)(int)((x[i] - G.gmin[0]) / CELLSIZE) * dim2 +
                                     (long long)(int)((y[i] - G.gmin[1]) / CELLSIZE) * G.dim +
                                     (int)((z[i] - G.gmin[2]) / CELLSIZE);
                 if (b < 0 || b >= dim3) return fa_fail(FA_ERR_INVALID, "receptor atom outside the cell grid");
                 box[i] = (int)b;
                 ++cnt[b + 1];
+                ++n_rigid;
             }
             for (long long b = 0; b < dim3; ++b) cnt[b + 1] += cnt[b];
-            std::vector<int> fill(cnt.begin(), cnt.end() - 1), atom(Rn);
-            std::vector<float4> xyzr(Rn);
+            std::vector<int> fill(cnt.begin(), cnt.end() - 1), atom(n_rigid);
+            std::vector<float4> xyzr(n_rigid);
             for (int i = 0; i < Rn; ++i) {  // stable: calc order inside a cell
+                if (scflag[i]) continue;
                 const int pos = fill[box[i]]++;
                 atom[pos] = i;
                 xyzr[pos] = make_float4(x[i], y[i], z[i], r->radius[i]);
             }
             if ((rc = upload(c, cnt.data(), (size_t)dim3 + 1, &G.cell_start)) ||
-                (rc = upload(c, atom.data(), Rn, &G.cell_atom)) || (rc = upload(c, xyzr.data(), Rn, &G.cell_xyzr)))
+                (rc = upload(c, atom.data(), n_rigid, &G.cell_atom)) ||
+                (rc = upload(c, xyzr.data(), n_rigid, &G.cell_xyzr)))
                 return rc;
             gid = (int)grids.size();
             grids.push_back(G);
@@ -418,11 +514,11 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
     // (in-range candidates of all atoms of a pose; ~77 per atom at most on
     // dense receptors, overflow is reported as a capacity status)
     int max_l = 0;
-    for (int li = 0; li < n_ligands; ++li) max_l = std::max(max_l, ligs[li].n_atoms);
+    for (int li = 0; li < n_ligands; ++li) max_l = std::max(max_l, ligs[li].n_atoms + c->n_sc);
     c->max_lig_atoms = max_l;
     const int per_cu = std::max(1, fa::cf_waves_per_cu());
     c->n_waves = c->n_cu * per_cu;
-    c->near_cap = max_l * NEAR_PER_ATOM;
+    c->near_cap = max_l * NEAR_PER_ATOM + NEAR_TAIL;
     const size_t n_near = (size_t)c->n_waves * c->near_cap;
     if ((rc = fa_ensure(c->near_idx, sizeof(int2) * n_near)) || (rc = fa_ensure(c->near_xyz, sizeof(float4) * n_near)) ||
         (rc = fa_ensure(c->rbox, sizeof(int) * (size_t)c->n_waves * 2 * (size_t)((Rn + 3) & ~3))))

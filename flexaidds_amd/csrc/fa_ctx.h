@@ -70,6 +70,7 @@ struct fa_ctx {
     DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops, prof, near_idx, near_xyz, rbox;
     DevBuf totals;  // 4 x u64 unscored-pose counts since the last reset (not cleared per launch)
     int max_genes = 0;  // largest gene count of the uploaded ligands
+    int n_sc = 0;       // scorable side-chain atoms of the resident receptor
     int prof_poses = 0;  // poses of the last profiled launch
     bool profile = false;
     std::vector<uint32_t> last_counters = std::vector<uint32_t>(4, 0);

@@ -9,11 +9,24 @@
 
 namespace fa {
 
-// Receptor atoms in calc order.
+// Receptor atoms in calc order.  The scorable atoms of a pose ("movable"
+// atoms, index m) are the side-chain atoms of the flexible residues, m < n_sc
+// in calc order, then the ligand's atoms, m = n_sc + ligand index; they are
+// placed per pose and never stored in the receptor cell lists.
 struct DevReceptor {
     int n;
-    const float4* xyzr;             // [n] x, y, z, radius
+    const float4* xyzr;             // [n] x, y, z, radius (conformation 0 for side chains)
     const int* type;                // [n]
+    int n_sc;                       // scorable side-chain atoms (0: rigid receptor)
+    int n_flex;                     // flexible residues
+    const int* sc_calc;             // [n_sc] calc index
+    const int* sc_res;              // [n_sc] flexible residue
+    const int* sc_rot_off;          // [n_sc] first of the atom's conformations in sc_rot
+    const float4* sc_rot;           // conformations of each side-chain atom (x, y, z, radius)
+    const int* flex_nrot;           // [n_flex] conformations per residue
+    const int2* sc_range;           // [n_sc] calc range [lo, hi) of the atom's residue
+    const unsigned long long* sc_bonded; // [2 * n_sc] bonded bits over calc index - lo
+    const unsigned char* sc_flag;   // [n] 1 = scorable side-chain atom, or null (rigid receptor)
 };
 
 // Cell list of the receptor for one grid (the grid index_protein builds when

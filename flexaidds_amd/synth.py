@@ -38,15 +38,40 @@ CAVITY_RADIUS = (20.0 ** 3 - 3.0 * 1500 * DENSITY_VOL / (4.0 * math.pi)) ** (1.0
 
 
 @dataclass
+class FlexResidue:
+    """A flexible side chain: receptor atoms [first, first + n_atoms) with
+    their conformations (0 = the receptor's own coordinates, then the
+    rotamers), scorable (non-backbone) atoms and bloops=2 bonded matrix."""
+    first: int
+    rot_xyz: np.ndarray   # (n_rot, n_atoms, 3) float32
+    scorable: np.ndarray  # (n_atoms,) uint8
+    bonded: np.ndarray    # (n_atoms, n_atoms) uint8
+    chi: np.ndarray = None  # (n_rot, n_chi) degrees, for reference
+
+    @property
+    def n_atoms(self) -> int:
+        return int(self.rot_xyz.shape[1])
+
+    @property
+    def n_rot(self) -> int:
+        return int(self.rot_xyz.shape[0])
+
+
+@dataclass
 class Receptor:
     xyz: np.ndarray      # (n,3) float32
     radius: np.ndarray   # (n,) float32
     type: np.ndarray     # (n,) int32, 1-based
     center: np.ndarray   # (3,) float64, cavity centre
+    flex: list = field(default_factory=list)  # FlexResidue, increasing `first`
 
     @property
     def n_atoms(self) -> int:
         return int(self.xyz.shape[0])
+
+    @property
+    def n_flex(self) -> int:
+        return len(self.flex)
 
 
 @dataclass
@@ -235,6 +260,133 @@ def make_receptor(rng, n_atoms, center, r_in=CAVITY_RADIUS, r_out=None, ntypes=4
     typ = rng.integers(1, ntypes + 1, size=n_atoms).astype(np.int32)
     rad = rng.uniform(1.5, 2.0, size=n_atoms).astype(np.float32)
     return Receptor(xyz=xyz, radius=rad, type=typ, center=np.asarray(center, np.float64))
+
+
+# ---------------------------------------------------------------------------
+# flexible side chains
+# ---------------------------------------------------------------------------
+
+#: a methionine-like side chain: backbone N CA C O (not scored), then CB CG
+#: SD CE built on chi1 (N-CA-CB-CG), chi2 (CA-CB-CG-SD), chi3 (CB-CG-SD-CE)
+FLEX_ATOMS = ("N", "CA", "C", "O", "CB", "CG", "SD", "CE")
+FLEX_BONDS = ((0, 1), (1, 2), (2, 3), (1, 4), (4, 5), (5, 6), (6, 7))
+_SC_GEOM = ((1.52, 114.0), (1.81, 112.7), (1.79, 100.9))  # bond (A), angle (deg) of CG, SD, CE
+CANONICAL_CHI = (-65.0, 180.0, 65.0)
+
+
+def _place(a, b, c, bond, angle, torsion):
+    """Atom d bonded to c with |cd| = bond, angle b-c-d, dihedral a-b-c-d (degrees)."""
+    bc = c - b
+    bc /= np.linalg.norm(bc)
+    n = np.cross(b - a, bc)
+    n /= np.linalg.norm(n)
+    m = np.cross(n, bc)
+    th, ph = math.radians(angle), math.radians(torsion)
+    d2 = np.array([-bond * math.cos(th), bond * math.sin(th) * math.cos(ph), bond * math.sin(th) * math.sin(ph)])
+    return c + d2[0] * bc + d2[1] * m + d2[2] * n
+
+
+def _flex_conformation(bb, chi):
+    n, ca, _, _, cb = bb
+    xyz = list(bb)
+    cg = _place(n, ca, cb, _SC_GEOM[0][0], _SC_GEOM[0][1], chi[0])
+    sd = _place(ca, cb, cg, _SC_GEOM[1][0], _SC_GEOM[1][1], chi[1])
+    ce = _place(cb, cg, sd, _SC_GEOM[2][0], _SC_GEOM[2][1], chi[2])
+    return np.array(xyz + [cg, sd, ce])
+
+
+def _flex_bonded(n_atoms, bonds, bloops=2):
+    adj = [[] for _ in range(n_atoms)]
+    for a, b in bonds:
+        adj[a].append(b)
+        adj[b].append(a)
+    out = np.zeros((n_atoms, n_atoms), np.uint8)
+    for s0 in range(n_atoms):
+        seen, frontier = {s0}, [s0]
+        for _ in range(bloops):
+            frontier = [b for a in frontier for b in adj[a] if b not in seen]
+            seen.update(frontier)
+        for b in seen:
+            out[s0, b] = 1
+    return out
+
+
+def make_flex_receptor(rng, n_atoms, center, n_flex=5, n_rot=10, r_in=CAVITY_RADIUS, ntypes=40,
+                       clear=4.2, min_sep_deg=50.0) -> Receptor:
+    """A shell receptor of `n_atoms` atoms in all, `n_flex` of whose residues
+    are flexible side chains lining the cavity (FLEX_ATOMS, 3 chi angles),
+    each with `n_rot` conformations: its own (chi near a canonical
+    combination) and n_rot - 1 other canonical combinations with a few
+    degrees of noise.  Shell atoms closer than `clear` A to a side-chain atom
+    of any conformation are left out (build_rotamers keeps only rotamers free
+    of clashes with rigid atoms, LIB/build_rotamers.cpp:214-228), and the
+    shell is refilled to `n_atoms`.  Residues sit between shell atoms in calc
+    order, as residues of a protein do."""
+    center = np.asarray(center, np.float64)
+    dirs = []
+    while len(dirs) < n_flex:
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        if all(math.degrees(math.acos(max(-1.0, min(1.0, float(v @ w))))) >= min_sep_deg for w in dirs):
+            dirs.append(v)
+    combos = [(a, b, c) for a in CANONICAL_CHI for b in CANONICAL_CHI for c in CANONICAL_CHI]
+    residues = []
+    for u in dirs:
+        t1 = np.cross(u, [0.0, 0.0, 1.0] if abs(u[2]) < 0.9 else [1.0, 0.0, 0.0])
+        t1 /= np.linalg.norm(t1)
+        t2 = np.cross(u, t1)
+        ca = center + u * (r_in + 0.5)
+        nn = ca + 1.46 * (t1 + 0.33 * u) / np.linalg.norm(t1 + 0.33 * u)
+        cc = ca + 1.52 * (-0.5 * t1 + 0.87 * t2 + 0.33 * u) / np.linalg.norm(-0.5 * t1 + 0.87 * t2 + 0.33 * u)
+        oo = cc + 1.23 * (t2 + 0.5 * u) / np.linalg.norm(t2 + 0.5 * u)
+        cb = ca - 1.53 * u
+        pick = rng.choice(len(combos), size=n_rot, replace=False)
+        chi = np.array([combos[k] for k in pick]) + rng.normal(0.0, 5.0, size=(n_rot, 3))
+        conf = np.stack([_flex_conformation([nn, ca, cc, oo, cb], c) for c in chi])
+        residues.append((conf, chi))
+    n_res_atoms = len(FLEX_ATOMS)
+    sc = np.concatenate([conf[:, 4:, :].reshape(-1, 3) for conf, _ in residues])  # scorable, every conformation
+    n_shell = n_atoms - n_flex * n_res_atoms
+    r_out = (r_in ** 3 + 3.0 * n_atoms * DENSITY_VOL / (4.0 * math.pi)) ** (1.0 / 3.0)
+    shell = np.zeros((0, 3))
+    while len(shell) < n_shell:
+        k = 2 * (n_shell - len(shell)) + 16
+        u = rng.random(k)
+        r = np.cbrt(r_in ** 3 + u * (r_out ** 3 - r_in ** 3))
+        v = rng.normal(size=(k, 3))
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        p = center[None, :] + v * r[:, None]
+        d = np.linalg.norm(p[:, None, :] - sc[None, :, :], axis=2).min(axis=1)
+        shell = np.vstack([shell, p[d >= clear]])[:n_shell]
+    # calc order: shell atoms with the residues spread between them
+    cut = np.linspace(0, n_shell, n_flex + 2).round().astype(int)[1:-1]
+    xyz, flex, pos = [], [], 0
+    for k, (conf, chi) in enumerate(residues):
+        xyz.append(shell[pos:cut[k]])
+        first = sum(len(x) for x in xyz)
+        xyz.append(conf[0])
+        flex.append(FlexResidue(first=first, rot_xyz=conf.astype(np.float32),
+                                scorable=np.array([0, 0, 0, 0, 1, 1, 1, 1], np.uint8),
+                                bonded=_flex_bonded(n_res_atoms, FLEX_BONDS), chi=chi))
+        pos = cut[k]
+    xyz.append(shell[pos:])
+    xyz = np.vstack(xyz).astype(np.float32)
+    for f in flex:  # conformation 0 is the receptor's own coordinates (float32 exactly)
+        f.rot_xyz[0] = xyz[f.first:f.first + f.n_atoms]
+    typ = rng.integers(1, ntypes + 1, size=n_atoms).astype(np.int32)
+    rad = rng.uniform(1.5, 2.0, size=n_atoms).astype(np.float32)
+    return Receptor(xyz=xyz, radius=rad, type=typ, center=center, flex=flex)
+
+
+def add_rotamer_genes(lig: "Ligand", rec: Receptor):
+    """Append one rotamer gene (typ 4, index gene over 0..n_rot-1) per
+    flexible side chain, after the ligand's own genes, as add2_optimiz_vec
+    does for "SC" (LIB/add2_optimiz_vec.cpp:28-62)."""
+    k = np.arange(rec.n_flex, dtype=np.int32)
+    lig.gene_typ = np.concatenate([lig.gene_typ, np.full(rec.n_flex, 4, np.int32)])
+    lig.gene_atm = np.concatenate([lig.gene_atm, k])
+    lig.gene_min = np.concatenate([lig.gene_min, np.zeros(rec.n_flex)])
+    lig.gene_max = np.concatenate([lig.gene_max, np.array([f.n_rot - 1 for f in rec.flex], np.float64)])
 
 
 # ---------------------------------------------------------------------------
@@ -506,12 +658,17 @@ def make_density_emat(rng, ntypes=40, frac=0.5, max_nodes=6):
 # ---------------------------------------------------------------------------
 
 def random_genes(rng, lig: Ligand, n_poses, grid_lo=1, grid_hi=None) -> np.ndarray:
-    """Gene vectors (ic values): grid index, uniform rotation, uniform torsions."""
+    """Gene vectors (ic values): grid index, uniform rotation, uniform
+    torsions, and (rotamer genes, typ 4) a uniform rotamer index."""
+    rot = np.nonzero(lig.gene_typ == 4)[0]
+    nl = lig.n_genes - len(rot)          # the ligand's own genes come first
     g = np.zeros((n_poses, lig.n_genes), np.float64)
     hi = lig.n_grid - 1 if grid_hi is None else grid_hi
     g[:, 0] = rng.integers(grid_lo, hi + 1, size=n_poses).astype(np.float64)
     g[:, 1] = np.degrees(np.arccos(rng.uniform(-1.0, 1.0, size=n_poses)))
-    g[:, 2:] = rng.uniform(-180.0, 180.0, size=(n_poses, lig.n_genes - 2))
+    g[:, 2:nl] = rng.uniform(-180.0, 180.0, size=(n_poses, nl - 2))
+    for c in rot:
+        g[:, c] = rng.integers(0, int(lig.gene_max[c]) + 1, size=n_poses).astype(np.float64)
     return g
 
 
@@ -531,7 +688,7 @@ def ga_limits(lig: Ligand, angle_step=2.0):
     out = []
     for g in range(lig.n_genes):
         lo, hi = float(lig.gene_min[g]), float(lig.gene_max[g])
-        if int(lig.gene_typ[g]) == -1:  # grid-index gene
+        if int(lig.gene_typ[g]) in (-1, 4):  # grid-index and rotamer genes (map_opt_par = 1)
             out.append((lo, hi, 1.0, 1))
         else:
             out.append((lo, hi, angle_step, 0))
@@ -560,17 +717,24 @@ def make_config0(n_poses=1000, seed=None, n_rec=1500, n_lig=30, n_tors=6) -> Sys
     return System(receptor=rec, ligands=[lig], params=params, genes=[genes], config=0)
 
 
-def make_config1(pop=2000, seed=None) -> System:
-    """Plain full-GA dock shape: 4,000-atom receptor, 45-atom/10-torsion
-    ligand, one population of `pop` chromosomes (the rigid-receptor part;
-    flexible side chains are not modelled here)."""
+def make_config1(pop=2000, seed=None, n_flex=5, n_rot=10) -> System:
+    """Plain full-GA dock shape: 4,000-atom receptor whose cavity is lined
+    by `n_flex` flexible side chains of 3 chi angles (`n_rot` conformations
+    each; n_flex = 0 gives the rigid shell), 45-atom/10-torsion ligand, one
+    population of `pop` chromosomes; 14 ligand genes + one rotamer gene per
+    side chain."""
     rng = np.random.default_rng(CONFIG_SEED[1] if seed is None else seed)
     center = np.array([-8.6, 17.2, 33.4])
-    rec = make_receptor(rng, 4000, center, r_in=CAVITY_RADIUS)
+    if n_flex:
+        rec = make_flex_receptor(rng, 4000, center, n_flex=n_flex, n_rot=n_rot)
+    else:
+        rec = make_receptor(rng, 4000, center, r_in=CAVITY_RADIUS)
     ori = rec.xyz.astype(np.float64).mean(axis=0).astype(np.float32)
     pts = _box_points(rng, center, 1000)
     lig = make_ligand(rng, 45, 10, ori, center, pts)
     finalize_frame(lig, rec)
+    if n_flex:
+        add_rotamer_genes(lig, rec)
     params = Params(ntypes=40, emat=make_emat(rng))
     genes = random_genes(rng, lig, pop)
     return System(receptor=rec, ligands=[lig], params=params, genes=[genes], config=1)
@@ -694,3 +858,61 @@ def add_constraints(system: System, lig_index=0, n_receptor=2, n_ligand=1, bond_
         cons.append((int(a), int(b), 1, 1, float(bond_len), float(max_dist)))
     lig.cons = cons
     return cons
+
+
+def make_engulf_system(n_poses=200, seed=1010) -> System:
+    """Config-0 shape whose ligand alternates small (1.5 A) and large (2.0 A)
+    atoms along its bonds (by depth parity in the tree).  A bonded neighbour
+    1.5 A away with a radius 0.5 A larger puts the radical plane beyond the
+    atom's centre (an engulfing plane, D > 0, LIB/Vcontacts.cpp:470-478), and
+    bonded pairs add no clash wall, so small atoms with two or three large
+    neighbours reach calc_areas' two- and three-plane engulfing branches
+    (LIB/Vcontacts.cpp:963-1042)."""
+    s = make_config0(n_poses=n_poses, seed=seed)
+    lig = s.ligands[0]
+    depth = np.zeros(lig.n_atoms, np.int32)
+    for i in range(1, lig.n_atoms):
+        depth[i] = depth[lig.parent[i]] + 1
+    lig.radius = np.where(depth % 2 == 0, 1.5, 2.0).astype(np.float32)
+    return s
+
+
+def fibonacci_sphere(n):
+    """n unit vectors spread evenly over the sphere (golden-angle spiral)."""
+    k = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * k / n
+    r = np.sqrt(1.0 - z * z)
+    phi = k * math.pi * (3.0 - math.sqrt(5.0))
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def make_shell_system(n_shell=80, dist=5.0, n_poses=64, n_far=63, seed=1011) -> System:
+    """A three-atom ligand (bonds of 0.6 A, so every pair is within two
+    bonds) whose cleft grid holds the centre of a shell of `n_shell`
+    receptor atoms (radius 1.8 A) `dist` A from it (every odd grid point)
+    and points 30 A away from any atom (the even ones).  At the centre each
+    ligand atom has about n_shell contacts, none within the clash distance,
+    so its polyhedron has about 3 n_shell - 6 edges: the hull walk reaches its
+    200-edge limit (LIB/Vcontacts.cpp:378-407) for n_shell >= ~70, and more
+    than 128 contacts exceed the device's per-atom capacity.  Far away the
+    ligand touches nothing and scores its solvent term only."""
+    rng = np.random.default_rng(seed)
+    center = np.array([3.0, -2.0, 5.0])
+    # a little jitter keeps four planes from meeting in one vertex (an exact
+    # degeneracy where the hull walk finds no plane ahead)
+    u = fibonacci_sphere(n_shell) + rng.normal(0.0, 0.03, size=(n_shell, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    xyz = (center[None, :] + dist * u).astype(np.float32)
+    typ = rng.integers(1, 41, size=n_shell).astype(np.int32)
+    rec = Receptor(xyz=xyz, radius=np.full(n_shell, 1.8, np.float32), type=typ, center=center)
+    ori = (center + np.array([7.0, 5.0, -6.0])).astype(np.float32)  # any frame origin off the anchor
+    far = center + np.array([30.0, 0.0, 0.0])
+    pts = np.vstack([center if k % 2 else far for k in range(1, n_far + 1)])
+    lig = make_ligand(rng, 3, 0, ori, center, pts)
+    lig.dis[1:] = np.float32(0.6)
+    lig.radius = np.full(lig.n_atoms, 1.6, np.float32)
+    finalize_frame(lig, rec)
+    params = Params(ntypes=40, emat=make_emat(rng))
+    genes = random_genes(rng, lig, n_poses)
+    genes[:, 0] = np.arange(n_poses) % (n_far + 1)
+    return System(receptor=rec, ligands=[lig], params=params, genes=[genes], config=-1)

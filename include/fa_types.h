@@ -33,8 +33,18 @@ enum fa_gene_type {
     FA_GENE_GRID = -1, /* index into the cleft grid: sets dis/ang/dih of atm */
     FA_GENE_DIS = 0,
     FA_GENE_ANG = 1,
-    FA_GENE_DIH = 2    /* dihedral; cascades along rec[3] shift chains */
+    FA_GENE_DIH = 2,   /* dihedral; cascades along rec[3] shift chains */
+    FA_GENE_ROT = 4    /* rotamer of flexible side chain `atm` (an index into
+                          fa_receptor_desc.flex): (unsigned)(v + 0.5f),
+                          LIB/ic2cf.cpp:90-103 */
 };
+
+/* Scorable atoms of one pose: every ligand atom plus the side-chain atoms of
+ * the flexible residues, at most FA_MAX_SCORABLE in all; a flexible residue
+ * has at most FA_MAX_FLEX_ATOMS atoms. */
+#define FA_MAX_FLEX_RESIDUES 32
+#define FA_MAX_FLEX_ATOMS 128
+#define FA_MAX_SCORABLE 128
 
 /* Per-pose evaluation status.  FA_OK, FA_HULL_FAIL and FA_CLASH carry the
  * reference's energies (the clash and hull-failure cfstr of LIB/ic2cf.cpp:129
@@ -50,12 +60,33 @@ enum fa_status {
     FA_BAD_LIGAND = 5     /* the pose names a ligand index the context does not hold */
 };
 
-/* Rigid receptor part (all non-scorable atoms). */
+/* One flexible side chain (a residue of FA->flex_res with its accepted
+ * rotamers, LIB/build_rotamers.cpp:7-375).  Its atoms are receptor atoms
+ * [first, first + n_atoms) in calc order.  Conformation 0 is the input one
+ * (the receptor's xyz); conformations 1..n_rot-1 are the rotamers, whose
+ * atoms the reference appends as copies that keep their PDB numbers, so
+ * that num_atm maps them back to the input atoms for the bonded lookups
+ * (LIB/vcfunction.cpp:189-240, LIB/Vcontacts.cpp:1813-1816).  The scorable
+ * atoms are those outside the backbone (update_optres, LIB/update_optres.cpp:
+ * 10-30); the residue is a protein residue (type 0). */
+typedef struct fa_flex_residue {
+    int32_t        first;     /* receptor index of the residue's first atom */
+    int32_t        n_atoms;   /* atoms of the residue, backbone included */
+    int32_t        n_rot;     /* conformations (>= 1) */
+    const uint8_t* scorable;  /* [n_atoms] 1 = side-chain atom (scored) */
+    const float*   rot_xyz;   /* [n_rot * n_atoms * 3]; row 0 equals the receptor's xyz */
+    const uint8_t* bonded;    /* [n_atoms * n_atoms] 1 = within bloops bonds (residue.bonded >= 0) */
+} fa_flex_residue;
+
+/* Receptor atoms in calc order.  The rigid part is every atom that is not a
+ * scorable atom of a flexible residue. */
 typedef struct fa_receptor_desc {
     int32_t        n_atoms;
     const float*   xyz;      /* [n_atoms*3] Å */
     const float*   radius;   /* [n_atoms] van der Waals radius */
     const int32_t* type;     /* [n_atoms] 1-based energy-matrix type */
+    int32_t        n_flex;   /* flexible side chains (0: rigid receptor) */
+    const fa_flex_residue* flex; /* [n_flex], increasing `first`, not overlapping */
 } fa_receptor_desc;
 
 /* Distance constraint (constraint_str, LIB/flexaid.h:168-190) between ligand
@@ -90,7 +121,7 @@ typedef struct fa_ligand_desc {
     const int32_t* mov;        /* [n_atoms] rebuild order (buildlist output) */
     const uint8_t* bonded;     /* [n_atoms*n_atoms] 1 = within bloops bonds (bonded[i][j] >= 0) */
     const int32_t* gene_typ;   /* [n_genes] fa_gene_type */
-    const int32_t* gene_atm;   /* [n_genes] ligand-local atom */
+    const int32_t* gene_atm;   /* [n_genes] ligand-local atom (FA_GENE_ROT: flexible residue index) */
     const double*  gene_min;   /* [n_genes] clamp bounds (genlim.min/max) */
     const double*  gene_max;
     const float*   grid_ic;    /* [n_grid*3] (dis, ang, dih) of each cleft-grid point */

@@ -21,6 +21,22 @@
  *   - contact records (save_areas)               LIB/Vcontacts.cpp:1149-1220
  *   - energy sum                                 LIB/vcfunction.cpp:14-574
  *
+ * Flexible side chains (fa_flex_residue): the rotamer genes pick each
+ * residue's conformation (LIB/ic2cf.cpp:90-103), the residue's non-backbone
+ * atoms are scorable atoms among the receptor atoms in calc order
+ * (LIB/Vcontacts.cpp:1598-1625, LIB/update_optres.cpp:10-30), bonded and
+ * intramolecular tests follow the residue (LIB/vcfunction.cpp:189-240,
+ * LIB/Vcontacts.cpp:1806-1821).
+ *
+ * Two evaluation modes:
+ *   0 (reference, the default) — the reference's serial semantics above;
+ *   1 (independent atoms) — the form the GPU computes: every scorable atom's
+ *     polyhedron from all of its in-range neighbours, with no done/stub
+ *     exclusion and no hull seeds, and each cell's candidates taken rigid
+ *     atoms first, then scorable atoms (both in calc order).  The polyhedra
+ *     are the same up to rounding and degenerate ties; the clash pre-pass
+ *     counts the same pairs.
+ *
  * The geometry type `R` is double for the oracle proper (the reference does
  * all Vcontacts geometry in double); the float instantiation exists only to
  * measure how far FP32 geometry drifts from it.
@@ -58,6 +74,16 @@ constexpr int EDGE_LIMIT = 200;            // LIB/Vcontacts.cpp:378
 // rounded; what the GPU kernel computes).
 int g_trig_mode = 0;
 
+// 0 = reference semantics, 1 = independent atoms (see the header); the
+// three parts of the independent form can be chosen alone (bits of
+// g_mode_bits: 1 no done/stub rule, 2 no hull seeds, 4 rigid atoms first in
+// a cell) to measure what each changes
+int g_eval_mode = 0;
+int g_mode_bits = 0;
+inline bool m_nodone() { return g_mode_bits & 1; }
+inline bool m_noseed() { return g_mode_bits & 2; }
+inline bool m_rigidfirst() { return g_mode_bits & 4; }
+
 inline float trig_cos(float x) { return g_trig_mode ? (float)std::cos((double)x) : std::cos(x); }
 inline float trig_sin(float x) { return g_trig_mode ? (float)std::sin((double)x) : std::sin(x); }
 
@@ -80,6 +106,13 @@ struct CapStats {
     int face_slots_run;
 };
 thread_local CapStats* t_caps = nullptr;
+
+// branches of calc_areas reached: atoms with one, two, three or more
+// engulfing planes (LIB/Vcontacts.cpp:963-1042), atoms with a polyhedron
+struct BranchStats {
+    long long epi1, epi2, epi3, atoms, hull_fail;
+};
+thread_local BranchStats* t_branch = nullptr;
 #define CAPMAX(field, v)                                      \
     do {                                                      \
         if (t_caps && (v) > t_caps->field) t_caps->field = (v); \
@@ -256,8 +289,9 @@ template <class R> struct Work {
     std::vector<int> boxnum, calclist;
     std::vector<char> done;
     std::vector<int> seed;                 // 3 per atom
-    std::vector<std::vector<int>> stubs;   // per ligand atom: earlier atoms with a record to it
-    std::vector<std::vector<fa_contact_rec>> records; // per ligand atom: own valid contacts (list order)
+    std::vector<std::vector<int>> stubs;   // per scorable slot: earlier atoms with a record to it
+    std::vector<std::vector<fa_contact_rec>> records; // per scorable slot: own valid contacts (list order)
+    std::vector<int> rank;                 // per calc index: processing rank of a scorable atom
     std::vector<int> boxfirst, boxcount;
     int near_count = 0;
     // per atomzero scratch
@@ -273,15 +307,53 @@ struct System {
     const fa_receptor_desc* rec;
     const fa_ligand_desc* lig;
     const fa_params* prm;
+    // per calc index: scorable slot (-1: not scorable) and residue (flexible
+    // residue k >= 0, the ligand -1, any other receptor atom -2)
+    std::vector<int> slot, res;
+    std::vector<int> scorable;  // calc indices of the scorable atoms, calc order
     int R() const { return rec->n_atoms; }
     int L() const { return lig->n_atoms; }
     int N() const { return rec->n_atoms + lig->n_atoms; }
+
+    System(const fa_receptor_desc* r, const fa_ligand_desc* l, const fa_params* p) : rec(r), lig(l), prm(p)
+    {
+        const int Rn = R(), n = N();
+        slot.assign(n, -1);
+        res.assign(n, -2);
+        for (int k = 0; k < rec->n_flex; ++k) {
+            const fa_flex_residue& f = rec->flex[k];
+            for (int i = 0; i < f.n_atoms; ++i) res[f.first + i] = k;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (i >= Rn) res[i] = -1;
+            const bool sc = i >= Rn || (res[i] >= 0 && rec->flex[res[i]].scorable[i - rec->flex[res[i]].first]);
+            if (sc) {
+                slot[i] = (int)scorable.size();
+                scorable.push_back(i);
+            }
+        }
+    }
+    // both atoms in one residue (the ligand or one flexible side chain)
+    bool same_res(int a, int b) const { return res[a] != -2 && res[a] == res[b]; }
+    // bonded within bloops (residue.bonded >= 0); a and b in one residue
+    bool bonded(int a, int b) const
+    {
+        const int Rn = R();
+        if (res[a] == -1) return lig->bonded[(a - Rn) * L() + (b - Rn)] != 0;
+        const fa_flex_residue& f = rec->flex[res[a]];
+        return f.bonded[(a - f.first) * f.n_atoms + (b - f.first)] != 0;
+    }
 };
 
 // ---------------------------------------------------------------- pose build
-// LIB/ic2cf.cpp:53-105 (gene decode) then LIB/buildcc.cpp:6-134 (float rebuild)
-void decode_and_build(const fa_ligand_desc* lig, const double* icv_in, float* xyz)
+// LIB/ic2cf.cpp:53-105 (gene decode) then LIB/buildcc.cpp:6-134 (float rebuild).
+// `rot` (optional, one entry per flexible residue of `rec`) receives the
+// conformation each rotamer gene selects; residues without a gene keep 0.
+void decode_and_build(const fa_ligand_desc* lig, const double* icv_in, float* xyz,
+                      const fa_receptor_desc* rec = nullptr, int* rot = nullptr)
 {
+    if (rot && rec)
+        for (int k = 0; k < rec->n_flex; ++k) rot[k] = 0;
     const int n = lig->n_atoms;
     std::vector<float> dis(lig->dis, lig->dis + n), ang(lig->ang, lig->ang + n), dih(lig->dih, lig->dih + n);
     for (int i = 0; i < lig->n_genes; ++i) {
@@ -310,6 +382,14 @@ void decode_and_build(const fa_ligand_desc* lig, const double* icv_in, float* xy
                     j = cat;
                     cat = lig->rec[4 * j + 3];
                 }
+            }
+            break;
+        }
+        case FA_GENE_ROT: {  // rot_idx = (uint)(icv[i] + 0.5f), LIB/ic2cf.cpp:90-97
+            if (rot && rec && atm >= 0 && atm < rec->n_flex) {
+                unsigned r = (unsigned)(v + 0.5f);
+                if ((int)r >= rec->flex[atm].n_rot) r = (unsigned)(rec->flex[atm].n_rot - 1);
+                rot[atm] = (int)r;
             }
             break;
         }
@@ -394,7 +474,8 @@ template <class R> bool index_atoms(const System& S, Work<R>& W)
     }
     float max_width = lig->maxwidth;
     int alter = 0;
-    for (int i = Rn; i < N; ++i) {
+    (void)Rn;
+    for (int i : S.scorable) {  // calc order: side-chain atoms, then the ligand
         const float c[3] = {W.x[i], W.y[i], W.z[i]};
         for (int j = 0; j < 3; ++j) {
             if (c[j] < gmin[j]) {
@@ -432,10 +513,22 @@ template <class R> bool index_atoms(const System& S, Work<R>& W)
         start += W.boxcount[b];
         W.boxcount[b] = 0;
     }
-    for (int i = 0; i < N; ++i) {
-        const int b = W.boxnum[i];
-        W.calclist[W.boxfirst[b] + W.boxcount[b]] = i;
-        ++W.boxcount[b];
+    // a box lists its atoms in calc order (reference); independent mode
+    // lists the rigid atoms first, then the scorable ones
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int i = 0; i < N; ++i) {
+            if (m_rigidfirst() && (pass == 0) == (S.slot[i] >= 0)) continue;
+            if (!m_rigidfirst() && pass == 1) break;
+            const int b = W.boxnum[i];
+            W.calclist[W.boxfirst[b] + W.boxcount[b]] = i;
+            ++W.boxcount[b];
+        }
+    }
+    // processing rank of each scorable atom (its place in the box order)
+    int k = 0;
+    for (int q = 0; q < N; ++q) {
+        const int i = W.calclist[q];
+        if (S.slot[i] >= 0) W.rank[i] = k++;
     }
     return true;
 }
@@ -445,12 +538,12 @@ template <class R> bool index_atoms(const System& S, Work<R>& W)
 template <class R>
 int get_contacts(const System& S, Work<R>& W, int a, float rado, double* clash)
 {
-    const int Rn = S.R();
     const int dim = W.dim, dim2 = dim * dim, dim3 = dim2 * dim;
     const double permea = (double)S.prm->permeability;
+    const bool indep = m_nodone();
     // earlier atoms that recorded a contact with `a` stay eligible ('C')
-    if (a >= Rn)
-        for (int j : W.stubs[a - Rn]) W.done[j] = 'C';
+    if (!indep)
+        for (int j : W.stubs[S.slot[a]]) W.done[j] = 'C';
     int NC = 0;
     W.contlist.clear();
     const int bz = W.boxnum[a];
@@ -465,7 +558,10 @@ int get_contacts(const System& S, Work<R>& W, int a, float rado, double* clash)
                 const double nd = (double)(rado + W.rad[j] + RW);
                 if (sq < nd * nd && sq != 0.0) ++W.near_count;
             }
-            if (W.done[j] == 'Y') continue;  // every atom is exposed here
+            if (!indep && W.done[j] == 'Y') continue;  // every atom is exposed here
+            // independent pre-pass: a pair of scorable atoms is counted by the
+            // one processed first (the pairs the reference's done marks leave)
+            if (indep && clash && S.slot[j] >= 0 && W.rank[j] < W.rank[a]) continue;
             if (j != a) OPS(cull, 1);        // the atom itself never makes a contact
             const double rAB = (double)(W.rad[a] + W.rad[j]);
             const float dx = W.x[a] - W.x[j], dy = W.y[a] - W.y[j], dz = W.z[a] - W.z[j];
@@ -477,19 +573,18 @@ int get_contacts(const System& S, Work<R>& W, int a, float rado, double* clash)
                 if (clash) {
                     const double clashdist = permea * rAB;
                     if (dist < clashdist) {
-                        const bool intra = (j >= Rn);  // same residue: both ligand atoms
-                        const bool bonded = intra && S.lig->bonded[(a - Rn) * S.L() + (j - Rn)];
-                        if (!intra || !bonded)
+                        const bool intra = S.same_res(a, j);
+                        if (!intra || !S.bonded(a, j))
                             *clash += (double)KWALL_F * (std::pow(dist, -12.0) - std::pow(clashdist, -12.0));
                     }
-                    W.done[a] = 'Y';
+                    if (!indep) W.done[a] = 'Y';
                 }
                 ++NC;
             }
         }
     }
-    if (a >= Rn)
-        for (int j : W.stubs[a - Rn]) W.done[j] = 'Y';
+    if (!indep)
+        for (int j : W.stubs[S.slot[a]]) W.done[j] = 'Y';
     return NC;
 }
 
@@ -499,7 +594,7 @@ template <class R> void get_firstvert(const Work<R>& W, const Plane<R>* cont, in
 {
     *pA = *pB = *pC = -1;
     const int* sd = &W.seed[3 * a];
-    if (sd[0] != -1) {
+    if (!m_noseed() && sd[0] != -1) {
         for (int c = 0; c < NC; ++c) {
             if (cont[c].index == sd[0])
                 *pA = c;
@@ -674,7 +769,11 @@ template <class R> int voronoi_poly(const System& S, Work<R>& W, int a, float ra
         if (!found) {
             add_vedge(vedge, edgenum, cont, B, C, A, poly, vn);
             ++edgenum;
-            if (edgenum >= EDGE_LIMIT) return -1;  // hull failure; VC->recalc == 0
+            if (edgenum >= EDGE_LIMIT) {  // hull failure; VC->recalc == 0
+                CAPMAX(vn, vn + 1);
+                CAPMAX(edges, edgenum);
+                return -1;
+            }
         }
         ++vn;
     }
@@ -718,7 +817,7 @@ template <class R> int voronoi_poly(const System& S, Work<R>& W, int a, float ra
         if (cont[c].Ai[3] > R(0)) cont[c].flag = 'E';
     }
     // seeds for atoms not yet done (LIB/Vcontacts.cpp:595-629)
-    for (int vi = 0; vi < vn; ++vi) {
+    for (int vi = 0; vi < vn && !m_noseed(); ++vi) {
         if (poly[vi].plane[2] == -1) continue;
         const int i0 = cont[poly[vi].plane[0]].index, i1 = cont[poly[vi].plane[1]].index, i2 = cont[poly[vi].plane[2]].index;
         int* s0 = &W.seed[3 * i0];
@@ -811,6 +910,12 @@ template <class R> void calc_areas(Work<R>& W, float rado, int NC, int NV, int a
             engplane[epi++] = X;
             if (epi == 3) break;
         }
+    }
+    if (t_branch) {
+        ++t_branch->atoms;
+        if (epi == 1) ++t_branch->epi1;
+        else if (epi == 2) ++t_branch->epi2;
+        else if (epi >= 3) ++t_branch->epi3;
     }
     if (eng) {
         R proj[3], p0[3], p1[3];
@@ -980,14 +1085,27 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
     const int Rn = S.R(), L = S.L(), N = S.N();
     fa_cf_result res{};
     std::vector<float> lxyz(3 * L);
-    decode_and_build(lig, icv, lxyz.data());
+    std::vector<int> rot(S.rec->n_flex > 0 ? S.rec->n_flex : 1, 0);
+    decode_and_build(lig, icv, lxyz.data(), S.rec, rot.data());
     if (xyz_out) std::memcpy(xyz_out, lxyz.data(), sizeof(float) * 3 * L);
     for (int i = 0; i < L; ++i) {
         W.x[Rn + i] = lxyz[3 * i];
         W.y[Rn + i] = lxyz[3 * i + 1];
         W.z[Rn + i] = lxyz[3 * i + 2];
     }
+    // side chains in the selected conformation (the atoms fatm[rot]..latm[rot]
+    // index_protein takes, LIB/Vcontacts.cpp:1598-1606)
+    for (int k = 0; k < S.rec->n_flex; ++k) {
+        const fa_flex_residue& f = S.rec->flex[k];
+        const float* c = f.rot_xyz + (size_t)rot[k] * f.n_atoms * 3;
+        for (int i = 0; i < f.n_atoms; ++i) {
+            W.x[f.first + i] = c[3 * i];
+            W.y[f.first + i] = c[3 * i + 1];
+            W.z[f.first + i] = c[3 * i + 2];
+        }
+    }
     if (!index_atoms(S, W)) {
+        res.cf = res.app = HUGE_VAL;  // not scored
         res.status = FA_GRID_RANGE;
         return res;
     }
@@ -998,7 +1116,7 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
     W.near_count = 0;
     for (int i = 0; i < N; ++i) {
         const int a = W.calclist[i];
-        if (a < Rn) continue;
+        if (S.slot[a] < 0) continue;
         get_contacts(S, W, a, W.rad[a] + RW, &clash);
     }
     CAPMAX(near_pose, W.near_count);
@@ -1015,7 +1133,7 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
     // calc_region (LIB/Vcontacts.cpp:74-139)
     for (int i = 0; i < N; ++i) {
         const int a = W.calclist[i];
-        if (a < Rn) continue;
+        if (S.slot[a] < 0) continue;
         const float rado = W.rad[a] + RW;
         const int NC = get_contacts(S, W, a, rado, nullptr);
         CAPMAX(nc, NC);
@@ -1025,6 +1143,7 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
         if ((int)W.centerpt.size() < NC) W.centerpt.resize(NC);
         int NV = voronoi_poly(S, W, a, rado, NC);
         if (NV == -1) {
+            if (t_branch) ++t_branch->hull_fail;
             res.wal = (double)POLY_PENALTY;
             res.cf = res.wal + 1.0;
             res.app = res.wal + 1.0;
@@ -1032,42 +1151,50 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
             return res;
         }
         if (NV < 0) {
+            res.cf = res.app = HUGE_VAL;
             res.status = FA_CAPACITY;
             return res;
         }
         int NV2 = order_faces(W, rado, NC, NV);
         if (NV2 < 0) {
+            res.cf = res.app = HUGE_VAL;
             res.status = FA_CAPACITY;
             return res;
         }
         calc_areas(W, rado, NC, NV, a);
         // save_areas (LIB/Vcontacts.cpp:1149-1220): own list in reverse
-        std::vector<fa_contact_rec>& rl = W.records[a - Rn];
+        std::vector<fa_contact_rec>& rl = W.records[S.slot[a]];
         for (int c = NC - 1; c >= 0; --c) {
             if (W.cont[c].area > R(0) && W.cont[c].flag != 'X')
                 rl.push_back({a, W.cont[c].index, (double)W.cont[c].area, (double)W.contlist[c].dist});
         }
-        W.done[a] = 'Y';
-        for (int c = 0; c < NC; ++c) {
-            const int j = W.cont[c].index;
-            if (W.done[j] != 'Y' && W.cont[c].area > R(0) && W.cont[c].flag != 'X' && j >= Rn)
-                W.stubs[j - Rn].push_back(a);
+        if (!m_nodone()) {
+            W.done[a] = 'Y';
+            for (int c = 0; c < NC; ++c) {
+                const int j = W.cont[c].index;
+                if (W.done[j] != 'Y' && W.cont[c].area > R(0) && W.cont[c].flag != 'X' && S.slot[j] >= 0)
+                    W.stubs[S.slot[j]].push_back(a);
+            }
         }
     }
-    // vcfunction (LIB/vcfunction.cpp:77-527), atoms in calc order
+    // vcfunction (LIB/vcfunction.cpp:77-527), scorable atoms in calc order,
+    // each residue's terms in its own cfstr (optres[].cf); ic2cf then adds the
+    // residues in optres order, the ligand first (LIB/ic2cf.cpp:143-245)
     const double permea = (double)prm->permeability;
     const double dee = (double)prm->dee_clash;
     const int nt = prm->ntypes;
-    for (int i = Rn; i < N; ++i) {
+    std::vector<fa_cf_result> acc(1 + S.rec->n_flex, fa_cf_result{});
+    for (int i : S.scorable) {
+        fa_cf_result& res = acc[S.res[i] + 1];
         const double radA = (double)W.rad[i];
         const double radoA = radA + RW;
         double SAS = 4.0 * PI_REF * radoA * radoA;
         const double surfA = SAS;
         // constraints of this atom in lookup order (atoms[].cons): a ligand
-        // pair is listed on both of its atoms
+        // pair is listed on both of its atoms; side-chain atoms carry none
         std::vector<const fa_constraint*> cons;
         std::vector<int> cons_partner;  // calc index
-        for (int q = 0; q < lig->n_cons; ++q) {
+        for (int q = 0; q < lig->n_cons && i >= Rn; ++q) {
             const fa_constraint& k = lig->cons[q];
             if (k.atom == i - Rn) {
                 cons.push_back(&k);
@@ -1079,16 +1206,19 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
         }
         for (const fa_constraint* k : cons)  // LIB/vcfunction.cpp:146-166
             if (k->type == 1) res.con += KDIST;
-        OPS(vc_contact, W.records[i - Rn].size());
-        for (const fa_contact_rec& r : W.records[i - Rn]) {
+        const std::vector<fa_contact_rec>& recs = W.records[S.slot[i]];
+        OPS(vc_contact, recs.size());
+        for (const fa_contact_rec& r : recs) {
             const int c = r.atom;
             const double rAB = radA + (double)W.rad[c];
             const double area = r.area;
             const int pk = (W.type[i] - 1) * nt + (W.type[c] - 1);
             SAS -= area;
-            const bool intra = (c >= Rn);
-            if (intra && lig->bonded[(c - Rn) * L + (i - Rn)]) continue;
-            if (intra && c < i) continue;  // FA->contacts: already visited
+            const bool intra = S.same_res(i, c);
+            if (intra && S.bonded(c, i)) continue;
+            if (S.slot[c] >= 0 && c < i) continue;  // FA->contacts: already visited
+            // both in one residue, or both protein residues (LIB/vcfunction.cpp:219-229)
+            const bool intramol = intra || (i < Rn && c < Rn);
             // covalent constraint (LIB/vcfunction.cpp:270-310)
             bool covalent = false;
             double dist_opt = 0.0;
@@ -1109,7 +1239,7 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
                 res.wal += (double)KWALL_F * (1.0 / (d6 * d6) - 1.0 / (cr6 * cr6));
                 if (r.dist <= dee * rAB) res.rclash = 1;
             }
-            if (!covalent && (prm->intramolecular || !intra)) res.com += pair_term(prm, pk, area, surfA);
+            if (!covalent && (prm->intramolecular || !intramol)) res.com += pair_term(prm, pk, area, surfA);
         }
         if (SAS < 0.0) SAS = 0.0;
         res.totsas += SAS;
@@ -1118,13 +1248,20 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
         else
             res.sas += pair_term(prm, (W.type[i] - 1) * nt + (nt - 1), SAS, surfA);
     }
+    for (const fa_cf_result& a : acc) {
+        res.com += a.com;
+        res.wal += a.wal;
+        res.sas += a.sas;
+        res.con += a.con;
+        res.totsas += a.totsas;
+        res.rclash |= a.rclash;
+    }
     res.cf = res.com + res.wal + res.sas + res.con;
     res.app = res.com + res.wal + res.sas;
     res.status = FA_OK;
-    if (recs_out) {
-        for (int i = 0; i < L; ++i)
-            for (const fa_contact_rec& r : W.records[i]) recs_out->push_back(r);
-    }
+    if (recs_out)  // atom by atom in calc order, as the ca_index lists are walked
+        for (int i : S.scorable)
+            for (const fa_contact_rec& r : W.records[S.slot[i]]) recs_out->push_back(r);
     return res;
 }
 
@@ -1151,8 +1288,9 @@ template <class R> void init_work(const System& S, Work<R>& W)
     W.calclist.assign(N, 0);
     W.done.assign(N, 'N');
     W.seed.assign(3 * N, -1);
-    W.stubs.assign(L, {});
-    W.records.assign(L, {});
+    W.rank.assign(N, -1);
+    W.stubs.assign(S.scorable.size(), {});
+    W.records.assign(S.scorable.size(), {});
     W.cont.resize(256);
     W.poly.resize(4096);
     W.centerpt.resize(256);
@@ -1162,7 +1300,8 @@ template <class R> void init_work(const System& S, Work<R>& W)
 
 template <class R>
 int eval_batch(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm, int n_poses,
-               const double* genes, fa_cf_result* out, int n_threads, fa_opcount* ops, CapStats* caps = nullptr)
+               const double* genes, fa_cf_result* out, int n_threads, fa_opcount* ops, CapStats* caps = nullptr,
+               BranchStats* branch = nullptr)
 {
     if (!rec || !lig || !prm || !genes || !out || n_poses < 0) return -1;
     System S{rec, lig, prm};
@@ -1176,8 +1315,10 @@ int eval_batch(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_
         init_work(S, W);
         fa_opcount local{};
         CapStats lcaps{};
+        BranchStats lbr{};
         t_ops = ops ? &local : nullptr;
         t_caps = caps ? &lcaps : nullptr;
+        t_branch = branch ? &lbr : nullptr;
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 4)
 #endif
@@ -1185,6 +1326,19 @@ int eval_batch(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_
             out[p] = eval_pose(S, W, genes + (size_t)p * lig->n_genes, nullptr, nullptr);
         t_ops = nullptr;
         t_caps = nullptr;
+        t_branch = nullptr;
+        if (branch) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+            {
+                branch->epi1 += lbr.epi1;
+                branch->epi2 += lbr.epi2;
+                branch->epi3 += lbr.epi3;
+                branch->atoms += lbr.atoms;
+                branch->hull_fail += lbr.hull_fail;
+            }
+        }
         if (caps) {
 #ifdef _OPENMP
 #pragma omp critical
@@ -1247,6 +1401,34 @@ int fa_oracle_cap_stats(const fa_receptor_desc* rec, const fa_ligand_desc* lig, 
     const int v[6] = {c.near_pose, c.nc, c.edges, c.vn, c.nv_opp, c.face_slots};
     for (int k = 0; k < 6; ++k) stats6[k] = v[k];
     return rc;
+}
+
+/* Counts of the calc_areas branches a batch reaches (5 values: atoms with one,
+ * two, three or more engulfing planes, atoms with a polyhedron, poses whose
+ * hull walk failed). */
+int fa_oracle_branch_stats(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm, int n_poses,
+                           const double* genes, fa_cf_result* out, int n_threads, long long* stats5)
+{
+    BranchStats b{};
+    const int rc = eval_batch<double>(rec, lig, prm, n_poses, genes, out, n_threads, nullptr, nullptr, &b);
+    const long long v[5] = {b.epi1, b.epi2, b.epi3, b.atoms, b.hull_fail};
+    for (int k = 0; k < 5; ++k) stats5[k] = v[k];
+    return rc;
+}
+
+/* 0: reference semantics (done/stub rule, hull seeds, calc order in a cell);
+ * 1: independent atoms, the form the GPU evaluates. */
+void fa_oracle_set_mode(int mode)
+{
+    g_eval_mode = mode ? 1 : 0;
+    g_mode_bits = mode ? 7 : 0;
+}
+
+/* The parts of the independent form one by one (bits as g_mode_bits). */
+void fa_oracle_set_mode_bits(int bits)
+{
+    g_mode_bits = bits & 7;
+    g_eval_mode = g_mode_bits == 7 ? 1 : 0;
 }
 
 /* 0: pose-build trig from the host libm (the reference binary's bits);

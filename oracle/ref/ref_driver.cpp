@@ -7,9 +7,17 @@
  * the oracle restatement; the shipped library never loads it.
  *
  * The in-memory system mirrors what LIB/top.cpp sets up before the GA:
- *   - residue 1 = the receptor (type 0), residue 2 = the ligand (type 1);
+ *   - receptor residues (type 0) in calc order: each flexible side chain is
+ *     a residue of its own, the rigid atoms between them one residue per
+ *     stretch; then the ligand (type 1);
  *   - atoms are 1-based, atom number == internal index (num_atm identity);
- *   - one optimised residue (the ligand), FA->mov[0] = rebuild order;
+ *   - rotamers as build_rotamers lays them out (LIB/build_rotamers.cpp:
+ *     103-172): each rotamer's copy of the residue's atoms appended after the
+ *     last atom, keeping the atoms' numbers, fatm[rot]/latm[rot] naming the
+ *     copy, the residue's bonded matrix (update_bonded) and optres (type 0)
+ *     appended after the ligand's, optres pointers on the non-backbone atoms
+ *     (update_optres), one typ-4 gene per side chain on fatm[0];
+ *   - one optimised residue rebuilt by buildcc (the ligand), FA->mov[0];
  *   - VC buffers as LIB/top.cpp:207-211 and :1011-1043 allocate them;
  *   - a first ic2cf call (the "initial structure", LIB/top.cpp:1132), then
  *     VC->recalc = 0, then the poses — the steady state the GA sees.
@@ -41,7 +49,10 @@ struct RefSystem {
     std::vector<energy_values> evals;
     std::vector<constraint> cons;
     std::vector<std::vector<constraint*>> cons_of;  // per atom number
-    int Rn = 0, L = 0, N = 0;
+    std::vector<std::vector<int>> rfatm, rlatm;     // per residue: fatm/latm per rotamer
+    std::vector<std::vector<int>> flex_bonded_data;
+    std::vector<std::vector<int*>> flex_bonded_rows;
+    int Rn = 0, L = 0, N = 0, lig_res = 0;
 
     ~RefSystem()
     {
@@ -71,19 +82,49 @@ void setup(RefSystem& S, const fa_receptor_desc* rec, const fa_ligand_desc* lig,
     const int N = S.N, Rn = S.Rn, L = S.L;
     S.FA = new FA_Global();
     FA_Global* FA = S.FA;
-    S.atoms.assign(N + 1, atom{});
-    S.residue.assign(3, resid{});
-    S.fatm = {1, Rn + 1};
-    S.latm = {Rn, N};
-    // residues
-    S.residue[1].type = 0;
-    S.residue[1].fatm = &S.fatm[0];
-    S.residue[1].latm = &S.latm[0];
-    S.residue[1].rot = 0;
-    S.residue[2].type = 1;
-    S.residue[2].fatm = &S.fatm[1];
-    S.residue[2].latm = &S.latm[1];
-    S.residue[2].rot = 0;
+    int n_copies = 0;
+    for (int k = 0; k < rec->n_flex; ++k) n_copies += (rec->flex[k].n_rot - 1) * rec->flex[k].n_atoms;
+    S.atoms.assign(N + n_copies + 1, atom{});
+    // residues in calc order: rigid stretches and side chains, then the ligand
+    std::vector<int> res_of(Rn, 0), flex_res(rec->n_flex, 0);
+    int nres = 0, pos = 0;
+    S.rfatm.assign(1, {});
+    S.rlatm.assign(1, {});
+    auto add_res = [&](int lo, int hi) {  // 0-based receptor atoms [lo, hi)
+        ++nres;
+        S.rfatm.push_back({lo + 1});
+        S.rlatm.push_back({hi});
+        for (int i = lo; i < hi; ++i) res_of[i] = nres;
+    };
+    for (int k = 0; k < rec->n_flex; ++k) {
+        const fa_flex_residue& f = rec->flex[k];
+        if (f.first > pos) add_res(pos, f.first);
+        add_res(f.first, f.first + f.n_atoms);
+        flex_res[k] = nres;
+        pos = f.first + f.n_atoms;
+    }
+    if (pos < Rn) add_res(pos, Rn);
+    S.lig_res = nres + 1;
+    S.rfatm.push_back({Rn + 1});
+    S.rlatm.push_back({N});
+    S.residue.assign(S.lig_res + 1, resid{});
+    // rotamer copies after the last atom (fatm[r] = start of copy r)
+    int next = N + 1;
+    for (int k = 0; k < rec->n_flex; ++k) {
+        const fa_flex_residue& f = rec->flex[k];
+        for (int r = 1; r < f.n_rot; ++r) {
+            S.rfatm[flex_res[k]].push_back(next);
+            S.rlatm[flex_res[k]].push_back(next + f.n_atoms - 1);
+            next += f.n_atoms;
+        }
+    }
+    for (int q = 1; q <= S.lig_res; ++q) {
+        S.residue[q].type = (q == S.lig_res) ? 1 : 0;
+        S.residue[q].fatm = S.rfatm[q].data();
+        S.residue[q].latm = S.rlatm[q].data();
+        S.residue[q].rot = 0;
+        S.residue[q].trot = (int)S.rfatm[q].size() - 1;
+    }
     S.bonded_data.assign((size_t)L * L, -1);
     S.bonded_rows.resize(L);
     for (int i = 0; i < L; ++i) {
@@ -91,15 +132,37 @@ void setup(RefSystem& S, const fa_receptor_desc* rec, const fa_ligand_desc* lig,
         for (int j = 0; j < L; ++j)
             if (lig->bonded[i * L + j]) S.bonded_rows[i][j] = (i == j) ? 0 : 1;
     }
-    S.residue[2].bonded = S.bonded_rows.data();
-    // optimised residue
-    S.optres.assign(1, OptRes{});
-    S.optres[0].rnum = 2;
+    S.residue[S.lig_res].bonded = S.bonded_rows.data();
+    S.flex_bonded_data.resize(rec->n_flex);
+    S.flex_bonded_rows.resize(rec->n_flex);
+    for (int k = 0; k < rec->n_flex; ++k) {
+        const fa_flex_residue& f = rec->flex[k];
+        const int n = f.n_atoms;
+        S.flex_bonded_data[k].assign((size_t)n * n, -1);
+        S.flex_bonded_rows[k].resize(n);
+        for (int i = 0; i < n; ++i) {
+            S.flex_bonded_rows[k][i] = &S.flex_bonded_data[k][(size_t)i * n];
+            for (int j = 0; j < n; ++j)
+                if (f.bonded[i * n + j]) S.flex_bonded_rows[k][i][j] = (i == j) ? 0 : 1;
+        }
+        S.residue[flex_res[k]].bonded = S.flex_bonded_rows[k].data();
+    }
+    // optimised residues: the ligand, then the side chains (build_rotamers appends them)
+    S.optres.assign(1 + rec->n_flex, OptRes{});
+    S.optres[0].rnum = S.lig_res;
     S.optres[0].type = 1;
     S.optres[0].tot = L;
+    for (int k = 0; k < rec->n_flex; ++k) {
+        S.optres[1 + k].rnum = flex_res[k];
+        S.optres[1 + k].type = 0;
+        S.optres[1 + k].tot = rec->flex[k].n_atoms;
+    }
     FA->optres = S.optres.data();
-    FA->num_optres = 1;
+    FA->num_optres = 1 + rec->n_flex;
     // atoms
+    std::vector<int> flex_of(Rn, -1);
+    for (int k = 0; k < rec->n_flex; ++k)
+        for (int i = 0; i < rec->flex[k].n_atoms; ++i) flex_of[rec->flex[k].first + i] = k;
     for (int i = 0; i < Rn; ++i) {
         atom& a = S.atoms[1 + i];
         a.coor[0] = rec->xyz[3 * i];
@@ -108,17 +171,32 @@ void setup(RefSystem& S, const fa_receptor_desc* rec, const fa_ligand_desc* lig,
         a.number = 1 + i;
         a.radius = rec->radius[i];
         a.type = rec->type[i];
-        a.ofres = 1;
+        a.ofres = res_of[i];
         a.recs = 'r';
         a.acs = -1.0f;
         a.optres = nullptr;
+        const int k = flex_of[i];
+        if (k >= 0) {
+            const fa_flex_residue& f = rec->flex[k];
+            const int w = i - f.first;
+            a.isbb = f.scorable[w] ? 0 : 1;
+            if (f.scorable[w]) {
+                a.optres = &S.optres[1 + k];
+                a.recs = 'm';
+            }
+            for (int r = 1; r < f.n_rot; ++r) {  // the rotamer copy keeps the number
+                atom& c = S.atoms[S.rfatm[flex_res[k]][r] + w];
+                c = a;
+                for (int q = 0; q < 3; ++q) c.coor[q] = f.rot_xyz[((size_t)r * f.n_atoms + w) * 3 + q];
+            }
+        }
     }
     for (int i = 0; i < L; ++i) {
         atom& a = S.atoms[Rn + 1 + i];
         a.number = Rn + 1 + i;
         a.radius = lig->radius[i];
         a.type = lig->type[i];
-        a.ofres = 2;
+        a.ofres = S.lig_res;
         a.recs = 'm';
         a.acs = -1.0f;
         a.dis = lig->dis[i];
@@ -160,14 +238,15 @@ void setup(RefSystem& S, const fa_receptor_desc* rec, const fa_ligand_desc* lig,
     FA->num_atm = S.num_atm.data();
     S.contacts.assign(MAX_ATOM_NUMBER, 0);
     FA->contacts = S.contacts.data();
-    FA->atm_cnt = N;
+    FA->atm_cnt = N + n_copies;
     FA->atm_cnt_real = N;
-    FA->res_cnt = 2;
-    // gene map
+    FA->res_cnt = S.lig_res;
+    // gene map (rotamer genes name the side chain's fatm[0])
     S.map_par.assign(lig->n_genes, optmap{});
     for (int g = 0; g < lig->n_genes; ++g) {
         S.map_par[g].typ = lig->gene_typ[g];
-        S.map_par[g].atm = Rn + 1 + lig->gene_atm[g];
+        S.map_par[g].atm = (lig->gene_typ[g] == FA_GENE_ROT) ? S.rfatm[flex_res[lig->gene_atm[g]]][0]
+                                                             : Rn + 1 + lig->gene_atm[g];
     }
     FA->map_par = S.map_par.data();
     FA->npar = lig->n_genes;
@@ -176,7 +255,7 @@ void setup(RefSystem& S, const fa_receptor_desc* rec, const fa_ligand_desc* lig,
     FA->nors = 1;
     FA->nmov[0] = L;
     FA->mov[0] = S.mov.data();
-    FA->opt_res[0] = 2;
+    FA->opt_res[0] = S.lig_res;
     // cleft grid
     S.grid.assign(lig->n_grid, gridpoint{});
     for (int g = 0; g < lig->n_grid; ++g) {
@@ -301,10 +380,14 @@ int fa_ref_eval(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa
         r.wal = cf.wal;
         r.sas = cf.sas;
         r.con = cf.con;
-        r.totsas = S.FA->optres[0].cf.totsas;
+        r.totsas = 0.0;
+        r.rclash = 0;
+        for (int q = 0; q < S.FA->num_optres; ++q) {  // every scored residue
+            r.totsas += S.FA->optres[q].cf.totsas;
+            r.rclash |= S.FA->optres[q].cf.rclash;
+        }
         r.cf = get_cf_evalue(&cf);
         r.app = get_apparent_cf_evalue(&cf);
-        r.rclash = S.FA->optres[0].cf.rclash;
         r.status = (S.FA->skipped != fail0) ? FA_HULL_FAIL : (S.FA->clashed != clash0) ? FA_CLASH : FA_OK;
         if (r.status != FA_OK) {
             r.totsas = 0.0;
@@ -316,7 +399,8 @@ int fa_ref_eval(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa
         if (rec_counts) {
             int n = 0;
             if (r.status == FA_OK) {
-                for (int i = S.Rn; i < S.N; ++i) {
+                for (int i = 0; i < S.N; ++i) {  // scorable atoms in calc order
+                    if (!S.VC->Calc[i].score) continue;  // receptor atoms hold reverse stubs only
                     for (int ci = S.VC->ca_index[i]; ci != -1; ci = S.VC->ca_rec[ci].prev) {
                         if (recs && n < recs_cap_per_pose) {
                             fa_contact_rec& o = recs[(size_t)p * recs_cap_per_pose + n];

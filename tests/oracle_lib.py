@@ -151,6 +151,27 @@ class trig_mode:
         self.lib.fa_oracle_set_trig(0)
 
 
+class gpu_mirror:
+    """The oracle computing exactly what the device computes: correctly
+    rounded pose-build trig (mode 1) and each cell's rigid atoms before its
+    scorable ones (mode bit 4; with side chains among the receptor atoms the
+    reference's calc-order cells differ from it only in summation order)."""
+
+    def __init__(self, lib):
+        self.lib = lib
+        lib.fa_oracle_set_mode_bits.argtypes = [C.c_int]
+        lib.fa_oracle_set_mode_bits.restype = None
+
+    def __enter__(self):
+        self.lib.fa_oracle_set_trig(1)
+        self.lib.fa_oracle_set_mode_bits(4)
+        return self.lib
+
+    def __exit__(self, *exc):
+        self.lib.fa_oracle_set_trig(0)
+        self.lib.fa_oracle_set_mode_bits(0)
+
+
 def oracle_pose(lib, desc, lig_index, genes1, cap=20000):
     genes1 = np.ascontiguousarray(genes1, np.float64)
     out = np.zeros(1, abi.CF_DTYPE)

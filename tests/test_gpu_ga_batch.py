@@ -56,7 +56,7 @@ def orc(oracle):
 
 def _cf_oracle(oracle, system, lig, ic):
     desc = abi.Descriptors(system.receptor, system.ligands, system.params)
-    with ol.trig_mode(oracle, 1):
+    with ol.gpu_mirror(oracle):
         return ol.oracle_eval(oracle, desc, lig, np.ascontiguousarray(ic))
 
 

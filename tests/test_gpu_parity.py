@@ -51,7 +51,7 @@ def desc0(sys0):
 
 @pytest.fixture(scope="module")
 def oracle_cr(oracle, desc0, sys0):
-    with ol.trig_mode(oracle, 1):
+    with ol.gpu_mirror(oracle):
         out, ops = ol.oracle_eval_counted(oracle, desc0, 0, sys0.genes[0])
     return out, ops
 
@@ -106,7 +106,7 @@ def test_density_matrix_against_oracle(hip, sys0, oracle, normalize):
         c.upload_system(s)
         g = c.eval(0, genes)
     desc = abi.Descriptors(s.receptor, s.ligands, prm)
-    with ol.trig_mode(oracle, 1):
+    with ol.gpu_mirror(oracle):
         ref = ol.oracle_eval(oracle, desc, 0, genes)
     assert np.array_equal(g["status"], ref["status"])
     ok = ref["status"] == 0
@@ -128,7 +128,7 @@ def test_constraints_against_oracle(hip, oracle):
         c.upload_system(s)
         g = c.eval(0, genes)
     desc = abi.Descriptors(s.receptor, s.ligands, s.params)
-    with ol.trig_mode(oracle, 1):
+    with ol.gpu_mirror(oracle):
         ref = ol.oracle_eval(oracle, desc, 0, genes)
     assert np.array_equal(g["status"], ref["status"])
     ok = ref["status"] == 0
@@ -142,7 +142,7 @@ def test_constraints_against_oracle(hip, oracle):
 def test_pose_build_bit_exact(ctx0, sys0, oracle, desc0):
     g = sys0.genes[0][:200]
     _, xyz, _ = ctx0.eval_debug(0, g)
-    with ol.trig_mode(oracle, 1):
+    with ol.gpu_mirror(oracle):
         for p in range(len(g)):
             ref = np.zeros((sys0.ligands[0].n_atoms, 3), np.float32)
             gp = np.ascontiguousarray(g[p])
@@ -154,7 +154,7 @@ def test_contact_records(ctx0, sys0, oracle, desc0):
     g = sys0.genes[0][:120]
     out, _, recs = ctx0.eval_debug(0, g)
     n_checked = 0
-    with ol.trig_mode(oracle, 1):
+    with ol.gpu_mirror(oracle):
         for p in range(len(g)):
             o, _, orec = ol.oracle_pose(oracle, desc0, 0, g[p])
             assert out[p]["status"] == o["status"]
@@ -192,7 +192,7 @@ def test_gene_clamping(ctx0, sys0, oracle, desc0):
     g[:, 0] = np.where(np.arange(40) % 2, 5000.0, -3.0)   # grid index beyond both limits
     g[:, 1:] += np.where(np.arange(40)[:, None] % 3 == 0, 400.0, 0.0)
     out = ctx0.eval(0, g)
-    with ol.trig_mode(oracle, 1):
+    with ol.gpu_mirror(oracle):
         ref = ol.oracle_eval(oracle, desc0, 0, g)
     assert np.array_equal(out["status"], ref["status"])
     ok = ref["status"] == 0
@@ -241,7 +241,7 @@ def test_slow_grid_path(hip, oracle):
     c.set_phase_profiling(False)
     assert np.array_equal(again["cf"], out["cf"])
     assert slow.sum() == n_slow and np.all(cyc > 0)
-    with ol.trig_mode(oracle, 1):
+    with ol.gpu_mirror(oracle):
         ref = ol.oracle_eval(oracle, d, 0, g)
     c.close()
     assert np.array_equal(out["status"], ref["status"])
@@ -278,7 +278,7 @@ def test_multi_ligand_launch(hip, oracle):
         sel = np.nonzero(lig_of == li)[0]
         alone = c.eval(li, genes[sel, :gl.shape[1]])
         assert alone.tobytes() == mixed[sel].tobytes(), li
-        with ol.trig_mode(oracle, 1):
+        with ol.gpu_mirror(oracle):
             ref = ol.oracle_eval(oracle, d, li, genes[sel, :gl.shape[1]])
         assert np.array_equal(alone["status"], ref["status"])
         ok = ref["status"] == 0
@@ -349,7 +349,7 @@ def test_config1_against_oracle(hip, oracle):
     ops = c.opcount()
     caps = c.counters()["capacity"]
     c.close()
-    with ol.trig_mode(oracle, 1):
+    with ol.gpu_mirror(oracle):
         ref, rops = ol.oracle_eval_counted(oracle, d, 0, s.genes[0])
     assert caps == 0
     _bit_level_parity(out, ref, min_ok=100)
@@ -368,7 +368,7 @@ def test_config3_receptor_sizes_against_oracle(hip, oracle):
         for li, gl in enumerate(s.genes):
             out = c.eval(li, gl)
             assert c.counters()["capacity"] == 0
-            with ol.trig_mode(oracle, 1):
+            with ol.gpu_mirror(oracle):
                 ref = ol.oracle_eval(oracle, d, li, gl)
             _bit_level_parity(out, ref, min_ok=0)
             n_ok += int((ref["status"] == 0).sum())
@@ -390,7 +390,7 @@ def test_config4_globule_against_oracle(hip, oracle):
     for li, gl in enumerate(s.genes):
         out = c.eval(li, gl)
         assert c.counters()["capacity"] == 0
-        with ol.trig_mode(oracle, 1):
+        with ol.gpu_mirror(oracle):
             ref = ol.oracle_eval(oracle, d, li, gl)
         _bit_level_parity(out, ref, min_ok=0)
         n_ok += int((ref["status"] == 0).sum())
@@ -418,7 +418,7 @@ def test_coarse_grid_receptor_against_oracle(hip, oracle):
         c.upload_system(s)
         out = c.eval(0, genes)
         caps = c.counters()["capacity"]
-    with ol.trig_mode(oracle, 1):
+    with ol.gpu_mirror(oracle):
         ref = ol.oracle_eval(oracle, d, 0, genes)
     assert caps == 0
     _bit_level_parity(out, ref, min_ok=5)
@@ -442,7 +442,7 @@ def test_largest_ligand_against_oracle(hip, oracle):
         c.upload_system(s)
         out = c.eval(0, s.genes[0])
         caps = c.counters()["capacity"]
-    with ol.trig_mode(oracle, 1):
+    with ol.gpu_mirror(oracle):
         ref = ol.oracle_eval(oracle, d, 0, s.genes[0])
     assert caps == 0
     _bit_level_parity(out, ref, min_ok=5)
@@ -461,7 +461,7 @@ def test_rna_receptor_file_against_oracle(hip, oracle):
             c.upload_system(s)
             out = c.eval(0, s.genes[0])
             caps = c.counters()["capacity"]
-        with ol.trig_mode(oracle, 1):
+        with ol.gpu_mirror(oracle):
             ref = ol.oracle_eval(oracle, d, 0, s.genes[0])
         assert caps == 0
         _bit_level_parity(out, ref, min_ok=5)
@@ -480,7 +480,7 @@ def test_rna_cleft_grid_against_oracle(hip, oracle):
         c.upload_system(s)
         out = c.eval(0, s.genes[0])
         caps = c.counters()["capacity"]
-    with ol.trig_mode(oracle, 1):
+    with ol.gpu_mirror(oracle):
         ref = ol.oracle_eval(oracle, d, 0, s.genes[0])
     assert caps == 0
     _bit_level_parity(out, ref, min_ok=100)

@@ -138,3 +138,135 @@ def test_strict_reference_bit_exact_rna_cleft_grid(oracle):
     assert np.array_equal(out["status"], r_out["status"])
     for f in ("com", "wal", "sas", "totsas", "cf", "rclash"):
         assert np.array_equal(out[f], r_out[f]), f
+
+
+def _branch_stats(oracle, desc, li, genes):
+    import ctypes as C
+    f = oracle.fa_oracle_branch_stats
+    f.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    g = np.ascontiguousarray(genes, np.float64)
+    out = np.zeros(len(g), abi.CF_DTYPE)
+    st = np.zeros(5, np.int64)
+    f(ol._addr(desc.receptor), ol._addr(desc.ligands[li]), ol._addr(desc.params), len(g), g.ctypes.data,
+      out.ctypes.data, 0, st.ctypes.data)
+    return out, dict(zip(("epi1", "epi2", "epi3", "atoms", "hull_fail"), (int(v) for v in st)))
+
+
+def test_strict_reference_bit_exact_flexible_side_chains(oracle):
+    """Config 1 as specified: five flexible side chains of 3 chi angles (ten
+    conformations each) lining the cavity, one rotamer gene each.  The
+    reference builds the rotamers as copies of the residue's atoms that keep
+    their numbers (LIB/build_rotamers.cpp:103-172), so its bonded lookups
+    through num_atm stay inside the residue for every rotamer; oracle and
+    strict build agree bit for bit on every term, the coordinates and every
+    record, over all rotamers."""
+    s = synth.make_config1(pop=300)
+    assert s.receptor.n_flex == 5 and s.ligands[0].n_genes == 19
+    desc = abi.Descriptors(s.receptor, s.ligands, s.params)
+    genes = s.genes[0]
+    assert len(np.unique(genes[:, 14:])) == 10            # every conformation drawn
+    r_out, r_xyz, r_recs = ol.ref_eval(ref, desc, 0, genes)
+    with ol.trig_mode(oracle, 0):
+        out = ol.oracle_eval(oracle, desc, 0, genes)
+        assert (out["status"] == 0).sum() >= 80
+        assert np.array_equal(out["status"], r_out["status"])
+        for f in ("com", "wal", "sas", "con", "totsas", "cf", "app", "rclash"):
+            assert np.array_equal(out[f], r_out[f]), f
+        n_sc = 0
+        for p in range(0, len(genes), 5):
+            o, xyz, orec = ol.oracle_pose(oracle, desc, 0, genes[p])
+            assert np.array_equal(xyz, r_xyz[p])
+            a = sorted((int(r["from"]), int(r["atom"]), float(r["area"]), float(r["dist"])) for r in orec)
+            b = sorted((int(r["from"]), int(r["atom"]), float(r["area"]), float(r["dist"])) for r in r_recs[p])
+            assert a == b, p
+            n_sc += sum(1 for r in orec if r["from"] < s.receptor.n_atoms)
+        assert n_sc > 0                                   # side-chain atoms made contacts of their own
+
+
+def test_rotamer_gene_selects_conformation(oracle):
+    """A rotamer gene of value v selects conformation (unsigned)(v + 0.5)
+    (LIB/ic2cf.cpp:90-97): the pose scores as the same pose on a receptor
+    whose residue is fixed in that conformation (one conformation only)."""
+    import copy
+    s = synth.make_config1(pop=40)
+    desc = abi.Descriptors(s.receptor, s.ligands, s.params)
+    genes = s.genes[0].copy()
+    k = 2                                                # the third side chain
+    for v, conf in ((0.0, 0), (2.49, 2), (2.5, 3), (6.7, 7), (9.0, 9)):
+        g = genes.copy()
+        g[:, 14 + k] = v
+        out = ol.oracle_eval(oracle, desc, 0, g)
+        rec = copy.deepcopy(s.receptor)
+        f = rec.flex[k]
+        rec.xyz[f.first:f.first + f.n_atoms] = f.rot_xyz[conf]
+        f.rot_xyz = f.rot_xyz[conf:conf + 1].copy()
+        lig = copy.deepcopy(s.ligands[0])
+        lig.gene_max[14 + k] = 0.0
+        g2 = g.copy()
+        g2[:, 14 + k] = 0.0
+        d2 = abi.Descriptors(rec, [lig], s.params)
+        fixed = ol.oracle_eval(oracle, d2, 0, g2)
+        assert np.array_equal(out.tobytes(), fixed.tobytes()), v
+        r2, _, _ = ol.ref_eval(ref, d2, 0, g2)
+        assert np.array_equal(fixed.tobytes(), r2.tobytes()), v
+
+
+def test_gpu_cell_order_equals_reference_order(oracle):
+    """The device takes a cell's rigid atoms before its scorable ones (both
+    in calc order); with side chains among the receptor atoms that differs
+    from the reference's calc-order cells, and changes no result beyond
+    summation order (bit 4 of the oracle's mode)."""
+    import ctypes as C
+    s = synth.make_config1(pop=600)
+    desc = abi.Descriptors(s.receptor, s.ligands, s.params)
+    a = ol.oracle_eval(oracle, desc, 0, s.genes[0])
+    oracle.fa_oracle_set_mode_bits.argtypes = [C.c_int]
+    oracle.fa_oracle_set_mode_bits(4)
+    try:
+        b = ol.oracle_eval(oracle, desc, 0, s.genes[0])
+    finally:
+        oracle.fa_oracle_set_mode_bits(0)
+    assert np.array_equal(a["status"], b["status"])
+    ok = a["status"] == 0
+    assert np.all(np.abs(a["cf"] - b["cf"])[ok] <= 1e-9 * np.maximum(1.0, np.abs(a["cf"][ok])))
+
+
+def test_engulfing_branches_reached_and_bit_exact(oracle):
+    """calc_areas' corrections for one, two and three or more engulfing
+    planes (LIB/Vcontacts.cpp:963-1042) are all reached (branch counters),
+    and the oracle equals the strict reference there bit for bit, except on
+    poses where the reference reads past a face's point list (a face with no
+    second point on its first edge line, LIB/Vcontacts.cpp:808-812): the
+    oracle reports those as not scored (FA_CAPACITY), as the device does."""
+    s = synth.make_engulf_system(n_poses=400)
+    desc = abi.Descriptors(s.receptor, s.ligands, s.params)
+    out, br = _branch_stats(oracle, desc, 0, s.genes[0])
+    assert br["epi1"] > 100 and br["epi2"] > 100 and br["epi3"] > 100, br
+    r_out, _, _ = ol.ref_eval(ref, desc, 0, s.genes[0])
+    keep = out["status"] != abi.FA_CAPACITY
+    assert keep.sum() >= len(keep) - 5
+    assert np.array_equal(out["status"][keep], r_out["status"][keep])
+    for f in ("com", "wal", "sas", "totsas", "cf"):
+        assert np.array_equal(out[f][keep], r_out[f][keep]), f
+    # the configs the record tests use reach the one-plane branch too
+    s0 = synth.make_config0(n_poses=200)
+    _, b0 = _branch_stats(oracle, abi.Descriptors(s0.receptor, s0.ligands, s0.params), 0, s0.genes[0])
+    assert b0["epi1"] > 0
+
+
+@pytest.mark.parametrize("n_shell", [80, 150])
+def test_hull_failure_against_reference(oracle, n_shell):
+    """A ligand inside a shell of receptor atoms with no clash: the hull walk
+    exceeds its 200-edge limit, Vcontacts returns -1 and vcfunction the
+    POLYHEDRON_PENALTY as wal, with metal_coord = 1 in the returned cfstr
+    (LIB/Vcontacts.cpp:378-407, LIB/vcfunction.cpp:56-71, LIB/ic2cf.cpp:
+    127-130): cf = 1e6 + 1.  Poses away from the shell score normally."""
+    s = synth.make_shell_system(n_shell=n_shell)
+    desc = abi.Descriptors(s.receptor, s.ligands, s.params)
+    r_out, _, _ = ol.ref_eval(ref, desc, 0, s.genes[0])
+    out, br = _branch_stats(oracle, desc, 0, s.genes[0])
+    assert np.array_equal(out.tobytes(), r_out.tobytes())
+    hf = out["status"] == abi.FA_HULL_FAIL
+    assert hf.sum() >= 30 and br["hull_fail"] == hf.sum()
+    assert np.all(out["wal"][hf] == 1e6) and np.all(out["cf"][hf] == 1e6 + 1.0)
+    assert np.all(out["status"][~hf] == abi.FA_OK)
