@@ -231,6 +231,7 @@ int fa_ctx_create(int device_id, int max_pop, fa_ctx** out)
     FA_HIP(hipSetDevice(device_id));
     hipDeviceProp_t p;
     FA_HIP(hipGetDeviceProperties(&p, device_id));
+    if (int rc = fa_noalloc_init_device(device_id)) return rc;
     fa_ctx* c = new fa_ctx();
     c->device = device_id;
     c->n_cu = p.multiProcessorCount;

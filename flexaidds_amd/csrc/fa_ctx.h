@@ -32,6 +32,10 @@ hipError_t launch_grand(const double* d_log_Z, int stride, const double* d_log_c
 // error state of the calling thread (fa_last_error)
 int fa_fail(int code, const std::string& msg);
 
+// static scratch of the allocation-free evaluation: its address on `device`
+// (the current device), looked up once (noalloc.hip)
+int fa_noalloc_init_device(int device);
+
 #define FA_HIP(expr)                                                                         \
     do {                                                                                     \
         hipError_t e_ = (expr);                                                              \

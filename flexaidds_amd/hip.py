@@ -57,6 +57,8 @@ _SIGS = {
     "fa_upload_system": ([_P, _P, _P, C.c_int, _P], C.c_int),
     "fa_eval_batch": ([_P, C.c_int, C.c_int, _P, _P], C.c_int),
     "fa_eval_batch_device": ([_P, C.c_int, _P, C.c_int, _P, _P, _P], C.c_int),
+    "fa_eval_batch_device_noalloc": ([_P, C.c_int, _P, C.c_int, _P, _P, _P], C.c_int),
+    "fa_noalloc_scratch_bytes": ([], C.c_size_t),
     "fa_eval_batch_debug": ([_P, C.c_int, C.c_int, _P, _P, _P, _P, C.c_int, _P], C.c_int),
     "fa_thermo_device": ([_P, _P, C.c_int, _P, C.c_int64, C.c_double, _P, _P], C.c_int),
     "fa_thermo_host": ([_P, _P, _P, C.c_int64, C.c_double, _P], C.c_int),
@@ -190,6 +192,13 @@ class Context:
         _check(_lib.fa_eval_batch_device(self._ctx, n_poses, C.c_void_p(genes_ptr), gene_stride,
                                          C.c_void_p(pose_lig_ptr) if pose_lig_ptr else None,
                                          C.c_void_p(out_ptr), C.c_void_p(stream) if stream else None))
+
+    def eval_device_noalloc(self, n_poses: int, genes_ptr: int, gene_stride: int, pose_lig_ptr: int, out_ptr: int,
+                            stream: int = 0):
+        """eval_device with the library's static scratch (allocates nothing)."""
+        _check(_lib.fa_eval_batch_device_noalloc(self._ctx, n_poses, C.c_void_p(genes_ptr), gene_stride,
+                                                 C.c_void_p(pose_lig_ptr) if pose_lig_ptr else None,
+                                                 C.c_void_p(out_ptr), C.c_void_p(stream) if stream else None))
 
     def thermo(self, energy: np.ndarray, temperature: float = 300.0, count=None) -> dict:
         e = np.ascontiguousarray(energy, np.float64)

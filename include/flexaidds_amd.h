@@ -13,6 +13,7 @@
 #ifndef FLEXAIDDS_AMD_H
 #define FLEXAIDDS_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "fa_types.h"
@@ -88,6 +89,22 @@ int fa_eval_batch(fa_ctx* ctx, int ligand, int n_poses, const double* genes, fa_
  * No reference counterpart (the reference path is synchronous per call). */
 int fa_eval_batch_device(fa_ctx* ctx, int n_poses, const double* genes, int gene_stride, const int* pose_lig,
                          fa_cf_result* out, void* stream);
+
+/* Allocation-free form of fa_eval_batch_device, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * Same arguments and results as fa_eval_batch_device; it calls no hipMalloc
+ * or hipFree: its scratch (launch counters, each resident wave's neighbour
+ * list and widened-grid cells) is one static device array of
+ * fa_noalloc_scratch_bytes() (128 MiB) per GPU, held until the process ends,
+ * so it runs as many resident waves as that array holds for the uploaded
+ * system (at most the normal count) and fails with FA_ERR_LIMIT when one
+ * wave's scratch does not fit.  Calls that overlap on one GPU race on the
+ * array (one call at a time per GPU); fa_last_counters and fa_last_opcount
+ * do not see these launches (fa_ctx_totals does). */
+int fa_eval_batch_device_noalloc(fa_ctx* ctx, int n_poses, const double* genes, int gene_stride, const int* pose_lig,
+                                 fa_cf_result* out, void* stream);
+size_t fa_noalloc_scratch_bytes(void);
 
 /* Debug/parity form of fa_eval_batch: also returns the rebuilt ligand
  * coordinates ([n_poses * n_atoms * 3], may be NULL) and each pose's contact
