@@ -83,6 +83,7 @@ struct Smem {
         } c;
         struct {
             float dis[MAXL], ang[MAXL], dih[MAXL];
+            int rot[FA_MAX_FLEX_RESIDUES];  // conformation of each flexible residue
         } b;                                // internal coordinates (pose build)
     } u;
     unsigned long long stub[2 * MAXL];      // stub[2m..2m+1]: atoms with a record of m
@@ -91,8 +92,6 @@ struct Smem {
     unsigned vpl[MAXV];                     // vertex planes p0 | p1 << 8 | p2 << 16
     float lx[MAXL], ly[MAXL], lz[MAXL];     // scorable atoms: side-chain atoms, then the ligand's
     float lr[MAXL];                         //   radius
-    int lcalc[MAXL];                        //   calc index
-    int rot[FA_MAX_FLEX_RESIDUES];          // conformation of each flexible residue
     int sbox[MAXL];                         // cell of the atom of each rank
     int nstart[MAXL + 1];                   // neighbour list of rank k: [nstart[k], nstart[k+1])
     int seed[3 * MAXL];                     // hull seeds (calc indices)
@@ -794,7 +793,7 @@ __device__ void build_pose(Smem& s, const DevReceptor& R, const DevLigand& lig, 
     const int n = lig.n_atoms;
     const int ns = R.n_sc;  // ligand atom i is scorable atom ns + i
     const int l = lane_id();
-    if (l < FA_MAX_FLEX_RESIDUES) s.rot[l] = 0;
+    if (l < FA_MAX_FLEX_RESIDUES) s.u.b.rot[l] = 0;
     for (int i = l; i < n; i += WAVE) {
         s.u.b.dis[i] = lig.dis[i];
         s.u.b.ang[i] = lig.ang[i];
@@ -833,7 +832,7 @@ __device__ void build_pose(Smem& s, const DevReceptor& R, const DevLigand& lig, 
                 unsigned r = (unsigned)(v + 0.5f);
                 const int nr = R.flex_nrot[atm];
                 if ((int)r >= nr) r = (unsigned)(nr - 1);
-                s.rot[atm] = (int)r;
+                s.u.b.rot[atm] = (int)r;
             } else if (typ == FA_GENE_DIH) {
                 s.u.b.dih[atm] = (float)v;
                 int j = atm, cat = lig.rec[4 * j + 3];
@@ -970,20 +969,25 @@ __device__ void build_pose(Smem& s, const DevReceptor& R, const DevLigand& lig, 
             s.ly[ns + a] = Y[q];
             s.lz[ns + a] = Z[q];
             s.lr[ns + a] = lig.radius[a];
-            s.lcalc[ns + a] = R.n + a;
         }
     }
     // side chains in the conformation their genes select (the atoms
     // fatm[rot]..latm[rot] that index_protein takes, LIB/Vcontacts.cpp:1598-1606)
     for (int m = l; m < ns; m += WAVE) {
-        const float4 p = R.sc_rot[R.sc_rot_off[m] + s.rot[R.sc_res[m]]];
+        const float4 p = R.sc_rot[R.sc_rot_off[m] + s.u.b.rot[R.sc_res[m]]];
         s.lx[m] = p.x;
         s.ly[m] = p.y;
         s.lz[m] = p.z;
         s.lr[m] = p.w;
-        s.lcalc[m] = R.sc_calc[m];
     }
     wave_sync();
+}
+
+// calc index of scorable atom m: side-chain atoms (m < n_sc) sit among the
+// receptor atoms, the ligand's follow the receptor
+__device__ __forceinline__ int calc_of(const DevReceptor& R, int m)
+{
+    return m < R.n_sc ? R.sc_calc[m] : R.n + (m - R.n_sc);
 }
 
 // ------------------------------------------------------------ pose grid
@@ -1211,7 +1215,8 @@ __device__ __forceinline__ SlotTab slot_table(const Smem& s, const DevGrid& G, c
 }
 
 // receptor cells from the precomputed cell list
-__device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig, const PoseGrid& pg, int Rn, int L,
+__device__ void cull_fast(const Smem& s, const DevReceptor& R, const DevGrid& G, const DevLigand& lig, const PoseGrid& pg,
+                          int Rn, int L,
                           const SlotTab& tab, int bz, int* span, const CullCtx& cc, CullAcc& acc)
 {
     const int l = lane_id();
@@ -1287,7 +1292,7 @@ __device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig,
                 c.y = s.ly[m];
                 c.z = s.lz[m];
                 c.r = s.lr[m];
-                c.j = s.lcalc[m];
+                c.j = calc_of(R, m);
                 c.lj = true;
             }
         }
@@ -1317,7 +1322,7 @@ __device__ void cull_fast(const Smem& s, const DevGrid& G, const DevLigand& lig,
 }
 
 // Ligand atoms of cell sb (ranks [lo, lo + nl) of the box order), visited.
-__device__ __forceinline__ void visit_ligand_cell(const Smem& s, const DevLigand& lig, int Rn, int L,
+__device__ __forceinline__ void visit_ligand_cell(const Smem& s, const DevReceptor& R, const DevLigand& lig, int Rn, int L,
                                                   const RankCells& rc, int sb, const CullCtx& cc, CullAcc& acc)
 {
     int lo, nl;
@@ -1332,7 +1337,7 @@ __device__ __forceinline__ void visit_ligand_cell(const Smem& s, const DevLigand
             c.y = s.ly[m];
             c.z = s.lz[m];
             c.r = s.lr[m];
-            c.j = s.lcalc[m];
+            c.j = calc_of(R, m);
             c.lj = true;
         }
         visit(cc, c, acc);
@@ -1440,7 +1445,7 @@ __device__ void cull_slow(const Smem& s, const DevReceptor& R, const DevLigand& 
             const Cand c{tx[q].x, tx[q].y, tx[q].z, tx[q].w, ti[q].x, 0, false};
             cand_emit(cc, c, mine, __int_as_float(ti[q].y), acc);
         }
-        visit_ligand_cell(s, lig, Rn, L, rc, sb, cc, acc);
+        visit_ligand_cell(s, R, lig, Rn, L, rc, sb, cc, acc);
     }
 }
 
@@ -1467,7 +1472,7 @@ __device__ void cull_slow_coarse(const Smem& s, const DevReceptor& R, const DevL
             if (ballot(v) == 0ull) continue;
             visit(cc, c, acc);
         }
-        visit_ligand_cell(s, lig, Rn, L, rc, sb, cc, acc);
+        visit_ligand_cell(s, R, lig, Rn, L, rc, sb, cc, acc);
     }
 }
 
@@ -1608,7 +1613,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 const int a = s.order[k];
                 CullCtx cc;
                 cc.a = a;
-                cc.acalc = s.lcalc[a];
+                cc.acalc = calc_of(R, a);
                 cc.ax = s.lx[a];
                 cc.ay = s.ly[a];
                 cc.az = s.lz[a];
@@ -1624,7 +1629,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 if (l == 0) s.nstart[k] = acc.n_near;
                 acc.total = acc.tested = acc.found = 0;
                 if (!pg.slow)
-                    cull_fast(s, G, lig, pg, Rn, L, tab, s.sbox[k], s.u.c.idx, cc, acc);
+                    cull_fast(s, R, G, lig, pg, Rn, L, tab, s.sbox[k], s.u.c.idx, cc, acc);
                 else if (pg.dim >= 3)
                     cull_slow(s, R, lig, pg, L, tab, s.sbox[k], cc, acc, wb, rcand);
                 else
@@ -1672,7 +1677,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
             const double permea = (double)prm.permeability;
             for (int k = 0; k < L && status == FA_OK; ++k) {
                 const int a = s.order[k];
-                const int acalc = s.lcalc[a];
+                const int acalc = calc_of(R, a);
                 const float ax = s.lx[a], ay = s.ly[a], az = s.lz[a];
                 const float ra = s.lr[a];
                 const float rado = ra + RW;
@@ -1695,7 +1700,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                         const int2 e = nidx[q];
                         p = nxyz[q];
                         mv = e.x < 0 ? -1 - e.x : -1;  // scorable neighbours are stored as -1 - index
-                        j = mv >= 0 ? s.lcalc[mv] : e.x;
+                        j = mv >= 0 ? calc_of(R, mv) : e.x;
                         sq = __int_as_float(e.y);
                         keep = mv < 0 || !bit_of(sk0, sk1, mv);
                     }
@@ -2678,7 +2683,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     if (!bit_of(dm0, dm1, i)) {
                         const double radoA = (double)s.lr[i] + RW;
                         const double SAS = 4.0 * PI_REF * radoA * radoA;
-                        const int ci = s.lcalc[i];
+                        const int ci = calc_of(R, i);
                         const int ti = ci < Rn ? R.type[ci] : lig.type[ci - Rn];
                         add_tot += SAS;
                         add_sas += (prm.solventterm != 0.0f)

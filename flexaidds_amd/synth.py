@@ -695,6 +695,63 @@ def ga_limits(lig: Ligand, angle_step=2.0):
     return out
 
 
+def _ligand_job(job):
+    """One ligand of a multi-ligand system from its own seeded stream
+    (picklable, so ligands can be built in worker processes): job =
+    (seed, k, kind, rec_xyz, center, ori, extra)."""
+    seed, k, kind, rec_xyz, center, ori, extra = job
+    rng = np.random.default_rng([int(seed), int(k)])
+    rec = Receptor(xyz=rec_xyz, radius=np.zeros(len(rec_xyz), np.float32), type=np.zeros(len(rec_xyz), np.int32),
+                   center=center)
+    if kind == "screen":            # config 2: 12-80 atoms, 0-15 torsions, a 10 A box
+        na = int(rng.integers(12, 81))
+        nt = int(rng.integers(0, 16))
+        pts = _box_points(rng, center, 256)
+        anchor = center
+    elif kind == "stream":          # config 3: log-normal sizes
+        na, nt = config3_ligand_size(rng)
+        pts = _box_points(rng, center, 256)
+        anchor = center
+    else:                           # config 4: 20-60 atoms over the globule's surface
+        R, n_grid = extra
+        na = int(rng.integers(20, 61))
+        nt = int(round(na / 5.0))
+        u = rng.random(n_grid)
+        r = np.cbrt((R - 4.0) ** 3 + u * ((R + 4.0) ** 3 - (R - 4.0) ** 3))
+        v = rng.normal(size=(n_grid, 3))
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        pts = center[None, :] + v * r[:, None]
+        anchor = center + v[0] * (R + 4.0)
+    lig = make_ligand(rng, na, nt, ori, anchor, pts)
+    finalize_frame(lig, rec)
+    return lig, rng
+
+
+def _build_ligands(jobs, pop, workers=None):
+    """Ligands of `jobs` and `pop` random gene vectors each, in worker
+    processes when there are many (results do not depend on the worker
+    count: every ligand has its own stream)."""
+    import os
+    import sys
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1) if len(jobs) >= 24 else 1
+    # spawned workers re-import the main module by path; a main read from
+    # stdin or `-c` has none, and its workers would die at start-up
+    main_file = getattr(sys.modules.get("__main__"), "__file__", None)
+    if main_file is not None and not os.path.isfile(main_file):
+        workers = 1
+    if workers > 1:
+        import concurrent.futures as cf
+        import multiprocessing as mp
+        with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+            built = list(ex.map(_ligand_job, jobs, chunksize=max(1, len(jobs) // (4 * workers))))
+    else:
+        built = [_ligand_job(j) for j in jobs]
+    ligs = [b[0] for b in built]
+    genes = [random_genes(b[1], b[0], pop) for b in built]
+    return ligs, genes
+
+
 def _box_points(rng, center, n, edge=10.0):
     return np.asarray(center)[None, :] + rng.uniform(-edge / 2, edge / 2, size=(n, 3))
 
@@ -743,20 +800,13 @@ def make_config1(pop=2000, seed=None, n_flex=5, n_rot=10) -> System:
 def make_config2(n_ligands=256, pop=512, seed=None) -> System:
     """Batched uneven screen: one 6,000-atom receptor, ligands with 12-80
     heavy atoms and 0-15 torsions, `pop` chromosomes each."""
-    rng = np.random.default_rng(CONFIG_SEED[2] if seed is None else seed)
+    base = CONFIG_SEED[2] if seed is None else seed
+    rng = np.random.default_rng(base)
     center = np.array([5.1, -12.4, 9.8])
     rec = make_receptor(rng, 6000, center, r_in=CAVITY_RADIUS)
     ori = rec.xyz.astype(np.float64).mean(axis=0).astype(np.float32)
     params = Params(ntypes=40, emat=make_emat(rng))
-    ligs, genes = [], []
-    for _ in range(n_ligands):
-        na = int(rng.integers(12, 81))
-        nt = int(rng.integers(0, 16))
-        pts = _box_points(rng, center, 256)
-        lig = make_ligand(rng, na, nt, ori, center, pts)
-        finalize_frame(lig, rec)
-        ligs.append(lig)
-        genes.append(random_genes(rng, lig, pop))
+    ligs, genes = _build_ligands([(base, k, "screen", rec.xyz, center, ori, None) for k in range(n_ligands)], pop)
     return System(receptor=rec, ligands=ligs, params=params, genes=genes, config=2)
 
 
@@ -772,23 +822,22 @@ def make_config3(n_receptors=8, ligands_per_receptor=16, seed=None, n_poses=8) -
     (cleft shells as in config 0), each with a resident library of
     `ligands_per_receptor` ligands sized as config3_ligand_size draws.  A
     request names a receptor and one of its ligands."""
-    rng = np.random.default_rng(CONFIG_SEED[3] if seed is None else seed)
+    base = CONFIG_SEED[3] if seed is None else seed
+    rng = np.random.default_rng(base)
     sizes = np.linspace(2000, 10000, n_receptors).round().astype(int)
-    systems = []
-    for n_rec in sizes:
+    recs, jobs = [], []
+    for q, n_rec in enumerate(sizes):
         center = rng.uniform(-30.0, 30.0, size=3)
         rec = make_receptor(rng, int(n_rec), center, r_in=CAVITY_RADIUS)
         ori = rec.xyz.astype(np.float64).mean(axis=0).astype(np.float32)
         params = Params(ntypes=40, emat=make_emat(rng))
-        ligs, genes = [], []
-        for _ in range(ligands_per_receptor):
-            na, nt = config3_ligand_size(rng)
-            pts = _box_points(rng, center, 256)
-            lig = make_ligand(rng, na, nt, ori, center, pts)
-            finalize_frame(lig, rec)
-            ligs.append(lig)
-            genes.append(random_genes(rng, lig, n_poses))
-        systems.append(System(receptor=rec, ligands=ligs, params=params, genes=genes, config=3))
+        recs.append((rec, params))
+        jobs += [(base, q * 1000 + k, "stream", rec.xyz, center, ori, None) for k in range(ligands_per_receptor)]
+    ligs, genes = _build_ligands(jobs, n_poses)
+    systems = []
+    for q, (rec, params) in enumerate(recs):
+        sl = slice(q * ligands_per_receptor, (q + 1) * ligands_per_receptor)
+        systems.append(System(receptor=rec, ligands=ligs[sl], params=params, genes=genes[sl], config=3))
     return systems
 
 
@@ -799,26 +848,15 @@ def make_config4(n_ligands=64, pop=8, n_rec=20000, n_grid=2000, seed=None):
     surface (points uniform in a shell from 4 Å inside to 4 Å outside the
     globule's radius), and concentrations log-uniform over 1 nM - 1 mM.
     Returns (system, concentrations in M)."""
-    rng = np.random.default_rng(CONFIG_SEED[4] if seed is None else seed)
+    base = CONFIG_SEED[4] if seed is None else seed
+    rng = np.random.default_rng(base)
     center = np.array([0.0, 0.0, 0.0])
     R = (3.0 * n_rec * DENSITY_VOL / (4.0 * math.pi)) ** (1.0 / 3.0)
     rec = make_receptor(rng, n_rec, center, r_in=0.0, r_out=R, solid=True)
     ori = rec.xyz.astype(np.float64).mean(axis=0).astype(np.float32)
     params = Params(ntypes=40, emat=make_emat(rng))
-    ligs, genes = [], []
-    for _ in range(n_ligands):
-        na = int(rng.integers(20, 61))
-        nt = int(round(na / 5.0))
-        u = rng.random(n_grid)
-        r = np.cbrt((R - 4.0) ** 3 + u * ((R + 4.0) ** 3 - (R - 4.0) ** 3))
-        v = rng.normal(size=(n_grid, 3))
-        v /= np.linalg.norm(v, axis=1, keepdims=True)
-        pts = center[None, :] + v * r[:, None]
-        anchor = center + v[0] * (R + 4.0)
-        lig = make_ligand(rng, na, nt, ori, anchor, pts)
-        finalize_frame(lig, rec)
-        ligs.append(lig)
-        genes.append(random_genes(rng, lig, pop))
+    ligs, genes = _build_ligands([(base, k, "blind", rec.xyz, center, ori, (R, n_grid)) for k in range(n_ligands)],
+                                 pop)
     conc = np.exp(rng.uniform(math.log(1e-9), math.log(1e-3), n_ligands))
     return System(receptor=rec, ligands=ligs, params=params, genes=genes, config=4), conc
 

@@ -32,6 +32,10 @@ def system_digest(system, lig_index) -> str:
     r = system.receptor
     for a in (r.xyz, r.radius, r.type):
         h.update(np.ascontiguousarray(a).tobytes())
+    for f in getattr(r, "flex", None) or []:  # flexible side chains (absent: rigid receptor)
+        h.update(np.int64(f.first).tobytes())
+        for a in (f.rot_xyz, f.scorable, f.bonded):
+            h.update(np.ascontiguousarray(a).tobytes())
     lig = system.ligands[lig_index]
     for name in ("rec", "dis", "ang", "dih", "shift", "radius", "type", "mov", "bonded", "gene_typ", "gene_atm",
                  "gene_min", "gene_max", "grid_ic"):
@@ -60,6 +64,8 @@ def main():
     s2 = synth.make_config2(n_ligands=4, pop=64)
     for li in range(4):
         make(s2, li, np.ascontiguousarray(s2.genes[li][:32]), 4, os.path.join(HERE, f"ref_config2_lig{li}.npz"))
+    s1 = synth.make_config1(pop=200)  # five flexible side chains, rotamer genes
+    make(s1, 0, np.ascontiguousarray(s1.genes[0][:96]), 12, os.path.join(HERE, "ref_config1_flex.npz"))
 
 
 if __name__ == "__main__":

@@ -19,6 +19,8 @@ FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "ref_*.npz")))
 def _system(config):
     if config == 0:
         return synth.make_config0(n_poses=1000)
+    if config == 1:
+        return synth.make_config1(pop=200)
     return synth.make_config2(n_ligands=4, pop=64)
 
 
@@ -28,7 +30,7 @@ def systems():
 
 
 def test_fixtures_present():
-    assert len(FIXTURES) >= 5
+    assert len(FIXTURES) >= 6
 
 
 @pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
