@@ -286,4 +286,15 @@ double boltzmann_batch(std::span<const double> energies, double beta, std::vecto
     return logz;
 }
 
+BoltzmannBatch boltzmann_batch(std::span<const double> energies, double beta)
+{
+    BoltzmannBatch r;
+    if (energies.empty()) return r;
+    r.log_Z = boltzmann_batch(energies, beta, r.weights);
+    r.E_min = energies[0];
+    for (double e : energies)
+        if (e < r.E_min) r.E_min = e;
+    return r;
+}
+
 }  // namespace rocm_dispatch

@@ -107,4 +107,14 @@ double log_sum_exp(std::span<const double> values);
 // returns log_Z
 double boltzmann_batch(std::span<const double> energies, double beta, std::vector<double>& weights);
 
+// The same with hw::BoltzmannBatchResult's fields (LIB/UnifiedHardwareDispatch.h:
+// 124-129, less the telemetry): weights, log_Z = ln sum w - beta E_min, E_min.
+// An empty ensemble gives {}, 0, 0 as the reference's early return (:768-772).
+struct BoltzmannBatch {
+    std::vector<double> weights;
+    double log_Z = 0.0;
+    double E_min = 0.0;
+};
+BoltzmannBatch boltzmann_batch(std::span<const double> energies, double beta);
+
 }  // namespace rocm_dispatch
