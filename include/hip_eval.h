@@ -17,7 +17,7 @@
 // with runtime device detection instead of the compile-time flag (:132-137).
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>  // hipStream_t; host compilers need -D__HIP_PLATFORM_AMD__
 
 #include <cstddef>
 #include <span>

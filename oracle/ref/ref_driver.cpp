@@ -222,8 +222,10 @@ void setup(RefSystem& S, const fa_receptor_desc* rec, const fa_ligand_desc* lig,
         c.max_dist = k.max_dist;
         const int a1 = Rn + 1 + k.atom;
         const int a2 = k.partner_is_ligand ? Rn + 1 + k.partner : 1 + k.partner;
-        c.anum1 = a1;
+        c.anum1 = a1;  // PDB numbers equal internal ones here (num_atm is the identity)
         c.anum2 = a2;
+        c.inum1 = a1;  // internal numbers (assign_constraint, LIB/assign_constraint.cpp:92-93)
+        c.inum2 = a2;
         S.cons_of[a1].push_back(&c);
         S.cons_of[a2].push_back(&c);
     }
@@ -318,7 +320,6 @@ void setup(RefSystem& S, const fa_receptor_desc* rec, const fa_ligand_desc* lig,
     FA->use_hbond = 0;
     FA->use_elec = 0;
     FA->use_metal_coord = 0;
-    FA->num_constraints = 0;
     // Vcontacts workspace (LIB/top.cpp:207-211, :1011-1043)
     VC_Global* VC = (VC_Global*)calloc(1, sizeof(VC_Global));
     S.VC = VC;
@@ -358,6 +359,30 @@ cfstr run_pose(RefSystem& S, const fa_ligand_desc* lig, const double* genes)
 }  // namespace
 
 extern "C" {
+
+/* The reference structures of a system (what LIB/top.cpp sets up), for
+ * harnesses that drive reference-side code on them: an opaque handle, its
+ * parts (FA_Global*, VC_Global*, atom* 1-based, resid* 1-based, gridpoint*),
+ * and its release. */
+void* fa_ref_system_create(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm)
+{
+    if (!rec || !lig || !prm) return nullptr;
+    RefSystem* S = new RefSystem();
+    setup(*S, rec, lig, prm);
+    return S;
+}
+
+void fa_ref_system_parts(void* h, void** FA, void** VC, void** atoms, void** residue, void** grid)
+{
+    RefSystem* S = static_cast<RefSystem*>(h);
+    *FA = S->FA;
+    *VC = S->VC;
+    *atoms = S->atoms.data();
+    *residue = S->residue.data();
+    *grid = S->grid.data();
+}
+
+void fa_ref_system_destroy(void* h) { delete static_cast<RefSystem*>(h); }
 
 /* Evaluate poses serially with the reference ic2cf.  `lig_xyz` (optional)
  * receives the rebuilt ligand coordinates per pose; `recs`/`rec_counts`
