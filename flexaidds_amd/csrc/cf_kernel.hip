@@ -1321,6 +1321,193 @@ __device__ void cull_fast(const Smem& s, const DevReceptor& R, const DevGrid& G,
     }
 }
 
+// Inclusive prefix minimum over the wave (all lanes active).
+template <int CTRL>
+__device__ __forceinline__ int dpp_min_shr(int v)
+{
+    const int o = __builtin_amdgcn_update_dpp(INT_MAX, v, CTRL, 0xF, 0xF, false);  // no source: INT_MAX
+    return v < o ? v : o;
+}
+
+__device__ __forceinline__ int wave_incl_min(int v)
+{
+    v = dpp_min_shr<0x111>(v);  // row_shr:1
+    v = dpp_min_shr<0x112>(v);  // row_shr:2
+    v = dpp_min_shr<0x114>(v);  // row_shr:4
+    v = dpp_min_shr<0x118>(v);  // row_shr:8
+    const int r0 = readlane_i(v, 15), r1 = readlane_i(v, 31), r2 = readlane_i(v, 47);
+    const int l = lane_id();
+    if (l >= 16) v = v < r0 ? v : r0;
+    if (l >= 32) v = v < r1 ? v : r1;
+    if (l >= 48) v = v < r2 ? v : r2;
+    return v;
+}
+
+// The neighbourhood list of one atom's cell (grids with neighbourhood
+// lists): wave-uniform start and length, and on lane s < 27 where slot s's
+// receptor atoms end in it.
+struct NbTab {
+    int start, len;
+    int bend;
+};
+
+__device__ __forceinline__ NbTab nb_table(const Smem& s, const DevGrid& G, int k)
+{
+    const int b = uni(s.sbox[k]);
+    NbTab t;
+    t.start = G.nb_start[b];
+    t.len = G.nb_start[b + 1] - t.start;
+    const int l = lane_id();
+    t.bend = l < 27 ? G.nb_bnd[28 * b + l + 1] : INT_MAX;
+    return t;
+}
+
+// The 27-cell scan of one atom over the cell's neighbourhood list (the
+// receptor atoms, slot by slot) and the scorable atoms, emitted in the
+// reference's order: per slot its receptor atoms, then its scorable atoms
+// in rank order.  With three or more cells a side a slot names one cell and
+// the slots' cells increase with the slot, so the ranks (sorted by cell)
+// list the scorable candidates in scan order.  Positions follow from two
+// per-slot prefix counts: in-range scorable candidates of the slots before
+// (lin, built from the first in-range candidate of each slot and a suffix
+// minimum) and in-range receptor atoms of the slots up to each slot (rin,
+// read off the receptor chunks at the slot ends).
+__device__ void cull_nb(const Smem& s, const DevReceptor& R, const DevGrid& G, const PoseGrid& pg, int L, int bz,
+                        const NbTab& tab, int* tbl, const CullCtx& cc, CullAcc& acc)
+{
+    const int l = lane_id();
+    const int base0 = acc.n_near;
+    // -- scorable candidates, in rank order
+    int sig[LSL];
+    float ssq[LSL];
+    unsigned long long im[LSL];
+#pragma unroll
+    for (int q = 0; q < LSL; ++q) {
+        const int r = l + WAVE * q;
+        Cand c{0.f, 0.f, 0.f, 0.f, -1, 0, false};
+        sig[q] = -1;
+        if (q * WAVE < L && r < L) {
+            const int sl = slot_of(pg, s.sbox[r] - bz);
+            if (sl >= 0) {
+                const int m = s.order[r];
+                c.m = m;
+                c.x = s.lx[m];
+                c.y = s.ly[m];
+                c.z = s.lz[m];
+                c.r = s.lr[m];
+                c.j = calc_of(R, m);
+                c.lj = true;
+                sig[q] = sl;
+            }
+        }
+        ssq[q] = 0.0f;
+        const bool in = (q * WAVE < L) ? cand_test(cc, c, acc, ssq[q]) : false;
+        im[q] = ballot(in);
+    }
+    static_assert(LSL == 2, "two rank slots per lane");
+    const int n0 = __popcll(im[0]);
+    const int nL = n0 + __popcll(im[1]);
+    // first in-range candidate of each slot: an in-range candidate whose
+    // previous in-range candidate (in rank order) has a smaller slot
+    if (l < 32) tbl[l] = nL;
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < LSL; ++q) {
+        const bool in = (im[q] >> l) & 1ull;
+        const unsigned long long below = im[q] & ((1ull << l) - 1ull);
+        int prev_lane = -1, prev_q = q;
+        if (below) {
+            prev_lane = 63 - __clzll((long long)below);
+        } else if (q == 1 && im[0]) {
+            prev_lane = 63 - __clzll((long long)im[0]);
+            prev_q = 0;
+        }
+        const int p0 = __shfl(sig[0], prev_lane < 0 ? 0 : prev_lane);
+        const int p1 = __shfl(sig[1], prev_lane < 0 ? 0 : prev_lane);
+        const int psig = prev_lane < 0 ? -1 : (prev_q == 0 ? p0 : p1);
+        if (in && psig < sig[q]) tbl[sig[q]] = (q ? n0 : 0) + lanes_below(im[q]);
+    }
+    wave_sync();
+    // lin(s) = in-range scorable candidates of slots < s: the first index of
+    // slot s or of the next slot that has one (a suffix minimum, run as a
+    // prefix minimum over lanes holding slots 26 - l)
+    const int lin_rev = wave_incl_min(l < 27 ? tbl[26 - l] : nL);
+    wave_sync();  // the table is read before the next atom writes it
+
+    // -- receptor atoms of the 27 slots
+    const int T = tab.len;
+    int nR = 0;
+    int rin = -1;  // lane s < 27: in-range receptor atoms of slots <= s
+    auto fetch = [&](int b, float4& p, int& a) {
+        const int t = b + l;
+        a = 0;
+        p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (t < T) {
+            p = G.nb_xyzr[tab.start + t];
+            a = G.nb_atom[tab.start + t];
+        }
+    };
+    auto visit_r = [&](int b, const float4& p, int a) {
+        const bool ok = b + l < T;  // entries pack calc index | slot << 27 (any sign)
+        Cand c{p.x, p.y, p.z, p.w, ok ? (a & 0x7FFFFFF) : -1, 0, false};
+        float sq;
+        const bool in = cand_test(cc, c, acc, sq);
+        const unsigned long long mk = ballot(in);
+        const int sl = ok ? (int)((unsigned)a >> 27) : 0;
+        const int lin = __shfl(lin_rev, 26 - sl);
+        if (in) {
+            const int pos = base0 + nR + lanes_below(mk) + lin;
+            if (pos < cc.cap) {
+                cc.nidx[pos] = make_int2(c.j, __float_as_int(sq));
+                cc.nxyz[pos] = make_float4(c.x, c.y, c.z, c.r);
+            }
+        }
+        const int e = tab.bend - b;
+        if (e >= 0 && e < WAVE) rin = nR + __popcll(mk & ((1ull << e) - 1ull));
+        nR += __popcll(mk);
+    };
+    if (T > 0) {
+        // four chunks in flight, as the cell-list scan
+        float4 p0, p1 = make_float4(0.f, 0.f, 0.f, 0.f), p2 = p1, p3 = p1;
+        int a0, a1 = 0, a2 = 0, a3 = 0;
+        fetch(0, p0, a0);
+        if (WAVE < T) fetch(WAVE, p1, a1);
+        if (2 * WAVE < T) fetch(2 * WAVE, p2, a2);
+        if (3 * WAVE < T) fetch(3 * WAVE, p3, a3);
+        for (int b = 0; b < T; b += 4 * WAVE) {
+            visit_r(b, p0, a0);
+            if (b + 4 * WAVE < T) fetch(b + 4 * WAVE, p0, a0);
+            if (b + WAVE >= T) break;
+            visit_r(b + WAVE, p1, a1);
+            if (b + 5 * WAVE < T) fetch(b + 5 * WAVE, p1, a1);
+            if (b + 2 * WAVE >= T) break;
+            visit_r(b + 2 * WAVE, p2, a2);
+            if (b + 6 * WAVE < T) fetch(b + 6 * WAVE, p2, a2);
+            if (b + 3 * WAVE >= T) break;
+            visit_r(b + 3 * WAVE, p3, a3);
+            if (b + 7 * WAVE < T) fetch(b + 7 * WAVE, p3, a3);
+        }
+    }
+    if (rin < 0) rin = nR;  // slots ending at the list's end
+
+    // -- the scorable candidates in range, after the receptor atoms of their slot
+#pragma unroll
+    for (int q = 0; q < LSL; ++q) {
+        const bool in = (im[q] >> l) & 1ull;
+        const int rs = __shfl(rin, sig[q] < 0 ? 0 : sig[q]);
+        if (in) {
+            const int pos = base0 + rs + (q ? n0 : 0) + lanes_below(im[q]);
+            if (pos < cc.cap) {
+                const int m = s.order[l + WAVE * q];
+                cc.nidx[pos] = make_int2(-1 - m, __float_as_int(ssq[q]));
+                cc.nxyz[pos] = make_float4(s.lx[m], s.ly[m], s.lz[m], s.lr[m]);
+            }
+        }
+    }
+    acc.n_near = base0 + nR + nL;
+    if (acc.n_near > cc.cap) acc.over = true;
+}
+
 // Ligand atoms of cell sb (ranks [lo, lo + nl) of the box order), visited.
 __device__ __forceinline__ void visit_ligand_cell(const Smem& s, const DevReceptor& R, const DevLigand& lig, int Rn, int L,
                                                   const RankCells& rc, int sb, const CullCtx& cc, CullAcc& acc)
@@ -1606,10 +1793,23 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 }
                 wave_sync();
             }
-            SlotTab tab = slot_table(s, G, pg, 0);
+            // neighbourhood lists when the pose keeps the receptor's grid
+            const bool use_nb = !pg.slow && G.nb_start != nullptr;
+            SlotTab tab{-1, 0, 0};
+            NbTab ntab{0, 0, INT_MAX};
+            if (use_nb)
+                ntab = nb_table(s, G, 0);
+            else
+                tab = slot_table(s, G, pg, 0);
             for (int k = 0; k < L; ++k) {
                 SlotTab next{-1, 0, 0};
-                if (k + 1 < L) next = slot_table(s, G, pg, k + 1);  // in flight during this atom
+                NbTab nnext{0, 0, INT_MAX};
+                if (k + 1 < L) {  // in flight during this atom
+                    if (use_nb)
+                        nnext = nb_table(s, G, k + 1);
+                    else
+                        next = slot_table(s, G, pg, k + 1);
+                }
                 const int a = s.order[k];
                 CullCtx cc;
                 cc.a = a;
@@ -1628,7 +1828,9 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 cc.cap = B.near_cap - SLOW_TMP;  // the tail parks slow-path entries
                 if (l == 0) s.nstart[k] = acc.n_near;
                 acc.total = acc.tested = acc.found = 0;
-                if (!pg.slow)
+                if (use_nb)
+                    cull_nb(s, R, G, pg, L, uni(s.sbox[k]), ntab, s.u.c.idx, cc, acc);
+                else if (!pg.slow)
                     cull_fast(s, R, G, lig, pg, Rn, L, tab, s.sbox[k], s.u.c.idx, cc, acc);
                 else if (pg.dim >= 3)
                     cull_slow(s, R, lig, pg, L, tab, s.sbox[k], cc, acc, wb, rcand);
@@ -1651,6 +1853,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                         totq[1] = total;
                 }
                 tab = next;
+                ntab = nnext;
             }
             if (l == 0) s.nstart[L] = acc.n_near;
             count_op(s.ops, OP_CULL, pre_tested);

@@ -22,6 +22,7 @@ thread_local std::string g_err;
 constexpr float CELLSIZE = 6.5f;  // LIB/Vcontacts.h:56
 constexpr int NEAR_PER_ATOM = 160;
 constexpr int NEAR_TAIL = 192;  // the pose kernel parks slow-path candidates past the lists (SLOW_TMP)
+constexpr long long NB_MAX_CELLS = 1LL << 18;  // grids with more cells scan the cell lists instead
 }  // namespace
 
 int fa_fail(int code, const std::string& msg)
@@ -436,6 +437,42 @@ int fa_upload_system(fa_ctx* c, const fa_receptor_desc* r, const fa_params* prm,
                 (rc = upload(c, atom.data(), n_rigid, &G.cell_atom)) ||
                 (rc = upload(c, xyzr.data(), n_rigid, &G.cell_xyzr)))
                 return rc;
+            // neighbourhood lists: the 27-slot scan of every cell laid out once
+            // (slot s names cell c + (s/9 - 1) dim^2 + ((s/3)%3 - 1) dim + s%3 - 1
+            // when that index is inside the grid, LIB/Vcontacts.cpp:1738-1741)
+            if (G.dim >= 3 && dim3 <= NB_MAX_CELLS && Rn < (1 << 27)) {  // entries pack index | slot << 27
+                std::vector<int> nb_start((size_t)dim3 + 1, 0), nb_bnd((size_t)dim3 * 28);
+                for (long long b = 0; b < dim3; ++b) {
+                    long long n = 0;
+                    for (int sl = 0; sl < 27; ++sl) {
+                        nb_bnd[(size_t)b * 28 + sl] = (int)n;
+                        const long long nbx = b + dim2 * (sl / 9 - 1) + G.dim * ((sl / 3) % 3 - 1) + sl % 3 - 1;
+                        if (nbx >= 0 && nbx < dim3) n += cnt[nbx + 1] - cnt[nbx];
+                    }
+                    nb_bnd[(size_t)b * 28 + 27] = (int)n;
+                    if ((long long)nb_start[b] + n > (1LL << 30)) return fa_fail(FA_ERR_LIMIT, "neighbourhood lists too large");
+                    nb_start[b + 1] = nb_start[b] + (int)n;
+                }
+                const size_t total = (size_t)nb_start[dim3];
+                std::vector<int> nb_atom(total);
+                std::vector<float4> nb_xyzr(total);
+                size_t w = 0;
+                for (long long b = 0; b < dim3; ++b)
+                    for (int sl = 0; sl < 27; ++sl) {
+                        const long long nbx = b + dim2 * (sl / 9 - 1) + G.dim * ((sl / 3) % 3 - 1) + sl % 3 - 1;
+                        if (nbx < 0 || nbx >= dim3) continue;
+                        for (int q = cnt[nbx]; q < cnt[nbx + 1]; ++q) {
+                            nb_atom[w] = atom[q] | (sl << 27);
+                            nb_xyzr[w] = xyzr[q];
+                            ++w;
+                        }
+                    }
+                if ((rc = upload(c, nb_start.data(), nb_start.size(), &G.nb_start)) ||
+                    (rc = upload(c, nb_bnd.data(), nb_bnd.size(), &G.nb_bnd)) ||
+                    (total && ((rc = upload(c, nb_atom.data(), total, &G.nb_atom)) ||
+                               (rc = upload(c, nb_xyzr.data(), total, &G.nb_xyzr)))))
+                    return rc;
+            }
             gid = (int)grids.size();
             grids.push_back(G);
             grid_ids[k] = gid;

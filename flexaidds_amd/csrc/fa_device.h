@@ -40,6 +40,15 @@ struct DevGrid {
     const int* cell_start;   // [dim^3 + 1]
     const int* cell_atom;    // [n_receptor] calc index
     const float4* cell_xyzr; // [n_receptor] x, y, z, radius
+    // Neighbourhood lists (grids with dim >= 3, else null): for every cell c
+    // the receptor atoms of its 27 neighbourhood slots in scan order (slot,
+    // then cell-list order), entries [nb_start[c], nb_start[c + 1]);
+    // nb_atom = calc index | slot << 27; nb_bnd[28 c + s] = where slot s
+    // starts, relative to nb_start[c] (nb_bnd[28 c + 27] = the list length).
+    const int* nb_start;     // [dim^3 + 1]
+    const int* nb_atom;
+    const float4* nb_xyzr;
+    const int* nb_bnd;       // [28 dim^3]
 };
 
 // One ligand topology (pointers into the context's device pool).

@@ -31,6 +31,9 @@ KB = 0.001987206
 def bridge():
     if not os.path.exists(SO):
         pytest.skip("oracle/_ref/libfa_bridge.so is not built (needs the reference sources: make -C oracle/ref)")
+    # one HIP runtime per process: torch's, loaded first with the library (as
+    # flexaidds_amd.hip does), before this object pulls the library in
+    from flexaidds_amd import hip  # noqa: F401
     lib = C.CDLL(SO)
     lib.fa_bridge_compare.argtypes = [C.c_void_p] * 6 + [C.c_char_p, C.c_int]
     lib.fa_bridge_compare.restype = C.c_int

@@ -33,6 +33,9 @@ class DropinOutputs(C.Structure):
 def dropin():
     if not os.path.exists(SO):
         pytest.fail(f"{SO} is not built (python tests/dropin/build.py)")
+    # one HIP runtime per process: torch's, loaded first with the library (as
+    # flexaidds_amd.hip does), before this object pulls the library in
+    from flexaidds_amd import hip  # noqa: F401
     lib = C.CDLL(SO)
     lib.dropin_session.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                    C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
