@@ -34,6 +34,7 @@
 #include <climits>
 #include <stdint.h>
 
+#include "cell_slots.h"
 #include "fa_device.h"
 
 namespace fa {
@@ -85,6 +86,11 @@ struct Smem {
             float dis[MAXL], ang[MAXL], dih[MAXL];
             int rot[FA_MAX_FLEX_RESIDUES];  // conformation of each flexible residue
         } b;                                // internal coordinates (pose build)
+        struct {
+            int span[64];                   // per-slot scratch of one atom's scan
+            int rcell[MAXL];                // cell of each rank, x | y << 10 | z << 20
+            int calc[MAXL];                 // calc index of each scorable atom
+        } p;                                // neighbour pre-pass
     } u;
     unsigned long long stub[2 * MAXL];      // stub[2m..2m+1]: atoms with a record of m
     unsigned long long ops[FA_OPCOUNT_N];   // operation counters of the pose
@@ -1034,42 +1040,23 @@ __device__ void pose_grid(const Smem& s, int M, const DevGrid& G, PoseGrid& pg)
     pg.slow = alter != 0;
 }
 
+// packed cell coordinates of a point (fa_cell_pack; dim < 1024 where used)
+__device__ __forceinline__ int cell_packed(const PoseGrid& pg, float x, float y, float z)
+{
+    return fa_cell_pack((int)((x - pg.gmin[0]) / CELLSIZE), (int)((y - pg.gmin[1]) / CELLSIZE),
+                        (int)((z - pg.gmin[2]) / CELLSIZE));
+}
+
 __device__ __forceinline__ int box_of(const PoseGrid& pg, float x, float y, float z)
 {
     return (int)((x - pg.gmin[0]) / CELLSIZE) * pg.dim2 + (int)((y - pg.gmin[1]) / CELLSIZE) * pg.dim +
            (int)((z - pg.gmin[2]) / CELLSIZE);
 }
 
-// cell of neighbourhood slot 0..26 around cell bz (LIB/Vcontacts.cpp:1738-1741)
-__device__ __forceinline__ int slot_box(const PoseGrid& pg, int bz, int slot)
-{
-    return bz + pg.dim2 * ((slot / 9) - 1) + pg.dim * (((slot / 3) % 3) - 1) + (slot % 3) - 1;
-}
-
-// neighbourhood slot (0..26) naming cell bz + d, -1 if none; with dim >= 3
-// at most one (dx, dy, dz) gives the offset
-__device__ __forceinline__ int slot_of(const PoseGrid& pg, int d)
-{
-    int slot = -1;
-    for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int r = d - pg.dim2 * dx - pg.dim * dy;
-            if (r >= -1 && r <= 1) slot = (dx + 1) * 9 + (dy + 1) * 3 + (r + 1);
-        }
-    return slot;
-}
-
-// how many of the 27 slots around bz name cell bz + d
-__device__ __forceinline__ int slot_multiplicity(const PoseGrid& pg, int d)
-{
-    int mult = 0;
-    for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int r = d - pg.dim2 * dx - pg.dim * dy;
-            mult += (r >= -1 && r <= 1);
-        }
-    return mult;
-}
+// the 27-slot neighbourhood (cell_slots.h, LIB/Vcontacts.cpp:1738-1741)
+__device__ __forceinline__ int slot_box(const PoseGrid& pg, int bz, int slot) { return fa_slot_box(pg.dim, bz, slot); }
+__device__ __forceinline__ int slot_of(const PoseGrid& pg, int d) { return fa_slot_of(pg.dim, d); }
+__device__ __forceinline__ int slot_multiplicity(const PoseGrid& pg, int d) { return fa_slot_multiplicity(pg.dim, d); }
 
 __device__ __forceinline__ bool bit_of(unsigned long long b0, unsigned long long b1, int m)
 {
@@ -1343,21 +1330,37 @@ __device__ __forceinline__ int wave_incl_min(int v)
     return v;
 }
 
+// Residue of scorable atom a for the pre-pass, by vector loads (lane 0 and
+// 1: the bonded row's two words over calc index - lo, lane 2: lo | hi << 32)
+// so that it can be fetched an atom ahead without holding up LDS waits.
+__device__ __forceinline__ uint64_t residue_fetch(const DevReceptor& R, const DevLigand& lig, int a)
+{
+    const int l = lane_id();
+    const int nsc = R.n_sc;
+    if (a >= nsc && l == 2) return (uint64_t)(uint32_t)R.n | ((uint64_t)(uint32_t)(R.n + lig.n_atoms) << 32);
+    // a lane-dependent pointer: the load is a vector load
+    const unsigned long long* p = a < nsc ? (l < 2 ? R.sc_bonded + 2 * a + l
+                                                    : reinterpret_cast<const unsigned long long*>(R.sc_range + a))
+                                          : lig.bonded + 2 * (a - nsc) + (l & 1);
+    return l < 3 ? (uint64_t)*p : 0ull;  // int2 {lo, hi} reads as lo | hi << 32
+}
+
 // The neighbourhood list of one atom's cell (grids with neighbourhood
 // lists): wave-uniform start and length, and on lane s < 27 where slot s's
 // receptor atoms end in it.
 struct NbTab {
-    int start, len;
+    int se;    // lane 0: the list's start, lane 1: its end
     int bend;
 };
 
+// Vector loads only (lane-varying addresses): a scalar load would share the
+// LDS wait counter and hold the atom's first LDS access until it returns.
 __device__ __forceinline__ NbTab nb_table(const Smem& s, const DevGrid& G, int k)
 {
     const int b = uni(s.sbox[k]);
-    NbTab t;
-    t.start = G.nb_start[b];
-    t.len = G.nb_start[b + 1] - t.start;
     const int l = lane_id();
+    NbTab t;
+    t.se = G.nb_start[b + (l & 1)];
     t.bend = l < 27 ? G.nb_bnd[28 * b + l + 1] : INT_MAX;
     return t;
 }
@@ -1372,11 +1375,12 @@ __device__ __forceinline__ NbTab nb_table(const Smem& s, const DevGrid& G, int k
 // (lin, built from the first in-range candidate of each slot and a suffix
 // minimum) and in-range receptor atoms of the slots up to each slot (rin,
 // read off the receptor chunks at the slot ends).
-__device__ void cull_nb(const Smem& s, const DevReceptor& R, const DevGrid& G, const PoseGrid& pg, int L, int bz,
+__device__ void cull_nb(const Smem& s, const DevReceptor& R, const DevGrid& G, const PoseGrid& pg, int L, int k,
                         const NbTab& tab, int* tbl, const CullCtx& cc, CullAcc& acc)
 {
     const int l = lane_id();
     const int base0 = acc.n_near;
+    const int kc = uni(s.u.p.rcell[k]);
     // -- scorable candidates, in rank order
     int sig[LSL];
     float ssq[LSL];
@@ -1387,7 +1391,7 @@ __device__ void cull_nb(const Smem& s, const DevReceptor& R, const DevGrid& G, c
         Cand c{0.f, 0.f, 0.f, 0.f, -1, 0, false};
         sig[q] = -1;
         if (q * WAVE < L && r < L) {
-            const int sl = slot_of(pg, s.sbox[r] - bz);
+            const int sl = fa_slot_of_cells(pg.dim, kc, s.u.p.rcell[r]);
             if (sl >= 0) {
                 const int m = s.order[r];
                 c.m = m;
@@ -1395,7 +1399,7 @@ __device__ void cull_nb(const Smem& s, const DevReceptor& R, const DevGrid& G, c
                 c.y = s.ly[m];
                 c.z = s.lz[m];
                 c.r = s.lr[m];
-                c.j = calc_of(R, m);
+                c.j = s.u.p.calc[m];
                 c.lj = true;
                 sig[q] = sl;
             }
@@ -1435,7 +1439,8 @@ __device__ void cull_nb(const Smem& s, const DevReceptor& R, const DevGrid& G, c
     wave_sync();  // the table is read before the next atom writes it
 
     // -- receptor atoms of the 27 slots
-    const int T = tab.len;
+    const int nb0 = readlane_i(tab.se, 0);
+    const int T = readlane_i(tab.se, 1) - nb0;
     int nR = 0;
     int rin = -1;  // lane s < 27: in-range receptor atoms of slots <= s
     auto fetch = [&](int b, float4& p, int& a) {
@@ -1443,8 +1448,8 @@ __device__ void cull_nb(const Smem& s, const DevReceptor& R, const DevGrid& G, c
         a = 0;
         p = make_float4(0.f, 0.f, 0.f, 0.f);
         if (t < T) {
-            p = G.nb_xyzr[tab.start + t];
-            a = G.nb_atom[tab.start + t];
+            p = G.nb_xyzr[nb0 + t];
+            a = G.nb_atom[nb0 + t];
         }
     };
     auto visit_r = [&](int b, const float4& p, int a) {
@@ -1768,6 +1773,8 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     s.order[rk[q]] = (unsigned char)i;
                     s.rank[i] = (unsigned char)rk[q];
                     s.sbox[rk[q]] = lb[q];
+                    s.u.p.rcell[rk[q]] = cell_packed(pg, s.lx[i], s.ly[i], s.lz[i]);
+                    s.u.p.calc[i] = calc_of(R, i);
                 }
             }
         }
@@ -1796,30 +1803,39 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
             // neighbourhood lists when the pose keeps the receptor's grid
             const bool use_nb = !pg.slow && G.nb_start != nullptr;
             SlotTab tab{-1, 0, 0};
-            NbTab ntab{0, 0, INT_MAX};
+            NbTab ntab{0, INT_MAX};
+            uint64_t rres = residue_fetch(R, lig, uni(s.order[0]));
             if (use_nb)
                 ntab = nb_table(s, G, 0);
             else
                 tab = slot_table(s, G, pg, 0);
             for (int k = 0; k < L; ++k) {
                 SlotTab next{-1, 0, 0};
-                NbTab nnext{0, 0, INT_MAX};
+                NbTab nnext{0, INT_MAX};
+                uint64_t rnext = 0ull;
                 if (k + 1 < L) {  // in flight during this atom
                     if (use_nb)
                         nnext = nb_table(s, G, k + 1);
                     else
                         next = slot_table(s, G, pg, k + 1);
+                    rnext = residue_fetch(R, lig, uni(s.order[k + 1]));
                 }
-                const int a = s.order[k];
+                const int a = uni(s.order[k]);
                 CullCtx cc;
                 cc.a = a;
-                cc.acalc = calc_of(R, a);
+                cc.acalc = uni(s.u.p.calc[a]);
                 cc.ax = s.lx[a];
                 cc.ay = s.ly[a];
                 cc.az = s.lz[a];
                 cc.ra = s.lr[a];
                 cc.rado = cc.ra + RW;
-                residue_of(a, cc.rlo, cc.rhi, cc.brow0, cc.brow1);
+                cc.brow0 = readlane_u64(rres, 0);
+                cc.brow1 = readlane_u64(rres, 1);
+                {
+                    const uint64_t rr = readlane_u64(rres, 2);
+                    cc.rlo = (int)(uint32_t)rr;
+                    cc.rhi = (int)(uint32_t)(rr >> 32);
+                }
                 cc.dpre0 = dpre0;
                 cc.dpre1 = dpre1;
                 cc.permea = (double)prm.permeability;
@@ -1829,9 +1845,9 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 if (l == 0) s.nstart[k] = acc.n_near;
                 acc.total = acc.tested = acc.found = 0;
                 if (use_nb)
-                    cull_nb(s, R, G, pg, L, uni(s.sbox[k]), ntab, s.u.c.idx, cc, acc);
+                    cull_nb(s, R, G, pg, L, k, ntab, s.u.p.span, cc, acc);
                 else if (!pg.slow)
-                    cull_fast(s, R, G, lig, pg, Rn, L, tab, s.sbox[k], s.u.c.idx, cc, acc);
+                    cull_fast(s, R, G, lig, pg, Rn, L, tab, s.sbox[k], s.u.p.span, cc, acc);
                 else if (pg.dim >= 3)
                     cull_slow(s, R, lig, pg, L, tab, s.sbox[k], cc, acc, wb, rcand);
                 else
@@ -1854,6 +1870,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 }
                 tab = next;
                 ntab = nnext;
+                rres = rnext;
             }
             if (l == 0) s.nstart[L] = acc.n_near;
             count_op(s.ops, OP_CULL, pre_tested);
