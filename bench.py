@@ -250,6 +250,29 @@ def ga_dock_leg(ctx, system, args):
             "unscored": _unscored(ctx)}
 
 
+def _tune_generation(ctx, system, pop, seed=5):
+    """Autotune the launch shape on one generation-sized launch of every
+    ligand of `system` (random genes, one launch for all of them), as the
+    batched GA scores its children."""
+    import torch
+    from flexaidds_amd import abi, autotune, synth
+    rng = np.random.default_rng(seed)
+    gmax = max(l.n_genes for l in system.ligands)
+    rows, lig = [], []
+    for li, l in enumerate(system.ligands):
+        g = np.zeros((pop, gmax))
+        g[:, :l.n_genes] = synth.random_genes(rng, l, pop)
+        rows.append(g)
+        lig.append(np.full(pop, li, np.int32))
+    d_g = torch.from_numpy(np.concatenate(rows)).cuda()
+    d_l = torch.from_numpy(np.concatenate(lig)).cuda()
+    n = d_l.numel()
+    d_o = torch.zeros(n * abi.CF_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    sh = torch.cuda.current_stream().cuda_stream
+    return autotune.tune(ctx, lambda: ctx.eval_device(n, d_g.data_ptr(), gmax, d_l.data_ptr(), d_o.data_ptr(), sh),
+                         reps=2)
+
+
 def batched_leg(args):
     """The batched screen: every ligand a job of one batched GA."""
     import torch  # noqa: F401  (same HIP runtime as the library)
@@ -258,6 +281,8 @@ def batched_leg(args):
     bs = synth.make_config2(n_ligands=args.batched_ligands, pop=8)
     bctx = hip.Context()
     bctx.upload_system(bs)
+    shape = _tune_generation(bctx, bs, args.batched_pop)   # kept on the context: the GA's graph uses it
+    log(f"batched launch shape: {shape['waves']} waves, flags {shape['flags']}")
     bctx.reset_totals()
     J = len(bs.ligands)
     ga = hip.GA(bctx, [synth.ga_limits(l) for l in bs.ligands], ligands=list(range(J)), population=args.batched_pop,
@@ -274,7 +299,7 @@ def batched_leg(args):
     return {"ligands": J, "population": args.batched_pop, "generations": args.batched_generations,
             "poses_per_generation": J * args.batched_pop, "ms_per_generation": round(ms / args.batched_generations, 3),
             "poses_per_s": round(poses / (ms / 1e3), 1), "ln_xi_300K": round(gpf["log_xi"], 6),
-            "p_empty": gpf["p_empty"], "unscored": uns}
+            "p_empty": gpf["p_empty"], "unscored": uns, "launch_shape": shape}
 
 
 def grand_leg(args):
@@ -509,6 +534,10 @@ def main():
     for k in range(args.warmup):
         launch(k)
     torch.cuda.synchronize()
+    from flexaidds_amd import autotune
+    plain_shape = autotune.tune(ctx, lambda: launch(0), reps=3)   # left set for the timed launches
+    log(f"launch shape: {plain_shape['waves']} waves, flags {plain_shape['flags']} ({plain_shape['ms']} ms)")
+    ctx.reset_totals()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     t_begin = torch.cuda.Event(enable_timing=True)
@@ -593,6 +622,7 @@ def main():
         "status_counts": {"ok": int(status[0]), "hull_fail": int(status[1]), "clash": int(status[2]),
                           "capacity": int(status[3]), "grid_range": int(status[4])},
         "unscored": plain_unscored,
+        "launch_shape": plain_shape,
         "free_energy_300K_last_population": round(th["free_energy"], 6),
     }
     print(json.dumps(line), flush=True)

@@ -1801,7 +1801,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 wave_sync();
             }
             // neighbourhood lists when the pose keeps the receptor's grid
-            const bool use_nb = !pg.slow && G.nb_start != nullptr;
+            const bool use_nb = !pg.slow && G.nb_start != nullptr && !B.cell_lists;
             SlotTab tab{-1, 0, 0};
             NbTab ntab{0, INT_MAX};
             uint64_t rres = residue_fetch(R, lig, uni(s.order[0]));

@@ -189,8 +189,11 @@ int fa_launch_poses(fa_ctx* c, int n_poses, const double* d_genes, int stride, c
     B.near_xyz = static_cast<float4*>(c->near_xyz.p);
     B.rbox = static_cast<int*>(c->rbox.p);
     B.near_cap = c->near_cap;
-    // one wave per resident slot; each wave loops over poses (work queue)
-    FA_HIP(fa::launch_cf(c->rec, c->d_grids, c->d_ligs, c->prm, B, c->n_waves, stream));
+    B.cell_lists = (c->shape_flags & FA_SHAPE_CELL_LISTS) ? 1 : 0;
+    // one wave per resident slot (or the shape's cap); each wave loops over
+    // poses (work queue)
+    const int waves = c->shape_waves > 0 ? std::min(c->shape_waves, c->n_waves) : c->n_waves;
+    FA_HIP(fa::launch_cf(c->rec, c->d_grids, c->d_ligs, c->prm, B, waves, stream));
     return FA_SUCCESS;
 }
 
@@ -201,6 +204,23 @@ const char* fa_last_error(void) { return g_err.c_str(); }
 int fa_kernel_lds_bytes(void) { return fa::smem_bytes(); }
 
 int fa_ctx_launch_waves(fa_ctx* c) { return c ? c->n_waves : 0; }
+
+int fa_ctx_set_launch_shape(fa_ctx* c, int waves, int flags)
+{
+    if (!c) return fa_fail(FA_ERR_INVALID, "null context");
+    if (waves < 0 || (flags & ~FA_SHAPE_CELL_LISTS)) return fa_fail(FA_ERR_INVALID, "bad launch shape");
+    c->shape_waves = waves;
+    c->shape_flags = flags;
+    return FA_SUCCESS;
+}
+
+int fa_ctx_get_launch_shape(fa_ctx* c, int* waves, int* flags)
+{
+    if (!c || !waves || !flags) return fa_fail(FA_ERR_INVALID, "null argument");
+    *waves = c->shape_waves > 0 ? std::min(c->shape_waves, c->n_waves) : c->n_waves;
+    *flags = c->shape_flags;
+    return FA_SUCCESS;
+}
 
 int fa_get_device_info(int device_id, fa_device_info* out)
 {

@@ -69,6 +69,8 @@ struct fa_ctx {
     // pose-kernel grid: one resident wave per slot, each with its own
     // neighbour-list scratch of near_cap entries
     int n_waves = 0;
+    int shape_waves = 0;   // launch shape (fa_ctx_set_launch_shape): wave cap, 0 = every resident slot
+    int shape_flags = 0;   //   FA_SHAPE_* flags
     int near_cap = 0;
     // pose buffers
     DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops, prof, near_idx, near_xyz, rbox;

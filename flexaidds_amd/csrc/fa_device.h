@@ -125,6 +125,7 @@ struct DevBatch {
     // every receptor atom in a pose's widened grid, then the atoms gathered
     // for one ligand atom
     int* rbox;
+    int cell_lists;                 // 1: scan the cell lists even where neighbourhood lists exist
 };
 
 }  // namespace fa

@@ -68,7 +68,8 @@ int fa_eval_batch_device_noalloc(fa_ctx* c, int n_poses, const double* genes, in
     if (fit < 1)
         return fa_fail(FA_ERR_LIMIT, "one wave's scratch (" + std::to_string(per_wave) + " B for this system) exceeds the " +
                                           std::to_string(SCRATCH_BYTES >> 20) + " MiB static scratch");
-    const int waves = (int)std::min<long long>(c->n_waves, fit);
+    const int cap = c->shape_waves > 0 ? std::min(c->shape_waves, c->n_waves) : c->n_waves;  // launch shape
+    const int waves = (int)std::min<long long>(cap, fit);
     fa::DevBatch B{};
     B.n_poses = n_poses;
     B.gene_stride = gene_stride;
@@ -87,6 +88,7 @@ int fa_eval_batch_device_noalloc(fa_ctx* c, int n_poses, const double* genes, in
     off = round256(off + sizeof(float4) * (size_t)waves * c->near_cap);
     B.rbox = reinterpret_cast<int*>(base + off);
     B.near_cap = c->near_cap;
+    B.cell_lists = (c->shape_flags & FA_SHAPE_CELL_LISTS) ? 1 : 0;
     FA_HIP(hipSetDevice(c->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     // the work-queue and overflow counters start at zero (a fill kernel, no memset node)

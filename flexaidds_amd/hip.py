@@ -74,6 +74,8 @@ _SIGS = {
     "fa_log_sum_exp_device": ([_P, _P, C.c_int, C.c_int64, _P, _P], C.c_int),
     "fa_kernel_lds_bytes": ([], C.c_int),
     "fa_ctx_launch_waves": ([_P], C.c_int),
+    "fa_ctx_set_launch_shape": ([_P, C.c_int, C.c_int], C.c_int),
+    "fa_ctx_get_launch_shape": ([_P, _P, _P], C.c_int),
     "fa_ga_create": ([_P, _P, _P, _P], C.c_int),
     "fa_ga_create_batch": ([_P, _P, C.c_int, _P, _P, _P], C.c_int),
     "fa_ga_shape": ([_P, _P, _P], C.c_int),
@@ -269,6 +271,16 @@ class Context:
 
     def reset_totals(self):
         _check(_lib.fa_ctx_reset_totals(self._ctx))
+
+    def set_launch_shape(self, waves: int = 0, flags: int = 0):
+        """Cap on resident waves (0: every slot) and scan variant (abi.FA_SHAPE_*)."""
+        _check(_lib.fa_ctx_set_launch_shape(self._ctx, int(waves), int(flags)))
+
+    def launch_shape(self) -> tuple:
+        """(waves a launch uses, flags)."""
+        w, f = C.c_int(), C.c_int()
+        _check(_lib.fa_ctx_get_launch_shape(self._ctx, C.byref(w), C.byref(f)))
+        return w.value, f.value
 
     def launch_waves(self) -> int:
         """Waves one pose launch runs (resident slots on the device)."""

@@ -348,6 +348,21 @@ int fa_kernel_lds_bytes(void);
  * of many times this many poses keep every slot busy to the end. */
 int fa_ctx_launch_waves(fa_ctx* ctx);
 
+/* Launch shape of the context's pose launches (every entry point that scores
+ * poses, the GA and the pool included): `waves` caps the resident waves
+ * (0 = one per resident slot), flags select variants of the kernel's
+ * neighbour scan.  A wave cap leaves every result unchanged; the scan
+ * variants find the same neighbours in the same order, so they differ only in
+ * the summation order of the clash pre-pass (the wall of an FA_CLASH pose, in
+ * its last bits).
+ * flexaidds_amd.autotune picks a shape per workload by timing the
+ * candidates.  No counterpart in the reference (its CPU loop and its GPU
+ * placeholder have compile-time launch constants, LIB/hip_eval.hip:57-71).
+ * fa_ctx_get_launch_shape reports the wave count a launch uses. */
+#define FA_SHAPE_CELL_LISTS 1  /* scan the receptor cell lists instead of the per-cell neighbourhood lists */
+int fa_ctx_set_launch_shape(fa_ctx* ctx, int waves, int flags);
+int fa_ctx_get_launch_shape(fa_ctx* ctx, int* waves, int* flags);
+
 #ifdef __cplusplus
 }
 #endif
