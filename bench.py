@@ -484,6 +484,8 @@ def main():
     ap.add_argument("--real-steps", type=int, default=20, help="launches on the RNA target's cleft (0: skip)")
     ap.add_argument("--real-generations", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shape (no autotuning launches; for profiler runs)")
     ap.add_argument("--sweep", action="store_true",
                     help="only time one launch per population size (1 .. 16,384 poses of the plain workload) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -535,7 +537,11 @@ def main():
         launch(k)
     torch.cuda.synchronize()
     from flexaidds_amd import autotune
-    plain_shape = autotune.tune(ctx, lambda: launch(0), reps=3)   # left set for the timed launches
+    if args.no_tune:
+        w0, f0 = ctx.launch_shape()
+        plain_shape = {"waves": w0, "flags": f0, "ms": None, "table": []}
+    else:
+        plain_shape = autotune.tune(ctx, lambda: launch(0), reps=3)   # left set for the timed launches
     log(f"launch shape: {plain_shape['waves']} waves, flags {plain_shape['flags']} ({plain_shape['ms']} ms)")
     ctx.reset_totals()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
