@@ -28,14 +28,19 @@
  * intramolecular tests follow the residue (LIB/vcfunction.cpp:189-240,
  * LIB/Vcontacts.cpp:1806-1821).
  *
- * Two evaluation modes:
- *   0 (reference, the default) — the reference's serial semantics above;
- *   1 (independent atoms) — the form the GPU computes: every scorable atom's
- *     polyhedron from all of its in-range neighbours, with no done/stub
- *     exclusion and no hull seeds, and each cell's candidates taken rigid
- *     atoms first, then scorable atoms (both in calc order).  The polyhedra
- *     are the same up to rounding and degenerate ties; the clash pre-pass
- *     counts the same pairs.
+ * Evaluation modes (bits of fa_oracle_set_mode_bits):
+ *   0 (the default) — the reference's serial semantics above;
+ *   4 — each cell's candidates taken rigid atoms first, then scorable atoms
+ *     (both in calc order): the order the GPU scans a cell in; with side
+ *     chains among the receptor atoms it changes only summation order;
+ *   1, 2 — no done/stub exclusion, no hull seeds: every scorable atom's
+ *     polyhedron on its own (independent atoms);
+ *   8 — the speculative form: independent atoms from starts found by ray
+ *     casting, and every pose whose reference start the checks of
+ *     spec_check cannot vouch for evaluated in the serial semantics.
+ * Bits 1, 2 and 8 exist to measure why the device keeps the serial order:
+ * the reference's areas depend on the hull walk's order, so neither form
+ * stays within the CF tolerance (tests/test_oracle_atom_order.py).
  *
  * The geometry type `R` is double for the oracle proper (the reference does
  * all Vcontacts geometry in double); the float instantiation exists only to
@@ -80,9 +85,27 @@ int g_trig_mode = 0;
 // a cell) to measure what each changes
 int g_eval_mode = 0;
 int g_mode_bits = 0;
-inline bool m_nodone() { return g_mode_bits & 1; }
-inline bool m_noseed() { return g_mode_bits & 2; }
-inline bool m_rigidfirst() { return g_mode_bits & 4; }
+// a pose re-evaluated in the serial form after the speculative form flagged
+// it runs with the speculative bit cleared (this thread only)
+thread_local int t_bits_override = -1;
+inline int mode_bits() { return t_bits_override >= 0 ? t_bits_override : g_mode_bits; }
+// bit 8: the speculative form, every atom's polyhedron from its full
+// neighbour list and a start found by ray casting, poses that the checks of
+// spec_check flag re-evaluated in the serial form (implies bits 1 and 2)
+inline bool m_spec() { return mode_bits() & 8; }
+inline bool m_nodone() { return (mode_bits() & 1) || m_spec(); }
+inline bool m_noseed() { return (mode_bits() & 2) || m_spec(); }
+inline bool m_rigidfirst() { return mode_bits() & 4; }
+
+// speculative form: an atom whose hull has this many edges sends its pose to
+// the serial form (the reference's reduced contact list can change the count
+// near its 200-edge failure limit)
+constexpr int SPEC_EDGE_MAX = 120;
+// a start vertex more than this outside a plane is not a polyhedron vertex
+constexpr double SPEC_VALID_EPS = 1e-10;
+// per pose: set when the speculative form flagged the pose (poses evaluated
+// in the serial form); per batch: the flags of the last eval_batch call
+thread_local int t_fallback = 0;
 
 inline float trig_cos(float x) { return g_trig_mode ? (float)std::cos((double)x) : std::cos(x); }
 inline float trig_sin(float x) { return g_trig_mode ? (float)std::sin((double)x) : std::sin(x); }
@@ -301,6 +324,7 @@ template <class R> struct Work {
     std::vector<Edge<R>> vedge;
     std::vector<std::vector<int>> ptorder;
     int dim = 0;
+    int last_edges = 0;   // hull edges of the last atom
 };
 
 struct System {
@@ -540,7 +564,9 @@ int get_contacts(const System& S, Work<R>& W, int a, float rado, double* clash)
 {
     const int dim = W.dim, dim2 = dim * dim, dim3 = dim2 * dim;
     const double permea = (double)S.prm->permeability;
-    const bool indep = m_nodone();
+    // the speculative form runs the serial pre-pass (done marks), then full
+    // neighbour lists for the polyhedra
+    const bool indep = m_nodone() && !(m_spec() && clash);
     // earlier atoms that recorded a contact with `a` stay eligible ('C')
     if (!indep)
         for (int j : W.stubs[S.slot[a]]) W.done[j] = 'C';
@@ -590,6 +616,8 @@ int get_contacts(const System& S, Work<R>& W, int a, float rado, double* clash)
 
 // ----------------------------------------------------------- first vertex
 // LIB/Vcontacts.cpp:499-590
+template <class R> void heuristic_firstvert(const Plane<R>* cont, int* pA, int* pB, int* pC, int NC);
+
 template <class R> void get_firstvert(const Work<R>& W, const Plane<R>* cont, int* pA, int* pB, int* pC, int NC, int a)
 {
     *pA = *pB = *pC = -1;
@@ -605,6 +633,16 @@ template <class R> void get_firstvert(const Work<R>& W, const Plane<R>* cont, in
         }
     }
     if (*pA != -1 && *pB != -1 && *pC != -1) return;
+    heuristic_firstvert(cont, pA, pB, pC, NC);
+}
+
+// The reference's start without a seed (LIB/Vcontacts.cpp:516-590): A the
+// closest plane, B the plane whose line with A passes closest to the atom, C
+// the plane cutting that line nearest to that point.  Nothing guarantees the
+// result is a vertex of the polyhedron.
+template <class R> void heuristic_firstvert(const Plane<R>* cont, int* pA, int* pB, int* pC, int NC)
+{
+    *pA = *pB = *pC = -1;
     R mindist = (R)9.9e+9;
     for (int c = 0; c < NC; ++c) {
         if (cont[c].dist < mindist) {
@@ -648,6 +686,72 @@ template <class R> void get_firstvert(const Work<R>& W, const Plane<R>* cont, in
     }
 }
 
+// -------------------------------------------------- speculative start
+// First plane of [0, n), other than x0 and x1, that the ray p + t d (t >= 0)
+// leaves through: the smallest parameter among planes the ray moves towards
+// (n.d > 0), the lowest index among equals.  -1 when none.
+template <class R> int first_hit(const Plane<R>* cont, int n, const R* p, const R* d, int x0, int x1, R* tmin)
+{
+    int best = -1;
+    R tb = (R)9.9e+9;
+    for (int c = 0; c < n; ++c) {
+        if (c == x0 || c == x1) continue;
+        const R den = cont[c].Ai[0] * d[0] + cont[c].Ai[1] * d[1] + cont[c].Ai[2] * d[2];
+        if (!(den > R(0))) continue;
+        const R t = -(cont[c].Ai[0] * p[0] + cont[c].Ai[1] * p[1] + cont[c].Ai[2] * p[2] + cont[c].Ai[3]) / den;
+        if (t < tb) {
+            tb = t;
+            best = c;
+        }
+    }
+    *tmin = tb;
+    return best;
+}
+
+// A vertex of the polyhedron, found by ray casting from the atom's centre
+// (inside the polyhedron when no plane engulfs it): out along a fixed
+// direction to the first plane, within that plane to the second, along their
+// line to the third.  false when a plane engulfs the atom or a ray escapes.
+template <class R> bool ray_firstvert(const Plane<R>* cont, int NC, int* pA, int* pB, int* pC)
+{
+    for (int c = 0; c < NC; ++c)
+        if (cont[c].Ai[3] > R(0)) return false;
+    const R o[3] = {0, 0, 0};
+    const R d1[3] = {(R)0.48, (R)0.6, (R)0.64};
+    R t1, t2, t3;
+    const int P1 = first_hit(cont, NC + 4, o, d1, -1, -1, &t1);
+    if (P1 < 0) return false;
+    const R* n1 = cont[P1].Ai;
+    const R p1[3] = {t1 * d1[0], t1 * d1[1], t1 * d1[2]};
+    const R nn = n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2];
+    const R dn = (d1[0] * n1[0] + d1[1] * n1[1] + d1[2] * n1[2]) / nn;
+    const R d2[3] = {d1[0] - dn * n1[0], d1[1] - dn * n1[1], d1[2] - dn * n1[2]};
+    const int P2 = first_hit(cont, NC + 4, p1, d2, P1, -1, &t2);
+    if (P2 < 0) return false;
+    const R* n2 = cont[P2].Ai;
+    const R p2[3] = {p1[0] + t2 * d2[0], p1[1] + t2 * d2[1], p1[2] + t2 * d2[2]};
+    const R d3[3] = {n1[1] * n2[2] - n1[2] * n2[1], n1[2] * n2[0] - n1[0] * n2[2], n1[0] * n2[1] - n1[1] * n2[0]};
+    const int P3 = first_hit(cont, NC + 4, p2, d3, P1, P2, &t3);
+    if (P3 < 0) return false;
+    *pA = P1;
+    *pB = P2;
+    *pC = P3;
+    return true;
+}
+
+// Is the point of planes (pA, pB, pC) inside every other plane of [0, n)?
+template <class R> bool vertex_valid(const Plane<R>* cont, int n, int pA, int pB, int pC)
+{
+    R tp[3];
+    if (solve_3x3(cont[pA].Ai, cont[pB].Ai, cont[pC].Ai, tp) == -1) return false;
+    for (int c = 0; c < n; ++c) {
+        if (c == pA || c == pB || c == pC) continue;
+        if (cont[c].Ai[0] * tp[0] + cont[c].Ai[1] * tp[1] + cont[c].Ai[2] * tp[2] + cont[c].Ai[3] > (R)SPEC_VALID_EPS)
+            return false;
+    }
+    return true;
+}
+
 // ----------------------------------------------------------------- hull
 // LIB/Vcontacts.cpp:147-490.  Returns the number of vertices kept inside the
 // sphere or -1 when the edge walk does not converge (VC->recalc == 0).
@@ -682,7 +786,16 @@ template <class R> int voronoi_poly(const System& S, Work<R>& W, int a, float ra
         cont[NC + k].area = 0;
     }
     int pA, pB, pC;
-    get_firstvert(W, cont, &pA, &pB, &pC, NC, a);
+    if (m_spec()) {
+        // speculative form: a start that is a vertex, by ray casting, else
+        // the heuristic start if it is one; otherwise the pose goes serial
+        if (!ray_firstvert(cont, NC, &pA, &pB, &pC)) {
+            heuristic_firstvert(cont, &pA, &pB, &pC, NC);
+            if (pA < 0 || pB < 0 || pC < 0 || !vertex_valid(cont, NC + 4, pA, pB, pC)) return -3;
+        }
+    } else {
+        get_firstvert(W, cont, &pA, &pB, &pC, NC, a);
+    }
     Vert<R>* poly = W.poly.data();
     Edge<R>* vedge = W.vedge.data();
     R tp[3];
@@ -772,12 +885,14 @@ template <class R> int voronoi_poly(const System& S, Work<R>& W, int a, float ra
             if (edgenum >= EDGE_LIMIT) {  // hull failure; VC->recalc == 0
                 CAPMAX(vn, vn + 1);
                 CAPMAX(edges, edgenum);
+                W.last_edges = edgenum;
                 return -1;
             }
         }
         ++vn;
     }
     CAPMAX(edges, edgenum);
+    W.last_edges = edgenum;
     // arc points on edges that cross the sphere (LIB/Vcontacts.cpp:419-457)
     for (int e = 0; e < edgenum; ++e)
         vedge[e].arc = ((R)rado < poly[vedge[e].startpt].dist || (R)rado < poly[vedge[e].endpt].dist) ? '?' : 'X';
@@ -1076,10 +1191,62 @@ double pair_term(const fa_params* prm, int k, double area, double surf)
     return prm->normalize_area ? y * area / surf : y * area;
 }
 
+// Speculative form: does the reference, atom `a`'s turn in the serial order
+// (LIB/Vcontacts.cpp:74-139), compute the polyhedron just computed from the
+// full neighbour list?  Within the sphere the polyhedron is the same
+// whenever the reference starts from a true vertex: the planes it leaves
+// out (earlier scorable atoms that recorded no contact with `a`) cut it only
+// outside the sphere, since radical planes are symmetric.  It starts from
+// a seed when an earlier scorable atom's polyhedron has a vertex inside its
+// sphere on `a`'s plane (LIB/Vcontacts.cpp:595-629) -- the same point is
+// then a vertex of `a`'s polyhedron inside `a`'s sphere, so the seed is
+// found here -- and from the heuristic vertex of its reduced list
+// otherwise, which has to be tested.  Checked: the edge count stays clear of
+// the 200-edge failure, every candidate seed names atoms of the reduced
+// list, and an unseeded atom's heuristic start is a vertex.
+template <class R> bool spec_check(const System& S, const Work<R>& W, int a, int NC, int NV)
+{
+    if (W.last_edges >= SPEC_EDGE_MAX) return false;
+    const Plane<R>* cont = W.cont.data();
+    const int ra = W.rank[a];
+    auto earlier = [&](int c) {
+        const int j = cont[c].index;
+        return S.slot[j] >= 0 && W.rank[j] < ra;
+    };
+    // left out of the reference's list: earlier, and no record of `a` (an
+    // area of the shared face, which both sides compute)
+    auto excluded = [&](int c) { return earlier(c) && !(cont[c].area > R(0) && cont[c].flag != 'X'); };
+    bool seeded = false;
+    for (int vi = 0; vi < NV; ++vi) {
+        const Vert<R>& v = W.poly[vi];
+        if (v.plane[2] == -1) continue;
+        for (int k = 0; k < 3; ++k) {
+            const int c = v.plane[k];
+            if (c >= NC || !earlier(c)) continue;
+            seeded = true;
+            for (int k2 = 0; k2 < 3; ++k2)
+                if (k2 != k && (v.plane[k2] >= NC || excluded(v.plane[k2]))) return false;
+        }
+    }
+    if (seeded) return true;
+    std::vector<Plane<R>> rl;
+    rl.reserve(NC + 4);
+    for (int c = 0; c < NC; ++c)
+        if (!excluded(c)) rl.push_back(cont[c]);
+    const int NR = (int)rl.size();
+    if (NR == 0) return true;  // no contact in the reference either: the whole sphere is exposed
+    for (int k = 0; k < 4; ++k) rl.push_back(cont[NC + k]);
+    int pA, pB, pC;
+    heuristic_firstvert(rl.data(), &pA, &pB, &pC, NR);
+    if (pA < 0 || pB < 0 || pC < 0) return false;
+    return vertex_valid(rl.data(), NR + 4, pA, pB, pC);
+}
+
 template <class R>
 fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xyz_out,
                        std::vector<fa_contact_rec>* recs_out)
 {
+    if (t_bits_override < 0) t_fallback = 0;
     const fa_ligand_desc* lig = S.lig;
     const fa_params* prm = S.prm;
     const int Rn = S.R(), L = S.L(), N = S.N();
@@ -1131,6 +1298,7 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
     std::fill(W.seed.begin(), W.seed.end(), -1);
     for (auto& r : W.records) r.clear();
     // calc_region (LIB/Vcontacts.cpp:74-139)
+    bool spec_flag = false;  // speculative form: the pose goes to the serial form
     for (int i = 0; i < N; ++i) {
         const int a = W.calclist[i];
         if (S.slot[a] < 0) continue;
@@ -1142,6 +1310,10 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
         if ((int)W.ptorder.size() < NC) W.ptorder.resize(NC);
         if ((int)W.centerpt.size() < NC) W.centerpt.resize(NC);
         int NV = voronoi_poly(S, W, a, rado, NC);
+        if (m_spec() && NV < 0) {  // hull failure, capacity or no start: every atom still runs
+            spec_flag = true;
+            continue;
+        }
         if (NV == -1) {
             if (t_branch) ++t_branch->hull_fail;
             res.wal = (double)POLY_PENALTY;
@@ -1156,12 +1328,17 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
             return res;
         }
         int NV2 = order_faces(W, rado, NC, NV);
+        if (m_spec() && NV2 < 0) {
+            spec_flag = true;
+            continue;
+        }
         if (NV2 < 0) {
             res.cf = res.app = HUGE_VAL;
             res.status = FA_CAPACITY;
             return res;
         }
         calc_areas(W, rado, NC, NV, a);
+        if (m_spec() && !spec_check(S, W, a, NC, NV)) spec_flag = true;
         // save_areas (LIB/Vcontacts.cpp:1149-1220): own list in reverse
         std::vector<fa_contact_rec>& rl = W.records[S.slot[a]];
         for (int c = NC - 1; c >= 0; --c) {
@@ -1176,6 +1353,15 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
                     W.stubs[S.slot[j]].push_back(a);
             }
         }
+    }
+    if (spec_flag) {
+        // the serial form decides this pose (build, pre-pass and polyhedra again)
+        t_fallback = 1;
+        const int saved = t_bits_override;
+        t_bits_override = mode_bits() & ~8;
+        const fa_cf_result r2 = eval_pose(S, W, icv, nullptr, recs_out);
+        t_bits_override = saved;
+        return r2;
     }
     // vcfunction (LIB/vcfunction.cpp:77-527), scorable atoms in calc order,
     // each residue's terms in its own cfstr (optres[].cf); ic2cf then adds the
@@ -1301,7 +1487,7 @@ template <class R> void init_work(const System& S, Work<R>& W)
 template <class R>
 int eval_batch(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm, int n_poses,
                const double* genes, fa_cf_result* out, int n_threads, fa_opcount* ops, CapStats* caps = nullptr,
-               BranchStats* branch = nullptr)
+               BranchStats* branch = nullptr, int* fallback = nullptr)
 {
     if (!rec || !lig || !prm || !genes || !out || n_poses < 0) return -1;
     System S{rec, lig, prm};
@@ -1322,8 +1508,10 @@ int eval_batch(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 4)
 #endif
-        for (int p = 0; p < n_poses; ++p)
+        for (int p = 0; p < n_poses; ++p) {
             out[p] = eval_pose(S, W, genes + (size_t)p * lig->n_genes, nullptr, nullptr);
+            if (fallback) fallback[p] = t_fallback;
+        }
         t_ops = nullptr;
         t_caps = nullptr;
         t_branch = nullptr;
@@ -1417,17 +1605,27 @@ int fa_oracle_branch_stats(const fa_receptor_desc* rec, const fa_ligand_desc* li
 }
 
 /* 0: reference semantics (done/stub rule, hull seeds, calc order in a cell);
- * 1: independent atoms, the form the GPU evaluates. */
+ * 1: independent atoms (no done/stub rule, no seeds); 2: the speculative
+ * form (independent atoms from ray-cast starts, poses its checks flag in the
+ * reference semantics). */
 void fa_oracle_set_mode(int mode)
 {
-    g_eval_mode = mode ? 1 : 0;
-    g_mode_bits = mode ? 7 : 0;
+    g_eval_mode = mode == 1 ? 1 : 0;
+    g_mode_bits = mode == 1 ? 7 : mode == 2 ? 8 : 0;
+}
+
+/* fa_oracle_eval in the current mode, with per pose 1 where the speculative
+ * form handed the pose to the serial form. */
+int fa_oracle_eval_fallback(const fa_receptor_desc* rec, const fa_ligand_desc* lig, const fa_params* prm, int n_poses,
+                            const double* genes, fa_cf_result* out, int n_threads, int* fallback)
+{
+    return eval_batch<double>(rec, lig, prm, n_poses, genes, out, n_threads, nullptr, nullptr, nullptr, fallback);
 }
 
 /* The parts of the independent form one by one (bits as g_mode_bits). */
 void fa_oracle_set_mode_bits(int bits)
 {
-    g_mode_bits = bits & 7;
+    g_mode_bits = bits & 15;
     g_eval_mode = g_mode_bits == 7 ? 1 : 0;
 }
 
