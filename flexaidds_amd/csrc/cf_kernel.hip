@@ -1537,47 +1537,59 @@ __device__ __forceinline__ void visit_ligand_cell(const Smem& s, const DevRecept
 }
 
 constexpr int SLOW_TMP = 3 * WAVE;  // in-range receptor atoms of one ligand atom (slow path)
+// receptor rows of the widened grid that no slot names (side-chain atoms)
+constexpr int FAR_CELL = -1;
+
+// slot of packed cell rc around packed cell kc, -1 for FAR_CELL
+__device__ __forceinline__ int slot_rc(int dim, int kc, int rc) { return rc == FAR_CELL ? -1 : fa_slot_of_cells(dim, kc, rc); }
 
 // Per-wave slow-path scratch: the cell of every receptor atom in the pose's
 // widened grid, then the receptor atoms of one ligand atom's 27 cells.
 __device__ __forceinline__ size_t slow_rows(int Rn) { return (size_t)((Rn + 3) & ~3); }
 
 // Grid widened by this pose: the receptor cells are not precomputed.  The
-// cells of the receptor atoms (computed once per pose) are streamed four per
-// lane and the atoms in the 27 cells gathered in index order with their slot;
-// those are tested (the in-range entries parked, with their slot, at the
+// packed cell of every receptor atom (computed once per pose) is streamed
+// four per lane and the atoms in the 27 cells around the ligand atom's cell
+// gathered in index order with their slot -- once per ligand cell, since
+// the atoms of a cell are consecutive in box order and share the gather
+// (`gcell` and `ncand` carry it from one atom to the next).  The gathered
+// atoms are tested (the in-range entries parked, with their slot, at the
 // tail of the wave's neighbour buffer) and a last pass emits them slot by
 // slot, each slot's ligand atoms after its receptor atoms, as the reference
 // scans.
 __device__ void cull_slow(const Smem& s, const DevReceptor& R, const DevLigand& lig, const PoseGrid& pg, int L,
-                          const SlotTab& tab, int bz, const CullCtx& cc, CullAcc& acc, const int* rbox, int* rcand)
+                          const SlotTab& tab, int kc, const CullCtx& cc, CullAcc& acc, const int* rbox, int* rcand,
+                          int& gcell, int& ncand)
 {
     const int l = lane_id();
     const int Rn = R.n;
-    const int Rq = (int)(slow_rows(Rn) >> 2);
-    const int4* const rbox4 = reinterpret_cast<const int4*>(rbox);
-    int ncand = 0;
-    int4 nxt = make_int4(-1, -1, -1, -1);
-    if (l < Rq) nxt = rbox4[l];
-    for (int base = 0; base < Rq; base += WAVE) {
-        const int4 bx = nxt;
-        if (base + WAVE < Rq) nxt = (base + WAVE + l < Rq) ? rbox4[base + WAVE + l] : make_int4(-1, -1, -1, -1);
-        const int t0 = 4 * (base + l);
-        // slots of the lane's four atoms (cells of rows past Rn are never read)
-        const int s0 = (t0 < Rn) ? slot_of(pg, bx.x - bz) : -1;
-        const int s1 = (t0 + 1 < Rn) ? slot_of(pg, bx.y - bz) : -1;
-        const int s2 = (t0 + 2 < Rn) ? slot_of(pg, bx.z - bz) : -1;
-        const int s3 = (t0 + 3 < Rn) ? slot_of(pg, bx.w - bz) : -1;
-        const int cnt = (s0 >= 0) + (s1 >= 0) + (s2 >= 0) + (s3 >= 0);
-        if (ballot(cnt > 0) == 0ull) continue;
-        int pos = ncand + wave_excl_scan(cnt);
-        if (s0 >= 0) rcand[pos++] = t0 | (s0 << 26);
-        if (s1 >= 0) rcand[pos++] = (t0 + 1) | (s1 << 26);
-        if (s2 >= 0) rcand[pos++] = (t0 + 2) | (s2 << 26);
-        if (s3 >= 0) rcand[pos++] = (t0 + 3) | (s3 << 26);
-        ncand += wave_sum_i(cnt);
+    if (kc != gcell) {
+        const int Rq = (int)(slow_rows(Rn) >> 2);
+        const int4* const rbox4 = reinterpret_cast<const int4*>(rbox);
+        ncand = 0;
+        int4 nxt = make_int4(-1, -1, -1, -1);
+        if (l < Rq) nxt = rbox4[l];
+        for (int base = 0; base < Rq; base += WAVE) {
+            const int4 bx = nxt;
+            if (base + WAVE < Rq) nxt = (base + WAVE + l < Rq) ? rbox4[base + WAVE + l] : make_int4(-1, -1, -1, -1);
+            const int t0 = 4 * (base + l);
+            // slots of the lane's four atoms (cells of rows past Rn are never read)
+            const int s0 = (t0 < Rn) ? slot_rc(pg.dim, kc, bx.x) : -1;
+            const int s1 = (t0 + 1 < Rn) ? slot_rc(pg.dim, kc, bx.y) : -1;
+            const int s2 = (t0 + 2 < Rn) ? slot_rc(pg.dim, kc, bx.z) : -1;
+            const int s3 = (t0 + 3 < Rn) ? slot_rc(pg.dim, kc, bx.w) : -1;
+            const int cnt = (s0 >= 0) + (s1 >= 0) + (s2 >= 0) + (s3 >= 0);
+            if (ballot(cnt > 0) == 0ull) continue;
+            int pos = ncand + wave_excl_scan(cnt);
+            if (s0 >= 0) rcand[pos++] = t0 | (s0 << 26);
+            if (s1 >= 0) rcand[pos++] = (t0 + 1) | (s1 << 26);
+            if (s2 >= 0) rcand[pos++] = (t0 + 2) | (s2 << 26);
+            if (s3 >= 0) rcand[pos++] = (t0 + 3) | (s3 << 26);
+            ncand += wave_sum_i(cnt);
+        }
+        gcell = kc;
+        wave_sync();  // gathered atoms visible to every lane
     }
-    wave_sync();  // gathered atoms visible to every lane
     int2* const tidx = cc.nidx + cc.cap;   // SLOW_TMP entries past the list capacity
     float4* const txyz = cc.nxyz + cc.cap;
     int ntmp = 0;
@@ -1754,7 +1766,10 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 if (lb[q] < 0 || lb[q] >= pg.dim3) bad = true;
             }
         }
-        if (ballot(bad) != 0ull) {
+        const bool off_grid = ballot(bad) != 0ull;
+        if (pg.dim >= 1024) {  // cell coordinates are packed in 10 bits
+            status = FA_CAPACITY;
+        } else if (off_grid) {
             status = FA_GRID_RANGE;
         } else {
             int rk[LSL] = {0, 0};
@@ -1795,13 +1810,14 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
             if (pg.slow && pg.dim >= 3) {
                 for (int t = l; t < R.n; t += WAVE) {
                     const float4 p = R.xyzr[t];
-                    // side-chain atoms come with the pose: a cell no slot names
-                    wb[t] = (R.sc_flag && R.sc_flag[t]) ? INT_MIN / 4 : box_of(pg, p.x, p.y, p.z);
+                    // packed cell; side-chain atoms come with the pose: a cell no slot names
+                    wb[t] = (R.sc_flag && R.sc_flag[t]) ? FAR_CELL : cell_packed(pg, p.x, p.y, p.z);
                 }
                 wave_sync();
             }
             // neighbourhood lists when the pose keeps the receptor's grid
             const bool use_nb = !pg.slow && G.nb_start != nullptr && !B.cell_lists;
+            int gcell = -1, ncand = 0;  // slow path: the cell whose 27-cell gather rcand holds
             SlotTab tab{-1, 0, 0};
             NbTab ntab{0, INT_MAX};
             uint64_t rres = residue_fetch(R, lig, uni(s.order[0]));
@@ -1849,7 +1865,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                 else if (!pg.slow)
                     cull_fast(s, R, G, lig, pg, Rn, L, tab, s.sbox[k], s.u.p.span, cc, acc);
                 else if (pg.dim >= 3)
-                    cull_slow(s, R, lig, pg, L, tab, s.sbox[k], cc, acc, wb, rcand);
+                    cull_slow(s, R, lig, pg, L, tab, uni(s.u.p.rcell[k]), cc, acc, wb, rcand, gcell, ncand);
                 else
                     cull_slow_coarse(s, R, lig, pg, L, tab, cc, acc);
                 const int total = wave_sum_i(acc.total);
