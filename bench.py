@@ -190,7 +190,8 @@ def measured_traffic(pop):
 def _unscored(ctx):
     """Poses a context's launches did not score since its last reset."""
     t = ctx.totals()
-    return {"capacity": t["capacity"], "grid_range": t["grid_range"], "bad_ligand": t["bad_ligand"]}
+    return {"capacity": t["capacity"], "grid_range": t["grid_range"], "bad_ligand": t["bad_ligand"],
+            "undefined_face": t["undefined_face"]}
 
 
 def config0_leg(args):

@@ -16,7 +16,7 @@ FA_MAX_FLEX_RESIDUES = 32
 FA_MAX_FLEX_ATOMS = 128
 FA_MAX_SCORABLE = 128
 
-FA_OK, FA_HULL_FAIL, FA_CLASH, FA_CAPACITY, FA_GRID_RANGE, FA_BAD_LIGAND = 0, 1, 2, 3, 4, 5
+FA_OK, FA_HULL_FAIL, FA_CLASH, FA_CAPACITY, FA_GRID_RANGE, FA_BAD_LIGAND, FA_UNDEFINED_FACE = 0, 1, 2, 3, 4, 5, 6
 FA_GENE_GRID, FA_GENE_DIS, FA_GENE_ANG, FA_GENE_DIH, FA_GENE_ROT = -1, 0, 1, 2, 4
 FA_SHAPE_CELL_LISTS = 1
 

@@ -2514,7 +2514,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
                     wave_sync();
                     count_op(s.ops, OP_ORDER, wave_sum_i(ncos));
                     if (ballot(noline) != 0ull) {
-                        status = FA_CAPACITY;
+                        status = FA_UNDEFINED_FACE;
                         break;
                     }
                 }
@@ -2962,6 +2962,7 @@ __global__ __launch_bounds__(WAVE, 2) void cf_pose_kernel(DevReceptor R, const D
             atomicAdd(&B.totals[1], 1ull);
         }
         if (status == FA_GRID_RANGE && l == 0) atomicAdd(&B.totals[2], 1ull);
+        if (status == FA_UNDEFINED_FACE && l == 0) atomicAdd(&B.totals[4], 1ull);
         wave_sync();
         if (B.ops && l < FA_OPCOUNT_N && s.ops[l]) atomicAdd(&B.ops[l], s.ops[l]);
         if (l == 0) {

@@ -265,11 +265,11 @@ int fa_ctx_create(int device_id, int max_pop, fa_ctx** out)
         (rc = fa_ensure(c->prof, 16 * sizeof(unsigned long long))) != FA_SUCCESS ||
         (rc = fa_ensure(c->genes, sizeof(double) * (size_t)c->max_pop * 32)) != FA_SUCCESS ||
         (rc = fa_ensure(c->out, sizeof(fa_cf_result) * (size_t)c->max_pop)) != FA_SUCCESS ||
-        (rc = fa_ensure(c->totals, 4 * sizeof(unsigned long long))) != FA_SUCCESS) {
+        (rc = fa_ensure(c->totals, FA_TOTALS_N * sizeof(unsigned long long))) != FA_SUCCESS) {
         delete c;
         return rc;
     }
-    if (hipMemset(c->totals.p, 0, 4 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMemset(c->totals.p, 0, FA_TOTALS_N * sizeof(unsigned long long)) != hipSuccess) {
         delete c;
         return fa_fail(FA_ERR_HIP, "clear totals");
     }
@@ -764,12 +764,22 @@ int fa_ctx_totals(fa_ctx* c, uint64_t* totals4)
     return FA_SUCCESS;
 }
 
+int fa_ctx_totals_n(fa_ctx* c, uint64_t* totals, int n)
+{
+    if (!c || (!totals && n > 0)) return fa_fail(FA_ERR_INVALID, "null argument");
+    if (n < 0 || n > FA_TOTALS_N) return fa_fail(FA_ERR_INVALID, "n outside [0, FA_TOTALS_N]");
+    FA_HIP(hipSetDevice(c->device));
+    FA_HIP(hipDeviceSynchronize());
+    if (n > 0) FA_HIP(hipMemcpy(totals, c->totals.p, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return FA_SUCCESS;
+}
+
 int fa_ctx_reset_totals(fa_ctx* c)
 {
     if (!c) return fa_fail(FA_ERR_INVALID, "null context");
     FA_HIP(hipSetDevice(c->device));
     FA_HIP(hipDeviceSynchronize());
-    FA_HIP(hipMemset(c->totals.p, 0, 4 * sizeof(uint64_t)));
+    FA_HIP(hipMemset(c->totals.p, 0, FA_TOTALS_N * sizeof(uint64_t)));
     return FA_SUCCESS;
 }
 

@@ -74,7 +74,7 @@ struct fa_ctx {
     int near_cap = 0;
     // pose buffers
     DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops, prof, near_idx, near_xyz, rbox;
-    DevBuf totals;  // 4 x u64 unscored-pose counts since the last reset (not cleared per launch)
+    DevBuf totals;  // FA_TOTALS_N x u64 unscored-pose counts since the last reset (not cleared per launch)
     int max_genes = 0;  // largest gene count of the uploaded ligands
     int n_sc = 0;       // scorable side-chain atoms of the resident receptor
     int prof_poses = 0;  // poses of the last profiled launch

@@ -111,8 +111,9 @@ struct DevBatch {
     int xyz_stride;
     int n_ligands;                  // ligands of the context (pose_lig entries outside [0, n) are rejected)
     unsigned int* counters;         // [4]: slow-grid poses, capacity overflows, work queue, spare
-    unsigned long long* totals;     // [4] never cleared by a launch: slow-grid, capacity, grid-range,
-                                    // bad-ligand poses since the context's last fa_ctx_reset_totals
+    unsigned long long* totals;     // [FA_TOTALS_N] never cleared by a launch: slow-grid, capacity,
+                                    // grid-range, bad-ligand, undefined-face poses since the
+                                    // context's last fa_ctx_reset_totals
     unsigned long long* ops;        // optional: [FA_OPCOUNT_N] operation counters (fa_opcount order)
     unsigned long long* prof;       // optional: [16] cycles per kernel phase (summed over waves)
     // per-wave neighbour lists (HBM scratch): wave w owns entries

@@ -65,6 +65,7 @@ _SIGS = {
     "fa_last_counters": ([_P, _P, _P], C.c_int),
     "fa_last_opcount": ([_P, _P], C.c_int),
     "fa_ctx_totals": ([_P, _P], C.c_int),
+    "fa_ctx_totals_n": ([_P, _P, C.c_int], C.c_int),
     "fa_ctx_reset_totals": ([_P], C.c_int),
     "fa_upload_emat": ([_P, _P, C.c_int], C.c_int),
     "fa_set_phase_profiling": ([_P, C.c_int], C.c_int),
@@ -265,9 +266,10 @@ class Context:
 
     def totals(self) -> dict:
         """Unscored and widened-grid poses over every launch since the last reset."""
-        a = np.zeros(4, np.uint64)
-        _check(_lib.fa_ctx_totals(self._ctx, a.ctypes.data))
-        return {"slow_grid": int(a[0]), "capacity": int(a[1]), "grid_range": int(a[2]), "bad_ligand": int(a[3])}
+        a = np.zeros(5, np.uint64)
+        _check(_lib.fa_ctx_totals_n(self._ctx, a.ctypes.data, 5))
+        return {"slow_grid": int(a[0]), "capacity": int(a[1]), "grid_range": int(a[2]), "bad_ligand": int(a[3]),
+                "undefined_face": int(a[4])}
 
     def reset_totals(self):
         _check(_lib.fa_ctx_reset_totals(self._ctx))

@@ -57,7 +57,10 @@ enum fa_status {
     FA_CLASH = 2,         /* clash pre-pass ≥ CLASH_THRESHOLD (decided before any capacity check) */
     FA_CAPACITY = 3,      /* a device capacity (contacts/vertices/edges) overflowed */
     FA_GRID_RANGE = 4,    /* an atom fell outside the cell grid (undefined in the reference) */
-    FA_BAD_LIGAND = 5     /* the pose names a ligand index the context does not hold */
+    FA_BAD_LIGAND = 5,    /* the pose names a ligand index the context does not hold */
+    FA_UNDEFINED_FACE = 6 /* a face with no second point on its first edge's line: the
+                             reference's order_faces reads past the face's point list
+                             there (LIB/Vcontacts.cpp:794-805), its result is undefined */
 };
 
 /* One flexible side chain (a residue of FA->flex_res with its accepted

@@ -162,6 +162,11 @@ int fa_last_counters(fa_ctx* ctx, uint32_t* slow_grid, uint32_t* capacity);
  * totals4 = [slow_grid, capacity, grid_range, bad_ligand].  Synchronises the
  * device.  No counterpart in the reference (it has no device limits). */
 int fa_ctx_totals(fa_ctx* ctx, uint64_t* totals4);
+/* The same counts and more: the first n of [slow_grid, capacity, grid_range,
+ * bad_ligand, undefined_face] (n <= FA_TOTALS_N; undefined_face counts the
+ * FA_UNDEFINED_FACE poses).  No counterpart in the reference. */
+#define FA_TOTALS_N 5
+int fa_ctx_totals_n(fa_ctx* ctx, uint64_t* totals, int n);
 int fa_ctx_reset_totals(fa_ctx* ctx);
 
 /* Operation counters (fa_opcount) summed over the last launch; they define

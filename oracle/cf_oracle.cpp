@@ -1332,9 +1332,9 @@ fa_cf_result eval_pose(const System& S, Work<R>& W, const double* icv, float* xy
             spec_flag = true;
             continue;
         }
-        if (NV2 < 0) {
+        if (NV2 < 0) {  // -3: the reference reads past a face's point list; -2: capacity
             res.cf = res.app = HUGE_VAL;
-            res.status = FA_CAPACITY;
+            res.status = NV2 == -3 ? FA_UNDEFINED_FACE : FA_CAPACITY;
             return res;
         }
         calc_areas(W, rado, NC, NV, a);

@@ -99,3 +99,29 @@ def test_upload_refused_while_a_ga_or_pool_lives(hip):
         c.upload_system(s)                                       # allowed again
         out = c.eval(0, s.genes[0])
         assert out.shape == (8,)
+
+
+def test_undefined_faces_are_their_own_status(hip, oracle):
+    """Config-4 GA poses (tests/golden/config4_undefined_face.npz, children
+    of the blind dock's first generations) on which a face has no second
+    point on its first edge's line: the reference's order_faces reads past
+    the face's point list there (LIB/Vcontacts.cpp:794-805).  Device and
+    oracle both report FA_UNDEFINED_FACE (cf = +inf), counted apart from
+    device capacity overflows, of which there are none."""
+    import os
+    import oracle_lib as ol
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "config4_undefined_face.npz"))
+    s, _ = synth.make_config4(n_ligands=64, pop=8)
+    d = abi.Descriptors(s.receptor, s.ligands, s.params)
+    with hip.Context() as c:
+        c.upload_system(s)
+        c.reset_totals()
+        for ic, k in zip(fx["ic"], fx["lig"]):
+            g = ic[None, :s.ligands[int(k)].n_genes]
+            out = c.eval(int(k), g)
+            with ol.gpu_mirror(oracle):
+                ref = ol.oracle_eval(oracle, d, int(k), g)
+            assert out["status"][0] == ref["status"][0] == abi.FA_UNDEFINED_FACE
+            assert np.isinf(out["cf"][0]) and out["cf"][0] > 0
+        t = c.totals()
+    assert t["undefined_face"] == len(fx["lig"]) and t["capacity"] == 0

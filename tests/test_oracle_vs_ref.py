@@ -237,13 +237,13 @@ def test_engulfing_branches_reached_and_bit_exact(oracle):
     and the oracle equals the strict reference there bit for bit, except on
     poses where the reference reads past a face's point list (a face with no
     second point on its first edge line, LIB/Vcontacts.cpp:808-812): the
-    oracle reports those as not scored (FA_CAPACITY), as the device does."""
+    oracle reports those as not scored (FA_UNDEFINED_FACE), as the device does."""
     s = synth.make_engulf_system(n_poses=400)
     desc = abi.Descriptors(s.receptor, s.ligands, s.params)
     out, br = _branch_stats(oracle, desc, 0, s.genes[0])
     assert br["epi1"] > 100 and br["epi2"] > 100 and br["epi3"] > 100, br
     r_out, _, _ = ol.ref_eval(ref, desc, 0, s.genes[0])
-    keep = out["status"] != abi.FA_CAPACITY
+    keep = out["status"] != abi.FA_UNDEFINED_FACE
     assert keep.sum() >= len(keep) - 5
     assert np.array_equal(out["status"][keep], r_out["status"][keep])
     for f in ("com", "wal", "sas", "totsas", "cf"):
