@@ -138,3 +138,111 @@ extern "C" int fa_ref_cleft_grid(const float* xyz, const float* radius, int n, c
     free(g);
     return ng;
 }
+
+/*
+ * The reference's direct-mode ligand set-up from a MOL2 file, in the order
+ * LIB/top.cpp:845-897 and :982-998 call it: read_mol2_ligand (atoms, SYBYL
+ * types and radii, bonds, BFS reconstruction tree, buildic, bonded matrix
+ * within bloops), assign_radii_types (which leaves the ligand's radii),
+ * the GPA block (ori = the ligand's centroid, gpa = its first three atoms,
+ * buildic_point of the first; restated from top.cpp:864-897, inline there),
+ * ic_bounds for an n_grid-point cleft grid ("locclf"), add2_optimiz_vec for
+ * the translation gene (opt[1] = -1) and the three rotation genes
+ * (opt[1] = 0), and create_rebuild_list.  Ligand atoms come out in file
+ * order, their rec[0..2] as ligand indices (-1 for the ori frame); bonded is
+ * the residue's matrix (bonds away, -1 beyond bloops); mov the rebuild order;
+ * the genes' typ, atom, min and max; grid0 the cleft grid's row 0 (the GPA
+ * atom: coordinates, then dis/ang/dih).  Returns the atom count, -1 when
+ * the reader fails, -2 when an output array is too small.
+ */
+int read_mol2_ligand(FA_Global* FA, atom** atoms, resid** residue, const char* mol2_file);
+void buildic_point(FA_Global* FA, float coor[], float* dis, float* ang, float* dih);
+
+extern "C" int fa_ref_read_mol2(const char* mol2, int n_grid, int max_atoms, float* xyz, float* radius, int* type,
+                                int* rec, float* ic, int* bonded, int* mov, int* n_mov, int max_genes, int* gene_typ,
+                                int* gene_atm, double* gene_min, double* gene_max, int* n_genes, float* ori,
+                                float* grid0)
+{
+    FA_Global* FA = new FA_Global();
+    FA->MIN_NUM_ATOM = 1000;  // LIB/top.cpp:222-230
+    FA->MIN_NUM_RESIDUE = 250;
+    FA->MIN_ROTAMER = 1;
+    FA->MIN_FLEX_BONDS = 5;
+    FA->MIN_OPTRES = 1;
+    FA->MIN_PAR = 6;
+    FA->ntypes = 40;
+    FA->bloops = 2;           // LIB/top.cpp:296
+    FA->htpmode = true;       // no per-atom printing in buildlist
+    FA->num_grd = n_grid;
+    FA->num_atm = (int*)calloc(MAX_ATOM_NUMBER, sizeof(int));
+    atom* atoms = (atom*)calloc(FA->MIN_NUM_ATOM, sizeof(atom));
+    resid* residue = (resid*)calloc(FA->MIN_NUM_RESIDUE, sizeof(resid));
+    if (!read_mol2_ligand(FA, &atoms, &residue, mol2)) return -1;
+    assign_radii_types(FA, atoms, residue);
+    const int lr = FA->res_cnt;
+    const int fa = residue[lr].fatm[0], la = residue[lr].latm[0], n = la - fa + 1;
+    if (n > max_atoms) return -2;
+    // LIB/top.cpp:864-897 (the GPA block of direct mode)
+    FA->ori[0] = FA->ori[1] = FA->ori[2] = 0.0f;
+    for (int ai = fa; ai <= la; ai++) {
+        FA->ori[0] += atoms[ai].coor[0];
+        FA->ori[1] += atoms[ai].coor[1];
+        FA->ori[2] += atoms[ai].coor[2];
+    }
+    if (n > 0) {
+        FA->ori[0] /= n;
+        FA->ori[1] /= n;
+        FA->ori[2] /= n;
+    }
+    int* gpa = (int*)malloc(3 * sizeof(int));
+    gpa[0] = fa;
+    gpa[1] = (n > 1) ? fa + 1 : fa;
+    gpa[2] = (n > 2) ? fa + 2 : fa;
+    residue[lr].gpa = gpa;
+    buildic_point(FA, atoms[fa].coor, &atoms[fa].dis, &atoms[fa].ang, &atoms[fa].dih);
+    // the cleft grid's row 0 (LIB/generate_grid.cpp:24-28)
+    for (int j = 0; j < 3; ++j) grid0[j] = atoms[gpa[0]].coor[j];
+    grid0[3] = atoms[gpa[0]].dis;
+    grid0[4] = atoms[gpa[0]].ang;
+    grid0[5] = atoms[gpa[0]].dih;
+    // LIB/top.cpp:940-998: parameter vectors, bounds, translation and rotation genes
+    FA->map_par = (optmap*)calloc(FA->MIN_PAR, sizeof(optmap));
+    FA->opt_par = (double*)calloc(FA->MIN_PAR, sizeof(double));
+    FA->del_opt_par = (double*)calloc(FA->MIN_PAR, sizeof(double));
+    FA->min_opt_par = (double*)calloc(FA->MIN_PAR, sizeof(double));
+    FA->max_opt_par = (double*)calloc(FA->MIN_PAR, sizeof(double));
+    FA->map_opt_par = (int*)calloc(FA->MIN_PAR, sizeof(int));
+    char rngopt[] = "locclf";
+    ic_bounds(FA, rngopt);
+    int opt[2] = {FA->resligand->number, -1};
+    add2_optimiz_vec(FA, atoms, residue, opt, ' ', "");
+    opt[1] = 0;
+    add2_optimiz_vec(FA, atoms, residue, opt, ' ', "");
+    create_rebuild_list(FA, atoms, residue);
+    // outputs
+    for (int i = 0; i < n; ++i) {
+        const atom& a = atoms[fa + i];
+        for (int j = 0; j < 3; ++j) {
+            xyz[3 * i + j] = a.coor[j];
+            rec[3 * i + j] = a.rec[j] == 0 ? -1 : a.rec[j] - fa;
+        }
+        radius[i] = a.radius;
+        type[i] = a.type;
+        ic[3 * i + 0] = a.dis;
+        ic[3 * i + 1] = a.ang;
+        ic[3 * i + 2] = a.dih;
+        for (int j = 0; j < n; ++j) bonded[i * n + j] = residue[lr].bonded[i][j];
+    }
+    *n_mov = FA->nmov[0];
+    for (int k = 0; k < FA->nmov[0]; ++k) mov[k] = FA->mov[0][k] - fa;
+    *n_genes = FA->npar;
+    if (FA->npar > max_genes) return -2;
+    for (int g = 0; g < FA->npar; ++g) {
+        gene_typ[g] = FA->map_par[g].typ;
+        gene_atm[g] = FA->map_par[g].atm - fa;
+        gene_min[g] = FA->min_opt_par[g];
+        gene_max[g] = FA->max_opt_par[g];
+    }
+    for (int j = 0; j < 3; ++j) ori[j] = FA->ori[j];
+    return n;
+}

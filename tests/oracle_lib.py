@@ -81,7 +81,40 @@ def load_ref_reader():
                                          C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
                                          _P, _P, _P, _P, _P, _P, _P, _P]
     lib.fa_ref_read_receptor.restype = C.c_int
+    lib.fa_ref_read_mol2.argtypes = [C.c_char_p, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int,
+                                     _P, _P, _P, _P, _P, _P, _P]
+    lib.fa_ref_read_mol2.restype = C.c_int
     return lib
+
+
+def ref_read_mol2(lib, path, n_grid, max_atoms=512, max_genes=16):
+    """The reference's direct-mode ligand set-up on a MOL2 file
+    (oracle/ref/ref_reader.cpp fa_ref_read_mol2)."""
+    n = max_atoms
+    xyz = np.zeros((n, 3), np.float32)
+    rad = np.zeros(n, np.float32)
+    typ = np.zeros(n, np.int32)
+    rec = np.zeros((n, 3), np.int32)
+    ic = np.zeros((n, 3), np.float32)
+    bonded = np.zeros(n * n, np.int32)
+    mov = np.zeros(n, np.int32)
+    nmov = np.zeros(1, np.int32)
+    gt = np.zeros(max_genes, np.int32)
+    ga = np.zeros(max_genes, np.int32)
+    gmin = np.zeros(max_genes)
+    gmax = np.zeros(max_genes)
+    ng = np.zeros(1, np.int32)
+    ori = np.zeros(3, np.float32)
+    grid0 = np.zeros(6, np.float32)
+    r = lib.fa_ref_read_mol2(path.encode(), n_grid, n, _ptr(xyz), _ptr(rad), _ptr(typ), _ptr(rec), _ptr(ic),
+                             _ptr(bonded), _ptr(mov), _ptr(nmov), max_genes, _ptr(gt), _ptr(ga), _ptr(gmin),
+                             _ptr(gmax), _ptr(ng), _ptr(ori), _ptr(grid0))
+    if r < 0:
+        raise ValueError(f"fa_ref_read_mol2 failed ({r})")
+    g = int(ng[0])
+    return dict(xyz=xyz[:r], radius=rad[:r], type=typ[:r], rec=rec[:r], ic=ic[:r],
+                bonded=bonded[:r * r].reshape(r, r), mov=mov[:int(nmov[0])], gene_typ=gt[:g], gene_atm=ga[:g],
+                gene_min=gmin[:g], gene_max=gmax[:g], ori=ori, grid0=grid0)
 
 
 def ref_read_receptor(lib, pdb, amino_def, nucl_def, tmp_path, ntypes=40, exclude_het=0, remove_water=1,
