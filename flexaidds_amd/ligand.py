@@ -1,4 +1,4 @@
-"""Ligand MOL2 files into the ligand descriptors the contact function takes.
+"""Ligand MOL2 (and SDF) files into the ligand descriptors the contact function takes.
 
 For a run with a MOL2 ligand the reference takes these steps (direct mode,
 LIB/top.cpp:845-897 and :982-998):
@@ -239,6 +239,72 @@ def read_mol2(src) -> Mol2Ligand:
                       radius=rad, bonds=blist)
 
 
+# element -> FlexAID type and radius of the SDF reader (LIB/SdfReader.cpp:26-60)
+ELEMENT_TYPE = {b"C": 1, b"N": 4, b"O": 10, b"S": 16, b"P": 20, b"F": 13, b"Cl": 14, b"Br": 15, b"I": 21,
+                b"H": 22, b"Fe": 30, b"Zn": 31, b"Ca": 32, b"Mg": 33}
+ELEMENT_RADIUS = {b"C": F(1.70), b"N": F(1.55), b"O": F(1.52), b"S": F(1.80), b"P": F(1.80), b"F": F(1.47),
+                  b"Cl": F(1.75), b"Br": F(1.85), b"I": F(1.98), b"H": F(1.20)}
+
+
+def _atoi3(b: bytes) -> int:
+    sc = _Scan(b[:3])
+    v = sc.d()
+    return 0 if v is None else v
+
+
+def read_sdf(src) -> Mol2Ligand:
+    """LIB/SdfReader.cpp:63-263: the first molecule of an MDL V2000 file --
+    the name (line 1), the counts (atoms and bonds in the first two
+    three-character columns of line 4), one atom per line (x y z read with
+    sscanf, the symbol from columns 31-33 without blanks; the line keeps
+    its newline there, which a symbol ending at column 33 takes in), one bond
+    per line (three-character columns).  Types and radii come from the
+    element; bonds naming an atom outside 1..n are skipped, six per atom at
+    most."""
+    data = src if isinstance(src, (bytes, bytearray)) else open(src, "rb").read()
+    lines = _fgets(bytes(data), LINE_BUF)
+    if len(lines) < 4:
+        raise ValueError("SDF file ends before its counts line")
+    name = b"LIG"
+    first = lines[0][:-1] if lines[0].endswith(b"\n") else lines[0]
+    tok = _Scan(first).s_(63)
+    if tok is not None:
+        name = tok
+    natoms, nbonds = _atoi3(lines[3][0:3]), _atoi3(lines[3][3:6])
+    if natoms <= 0 or natoms > 9999 or nbonds < 0 or nbonds > 9999:
+        raise ValueError(f"invalid SDF counts ({natoms} atoms, {nbonds} bonds)")
+    if len(lines) < 4 + natoms + nbonds:
+        raise ValueError("SDF file ends inside its atom or bond block")
+    xyz = np.zeros((natoms, 3), F)
+    elems = []
+    for i in range(natoms):
+        buf = lines[4 + i]
+        sc = _Scan(buf)
+        for j in range(3):               # a field that does not convert keeps 0
+            v = sc.f()
+            if v is None:
+                break
+            xyz[i, j] = v
+        k = buf.find(b"\0")
+        blen = len(buf) if k < 0 else k
+        sym = bytes(c for c in buf[31:min(34, blen)] if c not in (32, 0))
+        elems.append(sym[:3])
+    typ = np.array([ELEMENT_TYPE.get(e, DUMMY_TYPE) for e in elems], np.int32)
+    rad = np.array([ELEMENT_RADIUS.get(e, F(1.70)) for e in elems], F)
+    blist = [[] for _ in range(natoms)]
+    for b in range(nbonds):
+        buf = lines[4 + natoms + b]
+        i1, i2 = _atoi3(buf[0:3]), _atoi3(buf[3:6])
+        if i1 < 1 or i1 > natoms or i2 < 1 or i2 > natoms:
+            continue
+        if len(blist[i1 - 1]) < MAX_BONDS:
+            blist[i1 - 1].append(i2 - 1)
+        if len(blist[i2 - 1]) < MAX_BONDS:
+            blist[i2 - 1].append(i1 - 1)
+    names = [(e.decode(errors="replace").ljust(2) + str(i % 100))[:4].encode() for i, e in enumerate(elems)]
+    return Mol2Ligand(name=name[:3], atom_names=names, sybyl=elems, xyz=xyz, type=typ, radius=rad, bonds=blist)
+
+
 # ------------------------------------------------------------ set-up
 def reconstruction_tree(m: Mol2Ligand) -> np.ndarray:
     """rec[0..2] of every atom (LIB/Mol2Reader.cpp:313-355): breadth-first
@@ -405,16 +471,21 @@ def descriptor(st: LigandSetup, grid_ic: np.ndarray, globalmin, globalmax, maxwi
                   maxwidth=float(maxwidth), parent=parent)
 
 
-def direct_system(rec_xyz, rec_radius, rec_type, center, mol2, params, spacer=0.375, cleft_params=None):
+def direct_system(rec_xyz, rec_radius, rec_type, center, lig_file, params, spacer=0.375, cleft_params=None):
     """A receptor (atoms in the reference's order, e.g. from
     `receptor.read_receptor`, with calc_center's 11 values: ori, globalmin,
-    globalmax, maxdst, maxwidth) and a MOL2 ligand set up as the reference's
-    direct mode does (LIB/top.cpp:845-998): the ligand's centroid as ori,
-    the automatic binding-site grid over the receptor's atoms followed by
-    the ligand's, the translation and rotation genes.  Returns a
-    `synth.System` ready for `Context.upload_system`, and the cleft grid."""
+    globalmax, maxdst, maxwidth) and a MOL2 or SDF ligand (by extension:
+    .sdf/.mol are SDF) set up as the reference's direct mode does
+    (LIB/top.cpp:845-998): the ligand's centroid as ori, the automatic
+    binding-site grid over the receptor's atoms followed by the ligand's,
+    the translation and rotation genes.  An SDF ligand gets the MOL2 path's
+    reconstruction tree and bonded matrix: the reference's SDF reader
+    builds neither (LIB/SdfReader.cpp:205-260), which leaves it nothing to
+    rebuild a pose from.  Returns a `synth.System` ready for
+    `Context.upload_system`, and the cleft grid."""
     from .synth import Receptor, System
-    m = read_mol2(mol2)
+    is_sdf = isinstance(lig_file, str) and lig_file.lower().endswith((".sdf", ".mol"))
+    m = read_sdf(lig_file) if is_sdf else read_mol2(lig_file)
     ori = centroid(m)
     allxyz = np.vstack([np.asarray(rec_xyz, F), m.xyz]).astype(F)
     allrad = np.concatenate([np.asarray(rec_radius, F), m.radius]).astype(F)

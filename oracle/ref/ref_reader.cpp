@@ -246,3 +246,36 @@ extern "C" int fa_ref_read_mol2(const char* mol2, int n_grid, int max_atoms, flo
     for (int j = 0; j < 3; ++j) ori[j] = FA->ori[j];
     return n;
 }
+
+/*
+ * The reference's SDF reader (LIB/SdfReader.cpp:63-263) on a file: atoms in
+ * file order (coordinates, element-derived type and radius) and each atom's
+ * bond list as read_sdf_ligand leaves it (ligand indices, at most six).
+ * Returns the atom count, -1 when the reader fails, -2 when an output array
+ * is too small.
+ */
+int read_sdf_ligand(FA_Global* FA, atom** atoms, resid** residue, const char* sdf_file);
+
+extern "C" int fa_ref_read_sdf(const char* sdf, int max_atoms, float* xyz, float* radius, int* type, int* bonds7)
+{
+    FA_Global* FA = new FA_Global();
+    FA->MIN_NUM_ATOM = 1000;
+    FA->MIN_NUM_RESIDUE = 250;
+    FA->MIN_FLEX_BONDS = 5;
+    FA->MIN_OPTRES = 1;
+    FA->num_atm = (int*)calloc(MAX_ATOM_NUMBER, sizeof(int));
+    atom* atoms = (atom*)calloc(FA->MIN_NUM_ATOM, sizeof(atom));
+    resid* residue = (resid*)calloc(FA->MIN_NUM_RESIDUE, sizeof(resid));
+    if (!read_sdf_ligand(FA, &atoms, &residue, sdf)) return -1;
+    const int fa = residue[FA->res_cnt].fatm[0], la = residue[FA->res_cnt].latm[0], n = la - fa + 1;
+    if (n > max_atoms) return -2;
+    for (int i = 0; i < n; ++i) {
+        const atom& a = atoms[fa + i];
+        for (int j = 0; j < 3; ++j) xyz[3 * i + j] = a.coor[j];
+        radius[i] = a.radius;
+        type[i] = a.type;
+        bonds7[7 * i] = a.bond[0];
+        for (int k = 1; k <= 6; ++k) bonds7[7 * i + k] = k <= a.bond[0] ? a.bond[k] - fa : -1;
+    }
+    return n;
+}

@@ -84,7 +84,22 @@ def load_ref_reader():
     lib.fa_ref_read_mol2.argtypes = [C.c_char_p, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int,
                                      _P, _P, _P, _P, _P, _P, _P]
     lib.fa_ref_read_mol2.restype = C.c_int
+    lib.fa_ref_read_sdf.argtypes = [C.c_char_p, C.c_int, _P, _P, _P, _P]
+    lib.fa_ref_read_sdf.restype = C.c_int
     return lib
+
+
+def ref_read_sdf(lib, path, max_atoms=512):
+    """The reference's SDF reader on a file (oracle/ref/ref_reader.cpp fa_ref_read_sdf)."""
+    n = max_atoms
+    xyz = np.zeros((n, 3), np.float32)
+    rad = np.zeros(n, np.float32)
+    typ = np.zeros(n, np.int32)
+    b7 = np.zeros((n, 7), np.int32)
+    r = lib.fa_ref_read_sdf(path.encode(), n, _ptr(xyz), _ptr(rad), _ptr(typ), _ptr(b7))
+    if r < 0:
+        raise ValueError(f"fa_ref_read_sdf failed ({r})")
+    return dict(xyz=xyz[:r], radius=rad[:r], type=typ[:r], bonds=[list(b7[i, 1:1 + b7[i, 0]]) for i in range(r)])
 
 
 def ref_read_mol2(lib, path, n_grid, max_atoms=512, max_genes=16):

@@ -113,3 +113,51 @@ def test_mol2_ligand_builds_its_own_pose(tmp_path):
     d = abi.Descriptors(rec, [lig], synth.Params(ntypes=40, emat=np.zeros((40, 40), np.float32)))
     _, xyz, _ = ol.oracle_pose(oracle, d, 0, genes[0])
     assert np.abs(xyz - m.xyz).max() < 2e-3
+
+
+SDF_CASES = [
+    dict(n_atoms=15, seed=31),
+    dict(n_atoms=40, seed=32, extra_bonds=5, tail_symbol=True),
+    dict(n_atoms=22, seed=33, bad_bond=True, seventh_bond=True),
+    dict(n_atoms=30, seed=34, short_coord=True),
+]
+
+
+@needs_ref
+@pytest.mark.parametrize("case", SDF_CASES, ids=[f"n{c['n_atoms']}s{c['seed']}" for c in SDF_CASES])
+def test_sdf_reader_equals_reference(tmp_path, case):
+    """Atoms (coordinates, element types and radii, including a symbol that
+    runs into the line's newline and a coordinate that does not convert)
+    and bond lists as the reference's SDF reader leaves them."""
+    from mol2_gen import write_sdf
+    case = dict(case)
+    rng = np.random.default_rng(case.pop("seed"))
+    path = str(tmp_path / "lig.sdf")
+    write_sdf(path, rng, **case)
+    r = ol.ref_read_sdf(ref, path)
+    m = ligand.read_sdf(path)
+    assert m.n_atoms == len(r["xyz"])
+    assert np.array_equal(m.xyz, r["xyz"])
+    assert np.array_equal(m.type, r["type"])
+    assert np.array_equal(m.radius, r["radius"])
+    assert [list(b) for b in m.bonds] == [[int(x) for x in b] for b in r["bonds"]]
+    if case.get("tail_symbol"):  # the symbol runs into the newline: no element matches, dummy type
+        assert np.all(r["type"][::3] == ligand.DUMMY_TYPE) and np.all(m.type[::3] == ligand.DUMMY_TYPE)
+
+
+def test_sdf_ligand_builds_its_own_pose(tmp_path):
+    """An SDF ligand set up with the MOL2 path's tree rebuilds its own pose."""
+    from mol2_gen import write_sdf
+    rng = np.random.default_rng(41)
+    path = str(tmp_path / "lig.sdf")
+    write_sdf(path, rng, 26, extra_bonds=3)
+    m = ligand.read_sdf(path)
+    st = ligand.setup(m, 10)
+    lig = ligand.descriptor(st, np.tile(st.grid0[3:], (9, 1)), m.xyz.min(0) - 5, m.xyz.max(0) + 5, 50.0)
+    lig.gene_min[0] = 0.0
+    genes = np.array([0.0, st.ic[st.gpa[1], 1], st.ic[st.gpa[1], 2], st.ic[st.gpa[2], 2]])
+    rec = synth.Receptor(xyz=np.zeros((4, 3), np.float32) + 60, radius=np.full(4, 1.8, np.float32),
+                         type=np.ones(4, np.int32), center=np.zeros(3))
+    d = abi.Descriptors(rec, [lig], synth.Params(ntypes=40, emat=np.zeros((40, 40), np.float32)))
+    _, xyz, _ = ol.oracle_pose(ol.load_oracle(), d, 0, genes)
+    assert np.abs(xyz - m.xyz).max() < 2e-3
