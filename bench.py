@@ -447,12 +447,21 @@ def served_leg(args):
             plain_ms += ga.run(1, graph=False)
             plain_poses += args.served_pop
         ga.close()
+    # each receptor's grand partition function over every session it served
+    # (warm-up jobs included; a re-docked ligand keeps its last ln Z)
+    targets = []
+    for q, ts in enumerate(srv.targets):
+        g = ts.grand_partition()
+        top = ts.rank_ligands()[0]
+        targets.append({"receptor": q, "sessions": ts.completed_sessions(), "ligands": g.num_ligands(),
+                        "ln_xi_300K": round(g.log_Xi(), 6), "p_empty": g.empty_probability(),
+                        "top": top.name, "top_dG": round(top.dG, 4), "top_p_bound": round(top.p_bound, 6)})
     srv.close()
     plain = plain_poses / (plain_ms / 1e3)
     out = dict(main_s)
     out.update({"receptors": len(systems), "population": args.served_pop, "generations": args.served_generations,
                 "plain_poses_per_s": round(plain, 1), "vs_plain": round(main_s["poses_per_s"] / plain, 3),
-                "scenarios": scen,
+                "scenarios": scen, "targets": targets,
                 "note": "at the specified rate the stream offers more poses/s than one GPU scores, so every "
                         "request waits for the backlog and latency grows with the stream's length (no steady "
                         "state exists); the low-rate scenario is below capacity"})

@@ -138,6 +138,7 @@ class Context:
     def __init__(self, device: int = 0, max_pop: int = 0):
         self._ctx = C.c_void_p()
         _check(_lib.fa_ctx_create(device, max_pop, C.byref(self._ctx)))
+        self.device = int(device)
         self._desc = None
         self.n_genes = []
 

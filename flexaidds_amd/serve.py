@@ -9,6 +9,12 @@ resident jobs on the pool's stream and returns); finished jobs are collected
 then.  The pools' launches overlap on the device, and jobs that arrive while
 others run join their pool at its next generation.
 
+Each receptor also has a TargetServer (flexaidds_amd.target): every
+finished job is registered there as a completed docking session under its
+request's ligand name, so the receptor's grand partition function, binding
+probabilities, selectivities and ranking follow the served stream
+(LIB/TargetServer.h:81-102).
+
 There is no CPU fallback: the pools run through the HIP library.
 """
 from __future__ import annotations
@@ -18,13 +24,14 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import hip, synth
+from . import hip, synth, target
 
 
 @dataclass
 class Served:
     results: dict = field(default_factory=dict)      # request index -> result dict
     latency_s: dict = field(default_factory=dict)    # request index -> completion - arrival
+    finished: list = field(default_factory=list)      # request indices in completion order
     wall_s: float = 0.0
     steps: int = 0
     poses: int = 0
@@ -44,7 +51,7 @@ class Server:
     def __init__(self, systems, population=1024, offspring=None, max_slots=64, hbm_budget_per_pool=4 << 30,
                  temperature=300.0, fitness="LINEAR", device=0, max_running=0):
         self.systems = systems
-        self.ctx, self.pools, self.limits = [], [], []
+        self.ctx, self.pools, self.limits, self.targets = [], [], [], []
         for s in systems:
             c = hip.Context(device)
             c.upload_system(s)
@@ -53,6 +60,7 @@ class Server:
                                        hbm_budget=hbm_budget_per_pool, temperature=temperature, fitness=fitness,
                                        max_running=max_running))
             self.limits.append([synth.ga_limits(l) for l in s.ligands])
+            self.targets.append(target.TargetServer(c, temperature))
 
     def close(self):
         for p in self.pools:
@@ -61,8 +69,9 @@ class Server:
             c.close()
 
     def run(self, requests, realtime=True, on_step=None) -> Served:
-        """Serve `requests` (dicts with t, receptor, ligand, generations, seed)
-        in arrival order.  With realtime, a request is admitted once the wall
+        """Serve `requests` (dicts with t, receptor, ligand, generations, seed
+        and optionally name, the ligand's name for the receptor's
+        TargetServer, by default "L<ligand>") in arrival order.  With realtime, a request is admitted once the wall
         clock passes its arrival time; otherwise all arrive at once.
 
         Each pool is stepped as soon as its own previous step has ended (a
@@ -95,8 +104,13 @@ class Server:
                         i = owner.pop((p, res["id"]))
                         done = time.perf_counter() - t0
                         out.results[i] = res
+                        out.finished.append(i)
                         out.latency_s[i] = done - (requests[i]["t"] if realtime else 0.0)
                         out.poses += res["n_poses"]
+                        r = requests[i]
+                        lig = self.systems[p].ligands[r["ligand"]]
+                        self.targets[p].register_result(target.session_from_job(
+                            self.targets[p], r.get("name", f"L{r['ligand']}"), res, lig))
                         busy[p] -= 1
                     inflight[p] = False
                     progressed = True
