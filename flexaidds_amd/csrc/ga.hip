@@ -23,6 +23,8 @@ struct fa_ga {
     int jobs = 0, Gmax = 0;
     fa::GaDev d{};
     DevBuf rank, gi, gic, res, gi2, gic2, res2, cum, lim, ngenes, ens, cnt, lig, thermo, grand, fit, fit2, share, sig;
+    DevBuf dset, dzero, rkey, rval, rzero, csig, cdup, need, ddstats;  // duplicate rejection
+    long long gens_run = 0;   // generations enqueued (bounds the signatures accepted)
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -32,13 +34,61 @@ namespace {
 
 int ga_blocks(long long n) { return (int)std::max(1LL, (n + 255) / 256); }
 
+int ceil_log2(long long n)
+{
+    int l = 0;
+    while ((1LL << l) < n) ++l;
+    return l;
+}
+
+// signature set of each job large enough (at most half full) for the
+// signatures accepted up to `generations` generations: population + n_new
+// per generation; grown by rehashing into a table twice as large or more
+int ensure_dset(fa_ga* ga, long long generations)
+{
+    fa::GaDev& d = ga->d;
+    const long long want = 2 * ((long long)d.pop + generations * (long long)d.n_new);
+    if (d.dset && (1LL << d.dset_log2) >= want) return FA_SUCCESS;
+    const int log2 = std::max(ceil_log2(want), d.dset ? d.dset_log2 + 1 : 0);
+    if (log2 > 34) return fa_fail(FA_ERR_LIMIT, "duplicate-rejection table above 2^34 entries per job");
+    DevBuf nb;
+    int rc = fa_ensure(nb, sizeof(unsigned long long) * ((size_t)ga->jobs << log2));
+    if (rc) return rc;
+    fa::GaDev nd = d;
+    nd.dset = static_cast<unsigned long long*>(nb.p);
+    nd.dset_log2 = log2;
+    hipError_t e = hipMemsetAsync(nb.p, 0, sizeof(unsigned long long) * ((size_t)ga->jobs << log2), ga->stream);
+    if (e == hipSuccess && d.dset) {
+        fa::ga_launch_dset_rehash(nd, d.dset, d.dset_log2, ga->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ga->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(nb.p);
+        return fa_fail(FA_ERR_HIP, std::string("duplicate table: ") + hipGetErrorString(e));
+    }
+    if (ga->dset.p) (void)hipFree(ga->dset.p);
+    ga->dset = nb;
+    d.dset = nd.dset;
+    d.dset_log2 = log2;
+    // the captured generation holds the old table's address
+    if (ga->exec) (void)hipGraphExecDestroy(ga->exec);
+    if (ga->graph) (void)hipGraphDestroy(ga->graph);
+    ga->exec = nullptr;
+    ga->graph = nullptr;
+    return FA_SUCCESS;
+}
+
 // one generation of every job, all on ga->stream (capturable)
 int ga_generation(fa_ga* ga)
 {
     const fa::GaDev& d = ga->d;
     const size_t rows_parents = (size_t)d.jobs * d.pop;
     fa::ga_launch_tick(d, ga->stream);
-    fa::ga_launch_breed(d, ga->stream);
+    if (d.dd_rounds > 0)
+        fa::ga_launch_dedup_fill(d, 0, ga->stream);
+    else
+        fa::ga_launch_breed(d, ga->stream);
     FA_HIP(hipGetLastError());
     const int* lig = static_cast<const int*>(ga->lig.p) + rows_parents;
     int rc = fa_launch_poses(ga->ctx, d.jobs * d.n_new, d.gic + rows_parents * d.Gmax, d.Gmax, lig,
@@ -69,6 +119,13 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     const long long pop = p->population, n_new = p->offspring;
     if ((long long)n_jobs * (pop + n_new) > (1LL << 30) || pop + n_new > (1LL << 30))
         return fa_fail(FA_ERR_LIMIT, "jobs * (population + offspring) above 2^30");
+    const int R = p->dedup_rounds;
+    if (R < 0 || R > 1024) return fa_fail(FA_ERR_INVALID, "dedup_rounds in [0, 1024]");
+    // candidate rows per job (a batch of initial chromosomes or of children)
+    const long long n_cand = R > 0 ? std::max(pop, 2 * ((n_new + 1) / 2)) : 0;
+    if ((long long)R * n_cand >= (1LL << 31)) return fa_fail(FA_ERR_LIMIT, "dedup_rounds * batch above 2^31 draws");
+    if ((long long)n_jobs * (pop + n_new + n_cand) > (1LL << 30))
+        return fa_fail(FA_ERR_LIMIT, "jobs * (population + offspring + candidates) above 2^30");
     int Gmax = 0;
     std::vector<int> G(n_jobs);
     for (int k = 0; k < n_jobs; ++k) {
@@ -111,14 +168,14 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     ga->p = *p;
     ga->jobs = n_jobs;
     ga->Gmax = Gmax;
-    const size_t J = (size_t)n_jobs, n_all = (size_t)(pop + n_new);
+    const size_t J = (size_t)n_jobs, n_all = (size_t)(pop + n_new), rows_all = J * (n_all + (size_t)n_cand);
     auto fail_free = [&](int code) {
         fa_ga_destroy(ga);
         return code;
     };
     int rc;
-    if ((rc = fa_ensure(ga->rank, sizeof(int) * J * n_all)) || (rc = fa_ensure(ga->gi, sizeof(int32_t) * J * n_all * Gmax)) ||
-        (rc = fa_ensure(ga->gic, sizeof(double) * J * n_all * Gmax)) ||
+    if ((rc = fa_ensure(ga->rank, sizeof(int) * J * n_all)) || (rc = fa_ensure(ga->gi, sizeof(int32_t) * rows_all * Gmax)) ||
+        (rc = fa_ensure(ga->gic, sizeof(double) * rows_all * Gmax)) ||
         (rc = fa_ensure(ga->res, sizeof(fa_cf_result) * J * n_all)) ||
         (rc = fa_ensure(ga->gi2, sizeof(int32_t) * J * pop * Gmax)) ||
         (rc = fa_ensure(ga->gic2, sizeof(double) * J * pop * Gmax)) ||
@@ -129,6 +186,14 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
         (rc = fa_ensure(ga->ens, sizeof(fa::LseState) * J)) || (rc = fa_ensure(ga->cnt, 2 * sizeof(unsigned long long))) ||
         (rc = fa_ensure(ga->lig, sizeof(int) * J * n_all)) || (rc = fa_ensure(ga->thermo, sizeof(fa_thermo) * J)) ||
         (rc = fa_ensure(ga->grand, sizeof(double) * (2 * J + 2))))
+        return fail_free(rc);
+    const int rlog2 = R > 0 ? ceil_log2(2 * n_cand) : 0;
+    if (R > 0 &&
+        ((rc = fa_ensure(ga->dzero, sizeof(int) * J)) || (rc = fa_ensure(ga->rkey, sizeof(unsigned long long) * (J << rlog2))) ||
+         (rc = fa_ensure(ga->rval, sizeof(int) * (J << rlog2))) || (rc = fa_ensure(ga->rzero, sizeof(int) * J)) ||
+         (rc = fa_ensure(ga->csig, sizeof(unsigned long long) * J * n_cand)) ||
+         (rc = fa_ensure(ga->cdup, sizeof(int) * J * n_cand)) || (rc = fa_ensure(ga->need, sizeof(int) * J)) ||
+         (rc = fa_ensure(ga->ddstats, sizeof(unsigned long long) * 2 * J))))
         return fail_free(rc);
     // ligand of every row: parents of job 0, 1, ..., then children of job 0, 1, ...
     std::vector<int> lig(J * n_all);
@@ -142,6 +207,8 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     if (e == hipSuccess) e = hipMemcpy(ga->sig.p, sig.data(), sizeof(double) * J, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(ga->ens.p, 0, sizeof(fa::LseState) * J);
     if (e == hipSuccess) e = hipMemset(ga->cnt.p, 0, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess && R > 0) e = hipMemset(ga->dzero.p, 0, sizeof(int) * J);
+    if (e == hipSuccess && R > 0) e = hipMemset(ga->ddstats.p, 0, sizeof(unsigned long long) * 2 * J);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ga->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail_free(fa_fail(FA_ERR_HIP, std::string("GA setup: ") + hipGetErrorString(e)));
     fa::GaDev& d = ga->d;
@@ -185,8 +252,26 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     d.ens = static_cast<fa::LseState*>(ga->ens.p);
     d.n_eval = static_cast<unsigned long long*>(ga->cnt.p);
     d.gen = d.n_eval + 1;
+    d.dd_rounds = R;
+    if (R > 0) {
+        d.n_cand = (int)n_cand;
+        d.cand = J * n_all;
+        d.rtab_log2 = rlog2;
+        d.dzero = static_cast<int*>(ga->dzero.p);
+        d.rkey = static_cast<unsigned long long*>(ga->rkey.p);
+        d.rval = static_cast<int*>(ga->rval.p);
+        d.rzero = static_cast<int*>(ga->rzero.p);
+        d.csig = static_cast<unsigned long long*>(ga->csig.p);
+        d.cdup = static_cast<int*>(ga->cdup.p);
+        d.need = static_cast<int*>(ga->need.p);
+        d.dd_stats = static_cast<unsigned long long*>(ga->ddstats.p);
+        if ((rc = ensure_dset(ga, 16)) != FA_SUCCESS) return fail_free(rc);
+    }
     // initial population of every job: draw, evaluate, fold, rank
-    fa::ga_launch_init(d, ga->stream);
+    if (R > 0)
+        fa::ga_launch_dedup_fill(d, 1, ga->stream);
+    else
+        fa::ga_launch_init(d, ga->stream);
     if ((e = hipGetLastError()) != hipSuccess)
         return fail_free(fa_fail(FA_ERR_HIP, std::string("GA init: ") + hipGetErrorString(e)));
     rc = fa_launch_poses(c, (int)(J * pop), d.gic, Gmax, static_cast<const int*>(ga->lig.p), d.res, nullptr, 0, nullptr,
@@ -232,6 +317,11 @@ int fa_ga_run(fa_ga* ga, int generations, int use_graph, float* ms)
 {
     if (!ga || generations < 0) return fa_fail(FA_ERR_INVALID, "bad GA run");
     FA_HIP(hipSetDevice(ga->ctx->device));
+    if (ga->d.dd_rounds > 0) {
+        const int rc = ensure_dset(ga, ga->gens_run + generations);
+        if (rc != FA_SUCCESS) return rc;
+    }
+    ga->gens_run += generations;
     if (use_graph && !ga->exec) {
         FA_HIP(hipStreamBeginCapture(ga->stream, hipStreamCaptureModeThreadLocal));
         const int rc = ga_generation(ga);
@@ -337,6 +427,19 @@ int fa_ga_fitness(fa_ga* ga, double* fitness, double* sig_share)
     return FA_SUCCESS;
 }
 
+int fa_ga_dedup_stats(fa_ga* ga, uint64_t* stats)
+{
+    if (!ga || !stats) return fa_fail(FA_ERR_INVALID, "null argument");
+    FA_HIP(hipSetDevice(ga->ctx->device));
+    FA_HIP(hipStreamSynchronize(ga->stream));
+    if (ga->d.dd_rounds < 1) {
+        std::fill(stats, stats + 2 * (size_t)ga->jobs, (uint64_t)0);
+        return FA_SUCCESS;
+    }
+    FA_HIP(hipMemcpy(stats, ga->ddstats.p, sizeof(uint64_t) * 2 * ga->jobs, hipMemcpyDeviceToHost));
+    return FA_SUCCESS;
+}
+
 int fa_ga_destroy(fa_ga* ga)
 {
     if (!ga) return FA_SUCCESS;
@@ -346,7 +449,8 @@ int fa_ga_destroy(fa_ga* ga)
     if (ga->graph) (void)hipGraphDestroy(ga->graph);
     for (DevBuf* b : {&ga->rank, &ga->gi, &ga->gic, &ga->res, &ga->gi2, &ga->gic2, &ga->res2, &ga->cum, &ga->lim, &ga->ngenes,
                       &ga->ens, &ga->cnt, &ga->lig, &ga->thermo, &ga->grand, &ga->fit, &ga->fit2, &ga->share,
-                      &ga->sig})
+                      &ga->sig, &ga->dset, &ga->dzero, &ga->rkey, &ga->rval, &ga->rzero, &ga->csig, &ga->cdup,
+                      &ga->need, &ga->ddstats})
         if (b->p) (void)hipFree(b->p);
     if (ga->stream) (void)hipStreamDestroy(ga->stream);
     --ga->ctx->live_users;

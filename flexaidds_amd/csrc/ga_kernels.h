@@ -52,6 +52,25 @@ struct GaDev {
     LseState* ens;                 // [jobs] ensemble of every evaluated pose
     unsigned long long* n_eval;    // poses folded into ensembles
     unsigned long long* gen;       // shared generation counter (gens == null)
+    // Duplicate rejection (dd_rounds > 0).  Candidates are drawn in batches
+    // into candidate rows; a batch's launches set cand_mode, t0 (first draw
+    // stream of the batch) and dd_round.
+    int dd_rounds;                 // batches of candidates per generation at most
+    int dd_round;                  // batch of this launch
+    int cand_mode;                 // 1: init / breed write candidate rows
+    int t0;                        // first draw stream of this batch
+    int n_cand;                    // candidate rows per job
+    size_t cand;                   // first candidate row
+    int dset_log2, rtab_log2;      // table sizes (log2 entries per job)
+    unsigned long long* dset;      // [jobs << dset_log2] signatures of every accepted chromosome (0: empty)
+    int* dzero;                    // [jobs] signature 0 accepted
+    unsigned long long* rkey;      // [jobs << rtab_log2] signatures of this batch (0: empty)
+    int* rval;                     // [jobs << rtab_log2] first candidate of the batch holding each
+    int* rzero;                    // [jobs] first candidate of the batch with signature 0
+    unsigned long long* csig;      // [jobs * n_cand] candidate signatures
+    int* cdup;                     // [jobs * n_cand] signature accepted before this batch
+    int* need;                     // [jobs] rows still to fill
+    unsigned long long* dd_stats;  // [jobs * 2] duplicate candidates drawn, duplicates kept at the cap
 };
 
 // Launches on `s`; each covers the n_act jobs of `g` and is skipped when
@@ -63,5 +82,12 @@ void ga_launch_breed(const GaDev& g, hipStream_t s);             // children of 
 void ga_launch_ensemble(const GaDev& g, int children, hipStream_t s);
 void ga_launch_select(const GaDev& g, int initial, hipStream_t s);  // rank, survivors, fitness
 void ga_launch_thermo(const GaDev& g, int n_slots, double T, fa_thermo* out, hipStream_t s);
+// Initial population (initial = 1) or children with duplicate rejection:
+// up to dd_rounds batches of candidates, each kept in draw order when its
+// signature is new, until the job's rows are filled.
+void ga_launch_dedup_fill(const GaDev& g, int initial, hipStream_t s);
+// Re-inserts the signatures of `old_keys` (jobs << old_log2 entries) into
+// g.dset (cleared beforehand).
+void ga_launch_dset_rehash(const GaDev& g, const unsigned long long* old_keys, int old_log2, hipStream_t s);
 
 }  // namespace fa

@@ -141,7 +141,9 @@ __global__ void ga_tick_kernel(GaDev g)
 }
 
 // initial population of each job into its staged rows, generation 0
-// (LIB/gaboom.cpp:1871-1902); genes past a job's own count are 0
+// (LIB/gaboom.cpp:1871-1902); genes past a job's own count are 0.  In
+// candidate mode (duplicate rejection) the batch's draws go to the job's
+// candidate rows instead, from draw stream t0 + c.
 __global__ void ga_init_kernel(GaDev g)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -149,14 +151,16 @@ __global__ void ga_init_kernel(GaDev g)
     const int a = (int)(t / ((long long)g.pop * g.Gmax));
     const int c = (int)((t / g.Gmax) % g.pop), j = (int)(t % g.Gmax);
     const int k = slot_of(g, a);
-    const size_t at = stage_row(g, a, c) * g.Gmax + j;
+    if (g.cand_mode && g.need[k] == 0) return;
+    const size_t row = g.cand_mode ? g.cand + (size_t)a * g.n_cand + c : stage_row(g, a, c);
+    const size_t at = row * g.Gmax + j;
     if (j >= g.G[k]) {
         g.gi[at] = 0;
         g.gic[at] = 0.0;
         return;
     }
     const size_t l = (size_t)k * g.Gmax + j;
-    const int32_t gene = fa_rng_gene(seed_of(g, k), 0ull, stream_of(g, k, c), (uint64_t)j);
+    const int32_t gene = fa_rng_gene(seed_of(g, k), 0ull, stream_of(g, k, g.t0 + c), (uint64_t)j);
     g.gi[at] = gene;
     g.gic[at] = genetoic(g.lmin[l], g.ldel[l], g.lbin[l], gene);
 }
@@ -166,7 +170,14 @@ __global__ void ga_init_kernel(GaDev g)
 // children rows (lane j: gene j); then one quad per child gene applies that
 // gene's 32 mutation draws and converts it to an internal coordinate.  The
 // draws are the ones a single thread per pair would make (fa_rng.h layout),
-// so the split changes nothing but the parallelism.
+// so the split changes nothing but the parallelism.  In candidate mode
+// (duplicate rejection) pair t of the batch draws from stream t0 + t and
+// both of its children go to the job's candidate rows 2t, 2t + 1.
+__device__ __forceinline__ size_t out_row(const GaDev& g, int a, int i)
+{
+    return g.cand_mode ? g.cand + (size_t)a * g.n_cand + i : child_row(g, a, i);
+}
+
 __global__ __launch_bounds__(256) void ga_pair_kernel(GaDev g)
 {
     // one wave per offspring pair, lane j holding gene j of both children
@@ -176,8 +187,9 @@ __global__ __launch_bounds__(256) void ga_pair_kernel(GaDev g)
     if (tt >= (long long)g.n_act * pairs) return;  // whole waves
     const int a = (int)(tt / pairs), t = (int)(tt % pairs);
     const int k = slot_of(g, a);
+    if (g.cand_mode && g.need[k] == 0) return;     // whole waves: one job per wave
     const uint64_t gen = gen_of(g, k), seed = seed_of(g, k);
-    const uint64_t st = stream_of(g, k, t);
+    const uint64_t st = stream_of(g, k, g.t0 + t);
     const int G = g.G[k], Gm = g.Gmax;
     const double* cum = g.cum + (size_t)k * g.pop;
     const int p1 = roulette_wave(cum, g.pop, fa_rng_u01(seed, gen, st, FA_DRAW_P1), l);
@@ -205,10 +217,10 @@ __global__ __launch_bounds__(256) void ga_pair_kernel(GaDev g)
             ma = nm;
         }
     }
-    const size_t c1 = child_row(g, a, 2 * t);
+    const size_t c1 = out_row(g, a, 2 * t);
     if (l < G) {
         g.gi[c1 * Gm + l] = (int32_t)jo;
-        if (2 * t + 1 < g.n_new) g.gi[(c1 + 1) * Gm + l] = (int32_t)ma;
+        if (g.cand_mode || 2 * t + 1 < g.n_new) g.gi[(c1 + 1) * Gm + l] = (int32_t)ma;
     }
 }
 
@@ -221,14 +233,16 @@ __global__ __launch_bounds__(256) void ga_mutate_kernel(GaDev g)
     const int a = blockIdx.y;
     const int tq = blockIdx.x * blockDim.x + threadIdx.x;
     const int gi = tq >> 2, part = tq & 3;  // child gene of the job, quarter of its bits
-    const bool live = gi < g.n_new * g.Gmax;
+    const int n_out = g.cand_mode ? 2 * ((g.n_new + 1) / 2) : g.n_new;
+    const bool live = gi < n_out * g.Gmax;
     const int i = gi / g.Gmax, j = gi - i * g.Gmax;
     const int k = slot_of(g, a);
+    if (g.cand_mode && g.need[k] == 0) return;  // the whole block: grid y is the job
     const int G = g.G[k];
     unsigned optr = 0u;
     if (live && j < G) {
         const uint64_t gen = gen_of(g, k), seed = seed_of(g, k);
-        const uint64_t st = stream_of(g, k, i >> 1);
+        const uint64_t st = stream_of(g, k, g.t0 + (i >> 1));
         const uint64_t base = FA_DRAW_MUT + ((uint64_t)(i & 1) * G + j) * 32 + 8 * part;
         for (int b = 0; b < 8; ++b)
             if (fa_rng_u01(seed, gen, st, base + b) < g.mut) optr |= 1u << (8 * part + b);
@@ -236,7 +250,7 @@ __global__ __launch_bounds__(256) void ga_mutate_kernel(GaDev g)
     optr |= (unsigned)__shfl_xor((int)optr, 1);
     optr |= (unsigned)__shfl_xor((int)optr, 2);
     if (!live || part != 0) return;
-    const size_t at = child_row(g, a, i) * g.Gmax + j;
+    const size_t at = out_row(g, a, i) * g.Gmax + j;
     if (j >= G) {
         g.gi[at] = 0;
         g.gic[at] = 0.0;
@@ -555,6 +569,227 @@ __global__ void ga_thermo_kernel(GaDev g, int n, double T, fa_thermo* out)
     if (k < n) out[k] = thermo_of(g.ens[k], T);
 }
 
+// ------------------------------------------------------------ duplicates
+// Duplicate rejection (GB->duplicates = 0, the reference's default;
+// LIB/gaboom.cpp:971-1018 for children, :1966-1978 for the initial
+// population).  The reference draws candidates one at a time and keeps a
+// candidate whose signature (hash_genes, :1040-1045) is not yet in the job's
+// set, adding it, until the rows are filled; the set lives for the whole run.
+// Here a batch of candidates is drawn at once and kept in draw order: a
+// candidate is new when its signature is in neither the set nor an earlier
+// candidate of the batch; the first `need` new ones fill the rows and enter
+// the set, exactly the candidates the one-at-a-time loop keeps.  After
+// dd_rounds batches the rows still empty take the last batch's rejected
+// candidates in draw order (the reference would keep drawing).
+
+// hash_genes: h ^= hash(int32(ic + 0.5)) + 0x9e3779b9 + (h << 6) + (h >> 2)
+// over the genes, std::hash<int32_t> being the value as a size_t
+// (sign-extended); the same value, so signatures collide where the
+// reference's do
+__device__ __forceinline__ unsigned long long genes_sig(const double* ic, int G)
+{
+    unsigned long long h = 0ull;
+    for (int j = 0; j < G; ++j) {
+        const int32_t v = (int32_t)(ic[j] + 0.5);
+        h ^= (unsigned long long)(long long)v + 0x9e3779b9ull + (h << 6) + (h >> 2);
+    }
+    return h;
+}
+
+// table slot of a signature (splitmix64 finalizer: the signatures of nearby
+// angle vectors differ in few bits)
+__device__ __forceinline__ size_t sig_slot(unsigned long long x, int log2)
+{
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return (size_t)(x & ((1ull << log2) - 1ull));
+}
+
+// open addressing, linear probing, key 0 = empty (signature 0 is kept
+// beside the table); tables stay at most half full, so probes end early
+__device__ bool tab_has(const unsigned long long* t, int log2, unsigned long long key)
+{
+    const size_t mask = ((size_t)1 << log2) - 1;
+    size_t h = sig_slot(key, log2);
+    for (size_t p = 0; p <= mask; ++p) {
+        const unsigned long long v = t[h];
+        if (v == key) return true;
+        if (v == 0ull) return false;
+        h = (h + 1) & mask;
+    }
+    return false;
+}
+
+__device__ long long tab_insert(unsigned long long* t, int log2, unsigned long long key)
+{
+    const size_t mask = ((size_t)1 << log2) - 1;
+    size_t h = sig_slot(key, log2);
+    for (size_t p = 0; p <= mask; ++p) {
+        const unsigned long long prev = atomicCAS(&t[h], 0ull, key);
+        if (prev == 0ull || prev == key) return (long long)h;
+        h = (h + 1) & mask;
+    }
+    return -1;
+}
+
+__device__ long long tab_find(const unsigned long long* t, int log2, unsigned long long key)
+{
+    const size_t mask = ((size_t)1 << log2) - 1;
+    size_t h = sig_slot(key, log2);
+    for (size_t p = 0; p <= mask; ++p) {
+        const unsigned long long v = t[h];
+        if (v == key) return (long long)h;
+        if (v == 0ull) return -1;
+        h = (h + 1) & mask;
+    }
+    return -1;
+}
+
+__global__ void ga_dd_start_kernel(GaDev g, int initial)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < g.n_act) g.need[slot_of(g, a)] = initial ? g.pop : g.n_new;
+}
+
+// clears each job's batch table (jobs with rows to fill); grid y is the job
+__global__ void ga_dd_clear_kernel(GaDev g)
+{
+    const int a = blockIdx.y, k = slot_of(g, a);
+    if (g.need[k] == 0) return;
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < ((size_t)1 << g.rtab_log2)) {
+        g.rkey[((size_t)k << g.rtab_log2) + j] = 0ull;
+        g.rval[((size_t)k << g.rtab_log2) + j] = 0x7fffffff;
+    }
+    if (j == 0) g.rzero[k] = 0x7fffffff;
+}
+
+// signature of every candidate of the batch; whether the set holds it; the
+// first candidate of the batch holding each new signature
+__global__ __launch_bounds__(256) void ga_dd_sig_kernel(GaDev g, int n_round)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.n_act * n_round) return;
+    const int a = (int)(t / n_round), i = (int)(t % n_round);
+    const int k = slot_of(g, a);
+    if (g.need[k] == 0) return;
+    const size_t row = g.cand + (size_t)a * g.n_cand + i;
+    const unsigned long long sig = genes_sig(g.gic + row * g.Gmax, g.G[k]);
+    const size_t c = (size_t)a * g.n_cand + i;
+    g.csig[c] = sig;
+    const bool dup = sig == 0ull ? g.dzero[k] != 0 : tab_has(g.dset + ((size_t)k << g.dset_log2), g.dset_log2, sig);
+    g.cdup[c] = dup ? 1 : 0;
+    if (dup) return;
+    if (sig == 0ull) {
+        atomicMin(&g.rzero[k], i);
+    } else {
+        const long long h = tab_insert(g.rkey + ((size_t)k << g.rtab_log2), g.rtab_log2, sig);
+        if (h >= 0) atomicMin(&g.rval[((size_t)k << g.rtab_log2) + h], i);
+    }
+}
+
+// exclusive rank of p among the block's 256 threads (four waves), and the count
+__device__ __forceinline__ int block_rank(bool p, int& total, int* wsum)
+{
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(p);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int before = __popcll(m & ((1ull << l) - 1ull));
+    if (l == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int q = 0; q < 4; ++q) {
+        if (q < w) off += wsum[q];
+        tot += wsum[q];
+    }
+    __syncthreads();
+    total = tot;
+    return off + before;
+}
+
+__device__ __forceinline__ void copy_row(const GaDev& g, size_t from, size_t to)
+{
+    for (int j = 0; j < g.Gmax; ++j) {
+        g.gi[to * g.Gmax + j] = g.gi[from * g.Gmax + j];
+        g.gic[to * g.Gmax + j] = g.gic[from * g.Gmax + j];
+    }
+}
+
+// one block of 256 per job: the batch's new candidates in draw order fill
+// the job's remaining rows and enter its set; on the last batch the rows
+// still empty take the batch's rejected candidates in draw order
+__global__ __launch_bounds__(256) void ga_dd_accept_kernel(GaDev g, int n_round, int initial, int last)
+{
+    __shared__ int wsum[4];
+    __shared__ int s_last;
+    const int a = blockIdx.x, k = slot_of(g, a);
+    const int need0 = g.need[k];
+    if (need0 <= 0) return;
+    const int rows = initial ? g.pop : g.n_new;
+    const size_t c0 = (size_t)a * g.n_cand;
+    const unsigned long long* rkey = g.rkey + ((size_t)k << g.rtab_log2);
+    const int* rval = g.rval + ((size_t)k << g.rtab_log2);
+    auto is_new = [&](int i) {
+        if (i >= n_round || g.cdup[c0 + i]) return false;
+        const unsigned long long sig = g.csig[c0 + i];
+        if (sig == 0ull) return g.rzero[k] == i;
+        const long long h = tab_find(rkey, g.rtab_log2, sig);
+        return h >= 0 && rval[h] == i;
+    };
+    auto dest = [&](int r) { return initial ? stage_row(g, a, r) : child_row(g, a, r); };
+    if (threadIdx.x == 0) s_last = n_round - 1;
+    int base = 0;
+    for (int s0 = 0; s0 < n_round && base < need0; s0 += 256) {
+        const int i = s0 + (int)threadIdx.x;
+        const bool acc = is_new(i);
+        int total;
+        const int r = block_rank(acc, total, wsum);
+        if (acc && base + r < need0) {
+            copy_row(g, g.cand + c0 + i, dest(rows - need0 + base + r));
+            const unsigned long long sig = g.csig[c0 + i];
+            if (sig == 0ull)
+                g.dzero[k] = 1;
+            else
+                (void)tab_insert(g.dset + ((size_t)k << g.dset_log2), g.dset_log2, sig);
+            if (base + r == need0 - 1) s_last = i;   // the candidate that fills the last row
+        }
+        base += total;
+    }
+    __syncthreads();
+    const int kept = min(base, need0);
+    const int examined = (base >= need0) ? s_last + 1 : n_round;
+    int filled = 0;
+    if (last && kept < need0) {
+        int fb = 0;
+        for (int s0 = 0; s0 < n_round && fb < need0 - kept; s0 += 256) {
+            const int i = s0 + (int)threadIdx.x;
+            const bool rej = i < n_round && !is_new(i);
+            int total;
+            const int r = block_rank(rej, total, wsum);
+            if (rej && fb + r < need0 - kept) copy_row(g, g.cand + c0 + i, dest(rows - need0 + kept + fb + r));
+            fb += total;
+        }
+        filled = min(fb, need0 - kept);
+    }
+    if (threadIdx.x == 0) {
+        g.need[k] = need0 - kept - filled;
+        g.dd_stats[2 * (size_t)k] += (unsigned long long)(examined - kept);
+        g.dd_stats[2 * (size_t)k + 1] += (unsigned long long)filled;
+    }
+}
+
+__global__ void ga_dset_rehash_kernel(GaDev g, const unsigned long long* old, int old_log2)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ((size_t)g.jobs << old_log2)) return;
+    const unsigned long long key = old[t];
+    if (key == 0ull) return;
+    const size_t k = t >> old_log2;
+    (void)tab_insert(g.dset + (k << g.dset_log2), g.dset_log2, key);
+}
+
 namespace {
 unsigned blocks_of(long long n) { return (unsigned)std::max(1LL, (n + 255) / 256); }
 }  // namespace
@@ -577,8 +812,38 @@ void ga_launch_breed(const GaDev& g, hipStream_t s)
     // four pairs per 256-thread block (a wave each); 64 child genes per
     // block (a quad each), one row of blocks per job
     const long long pairs = (long long)g.n_act * ((g.n_new + 1) / 2);
+    const long long n_out = g.cand_mode ? 2ll * ((g.n_new + 1) / 2) : g.n_new;
     hipLaunchKernelGGL(ga_pair_kernel, dim3((unsigned)std::max(1LL, (pairs + 3) / 4)), dim3(256), 0, s, g);
-    hipLaunchKernelGGL(ga_mutate_kernel, dim3(blocks_of(4ll * g.n_new * g.Gmax), g.n_act), dim3(256), 0, s, g);
+    hipLaunchKernelGGL(ga_mutate_kernel, dim3(blocks_of(4ll * n_out * g.Gmax), g.n_act), dim3(256), 0, s, g);
+}
+
+void ga_launch_dedup_fill(const GaDev& g, int initial, hipStream_t s)
+{
+    if (g.n_act < 1 || g.dd_rounds < 1) return;
+    const int P = (g.n_new + 1) / 2;
+    const int n_round = initial ? g.pop : 2 * P;
+    hipLaunchKernelGGL(ga_dd_start_kernel, dim3(blocks_of(g.n_act)), dim3(256), 0, s, g, initial);
+    for (int r = 0; r < g.dd_rounds; ++r) {
+        GaDev b = g;
+        b.cand_mode = 1;
+        b.dd_round = r;
+        b.t0 = r * (initial ? g.pop : P);
+        hipLaunchKernelGGL(ga_dd_clear_kernel, dim3(blocks_of(1ll << g.rtab_log2), g.n_act), dim3(256), 0, s, b);
+        if (initial)
+            ga_launch_init(b, s);
+        else
+            ga_launch_breed(b, s);
+        hipLaunchKernelGGL(ga_dd_sig_kernel, dim3(blocks_of((long long)g.n_act * n_round)), dim3(256), 0, s, b,
+                           n_round);
+        hipLaunchKernelGGL(ga_dd_accept_kernel, dim3(g.n_act), dim3(256), 0, s, b, n_round, initial,
+                           r == g.dd_rounds - 1 ? 1 : 0);
+    }
+}
+
+void ga_launch_dset_rehash(const GaDev& g, const unsigned long long* old_keys, int old_log2, hipStream_t s)
+{
+    hipLaunchKernelGGL(ga_dset_rehash_kernel, dim3(blocks_of((long long)g.jobs << old_log2)), dim3(256), 0, s, g,
+                       old_keys, old_log2);
 }
 
 void ga_launch_ensemble(const GaDev& g, int children, hipStream_t s)

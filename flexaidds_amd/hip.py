@@ -81,6 +81,7 @@ _SIGS = {
     "fa_ga_create_batch": ([_P, _P, C.c_int, _P, _P, _P], C.c_int),
     "fa_ga_shape": ([_P, _P, _P], C.c_int),
     "fa_ga_fitness": ([_P, _P, _P], C.c_int),
+    "fa_ga_dedup_stats": ([_P, _P], C.c_int),
     "fa_ga_grand": ([_P, _P, _P, _P, _P], C.c_int),
     "fa_grand_device": ([_P, _P, C.c_int, _P, C.c_int, _P, _P], C.c_int),
     "fa_ga_run": ([_P, C.c_int, C.c_int, _P], C.c_int),
@@ -303,7 +304,7 @@ class GeneLim(C.Structure):
 class GAParams(C.Structure):
     _fields_ = [("ligand", C.c_int32), ("population", C.c_int32), ("offspring", C.c_int32),
                 ("intragenes", C.c_int32), ("cross_rate", C.c_double), ("mut_rate", C.c_double),
-                ("temperature", C.c_double), ("seed", C.c_uint64), ("fitness", C.c_int32), ("pad", C.c_int32),
+                ("temperature", C.c_double), ("seed", C.c_uint64), ("fitness", C.c_int32), ("dedup_rounds", C.c_int32),
                 ("share_alpha", C.c_double), ("share_peaks", C.c_double), ("share_scale", C.c_double),
                 ("sig_share", C.c_double), ("entropy_weight", C.c_double)]
 
@@ -320,7 +321,8 @@ class GA:
 
     def __init__(self, ctx: Context, limits, ligand=0, population=2000, offspring=None, cross_rate=0.8,
                  mut_rate=0.03, temperature=300.0, seed=1, intragenes=False, ligands=None, fitness="LINEAR",
-                 share_alpha=1.0, share_peaks=5.0, share_scale=10.0, sig_share=0.0, entropy_weight=0.5):
+                 share_alpha=1.0, share_peaks=5.0, share_scale=10.0, sig_share=0.0, entropy_weight=0.5,
+                 dedup_rounds=0):
         self.ctx = ctx
         per_job = [limits] if ligands is None else list(limits)
         flat = [x for job in per_job for x in job]
@@ -328,8 +330,8 @@ class GA:
                                              for a, b, d, m in flat])
         model = FITNESS[fitness] if isinstance(fitness, str) else int(fitness)
         self.params = GAParams(ligand, population, population if offspring is None else offspring,
-                               1 if intragenes else 0, cross_rate, mut_rate, temperature, seed, model, 0,
-                               share_alpha, share_peaks, share_scale, sig_share, entropy_weight)
+                               1 if intragenes else 0, cross_rate, mut_rate, temperature, seed, model,
+                               int(dedup_rounds), share_alpha, share_peaks, share_scale, sig_share, entropy_weight)
         self.job_genes = [len(j) for j in per_job]
         self._ga = C.c_void_p()
         if ligands is None:
@@ -399,6 +401,13 @@ class GA:
         g = C.c_uint64()
         _check(_lib.fa_ga_generation(self._ga, C.byref(g)))
         return int(g.value)
+
+    def dedup_stats(self) -> np.ndarray:
+        """[jobs, 2]: duplicate candidates drawn, rows that kept a duplicate
+        at the batch cap (fa_ga_dedup_stats)."""
+        out = np.zeros((self.jobs, 2), np.uint64)
+        _check(_lib.fa_ga_dedup_stats(self._ga, out.ctypes.data))
+        return out
 
     def close(self):
         if self._ga:

@@ -197,7 +197,8 @@ int fa_last_pose_cycles(fa_ctx* ctx, int n, uint64_t* cycles);
  * offspring, survivors = best `population` of parents + offspring by energy
  * (QuickSort by evalue; ties keep index order), rank fitness, and the
  * ensemble reduction over every evaluated pose at `temperature`.
- * Random draws come from include/fa_rng.h; duplicate rejection is not done.
+ * Random draws come from include/fa_rng.h; duplicate rejection is an option
+ * (fa_ga_params.dedup_rounds).
  * ------------------------------------------------------------------- */
 typedef struct fa_gene_lim {   /* LIB/gaboom.h:46-54 (genlim) */
     double min, max, del;
@@ -222,7 +223,12 @@ typedef struct fa_ga_params {
     double temperature;        /* K, ensemble reduction and SMFREE weights */
     uint64_t seed;
     int32_t fitness;           /* fa_fitness_model */
-    int32_t pad;
+    int32_t dedup_rounds;      /* 0: duplicate chromosomes allowed (GB->duplicates = 1).  R > 0:
+                                  rejected as GB->duplicates = 0 (the reference's default) does,
+                                  LIB/gaboom.cpp:971-1018, :1966-1978: a child or initial chromosome
+                                  whose signature (hash_genes, :1040-1045) was accepted before in the
+                                  job is drawn again; at most R batches of candidates per generation,
+                                  then the remaining rows keep duplicates (fa_ga_dedup_stats) */
     double share_alpha;        /* sharing_alpha (reference default 1) */
     double share_peaks;        /* sharing_peaks (5) */
     double share_scale;        /* sharing_scale (10) */
@@ -271,6 +277,11 @@ int fa_ga_generation(fa_ga* ga, uint64_t* generation);
  * fa_ga_population) as the next generation's roulette uses it, and each
  * job's niche radius (sig_share, [n_jobs]); either may be null. */
 int fa_ga_fitness(fa_ga* ga, double* fitness, double* sig_share);
+/* Duplicate rejection counts per job since creation: stats[2k] candidates
+ * drawn whose signature the job had already accepted, stats[2k + 1] rows
+ * that kept a duplicate because dedup_rounds batches did not fill them (the
+ * reference draws until they are filled).  Zeros when dedup_rounds is 0. */
+int fa_ga_dedup_stats(fa_ga* ga, uint64_t* stats);
 int fa_ga_destroy(fa_ga* ga);
 
 /* ---------------------------------------------------------------------

@@ -11,7 +11,11 @@
  *   roullete_wheel    LIB/gaboom.cpp:1158-1182 (sequential scan, as written)
  *   crossover         LIB/gaboom.cpp:2517-2618
  *   mutate            LIB/gaboom.cpp:2620-2641
- *   reproduce (BOOM)  LIB/gaboom.cpp:886-973, without duplicate rejection
+ *   reproduce (BOOM)  LIB/gaboom.cpp:886-973 (fa_oracle_ga_breed_fit)
+ *   duplicate rejection  LIB/gaboom.cpp:971-1018 (children) and :1966-1978
+ *                     (initial population) with hash_genes, :1040-1045:
+ *                     candidates drawn one at a time, kept when their
+ *                     signature is new to the job (fa_oracle_ga_*_dedup)
  *   LINEAR fitness    LIB/gaboom.cpp:1627-1635
  *   PSHARE fitness    LIB/gaboom.cpp:1637-1664 (calc_rmsp, :2874-2885)
  *   SMFREE fitness    LIB/gaboom.cpp:1666-1747 (boltzmann_weights,
@@ -24,6 +28,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <numeric>
+#include <unordered_set>
 #include <vector>
 
 #include "fa_rng.h"
@@ -184,6 +189,108 @@ void fa_oracle_ga_breed_fit(uint64_t seed, uint64_t job, uint64_t gen, int pop, 
             }
         }
     }
+}
+
+/* hash_genes (LIB/gaboom.cpp:1040-1045) over internal coordinates: the
+ * boost-style combine of std::hash<int32_t>(int32(ic + 0.5)), the hash of an
+ * int being its value as a size_t */
+uint64_t fa_oracle_genes_sig(const double* ic, int G)
+{
+    std::size_t h = 0;
+    for (int i = 0; i < G; ++i)
+        h ^= std::hash<int32_t>{}(static_cast<int32_t>(ic[i] + 0.5)) + 0x9e3779b9 + (h << 6) + (h >> 2);
+    return (uint64_t)h;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The reference's loop: candidates in draw order, each kept when its
+// signature is not yet in `seen` (then added), until `rows` are kept.  The
+// device draws at most `rounds` batches of `batch` candidates; when they do
+// not fill the rows, the rest take the last batch's rejected candidates in
+// draw order.  draw(a, genes) makes candidate a.  stats += {rejected, filled}.
+template <class Draw>
+void dedup_fill(int rows, int batch, int rounds, int G, const fa_gene_lim* lim, const uint64_t* seen_in, int64_t n_seen,
+                Draw draw, int32_t* out_genes, double* out_ic, uint64_t* stats)
+{
+    std::unordered_set<uint64_t> seen(seen_in, seen_in + n_seen);
+    std::vector<int32_t> g((std::size_t)batch * rounds * G);
+    std::vector<double> ic(g.size());
+    std::vector<char> kept((std::size_t)batch * rounds, 0);
+    int n = 0;
+    long long a = 0;
+    const long long total = (long long)batch * rounds;
+    for (; a < total && n < rows; ++a) {
+        int32_t* ga = &g[(std::size_t)a * G];
+        double* ia = &ic[(std::size_t)a * G];
+        draw(a, ga);
+        for (int j = 0; j < G; ++j) ia[j] = genetoic(lim[j], 1.0 / nbins(lim[j]), ga[j]);
+        const uint64_t sig = fa_oracle_genes_sig(ia, G);
+        if (seen.count(sig)) continue;
+        seen.insert(sig);
+        kept[a] = 1;
+        std::copy(ga, ga + G, out_genes + (std::size_t)n * G);
+        std::copy(ia, ia + G, out_ic + (std::size_t)n * G);
+        ++n;
+    }
+    const int kept_n = n;
+    if (stats) stats[0] += (uint64_t)(a - kept_n);
+    for (long long b = total - batch; n < rows && b < total; ++b) {   // the last batch's rejected candidates
+        if (kept[b]) continue;
+        std::copy(&g[(std::size_t)b * G], &g[(std::size_t)b * G] + G, out_genes + (std::size_t)n * G);
+        std::copy(&ic[(std::size_t)b * G], &ic[(std::size_t)b * G] + G, out_ic + (std::size_t)n * G);
+        ++n;
+    }
+    if (stats) stats[1] += (uint64_t)(n - kept_n);
+}
+
+}  // namespace
+
+extern "C" {
+
+/* initial population with duplicate rejection: candidate a draws from stream
+ * (job << 32) | a; batches of pop candidates */
+void fa_oracle_ga_init_dedup(uint64_t seed, uint64_t job, int pop, int G, const fa_gene_lim* lim, int rounds,
+                             const uint64_t* seen, int64_t n_seen, int32_t* genes, double* ic, uint64_t* stats)
+{
+    auto draw = [&](long long a, int32_t* out) {
+        for (int j = 0; j < G; ++j) out[j] = fa_rng_gene(seed, 0, (job << 32) | (uint64_t)a, (uint64_t)j);
+    };
+    dedup_fill(pop, pop, rounds, G, lim, seen, n_seen, draw, genes, ic, stats);
+}
+
+/* children with duplicate rejection: candidate a is child a & 1 of pair a >> 1
+ * (stream (job << 32) | pair), as fa_oracle_ga_breed_fit makes it; batches of
+ * 2 * ceil(n_new / 2) candidates */
+void fa_oracle_ga_breed_dedup(uint64_t seed, uint64_t job, uint64_t gen, int pop, int n_new, int G, int intragenes,
+                              double cross_rate, double mut_rate, const fa_gene_lim* lim, const double* fit_in,
+                              const int32_t* parents, int rounds, const uint64_t* seen, int64_t n_seen,
+                              int32_t* child_genes, double* child_ic, uint64_t* stats)
+{
+    std::vector<double> fit(fit_in, fit_in + pop);
+    std::vector<int32_t> john(G), mary(G);
+    long long pair = -1;
+    auto draw = [&](long long a, int32_t* out) {
+        if ((a >> 1) != pair) {
+            pair = a >> 1;
+            const uint64_t st = (job << 32) | (uint64_t)pair;
+            const int p1 = roulette(fit.data(), pop, fa_rng_u01(seed, gen, st, FA_DRAW_P1));
+            const int p2 = roulette(fit.data(), pop, fa_rng_u01(seed, gen, st, FA_DRAW_P2));
+            for (int j = 0; j < G; ++j) {
+                john[j] = parents[(std::size_t)p1 * G + j];
+                mary[j] = parents[(std::size_t)p2 * G + j];
+            }
+            if (fa_rng_u01(seed, gen, st, FA_DRAW_CROSS) < cross_rate)
+                crossover(john.data(), mary.data(), G, intragenes, seed, gen, st);
+            mutate(john.data(), G, mut_rate, seed, gen, st, 0);
+            mutate(mary.data(), G, mut_rate, seed, gen, st, 1);
+        }
+        const int32_t* src = (a & 1) ? mary.data() : john.data();
+        std::copy(src, src + G, out);
+    };
+    dedup_fill(n_new, 2 * ((n_new + 1) / 2), rounds, G, lim, seen, n_seen, draw, child_genes, child_ic, stats);
 }
 
 /* as fa_oracle_ga_breed_fit with LINEAR fitness (parents ranked best first) */

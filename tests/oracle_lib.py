@@ -266,7 +266,50 @@ def ga_bind(lib):
                                          C.c_double, _P]
     lib.fa_oracle_sig_share.argtypes = [_P, C.c_int, C.c_double, C.c_double]
     lib.fa_oracle_sig_share.restype = C.c_double
+    lib.fa_oracle_genes_sig.argtypes = [_P, C.c_int]
+    lib.fa_oracle_genes_sig.restype = C.c_uint64
+    lib.fa_oracle_ga_init_dedup.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int64,
+                                            _P, _P, _P]
+    lib.fa_oracle_ga_breed_dedup.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.c_double, C.c_double, _P, _P, _P, C.c_int, _P, C.c_int64, _P, _P, _P]
     return lib
+
+
+def genes_sig(lib, ic):
+    ic = np.ascontiguousarray(ic, np.float64)
+    return int(lib.fa_oracle_genes_sig(ic.ctypes.data, ic.shape[-1]))
+
+
+def _seen(seen):
+    return np.ascontiguousarray(np.array(sorted(seen), dtype=np.uint64) if len(seen) else np.zeros(1, np.uint64))
+
+
+def ga_init_dedup(lib, seed, pop, limits, rounds, seen=(), job=0):
+    """(genes, ic, [rejected, filled]) of an initial population drawn with
+    duplicate rejection against the signatures `seen`."""
+    G = len(limits)
+    genes = np.zeros((pop, G), np.int32)
+    ic = np.zeros((pop, G))
+    st = np.zeros(2, np.uint64)
+    sv = _seen(seen)
+    lib.fa_oracle_ga_init_dedup(seed, job, pop, G, gene_lims(limits), rounds, sv.ctypes.data, len(seen),
+                                genes.ctypes.data, ic.ctypes.data, st.ctypes.data)
+    return genes, ic, st
+
+
+def ga_breed_dedup(lib, seed, gen, parents, fit, n_new, limits, rounds, seen=(), cross=0.8, mut=0.03, intragenes=0,
+                   job=0):
+    parents = np.ascontiguousarray(parents, np.int32)
+    fit = np.ascontiguousarray(fit, np.float64)
+    pop, G = parents.shape
+    child = np.zeros((n_new, G), np.int32)
+    ic = np.zeros((n_new, G))
+    st = np.zeros(2, np.uint64)
+    sv = _seen(seen)
+    lib.fa_oracle_ga_breed_dedup(seed, job, gen, pop, n_new, G, intragenes, cross, mut, gene_lims(limits),
+                                 fit.ctypes.data, parents.ctypes.data, rounds, sv.ctypes.data, len(seen),
+                                 child.ctypes.data, ic.ctypes.data, st.ctypes.data)
+    return child, ic, st
 
 
 def ga_breed_fit(lib, seed, gen, parents, fit, n_new, limits, cross=0.8, mut=0.03, intragenes=0, job=0):

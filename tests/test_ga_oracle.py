@@ -208,3 +208,94 @@ def test_crossover_swaps_one_contiguous_segment(ga):
             assert on[-1] - on[0] + 1 == on.size, (t, G, bits)
             runs += 1
     assert runs > 300
+
+
+# ------------------------------------------------------------ duplicate rejection
+def _rounded(ic):
+    # the duplicate relation the reference's signature encodes: every gene's
+    # internal coordinate as int32(ic + 0.5) (truncation toward zero)
+    return tuple(int(np.trunc(v + 0.5)) for v in ic)
+
+
+def _first_new(cands_ic, rows, seen_tuples):
+    # the reference's loop over candidates in draw order, on rounded tuples
+    # rather than signatures (equal barring 64-bit collisions)
+    seen = set(seen_tuples)
+    kept = []
+    for a, ic in enumerate(cands_ic):
+        t = _rounded(ic)
+        if t in seen:
+            continue
+        seen.add(t)
+        kept.append(a)
+        if len(kept) == rows:
+            break
+    return kept
+
+
+def test_genes_sig_closed_forms(gao):
+    C9 = 0x9E3779B9
+    M = (1 << 64) - 1
+    # one gene: the int's hash plus the constant; rounding is int32(v + 0.5),
+    # truncating toward zero (-3.7 -> -3, not -4), and the int's hash
+    # sign-extends (-3 -> 2^64 - 3, not 2^32 - 3)
+    assert ol.genes_sig(gao, np.array([3.7])) == 4 + C9
+    assert ol.genes_sig(gao, np.array([-3.7])) == ((1 << 64) - 3 + C9) & M
+    assert ol.genes_sig(gao, np.array([-0.6])) == C9                     # int32(-0.1) = 0
+    # two genes: the second term mixes in the first's hash shifted both ways
+    h1 = 4 + C9
+    assert ol.genes_sig(gao, np.array([3.7, 1.2])) == h1 ^ ((1 + C9 + (h1 << 6) + (h1 >> 2)) & M)
+    # order matters; equal rounded vectors collide
+    assert ol.genes_sig(gao, np.array([1.2, 3.7])) != ol.genes_sig(gao, np.array([3.7, 1.2]))
+    assert ol.genes_sig(gao, np.array([3.6, 1.4, -6.8])) == ol.genes_sig(gao, np.array([3.9, 0.5, -7.4]))
+
+
+def test_init_dedup_keeps_first_new_draws(gao):
+    # candidate a of the initial population is draw stream a; with rejection
+    # the kept rows are the first `pop` candidates whose rounded genes are new
+    G, pop, R = 4, 40, 6
+    lims = [(0.0, 3.0, 1.0, 0)] * G                      # 3 bins a gene: 81 vectors, frequent repeats
+    cands, cic = ol.ga_init(gao, 5, pop * R, lims)       # every candidate the R batches can draw
+    genes, ic, st = ol.ga_init_dedup(gao, 5, pop, lims, R)
+    kept = _first_new(cic, pop, [])
+    assert len(kept) == pop and kept[-1] >= pop          # some draws were rejected
+    assert np.array_equal(genes, cands[kept]) and np.array_equal(ic, cic[kept])
+    assert int(st[0]) == kept[-1] + 1 - pop and int(st[1]) == 0
+    # no repeats among the rows; an empty rejection list reproduces the plain draws
+    assert len({_rounded(r) for r in ic}) == pop
+    wide = [(-180.0, 180.0, 1.0, 0)] * 6
+    g0, i0 = ol.ga_init(gao, 5, pop, wide)
+    g1, i1, s1 = ol.ga_init_dedup(gao, 5, pop, wide, 1)
+    assert np.array_equal(g0, g1) and np.array_equal(i0, i1) and int(s1[0]) == 0
+
+
+def test_init_dedup_cap_fills_from_last_batch(gao):
+    # 2 genes of 2 bins: 4 distinct vectors for 10 rows; after R batches the
+    # rest are the last batch's rejected candidates in draw order
+    G, pop, R = 2, 10, 3
+    lims = [(0.0, 2.0, 1.0, 0)] * G
+    cands, cic = ol.ga_init(gao, 8, pop * R, lims)
+    genes, ic, st = ol.ga_init_dedup(gao, 8, pop, lims, R)
+    kept = _first_new(cic, pop, [])
+    assert len(kept) == 4
+    last = [a for a in range(pop * (R - 1), pop * R) if a not in kept][:pop - 4]
+    assert np.array_equal(genes, cands[kept + last])
+    assert int(st[1]) == pop - 4 and int(st[0]) == pop * R - 4
+
+
+def test_breed_dedup_against_seen(gao):
+    # children: candidate a is child a & 1 of pair a >> 1; signatures accepted
+    # earlier (the parents here) and earlier candidates are rejected
+    G, pop, n_new, R = 4, 30, 25, 8
+    lims = [(0.0, 4.0, 1.0, 0)] * G
+    parents, pic = ol.ga_init(gao, 2, pop, lims)
+    fit = (pop - np.arange(pop)).astype(float)
+    P = (n_new + 1) // 2
+    cands, cic = ol.ga_breed_fit(gao, 2, 3, parents, fit, 2 * P * R, lims, mut=0.05)
+    seen = {ol.genes_sig(gao, r) for r in pic}
+    child, ic, st = ol.ga_breed_dedup(gao, 2, 3, parents, fit, n_new, lims, R, seen, mut=0.05)
+    kept = _first_new(cic, n_new, [_rounded(r) for r in pic])
+    assert len(kept) == n_new and int(st[1]) == 0
+    assert np.array_equal(child, cands[kept])
+    assert not {_rounded(r) for r in ic} & {_rounded(r) for r in pic}
+    assert int(st[0]) == kept[-1] + 1 - n_new > 0
