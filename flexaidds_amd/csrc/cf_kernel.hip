@@ -2987,6 +2987,41 @@ hipError_t launch_cf(const DevReceptor& R, const DevGrid* grids, const DevLigand
     return hipGetLastError();
 }
 
+// Ligand coordinates of poses without scoring them: the pose kernel's build
+// (LIB/ic2cf.cpp:53-105, LIB/buildcc.cpp) of every pose, one wave per pose,
+// written component-major (xyzT[(3 * atom + k) * ld + pose]) so that a pass
+// over many poses reads one coordinate of 64 poses per load.  The input of
+// the pose clustering (the coordinates calc_rmsd_chrom rebuilds,
+// LIB/calc_rmsd_chrom.cpp).
+__global__ __launch_bounds__(WAVE) void pose_coords_kernel(DevReceptor R, const DevLigand* __restrict__ ligs,
+                                                           int ligand, const double* __restrict__ genes, int stride,
+                                                           int n, float* __restrict__ xyzT, int ld)
+{
+    __shared__ Smem s;
+    const int l = lane_id();
+    const DevLigand lig = ligs[ligand];
+    const int ns = R.n_sc;
+    for (int pose = blockIdx.x; pose < n; pose += gridDim.x) {
+        build_pose(s, R, lig, genes + (size_t)pose * stride);
+        for (int i = l; i < lig.n_atoms; i += WAVE) {
+            xyzT[(size_t)(3 * i) * ld + pose] = s.lx[ns + i];
+            xyzT[(size_t)(3 * i + 1) * ld + pose] = s.ly[ns + i];
+            xyzT[(size_t)(3 * i + 2) * ld + pose] = s.lz[ns + i];
+        }
+        wave_sync();
+    }
+}
+
+hipError_t launch_coords(const DevReceptor& R, const DevLigand* ligs, int ligand, const double* genes, int stride,
+                         int n, float* xyzT, int ld, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const int blocks = n < 8192 ? n : 8192;
+    hipLaunchKernelGGL(pose_coords_kernel, dim3(blocks), dim3(WAVE), 0, stream, R, ligs, ligand, genes, stride, n,
+                       xyzT, ld);
+    return hipGetLastError();
+}
+
 int smem_bytes() { return (int)sizeof(Smem); }
 
 // resident waves per CU for the pose kernel (LDS and registers), 0 on error

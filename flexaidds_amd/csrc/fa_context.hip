@@ -283,7 +283,7 @@ int fa_ctx_destroy(fa_ctx* c)
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     c->free_system();
-    for (DevBuf* b : {&c->genes, &c->out, &c->counters, &c->recs, &c->reccnt, &c->xyz, &c->scratch, &c->thermo,
+    for (DevBuf* b : {&c->genes, &c->out, &c->counters, &c->recs, &c->reccnt, &c->xyz, &c->scratch, &c->cluster, &c->thermo,
                       &c->ops, &c->prof, &c->near_idx, &c->near_xyz, &c->rbox, &c->totals})
         if (b->p) (void)hipFree(b->p);
     delete c;

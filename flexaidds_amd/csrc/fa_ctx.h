@@ -18,6 +18,8 @@ hipError_t launch_cf(const DevReceptor& R, const DevGrid* grids, const DevLigand
                      const DevBatch& B, int n_blocks, hipStream_t stream);
 int cf_waves_per_cu();
 int smem_bytes();
+hipError_t launch_coords(const DevReceptor& R, const DevLigand* ligs, int ligand, const double* genes, int stride,
+                         int n, float* xyzT, int ld, hipStream_t stream);
 size_t lse_scratch_bytes();
 hipError_t launch_thermo(const double* d_energy, int stride, const int* d_count, long long n, double T,
                          void* d_scratch, fa_thermo* d_out, hipStream_t stream);
@@ -75,6 +77,7 @@ struct fa_ctx {
     // pose buffers
     DevBuf genes, out, counters, recs, reccnt, xyz, scratch, thermo, ops, prof, near_idx, near_xyz, rbox;
     DevBuf totals;  // FA_TOTALS_N x u64 unscored-pose counts since the last reset (not cleared per launch)
+    DevBuf cluster; // pose clustering (fa_cluster_cf): coordinates, inputs, per-pose and per-mode arrays
     int max_genes = 0;  // largest gene count of the uploaded ligands
     int n_sc = 0;       // scorable side-chain atoms of the resident receptor
     int prof_poses = 0;  // poses of the last profiled launch

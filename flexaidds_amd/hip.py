@@ -84,6 +84,7 @@ _SIGS = {
     "fa_ga_dedup_stats": ([_P, _P], C.c_int),
     "fa_ga_grand": ([_P, _P, _P, _P, _P], C.c_int),
     "fa_grand_device": ([_P, _P, C.c_int, _P, C.c_int, _P, _P], C.c_int),
+    "fa_cluster_cf": ([_P, C.c_int, C.c_int, _P, C.c_int, _P, _P, _P, _P, _P, _P, _P], C.c_int),
     "fa_ga_run": ([_P, C.c_int, C.c_int, _P], C.c_int),
     "fa_pool_create": ([_P, _P, _P], C.c_int),
     "fa_pool_shape": ([_P, _P, _P, _P], C.c_int),
@@ -244,6 +245,30 @@ class Context:
         _check(_lib.fa_grand_device(self._ctx, C.c_void_p(log_z_ptr), stride, C.c_void_p(log_c_ptr), n,
                                     C.c_void_p(out_ptr), C.c_void_p(stream) if stream else None))
 
+    def cluster_cf(self, ligand: int, genes, energy, temperature: float = 0.0, rmsd: float = 2.0,
+                   max_results: int = 10) -> dict:
+        """Binding modes of poses of `ligand` (fa_cluster_cf): genes [n, >= genes of
+        the ligand] internal coordinates in the order to cluster (by increasing
+        energy, as the reference passes them), energy [n] (app energies)."""
+        g = np.ascontiguousarray(genes, np.float64)
+        e = np.ascontiguousarray(energy, np.float64)
+        n = int(e.shape[0])
+        if g.ndim != 2 or g.shape[0] != n:
+            raise ValueError("genes must be [n, genes] with one row per energy")
+        p = ClusterParams(float(temperature), float(rmsd), int(max_results))
+        modes = (ClusterMode * max(1, int(max_results)))()
+        member = np.zeros(max(n, 1), np.int32)
+        rh = np.zeros(max(n, 1), np.float32)
+        trm = np.zeros(max(1, int(max_results)) ** 2, np.float32)
+        out = ClusterOut()
+        _check(_lib.fa_cluster_cf(self._ctx, ligand, n, g.ctypes.data, g.shape[1], e.ctypes.data, C.byref(p), modes,
+                                  member.ctypes.data, rh.ctypes.data, trm.ctypes.data, C.byref(out)))
+        k = int(out.n_modes)
+        return {"modes": [{"top": int(m.top), "freq": int(m.freq), "tcf": float(m.tcf), "acf": float(m.acf)}
+                          for m in modes[:k]],
+                "member": member[:n], "rmsd_head": rh[:n], "top_rmsd": trm[:k * k].reshape(k, k),
+                "n_clustered": int(out.n_clustered), "device_ms": float(out.device_ms)}
+
     def opcount(self) -> dict:
         o = OpCount()
         _check(_lib.fa_last_opcount(self._ctx, C.byref(o)))
@@ -299,6 +324,18 @@ class Context:
 class GeneLim(C.Structure):
     _fields_ = [("min", C.c_double), ("max", C.c_double), ("del_", C.c_double), ("map", C.c_int32),
                 ("pad", C.c_int32)]
+
+
+class ClusterParams(C.Structure):
+    _fields_ = [("temperature", C.c_double), ("rmsd", C.c_float), ("max_results", C.c_int32)]
+
+
+class ClusterMode(C.Structure):
+    _fields_ = [("top", C.c_int32), ("freq", C.c_int32), ("tcf", C.c_double), ("acf", C.c_double)]
+
+
+class ClusterOut(C.Structure):
+    _fields_ = [("n_modes", C.c_int32), ("n_clustered", C.c_int32), ("device_ms", C.c_float), ("pad", C.c_int32)]
 
 
 class GAParams(C.Structure):

@@ -285,6 +285,54 @@ int fa_ga_dedup_stats(fa_ga* ga, uint64_t* stats);
 int fa_ga_destroy(fa_ga* ga);
 
 /* ---------------------------------------------------------------------
+ * Binding modes: clustering of docked poses (replaces cluster(),
+ * LIB/cluster.cpp:9-176, the reference's default CLUSTA CF, called after
+ * the GA at LIB/top.cpp:1350-1353).  The poses, in the order given (the
+ * reference passes them by increasing energy), are grouped by repeatedly
+ * taking the first pose not yet clustered as the head of a new mode and
+ * adding every later unclustered pose whose ligand RMSD to it (no
+ * superposition) is at most `rmsd`, until every pose is clustered or
+ * max_results modes exist.  Each mode records its head (TOP), its size
+ * (FRE) and TCF / ACF: energies at temperature 0; otherwise, with
+ * P = exp(-E / T) / sum exp(-E / T) (the reference's beta = 1 / T),
+ * TCF = P E - T P ln P of the head and ACF = TCF + sum over members of
+ * P E + T P ln P; at T > 0 the first max_results records are then ordered
+ * by ACF with the reference's quicksort (records never filled take part as
+ * zeros, as there).  Energies are what the reference clusters on
+ * (app_evalue, fa_cf_result.app).
+ * ------------------------------------------------------------------- */
+typedef struct fa_cluster_params {
+    double temperature;        /* K (FA->temperature, an integer there); 0: plain energies */
+    float rmsd;                /* FA->cluster_rmsd, A (reference default 2.0) */
+    int32_t max_results;       /* FA->max_results (default 10), 1..64 */
+} fa_cluster_params;
+
+typedef struct fa_cluster_mode {
+    int32_t top;               /* pose index of the mode's head (Clus_TOP) */
+    int32_t freq;              /* poses in the mode (Clus_FRE) */
+    double tcf;                /* Clus_TCF */
+    double acf;                /* Clus_ACF */
+} fa_cluster_mode;
+
+typedef struct fa_cluster_out {
+    int32_t n_modes;           /* records written to `modes` */
+    int32_t n_clustered;       /* sum of their freq */
+    float device_ms;           /* coordinates and clustering kernels */
+    int32_t pad;
+} fa_cluster_out;
+
+/* Clusters n_poses poses of `ligand` (genes: [n_poses * gene_stride]
+ * internal coordinates, energy: [n_poses]; host arrays) on the context's
+ * device.  modes: [max_results]; member (optional, [n_poses]): head pose of
+ * each pose, -1 when unclustered; rmsd_head (optional, [n_poses]): RMSD to
+ * that head; top_rmsd (optional, [max_results^2], row-major n_modes x
+ * n_modes): RMSD between the heads of the output modes (the .cad file's
+ * "RMSD between clusters").  Synchronous. */
+int fa_cluster_cf(fa_ctx* ctx, int ligand, int n_poses, const double* genes, int gene_stride, const double* energy,
+                  const fa_cluster_params* params, fa_cluster_mode* modes, int* member, float* rmsd_head,
+                  float* top_rmsd, fa_cluster_out* out);
+
+/* ---------------------------------------------------------------------
  * Docking pool: continuous batching of docking jobs on one context.
  * Replaces running GA() once per request (LIB/gaboom.cpp:346-500) with a
  * resident pool that advances every in-flight job by one generation per

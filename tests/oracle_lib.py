@@ -60,6 +60,9 @@ def load_oracle():
     lib.fa_oracle_boltzmann_batch.restype = C.c_int
     lib.fa_oracle_log_xi.argtypes = [_P, _P, C.c_int, _P, _P]
     lib.fa_oracle_log_xi.restype = C.c_double
+    lib.fa_oracle_cluster_cf.argtypes = [C.c_int, C.c_int, _P, _P, C.c_double, C.c_float, C.c_int, C.c_int, _P, _P,
+                                         _P, _P, _P, _P, _P]
+    lib.fa_oracle_cluster_cf.restype = C.c_int
     return lib
 
 
@@ -70,7 +73,58 @@ def load_ref(fast=False):
     lib = C.CDLL(path)
     lib.fa_ref_eval.argtypes = [_P, _P, _P, C.c_int, _P, _P, _P, _P, C.c_int, _P]
     lib.fa_ref_eval.restype = C.c_int
+    lib.fa_ref_cluster.argtypes = [_P, _P, _P, C.c_int, _P, _P, _P, C.c_uint, C.c_float, C.c_int, C.c_char_p]
+    lib.fa_ref_cluster.restype = C.c_int
     return lib
+
+
+def oracle_cluster(lib, xyz, energy, T=0.0, rmsd=2.0, max_results=10, mode=1):
+    """The oracle's CF clustering of poses (xyz [n, atoms, 3] float32, in
+    clustering order); mode 0: scalar distance sums, 1: AVX2 order."""
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    e = np.ascontiguousarray(energy, np.float64)
+    n, L = xyz.shape[0], xyz.shape[1]
+    R = max_results
+    top, fre = np.zeros(R, np.int32), np.zeros(R, np.int32)
+    tcf, acf = np.zeros(R), np.zeros(R)
+    member, rh = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.float32)
+    trm = np.zeros(R * R, np.float32)
+    nc = lib.fa_oracle_cluster_cf(n, L, xyz.ctypes.data, e.ctypes.data, float(T), float(rmsd), R, mode,
+                                  top.ctypes.data, fre.ctypes.data, tcf.ctypes.data, acf.ctypes.data,
+                                  member.ctypes.data, rh.ctypes.data, trm.ctypes.data)
+    return {"modes": [{"top": int(top[k]), "freq": int(fre[k]), "tcf": float(tcf[k]), "acf": float(acf[k])}
+                      for k in range(nc)],
+            "member": member[:n], "rmsd_head": rh[:n], "top_rmsd": trm[:nc * nc].reshape(nc, nc)}
+
+
+def ref_cluster(lib, desc, lig_index, ic, app, evalue, T, rmsd, max_results, workdir):
+    """The reference's cluster() on these chromosomes (in this order); parsed
+    from the .cad file it writes: modes (top, freq, tcf, acf to six decimals)
+    and the RMSD between heads."""
+    ic = np.ascontiguousarray(ic, np.float64)
+    app = np.ascontiguousarray(app, np.float64)
+    ev = np.ascontiguousarray(evalue, np.float64)
+    prefix = os.path.join(str(workdir), "res")
+    rc = lib.fa_ref_cluster(_addr(desc.receptor), _addr(desc.ligands[lig_index]), _addr(desc.params), len(app),
+                            ic.ctypes.data, app.ctypes.data, ev.ctypes.data, int(T), float(rmsd), int(max_results),
+                            prefix.encode())
+    assert rc == 0
+    modes, rm = [], {}
+    with open(prefix + ".cad") as f:
+        for line in f:
+            if line.startswith("Cluster "):
+                head, rest = line.split(":", 1)
+                kv = dict(t.split("=") for t in rest.split())
+                modes.append({"top": int(kv["TOP"]), "freq": int(kv["freq"]), "tcf": float(kv["TCF"]),
+                              "acf": float(kv["ACF"])})
+            elif line.startswith("rmsd("):
+                a, b = line[5:line.index(")")].split(",")
+                rm[(int(a), int(b))] = float(line.split("=")[1])
+    k = len(modes)
+    trm = np.zeros((k, k))
+    for (a, b), v in rm.items():
+        trm[a, b] = trm[b, a] = v
+    return {"modes": modes, "top_rmsd": trm}
 
 
 def load_ref_reader():

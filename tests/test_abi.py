@@ -57,7 +57,8 @@ def test_struct_sizes_against_the_c_compiler(tmp_path):
     mirrors = {"fa_params": abi.Params, "fa_ligand_desc": abi.LigandDesc, "fa_constraint": abi.Constraint, "fa_receptor_desc": abi.ReceptorDesc,
                "fa_cf_result": abi.CFResult, "fa_thermo": abi.Thermo, "fa_gene_lim": hip.GeneLim,
                "fa_ga_params": hip.GAParams, "fa_pool_params": hip.PoolParams, "fa_job": hip.Job,
-               "fa_job_result": hip.JobResult}
+               "fa_job_result": hip.JobResult, "fa_cluster_params": hip.ClusterParams,
+               "fa_cluster_mode": hip.ClusterMode, "fa_cluster_out": hip.ClusterOut}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "flexaidds_amd.h"', "int main(void) {"]
     for cname, py in mirrors.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

@@ -1,0 +1,117 @@
+"""Binding modes on the device (fa_cluster_cf) against the oracle's CF
+clustering (GPU tests).
+
+The device builds the poses' ligand coordinates itself (the pose kernel's
+build, bit-identical to the oracle's) and clusters them; the oracle clusters
+the coordinates it builds, in the AVX2 summation order the device follows.
+Every pose's head and RMSD to it, the heads and sizes of the modes and the
+RMSD between heads must be identical; TCF / ACF agree to 1e-12 relative (the
+device sums the partition function and a mode's member terms as a tree, the
+reference in pose order).  Cases: scoring poses of the config-0 ligand in
+energy order at T = 0 and T = 300 K (the zero records of the reference's
+ordering included), a max_results cap, and 20,000 jittered poses of the
+config-1 ligand (flexible side chains) with energies in increasing order.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from flexaidds_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from flexaidds_amd import hip as h
+    return h
+
+
+def _coords(oracle, desc, genes):
+    # the oracle's build in the device's trigonometry (its bit-exact mirror)
+    L = desc.ligands[0].n_atoms
+    out = np.zeros((len(genes), L, 3), np.float32)
+    with ol.gpu_mirror(oracle):
+        for i, g in enumerate(np.ascontiguousarray(genes, np.float64)):
+            oracle.fa_oracle_build_pose(ol._addr(desc.ligands[0]), g.ctypes.data, out[i].ctypes.data)
+    return out
+
+
+def _jittered(rng, lig, n_seeds, per_seed, spread):
+    lo, hi = np.asarray(lig.gene_min, float), np.asarray(lig.gene_max, float)
+    angle = (np.asarray(lig.gene_typ) != -1) & (np.asarray(lig.gene_typ) != 4)
+    seeds = synth.random_genes(rng, lig, n_seeds)
+    kids = np.repeat(seeds, per_seed, axis=0)
+    kids[:, angle] += rng.normal(0.0, spread, size=(len(kids), int(angle.sum())))
+    return np.clip(kids, lo, hi)
+
+
+def _same(dev, ora, rel=1e-12):
+    assert np.array_equal(dev["member"], ora["member"])
+    assert np.array_equal(dev["rmsd_head"].view(np.uint32), ora["rmsd_head"].view(np.uint32))
+    assert [(m["top"], m["freq"]) for m in dev["modes"]] == [(m["top"], m["freq"]) for m in ora["modes"]]
+    for a, b in zip(dev["modes"], ora["modes"]):
+        for k in ("tcf", "acf"):
+            assert abs(a[k] - b[k]) <= rel * max(1.0, abs(b[k])), (k, a, b)
+    assert np.array_equal(dev["top_rmsd"].view(np.uint32), ora["top_rmsd"].view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def scored(hip, oracle):
+    # scoring poses of the config-0 ligand, in increasing energy (device energies)
+    s = synth.make_config0(n_poses=8)
+    rng = np.random.default_rng(12)
+    genes = _jittered(rng, s.ligands[0], 40, 16, 6.0)
+    ctx = hip.Context()
+    ctx.upload_system(s)
+    res = ctx.eval(0, genes)
+    ok = res["status"] == 0
+    genes, res = genes[ok], res[ok]
+    order = np.argsort(res["cf"], kind="stable")
+    desc = abi.Descriptors(s.receptor, s.ligands, s.params)
+    yield ctx, desc, genes[order], res[order]
+    ctx.close()
+
+
+@pytest.mark.parametrize("T,max_results,rmsd", [(0.0, 10, 2.0), (300.0, 10, 2.0), (300.0, 64, 1.0), (0.0, 64, 3.0)])
+def test_modes_equal_oracle(oracle, scored, T, max_results, rmsd):
+    ctx, desc, genes, res = scored
+    assert len(genes) >= 100
+    dev = ctx.cluster_cf(0, genes, res["app"], temperature=T, rmsd=rmsd, max_results=max_results)
+    ora = ol.oracle_cluster(oracle, _coords(oracle, desc, genes), res["app"], T=T, rmsd=rmsd,
+                            max_results=max_results, mode=1)
+    _same(dev, ora)
+    assert dev["n_clustered"] == sum(m["freq"] for m in ora["modes"])
+
+
+def test_zero_records_in_the_ordering(oracle, scored):
+    ctx, desc, genes, res = scored
+    e = res["app"] + 400.0
+    dev = ctx.cluster_cf(0, genes, e, temperature=300.0, rmsd=2.0, max_results=64)
+    ora = ol.oracle_cluster(oracle, _coords(oracle, desc, genes), e, T=300.0, rmsd=2.0, max_results=64, mode=1)
+    _same(dev, ora)
+    assert any(m["freq"] == 0 for m in dev["modes"])
+
+
+def test_twenty_thousand_flexible_poses(hip, oracle):
+    s = synth.make_config1(pop=8)
+    rng = np.random.default_rng(3)
+    genes = _jittered(rng, s.ligands[0], 500, 40, 4.0)
+    e = np.sort(rng.normal(-20.0, 8.0, len(genes)))
+    desc = abi.Descriptors(s.receptor, s.ligands, s.params)
+    with hip.Context() as ctx:
+        ctx.upload_system(s)
+        dev = ctx.cluster_cf(0, genes, e, temperature=300.0, rmsd=2.0, max_results=32)
+    ora = ol.oracle_cluster(oracle, _coords(oracle, desc, genes), e, T=300.0, rmsd=2.0, max_results=32, mode=1)
+    _same(dev, ora)
+    assert dev["n_clustered"] > 32
+
+
+def test_empty_and_errors(hip, scored):
+    ctx, _, genes, res = scored
+    out = ctx.cluster_cf(0, genes[:0], res["app"][:0])
+    assert out["modes"] == [] and out["n_clustered"] == 0
+    with pytest.raises(hip.FAError):
+        ctx.cluster_cf(0, genes, res["app"], max_results=0)
+    with pytest.raises(hip.FAError):
+        ctx.cluster_cf(5, genes, res["app"])
