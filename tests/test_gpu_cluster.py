@@ -115,3 +115,46 @@ def test_empty_and_errors(hip, scored):
         ctx.cluster_cf(0, genes, res["app"], max_results=0)
     with pytest.raises(hip.FAError):
         ctx.cluster_cf(5, genes, res["app"])
+
+
+def _oracle_thermo(oracle, e, T, count=None):
+    import ctypes as C
+    e = np.ascontiguousarray(e, np.float64)
+    out = abi.Thermo()
+    cnt = None if count is None else np.ascontiguousarray(count, np.int32)
+    oracle.fa_oracle_thermo(e.ctypes.data, None if cnt is None else cnt.ctypes.data, len(e), T, C.byref(out))
+    return {f: getattr(out, f) for f, _ in abi.Thermo._fields_}
+
+
+def test_binding_population(oracle, scored):
+    # the modes' thermodynamics (one sample per pose), their Boltzmann
+    # weights, the free-energy order, the dG matrix and the global ensemble
+    # (multiplicity = int(weight)) against the oracle's statistics
+    from flexaidds_amd import modes
+    ctx, _, genes, res = scored
+    T = 300.0
+    pop = modes.BindingPopulation(ctx, 0, genes, res["app"], temperature=T, rmsd=2.0, max_results=10)
+    assert pop.size() >= 3
+    F = [m.free_energy() for m in pop.modes]
+    assert F == sorted(F)
+    for m in pop.modes:
+        ref = _oracle_thermo(oracle, m.energies, T)
+        for k in ("free_energy", "mean_energy", "log_Z"):
+            assert abs(m.thermodynamics()[k] - ref[k]) <= 1e-9 * max(1.0, abs(ref[k])), (k, m.top)
+        w = m.boltzmann_weights()
+        wr = np.zeros(m.size())
+        oracle.fa_oracle_boltzmann(np.ascontiguousarray(m.energies).ctypes.data, None, m.size(), T, wr.ctypes.data)
+        assert np.allclose(w, wr, rtol=1e-12, atol=1e-300) and abs(w.sum() - 1.0) < 1e-12
+        assert m.energies[0] == res["app"][m.top] and m.poses[0] == m.top
+    D = pop.deltaG_matrix()
+    assert np.allclose(D, -D.T) and abs(D[0, 1] - (F[0] - F[1])) <= 1e-12 * max(1.0, abs(F[0]))
+    ge = pop.global_ensemble()
+    singles = [m for m in pop.modes if m.size() == 1]
+    e = np.concatenate([m.energies for m in pop.modes])
+    c = np.concatenate([m.boltzmann_weights().astype(np.int32) for m in pop.modes])
+    assert int(c.sum()) >= len(singles)
+    if c.any():
+        ref = _oracle_thermo(oracle, e, T, c)
+        assert abs(ge["free_energy"] - ref["free_energy"]) <= 1e-9 * max(1.0, abs(ref["free_energy"]))
+    else:
+        assert ge is None
