@@ -23,6 +23,10 @@ path (one ligand population per launch).  Beside it:
   real_inputs  the plain launch and a device-GA dock on a receptor read from
             a PDB file (the reference's RNA target) with the automatic
             binding-site grid built from it
+  binding_modes  the reference's default clustering of docked poses into
+            binding modes on the device, over 200,000 poses (ga_dock also
+            reports the modes of its final population and the same dock with
+            duplicate rejection)
 
 The roofline numbers come from the kernel's own operation counters (the
 algorithmic FLOP model, see include/fa_types.h fa_opcount) and HIP-event
@@ -240,15 +244,64 @@ def ga_dock_leg(ctx, system, args):
     ga = hip.GA(ctx, synth.ga_limits(lig), population=args.pop, offspring=args.pop, seed=2024, temperature=300.0)
     ms = ga.run(args.ga_generations, graph=True)
     th, n = ga.thermo()
-    _, _, res = ga.population()
+    _, ic, res = ga.population()
     wall = time.perf_counter() - t0
     ga.close()
     poses = args.pop * args.ga_generations
-    return {"population": args.pop, "generations": args.ga_generations, "poses": int(n),
-            "generations_ms": round(ms, 2), "ms_per_generation": round(ms / args.ga_generations, 4),
-            "poses_per_s": round(poses / (ms / 1e3), 1), "dock_wall_s": round(wall, 3),
-            "best_cf": round(float(res["cf"][0]), 6), "free_energy_300K": round(th["free_energy"], 6),
-            "unscored": _unscored(ctx)}
+    out = {"population": args.pop, "generations": args.ga_generations, "poses": int(n),
+           "generations_ms": round(ms, 2), "ms_per_generation": round(ms / args.ga_generations, 4),
+           "poses_per_s": round(poses / (ms / 1e3), 1), "dock_wall_s": round(wall, 3),
+           "best_cf": round(float(res["cf"][0]), 6), "free_energy_300K": round(th["free_energy"], 6),
+           "unscored": _unscored(ctx)}
+    # binding modes of the final population (survivors in energy order)
+    from flexaidds_amd import modes
+    pop = modes.BindingPopulation(ctx, 0, ic[:args.pop, :lig.n_genes], res["app"][:args.pop], temperature=300.0,
+                                  rmsd=2.0, max_results=10)
+    out["binding_modes"] = {"modes": pop.size(), "clustered": pop.clusters["n_clustered"],
+                            "device_ms": round(pop.clusters["device_ms"], 4),
+                            "best_mode_free_energy_300K": round(pop.modes[0].free_energy(), 6) if pop.modes else None,
+                            "best_mode_size": pop.modes[0].size() if pop.modes else 0}
+    # the same dock with the reference's default duplicate rejection
+    if args.dedup_generations > 0:
+        ga = hip.GA(ctx, synth.ga_limits(lig), population=args.pop, offspring=args.pop, seed=2024, temperature=300.0,
+                    dedup_rounds=8)
+        dms = ga.run(args.dedup_generations, graph=True)
+        st = ga.dedup_stats()[0]
+        _, _, dres = ga.population()
+        ga.close()
+        out["dedup"] = {"generations": args.dedup_generations, "rounds": 8,
+                        "ms_per_generation": round(dms / args.dedup_generations, 4),
+                        "duplicates_drawn": int(st[0]), "duplicates_kept_at_cap": int(st[1]),
+                        "best_cf": round(float(dres["cf"][0]), 6)}
+    return out
+
+
+def modes_leg(args):
+    """Clustering throughput: --modes-poses jittered poses of the config-1
+    ligand (a GA history's size) in energy order, T = 300 K, 10 modes."""
+    from flexaidds_amd import hip, synth
+    log(f"binding modes: {args.modes_poses} poses")
+    system = synth.make_config1(pop=8)
+    lig = system.ligands[0]
+    rng = np.random.default_rng(77)
+    seeds = synth.random_genes(rng, lig, max(1, args.modes_poses // 400))
+    genes = np.repeat(seeds, 400, axis=0)[:args.modes_poses]
+    ang = (np.asarray(lig.gene_typ) != -1) & (np.asarray(lig.gene_typ) != 4)
+    genes[:, ang] += rng.normal(0.0, 5.0, size=(len(genes), int(ang.sum())))
+    genes = np.clip(genes, lig.gene_min, lig.gene_max)
+    e = np.sort(rng.normal(-20.0, 8.0, len(genes)))
+    with hip.Context() as ctx:
+        ctx.upload_system(system)
+        ctx.cluster_cf(0, genes[:1000], e[:1000], temperature=300.0)          # warm
+        r = [ctx.cluster_cf(0, genes, e, temperature=300.0, rmsd=2.0, max_results=10) for _ in range(3)]
+    ms = float(np.median([x["device_ms"] for x in r]))
+    passes = len(r[0]["modes"])
+    # coordinate bytes one head pass reads (component-major float32)
+    pass_bytes = len(genes) * 3 * lig.n_atoms * 4
+    return {"poses": len(genes), "ligand_atoms": lig.n_atoms, "modes": passes, "clustered": r[0]["n_clustered"],
+            "device_ms": round(ms, 4), "poses_per_s": round(len(genes) / (ms / 1e3), 1),
+            "coordinate_bytes_per_pass": pass_bytes,
+            "note": "device_ms covers the coordinate build and every head pass"}
 
 
 def _tune_generation(ctx, system, pop, seed=5):
@@ -475,6 +528,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--pop", type=int, default=2000)
     ap.add_argument("--ga-generations", type=int, default=500, help="plain GA dock length (0: skip)")
+    ap.add_argument("--dedup-generations", type=int, default=100,
+                    help="the GA dock again with duplicate rejection (0: skip)")
+    ap.add_argument("--modes-poses", type=int, default=200000, help="binding-mode clustering poses (0: skip)")
     ap.add_argument("--config0", type=int, default=1, help="time the config-0 case (1,000 poses, Z/F at 300 K; 0: skip)")
     ap.add_argument("--batched-ligands", type=int, default=256, help="batched screen ligands (0: skip)")
     ap.add_argument("--batched-pop", type=int, default=512)
@@ -595,6 +651,7 @@ def main():
     served = served_leg(args) if args.served_requests > 0 else None
     grand = grand_leg(args) if args.grand_ligands > 0 else None
     real = real_leg(args) if args.real_steps > 0 else None
+    modes_l = modes_leg(args) if args.modes_poses > 0 else None
 
     ctx.close()
     log("CPU baseline")
@@ -635,6 +692,7 @@ def main():
         "served": served,
         "grand": grand,
         "real_inputs": real,
+        "binding_modes": modes_l,
         "status_counts": {"ok": int(status[0]), "hull_fail": int(status[1]), "clash": int(status[2]),
                           "capacity": int(status[3]), "grid_range": int(status[4])},
         "unscored": plain_unscored,

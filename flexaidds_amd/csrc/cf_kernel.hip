@@ -789,6 +789,46 @@ __device__ __forceinline__ void put_vertex(Smem& s, int vn, double x, double y, 
 }
 
 // ------------------------------------------------------------ pose build
+// One atom from its internal coordinates: the plane of reference atoms 1-3,
+// the bond of length dis along 1 -> 2 turned by the angle and the dihedral
+// (LIB/buildcc.cpp:40-120, FP32 in the reference's operation order)
+__device__ __forceinline__ void place_atom(const float* x, const float* y, const float* z, float dis, float cta,
+                                           float sta, float ctd, float std_, float& ox, float& oy, float& oz)
+{
+    float A = y[1] * (z[2] - z[3]) + y[2] * (z[3] - z[1]) + y[3] * (z[1] - z[2]);
+    float B = z[1] * (x[2] - x[3]) + z[2] * (x[3] - x[1]) + z[3] * (x[1] - x[2]);
+    float C = x[1] * (y[2] - y[3]) + x[2] * (y[3] - y[1]) + x[3] * (y[1] - y[2]);
+    float op = sqrtf(A * A + B * B + C * C);
+    float cx = A / op, cy = B / op, cz = C / op;
+    A = x[2] - x[1];
+    B = y[2] - y[1];
+    C = z[2] - z[1];
+    float d = 1.0f / sqrtf(A * A + B * B + C * C);
+    op = dis * d;
+    float xn = A * op, yn = B * op, zn = C * op;
+    A = cx * cx;
+    B = cy * cy;
+    C = cz * cz;
+    float ct = cta;
+    float st = -sta;
+    op = 1.0f - ct;
+    float xk = (cx * cz * op - cy * st) * zn + ((1.0f - A) * ct + A) * xn + (cx * cy * op + cz * st) * yn;
+    float yk = (cy * cx * op - cz * st) * xn + ((1.0f - B) * ct + B) * yn + (cy * cz * op + cx * st) * zn;
+    float zk = (cz * cy * op - cx * st) * yn + ((1.0f - C) * ct + C) * zn + (cz * cx * op + cy * st) * xn;
+    ct = ctd;
+    st = std_;
+    op = 1.0f - ct;
+    cx = (x[2] - x[1]) * d;
+    cy = (y[2] - y[1]) * d;
+    cz = (z[2] - z[1]) * d;
+    A = cx * cx;
+    B = cy * cy;
+    C = cz * cz;
+    ox = (cx * cz * op - cy * st) * zk + ((1.0f - A) * ct + A) * xk + (cx * cy * op + cz * st) * yk + x[1];
+    oy = (cy * cx * op - cz * st) * xk + ((1.0f - B) * ct + B) * yk + (cy * cz * op + cx * st) * zk + y[1];
+    oz = (cz * cy * op - cx * st) * yk + ((1.0f - C) * ct + C) * zk + (cz * cx * op + cy * st) * xk + z[1];
+}
+
 // LIB/ic2cf.cpp:53-105 then LIB/buildcc.cpp:6-134 (FP32 like the reference;
 // cos/sin are taken in double and rounded, i.e. correctly rounded floats).
 // The gene decode is serial (shift chains); the trig runs one atom per lane;
@@ -923,38 +963,8 @@ __device__ void build_pose(Smem& s, const DevReceptor& R, const DevLigand& lig, 
             y[i] += 1e-10f;
             z[i] += 1e-10f;
         }
-        float A = y[1] * (z[2] - z[3]) + y[2] * (z[3] - z[1]) + y[3] * (z[1] - z[2]);
-        float B = z[1] * (x[2] - x[3]) + z[2] * (x[3] - x[1]) + z[3] * (x[1] - x[2]);
-        float C = x[1] * (y[2] - y[3]) + x[2] * (y[3] - y[1]) + x[3] * (y[1] - y[2]);
-        float op = sqrtf(A * A + B * B + C * C);
-        float cx = A / op, cy = B / op, cz = C / op;
-        A = x[2] - x[1];
-        B = y[2] - y[1];
-        C = z[2] - z[1];
-        float d = 1.0f / sqrtf(A * A + B * B + C * C);
-        op = dis * d;
-        float xn = A * op, yn = B * op, zn = C * op;
-        A = cx * cx;
-        B = cy * cy;
-        C = cz * cz;
-        float ct = cta;
-        float st = -sta;
-        op = 1.0f - ct;
-        float xk = (cx * cz * op - cy * st) * zn + ((1.0f - A) * ct + A) * xn + (cx * cy * op + cz * st) * yn;
-        float yk = (cy * cx * op - cz * st) * xn + ((1.0f - B) * ct + B) * yn + (cy * cz * op + cx * st) * zn;
-        float zk = (cz * cy * op - cx * st) * yn + ((1.0f - C) * ct + C) * zn + (cz * cx * op + cy * st) * xn;
-        ct = ctd;
-        st = std_;
-        op = 1.0f - ct;
-        cx = (x[2] - x[1]) * d;
-        cy = (y[2] - y[1]) * d;
-        cz = (z[2] - z[1]) * d;
-        A = cx * cx;
-        B = cy * cy;
-        C = cz * cz;
-        const float ox = (cx * cz * op - cy * st) * zk + ((1.0f - A) * ct + A) * xk + (cx * cy * op + cz * st) * yk + x[1];
-        const float oy = (cy * cx * op - cz * st) * xk + ((1.0f - B) * ct + B) * yk + (cy * cz * op + cx * st) * zk + y[1];
-        const float oz = (cz * cy * op - cx * st) * yk + ((1.0f - C) * ct + C) * zk + (cz * cx * op + cy * st) * xk + z[1];
+        float ox, oy, oz;
+        place_atom(x, y, z, dis, cta, sta, ctd, std_, ox, oy, oz);
         if (l == la) {
             if (hi) {
                 X[1] = ox;
@@ -2987,38 +2997,112 @@ hipError_t launch_cf(const DevReceptor& R, const DevGrid* grids, const DevLigand
     return hipGetLastError();
 }
 
-// Ligand coordinates of poses without scoring them: the pose kernel's build
-// (LIB/ic2cf.cpp:53-105, LIB/buildcc.cpp) of every pose, one wave per pose,
-// written component-major (xyzT[(3 * atom + k) * ld + pose]) so that a pass
-// over many poses reads one coordinate of 64 poses per load.  The input of
-// the pose clustering (the coordinates calc_rmsd_chrom rebuilds,
-// LIB/calc_rmsd_chrom.cpp).
-__global__ __launch_bounds__(WAVE) void pose_coords_kernel(DevReceptor R, const DevLigand* __restrict__ ligs,
-                                                           int ligand, const double* __restrict__ genes, int stride,
-                                                           int n, float* __restrict__ xyzT, int ld)
+// Ligand coordinates of poses without scoring them, for the pose
+// clustering (the coordinates calc_rmsd_chrom rebuilds,
+// LIB/calc_rmsd_chrom.cpp): build_pose's decode and rebuild (LIB/ic2cf.cpp:
+// 53-105, LIB/buildcc.cpp) with one pose per thread instead of one per wave,
+// the same operations in the same order (place_atom), so the coordinates
+// are the pose kernel's bit for bit.  Internal coordinates and coordinates
+// live component-major in global memory (row k of pose t at k * ld + t), so
+// the threads of a wave touch consecutive words.
+__global__ __launch_bounds__(256) void pose_coords_kernel(const DevLigand* __restrict__ ligs, int ligand,
+                                                          const double* __restrict__ genes, int stride, int n,
+                                                          float* __restrict__ xyzT, float* __restrict__ icT, int ld)
 {
-    __shared__ Smem s;
-    const int l = lane_id();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
     const DevLigand lig = ligs[ligand];
-    const int ns = R.n_sc;
-    for (int pose = blockIdx.x; pose < n; pose += gridDim.x) {
-        build_pose(s, R, lig, genes + (size_t)pose * stride);
-        for (int i = l; i < lig.n_atoms; i += WAVE) {
-            xyzT[(size_t)(3 * i) * ld + pose] = s.lx[ns + i];
-            xyzT[(size_t)(3 * i + 1) * ld + pose] = s.ly[ns + i];
-            xyzT[(size_t)(3 * i + 2) * ld + pose] = s.lz[ns + i];
+    const int L = lig.n_atoms;
+    float* D = icT;
+    float* A = icT + (size_t)L * ld;
+    float* H = icT + (size_t)2 * L * ld;
+    for (int a = 0; a < L; ++a) {
+        D[(size_t)a * ld + t] = lig.dis[a];
+        A[(size_t)a * ld + t] = lig.ang[a];
+        H[(size_t)a * ld + t] = lig.dih[a];
+        // atoms not yet built read as 0, as build_pose's registers start
+        xyzT[(size_t)(3 * a) * ld + t] = 0.0f;
+        xyzT[(size_t)(3 * a + 1) * ld + t] = 0.0f;
+        xyzT[(size_t)(3 * a + 2) * ld + t] = 0.0f;
+    }
+    const double* icv = genes + (size_t)t * stride;
+    for (int g = 0; g < lig.n_genes; ++g) {
+        double v = icv[g];
+        const double gmax = lig.gene_max[g], gmin = lig.gene_min[g];
+        if (v > gmax) v = gmax;
+        if (v < gmin) v = gmin;
+        const int atm = lig.gene_atm[g], typ = lig.gene_typ[g];
+        if (typ == FA_GENE_GRID) {
+            unsigned gi = (unsigned)v;
+            if ((int)gi >= lig.n_grid) gi = lig.n_grid - 1;
+            D[(size_t)atm * ld + t] = lig.grid_ic[3 * gi];
+            A[(size_t)atm * ld + t] = lig.grid_ic[3 * gi + 1];
+            H[(size_t)atm * ld + t] = lig.grid_ic[3 * gi + 2];
+        } else if (typ == FA_GENE_DIS) {
+            D[(size_t)atm * ld + t] = (float)v;
+        } else if (typ == FA_GENE_ANG) {
+            A[(size_t)atm * ld + t] = (float)v;
+        } else if (typ == FA_GENE_DIH) {
+            H[(size_t)atm * ld + t] = (float)v;
+            int j = atm, cat = lig.rec[4 * j + 3];
+            if (cat >= 0) {
+                for (int guard = 0; cat != atm && guard < L; ++guard) {
+                    H[(size_t)cat * ld + t] = H[(size_t)j * ld + t] + lig.shift[cat];
+                    j = cat;
+                    cat = lig.rec[4 * j + 3];
+                }
+            }
+        }  // rotamer genes move side chains only
+    }
+    const float ox0 = lig.ori[0], oy0 = lig.ori[1], oz0 = lig.ori[2];
+    for (int an = 0; an < L; ++an) {
+        const int a = lig.mov[an];
+        const float angPI = (float)(A[(size_t)a * ld + t] * PI_REF / 180.0f);
+        const float dihPI = (float)(H[(size_t)a * ld + t] * PI_REF / 180.0f);
+        const float cta = (float)cos((double)angPI);
+        const float sta = (float)sin((double)angPI);
+        const float ctd = (float)cos((double)dihPI);
+        const float std_ = (float)sin((double)dihPI);
+        const float dis = D[(size_t)a * ld + t];
+        float x[4], y[4], z[4];
+        x[0] = y[0] = z[0] = 0.0f;
+        for (int i = 1; i <= 3; ++i) {
+            const int j = lig.rec[4 * a + i - 1];
+            if (j >= 0) {
+                x[i] = xyzT[(size_t)(3 * j) * ld + t];
+                y[i] = xyzT[(size_t)(3 * j + 1) * ld + t];
+                z[i] = xyzT[(size_t)(3 * j + 2) * ld + t];
+            } else if (i == 1) {
+                x[i] = 1.0f + ox0;
+                y[i] = 0.0f + oy0;
+                z[i] = 0.0f + oz0;
+            } else if (i == 3) {
+                x[i] = 0.0f + ox0;
+                y[i] = 1.0f + oy0;
+                z[i] = 0.0f + oz0;
+            } else {
+                x[i] = 0.0f + ox0;
+                y[i] = 0.0f + oy0;
+                z[i] = 0.0f + oz0;
+            }
+            x[i] += 1e-10f;
+            y[i] += 1e-10f;
+            z[i] += 1e-10f;
         }
-        wave_sync();
+        float px, py, pz;
+        place_atom(x, y, z, dis, cta, sta, ctd, std_, px, py, pz);
+        xyzT[(size_t)(3 * a) * ld + t] = px;
+        xyzT[(size_t)(3 * a + 1) * ld + t] = py;
+        xyzT[(size_t)(3 * a + 2) * ld + t] = pz;
     }
 }
 
-hipError_t launch_coords(const DevReceptor& R, const DevLigand* ligs, int ligand, const double* genes, int stride,
-                         int n, float* xyzT, int ld, hipStream_t stream)
+hipError_t launch_coords(const DevLigand* ligs, int ligand, const double* genes, int stride, int n, float* xyzT,
+                         float* icT, int ld, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    const int blocks = n < 8192 ? n : 8192;
-    hipLaunchKernelGGL(pose_coords_kernel, dim3(blocks), dim3(WAVE), 0, stream, R, ligs, ligand, genes, stride, n,
-                       xyzT, ld);
+    hipLaunchKernelGGL(pose_coords_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, ligs, ligand, genes, stride, n,
+                       xyzT, icT, ld);
     return hipGetLastError();
 }
 

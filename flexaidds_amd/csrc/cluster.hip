@@ -15,7 +15,7 @@
 // :323-359; records never filled take part as zeros).
 //
 // Device form: the poses' ligand coordinates are built once (the pose
-// kernel's build, component-major); each cluster is one pass over the
+// kernel's build, a thread per pose, component-major); each cluster is one pass over the
 // poses (a thread per pose) against its head's coordinates in LDS, which
 // also finds the next head (the first pose it leaves unclustered); a small
 // kernel folds the pass's member terms in block order and starts the next
@@ -105,13 +105,26 @@ __global__ __launch_bounds__(CL_BLOCK) void cl_init_kernel(ClDev c)
     }
 }
 
-// sum_j exp(-beta E_j), one block (the reference sums in pose order; this
-// is a tree, equal to rounding)
+// sum_j exp(-beta E_j): per-block tree sums, then their fold in block order
+// (the reference sums in pose order; equal to rounding)
 __global__ __launch_bounds__(CL_BLOCK) void cl_z_kernel(ClDev c)
 {
     __shared__ double sh[CL_BLOCK];
+    const int i = blockIdx.x * CL_BLOCK + threadIdx.x;
+    sh[threadIdx.x] = i < c.n ? exp(-c.beta * c.energy[i]) : 0.0;
+    __syncthreads();
+    for (int w = CL_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) c.part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(CL_BLOCK) void cl_z_final_kernel(ClDev c, int n_blocks)
+{
+    __shared__ double sh[CL_BLOCK];
     double v = 0.0;
-    for (int i = threadIdx.x; i < c.n; i += CL_BLOCK) v += exp(-c.beta * c.energy[i]);
+    for (int b = threadIdx.x; b < n_blocks; b += CL_BLOCK) v += c.part[b];
     sh[threadIdx.x] = v;
     __syncthreads();
     for (int w = CL_BLOCK / 2; w > 0; w >>= 1) {
@@ -320,13 +333,13 @@ int fa_cluster_cf(fa_ctx* ctx, int ligand, int n_poses, const double* genes, int
     const int n = n_poses, ld = std::max(n, 1);
     const int nb = std::max(1, (n + fa::CL_BLOCK - 1) / fa::CL_BLOCK);
     // one allocation: coordinates, genes, energies, per-pose and per-block arrays, records
-    const size_t b_xyz = sizeof(float) * 3 * L * (size_t)ld, b_gen = sizeof(double) * G * (size_t)ld,
+    const size_t b_xyz = sizeof(float) * 3 * L * (size_t)ld, b_ic = b_xyz, b_gen = sizeof(double) * G * (size_t)ld,
                  b_e = sizeof(double) * (size_t)ld, b_pose = (sizeof(int) + sizeof(float)) * (size_t)ld,
                  b_blk = (sizeof(double) + sizeof(int)) * (size_t)nb, b_rec = (2 * sizeof(int) + 2 * sizeof(double)) * R +
                                                                          sizeof(int) * (R + 4) + sizeof(double),
                  b_trm = sizeof(float) * R * R;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t total = al(b_xyz) + al(b_gen) + al(b_e) + al(b_pose) + al(b_blk) + al(b_rec) + al(b_trm);
+    const size_t total = al(b_xyz) + al(b_ic) + al(b_gen) + al(b_e) + al(b_pose) + al(b_blk) + al(b_rec) + al(b_trm);
     FA_HIP(hipDeviceSynchronize());  // the buffer may be regrown: no earlier clustering still reads it
     int rc = fa_ensure(ctx->cluster, total);
     if (rc) return rc;
@@ -338,6 +351,7 @@ int fa_cluster_cf(fa_ctx* ctx, int ligand, int n_poses, const double* genes, int
         return q;
     };
     float* xyzT = reinterpret_cast<float*>(take(b_xyz));
+    float* icT = reinterpret_cast<float*>(take(b_ic));
     double* d_genes = reinterpret_cast<double*>(take(b_gen));
     double* d_e = reinterpret_cast<double*>(take(b_e));
     char* pose_area = take(b_pose);
@@ -377,10 +391,13 @@ int fa_cluster_cf(fa_ctx* ctx, int ligand, int n_poses, const double* genes, int
     FA_HIP(hipEventCreate(&e0));
     FA_HIP(hipEventCreate(&e1));
     FA_HIP(hipEventRecord(e0, s));
-    FA_HIP(fa::launch_coords(ctx->rec, ctx->d_ligs, ligand, d_genes, G, n, xyzT, ld, s));
+    FA_HIP(fa::launch_coords(ctx->d_ligs, ligand, d_genes, G, n, xyzT, icT, ld, s));
     const int ni = std::max(nb, (R + fa::CL_BLOCK) / fa::CL_BLOCK);
     hipLaunchKernelGGL(fa::cl_init_kernel, dim3(ni), dim3(fa::CL_BLOCK), 0, s, c);
-    if (c.T > 0.0) hipLaunchKernelGGL(fa::cl_z_kernel, dim3(1), dim3(fa::CL_BLOCK), 0, s, c);
+    if (c.T > 0.0 && n > 0) {
+        hipLaunchKernelGGL(fa::cl_z_kernel, dim3(nb), dim3(fa::CL_BLOCK), 0, s, c);
+        hipLaunchKernelGGL(fa::cl_z_final_kernel, dim3(1), dim3(fa::CL_BLOCK), 0, s, c, nb);
+    }
     for (int r = 0; r < R && n > 0; ++r) {
         hipLaunchKernelGGL(fa::cl_assign_kernel, dim3(nb), dim3(fa::CL_BLOCK), 0, s, c, r);
         hipLaunchKernelGGL(fa::cl_finish_kernel, dim3(1), dim3(fa::CL_BLOCK), 0, s, c, r, nb);
