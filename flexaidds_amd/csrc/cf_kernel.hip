@@ -3003,27 +3003,31 @@ hipError_t launch_cf(const DevReceptor& R, const DevGrid* grids, const DevLigand
 // 53-105, LIB/buildcc.cpp) with one pose per thread instead of one per wave,
 // the same operations in the same order (place_atom), so the coordinates
 // are the pose kernel's bit for bit.  Internal coordinates and coordinates
-// live component-major in global memory (row k of pose t at k * ld + t), so
-// the threads of a wave touch consecutive words.
-__global__ __launch_bounds__(256) void pose_coords_kernel(const DevLigand* __restrict__ ligs, int ligand,
+// live in global memory in tiles of COORD_TILE poses (coord_at), so the
+// threads of a block touch consecutive words of one region (keeping a
+// thread's coordinates in LDS instead measured slower: the LDS it needs
+// leaves too few waves to hide the trigonometry).
+__global__ __launch_bounds__(COORD_TILE) void pose_coords_kernel(const DevLigand* __restrict__ ligs, int ligand,
                                                           const double* __restrict__ genes, int stride, int n,
-                                                          float* __restrict__ xyzT, float* __restrict__ icT, int ld)
+                                                          float* __restrict__ xyzT, float* __restrict__ icT)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const DevLigand lig = ligs[ligand];
-    const int L = lig.n_atoms;
-    float* D = icT;
-    float* A = icT + (size_t)L * ld;
-    float* H = icT + (size_t)2 * L * ld;
+    const int L = lig.n_atoms, R3 = 3 * L;
+    // rows of icT: dis of atom a at a, ang at L + a, dih at 2L + a
+    auto D = [&](int a) -> float& { return icT[coord_at(R3, a, t)]; };
+    auto A = [&](int a) -> float& { return icT[coord_at(R3, L + a, t)]; };
+    auto H = [&](int a) -> float& { return icT[coord_at(R3, 2 * L + a, t)]; };
+    auto X = [&](int a, int k) -> float& { return xyzT[coord_at(R3, 3 * a + k, t)]; };
     for (int a = 0; a < L; ++a) {
-        D[(size_t)a * ld + t] = lig.dis[a];
-        A[(size_t)a * ld + t] = lig.ang[a];
-        H[(size_t)a * ld + t] = lig.dih[a];
+        D(a) = lig.dis[a];
+        A(a) = lig.ang[a];
+        H(a) = lig.dih[a];
         // atoms not yet built read as 0, as build_pose's registers start
-        xyzT[(size_t)(3 * a) * ld + t] = 0.0f;
-        xyzT[(size_t)(3 * a + 1) * ld + t] = 0.0f;
-        xyzT[(size_t)(3 * a + 2) * ld + t] = 0.0f;
+        X(a, 0) = 0.0f;
+        X(a, 1) = 0.0f;
+        X(a, 2) = 0.0f;
     }
     const double* icv = genes + (size_t)t * stride;
     for (int g = 0; g < lig.n_genes; ++g) {
@@ -3035,19 +3039,19 @@ __global__ __launch_bounds__(256) void pose_coords_kernel(const DevLigand* __res
         if (typ == FA_GENE_GRID) {
             unsigned gi = (unsigned)v;
             if ((int)gi >= lig.n_grid) gi = lig.n_grid - 1;
-            D[(size_t)atm * ld + t] = lig.grid_ic[3 * gi];
-            A[(size_t)atm * ld + t] = lig.grid_ic[3 * gi + 1];
-            H[(size_t)atm * ld + t] = lig.grid_ic[3 * gi + 2];
+            D(atm) = lig.grid_ic[3 * gi];
+            A(atm) = lig.grid_ic[3 * gi + 1];
+            H(atm) = lig.grid_ic[3 * gi + 2];
         } else if (typ == FA_GENE_DIS) {
-            D[(size_t)atm * ld + t] = (float)v;
+            D(atm) = (float)v;
         } else if (typ == FA_GENE_ANG) {
-            A[(size_t)atm * ld + t] = (float)v;
+            A(atm) = (float)v;
         } else if (typ == FA_GENE_DIH) {
-            H[(size_t)atm * ld + t] = (float)v;
+            H(atm) = (float)v;
             int j = atm, cat = lig.rec[4 * j + 3];
             if (cat >= 0) {
                 for (int guard = 0; cat != atm && guard < L; ++guard) {
-                    H[(size_t)cat * ld + t] = H[(size_t)j * ld + t] + lig.shift[cat];
+                    H(cat) = H(j) + lig.shift[cat];
                     j = cat;
                     cat = lig.rec[4 * j + 3];
                 }
@@ -3057,21 +3061,21 @@ __global__ __launch_bounds__(256) void pose_coords_kernel(const DevLigand* __res
     const float ox0 = lig.ori[0], oy0 = lig.ori[1], oz0 = lig.ori[2];
     for (int an = 0; an < L; ++an) {
         const int a = lig.mov[an];
-        const float angPI = (float)(A[(size_t)a * ld + t] * PI_REF / 180.0f);
-        const float dihPI = (float)(H[(size_t)a * ld + t] * PI_REF / 180.0f);
+        const float angPI = (float)(A(a) * PI_REF / 180.0f);
+        const float dihPI = (float)(H(a) * PI_REF / 180.0f);
         const float cta = (float)cos((double)angPI);
         const float sta = (float)sin((double)angPI);
         const float ctd = (float)cos((double)dihPI);
         const float std_ = (float)sin((double)dihPI);
-        const float dis = D[(size_t)a * ld + t];
+        const float dis = D(a);
         float x[4], y[4], z[4];
         x[0] = y[0] = z[0] = 0.0f;
         for (int i = 1; i <= 3; ++i) {
             const int j = lig.rec[4 * a + i - 1];
             if (j >= 0) {
-                x[i] = xyzT[(size_t)(3 * j) * ld + t];
-                y[i] = xyzT[(size_t)(3 * j + 1) * ld + t];
-                z[i] = xyzT[(size_t)(3 * j + 2) * ld + t];
+                x[i] = X(j, 0);
+                y[i] = X(j, 1);
+                z[i] = X(j, 2);
             } else if (i == 1) {
                 x[i] = 1.0f + ox0;
                 y[i] = 0.0f + oy0;
@@ -3091,18 +3095,19 @@ __global__ __launch_bounds__(256) void pose_coords_kernel(const DevLigand* __res
         }
         float px, py, pz;
         place_atom(x, y, z, dis, cta, sta, ctd, std_, px, py, pz);
-        xyzT[(size_t)(3 * a) * ld + t] = px;
-        xyzT[(size_t)(3 * a + 1) * ld + t] = py;
-        xyzT[(size_t)(3 * a + 2) * ld + t] = pz;
+        X(a, 0) = px;
+        X(a, 1) = py;
+        X(a, 2) = pz;
     }
 }
 
-hipError_t launch_coords(const DevLigand* ligs, int ligand, const double* genes, int stride, int n, float* xyzT,
-                         float* icT, int ld, hipStream_t stream)
+hipError_t launch_coords(const DevLigand* ligs, int ligand, int n_atoms, const double* genes, int stride, int n,
+                         float* xyzT, float* icT, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(pose_coords_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, ligs, ligand, genes, stride, n,
-                       xyzT, icT, ld);
+    if (n_atoms < 1 || n_atoms > FA_MAX_LIG_ATOMS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pose_coords_kernel, dim3((n + COORD_TILE - 1) / COORD_TILE), dim3(COORD_TILE), 0, stream, ligs,
+                       ligand, genes, stride, n, xyzT, icT);
     return hipGetLastError();
 }
 

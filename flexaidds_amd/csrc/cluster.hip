@@ -15,7 +15,7 @@
 // :323-359; records never filled take part as zeros).
 //
 // Device form: the poses' ligand coordinates are built once (the pose
-// kernel's build, a thread per pose, component-major); each cluster is one pass over the
+// kernel's build, a thread per pose, tiles of 256 poses); each cluster is one pass over the
 // poses (a thread per pose) against its head's coordinates in LDS, which
 // also finds the next head (the first pose it leaves unclustered); a small
 // kernel folds the pass's member terms in block order and starts the next
@@ -39,8 +39,8 @@ constexpr int CL_BLOCK = 256;
 constexpr int NO_HEAD = 0x7fffffff;
 
 struct ClDev {
-    int n, n_atoms, ld, max_results;
-    const float* xyzT;        // [3 * n_atoms][ld]
+    int n, n_atoms, max_results;
+    const float* xyzT;        // 3 * n_atoms rows per pose, tiled (coord_at)
     const double* energy;     // [n]
     double T, beta;           // temperature (K) and 1 / T; T = 0: plain energies
     float rmsd;
@@ -67,7 +67,7 @@ __device__ __forceinline__ float sum_sq_avx2(const ClDev& c, const float* __rest
     for (; i + 7 < nf; i += 8) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const float d = c.xyzT[(size_t)(i + q) * c.ld + pose] - hx[i + q];
+            const float d = c.xyzT[coord_at(nf, i + q, pose)] - hx[i + q];
             acc[q] = __builtin_fmaf(d, d, acc[q]);
         }
     }
@@ -76,7 +76,7 @@ __device__ __forceinline__ float sum_sq_avx2(const ClDev& c, const float* __rest
     const float t0 = s0 + s2, t1 = s1 + s3;
     float sum = t0 + t1;
     for (; i < nf; ++i) {
-        const float d = c.xyzT[(size_t)i * c.ld + pose] - hx[i];
+        const float d = c.xyzT[coord_at(nf, i, pose)] - hx[i];
         sum = __builtin_fmaf(d, d, sum);
     }
     return sum;
@@ -143,11 +143,13 @@ __global__ __launch_bounds__(CL_BLOCK) void cl_assign_kernel(ClDev c, int r)
     const int j = c.head[r];
     if (j == NO_HEAD) return;  // the whole grid: no cluster left
     const int nf = 3 * c.n_atoms;
-    for (int k = threadIdx.x; k < nf; k += CL_BLOCK) hx[k] = c.xyzT[(size_t)k * c.ld + j];
+    for (int k = threadIdx.x; k < nf; k += CL_BLOCK) hx[k] = c.xyzT[coord_at(nf, k, j)];
     __syncthreads();
+    __shared__ int wfirst[CL_BLOCK / 64];
     const int i = blockIdx.x * CL_BLOCK + threadIdx.x;
     double v = 0.0;
     int joined = 0;
+    bool left = false;  // unclustered after this pass
     if (i > j && i < c.n && c.member[i] == -1) {
         const float d = sum_sq_avx2(c, hx, i);
         const float rm = sqrtf(d / (float)c.n_atoms);
@@ -162,12 +164,21 @@ __global__ __launch_bounds__(CL_BLOCK) void cl_assign_kernel(ClDev c, int r)
             }
             joined = 1;
         } else {
-            atomicMin(c.next, i);
+            left = true;
         }
     }
+    // the first pose of the block left unclustered (poses increase with
+    // the thread), one atomic per block for the next head
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(left);
+    if ((threadIdx.x & 63) == 0) wfirst[threadIdx.x >> 6] = m ? (int)(blockIdx.x * CL_BLOCK + (threadIdx.x & ~63) + __ffsll((long long)m) - 1) : NO_HEAD;
     sh[threadIdx.x] = v;
     shn[threadIdx.x] = joined;
     __syncthreads();
+    if (threadIdx.x == 0) {
+        int f = NO_HEAD;
+        for (int w = 0; w < CL_BLOCK / 64; ++w) f = min(f, wfirst[w]);
+        if (f != NO_HEAD) atomicMin(c.next, f);
+    }
     for (int w = CL_BLOCK / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             sh[threadIdx.x] += sh[threadIdx.x + w];
@@ -301,10 +312,11 @@ __global__ void cl_top_rmsd_kernel(ClDev c, int n_out, float* out)
     }
     const int pa = c.top[a], pb = c.top[b];
     float s = 0.f;
+    const int nf = 3 * c.n_atoms;
     for (int i = 0; i < c.n_atoms; ++i) {
-        const float d0 = c.xyzT[(size_t)(3 * i) * c.ld + pa] - c.xyzT[(size_t)(3 * i) * c.ld + pb];
-        const float d1 = c.xyzT[(size_t)(3 * i + 1) * c.ld + pa] - c.xyzT[(size_t)(3 * i + 1) * c.ld + pb];
-        const float d2 = c.xyzT[(size_t)(3 * i + 2) * c.ld + pa] - c.xyzT[(size_t)(3 * i + 2) * c.ld + pb];
+        const float d0 = c.xyzT[coord_at(nf, 3 * i, pa)] - c.xyzT[coord_at(nf, 3 * i, pb)];
+        const float d1 = c.xyzT[coord_at(nf, 3 * i + 1, pa)] - c.xyzT[coord_at(nf, 3 * i + 1, pb)];
+        const float d2 = c.xyzT[coord_at(nf, 3 * i + 2, pa)] - c.xyzT[coord_at(nf, 3 * i + 2, pb)];
         s += d0 * d0 + d1 * d1 + d2 * d2;
     }
     out[t] = (float)sqrt((double)(s / (float)c.n_atoms));
@@ -333,7 +345,7 @@ int fa_cluster_cf(fa_ctx* ctx, int ligand, int n_poses, const double* genes, int
     const int n = n_poses, ld = std::max(n, 1);
     const int nb = std::max(1, (n + fa::CL_BLOCK - 1) / fa::CL_BLOCK);
     // one allocation: coordinates, genes, energies, per-pose and per-block arrays, records
-    const size_t b_xyz = sizeof(float) * 3 * L * (size_t)ld, b_ic = b_xyz, b_gen = sizeof(double) * G * (size_t)ld,
+    const size_t b_xyz = sizeof(float) * fa::coord_floats(3 * L, std::max(n, 1)), b_ic = b_xyz, b_gen = sizeof(double) * G * (size_t)ld,
                  b_e = sizeof(double) * (size_t)ld, b_pose = (sizeof(int) + sizeof(float)) * (size_t)ld,
                  b_blk = (sizeof(double) + sizeof(int)) * (size_t)nb, b_rec = (2 * sizeof(int) + 2 * sizeof(double)) * R +
                                                                          sizeof(int) * (R + 4) + sizeof(double),
@@ -361,7 +373,6 @@ int fa_cluster_cf(fa_ctx* ctx, int ligand, int n_poses, const double* genes, int
     fa::ClDev c{};
     c.n = n;
     c.n_atoms = L;
-    c.ld = ld;
     c.max_results = R;
     c.xyzT = xyzT;
     c.energy = d_e;
@@ -391,7 +402,7 @@ int fa_cluster_cf(fa_ctx* ctx, int ligand, int n_poses, const double* genes, int
     FA_HIP(hipEventCreate(&e0));
     FA_HIP(hipEventCreate(&e1));
     FA_HIP(hipEventRecord(e0, s));
-    FA_HIP(fa::launch_coords(ctx->d_ligs, ligand, d_genes, G, n, xyzT, icT, ld, s));
+    FA_HIP(fa::launch_coords(ctx->d_ligs, ligand, L, d_genes, G, n, xyzT, icT, s));
     const int ni = std::max(nb, (R + fa::CL_BLOCK) / fa::CL_BLOCK);
     hipLaunchKernelGGL(fa::cl_init_kernel, dim3(ni), dim3(fa::CL_BLOCK), 0, s, c);
     if (c.T > 0.0 && n > 0) {
@@ -403,7 +414,7 @@ int fa_cluster_cf(fa_ctx* ctx, int ligand, int n_poses, const double* genes, int
         hipLaunchKernelGGL(fa::cl_finish_kernel, dim3(1), dim3(fa::CL_BLOCK), 0, s, c, r, nb);
     }
     hipLaunchKernelGGL(fa::cl_sort_kernel, dim3(1), dim3(64), 0, s, c);
-    hipLaunchKernelGGL(fa::cl_top_rmsd_kernel, dim3((R * R + 255) / 256), dim3(256), 0, s, c, R, d_trm);
+    if (n > 0) hipLaunchKernelGGL(fa::cl_top_rmsd_kernel, dim3((R * R + 255) / 256), dim3(256), 0, s, c, R, d_trm);
     FA_HIP(hipGetLastError());
     FA_HIP(hipEventRecord(e1, s));
     FA_HIP(hipEventSynchronize(e1));

@@ -18,8 +18,8 @@ hipError_t launch_cf(const DevReceptor& R, const DevGrid* grids, const DevLigand
                      const DevBatch& B, int n_blocks, hipStream_t stream);
 int cf_waves_per_cu();
 int smem_bytes();
-hipError_t launch_coords(const DevLigand* ligs, int ligand, const double* genes, int stride, int n, float* xyzT,
-                         float* icT, int ld, hipStream_t stream);
+hipError_t launch_coords(const DevLigand* ligs, int ligand, int n_atoms, const double* genes, int stride, int n,
+                         float* xyzT, float* icT, hipStream_t stream);
 size_t lse_scratch_bytes();
 hipError_t launch_thermo(const double* d_energy, int stride, const int* d_count, long long n, double T,
                          void* d_scratch, fa_thermo* d_out, hipStream_t stream);

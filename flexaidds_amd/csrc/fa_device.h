@@ -129,4 +129,18 @@ struct DevBatch {
     int cell_lists;                 // 1: scan the cell lists even where neighbourhood lists exist
 };
 
+// Per-pose rows (coordinates, internal coordinates) of many poses, tiled for
+// streaming: tiles of COORD_TILE poses, each tile holding its poses' row k
+// contiguously (row k of pose p at coord_at(rows, k, p)), so a block of
+// COORD_TILE threads reads and writes one contiguous region.
+constexpr int COORD_TILE = 256;
+__host__ __device__ __forceinline__ size_t coord_at(int rows, int k, int pose)
+{
+    return ((size_t)(pose / COORD_TILE) * rows + k) * COORD_TILE + (size_t)(pose % COORD_TILE);
+}
+__host__ __device__ __forceinline__ size_t coord_floats(int rows, int n)
+{
+    return (size_t)((n + COORD_TILE - 1) / COORD_TILE) * rows * COORD_TILE;
+}
+
 }  // namespace fa
