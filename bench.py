@@ -248,7 +248,9 @@ def ga_dock_leg(ctx, system, args):
     wall = time.perf_counter() - t0
     ga.close()
     poses = args.pop * args.ga_generations
+    w, f = ctx.launch_shape()
     out = {"population": args.pop, "generations": args.ga_generations, "poses": int(n),
+           "launch_shape": {"waves": w, "flags": f},
            "generations_ms": round(ms, 2), "ms_per_generation": round(ms / args.ga_generations, 4),
            "poses_per_s": round(poses / (ms / 1e3), 1), "dock_wall_s": round(wall, 3),
            "best_cf": round(float(res["cf"][0]), 6), "free_energy_300K": round(th["free_energy"], 6),

@@ -37,10 +37,15 @@ def candidates(ctx, fractions=(1.0, 0.75, 0.5), flag_sets=(0, abi.FA_SHAPE_CELL_
     return out
 
 
-def tune(ctx, run, reps: int = 3, shapes=None, sync=None) -> dict:
+def tune(ctx, run, reps: int = 3, shapes=None, sync=None, tie: float = 0.03) -> dict:
     """Time `run()` under each launch shape (one warm-up, then the best of
     `reps` runs), set the fastest on `ctx` and return
     {"waves", "flags", "ms", "table": [{"waves", "flags", "ms"}, ...]}.
+    Shapes within `tie` (relative) of the fastest count as equal and the
+    first of them in candidate order (most waves first) is taken: a wave cap
+    below the launch's pose count only pays when it is clearly faster, since
+    the same launch on a later, costlier population (a GA's converged
+    children, fewer cheap clash exits) then runs in two rounds.
     `sync` (default: torch.cuda.synchronize when torch is present) waits for
     the device before and after each timed run."""
     if sync is None:
@@ -62,6 +67,7 @@ def tune(ctx, run, reps: int = 3, shapes=None, sync=None) -> dict:
             sync()
             best = min(best, time.perf_counter() - t0)
         table.append({"waves": ctx.launch_shape()[0], "flags": flags, "ms": round(best * 1e3, 4)})
-    win = min(table, key=lambda r: r["ms"])
+    fastest = min(r["ms"] for r in table)
+    win = next(r for r in table if r["ms"] <= fastest * (1.0 + tie))
     ctx.set_launch_shape(win["waves"], win["flags"])
     return {"waves": win["waves"], "flags": win["flags"], "ms": win["ms"], "table": table}

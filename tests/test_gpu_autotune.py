@@ -61,7 +61,10 @@ def test_tune_sets_the_fastest_shape(hip):
         c.upload_system(s)
         res = autotune.tune(c, lambda: c.eval(0, g), reps=2)
         assert len(res["table"]) == 6
-        assert res["ms"] == min(r["ms"] for r in res["table"])
+        # the first shape (most waves first) within 3% of the fastest
+        fastest = min(r["ms"] for r in res["table"])
+        first = next(r for r in res["table"] if r["ms"] <= fastest * 1.03)
+        assert (res["waves"], res["flags"], res["ms"]) == (first["waves"], first["flags"], first["ms"])
         assert c.launch_shape() == (res["waves"], res["flags"])
         full = c.launch_waves()
         assert 0 < res["waves"] <= full
