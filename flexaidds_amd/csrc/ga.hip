@@ -24,6 +24,7 @@ struct fa_ga {
     fa::GaDev d{};
     DevBuf rank, gi, gic, res, gi2, gic2, res2, cum, lim, ngenes, ens, cnt, lig, thermo, grand, fit, fit2, share, sig;
     DevBuf dset, dzero, rkey, rval, rzero, csig, cdup, need, ddstats;  // duplicate rejection
+    DevBuf skey, sidx;  // survivor ranking: candidates sorted within tiles
     long long gens_run = 0;   // generations enqueued (bounds the signatures accepted)
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -174,7 +175,8 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
         return code;
     };
     int rc;
-    if ((rc = fa_ensure(ga->rank, sizeof(int) * J * n_all)) || (rc = fa_ensure(ga->gi, sizeof(int32_t) * rows_all * Gmax)) ||
+    if ((rc = fa_ensure(ga->rank, sizeof(int) * J * n_all)) || (rc = fa_ensure(ga->skey, sizeof(unsigned long long) * J * n_all)) ||
+        (rc = fa_ensure(ga->sidx, sizeof(int) * J * n_all)) || (rc = fa_ensure(ga->gi, sizeof(int32_t) * rows_all * Gmax)) ||
         (rc = fa_ensure(ga->gic, sizeof(double) * rows_all * Gmax)) ||
         (rc = fa_ensure(ga->res, sizeof(fa_cf_result) * J * n_all)) ||
         (rc = fa_ensure(ga->gi2, sizeof(int32_t) * J * pop * Gmax)) ||
@@ -235,6 +237,8 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     d.ldel = lp + plane;
     d.lbin = lp + 2 * plane;
     d.rank = static_cast<int*>(ga->rank.p);
+    d.skey = static_cast<unsigned long long*>(ga->skey.p);
+    d.sidx = static_cast<int*>(ga->sidx.p);
     d.gi = static_cast<int32_t*>(ga->gi.p);
     d.gic = static_cast<double*>(ga->gic.p);
     d.res = static_cast<fa_cf_result*>(ga->res.p);
@@ -450,7 +454,7 @@ int fa_ga_destroy(fa_ga* ga)
     for (DevBuf* b : {&ga->rank, &ga->gi, &ga->gic, &ga->res, &ga->gi2, &ga->gic2, &ga->res2, &ga->cum, &ga->lim, &ga->ngenes,
                       &ga->ens, &ga->cnt, &ga->lig, &ga->thermo, &ga->grand, &ga->fit, &ga->fit2, &ga->share,
                       &ga->sig, &ga->dset, &ga->dzero, &ga->rkey, &ga->rval, &ga->rzero, &ga->csig, &ga->cdup,
-                      &ga->need, &ga->ddstats})
+                      &ga->need, &ga->ddstats, &ga->skey, &ga->sidx})
         if (b->p) (void)hipFree(b->p);
     if (ga->stream) (void)hipStreamDestroy(ga->stream);
     --ga->ctx->live_users;

@@ -39,6 +39,8 @@ struct GaDev {
     double* gic;                   // internal coordinates, same rows
     fa_cf_result* res;             // energies of every row
     int* rank;                     // [jobs * n_all] candidate ranks (by launch position)
+    unsigned long long* skey;      // [jobs * n_all] energy keys, sorted within tiles of RANK_TILE
+    int* sidx;                     // [jobs * n_all] their candidate indices
     int32_t* gi2;                  // [jobs * pop * Gmax] survivors being gathered
     double* gic2;
     fa_cf_result* res2;

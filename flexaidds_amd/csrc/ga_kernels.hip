@@ -306,54 +306,120 @@ __device__ __forceinline__ size_t cand_row(const GaDev& g, int a, int k, int r, 
 
 // Rank of each of a job's n candidates by energy, ties by candidate index
 // (a total order, so ranks are a permutation); the best `pop` are gathered
-// in order (QuickSort by evalue, LIB/gaboom.cpp:2674-2726).  The counting is
-// split over candidate chunks so that one job's ranking spreads over many
-// blocks: zero, count (integer atomics: order-free), gather.
-constexpr int RANK_CHUNK = 256;  // candidates compared per block (more blocks in flight)
+// in order (QuickSort by evalue, LIB/gaboom.cpp:2674-2726).  Ranking by
+// sorted tiles: every tile of RANK_TILE candidates is sorted in LDS
+// (bitonic, by (energy key, index)); a candidate's rank is the number of
+// candidates before it in each sorted tile, found by binary search with the
+// tile staged in LDS.  n (n / RANK_TILE) log2(RANK_TILE) steps instead of n^2
+// comparisons; exact, no atomics.
+constexpr int RANK_TILE = 1024;
+constexpr int RANK_SORT_THREADS = RANK_TILE / 2;
 
-__global__ void ga_rank_zero_kernel(GaDev g, int n)
+__global__ __launch_bounds__(RANK_SORT_THREADS) void ga_rank_sort_kernel(GaDev g, int initial)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < (long long)g.n_act * n) g.rank[(t / n) * (size_t)g.n_all + (t % n)] = 0;
-}
-
-__global__ __launch_bounds__(256) void ga_rank_count_kernel(GaDev g, int initial)
-{
-    __shared__ uint64_t keys[RANK_CHUNK];
-    const int a = blockIdx.z;
+    __shared__ unsigned long long sk[RANK_TILE];
+    __shared__ int si[RANK_TILE];
+    const int a = blockIdx.y;
     const int k = slot_of(g, a);
     const int n = initial ? g.pop : g.n_all;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int base = blockIdx.y * RANK_CHUNK;
-    const int lim = min(RANK_CHUNK, n - base);
-    for (int q = threadIdx.x; q < lim; q += blockDim.x) keys[q] = order_key(g.res[cand_row(g, a, k, base + q, initial)].cf);
-    __syncthreads();
-    if (i >= n) return;
-    const uint64_t ki = order_key(g.res[cand_row(g, a, k, i, initial)].cf);
-    int cnt = 0;
-#pragma unroll 8
-    for (int q = 0; q < lim; ++q) {
-        const uint64_t kj = keys[q];
-        cnt += (kj < ki) || (kj == ki && base + q < i);
+    const int base = blockIdx.x * RANK_TILE;
+    const int m = min(RANK_TILE, n - base);
+    for (int q = threadIdx.x; q < RANK_TILE; q += RANK_SORT_THREADS) {
+        if (q < m) {
+            sk[q] = order_key(g.res[cand_row(g, a, k, base + q, initial)].cf);
+            si[q] = base + q;
+        } else {  // padding sorts last
+            sk[q] = ~0ull;
+            si[q] = 0x7fffffff;
+        }
     }
-    if (cnt) atomicAdd(&g.rank[(size_t)a * g.n_all + i], cnt);
+    __syncthreads();
+    const int q = threadIdx.x;  // one compare-exchange per thread per stage
+    for (int size = 2; size <= RANK_TILE; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const int lo = 2 * stride * (q / stride) + (q % stride), hi = lo + stride;
+            const bool up = (lo & size) == 0;
+            const unsigned long long ka = sk[lo], kb = sk[hi];
+            const int ia = si[lo], ib = si[hi];
+            const bool gt = (ka > kb) || (ka == kb && ia > ib);
+            if (gt == up) {
+                sk[lo] = kb;
+                sk[hi] = ka;
+                si[lo] = ib;
+                si[hi] = ia;
+            }
+            __syncthreads();
+        }
+    }
+    const size_t out = (size_t)a * g.n_all + base;
+    for (int p = threadIdx.x; p < m; p += RANK_SORT_THREADS) {
+        g.skey[out + p] = sk[p];
+        g.sidx[out + p] = si[p];
+    }
 }
 
+// four candidates per thread (one tile per block) against every sorted tile
+__global__ __launch_bounds__(256) void ga_rank_search_kernel(GaDev g, int initial)
+{
+    __shared__ unsigned long long sk[RANK_TILE];
+    __shared__ int si[RANK_TILE];
+    const int a = blockIdx.y;
+    const int k = slot_of(g, a);
+    const int n = initial ? g.pop : g.n_all;
+    const int tiles = (n + RANK_TILE - 1) / RANK_TILE;
+    unsigned long long key[RANK_TILE / 256];
+    int idx[RANK_TILE / 256], rank[RANK_TILE / 256];
+#pragma unroll
+    for (int r = 0; r < RANK_TILE / 256; ++r) {
+        idx[r] = blockIdx.x * RANK_TILE + r * 256 + (int)threadIdx.x;
+        key[r] = idx[r] < n ? order_key(g.res[cand_row(g, a, k, idx[r], initial)].cf) : 0ull;
+        rank[r] = 0;
+    }
+    const size_t row = (size_t)a * g.n_all;
+    for (int c = 0; c < tiles; ++c) {
+        const int m = min(RANK_TILE, n - c * RANK_TILE);
+        __syncthreads();
+        for (int q = threadIdx.x; q < m; q += 256) {
+            sk[q] = g.skey[row + (size_t)c * RANK_TILE + q];
+            si[q] = g.sidx[row + (size_t)c * RANK_TILE + q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RANK_TILE / 256; ++r) {
+            if (idx[r] >= n) continue;
+            int lo = 0, hi = m;  // first entry not before (key, idx)
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                const bool before = sk[mid] < key[r] || (sk[mid] == key[r] && si[mid] < idx[r]);
+                if (before)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            rank[r] += lo;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RANK_TILE / 256; ++r)
+        if (idx[r] < n) g.rank[row + idx[r]] = rank[r];
+}
+
+// one thread per (candidate, gene): a survivor's row moves in consecutive words
 __global__ void ga_rank_gather_kernel(GaDev g, int initial)
 {
     const int n = initial ? g.pop : g.n_all;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)g.n_act * n) return;
-    const int a = (int)(t / n), i = (int)(t % n);
+    if (t >= (long long)g.n_act * n * g.Gmax) return;
+    const long long c = t / g.Gmax;
+    const int j = (int)(t - c * g.Gmax);
+    const int a = (int)(c / n), i = (int)(c % n);
     const int k = slot_of(g, a);
     const int rank = g.rank[(size_t)a * g.n_all + i];
     if (rank >= g.pop) return;
     const size_t src = cand_row(g, a, k, i, initial), dst = parent_row(g, k, rank);
-    for (int j = 0; j < g.Gmax; ++j) {
-        g.gi2[dst * g.Gmax + j] = g.gi[src * g.Gmax + j];
-        g.gic2[dst * g.Gmax + j] = g.gic[src * g.Gmax + j];
-    }
-    g.res2[dst] = g.res[src];
+    g.gi2[dst * g.Gmax + j] = g.gi[src * g.Gmax + j];
+    g.gic2[dst * g.Gmax + j] = g.gic[src * g.Gmax + j];
+    if (j == 0) g.res2[dst] = g.res[src];
 }
 
 // gathered survivors (and, with fit_too, their fitness) into the parent rows
@@ -858,10 +924,10 @@ void ga_launch_select(const GaDev& g, int initial, hipStream_t s)
     const int n = initial ? g.pop : g.n_all;
     const unsigned rows = blocks_of((long long)g.n_act * g.pop * g.Gmax);
     const long long cand = (long long)g.n_act * n;
-    hipLaunchKernelGGL(ga_rank_zero_kernel, dim3(blocks_of(cand)), dim3(256), 0, s, g, n);
-    hipLaunchKernelGGL(ga_rank_count_kernel, dim3(blocks_of(n), (n + RANK_CHUNK - 1) / RANK_CHUNK, g.n_act), dim3(256),
-                       0, s, g, initial);
-    hipLaunchKernelGGL(ga_rank_gather_kernel, dim3(blocks_of(cand)), dim3(256), 0, s, g, initial);
+    const unsigned tiles = (unsigned)((n + RANK_TILE - 1) / RANK_TILE);
+    hipLaunchKernelGGL(ga_rank_sort_kernel, dim3(tiles, g.n_act), dim3(RANK_SORT_THREADS), 0, s, g, initial);
+    hipLaunchKernelGGL(ga_rank_search_kernel, dim3(tiles, g.n_act), dim3(256), 0, s, g, initial);
+    hipLaunchKernelGGL(ga_rank_gather_kernel, dim3(blocks_of(cand * g.Gmax)), dim3(256), 0, s, g, initial);
     hipLaunchKernelGGL(ga_copy_back_kernel, dim3(rows), dim3(256), 0, s, g, 0);
     if (g.model == FA_FIT_LINEAR) {
         hipLaunchKernelGGL(ga_linear_kernel, dim3(g.n_act), dim3(256), 0, s, g);

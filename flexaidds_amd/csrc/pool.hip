@@ -157,6 +157,7 @@ struct fa_pool {
     int slots = 0, Gmax = 0;
     size_t slot_bytes = 0;
     fa::GaDev d{};
+    DevBuf skey, sidx;  // survivor ranking: candidates sorted within tiles
     DevBuf rank, gi, gic, res, gi2, gic2, res2, cum, fit, fit2, share, sig, lim, ngenes, seeds, gens, ens, cnt, slot_lig,
         lig_rows, run, adm, admrec, done, out;
     hipStream_t stream = nullptr;
@@ -218,7 +219,8 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
     // arrays, limits and small per-slot values
     const size_t row = (size_t)Gmax * (sizeof(int32_t) + sizeof(double)) + sizeof(fa_cf_result);
     const size_t slot_bytes = row * (pop + nn + pop) + row * pop + 4 * sizeof(double) * pop +
-                              3 * sizeof(double) * Gmax + 2 * sizeof(int) * (nn + pop) + 256;
+                              3 * sizeof(double) * Gmax + 2 * sizeof(int) * (nn + pop) +
+                              (sizeof(unsigned long long) + sizeof(int)) * (nn + pop) + 256;
     long long slots = std::min<long long>(p->max_slots, (long long)(p->hbm_budget / slot_bytes));
     if (slots < 1) return fa_fail(FA_ERR_LIMIT, "HBM budget below one job's state");
     if ((long long)slots * (long long)(pop + nn + pop) > (1LL << 30)) return fa_fail(FA_ERR_LIMIT, "pool rows above 2^30");
@@ -238,7 +240,9 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
         return code;
     };
     int rc;
-    if ((rc = fa_ensure(pl->rank, sizeof(int) * S * (pop + nn))) || (rc = fa_ensure(pl->gi, sizeof(int32_t) * rows * Gmax)) || (rc = fa_ensure(pl->gic, sizeof(double) * rows * Gmax)) ||
+    if ((rc = fa_ensure(pl->rank, sizeof(int) * S * (pop + nn))) ||
+        (rc = fa_ensure(pl->skey, sizeof(unsigned long long) * S * (pop + nn))) ||
+        (rc = fa_ensure(pl->sidx, sizeof(int) * S * (pop + nn))) || (rc = fa_ensure(pl->gi, sizeof(int32_t) * rows * Gmax)) || (rc = fa_ensure(pl->gic, sizeof(double) * rows * Gmax)) ||
         (rc = fa_ensure(pl->res, sizeof(fa_cf_result) * rows)) || (rc = fa_ensure(pl->gi2, sizeof(int32_t) * S * pop * Gmax)) ||
         (rc = fa_ensure(pl->gic2, sizeof(double) * S * pop * Gmax)) || (rc = fa_ensure(pl->res2, sizeof(fa_cf_result) * S * pop)) ||
         (rc = fa_ensure(pl->cum, sizeof(double) * S * pop)) || (rc = fa_ensure(pl->fit, sizeof(double) * S * pop)) ||
@@ -279,6 +283,8 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
     d.ldel = lp + S * Gmax;
     d.lbin = lp + 2 * S * Gmax;
     d.rank = static_cast<int*>(pl->rank.p);
+    d.skey = static_cast<unsigned long long*>(pl->skey.p);
+    d.sidx = static_cast<int*>(pl->sidx.p);
     d.gi = static_cast<int32_t*>(pl->gi.p);
     d.gic = static_cast<double*>(pl->gic.p);
     d.res = static_cast<fa_cf_result*>(pl->res.p);
@@ -531,7 +537,7 @@ int fa_pool_destroy(fa_pool* pl)
     if (!pl) return FA_SUCCESS;
     (void)hipSetDevice(pl->ctx->device);
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    for (DevBuf* b : {&pl->rank, &pl->gi, &pl->gic, &pl->res, &pl->gi2, &pl->gic2, &pl->res2, &pl->cum, &pl->fit, &pl->fit2,
+    for (DevBuf* b : {&pl->skey, &pl->sidx, &pl->rank, &pl->gi, &pl->gic, &pl->res, &pl->gi2, &pl->gic2, &pl->res2, &pl->cum, &pl->fit, &pl->fit2,
                       &pl->share, &pl->sig, &pl->lim, &pl->ngenes, &pl->seeds, &pl->gens, &pl->ens, &pl->cnt,
                       &pl->slot_lig, &pl->lig_rows, &pl->run, &pl->adm, &pl->admrec, &pl->done, &pl->out})
         if (b->p) (void)hipFree(b->p);
