@@ -25,6 +25,7 @@ struct fa_ga {
     DevBuf rank, gi, gic, res, gi2, gic2, res2, cum, lim, ngenes, ens, cnt, lig, thermo, grand, fit, fit2, share, sig;
     DevBuf dset, dzero, rkey, rval, rzero, csig, cdup, need, ddstats;  // duplicate rejection
     DevBuf skey, sidx;  // survivor ranking: candidates sorted within tiles
+    DevBuf prank;       // STEADY: parent ranks among the parents
     long long gens_run = 0;   // generations enqueued (bounds the signatures accepted)
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -120,6 +121,10 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     const long long pop = p->population, n_new = p->offspring;
     if ((long long)n_jobs * (pop + n_new) > (1LL << 30) || pop + n_new > (1LL << 30))
         return fa_fail(FA_ERR_LIMIT, "jobs * (population + offspring) above 2^30");
+    if (p->reproduction != FA_REP_BOOM && p->reproduction != FA_REP_STEADY)
+        return fa_fail(FA_ERR_INVALID, "unknown reproduction model");
+    if (p->reproduction == FA_REP_STEADY && p->offspring > p->population)
+        return fa_fail(FA_ERR_INVALID, "STEADY reproduction needs offspring <= population");
     const int R = p->dedup_rounds;
     if (R < 0 || R > 1024) return fa_fail(FA_ERR_INVALID, "dedup_rounds in [0, 1024]");
     // candidate rows per job (a batch of initial chromosomes or of children)
@@ -189,6 +194,7 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
         (rc = fa_ensure(ga->lig, sizeof(int) * J * n_all)) || (rc = fa_ensure(ga->thermo, sizeof(fa_thermo) * J)) ||
         (rc = fa_ensure(ga->grand, sizeof(double) * (2 * J + 2))))
         return fail_free(rc);
+    if (p->reproduction == FA_REP_STEADY && (rc = fa_ensure(ga->prank, sizeof(int) * J * pop))) return fail_free(rc);
     const int rlog2 = R > 0 ? ceil_log2(2 * n_cand) : 0;
     if (R > 0 &&
         ((rc = fa_ensure(ga->dzero, sizeof(int) * J)) || (rc = fa_ensure(ga->rkey, sizeof(unsigned long long) * (J << rlog2))) ||
@@ -239,6 +245,8 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     d.rank = static_cast<int*>(ga->rank.p);
     d.skey = static_cast<unsigned long long*>(ga->skey.p);
     d.sidx = static_cast<int*>(ga->sidx.p);
+    d.steady = p->reproduction == FA_REP_STEADY ? 1 : 0;
+    d.prank = static_cast<int*>(ga->prank.p);
     d.gi = static_cast<int32_t*>(ga->gi.p);
     d.gic = static_cast<double*>(ga->gic.p);
     d.res = static_cast<fa_cf_result*>(ga->res.p);
@@ -454,7 +462,7 @@ int fa_ga_destroy(fa_ga* ga)
     for (DevBuf* b : {&ga->rank, &ga->gi, &ga->gic, &ga->res, &ga->gi2, &ga->gic2, &ga->res2, &ga->cum, &ga->lim, &ga->ngenes,
                       &ga->ens, &ga->cnt, &ga->lig, &ga->thermo, &ga->grand, &ga->fit, &ga->fit2, &ga->share,
                       &ga->sig, &ga->dset, &ga->dzero, &ga->rkey, &ga->rval, &ga->rzero, &ga->csig, &ga->cdup,
-                      &ga->need, &ga->ddstats, &ga->skey, &ga->sidx})
+                      &ga->need, &ga->ddstats, &ga->skey, &ga->sidx, &ga->prank})
         if (b->p) (void)hipFree(b->p);
     if (ga->stream) (void)hipStreamDestroy(ga->stream);
     --ga->ctx->live_users;

@@ -41,6 +41,8 @@ struct GaDev {
     int* rank;                     // [jobs * n_all] candidate ranks (by launch position)
     unsigned long long* skey;      // [jobs * n_all] energy keys, sorted within tiles of RANK_TILE
     int* sidx;                     // [jobs * n_all] their candidate indices
+    int steady;                    // 1: STEADY reproduction (the worst offspring-many parents replaced)
+    int* prank;                    // [jobs * pop] parent ranks among the parents (steady)
     int32_t* gi2;                  // [jobs * pop * Gmax] survivors being gathered
     double* gic2;
     fa_cf_result* res2;

@@ -304,6 +304,22 @@ __device__ __forceinline__ size_t cand_row(const GaDev& g, int a, int k, int r, 
     return r < g.pop ? parent_row(g, k, r) : child_row(g, a, r - g.pop);
 }
 
+// What a ranking launch ranks: every candidate (parents then children), the
+// staged initial population, or the parents alone (STEADY's first step)
+enum RankMode { RANK_ALL = 0, RANK_INITIAL = 1, RANK_PARENTS = 2 };
+
+__device__ __forceinline__ int rank_n(const GaDev& g, int mode) { return mode == RANK_ALL ? g.n_all : g.pop; }
+
+// the key a candidate is ranked by: its energy's order key; under STEADY
+// the parents beyond the best pop - offspring (by their rank among the
+// parents) rank after every other candidate, so they never survive
+__device__ __forceinline__ uint64_t rank_key(const GaDev& g, int a, int k, int r, int mode)
+{
+    const uint64_t e = order_key(g.res[cand_row(g, a, k, r, mode == RANK_INITIAL)].cf);
+    if (mode == RANK_ALL && g.steady && r < g.pop && g.prank[(size_t)a * g.pop + r] >= g.pop - g.n_new) return ~0ull;
+    return e;
+}
+
 // Rank of each of a job's n candidates by energy, ties by candidate index
 // (a total order, so ranks are a permutation); the best `pop` are gathered
 // in order (QuickSort by evalue, LIB/gaboom.cpp:2674-2726).  Ranking by
@@ -315,18 +331,18 @@ __device__ __forceinline__ size_t cand_row(const GaDev& g, int a, int k, int r, 
 constexpr int RANK_TILE = 1024;
 constexpr int RANK_SORT_THREADS = RANK_TILE / 2;
 
-__global__ __launch_bounds__(RANK_SORT_THREADS) void ga_rank_sort_kernel(GaDev g, int initial)
+__global__ __launch_bounds__(RANK_SORT_THREADS) void ga_rank_sort_kernel(GaDev g, int mode)
 {
     __shared__ unsigned long long sk[RANK_TILE];
     __shared__ int si[RANK_TILE];
     const int a = blockIdx.y;
     const int k = slot_of(g, a);
-    const int n = initial ? g.pop : g.n_all;
+    const int n = rank_n(g, mode);
     const int base = blockIdx.x * RANK_TILE;
     const int m = min(RANK_TILE, n - base);
     for (int q = threadIdx.x; q < RANK_TILE; q += RANK_SORT_THREADS) {
         if (q < m) {
-            sk[q] = order_key(g.res[cand_row(g, a, k, base + q, initial)].cf);
+            sk[q] = rank_key(g, a, k, base + q, mode);
             si[q] = base + q;
         } else {  // padding sorts last
             sk[q] = ~0ull;
@@ -359,20 +375,20 @@ __global__ __launch_bounds__(RANK_SORT_THREADS) void ga_rank_sort_kernel(GaDev g
 }
 
 // four candidates per thread (one tile per block) against every sorted tile
-__global__ __launch_bounds__(256) void ga_rank_search_kernel(GaDev g, int initial)
+__global__ __launch_bounds__(256) void ga_rank_search_kernel(GaDev g, int mode)
 {
     __shared__ unsigned long long sk[RANK_TILE];
     __shared__ int si[RANK_TILE];
     const int a = blockIdx.y;
     const int k = slot_of(g, a);
-    const int n = initial ? g.pop : g.n_all;
+    const int n = rank_n(g, mode);
     const int tiles = (n + RANK_TILE - 1) / RANK_TILE;
     unsigned long long key[RANK_TILE / 256];
     int idx[RANK_TILE / 256], rank[RANK_TILE / 256];
 #pragma unroll
     for (int r = 0; r < RANK_TILE / 256; ++r) {
         idx[r] = blockIdx.x * RANK_TILE + r * 256 + (int)threadIdx.x;
-        key[r] = idx[r] < n ? order_key(g.res[cand_row(g, a, k, idx[r], initial)].cf) : 0ull;
+        key[r] = idx[r] < n ? rank_key(g, a, k, idx[r], mode) : 0ull;
         rank[r] = 0;
     }
     const size_t row = (size_t)a * g.n_all;
@@ -399,9 +415,10 @@ __global__ __launch_bounds__(256) void ga_rank_search_kernel(GaDev g, int initia
             rank[r] += lo;
         }
     }
+    int* out = mode == RANK_PARENTS ? g.prank + (size_t)a * g.pop : g.rank + row;
 #pragma unroll
     for (int r = 0; r < RANK_TILE / 256; ++r)
-        if (idx[r] < n) g.rank[row + idx[r]] = rank[r];
+        if (idx[r] < n) out[idx[r]] = rank[r];
 }
 
 // one thread per (candidate, gene): a survivor's row moves in consecutive words
@@ -924,9 +941,16 @@ void ga_launch_select(const GaDev& g, int initial, hipStream_t s)
     const int n = initial ? g.pop : g.n_all;
     const unsigned rows = blocks_of((long long)g.n_act * g.pop * g.Gmax);
     const long long cand = (long long)g.n_act * n;
-    const unsigned tiles = (unsigned)((n + RANK_TILE - 1) / RANK_TILE);
-    hipLaunchKernelGGL(ga_rank_sort_kernel, dim3(tiles, g.n_act), dim3(RANK_SORT_THREADS), 0, s, g, initial);
-    hipLaunchKernelGGL(ga_rank_search_kernel, dim3(tiles, g.n_act), dim3(256), 0, s, g, initial);
+    auto rank = [&](int mode) {
+        const int m = mode == RANK_ALL ? g.n_all : g.pop;
+        const unsigned tiles = (unsigned)((m + RANK_TILE - 1) / RANK_TILE);
+        hipLaunchKernelGGL(ga_rank_sort_kernel, dim3(tiles, g.n_act), dim3(RANK_SORT_THREADS), 0, s, g, mode);
+        hipLaunchKernelGGL(ga_rank_search_kernel, dim3(tiles, g.n_act), dim3(256), 0, s, g, mode);
+    };
+    // STEADY (LIB/gaboom.cpp:1022-1027): the parents by energy first; the
+    // worst offspring-many of them give way to the children
+    if (!initial && g.steady) rank(RANK_PARENTS);
+    rank(initial ? RANK_INITIAL : RANK_ALL);
     hipLaunchKernelGGL(ga_rank_gather_kernel, dim3(blocks_of(cand * g.Gmax)), dim3(256), 0, s, g, initial);
     hipLaunchKernelGGL(ga_copy_back_kernel, dim3(rows), dim3(256), 0, s, g, 0);
     if (g.model == FA_FIT_LINEAR) {

@@ -343,10 +343,12 @@ class GAParams(C.Structure):
                 ("intragenes", C.c_int32), ("cross_rate", C.c_double), ("mut_rate", C.c_double),
                 ("temperature", C.c_double), ("seed", C.c_uint64), ("fitness", C.c_int32), ("dedup_rounds", C.c_int32),
                 ("share_alpha", C.c_double), ("share_peaks", C.c_double), ("share_scale", C.c_double),
-                ("sig_share", C.c_double), ("entropy_weight", C.c_double)]
+                ("sig_share", C.c_double), ("entropy_weight", C.c_double), ("reproduction", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 FITNESS = {"LINEAR": 0, "PSHARE": 1, "SMFREE": 2}
+REPRODUCTION = {"BOOM": 0, "STEADY": 1}
 
 
 class GA:
@@ -359,7 +361,7 @@ class GA:
     def __init__(self, ctx: Context, limits, ligand=0, population=2000, offspring=None, cross_rate=0.8,
                  mut_rate=0.03, temperature=300.0, seed=1, intragenes=False, ligands=None, fitness="LINEAR",
                  share_alpha=1.0, share_peaks=5.0, share_scale=10.0, sig_share=0.0, entropy_weight=0.5,
-                 dedup_rounds=0):
+                 dedup_rounds=0, reproduction="BOOM"):
         self.ctx = ctx
         per_job = [limits] if ligands is None else list(limits)
         flat = [x for job in per_job for x in job]
@@ -368,7 +370,8 @@ class GA:
         model = FITNESS[fitness] if isinstance(fitness, str) else int(fitness)
         self.params = GAParams(ligand, population, population if offspring is None else offspring,
                                1 if intragenes else 0, cross_rate, mut_rate, temperature, seed, model,
-                               int(dedup_rounds), share_alpha, share_peaks, share_scale, sig_share, entropy_weight)
+                               int(dedup_rounds), share_alpha, share_peaks, share_scale, sig_share, entropy_weight,
+                               REPRODUCTION[reproduction] if isinstance(reproduction, str) else int(reproduction), 0)
         self.job_genes = [len(j) for j in per_job]
         self._ga = C.c_void_p()
         if ligands is None:

@@ -195,7 +195,8 @@ int fa_last_pose_cycles(fa_ctx* ctx, int n, uint64_t* cycles);
  * (crossover, :2517-2618), per-bit mutation at mut_rate (mutate, :2620-2641),
  * genes to internal coordinates (genetoic, :2887-2906), evaluation of the
  * offspring, survivors = best `population` of parents + offspring by energy
- * (QuickSort by evalue; ties keep index order), rank fitness, and the
+ * (QuickSort by evalue; ties keep index order; or STEADY replacement,
+ * fa_ga_params.reproduction), rank fitness, and the
  * ensemble reduction over every evaluated pose at `temperature`.
  * Random draws come from include/fa_rng.h; duplicate rejection is an option
  * (fa_ga_params.dedup_rounds).
@@ -235,7 +236,16 @@ typedef struct fa_ga_params {
     double sig_share;          /* > 0: niche radius for every job; else derived per job from its
                                   limits as LIB/gaboom.cpp:300-309 does */
     double entropy_weight;     /* SMFREE blend w (0.5) */
+    int32_t reproduction;      /* fa_reproduction (GB->rep_model) */
+    int32_t pad;
 } fa_ga_params;
+
+/* Reproduction models (GB->rep_model, LIB/gaboom.cpp:1022-1032). */
+enum fa_reproduction {
+    FA_REP_BOOM = 0,           /* survivors = the best `population` of parents + offspring */
+    FA_REP_STEADY = 1          /* the worst `offspring` parents (by energy) replaced by the
+                                  offspring, whatever their energies; needs offspring <= population */
+};
 
 typedef struct fa_ga fa_ga;
 

@@ -176,3 +176,30 @@ def test_fitness_models(hip, ctx, limits, orc, model):
         child, cic = ol.ga_breed_fit(orc, SEED + 3, gen + 1, g[:POP], f, NEW, limits)
         assert np.array_equal(g1[POP:], child), gen
     ga.close()
+
+
+def test_steady_reproduction(hip, ctx, limits, orc):
+    # STEADY (LIB/gaboom.cpp:1022-1027): the parents by energy, the worst
+    # offspring-many replaced by the children whatever their energies, the
+    # population then in energy order (ties by candidate index); children
+    # bred exactly as under BOOM
+    pop, new = 192, 64
+    ga = hip.GA(ctx, limits, population=pop, offspring=new, seed=SEED + 5, reproduction="STEADY")
+    for gen in (1, 2, 3):
+        g0, _, r0 = ga.population()
+        f, _ = ga.fitness()
+        ga.run(1, graph=(gen == 2))
+        g1, _, r1 = ga.population()
+        child, _ = ol.ga_breed_fit(orc, SEED + 5, gen, g0[:pop], f, new, limits)
+        assert np.array_equal(g1[pop:], child), gen
+        pe = r0["cf"][:pop]
+        keep = np.argsort(pe, kind="stable")[:pop - new]
+        cand_e = np.concatenate([pe[keep], r1["cf"][pop:]])
+        cand_i = np.concatenate([keep, pop + np.arange(new)])
+        order = np.lexsort((cand_i, cand_e))
+        rows = np.concatenate([g0[:pop], child])[cand_i[order]]
+        assert np.array_equal(g1[:pop], rows), gen
+        assert np.all(np.diff(r1["cf"][:pop]) >= 0), gen
+    ga.close()
+    with pytest.raises(hip.FAError):
+        hip.GA(ctx, limits, population=64, offspring=65, reproduction="STEADY")
