@@ -322,12 +322,14 @@ __device__ __forceinline__ uint64_t rank_key(const GaDev& g, int a, int k, int r
 
 // Rank of each of a job's n candidates by energy, ties by candidate index
 // (a total order, so ranks are a permutation); the best `pop` are gathered
-// in order (QuickSort by evalue, LIB/gaboom.cpp:2674-2726).  Ranking by
-// sorted tiles: every tile of RANK_TILE candidates is sorted in LDS
-// (bitonic, by (energy key, index)); a candidate's rank is the number of
-// candidates before it in each sorted tile, found by binary search with the
-// tile staged in LDS.  n (n / RANK_TILE) log2(RANK_TILE) steps instead of n^2
-// comparisons; exact, no atomics.
+// in order (QuickSort by evalue, LIB/gaboom.cpp:2674-2726).  Above
+// RANK_COUNT_MAX candidates, ranking by sorted tiles: every tile of
+// RANK_TILE candidates is sorted in LDS (bitonic, by (energy key, index)); a
+// candidate's rank is the number of candidates before it in each sorted
+// tile, found by binary search with the tile staged in LDS:
+// n (n / RANK_TILE) log2(RANK_TILE) steps instead of n^2 comparisons, exact,
+// no atomics.  Smaller sets are counted directly (below), which spreads
+// over more blocks.
 constexpr int RANK_TILE = 1024;
 constexpr int RANK_SORT_THREADS = RANK_TILE / 2;
 
@@ -374,6 +376,47 @@ __global__ __launch_bounds__(RANK_SORT_THREADS) void ga_rank_sort_kernel(GaDev g
     }
 }
 
+// Small rankings (n <= RANK_COUNT_MAX, where the tiles would leave the
+// device idle): every candidate counts the candidates before it, in chunks
+// of RANK_CHUNK keys staged in LDS, one block per (candidates, chunk) pair,
+// integer atomics (order-free) into zeroed ranks.
+constexpr int RANK_CHUNK = 256;
+constexpr int RANK_COUNT_MAX = 8192;
+
+__device__ __forceinline__ int* rank_out(const GaDev& g, int a, int mode)
+{
+    return mode == RANK_PARENTS ? g.prank + (size_t)a * g.pop : g.rank + (size_t)a * g.n_all;
+}
+
+__global__ void ga_rank_zero_kernel(GaDev g, int mode)
+{
+    const int n = rank_n(g, mode);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (long long)g.n_act * n) rank_out(g, (int)(t / n), mode)[t % n] = 0;
+}
+
+__global__ __launch_bounds__(256) void ga_rank_count_kernel(GaDev g, int mode)
+{
+    __shared__ uint64_t keys[RANK_CHUNK];
+    const int a = blockIdx.z;
+    const int k = slot_of(g, a);
+    const int n = rank_n(g, mode);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int base = blockIdx.y * RANK_CHUNK;
+    const int lim = min(RANK_CHUNK, n - base);
+    for (int q = threadIdx.x; q < lim; q += blockDim.x) keys[q] = rank_key(g, a, k, base + q, mode);
+    __syncthreads();
+    if (i >= n) return;
+    const uint64_t ki = rank_key(g, a, k, i, mode);
+    int cnt = 0;
+#pragma unroll 8
+    for (int q = 0; q < lim; ++q) {
+        const uint64_t kj = keys[q];
+        cnt += (kj < ki) || (kj == ki && base + q < i);
+    }
+    if (cnt) atomicAdd(&rank_out(g, a, mode)[i], cnt);
+}
+
 // four candidates per thread (one tile per block) against every sorted tile
 __global__ __launch_bounds__(256) void ga_rank_search_kernel(GaDev g, int mode)
 {
@@ -415,7 +458,7 @@ __global__ __launch_bounds__(256) void ga_rank_search_kernel(GaDev g, int mode)
             rank[r] += lo;
         }
     }
-    int* out = mode == RANK_PARENTS ? g.prank + (size_t)a * g.pop : g.rank + row;
+    int* out = rank_out(g, a, mode);
 #pragma unroll
     for (int r = 0; r < RANK_TILE / 256; ++r)
         if (idx[r] < n) out[idx[r]] = rank[r];
@@ -943,6 +986,12 @@ void ga_launch_select(const GaDev& g, int initial, hipStream_t s)
     const long long cand = (long long)g.n_act * n;
     auto rank = [&](int mode) {
         const int m = mode == RANK_ALL ? g.n_all : g.pop;
+        if (m <= RANK_COUNT_MAX) {
+            hipLaunchKernelGGL(ga_rank_zero_kernel, dim3(blocks_of((long long)g.n_act * m)), dim3(256), 0, s, g, mode);
+            hipLaunchKernelGGL(ga_rank_count_kernel, dim3(blocks_of(m), (m + RANK_CHUNK - 1) / RANK_CHUNK, g.n_act),
+                               dim3(256), 0, s, g, mode);
+            return;
+        }
         const unsigned tiles = (unsigned)((m + RANK_TILE - 1) / RANK_TILE);
         hipLaunchKernelGGL(ga_rank_sort_kernel, dim3(tiles, g.n_act), dim3(RANK_SORT_THREADS), 0, s, g, mode);
         hipLaunchKernelGGL(ga_rank_search_kernel, dim3(tiles, g.n_act), dim3(256), 0, s, g, mode);
