@@ -158,6 +158,7 @@ struct fa_pool {
     size_t slot_bytes = 0;
     fa::GaDev d{};
     DevBuf skey, sidx;  // survivor ranking: candidates sorted within tiles
+    DevBuf prank;       // STEADY: parent ranks among the parents
     DevBuf rank, gi, gic, res, gi2, gic2, res2, cum, fit, fit2, share, sig, lim, ngenes, seeds, gens, ens, cnt, slot_lig,
         lig_rows, run, adm, admrec, done, out;
     hipStream_t stream = nullptr;
@@ -203,6 +204,10 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
     if (p->population < 2 || p->offspring < 1) return fa_fail(FA_ERR_INVALID, "population >= 2 and offspring >= 1");
     if (p->temperature <= 0.0) return fa_fail(FA_ERR_INVALID, "temperature must be positive");
     if (p->fitness < FA_FIT_LINEAR || p->fitness > FA_FIT_SMFREE) return fa_fail(FA_ERR_INVALID, "unknown fitness model");
+    if (p->reproduction != FA_REP_BOOM && p->reproduction != FA_REP_STEADY)
+        return fa_fail(FA_ERR_INVALID, "unknown reproduction model");
+    if (p->reproduction == FA_REP_STEADY && p->offspring > p->population)
+        return fa_fail(FA_ERR_INVALID, "STEADY reproduction needs offspring <= population");
     if (p->fitness != FA_FIT_LINEAR && !(p->share_alpha > 0.0))
         return fa_fail(FA_ERR_INVALID, "sharing alpha must be positive");
     if (p->fitness != FA_FIT_LINEAR && !(p->sig_share > 0.0) && !(p->share_peaks > 0.0 && p->share_scale > 0.0))
@@ -220,7 +225,7 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
     const size_t row = (size_t)Gmax * (sizeof(int32_t) + sizeof(double)) + sizeof(fa_cf_result);
     const size_t slot_bytes = row * (pop + nn + pop) + row * pop + 4 * sizeof(double) * pop +
                               3 * sizeof(double) * Gmax + 2 * sizeof(int) * (nn + pop) +
-                              (sizeof(unsigned long long) + sizeof(int)) * (nn + pop) + 256;
+                              (sizeof(unsigned long long) + sizeof(int)) * (nn + pop) + sizeof(int) * pop + 256;
     long long slots = std::min<long long>(p->max_slots, (long long)(p->hbm_budget / slot_bytes));
     if (slots < 1) return fa_fail(FA_ERR_LIMIT, "HBM budget below one job's state");
     if ((long long)slots * (long long)(pop + nn + pop) > (1LL << 30)) return fa_fail(FA_ERR_LIMIT, "pool rows above 2^30");
@@ -242,7 +247,8 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
     int rc;
     if ((rc = fa_ensure(pl->rank, sizeof(int) * S * (pop + nn))) ||
         (rc = fa_ensure(pl->skey, sizeof(unsigned long long) * S * (pop + nn))) ||
-        (rc = fa_ensure(pl->sidx, sizeof(int) * S * (pop + nn))) || (rc = fa_ensure(pl->gi, sizeof(int32_t) * rows * Gmax)) || (rc = fa_ensure(pl->gic, sizeof(double) * rows * Gmax)) ||
+        (rc = fa_ensure(pl->sidx, sizeof(int) * S * (pop + nn))) ||
+        (p->reproduction == FA_REP_STEADY && (rc = fa_ensure(pl->prank, sizeof(int) * S * pop))) || (rc = fa_ensure(pl->gi, sizeof(int32_t) * rows * Gmax)) || (rc = fa_ensure(pl->gic, sizeof(double) * rows * Gmax)) ||
         (rc = fa_ensure(pl->res, sizeof(fa_cf_result) * rows)) || (rc = fa_ensure(pl->gi2, sizeof(int32_t) * S * pop * Gmax)) ||
         (rc = fa_ensure(pl->gic2, sizeof(double) * S * pop * Gmax)) || (rc = fa_ensure(pl->res2, sizeof(fa_cf_result) * S * pop)) ||
         (rc = fa_ensure(pl->cum, sizeof(double) * S * pop)) || (rc = fa_ensure(pl->fit, sizeof(double) * S * pop)) ||
@@ -285,6 +291,8 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
     d.rank = static_cast<int*>(pl->rank.p);
     d.skey = static_cast<unsigned long long*>(pl->skey.p);
     d.sidx = static_cast<int*>(pl->sidx.p);
+    d.steady = p->reproduction == FA_REP_STEADY ? 1 : 0;
+    d.prank = static_cast<int*>(pl->prank.p);
     d.gi = static_cast<int32_t*>(pl->gi.p);
     d.gic = static_cast<double*>(pl->gic.p);
     d.res = static_cast<fa_cf_result*>(pl->res.p);
@@ -537,7 +545,7 @@ int fa_pool_destroy(fa_pool* pl)
     if (!pl) return FA_SUCCESS;
     (void)hipSetDevice(pl->ctx->device);
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    for (DevBuf* b : {&pl->skey, &pl->sidx, &pl->rank, &pl->gi, &pl->gic, &pl->res, &pl->gi2, &pl->gic2, &pl->res2, &pl->cum, &pl->fit, &pl->fit2,
+    for (DevBuf* b : {&pl->skey, &pl->sidx, &pl->prank, &pl->rank, &pl->gi, &pl->gic, &pl->res, &pl->gi2, &pl->gic2, &pl->res2, &pl->cum, &pl->fit, &pl->fit2,
                       &pl->share, &pl->sig, &pl->lim, &pl->ngenes, &pl->seeds, &pl->gens, &pl->ens, &pl->cnt,
                       &pl->slot_lig, &pl->lig_rows, &pl->run, &pl->adm, &pl->admrec, &pl->done, &pl->out})
         if (b->p) (void)hipFree(b->p);

@@ -368,7 +368,7 @@ typedef struct fa_pool_params {
     uint64_t hbm_budget;       /* bytes of job state; slots = min(max_slots, budget / slot bytes) */
     int32_t max_running;       /* at most this many resident jobs (0: every slot); a smaller
                                   batch per step finishes each job sooner */
-    int32_t pad;
+    int32_t reproduction;      /* fa_reproduction, as fa_ga_params */
 } fa_pool_params;
 
 typedef struct fa_job {

@@ -56,15 +56,17 @@ def _same(a, b):
     assert a["n_poses"] == b["n_poses"]
 
 
-@pytest.mark.parametrize("fitness", ["LINEAR", "PSHARE"])
-def test_job_equals_single_ga(hip, ctx, limits, fitness):
+@pytest.mark.parametrize("fitness,rep", [("LINEAR", "BOOM"), ("PSHARE", "BOOM"), ("LINEAR", "STEADY")])
+def test_job_equals_single_ga(hip, ctx, limits, fitness, rep):
     gens, seed, lig = 6, 1234, 3
-    ga = hip.GA(ctx, limits[lig], ligand=lig, population=POP, offspring=NEW, seed=seed, fitness=fitness)
+    new = NEW // 2 if rep == "STEADY" else NEW
+    ga = hip.GA(ctx, limits[lig], ligand=lig, population=POP, offspring=new, seed=seed, fitness=fitness,
+                reproduction=rep)
     ga.run(gens, graph=True)
     g, ic, res = ga.population()
     th, n = ga.thermo()
     ga.close()
-    pool = hip.Pool(ctx, population=POP, offspring=NEW, max_slots=4, fitness=fitness)
+    pool = hip.Pool(ctx, population=POP, offspring=new, max_slots=4, fitness=fitness, reproduction=rep)
     jid = pool.submit(lig, limits[lig], gens, seed)
     out = _drain(pool)
     pool.close()
@@ -76,7 +78,7 @@ def test_job_equals_single_ga(hip, ctx, limits, fitness):
     assert np.array_equal(r["best_genes"], g[best, :G])
     assert np.array_equal(r["best_ic"], ic[best, :G])
     assert r["thermo"] == th
-    assert r["n_poses"] == n == POP + gens * NEW
+    assert r["n_poses"] == n == POP + gens * new
 
 
 def test_result_same_alone_and_in_mix(hip, ctx, limits):
