@@ -90,6 +90,9 @@ void ga_launch_thermo(const GaDev& g, int n_slots, double T, fa_thermo* out, hip
 // up to dd_rounds batches of candidates, each kept in draw order when its
 // signature is new, until the job's rows are filled.
 void ga_launch_dedup_fill(const GaDev& g, int initial, hipStream_t s);
+// Empties the signature sets and counts of the n_act jobs of `g` (a job
+// newly admitted into a reused slot).
+void ga_launch_dedup_reset(const GaDev& g, hipStream_t s);
 // Re-inserts the signatures of `old_keys` (jobs << old_log2 entries) into
 // g.dset (cleared beforehand).
 void ga_launch_dset_rehash(const GaDev& g, const unsigned long long* old_keys, int old_log2, hipStream_t s);

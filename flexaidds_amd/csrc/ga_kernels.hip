@@ -906,6 +906,19 @@ __global__ __launch_bounds__(256) void ga_dd_accept_kernel(GaDev g, int n_round,
     }
 }
 
+// grid y is the job: its set, signature-0 flag and counts cleared
+__global__ void ga_dd_reset_kernel(GaDev g)
+{
+    const int a = blockIdx.y, k = slot_of(g, a);
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < ((size_t)1 << g.dset_log2)) g.dset[((size_t)k << g.dset_log2) + j] = 0ull;
+    if (j == 0) {
+        g.dzero[k] = 0;
+        g.dd_stats[2 * (size_t)k] = 0ull;
+        g.dd_stats[2 * (size_t)k + 1] = 0ull;
+    }
+}
+
 __global__ void ga_dset_rehash_kernel(GaDev g, const unsigned long long* old, int old_log2)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -964,6 +977,12 @@ void ga_launch_dedup_fill(const GaDev& g, int initial, hipStream_t s)
         hipLaunchKernelGGL(ga_dd_accept_kernel, dim3(g.n_act), dim3(256), 0, s, b, n_round, initial,
                            r == g.dd_rounds - 1 ? 1 : 0);
     }
+}
+
+void ga_launch_dedup_reset(const GaDev& g, hipStream_t s)
+{
+    if (g.n_act < 1 || g.dd_rounds < 1) return;
+    hipLaunchKernelGGL(ga_dd_reset_kernel, dim3(blocks_of(1ll << g.dset_log2), g.n_act), dim3(256), 0, s, g);
 }
 
 void ga_launch_dset_rehash(const GaDev& g, const unsigned long long* old_keys, int old_log2, hipStream_t s)

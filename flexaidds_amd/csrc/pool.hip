@@ -159,6 +159,7 @@ struct fa_pool {
     fa::GaDev d{};
     DevBuf skey, sidx;  // survivor ranking: candidates sorted within tiles
     DevBuf prank;       // STEADY: parent ranks among the parents
+    DevBuf dset, dzero, rkey, rval, rzero, csig, cdup, need, ddstats;  // duplicate rejection, per slot
     DevBuf rank, gi, gic, res, gi2, gic2, res2, cum, fit, fit2, share, sig, lim, ngenes, seeds, gens, ens, cnt, slot_lig,
         lig_rows, run, adm, admrec, done, out;
     hipStream_t stream = nullptr;
@@ -213,22 +214,42 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
     if (p->fitness != FA_FIT_LINEAR && !(p->sig_share > 0.0) && !(p->share_peaks > 0.0 && p->share_scale > 0.0))
         return fa_fail(FA_ERR_INVALID, "sharing needs sig_share > 0 or peaks and scale > 0");
     if (p->max_slots < 1) return fa_fail(FA_ERR_INVALID, "max_slots >= 1");
+    if (p->dedup_rounds < 0 || p->dedup_rounds > 1024) return fa_fail(FA_ERR_INVALID, "dedup_rounds in [0, 1024]");
+    if (p->dedup_rounds > 0 && p->dedup_max_generations < 0)
+        return fa_fail(FA_ERR_INVALID, "dedup_max_generations must be >= 0");
     if (p->max_running < 0) return fa_fail(FA_ERR_INVALID, "max_running >= 0");
     int Gmax = p->max_genes;
     if (Gmax <= 0)
         for (int g : c->lig_genes) Gmax = std::max(Gmax, g);
     if (Gmax < 1 || Gmax > FA_MAX_GENES) return fa_fail(FA_ERR_LIMIT, "between 1 and FA_MAX_GENES genes per row");
     const size_t pop = (size_t)p->population, nn = (size_t)p->offspring;
+    // duplicate rejection: candidate rows, a batch table and a signature set
+    // sized for dedup_max_generations per slot
+    const int R = p->dedup_rounds;
+    const size_t n_cand = R > 0 ? std::max(pop, 2 * ((nn + 1) / 2)) : 0;
+    auto log2_at_least = [](long long x) {
+        int l = 0;
+        while ((1LL << l) < x) ++l;
+        return l;
+    };
+    const int rlog2 = R > 0 ? log2_at_least(2 * (long long)n_cand) : 0;
+    const int slog2 = R > 0 ? log2_at_least(2 * ((long long)pop + (long long)p->dedup_max_generations * (long long)nn)) : 0;
+    if (slog2 > 34) return fa_fail(FA_ERR_LIMIT, "duplicate-rejection table above 2^34 entries per slot");
+    if ((long long)R * (long long)n_cand >= (1LL << 31)) return fa_fail(FA_ERR_LIMIT, "dedup_rounds * batch above 2^31 draws");
+    const size_t dd_bytes = R > 0 ? n_cand * ((size_t)Gmax * (sizeof(int32_t) + sizeof(double)) + sizeof(unsigned long long) + sizeof(int)) +
+                                        (sizeof(unsigned long long) + sizeof(int)) * ((size_t)1 << rlog2) +
+                                        sizeof(unsigned long long) * ((size_t)1 << slog2) + 32
+                                  : 0;
     // one slot's device state: its parent rows, a share of the scoring area
     // (children + a staged initial population), gathered survivors, fitness
     // arrays, limits and small per-slot values
     const size_t row = (size_t)Gmax * (sizeof(int32_t) + sizeof(double)) + sizeof(fa_cf_result);
     const size_t slot_bytes = row * (pop + nn + pop) + row * pop + 4 * sizeof(double) * pop +
                               3 * sizeof(double) * Gmax + 2 * sizeof(int) * (nn + pop) +
-                              (sizeof(unsigned long long) + sizeof(int)) * (nn + pop) + sizeof(int) * pop + 256;
+                              (sizeof(unsigned long long) + sizeof(int)) * (nn + pop) + sizeof(int) * pop + dd_bytes + 256;
     long long slots = std::min<long long>(p->max_slots, (long long)(p->hbm_budget / slot_bytes));
     if (slots < 1) return fa_fail(FA_ERR_LIMIT, "HBM budget below one job's state");
-    if ((long long)slots * (long long)(pop + nn + pop) > (1LL << 30)) return fa_fail(FA_ERR_LIMIT, "pool rows above 2^30");
+    if ((long long)slots * (long long)(pop + nn + pop + n_cand) > (1LL << 30)) return fa_fail(FA_ERR_LIMIT, "pool rows above 2^30");
     FA_HIP(hipSetDevice(c->device));
     fa_pool* pl = new fa_pool();
     pl->ctx = c;
@@ -239,7 +260,7 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
     pl->slot_bytes = slot_bytes;
     pl->slot.resize(slots);
     const size_t S = (size_t)slots;
-    const size_t rows = S * pop + S * (nn + pop);  // parents, then the scoring area
+    const size_t rows = S * pop + S * (nn + pop) + S * n_cand;  // parents, the scoring area, candidates
     auto fail_free = [&](int code) {
         fa_pool_destroy(pl);
         return code;
@@ -260,6 +281,14 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
         (rc = fa_ensure(pl->lig_rows, sizeof(int) * S * (nn + pop))) || (rc = fa_ensure(pl->run, sizeof(int) * S)) ||
         (rc = fa_ensure(pl->adm, sizeof(int) * S)) || (rc = fa_ensure(pl->admrec, sizeof(fa::PoolAdmit) * S)) ||
         (rc = fa_ensure(pl->done, sizeof(int) * S)) || (rc = fa_ensure(pl->out, sizeof(fa::PoolOut) * S)))
+        return fail_free(rc);
+    if (R > 0 &&
+        ((rc = fa_ensure(pl->dset, sizeof(unsigned long long) * (S << slog2))) ||
+         (rc = fa_ensure(pl->dzero, sizeof(int) * S)) || (rc = fa_ensure(pl->rkey, sizeof(unsigned long long) * (S << rlog2))) ||
+         (rc = fa_ensure(pl->rval, sizeof(int) * (S << rlog2))) || (rc = fa_ensure(pl->rzero, sizeof(int) * S)) ||
+         (rc = fa_ensure(pl->csig, sizeof(unsigned long long) * S * n_cand)) ||
+         (rc = fa_ensure(pl->cdup, sizeof(int) * S * n_cand)) || (rc = fa_ensure(pl->need, sizeof(int) * S)) ||
+         (rc = fa_ensure(pl->ddstats, sizeof(unsigned long long) * 2 * S))))
         return fail_free(rc);
     hipError_t e = hipHostMalloc((void**)&pl->h_admrec, sizeof(fa::PoolAdmit) * S, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&pl->h_lists, sizeof(int) * 3 * S, hipHostMallocDefault);
@@ -293,6 +322,22 @@ int fa_pool_create(fa_ctx* c, const fa_pool_params* p, fa_pool** out)
     d.sidx = static_cast<int*>(pl->sidx.p);
     d.steady = p->reproduction == FA_REP_STEADY ? 1 : 0;
     d.prank = static_cast<int*>(pl->prank.p);
+    d.dd_rounds = R;
+    if (R > 0) {
+        d.n_cand = (int)n_cand;
+        d.cand = S * pop + S * (nn + pop);
+        d.dset_log2 = slog2;
+        d.rtab_log2 = rlog2;
+        d.dset = static_cast<unsigned long long*>(pl->dset.p);
+        d.dzero = static_cast<int*>(pl->dzero.p);
+        d.rkey = static_cast<unsigned long long*>(pl->rkey.p);
+        d.rval = static_cast<int*>(pl->rval.p);
+        d.rzero = static_cast<int*>(pl->rzero.p);
+        d.csig = static_cast<unsigned long long*>(pl->csig.p);
+        d.cdup = static_cast<int*>(pl->cdup.p);
+        d.need = static_cast<int*>(pl->need.p);
+        d.dd_stats = static_cast<unsigned long long*>(pl->ddstats.p);
+    }
     d.gi = static_cast<int32_t*>(pl->gi.p);
     d.gic = static_cast<double*>(pl->gic.p);
     d.res = static_cast<fa_cf_result*>(pl->res.p);
@@ -329,6 +374,8 @@ int fa_pool_submit(fa_pool* pl, const fa_job* job, int64_t* id)
     fa_ctx* c = pl->ctx;
     if (job->ligand < 0 || job->ligand >= c->n_ligands) return fa_fail(FA_ERR_INVALID, "ligand index out of range");
     if (job->generations < 0) return fa_fail(FA_ERR_INVALID, "generations must be >= 0");
+    if (pl->p.dedup_rounds > 0 && job->generations > pl->p.dedup_max_generations)
+        return fa_fail(FA_ERR_LIMIT, "job longer than the pool's dedup_max_generations");
     const int G = c->lig_genes[job->ligand];
     if (G > pl->Gmax) return fa_fail(FA_ERR_LIMIT, "ligand has more genes than the pool's rows");
     for (int j = 0; j < G; ++j) {
@@ -428,8 +475,14 @@ int fa_pool_step(fa_pool* pl, int* n_running, int* n_queued)
     da.act = static_cast<const int*>(pl->adm.p);
     da.stage = (size_t)S * pop + (size_t)n_run * nn;
     fa::ga_launch_tick(dr, s);
-    fa::ga_launch_breed(dr, s);
-    fa::ga_launch_init(da, s);
+    if (p.dedup_rounds > 0) {
+        fa::ga_launch_dedup_fill(dr, 0, s);
+        fa::ga_launch_dedup_reset(da, s);   // a new job in the slot: an empty set
+        fa::ga_launch_dedup_fill(da, 1, s);
+    } else {
+        fa::ga_launch_breed(dr, s);
+        fa::ga_launch_init(da, s);
+    }
     const long long n_rows = (long long)n_run * nn + (long long)n_adm * pop;
     if (n_rows > 0) {
         const size_t base = (size_t)S * pop;
@@ -545,7 +598,8 @@ int fa_pool_destroy(fa_pool* pl)
     if (!pl) return FA_SUCCESS;
     (void)hipSetDevice(pl->ctx->device);
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    for (DevBuf* b : {&pl->skey, &pl->sidx, &pl->prank, &pl->rank, &pl->gi, &pl->gic, &pl->res, &pl->gi2, &pl->gic2, &pl->res2, &pl->cum, &pl->fit, &pl->fit2,
+    for (DevBuf* b : {&pl->skey, &pl->sidx, &pl->prank, &pl->dset, &pl->dzero, &pl->rkey, &pl->rval, &pl->rzero,
+                      &pl->csig, &pl->cdup, &pl->need, &pl->ddstats, &pl->rank, &pl->gi, &pl->gic, &pl->res, &pl->gi2, &pl->gic2, &pl->res2, &pl->cum, &pl->fit, &pl->fit2,
                       &pl->share, &pl->sig, &pl->lim, &pl->ngenes, &pl->seeds, &pl->gens, &pl->ens, &pl->cnt,
                       &pl->slot_lig, &pl->lig_rows, &pl->run, &pl->adm, &pl->admrec, &pl->done, &pl->out})
         if (b->p) (void)hipFree(b->p);

@@ -470,7 +470,8 @@ class PoolParams(C.Structure):
                 ("temperature", C.c_double), ("share_alpha", C.c_double), ("share_peaks", C.c_double),
                 ("share_scale", C.c_double), ("sig_share", C.c_double), ("entropy_weight", C.c_double),
                 ("max_slots", C.c_int32), ("max_genes", C.c_int32), ("hbm_budget", C.c_uint64),
-                ("max_running", C.c_int32), ("reproduction", C.c_int32)]
+                ("max_running", C.c_int32), ("reproduction", C.c_int32), ("dedup_rounds", C.c_int32),
+                ("dedup_max_generations", C.c_int32)]
 
 
 class Job(C.Structure):
@@ -494,14 +495,15 @@ class Pool:
     def __init__(self, ctx: Context, population=1024, offspring=None, max_slots=64, hbm_budget=8 << 30,
                  cross_rate=0.8, mut_rate=0.03, temperature=300.0, intragenes=False, fitness="LINEAR",
                  share_alpha=1.0, share_peaks=5.0, share_scale=10.0, sig_share=0.0, entropy_weight=0.5,
-                 max_genes=0, max_running=0, reproduction="BOOM"):
+                 max_genes=0, max_running=0, reproduction="BOOM", dedup_rounds=0, dedup_max_generations=0):
         self.ctx = ctx
         model = FITNESS[fitness] if isinstance(fitness, str) else int(fitness)
         self.params = PoolParams(population, population if offspring is None else offspring,
                                  1 if intragenes else 0, model, cross_rate, mut_rate, temperature, share_alpha,
                                  share_peaks, share_scale, sig_share, entropy_weight, max_slots, max_genes,
                                  int(hbm_budget), max_running,
-                                 REPRODUCTION[reproduction] if isinstance(reproduction, str) else int(reproduction))
+                                 REPRODUCTION[reproduction] if isinstance(reproduction, str) else int(reproduction),
+                                 int(dedup_rounds), int(dedup_max_generations))
         self._pool = C.c_void_p()
         _check(_lib.fa_pool_create(ctx._ctx, C.byref(self.params), C.byref(self._pool)))
         s, b, g = C.c_int(), C.c_uint64(), C.c_int()

@@ -369,6 +369,9 @@ typedef struct fa_pool_params {
     int32_t max_running;       /* at most this many resident jobs (0: every slot); a smaller
                                   batch per step finishes each job sooner */
     int32_t reproduction;      /* fa_reproduction, as fa_ga_params */
+    int32_t dedup_rounds;      /* as fa_ga_params (0: duplicates allowed) */
+    int32_t dedup_max_generations;  /* with dedup_rounds > 0: the longest job a slot's signature
+                                       set is sized for; longer jobs are refused at submission */
 } fa_pool_params;
 
 typedef struct fa_job {

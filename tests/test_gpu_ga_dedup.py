@@ -157,3 +157,41 @@ def test_batched_jobs_follow_their_own_streams(hip, ctx, system, orc):
             stats[k] += st
     assert np.array_equal(ga.dedup_stats(), np.stack(stats))
     ga.close()
+
+
+def test_pool_jobs_equal_the_ga(hip, ctx, system):
+    # a pool with duplicate rejection: each job equals the GA with the same
+    # seed, also when it reuses the slot of a finished job (its set starts
+    # empty); a job longer than the pool's sizing is refused
+    lim = narrow(system.ligands[0], 5)
+    pop, new, R, gens = 64, 48, 4, 5
+    seeds = (SEED + 7, SEED + 8, SEED + 9)
+    refs = []
+    for sd in seeds:
+        ga = hip.GA(ctx, lim, population=pop, offspring=new, seed=sd, dedup_rounds=R)
+        ga.run(gens, graph=False)
+        g, ic, res = ga.population()
+        th, n = ga.thermo()
+        assert int(ga.dedup_stats()[0, 0]) > 0
+        ga.close()
+        b = int(np.argmin(res["cf"][:pop]))
+        refs.append((g[b], ic[b], res["cf"][b], th, n))
+    pool = hip.Pool(ctx, population=pop, offspring=new, max_slots=1, dedup_rounds=R, dedup_max_generations=gens)
+    ids = [pool.submit(0, lim, gens, sd) for sd in seeds]
+    with pytest.raises(hip.FAError):
+        pool.submit(0, lim, gens + 1, 1)
+    out = {}
+    for _ in range(100):
+        running, queued = pool.step()
+        for r in pool.collect():
+            out[r["id"]] = r
+        if running == 0 and queued == 0:
+            break
+    pool.close()
+    assert sorted(out) == sorted(ids)
+    G = len(lim)
+    for jid, (gb, icb, cf, th, n) in zip(ids, refs):
+        r = out[jid]
+        assert r["best"]["cf"] == cf
+        assert np.array_equal(r["best_genes"], gb[:G]) and np.array_equal(r["best_ic"], icb[:G])
+        assert r["thermo"] == th and r["n_poses"] == n
