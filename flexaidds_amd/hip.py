@@ -84,6 +84,7 @@ _SIGS = {
     "fa_ga_dedup_stats": ([_P, _P], C.c_int),
     "fa_ga_grand": ([_P, _P, _P, _P, _P], C.c_int),
     "fa_grand_device": ([_P, _P, C.c_int, _P, C.c_int, _P, _P], C.c_int),
+    "fa_serve_run": ([_P, C.c_int, _P, C.c_int, C.c_int, _P, _P], C.c_int),
     "fa_cluster_cf": ([_P, C.c_int, C.c_int, _P, C.c_int, _P, _P, _P, _P, _P, _P, _P], C.c_int),
     "fa_ga_run": ([_P, C.c_int, C.c_int, _P], C.c_int),
     "fa_pool_create": ([_P, _P, _P], C.c_int),
@@ -485,6 +486,49 @@ class JobResult(C.Structure):
                 ("best_ic", C.c_double * MAX_GENES), ("thermo", abi.Thermo), ("n_poses", C.c_uint64)]
 
 
+def _job_dict(r) -> dict:
+    return {"id": int(r.id), "ligand": int(r.ligand), "generations": int(r.generations),
+            "best": {f: getattr(r.best, f) for f, _ in abi.CFResult._fields_},
+            "best_genes": np.array(r.best_genes[:r.n_genes], np.int32),
+            "best_ic": np.array(r.best_ic[:r.n_genes]),
+            "thermo": {f: getattr(r.thermo, f) for f, _ in abi.Thermo._fields_},
+            "n_poses": int(r.n_poses)}
+
+
+class Request(C.Structure):
+    _fields_ = [("t", C.c_double), ("pool", C.c_int32), ("pad", C.c_int32), ("job", Job)]
+
+
+class ServedRec(C.Structure):
+    _fields_ = [("request", C.c_int64), ("latency_s", C.c_double), ("result", JobResult)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("steps", C.c_int64), ("poses", C.c_uint64)]
+
+
+def serve_run(pools, requests, realtime=True):
+    """The served request loop in C++ (fa_serve_run).  requests: dicts with
+    t, pool, ligand, generations, seed and limits (a sequence of
+    (min, max, del, map)).  Returns ([(request index, latency_s, result
+    dict)] in completion order, {"wall_s", "steps", "poses"})."""
+    n = len(requests)
+    keep = []
+    arr = (Request * max(n, 1))()
+    for i, r in enumerate(requests):
+        lim = (GeneLim * len(r["limits"]))(*[GeneLim(float(a), float(b), float(d), int(m), 0)
+                                              for a, b, d, m in r["limits"]])
+        keep.append(lim)
+        arr[i] = Request(float(r["t"]), int(r["pool"]), 0,
+                         Job(int(r["ligand"]), int(r["generations"]), int(r["seed"]), lim))
+    handles = (C.c_void_p * len(pools))(*[p._pool for p in pools])
+    out = (ServedRec * max(n, 1))()
+    st = ServeStats()
+    _check(_lib.fa_serve_run(handles, len(pools), arr, n, 1 if realtime else 0, out, C.byref(st)))
+    res = [(int(o.request), float(o.latency_s), _job_dict(o.result)) for o in out[:n]]
+    return res, {"wall_s": float(st.wall_s), "steps": int(st.steps), "poses": int(st.poses)}
+
+
 class Pool:
     """Continuous-batching docking pool on one context (fa_pool_* of the C ABI).
 
@@ -527,15 +571,7 @@ class Pool:
     def collect(self):
         n = C.c_int()
         _check(_lib.fa_pool_collect(self._pool, self._out, self.slots, C.byref(n)))
-        out = []
-        for r in self._out[:n.value]:
-            out.append({"id": int(r.id), "ligand": int(r.ligand), "generations": int(r.generations),
-                        "best": {f: getattr(r.best, f) for f, _ in abi.CFResult._fields_},
-                        "best_genes": np.array(r.best_genes[:r.n_genes], np.int32),
-                        "best_ic": np.array(r.best_ic[:r.n_genes]),
-                        "thermo": {f: getattr(r.thermo, f) for f, _ in abi.Thermo._fields_},
-                        "n_poses": int(r.n_poses)})
-        return out
+        return [_job_dict(r) for r in self._out[:n.value]]
 
     def set_max_running(self, n: int):
         _check(_lib.fa_pool_set_max_running(self._pool, int(n)))

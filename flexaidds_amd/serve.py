@@ -2,12 +2,13 @@
 
 Each receptor is uploaded once into its own context (resident in HBM with
 its ligand library) and gets a docking pool (fa_pool_*) whose slot count
-follows from an HBM budget.  The loop admits requests as their arrival time
-passes and steps every pool that has work as soon as that pool's previous
-step has ended (each step enqueues one generation of all of the pool's
-resident jobs on the pool's stream and returns); finished jobs are collected
-then.  The pools' launches overlap on the device, and jobs that arrive while
-others run join their pool at its next generation.
+follows from an HBM budget.  The loop (fa_serve_run, in the library)
+admits requests as their arrival time passes and steps every pool that has
+work as soon as that pool's previous step has ended (each step enqueues one
+generation of all of the pool's resident jobs on the pool's stream and
+returns); finished jobs are collected then.  The pools' launches overlap on
+the device, and jobs that arrive while others run join their pool at its
+next generation.
 
 Each receptor also has a TargetServer (flexaidds_amd.target): every
 finished job is registered there as a completed docking session under its
@@ -19,7 +20,6 @@ There is no CPU fallback: the pools run through the HIP library.
 """
 from __future__ import annotations
 
-import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -68,63 +68,30 @@ class Server:
         for c in self.ctx:
             c.close()
 
-    def run(self, requests, realtime=True, on_step=None) -> Served:
+    def run(self, requests, realtime=True) -> Served:
         """Serve `requests` (dicts with t, receptor, ligand, generations, seed
         and optionally name, the ligand's name for the receptor's
-        TargetServer, by default "L<ligand>") in arrival order.  With realtime, a request is admitted once the wall
-        clock passes its arrival time; otherwise all arrive at once.
+        TargetServer, by default "L<ligand>") in arrival order.  With
+        realtime, a request is admitted once the wall clock passes its
+        arrival time; otherwise all arrive at once.
 
-        Each pool is stepped as soon as its own previous step has ended (a
-        non-blocking query), so one pool's slow launch does not hold the
-        others back and the pools' launches keep overlapping on the device."""
-        out = Served()
-        order = sorted(range(len(requests)), key=lambda i: requests[i]["t"])
-        nxt = 0
-        owner = {}                                   # (pool, job id) -> request index
-        P = len(self.pools)
-        busy = [0] * P                               # resident + queued jobs per pool
-        inflight = [False] * P                       # a step enqueued and not yet collected
-        t0 = time.perf_counter()
-        while nxt < len(order) or any(busy) or any(inflight):
-            now = time.perf_counter() - t0
-            while nxt < len(order) and (not realtime or requests[order[nxt]]["t"] <= now):
-                i = order[nxt]
-                r = requests[i]
-                p = r["receptor"]
-                jid = self.pools[p].submit(r["ligand"], self.limits[p][r["ligand"]], r["generations"], r["seed"])
-                owner[(p, jid)] = i
-                busy[p] += 1
-                nxt += 1
-            progressed = False
-            for p in range(P):
-                if inflight[p]:
-                    if self.pools[p].busy():
-                        continue
-                    for res in self.pools[p].collect():
-                        i = owner.pop((p, res["id"]))
-                        done = time.perf_counter() - t0
-                        out.results[i] = res
-                        out.finished.append(i)
-                        out.latency_s[i] = done - (requests[i]["t"] if realtime else 0.0)
-                        out.poses += res["n_poses"]
-                        r = requests[i]
-                        lig = self.systems[p].ligands[r["ligand"]]
-                        self.targets[p].register_result(target.session_from_job(
-                            self.targets[p], r.get("name", f"L{r['ligand']}"), res, lig))
-                        busy[p] -= 1
-                    inflight[p] = False
-                    progressed = True
-                if busy[p] > 0:
-                    self.pools[p].step()
-                    inflight[p] = True
-                    out.steps += 1
-                    progressed = True
-            if progressed:
-                if on_step:
-                    on_step(out)
-            elif not any(inflight) and nxt < len(order):
-                time.sleep(max(0.0, requests[order[nxt]]["t"] - (time.perf_counter() - t0)))
-            else:
-                time.sleep(0.0002)
-        out.wall_s = time.perf_counter() - t0
+        The loop is the library's (fa_serve_run, C++): each pool is stepped
+        as soon as its own previous step has ended (a non-blocking query),
+        so one pool's slow launch does not hold the others back and the
+        pools' launches keep overlapping on the device.  Finished jobs are
+        then registered with their receptor's TargetServer in completion
+        order."""
+        reqs = [{"t": r["t"], "pool": r["receptor"], "ligand": r["ligand"], "generations": r["generations"],
+                 "seed": r["seed"], "limits": self.limits[r["receptor"]][r["ligand"]]} for r in requests]
+        done, st = hip.serve_run(self.pools, reqs, realtime=realtime)
+        out = Served(wall_s=st["wall_s"], steps=st["steps"], poses=st["poses"])
+        for i, lat, res in done:
+            r = requests[i]
+            p = r["receptor"]
+            out.results[i] = res
+            out.latency_s[i] = lat
+            out.finished.append(i)
+            lig = self.systems[p].ligands[r["ligand"]]
+            self.targets[p].register_result(target.session_from_job(
+                self.targets[p], r.get("name", f"L{r['ligand']}"), res, lig))
         return out

@@ -417,6 +417,39 @@ int fa_pool_set_max_running(fa_pool* pool, int max_running);
 int fa_pool_busy(fa_pool* pool, int* busy);
 int fa_pool_destroy(fa_pool* pool);
 
+/* ---------------------------------------------------------------------
+ * Served stream: the request loop over resident receptors' pools, in C++.
+ * No counterpart in the reference (it docks one ligand per process run);
+ * its TargetServer (LIB/TargetServer.h) is what receives the results.
+ * Requests are admitted in arrival order once the wall clock passes their
+ * arrival time (or at once when realtime is 0) into their receptor's pool;
+ * each pool is stepped as soon as its own previous step has ended, so the
+ * pools' launches overlap on the device.  out[i] receives the i-th request
+ * to finish: its index, its job result and its latency (completion minus
+ * arrival, from the loop's start when not realtime).
+ * ------------------------------------------------------------------- */
+typedef struct fa_request {
+    double t;                  /* arrival, seconds after the loop starts */
+    int32_t pool;              /* index into the pools array (the receptor) */
+    int32_t pad;
+    fa_job job;                /* ligand of that pool's context, generations, seed, limits */
+} fa_request;
+
+typedef struct fa_served {
+    int64_t request;           /* index into the requests array */
+    double latency_s;
+    fa_job_result result;
+} fa_served;
+
+typedef struct fa_serve_stats {
+    double wall_s;
+    int64_t steps;             /* pool steps taken */
+    uint64_t poses;            /* poses the finished jobs evaluated */
+} fa_serve_stats;
+
+int fa_serve_run(fa_pool* const* pools, int n_pools, const fa_request* requests, int n_requests, int realtime,
+                 fa_served* out, fa_serve_stats* stats);
+
 /* Size of the kernel's per-wave LDS workspace (bytes). */
 int fa_kernel_lds_bytes(void);
 
