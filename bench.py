@@ -24,9 +24,10 @@ path (one ligand population per launch).  Beside it:
             a PDB file (the reference's RNA target) with the automatic
             binding-site grid built from it
   binding_modes  the reference's default clustering of docked poses into
-            binding modes on the device, over 200,000 poses (ga_dock also
-            reports the modes of its final population and the same dock with
-            duplicate rejection)
+            binding modes on the device, over 200,000 poses, and the
+            FastOPTICS super-cluster of 8,192 pose energies (ga_dock also
+            reports the modes and super-cluster of its final population and
+            the same dock with duplicate rejection)
 
 The roofline numbers come from the kernel's own operation counters (the
 algorithmic FLOP model, see include/fa_types.h fa_opcount) and HIP-event
@@ -263,6 +264,15 @@ def ga_dock_leg(ctx, system, args):
                             "device_ms": round(pop.clusters["device_ms"], 4),
                             "best_mode_free_energy_300K": round(pop.modes[0].free_energy(), 6) if pop.modes else None,
                             "best_mode_size": pop.modes[0].size() if pop.modes else 0}
+    # the dominant energy basin of those poses (FastOPTICS super-cluster)
+    if pop.modes:
+        e_all = np.concatenate([m.energies for m in pop.modes])
+        if 4 < len(e_all) <= hip.SUPER_CLUSTER_MAX:
+            sc = ctx.super_cluster(e_all, max(4, len(e_all) // 20))
+            sce = pop.super_cluster_ensemble()
+            out["binding_modes"]["super_cluster"] = {"poses": int(len(e_all)), "members": int(len(sc["members"])),
+                                                     "device_ms": round(sc["device_ms"], 4),
+                                                     "free_energy_300K": round(sce["free_energy"], 6)}
     # the same dock with the reference's default duplicate rejection
     if args.dedup_generations > 0:
         ga = hip.GA(ctx, synth.ga_limits(lig), population=args.pop, offspring=args.pop, seed=2024, temperature=300.0,
@@ -296,6 +306,11 @@ def modes_leg(args):
         ctx.upload_system(system)
         ctx.cluster_cf(0, genes[:1000], e[:1000], temperature=300.0)          # warm
         r = [ctx.cluster_cf(0, genes, e, temperature=300.0, rmsd=2.0, max_results=10) for _ in range(3)]
+        # the FastOPTICS super-cluster at its largest size
+        n_sc = hip.SUPER_CLUSTER_MAX
+        e_sc = rng.normal(-20.0, 8.0, n_sc)
+        ctx.super_cluster(e_sc[:100], 4)                                      # warm
+        sc = [ctx.super_cluster(e_sc, max(4, n_sc // 20)) for _ in range(3)]
     ms = float(np.median([x["device_ms"] for x in r]))
     passes = len(r[0]["modes"])
     # coordinate bytes one head pass reads (component-major float32)
@@ -303,7 +318,9 @@ def modes_leg(args):
     return {"poses": len(genes), "ligand_atoms": lig.n_atoms, "modes": passes, "clustered": r[0]["n_clustered"],
             "device_ms": round(ms, 4), "poses_per_s": round(len(genes) / (ms / 1e3), 1),
             "coordinate_bytes_per_pass": pass_bytes,
-            "note": "device_ms covers the coordinate build and every head pass"}
+            "note": "device_ms covers the coordinate build and every head pass",
+            "super_cluster": {"points": n_sc, "min_pts": max(4, n_sc // 20), "members": int(len(sc[0]["members"])),
+                              "device_ms": round(float(np.median([x["device_ms"] for x in sc])), 4)}}
 
 
 def _tune_generation(ctx, system, pop, seed=5):

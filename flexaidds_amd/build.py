@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libflexaidds_amd.so")
 SOURCES = ["cf_kernel.hip", "statmech.hip", "fa_context.hip", "ga_kernels.hip", "ga.hip", "pool.hip", "hip_eval.hip",
-           "noalloc.hip", "cluster.hip", "serve.hip"]
+           "noalloc.hip", "cluster.hip", "optics.hip", "serve.hip"]
 ARCH = os.environ.get("FA_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

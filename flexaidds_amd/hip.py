@@ -86,6 +86,7 @@ _SIGS = {
     "fa_grand_device": ([_P, _P, C.c_int, _P, C.c_int, _P, _P], C.c_int),
     "fa_serve_run": ([_P, C.c_int, _P, C.c_int, C.c_int, _P, _P], C.c_int),
     "fa_cluster_cf": ([_P, C.c_int, C.c_int, _P, C.c_int, _P, _P, _P, _P, _P, _P, _P], C.c_int),
+    "fa_super_cluster": ([_P, _P, C.c_int, C.c_int, _P, _P], C.c_int),
     "fa_ga_run": ([_P, C.c_int, C.c_int, _P], C.c_int),
     "fa_pool_create": ([_P, _P, _P], C.c_int),
     "fa_pool_shape": ([_P, _P, _P, _P], C.c_int),
@@ -270,6 +271,17 @@ class Context:
                 "member": member[:n], "rmsd_head": rh[:n], "top_rmsd": trm[:k * k].reshape(k, k),
                 "n_clustered": int(out.n_clustered), "device_ms": float(out.device_ms)}
 
+    def super_cluster(self, energy, min_pts: int) -> dict:
+        """FastOPTICS super-cluster of the energies as one-dimensional points
+        (fa_super_cluster): {"members": point indices in the flood fill's
+        visiting order, "seed", "device_ms"}."""
+        e = np.ascontiguousarray(energy, np.float64)
+        n = int(e.shape[0])
+        members = np.zeros(max(n, 1), np.int32)
+        out = SuperClusterOut()
+        _check(_lib.fa_super_cluster(self._ctx, e.ctypes.data, n, int(min_pts), members.ctypes.data, C.byref(out)))
+        return {"members": members[:int(out.n_members)], "seed": int(out.seed), "device_ms": float(out.device_ms)}
+
     def opcount(self) -> dict:
         o = OpCount()
         _check(_lib.fa_last_opcount(self._ctx, C.byref(o)))
@@ -337,6 +349,13 @@ class ClusterMode(C.Structure):
 
 class ClusterOut(C.Structure):
     _fields_ = [("n_modes", C.c_int32), ("n_clustered", C.c_int32), ("device_ms", C.c_float), ("pad", C.c_int32)]
+
+
+SUPER_CLUSTER_MAX = 8192   # FA_SUPER_CLUSTER_MAX
+
+
+class SuperClusterOut(C.Structure):
+    _fields_ = [("n_members", C.c_int32), ("seed", C.c_int32), ("device_ms", C.c_float), ("pad", C.c_int32)]
 
 
 class GAParams(C.Structure):

@@ -18,6 +18,10 @@ LIB/BindingMode.cpp:613-625).
                                 weight is exactly 1 (a mode's only pose, or
                                 one whose companions underflow) count
   delta_G(a, b), deltaG_matrix()   free-energy differences (:99-102, :207-222)
+  super_cluster_ensemble()      the dominant energy basin (:153-180): the
+                                FastOPTICS super-cluster of every pose energy
+                                (fa_super_cluster, minPts = max(4, n / 20)),
+                                one sample per member
 
 The vibrational and co-translational corrections the reference adds to a
 mode's free energy are 0 unless their (default-off) models are enabled;
@@ -122,3 +126,19 @@ class BindingPopulation:
         if not c.any():
             return None
         return self.ctx.thermo(e, self.temperature, count=c)
+
+    def super_cluster_ensemble(self) -> dict | None:
+        """Thermodynamics of the poses in the super-cluster of all pose
+        energies (modes in free-energy order, each mode's poses in order),
+        one sample each in the flood fill's order; five poses or fewer fall
+        back to global_ensemble(), as in the reference.  fa_super_cluster
+        takes at most FA_SUPER_CLUSTER_MAX (8192) poses and raises above."""
+        if not self.modes:
+            return self.global_ensemble()
+        e = np.concatenate([m.energies for m in self.modes])
+        if len(e) <= 4:
+            return self.global_ensemble()
+        members = self.ctx.super_cluster(e, max(4, len(e) // 20))["members"]
+        if len(members) == 0:
+            return self.global_ensemble()
+        return self.ctx.thermo(e[members], self.temperature)

@@ -343,6 +343,28 @@ int fa_cluster_cf(fa_ctx* ctx, int ligand, int n_poses, const double* genes, int
                   float* top_rmsd, fa_cluster_out* out);
 
 /* ---------------------------------------------------------------------
+ * Super-cluster of pose energies: FastOPTICS over the energies as
+ * one-dimensional points with SUPER_CLUSTER_ONLY extraction, the filter of
+ * BindingPopulation::get_super_cluster_ensemble (LIB/BindingMode.cpp:153-180;
+ * FastOPTICS::FastOPTICS, computeOrdering, coreDist and extractSuperCluster,
+ * LIB/fast_optics.cpp:8-135).
+ */
+#define FA_SUPER_CLUSTER_MAX 8192
+
+typedef struct fa_super_cluster_out {
+    int32_t n_members;         /* indices written to `members` */
+    int32_t seed;              /* the fill's first point: the ordering entry of least (reach, index) */
+    float device_ms;           /* ranking, core distances, ordering and fill */
+    int32_t pad;
+} fa_super_cluster_out;
+
+/* energy: [n] host, finite; min_pts: FastOPTICS minPts (the binding
+ * population passes max(4, n / 20)); members: [n] host, receives the
+ * super-cluster's point indices in the flood fill's visiting order.
+ * n <= FA_SUPER_CLUSTER_MAX (FA_ERR_LIMIT above).  Synchronous. */
+int fa_super_cluster(fa_ctx* ctx, const double* energy, int n, int min_pts, int* members, fa_super_cluster_out* out);
+
+/* ---------------------------------------------------------------------
  * Docking pool: continuous batching of docking jobs on one context.
  * Replaces running GA() once per request (LIB/gaboom.cpp:346-500) with a
  * resident pool that advances every in-flight job by one generation per

@@ -10,6 +10,11 @@
  *                          calc_rmsd_chrom (LIB/calc_rmsd_chrom.cpp) and
  *                          flexaids::sum_sq_distances_f
  *                          (LIB/simd_distance.h:680-716)
+ *   fa_oracle_super_cluster  FastOPTICS on one-dimensional points with
+ *                          SUPER_CLUSTER_ONLY extraction (LIB/fast_optics.cpp:
+ *                          computeOrdering, coreDist, extractSuperCluster), as
+ *                          get_super_cluster_ensemble calls it
+ *                          (LIB/BindingMode.cpp:150-180)
  *
  * Poses come as ligand coordinates (what calc_rmsd_chrom rebuilds from the
  * genes) in the order given.  dist_mode selects the squared-distance sum of
@@ -24,9 +29,12 @@
  * RMSD between heads to its six printed decimals), and by brute-force
  * properties in tests/test_cluster_oracle.py.
  */
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <limits>
+#include <queue>
 #include <utility>
 #include <vector>
 
@@ -179,6 +187,78 @@ int fa_oracle_cluster_cf(int n, int n_atoms, const float* xyz, const double* ene
                 top_rmsd[a * nc + b] = (float)std::sqrt(s / ((float)n_atoms));
             }
     return nc;
+}
+
+/* The FastOPTICS super-cluster of one-dimensional points: out_idx receives
+ * the point indices in the flood fill's visiting order, *n_out their count.
+ * Distances are sqrt(d * d) in double, as the reference's distance(). */
+int fa_oracle_super_cluster(const double* e, int n, int min_pts, int* out_idx, int* n_out)
+{
+    const double inf = std::numeric_limits<double>::infinity();
+    auto dist = [&](int a, int b) {
+        const double d = e[a] - e[b];
+        return std::sqrt(d * d);
+    };
+    auto core = [&](int v) {
+        if (min_pts <= 1) return 0.0;
+        if (n < min_pts) return inf;
+        std::vector<double> d;
+        for (int j = 0; j < n; ++j)
+            if (j != v) d.push_back(dist(v, j));
+        if ((int)d.size() < min_pts - 1) return inf;
+        std::nth_element(d.begin(), d.begin() + (min_pts - 2), d.end());
+        return d[min_pts - 2];
+    };
+    *n_out = 0;
+    if (n <= 0) return 0;
+    std::vector<double> cd(n);
+    for (int v = 0; v < n; ++v) cd[v] = core(v);
+    // computeOrdering's priority queue holds (max(cd[u], dist(u, j)), j) for
+    // every processed u and every j unprocessed when u was processed; a pop
+    // skips processed j, so it yields the unprocessed j of least (best[j], j),
+    // best[j] the least entry for j so far.  The queue empties only when no
+    // processed point of the run had a finite core distance (one that has
+    // pushes every unprocessed point): every best[j] is then infinite and the
+    // least unprocessed index opens the next run with an infinite reach.
+    std::vector<char> done(n, 0);
+    std::vector<double> best(n, inf);
+    std::vector<std::pair<int, double>> order;
+    int u = 0;
+    done[0] = 1;
+    order.push_back({0, inf});
+    for (int step = 1; step < n; ++step) {
+        int w = -1;
+        for (int j = 0; j < n; ++j) {
+            if (done[j]) continue;
+            if (cd[u] < inf) best[j] = std::min(best[j], std::max(cd[u], dist(u, j)));
+            if (w < 0 || best[j] < best[w]) w = j;
+        }
+        done[w] = 1;
+        order.push_back({w, best[w]});
+        u = w;
+    }
+    // the entry of least reach (ties: the smaller point index) seeds a flood
+    // fill: v joins from u when dist(u, v) <= 0.8 coreDist(v)
+    size_t sp = 0;
+    for (size_t q = 1; q < order.size(); ++q)
+        if (order[q].second < order[sp].second || (order[q].second == order[sp].second && order[q].first < order[sp].first))
+            sp = q;
+    const int seed = order[sp].first;
+    std::vector<char> vis(n, 0);
+    std::queue<int> fifo;
+    fifo.push(seed);
+    vis[seed] = 1;
+    while (!fifo.empty()) {
+        const int u = fifo.front();
+        fifo.pop();
+        out_idx[(*n_out)++] = u;
+        for (int v = 0; v < n; ++v)
+            if (!vis[v] && dist(u, v) <= 0.8 * cd[v]) {
+                vis[v] = 1;
+                fifo.push(v);
+            }
+    }
+    return 0;
 }
 
 }  // extern "C"

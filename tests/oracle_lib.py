@@ -63,6 +63,8 @@ def load_oracle():
     lib.fa_oracle_cluster_cf.argtypes = [C.c_int, C.c_int, _P, _P, C.c_double, C.c_float, C.c_int, C.c_int, _P, _P,
                                          _P, _P, _P, _P, _P]
     lib.fa_oracle_cluster_cf.restype = C.c_int
+    lib.fa_oracle_super_cluster.argtypes = [_P, C.c_int, C.c_int, _P, _P]
+    lib.fa_oracle_super_cluster.restype = C.c_int
     return lib
 
 
@@ -75,7 +77,20 @@ def load_ref(fast=False):
     lib.fa_ref_eval.restype = C.c_int
     lib.fa_ref_cluster.argtypes = [_P, _P, _P, C.c_int, _P, _P, _P, C.c_uint, C.c_float, C.c_int, C.c_char_p]
     lib.fa_ref_cluster.restype = C.c_int
+    lib.fa_ref_super_cluster.argtypes = [_P, C.c_int, C.c_int, _P, _P]
+    lib.fa_ref_super_cluster.restype = C.c_int
     return lib
+
+
+def super_cluster(lib, energy, min_pts, ref=False):
+    """Point indices of the FastOPTICS super-cluster, in visiting order
+    (the oracle's, or with ref=True the reference's own)."""
+    e = np.ascontiguousarray(energy, np.float64)
+    out = np.zeros(max(len(e), 1), np.int32)
+    n = C.c_int()
+    f = lib.fa_ref_super_cluster if ref else lib.fa_oracle_super_cluster
+    assert f(e.ctypes.data, len(e), int(min_pts), out.ctypes.data, C.byref(n)) == 0
+    return out[:n.value]
 
 
 def oracle_cluster(lib, xyz, energy, T=0.0, rmsd=2.0, max_results=10, mode=1):

@@ -58,7 +58,8 @@ def test_struct_sizes_against_the_c_compiler(tmp_path):
                "fa_cf_result": abi.CFResult, "fa_thermo": abi.Thermo, "fa_gene_lim": hip.GeneLim,
                "fa_ga_params": hip.GAParams, "fa_pool_params": hip.PoolParams, "fa_job": hip.Job,
                "fa_job_result": hip.JobResult, "fa_cluster_params": hip.ClusterParams,
-               "fa_cluster_mode": hip.ClusterMode, "fa_cluster_out": hip.ClusterOut, "fa_request": hip.Request,
+               "fa_cluster_mode": hip.ClusterMode, "fa_cluster_out": hip.ClusterOut, "fa_super_cluster_out": hip.SuperClusterOut,
+               "fa_request": hip.Request,
                "fa_served": hip.ServedRec, "fa_serve_stats": hip.ServeStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "flexaidds_amd.h"', "int main(void) {"]
     for cname, py in mirrors.items():
@@ -76,3 +77,10 @@ def test_struct_sizes_against_the_c_compiler(tmp_path):
         assert int(got[cname]) == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
+
+
+def test_super_cluster_limit_matches_header():
+    import re
+    from flexaidds_amd import hip
+    text = open(os.path.join(ROOT, "include", "flexaidds_amd.h")).read()
+    assert int(re.search(r"#define FA_SUPER_CLUSTER_MAX (\d+)", text).group(1)) == hip.SUPER_CLUSTER_MAX

@@ -158,3 +158,57 @@ def test_zero_records_sorted_in_like_the_reference(tmp_path, oracle, system):
     _same_modes(ora, refc)
     zeros = [m for m in ora["modes"] if m["freq"] == 0]
     assert zeros and all(m["acf"] == 0 and m["tcf"] == 0 and m["top"] == 0 for m in zeros)
+
+
+@pytest.mark.parametrize("case", range(8))
+def test_super_cluster_equals_reference(oracle, case):
+    # the oracle's FastOPTICS super-cluster against the reference's own
+    # (fast_optics.cpp built from its sources) on one-dimensional energy
+    # sets: spread, clustered, with repeated values (ties in the reach and
+    # in the core distances), the minPts the binding population uses
+    ref = ol.load_ref()
+    if ref is None:
+        pytest.skip("reference build (oracle/_ref) absent")
+    rng = np.random.default_rng(100 + case)
+    n = [5, 12, 60, 200, 333, 400, 90, 150][case]
+    if case % 3 == 0:
+        e = rng.normal(-20.0, 6.0, n)
+    elif case % 3 == 1:
+        e = np.concatenate([rng.normal(c, 0.5, n // 3) for c in (-40.0, -25.0, -5.0)] + [rng.normal(0, 30, n - 3 * (n // 3))])
+    else:
+        e = np.round(rng.normal(-15.0, 4.0, n), 1)        # many equal energies
+    mp = max(4, n // 20)
+    got = ol.super_cluster(oracle, e, mp)
+    want = ol.super_cluster(ref, e, mp, ref=True)
+    assert np.array_equal(got, want)
+    assert len(got) >= 1 and len(set(got.tolist())) == len(got)
+
+
+def test_super_cluster_by_definition(oracle):
+    # two far groups: the flood fill never crosses the gap, every member is
+    # within 0.8 core distances of a member visited before it, and no
+    # outsider is within reach of any member
+    rng = np.random.default_rng(7)
+    e = np.concatenate([rng.normal(-50.0, 0.3, 40), rng.normal(10.0, 0.3, 25)])
+    mp = max(4, len(e) // 20)
+    got = ol.super_cluster(oracle, e, mp)
+    inside = set(got.tolist())
+    assert inside <= set(range(40)) or inside <= set(range(40, 65))
+    d = np.abs(e[:, None] - e[None, :])
+    cd = np.array([np.sort(np.delete(d[v], v))[mp - 2] for v in range(len(e))])
+    for q, v in enumerate(got[1:], 1):
+        assert np.any(d[got[:q], v] <= 0.8 * cd[v])
+    for v in set(range(len(e))) - inside:
+        assert not np.any(d[got, v] <= 0.8 * cd[v])
+
+
+@pytest.mark.parametrize("mp", [1, 2, 3, 7, 149, 150, 151, 1000])
+def test_super_cluster_min_pts_edges(oracle, mp):
+    # zero core distances (minPts <= 1), minPts near and above n (infinite
+    # core distances: every point opens its own run), against the reference
+    ref = ol.load_ref()
+    if ref is None:
+        pytest.skip("reference build (oracle/_ref) absent")
+    rng = np.random.default_rng(mp)
+    e = np.round(rng.normal(-12.0, 5.0, 150), 2)
+    assert np.array_equal(ol.super_cluster(oracle, e, mp), ol.super_cluster(ref, e, mp, ref=True))

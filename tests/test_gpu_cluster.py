@@ -158,3 +158,71 @@ def test_binding_population(oracle, scored):
         assert abs(ge["free_energy"] - ref["free_energy"]) <= 1e-9 * max(1.0, abs(ref["free_energy"]))
     else:
         assert ge is None
+
+
+def _energy_sets():
+    # spread, three basins with outliers, many equal energies, two far groups
+    rng = np.random.default_rng(31)
+    out = []
+    for n in (1, 2, 5, 60, 333, 1000, 1025, 2000, 4097, 8192):
+        kind = len(out) % 4
+        if kind == 0:
+            e = rng.normal(-20.0, 6.0, n)
+        elif kind == 1:
+            e = np.concatenate([rng.normal(c, 0.5, n // 3) for c in (-40.0, -25.0, -5.0)] +
+                               [rng.normal(0.0, 30.0, n - 3 * (n // 3))])
+        elif kind == 2:
+            e = np.round(rng.normal(-15.0, 4.0, n), 1)
+        else:
+            e = np.concatenate([rng.normal(-50.0, 0.3, n - n // 3), rng.normal(10.0, 0.3, n // 3)])
+        out.append(e)
+    return out
+
+
+@pytest.mark.parametrize("case", range(10))
+def test_super_cluster_equals_oracle(hip, oracle, case):
+    # the device's FastOPTICS super-cluster (ranking, core distances,
+    # ordering, seed, flood fill) against the oracle's, member for member in
+    # visiting order, at the binding population's minPts and at the edges
+    # (minPts 1: zero core distances; minPts above n: infinite ones)
+    e = _energy_sets()[case]
+    n = len(e)
+    ctx = hip.Context()
+    try:
+        for mp in sorted({max(4, n // 20), 2, 1, n + 1}):
+            print(f"super_cluster n={n} min_pts={mp}", flush=True)
+            got = ctx.super_cluster(e, mp)
+            want = ol.super_cluster(oracle, e, mp)
+            assert np.array_equal(got["members"], want), (n, mp)
+            assert got["seed"] == want[0]
+    finally:
+        ctx.close()
+
+
+def test_super_cluster_limits(hip):
+    ctx = hip.Context()
+    try:
+        assert len(ctx.super_cluster(np.zeros(0), 4)["members"]) == 0
+        with pytest.raises(hip.FAError):
+            ctx.super_cluster(np.zeros(8193), 4)
+        with pytest.raises(hip.FAError):
+            ctx.super_cluster(np.array([0.0, np.nan, 1.0, 2.0, 3.0]), 4)
+    finally:
+        ctx.close()
+
+
+def test_super_cluster_ensemble(oracle, scored):
+    # get_super_cluster_ensemble: all pose energies in mode order, minPts
+    # max(4, n / 20), one sample per member, against the oracle's members
+    # and statistics
+    from flexaidds_amd import modes
+    ctx, _, genes, res = scored
+    T = 300.0
+    pop = modes.BindingPopulation(ctx, 0, genes, res["app"], temperature=T, rmsd=2.0, max_results=10)
+    e = np.concatenate([m.energies for m in pop.modes])
+    assert len(e) > 4
+    members = ol.super_cluster(oracle, e, max(4, len(e) // 20))
+    ref = _oracle_thermo(oracle, e[members], T)
+    sc = pop.super_cluster_ensemble()
+    for k in ("free_energy", "mean_energy", "log_Z"):
+        assert abs(sc[k] - ref[k]) <= 1e-9 * max(1.0, abs(ref[k])), k
