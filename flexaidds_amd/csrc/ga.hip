@@ -7,7 +7,17 @@
 // ensemble, which gives its ln Z and, over jobs, the grand partition
 // function.  One generation (ga_kernels.h) allocates nothing and records no
 // memset, so it is captured once as a HIP graph and replayed.
+//
+// Snapshot (fa_ga_snapshot_enable / fa_ga_snapshot): after each generation
+// the population is recorded in HBM (the reference's save_snapshot after
+// reproduce, SAVE_CHROM_FRACTION 1.0: LIB/gaboom.cpp:333, :494); at the end
+// a job's records are sorted by energy (a radix sort of order-preserving
+// keys, stable: records of equal energy keep record order) and thinned by
+// remove_dups (LIB/gaboom.cpp:2728-2746: a record is dropped when every gene
+// lies within 0.1 of the last record kept), one wave testing 64 records
+// against the last kept record at a time.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -26,6 +36,9 @@ struct fa_ga {
     DevBuf dset, dzero, rkey, rval, rzero, csig, cdup, need, ddstats;  // duplicate rejection
     DevBuf skey, sidx;  // survivor ranking: candidates sorted within tiles
     DevBuf prank;       // STEADY: parent ranks among the parents
+    DevBuf snap, snapwork;  // snapshot records; sorting and thinning scratch
+    int snap_max = 0, snap_gens = 0;  // generations the snapshot holds / has taken
+    std::vector<int> Gjob;  // genes of each job
     long long gens_run = 0;   // generations enqueued (bounds the signatures accepted)
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -35,6 +48,97 @@ struct fa_ga {
 namespace {
 
 int ga_blocks(long long n) { return (int)std::max(1LL, (n + 255) / 256); }
+
+// ------------------------------------------------------------ snapshot
+// record slot `slot` of every job: its population rows -> records
+// [job * cap + slot * pop, +pop)
+__global__ void ga_snap_append_kernel(const double* __restrict__ gic, const fa_cf_result* __restrict__ res, int jobs,
+                                      int pop, int Gmax, long long cap, int slot, double* __restrict__ sic,
+                                      double* __restrict__ se, double* __restrict__ sa)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)jobs * pop * Gmax) return;
+    const long long row = i / Gmax;
+    const int g = (int)(i % Gmax);
+    const long long job = row / pop, r = row % pop;
+    const long long dst = job * cap + (long long)slot * pop + r;
+    sic[dst * Gmax + g] = gic[row * Gmax + g];
+    if (g == 0) {
+        se[dst] = res[row].cf;
+        sa[dst] = res[row].app;
+    }
+}
+
+// ascending doubles -> ascending unsigned keys
+__global__ void ga_snap_keys_kernel(const double* __restrict__ e, int n, unsigned long long* __restrict__ key,
+                                    int* __restrict__ idx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = e[i] == 0.0 ? 0.0 : e[i];  // -0 sorts with +0, as the comparison e_a < e_b has it
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    key[i] = (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+    idx[i] = i;
+}
+
+// records in sorted order, the job's G genes each
+__global__ void ga_snap_gather_kernel(const int* __restrict__ order, int n, int G, int Gmax,
+                                      const double* __restrict__ sic, const double* __restrict__ se,
+                                      const double* __restrict__ sa, double* __restrict__ oic,
+                                      double* __restrict__ oe, double* __restrict__ oa)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n * G) return;
+    const long long k = i / G;
+    const int g = (int)(i % G);
+    const long long r = order[k];
+    oic[i] = sic[r * Gmax + g];
+    if (g == 0) {
+        oe[k] = se[r];
+        oa[k] = sa[r];
+    }
+}
+
+// remove_dups over sorted records (rows of G genes): record 0 is kept; a
+// record is kept when some gene differs by 0.1 or more from the last record
+// kept.  One wave: 64 records are tested against the last kept record at a
+// time, and the first that differs is kept next.
+__global__ __launch_bounds__(64) void ga_snap_dedup_kernel(const double* __restrict__ sic, int n, int G,
+                                                            int* __restrict__ kept, int* __restrict__ n_kept)
+{
+    __shared__ double last[FA_MAX_GENES];
+    const int lane = threadIdx.x;
+    if (n <= 0) {
+        if (lane == 0) *n_kept = 0;
+        return;
+    }
+    if (lane < G) last[lane] = sic[lane];
+    if (lane == 0) kept[0] = 0;
+    __syncthreads();
+    int cnt = 1, pos = 1;
+    while (pos < n) {
+        const int j = pos + lane;
+        bool differs = false;
+        if (j < n) {
+            int flag = 0;
+            for (int g = 0; g < G; ++g) flag += fabs(sic[(size_t)j * G + g] - last[g]) < 0.1;
+            differs = flag != G;
+        }
+        const unsigned long long b = __builtin_amdgcn_ballot_w64(differs);
+        if (b == 0ull) {
+            pos += 64;
+            continue;
+        }
+        const int r = pos + (int)__builtin_ctzll(b);
+        if (lane == 0) kept[cnt] = r;
+        ++cnt;
+        __syncthreads();
+        if (lane < G) last[lane] = sic[(size_t)r * G + lane];
+        __syncthreads();
+        pos = r + 1;
+    }
+    if (lane == 0) *n_kept = cnt;
+}
 
 int ceil_log2(long long n)
 {
@@ -174,6 +278,7 @@ int ga_build(fa_ctx* c, const fa_ga_params* p, int n_jobs, const int* ligands, c
     ga->p = *p;
     ga->jobs = n_jobs;
     ga->Gmax = Gmax;
+    ga->Gjob = G;
     const size_t J = (size_t)n_jobs, n_all = (size_t)(pop + n_new), rows_all = J * (n_all + (size_t)n_cand);
     auto fail_free = [&](int code) {
         fa_ga_destroy(ga);
@@ -359,6 +464,18 @@ int fa_ga_run(fa_ga* ga, int generations, int use_graph, float* ms)
         } else {
             rc = ga_generation(ga);
         }
+        if (rc == FA_SUCCESS && ga->snap_gens < ga->snap_max) {
+            const fa::GaDev& d = ga->d;
+            const long long cap = (long long)ga->snap_max * d.pop;
+            double* sic = static_cast<double*>(ga->snap.p);
+            double* se = sic + (size_t)ga->jobs * cap * d.Gmax;
+            hipLaunchKernelGGL(ga_snap_append_kernel, dim3(ga_blocks((long long)ga->jobs * d.pop * d.Gmax)), dim3(256),
+                               0, ga->stream, d.gic, d.res, ga->jobs, d.pop, d.Gmax, cap, ga->snap_gens, sic, se,
+                               se + (size_t)ga->jobs * cap);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) rc = fa_fail(FA_ERR_HIP, std::string("snapshot: ") + hipGetErrorString(e));
+            ++ga->snap_gens;
+        }
     }
     FA_HIP(hipEventRecord(e1, ga->stream));
     FA_HIP(hipEventSynchronize(e1));
@@ -452,6 +569,90 @@ int fa_ga_dedup_stats(fa_ga* ga, uint64_t* stats)
     return FA_SUCCESS;
 }
 
+int fa_ga_snapshot_enable(fa_ga* ga, int max_generations)
+{
+    if (!ga) return fa_fail(FA_ERR_INVALID, "null GA");
+    if (max_generations < 0) return fa_fail(FA_ERR_INVALID, "negative generation count");
+    const long long cap = (long long)max_generations * ga->d.pop;
+    if (cap >= (1LL << 31)) return fa_fail(FA_ERR_LIMIT, "snapshot above 2^31 records per job");
+    FA_HIP(hipSetDevice(ga->ctx->device));
+    FA_HIP(hipStreamSynchronize(ga->stream));
+    ga->snap_max = 0;
+    ga->snap_gens = 0;
+    if (cap > 0) {
+        const int rc = fa_ensure(ga->snap, sizeof(double) * (size_t)ga->jobs * cap * (ga->d.Gmax + 2));
+        if (rc != FA_SUCCESS) return rc;
+    }
+    ga->snap_max = max_generations;
+    return FA_SUCCESS;
+}
+
+int fa_ga_snapshot(fa_ga* ga, int job, double* ic, double* evalue, double* app, int32_t* n_raw, int32_t* n_kept)
+{
+    if (!ga || !n_raw || !n_kept) return fa_fail(FA_ERR_INVALID, "null argument");
+    if (job < 0 || job >= ga->jobs) return fa_fail(FA_ERR_INVALID, "job index out of range");
+    const long long n = (long long)ga->snap_gens * ga->d.pop;
+    *n_raw = (int32_t)n;
+    *n_kept = 0;
+    if (!ic && !evalue && !app) return FA_SUCCESS;  // the size alone
+    if (!ic || !evalue || !app) return fa_fail(FA_ERR_INVALID, "ic, evalue and app together");
+    if (n == 0) return FA_SUCCESS;
+    FA_HIP(hipSetDevice(ga->ctx->device));
+    hipStream_t s = ga->stream;
+    const int G = ga->Gjob[job], Gmax = ga->d.Gmax;
+    const long long cap = (long long)ga->snap_max * ga->d.pop;
+    const double* sic = static_cast<const double*>(ga->snap.p) + (size_t)job * cap * Gmax;
+    const double* se = static_cast<const double*>(ga->snap.p) + (size_t)ga->jobs * cap * Gmax + (size_t)job * cap;
+    const double* sa = se + (size_t)ga->jobs * cap;
+    size_t tmp = 0;
+    FA_HIP(rocprim::radix_sort_pairs(nullptr, tmp, (unsigned long long*)nullptr, (unsigned long long*)nullptr,
+                                     (int*)nullptr, (int*)nullptr, (unsigned)n, 0, 64, s));
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t bk = al(8 * (size_t)n), bi = al(4 * (size_t)n), bg = al(8 * (size_t)n * G);
+    FA_HIP(hipStreamSynchronize(s));  // the scratch may be regrown
+    const int rc = fa_ensure(ga->snapwork, 2 * bk + 2 * bi + 2 * bg + 4 * bk + bi + 256 + al(tmp));
+    if (rc != FA_SUCCESS) return rc;
+    char* w = static_cast<char*>(ga->snapwork.p);
+    auto take = [&](size_t b) {
+        char* q = w;
+        w += b;
+        return q;
+    };
+    auto* k0 = reinterpret_cast<unsigned long long*>(take(bk));
+    auto* k1 = reinterpret_cast<unsigned long long*>(take(bk));
+    int* v0 = reinterpret_cast<int*>(take(bi));
+    int* v1 = reinterpret_cast<int*>(take(bi));
+    double* tic = reinterpret_cast<double*>(take(bg));
+    double* oic = reinterpret_cast<double*>(take(bg));
+    double* te = reinterpret_cast<double*>(take(bk));
+    double* ta = reinterpret_cast<double*>(take(bk));
+    double* oe = reinterpret_cast<double*>(take(bk));
+    double* oa = reinterpret_cast<double*>(take(bk));
+    int* kept = reinterpret_cast<int*>(take(bi));
+    int* nk = reinterpret_cast<int*>(take(256));
+    void* t = take(al(tmp));
+    hipLaunchKernelGGL(ga_snap_keys_kernel, dim3(ga_blocks(n)), dim3(256), 0, s, se, (int)n, k0, v0);
+    FA_HIP(hipGetLastError());
+    FA_HIP(rocprim::radix_sort_pairs(t, tmp, k0, k1, v0, v1, (unsigned)n, 0, 64, s));
+    hipLaunchKernelGGL(ga_snap_gather_kernel, dim3(ga_blocks(n * G)), dim3(256), 0, s, v1, (int)n, G, Gmax, sic, se,
+                       sa, tic, te, ta);
+    hipLaunchKernelGGL(ga_snap_dedup_kernel, dim3(1), dim3(64), 0, s, tic, (int)n, G, kept, nk);
+    FA_HIP(hipGetLastError());
+    int m = 0;
+    FA_HIP(hipMemcpyAsync(&m, nk, sizeof(int), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    if (m < 1 || m > n) return fa_fail(FA_ERR_INVALID, "snapshot: bad kept count from the device");
+    hipLaunchKernelGGL(ga_snap_gather_kernel, dim3(ga_blocks((long long)m * G)), dim3(256), 0, s, kept, m, G, G, tic,
+                       te, ta, oic, oe, oa);
+    FA_HIP(hipGetLastError());
+    FA_HIP(hipMemcpyAsync(ic, oic, sizeof(double) * (size_t)m * G, hipMemcpyDeviceToHost, s));
+    FA_HIP(hipMemcpyAsync(evalue, oe, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, s));
+    FA_HIP(hipMemcpyAsync(app, oa, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    *n_kept = m;
+    return FA_SUCCESS;
+}
+
 int fa_ga_destroy(fa_ga* ga)
 {
     if (!ga) return FA_SUCCESS;
@@ -462,7 +663,7 @@ int fa_ga_destroy(fa_ga* ga)
     for (DevBuf* b : {&ga->rank, &ga->gi, &ga->gic, &ga->res, &ga->gi2, &ga->gic2, &ga->res2, &ga->cum, &ga->lim, &ga->ngenes,
                       &ga->ens, &ga->cnt, &ga->lig, &ga->thermo, &ga->grand, &ga->fit, &ga->fit2, &ga->share,
                       &ga->sig, &ga->dset, &ga->dzero, &ga->rkey, &ga->rval, &ga->rzero, &ga->csig, &ga->cdup,
-                      &ga->need, &ga->ddstats, &ga->skey, &ga->sidx, &ga->prank})
+                      &ga->need, &ga->ddstats, &ga->skey, &ga->sidx, &ga->prank, &ga->snap, &ga->snapwork})
         if (b->p) (void)hipFree(b->p);
     if (ga->stream) (void)hipStreamDestroy(ga->stream);
     --ga->ctx->live_users;

@@ -82,6 +82,8 @@ _SIGS = {
     "fa_ga_shape": ([_P, _P, _P], C.c_int),
     "fa_ga_fitness": ([_P, _P, _P], C.c_int),
     "fa_ga_dedup_stats": ([_P, _P], C.c_int),
+    "fa_ga_snapshot_enable": ([_P, C.c_int], C.c_int),
+    "fa_ga_snapshot": ([_P, C.c_int, _P, _P, _P, _P, _P], C.c_int),
     "fa_ga_grand": ([_P, _P, _P, _P, _P], C.c_int),
     "fa_grand_device": ([_P, _P, C.c_int, _P, C.c_int, _P, _P], C.c_int),
     "fa_serve_run": ([_P, C.c_int, _P, C.c_int, C.c_int, _P, _P], C.c_int),
@@ -461,6 +463,30 @@ class GA:
         g = C.c_uint64()
         _check(_lib.fa_ga_generation(self._ga, C.byref(g)))
         return int(g.value)
+
+    def snapshot_enable(self, max_generations: int):
+        """Record each job's population after every later generation, for up
+        to max_generations generations (fa_ga_snapshot_enable)."""
+        _check(_lib.fa_ga_snapshot_enable(self._ga, int(max_generations)))
+
+    def snapshot(self, job: int = 0) -> dict:
+        """Job `job`'s records sorted by energy without duplicates
+        (fa_ga_snapshot): {"ic" [kept, genes], "evalue", "app", "n_raw"}."""
+        nr, nk = C.c_int32(), C.c_int32()
+        _check(_lib.fa_ga_snapshot(self._ga, job, None, None, None, C.byref(nr), C.byref(nk)))
+        n, G = int(nr.value), self.job_genes[job]
+        ic = np.zeros((max(n, 1), G))
+        e = np.zeros(max(n, 1))
+        a = np.zeros(max(n, 1))
+        _check(_lib.fa_ga_snapshot(self._ga, job, ic.ctypes.data, e.ctypes.data, a.ctypes.data, C.byref(nr),
+                                   C.byref(nk)))
+        k = int(nk.value)
+        return {"ic": ic[:k], "evalue": e[:k], "app": a[:k], "n_raw": n}
+
+    def snapshot_thermo(self, job: int = 0) -> dict:
+        """The end-of-run ensemble: every kept record's energy (cf) once
+        (LIB/gaboom.cpp:535-540), on the device."""
+        return self.ctx.thermo(self.snapshot(job)["evalue"], self.params.temperature)
 
     def dedup_stats(self) -> np.ndarray:
         """[jobs, 2]: duplicate candidates drawn, rows that kept a duplicate

@@ -292,6 +292,23 @@ int fa_ga_fitness(fa_ga* ga, double* fitness, double* sig_share);
  * that kept a duplicate because dedup_rounds batches did not fill them (the
  * reference draws until they are filled).  Zeros when dedup_rounds is 0. */
 int fa_ga_dedup_stats(fa_ga* ga, uint64_t* stats);
+
+/* Snapshot of the GA's populations for the end-of-run ensemble and the
+ * clustering (the reference's chrom_snapshot, LIB/gaboom.cpp:333, :494,
+ * :517-528).  fa_ga_snapshot_enable records each job's population after
+ * every later generation of fa_ga_run (save_snapshot after reproduce,
+ * SAVE_CHROM_FRACTION 1.0), for up to max_generations generations (the
+ * reference sizes its snapshot for max_generations); it clears earlier
+ * records (0 turns recording off).  fa_ga_snapshot sorts job `job`'s
+ * records by energy (QuickSort by evalue; records of equal energy keep
+ * record order: generation, then population rank) and drops every record
+ * whose genes all lie within 0.1 of the last record kept (remove_dups,
+ * LIB/gaboom.cpp:2728-2746).  n_raw receives the records taken, n_kept the
+ * records kept; ic [n_raw * genes of the job], evalue [n_raw] (cf) and app
+ * [n_raw] (host) receive the kept records in order.  With ic, evalue and
+ * app all null only n_raw is set.  Synchronous. */
+int fa_ga_snapshot_enable(fa_ga* ga, int max_generations);
+int fa_ga_snapshot(fa_ga* ga, int job, double* ic, double* evalue, double* app, int32_t* n_raw, int32_t* n_kept);
 int fa_ga_destroy(fa_ga* ga);
 
 /* ---------------------------------------------------------------------

@@ -22,6 +22,11 @@
  *                     LIB/statmech.cpp:178-201; boltzmann_scalar,
  *                     LIB/UnifiedHardwareDispatch.cpp:608-624)
  *   sig_share         LIB/gaboom.cpp:300-309
+ *   snapshot          LIB/gaboom.cpp:517-528: the records sorted by evalue
+ *                     (a stable sort: records of equal energy keep record
+ *                     order, where the reference's QuickSort, :2674-2724,
+ *                     leaves an order of its own) and remove_dups
+ *                     (:2728-2746) (fa_oracle_snapshot)
  */
 #include <algorithm>
 #include <cmath>
@@ -384,6 +389,28 @@ void fa_oracle_crossover(int32_t* john, int32_t* mary, int G, int intragenes, ui
                          uint64_t stream)
 {
     crossover(john, mary, G, intragenes, seed, gen, stream);
+}
+
+/* Kept records of a snapshot (rows of G internal coordinates, energies
+ * evalue) in order: order[0..*n_kept) receives record indices. */
+int fa_oracle_snapshot(const double* ic, const double* evalue, int n, int G, int* order, int* n_kept)
+{
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return evalue[a] < evalue[b]; });
+    int m = 0;
+    for (int j = 0; j < n; ++j) {
+        if (m > 0) {
+            const double* last = ic + (size_t)order[m - 1] * G;
+            const double* cur = ic + (size_t)idx[j] * G;
+            int close = 0;
+            for (int g = 0; g < G; ++g) close += std::fabs(cur[g] - last[g]) < 0.1;
+            if (close == G) continue;
+        }
+        order[m++] = idx[j];
+    }
+    *n_kept = m;
+    return 0;
 }
 
 }  // extern "C"

@@ -63,6 +63,8 @@ def load_oracle():
     lib.fa_oracle_cluster_cf.argtypes = [C.c_int, C.c_int, _P, _P, C.c_double, C.c_float, C.c_int, C.c_int, _P, _P,
                                          _P, _P, _P, _P, _P]
     lib.fa_oracle_cluster_cf.restype = C.c_int
+    lib.fa_oracle_snapshot.argtypes = [_P, _P, C.c_int, C.c_int, _P, _P]
+    lib.fa_oracle_snapshot.restype = C.c_int
     lib.fa_oracle_super_cluster.argtypes = [_P, C.c_int, C.c_int, _P, _P]
     lib.fa_oracle_super_cluster.restype = C.c_int
     return lib
@@ -80,6 +82,17 @@ def load_ref(fast=False):
     lib.fa_ref_super_cluster.argtypes = [_P, C.c_int, C.c_int, _P, _P]
     lib.fa_ref_super_cluster.restype = C.c_int
     return lib
+
+
+def snapshot(lib, ic, evalue):
+    """Record indices the oracle's snapshot keeps (sorted by energy, then
+    remove_dups), in order."""
+    ic = np.ascontiguousarray(ic, np.float64)
+    e = np.ascontiguousarray(evalue, np.float64)
+    out = np.zeros(max(len(e), 1), np.int32)
+    m = C.c_int()
+    assert lib.fa_oracle_snapshot(ic.ctypes.data, e.ctypes.data, len(e), ic.shape[1], out.ctypes.data, C.byref(m)) == 0
+    return out[:m.value]
 
 
 def super_cluster(lib, energy, min_pts, ref=False):

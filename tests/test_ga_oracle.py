@@ -299,3 +299,50 @@ def test_breed_dedup_against_seen(gao):
     assert np.array_equal(child, cands[kept])
     assert not {_rounded(r) for r in ic} & {_rounded(r) for r in pic}
     assert int(st[0]) == kept[-1] + 1 - n_new > 0
+
+
+def _snapshot_brute(ic, e):
+    # the definition: records by increasing energy (ties in record order),
+    # then each kept unless every gene is within 0.1 of the last kept one
+    order = sorted(range(len(e)), key=lambda i: (e[i], i))
+    kept = []
+    for i in order:
+        if kept and all(abs(ic[i][g] - ic[kept[-1]][g]) < 0.1 for g in range(len(ic[i]))):
+            continue
+        kept.append(i)
+    return kept
+
+
+def test_snapshot_oracle_by_definition(oracle):
+    rng = np.random.default_rng(3)
+    base = rng.uniform(-10, 10, (40, 6))
+    rows, e = [], []
+    for k in range(300):
+        b = base[rng.integers(40)]
+        kind = k % 4
+        if kind == 0:
+            r = b.copy()                                     # an exact repeat (a survivor recorded again)
+        elif kind == 1:
+            r = b + rng.uniform(-0.09, 0.09, 6)              # within 0.1 of it
+        elif kind == 2:
+            r = b.copy()
+            r[rng.integers(6)] += rng.choice([-1, 1]) * rng.uniform(0.1, 0.3)   # one gene apart
+        else:
+            r = rng.uniform(-10, 10, 6)
+        rows.append(r)
+        e.append(round(float(np.sum(r)) * 0.3, 1 if k % 5 == 0 else 9))   # some equal energies
+    ic, e = np.array(rows), np.array(e)
+    got = ol.snapshot(oracle, ic, e).tolist()
+    assert got == _snapshot_brute(ic, e)
+    assert len(got) < len(e)
+    assert np.all(np.diff(e[got]) >= 0)
+
+
+def test_snapshot_oracle_chain(oracle):
+    # not transitive: b is within 0.1 of a and dropped, c is within 0.1 of b
+    # but 0.16 from a (the last kept), so c is kept
+    ic = np.array([[0.0, 0.0], [0.08, 0.0], [0.16, 0.0], [0.17, 0.0]])
+    e = np.array([-3.0, -2.0, -1.0, 0.0])
+    assert ol.snapshot(oracle, ic, e).tolist() == [0, 2]
+    assert ol.snapshot(oracle, ic[:1], e[:1]).tolist() == [0]
+    assert ol.snapshot(oracle, ic[:0], e[:0]).tolist() == []
