@@ -15,9 +15,11 @@
 // keys, stable: records of equal energy keep record order) and thinned by
 // remove_dups (LIB/gaboom.cpp:2728-2746: a record is dropped when every gene
 // lies within 0.1 of the last record kept), one wave testing 64 records
-// against the last kept record at a time.
+// (held in registers) against the last kept record at a time.
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -99,46 +101,109 @@ __global__ void ga_snap_gather_kernel(const int* __restrict__ order, int n, int 
     }
 }
 
-// remove_dups over sorted records (rows of G genes): record 0 is kept; a
-// record is kept when some gene differs by 0.1 or more from the last record
-// kept.  One wave: 64 records are tested against the last kept record at a
-// time, and the first that differs is kept next.
+// remove_dups over sorted records (rows of G <= GM genes): record 0 is
+// kept; a record is kept when some gene differs by 0.1 or more from the last
+// record kept.  One wave: each lane holds one of 64 consecutive records and
+// the last record kept in registers; all test at once, the first that
+// differs is kept (its genes move to every lane through readlane) and the
+// records after it are tested again.
+// The kept records are 0, nxt(0), nxt(nxt(0)), ... with nxt(i) the first
+// record after i that differs from it (some gene 0.1 or more apart).  nxt by
+// a forward scan per record (at most SNAP_SCAN steps; longer runs of close
+// records set *overflow and the one-wave kernel below thins instead); the
+// chain from record 0 by pointer doubling: jump[k+1] = jump[k] o jump[k],
+// then, from the largest jump down, every marked record marks its jump[k]
+// (a record marked during a pass is on the chain as well, so the races of
+// a pass only add chain records).
+constexpr int SNAP_SCAN = 1024;
+
+__global__ void ga_snap_next_kernel(const double* __restrict__ sic, int n, int G, int* __restrict__ nxt,
+                                    int* __restrict__ overflow)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) {
+        nxt[n] = n;
+        return;
+    }
+    const double* a = sic + (size_t)i * G;
+    int j = i + 1;
+    for (int steps = 0; j < n; ++j, ++steps) {
+        if (steps == SNAP_SCAN) {
+            *overflow = 1;
+            break;
+        }
+        const double* b = sic + (size_t)j * G;
+        int flag = 0;
+        for (int g = 0; g < G; ++g) flag += fabs(b[g] - a[g]) < 0.1;
+        if (flag != G) break;
+    }
+    nxt[i] = j;
+}
+
+__global__ void ga_snap_jump_kernel(const int* __restrict__ from, int n, int* __restrict__ to)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) to[i] = from[from[i]];
+}
+
+__global__ void ga_snap_mark_kernel(const int* __restrict__ jump, int n, unsigned char* __restrict__ mark)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && mark[i]) {
+        const int j = jump[i];
+        if (j < n) mark[j] = 1;
+    }
+}
+
+template <int GM>
 __global__ __launch_bounds__(64) void ga_snap_dedup_kernel(const double* __restrict__ sic, int n, int G,
                                                             int* __restrict__ kept, int* __restrict__ n_kept)
 {
-    __shared__ double last[FA_MAX_GENES];
     const int lane = threadIdx.x;
     if (n <= 0) {
         if (lane == 0) *n_kept = 0;
         return;
     }
-    if (lane < G) last[lane] = sic[lane];
+    double last[GM], row[GM];
+#pragma unroll
+    for (int g = 0; g < GM; ++g) last[g] = g < G ? sic[g] : 0.0;
     if (lane == 0) kept[0] = 0;
-    __syncthreads();
-    int cnt = 1, pos = 1;
-    while (pos < n) {
-        const int j = pos + lane;
-        bool differs = false;
-        if (j < n) {
-            int flag = 0;
-            for (int g = 0; g < G; ++g) flag += fabs(sic[(size_t)j * G + g] - last[g]) < 0.1;
-            differs = flag != G;
+    int cnt = 1;
+    for (int base = 1; base < n; base += 64) {
+        const int j = base + lane;
+        const bool have = j < n;
+#pragma unroll
+        for (int g = 0; g < GM; ++g) row[g] = (have && g < G) ? sic[(size_t)j * G + g] : 0.0;
+        int start = 0;
+        while (true) {
+            bool differs = false;
+            if (have && lane >= start) {
+                int flag = 0;
+#pragma unroll
+                for (int g = 0; g < GM; ++g)
+                    if (g < G) flag += fabs(row[g] - last[g]) < 0.1;
+                differs = flag != G;
+            }
+            const unsigned long long b = __builtin_amdgcn_ballot_w64(differs);
+            if (b == 0ull) break;
+            const int f = (int)__builtin_ctzll(b);
+            if (lane == 0) kept[cnt] = base + f;
+            ++cnt;
+#pragma unroll
+            for (int g = 0; g < GM; ++g) {
+                const unsigned long long u = (unsigned long long)__double_as_longlong(row[g]);
+                const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, f);
+                const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), f);
+                last[g] = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+            }
+            start = f + 1;
         }
-        const unsigned long long b = __builtin_amdgcn_ballot_w64(differs);
-        if (b == 0ull) {
-            pos += 64;
-            continue;
-        }
-        const int r = pos + (int)__builtin_ctzll(b);
-        if (lane == 0) kept[cnt] = r;
-        ++cnt;
-        __syncthreads();
-        if (lane < G) last[lane] = sic[(size_t)r * G + lane];
-        __syncthreads();
-        pos = r + 1;
     }
     if (lane == 0) *n_kept = cnt;
 }
+
+
 
 int ceil_log2(long long n)
 {
@@ -609,8 +674,16 @@ int fa_ga_snapshot(fa_ga* ga, int job, double* ic, double* evalue, double* app, 
                                      (int*)nullptr, (int*)nullptr, (unsigned)n, 0, 64, s));
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t bk = al(8 * (size_t)n), bi = al(4 * (size_t)n), bg = al(8 * (size_t)n * G);
+    int K = 1;  // jump levels: 2^K > n
+    while ((1LL << K) <= n) ++K;
+    size_t tsel = 0;
+    FA_HIP(rocprim::select(nullptr, tsel, rocprim::counting_iterator<int>(0), (unsigned char*)nullptr, (int*)nullptr,
+                           (int*)nullptr, (size_t)n, s));
+    tmp = std::max(tmp, tsel);
+    const size_t bj = al(4 * (size_t)(n + 1));
     FA_HIP(hipStreamSynchronize(s));  // the scratch may be regrown
-    const int rc = fa_ensure(ga->snapwork, 2 * bk + 2 * bi + 2 * bg + 4 * bk + bi + 256 + al(tmp));
+    const int rc = fa_ensure(ga->snapwork, 2 * bk + 2 * bi + 2 * bg + 4 * bk + bi + 256 + al(tmp) + (size_t)K * bj +
+                                               al((size_t)n));
     if (rc != FA_SUCCESS) return rc;
     char* w = static_cast<char*>(ga->snapwork.p);
     auto take = [&](size_t b) {
@@ -631,12 +704,37 @@ int fa_ga_snapshot(fa_ga* ga, int job, double* ic, double* evalue, double* app, 
     int* kept = reinterpret_cast<int*>(take(bi));
     int* nk = reinterpret_cast<int*>(take(256));
     void* t = take(al(tmp));
+    int* jump = reinterpret_cast<int*>(take((size_t)K * bj));
+    auto* mark = reinterpret_cast<unsigned char*>(take(al((size_t)n)));
     hipLaunchKernelGGL(ga_snap_keys_kernel, dim3(ga_blocks(n)), dim3(256), 0, s, se, (int)n, k0, v0);
     FA_HIP(hipGetLastError());
     FA_HIP(rocprim::radix_sort_pairs(t, tmp, k0, k1, v0, v1, (unsigned)n, 0, 64, s));
     hipLaunchKernelGGL(ga_snap_gather_kernel, dim3(ga_blocks(n * G)), dim3(256), 0, s, v1, (int)n, G, Gmax, sic, se,
                        sa, tic, te, ta);
-    hipLaunchKernelGGL(ga_snap_dedup_kernel, dim3(1), dim3(64), 0, s, tic, (int)n, G, kept, nk);
+    // nxt, then the chain from record 0; nk[0] = kept count, nk[1] = overflow
+    FA_HIP(hipMemsetAsync(nk, 0, 2 * sizeof(int), s));
+    FA_HIP(hipMemsetAsync(mark, 0, (size_t)n, s));
+    FA_HIP(hipMemsetAsync(mark, 1, 1, s));
+    hipLaunchKernelGGL(ga_snap_next_kernel, dim3(ga_blocks(n + 1)), dim3(256), 0, s, tic, (int)n, G, jump, nk + 1);
+    for (int k = 0; k + 1 < K; ++k)
+        hipLaunchKernelGGL(ga_snap_jump_kernel, dim3(ga_blocks(n + 1)), dim3(256), 0, s, jump + (size_t)k * (bj / 4),
+                           (int)n, jump + (size_t)(k + 1) * (bj / 4));
+    for (int k = K - 1; k >= 0; --k)
+        hipLaunchKernelGGL(ga_snap_mark_kernel, dim3(ga_blocks(n)), dim3(256), 0, s, jump + (size_t)k * (bj / 4),
+                           (int)n, mark);
+    FA_HIP(hipGetLastError());
+    FA_HIP(rocprim::select(t, tsel, rocprim::counting_iterator<int>(0), mark, kept, nk, (size_t)n, s));
+    int ovf = 0;
+    FA_HIP(hipMemcpyAsync(&ovf, nk + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    if (ovf) {  // a run of more than SNAP_SCAN close records: thin in one wave
+        if (G <= 16)
+            hipLaunchKernelGGL(ga_snap_dedup_kernel<16>, dim3(1), dim3(64), 0, s, tic, (int)n, G, kept, nk);
+        else if (G <= 32)
+            hipLaunchKernelGGL(ga_snap_dedup_kernel<32>, dim3(1), dim3(64), 0, s, tic, (int)n, G, kept, nk);
+        else
+            hipLaunchKernelGGL(ga_snap_dedup_kernel<FA_MAX_GENES>, dim3(1), dim3(64), 0, s, tic, (int)n, G, kept, nk);
+    }
     FA_HIP(hipGetLastError());
     int m = 0;
     FA_HIP(hipMemcpyAsync(&m, nk, sizeof(int), hipMemcpyDeviceToHost, s));
