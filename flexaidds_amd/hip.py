@@ -483,10 +483,20 @@ class GA:
         k = int(nk.value)
         return {"ic": ic[:k], "evalue": e[:k], "app": a[:k], "n_raw": n}
 
-    def snapshot_thermo(self, job: int = 0) -> dict:
+    def snapshot_thermo(self, job: int = 0, super_cluster: bool = False) -> dict:
         """The end-of-run ensemble: every kept record's energy (cf) once
-        (LIB/gaboom.cpp:535-540), on the device."""
-        return self.ctx.thermo(self.snapshot(job)["evalue"], self.params.temperature)
+        (LIB/gaboom.cpp:535-540), on the device.  With super_cluster (the
+        reference's use_super_cluster, :542-560) and more than 4 records,
+        only the FastOPTICS super-cluster of the energies (minPts
+        max(4, n / 20), fa_super_cluster: at most 8,192 records) when it
+        leaves some record out."""
+        e = self.snapshot(job)["evalue"]
+        T = self.params.temperature
+        if super_cluster and len(e) > 4:
+            members = self.ctx.super_cluster(e, max(4, len(e) // 20))["members"]
+            if 0 < len(members) < len(e):
+                return self.ctx.thermo(e[members], T)
+        return self.ctx.thermo(e, T)
 
     def dedup_stats(self) -> np.ndarray:
         """[jobs, 2]: duplicate candidates drawn, rows that kept a duplicate

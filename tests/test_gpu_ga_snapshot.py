@@ -89,3 +89,23 @@ def test_snapshot_limit_and_clusters(hip, ctx, oracle):
     assert ga.snapshot()["n_raw"] == 0
     with pytest.raises(hip.FAError):
         ga.snapshot(job=1)
+
+
+def test_snapshot_super_cluster_prefilter(hip, ctx, oracle):
+    # use_super_cluster: the end-of-run ensemble over the FastOPTICS
+    # super-cluster of the kept energies when it leaves records out
+    s = synth.make_config0(n_poses=8)
+    ga = hip.GA(ctx, synth.ga_limits(s.ligands[0]), population=200, offspring=200, seed=9, temperature=300.0)
+    try:
+        ga.snapshot_enable(12)
+        ga.run(12)
+        e = ga.snapshot()["evalue"]
+        assert 4 < len(e) <= 8192
+        members = ol.super_cluster(oracle, e, max(4, len(e) // 20))
+        want = e[members] if 0 < len(members) < len(e) else e
+        got = ga.snapshot_thermo(super_cluster=True)
+        ref = ctx.thermo(want, 300.0)
+        assert got["log_Z"] == ref["log_Z"] and got["free_energy"] == ref["free_energy"]
+        print(f"super-cluster kept {len(members)} of {len(e)}", flush=True)
+    finally:
+        ga.close()
