@@ -24,8 +24,9 @@
 // before it), and a point's core distance is the (minPts-1)-th step of a
 // merge of its left and right neighbours in energy order.  The ordering and
 // the fill run in one workgroup that holds every point's energy, core
-// distance and reach in registers (a thread owns up to 8 consecutive
-// points): an ordering step is an argmin over (reach, index) — a wave
+// distance and reach in registers (up to 512 threads, at most 8 waves, so
+// that a step's cross-wave part stays short; a thread owns up to 16
+// consecutive points): an ordering step is an argmin over (reach, index) — a wave
 // butterfly, one barrier, a read of the waves' winners — and a fill step a
 // block-wide scan of the points that join (a ballot per owned point, the
 // waves' counts through one barrier).  The reference's distance sqrt(d * d)
@@ -44,7 +45,7 @@ namespace fa {
 namespace {
 
 constexpr int SC_MAX = FA_SUPER_CLUSTER_MAX;
-constexpr int SC_THREADS = 1024;
+constexpr int SC_THREADS = 512;
 constexpr int SC_BLOCK = 256;
 
 __device__ __forceinline__ double sc_dist(double a, double b)
@@ -363,13 +364,15 @@ int fa_super_cluster(fa_ctx* ctx, const double* energy, int n, int min_pts, int*
     hipLaunchKernelGGL(fa::sc_rank_kernel, dim3(nb), dim3(fa::SC_BLOCK), 0, s, d_x, n, d_rank, d_sorted);
     hipLaunchKernelGGL(fa::sc_core_kernel, dim3(nb), dim3(fa::SC_BLOCK), 0, s, d_x, d_rank, d_sorted, n, min_pts, d_cd);
     FA_HIP(hipGetLastError());
-    // points per thread: the least K in {1, 2, 4, 8} with 1024 K >= n
-    const int K = n <= fa::SC_THREADS ? 1 : n <= 2 * fa::SC_THREADS ? 2 : n <= 4 * fa::SC_THREADS ? 4 : 8;
+    // points per thread: the least K in {1, 2, 4, 8, 16} with SC_THREADS K >= n
+    // (at most 8 waves: a step's cross-wave part shrinks with the waves)
+    const int K = n <= fa::SC_THREADS ? 1 : n <= 2 * fa::SC_THREADS ? 2 : n <= 4 * fa::SC_THREADS ? 4 : n <= 8 * fa::SC_THREADS ? 8 : 16;
     const int threads = ((n + K - 1) / K + 63) / 64 * 64;
     hipError_t le = K == 1   ? fa::launch_optics<1>(exact, threads, d_x, d_cd, n, d_out, d_info, s)
                     : K == 2 ? fa::launch_optics<2>(exact, threads, d_x, d_cd, n, d_out, d_info, s)
                     : K == 4 ? fa::launch_optics<4>(exact, threads, d_x, d_cd, n, d_out, d_info, s)
-                             : fa::launch_optics<8>(exact, threads, d_x, d_cd, n, d_out, d_info, s);
+                    : K == 8 ? fa::launch_optics<8>(exact, threads, d_x, d_cd, n, d_out, d_info, s)
+                             : fa::launch_optics<16>(exact, threads, d_x, d_cd, n, d_out, d_info, s);
     FA_HIP(le);
     FA_HIP(hipEventRecord(e1, s));
     FA_HIP(hipEventSynchronize(e1));
