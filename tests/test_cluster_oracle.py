@@ -212,3 +212,18 @@ def test_super_cluster_min_pts_edges(oracle, mp):
     rng = np.random.default_rng(mp)
     e = np.round(rng.normal(-12.0, 5.0, 150), 2)
     assert np.array_equal(ol.super_cluster(oracle, e, mp), ol.super_cluster(ref, e, mp, ref=True))
+
+
+def test_super_cluster_tiny_and_huge_energies(oracle):
+    # differences whose square underflows or overflows: the oracle's
+    # sqrt(d * d) against the reference's own
+    ref = ol.load_ref()
+    if ref is None:
+        pytest.skip("reference build (oracle/_ref) absent")
+    rng = np.random.default_rng(44)
+    tiny = rng.uniform(1.0, 2.0, 40) * 1e-300
+    huge = rng.uniform(1.0, 2.0, 10) * 1e200
+    e = np.concatenate([rng.normal(-20.0, 5.0, 150), tiny, -tiny[:10], huge, [0.0, 0.0]])
+    e = e[rng.permutation(len(e))]
+    for mp in (max(4, len(e) // 20), 2, 30):
+        assert np.array_equal(ol.super_cluster(oracle, e, mp), ol.super_cluster(ref, e, mp, ref=True)), mp

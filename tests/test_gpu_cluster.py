@@ -226,3 +226,22 @@ def test_super_cluster_ensemble(oracle, scored):
     sc = pop.super_cluster_ensemble()
     for k in ("free_energy", "mean_energy", "log_Z"):
         assert abs(sc[k] - ref[k]) <= 1e-9 * max(1.0, abs(ref[k])), k
+
+
+def test_super_cluster_guarded_distances(hip, oracle):
+    # energies outside [2^-440, 2^498] take the guarded distance, which
+    # evaluates sqrt(d * d) where it is not |d|: tiny differences underflow
+    # to 0, huge ones overflow to inf, as in the reference's distance()
+    rng = np.random.default_rng(44)
+    tiny = rng.uniform(1.0, 2.0, 40) * 1e-300
+    huge = rng.uniform(1.0, 2.0, 10) * 1e200
+    e = np.concatenate([rng.normal(-20.0, 5.0, 150), tiny, -tiny[:10], huge, [0.0, 0.0]])
+    e = e[rng.permutation(len(e))]
+    ctx = hip.Context()
+    try:
+        for mp in (max(4, len(e) // 20), 2, 30):
+            got = ctx.super_cluster(e, mp)
+            want = ol.super_cluster(oracle, e, mp)
+            assert np.array_equal(got["members"], want), mp
+    finally:
+        ctx.close()
