@@ -27,9 +27,10 @@
 // distance and reach in registers (up to 512 threads, at most 8 waves, so
 // that a step's cross-wave part stays short; a thread owns up to 16
 // consecutive points): an ordering step is an argmin over (reach, index) — a wave
-// butterfly, one barrier, a read of the waves' winners — and a fill step a
-// block-wide scan of the points that join (a ballot per owned point, the
-// waves' counts through one barrier).  The reference's distance sqrt(d * d)
+// butterfly, one barrier, a read of the waves' winners — and a fill step
+// takes up to 32 queue entries at once (each point finds the first of them
+// that it joins; ballots per entry order the joins, the waves' counts pass
+// through one barrier).  The reference's distance sqrt(d * d)
 // is |d|, exactly, in binary floating point while d * d neither underflows
 // nor overflows; energies outside the range that guarantees it take a
 // guarded form that evaluates sqrt(d * d) there.
@@ -47,6 +48,7 @@ namespace {
 constexpr int SC_MAX = FA_SUPER_CLUSTER_MAX;
 constexpr int SC_THREADS = 512;
 constexpr int SC_BLOCK = 256;
+constexpr int SC_BATCH = 32;  // fill: queue entries taken at a time
 
 __device__ __forceinline__ double sc_dist(double a, double b)
 {
@@ -196,7 +198,7 @@ __global__ __launch_bounds__(SC_THREADS) void sc_optics_kernel(const double* __r
     __shared__ double xs[SC_MAX];
     __shared__ int queue[SC_MAX];
     __shared__ ScSlot slot[2][SC_THREADS / 64];
-    __shared__ int wsum[2][SC_THREADS / 64];
+    __shared__ int wsum[SC_BATCH][SC_THREADS / 64];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
     const double taken = __longlong_as_double((long long)SC_TAKEN);
     double xr[K], cr[K], br[K];
@@ -256,48 +258,85 @@ __global__ __launch_bounds__(SC_THREADS) void sc_optics_kernel(const double* __r
         for (int k = 0; k < K; ++k)
             if (t * K + k == widx) br[k] = taken;
     }
-    // flood fill from the seed
+    // flood fill from the seed, up to SC_BATCH queue entries at a time: the
+    // entries [head, head + B) are all in the queue when the batch starts,
+    // so the one-at-a-time fill adds, in order, the points whose first
+    // qualifying entry is the batch's first, then its second, ..., each
+    // group in point order; here each point finds its first qualifying
+    // entry, and the groups are counted per wave and laid out in that order
     unsigned vis = absent;
 #pragma unroll
     for (int k = 0; k < K; ++k)
         if (t * K + k == s_idx) vis |= 1u << k;
     if (t == 0) queue[0] = s_idx;
     __syncthreads();
-    // entries [fresh, tail) were appended by the last pop: reading one needs
-    // a barrier first
-    int head = 0, tail = 1, fresh = 1;
-    for (int pop = 0; head < tail; ++pop) {
-        if (head >= fresh) __syncthreads();
-        const double xq = xs[queue[head]];
-        ++head;
-        unsigned join = 0;
+    int head = 0, tail = 1;
+    while (head < tail) {
+        const int B = min(SC_BATCH, tail - head);
+        const double xb = lane < B ? xs[queue[head + lane]] : 0.0;
+        int fp[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) fp[k] = -1;
+        for (int b = 0; b < B; ++b) {
+            const double xq = readlane_f64(xb, b);
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (fp[k] < 0 && !(vis >> k & 1u) && sc_dist_t<EXACT>(xq, xr[k]) <= 0.8 * cr[k]) fp[k] = b;
+        }
+        // the batch entries that add points in this wave
+        unsigned lm = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            if (!(vis >> k & 1u) && sc_dist_t<EXACT>(xq, xr[k]) <= 0.8 * cr[k]) join |= 1u << k;
-        vis |= join;
-        int below = 0, wtot = 0;
+            if (fp[k] >= 0) lm |= 1u << fp[k];
+        unsigned wm = lm;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) wm |= (unsigned)__shfl_xor((int)wm, o);
+        wm = (unsigned)__builtin_amdgcn_readfirstlane((int)wm);
+        int rank[K];  // a joining point's place among this wave's points of its entry
+        int mycnt = 0;  // lane l: this wave's points added by entry l
+        for (unsigned rem = wm; rem != 0u; rem &= rem - 1u) {
+            const int f = __builtin_ctz(rem);
+            int below = 0, tot = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const unsigned long long bk = ballot(fp[k] == f);
+                below += lanes_below(bk);
+                tot += __popcll(bk);
+            }
+            int own = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (fp[k] == f) rank[k] = below + own++;
+            if (lane == f) mycnt = tot;
+        }
+        if (lane < SC_BATCH) wsum[lane][wave] = mycnt;
+        __syncthreads();
+        int T = 0, P = 0;
+        if (lane < B)
+            for (int w = 0; w < nw; ++w) {
+                const int c = wsum[lane][w];
+                T += c;
+                P += w < wave ? c : 0;
+            }
+        int incl = T;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        const int base = incl - T + P;  // lane l: where this wave's points of entry l start
+        const int total = __builtin_amdgcn_readlane(incl, 63);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const unsigned long long b = ballot(join >> k & 1u);
-            below += lanes_below(b);
-            wtot += __popcll(b);
+            const int at = __shfl(base, fp[k] < 0 ? 0 : fp[k]);
+            if (fp[k] >= 0) {
+                queue[tail + at + rank[k]] = t * K + k;
+                vis |= 1u << k;
+            }
         }
-        const int par = pop & 1;
-        if (lane == 0) wsum[par][wave] = wtot;
-        __syncthreads();
-        const int ws = lane < nw ? wsum[par][lane] : 0;
-        int before = 0, total = 0;
-        for (int w = 0; w < nw; ++w) {
-            const int s = __builtin_amdgcn_readlane(ws, w);
-            before += w < wave ? s : 0;
-            total += s;
-        }
-        int pos = tail + before + below;
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (join >> k & 1u) queue[pos++] = t * K + k;
-        fresh = tail;
         tail += total;
+        head += B;
+        __syncthreads();
     }
     __syncthreads();
     for (int q = t; q < tail; q += blockDim.x) out[q] = queue[q];
