@@ -178,29 +178,29 @@ __device__ __forceinline__ double sc_dist_t(double a, double b)
 
 struct ScSlot {
     uint64_t key;  // reach bits: non-negative doubles order as their bit patterns
-    double x, cd;
     int idx, pad;
 };
-
-constexpr uint64_t SC_TAKEN = 0x7ff8000000000000ull;  // a quiet NaN: above every reach, never lowered
 
 // One workgroup of blockDim.x (a multiple of 64, blockDim.x * K >= n)
 // threads; thread t owns points t*K .. t*K+K-1, so lane order, then k, is
 // point order.  out[0..info[0]) receives the fill's visiting order, info[1]
 // the seed.  Reaches are kept as doubles whose bit patterns order like their
-// values (all are >= +0 or +inf); a processed or absent point holds a NaN,
-// which no reach lowers and which sorts above +inf.
+// values (all are >= +0 or +inf); a processed or absent point holds -inf,
+// which the minimum keeps and whose bit pattern sorts above +inf's.  The
+// winner's energy and core distance come from LDS (the core distances'
+// area holds the fill's queue afterwards).
 template <int K, bool EXACT>
 __global__ __launch_bounds__(SC_THREADS) void sc_optics_kernel(const double* __restrict__ x,
                                                                const double* __restrict__ cd, int n,
                                                                int* __restrict__ out, int* __restrict__ info)
 {
     __shared__ double xs[SC_MAX];
-    __shared__ int queue[SC_MAX];
+    __shared__ double cds[SC_MAX];
+    int* const queue = reinterpret_cast<int*>(cds);
     __shared__ ScSlot slot[2][SC_THREADS / 64];
     __shared__ int wsum[SC_BATCH][SC_THREADS / 64];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
-    const double taken = __longlong_as_double((long long)SC_TAKEN);
+    const double taken = -INFINITY;
     double xr[K], cr[K], br[K];
     unsigned absent = 0;
 #pragma unroll
@@ -211,7 +211,10 @@ __global__ __launch_bounds__(SC_THREADS) void sc_optics_kernel(const double* __r
         br[k] = v < n ? INFINITY : taken;
         if (v >= n) absent |= 1u << k;
     }
-    for (int v = t; v < n; v += blockDim.x) xs[v] = x[v];
+    for (int v = t; v < n; v += blockDim.x) {
+        xs[v] = x[v];
+        cds[v] = cd[v];
+    }
     if (t == 0) br[0] = taken;  // point 0 opens the ordering (reach infinite)
     double xu = x[0], cu = cd[0];
     // least (reach, index) of the ordering so far: (+inf, 0)
@@ -220,25 +223,22 @@ __global__ __launch_bounds__(SC_THREADS) void sc_optics_kernel(const double* __r
     for (int step = 1; step < n; ++step) {
         uint64_t bkey = ~0ull;
         int bk = 0;
-        double bx = 0.0, bc = 0.0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const double d = sc_dist_t<EXACT>(xu, xr[k]);
-            const double w = cu < d ? d : cu;
-            if (w < br[k]) br[k] = w;
+            // no operand is a NaN: fmax / fmin are the reference's max and
+            // the queue's least entry
+            br[k] = fmin(br[k], fmax(cu, sc_dist_t<EXACT>(xu, xr[k])));
             const uint64_t key = (uint64_t)__double_as_longlong(br[k]);
             if (key < bkey) {
                 bkey = key;
                 bk = k;
-                bx = xr[k];
-                bc = cr[k];
             }
         }
         // the wave's least key; its first lane holds the least index
         const uint64_t wmin = wave_min_u64(bkey);
         const int wl = (int)__builtin_ctzll(ballot(bkey == wmin));
         const int par = step & 1;
-        if (lane == wl) slot[par][wave] = ScSlot{wmin, bx, bc, t * K + bk, 0};
+        if (lane == wl) slot[par][wave] = ScSlot{wmin, t * K + bk, 0};
         __syncthreads();
         // the block's: lanes 0..nw-1 hold the waves' winners, in point order
         const bool live = lane < nw;
@@ -248,8 +248,8 @@ __global__ __launch_bounds__(SC_THREADS) void sc_optics_kernel(const double* __r
         const uint64_t gmin = readlane_u64(r, 0);
         const int ww = (int)__builtin_ctzll(ballot(live && ck == gmin));
         const int widx = __builtin_amdgcn_readlane(c.idx, ww);
-        xu = readlane_f64(c.x, ww);
-        cu = readlane_f64(c.cd, ww);
+        xu = xs[widx];
+        cu = cds[widx];
         if (gmin < s_key || (gmin == s_key && widx < s_idx)) {
             s_key = gmin;
             s_idx = widx;
@@ -268,6 +268,7 @@ __global__ __launch_bounds__(SC_THREADS) void sc_optics_kernel(const double* __r
 #pragma unroll
     for (int k = 0; k < K; ++k)
         if (t * K + k == s_idx) vis |= 1u << k;
+    __syncthreads();  // the queue takes the core distances' area
     if (t == 0) queue[0] = s_idx;
     __syncthreads();
     int head = 0, tail = 1;
