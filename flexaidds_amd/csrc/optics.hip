@@ -21,7 +21,7 @@
 // and the super-cluster is the fill's visiting order.
 //
 // Device form: the energies are ranked (a thread per point counts the points
-// before it), and a point's core distance is the (minPts-1)-th step of a
+// before it, chunks of 1,024 in separate blocks), and a point's core distance is the (minPts-1)-th step of a
 // merge of its left and right neighbours in energy order.  The ordering and
 // the fill run in one workgroup that holds every point's energy, core
 // distance and reach in registers (up to 512 threads, at most 8 waves, so
@@ -58,29 +58,37 @@ __device__ __forceinline__ double sc_dist(double a, double b)
     return sqrt(d * d);
 }
 
-// rank[j] = #points before j in (energy, index) order; sorted[rank[j]] = e_j
-__global__ __launch_bounds__(SC_BLOCK) void sc_rank_kernel(const double* __restrict__ x, int n, int* __restrict__ rank,
-                                                           double* __restrict__ sorted)
+constexpr int SC_RANK_CHUNK = 1024;  // points counted by one block of the ranking
+
+// rank[j] += #points i of this block's chunk before j in (energy, index)
+// order (rank zeroed first); blocks (j tile, i chunk)
+__global__ __launch_bounds__(SC_BLOCK) void sc_rank_kernel(const double* __restrict__ x, int n, int* __restrict__ rank)
 {
     __shared__ double tile[SC_BLOCK];
     const int j = blockIdx.x * SC_BLOCK + threadIdx.x;
     const double xj = j < n ? x[j] : 0.0;
     int r = 0;
-    for (int base = 0; base < n; base += SC_BLOCK) {
+    const int lo = blockIdx.y * SC_RANK_CHUNK, hi = min(n, lo + SC_RANK_CHUNK);
+    for (int base = lo; base < hi; base += SC_BLOCK) {
         const int i = base + threadIdx.x;
         tile[threadIdx.x] = i < n ? x[i] : 0.0;
         __syncthreads();
-        const int m = min(SC_BLOCK, n - base);
+        const int m = min(SC_BLOCK, hi - base);
         for (int q = 0; q < m; ++q) {
             const double xi = tile[q];
             r += (xi < xj) || (xi == xj && base + q < j);
         }
         __syncthreads();
     }
-    if (j < n) {
-        rank[j] = r;
-        sorted[r] = xj;
-    }
+    if (j < n && r > 0) atomicAdd(&rank[j], r);
+}
+
+// sorted[rank[j]] = e_j
+__global__ __launch_bounds__(SC_BLOCK) void sc_scatter_kernel(const double* __restrict__ x, int n,
+                                                              const int* __restrict__ rank, double* __restrict__ sorted)
+{
+    const int j = blockIdx.x * SC_BLOCK + threadIdx.x;
+    if (j < n) sorted[rank[j]] = x[j];
 }
 
 // coreDist of every point: the distances to its neighbours grow away from
@@ -401,7 +409,10 @@ int fa_super_cluster(fa_ctx* ctx, const double* energy, int n, int min_pts, int*
     FA_HIP(hipEventCreate(&e1));
     FA_HIP(hipEventRecord(e0, s));
     const int nb = (n + fa::SC_BLOCK - 1) / fa::SC_BLOCK;
-    hipLaunchKernelGGL(fa::sc_rank_kernel, dim3(nb), dim3(fa::SC_BLOCK), 0, s, d_x, n, d_rank, d_sorted);
+    FA_HIP(hipMemsetAsync(d_rank, 0, sizeof(int) * n, s));
+    hipLaunchKernelGGL(fa::sc_rank_kernel, dim3(nb, (n + fa::SC_RANK_CHUNK - 1) / fa::SC_RANK_CHUNK), dim3(fa::SC_BLOCK), 0,
+                       s, d_x, n, d_rank);
+    hipLaunchKernelGGL(fa::sc_scatter_kernel, dim3(nb), dim3(fa::SC_BLOCK), 0, s, d_x, n, d_rank, d_sorted);
     hipLaunchKernelGGL(fa::sc_core_kernel, dim3(nb), dim3(fa::SC_BLOCK), 0, s, d_x, d_rank, d_sorted, n, min_pts, d_cd);
     FA_HIP(hipGetLastError());
     // points per thread: the least K in {1, 2, 4, 8, 16} with SC_THREADS K >= n
